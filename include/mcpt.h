@@ -1,0 +1,284 @@
+/*
+ * mcpt.h — C ABI of the MI355X (gfx950) path-tracing core.
+ *
+ * The reference renderer (MonoCUDA-PT, src/path_tracer.cu) is one CUDA
+ * translation unit with no library boundary; its in-TU entry points are
+ *   BVH::generateHierarchy          (src/path_tracer.cu:1063)
+ *   trace_sample_kernel             (src/path_tracer.cu:2900, looped at :4338)
+ *   renderWithWavefront             (src/path_tracer.cu:3757, called at :4336)
+ *   resolve_kernel                  (src/path_tracer.cu:3092)
+ * This header is the boundary that replaces them. Every function returns an
+ * int status (MCPT_OK = 0, negative on error); mcpt_last_error() returns the
+ * calling thread's last message. The reference instead prints and exits
+ * (CUDA_CHECK, src/path_tracer.cu:55-69).
+ *
+ * Threading: one mcpt_ctx per host thread. A context owns one HIP stream on
+ * one device and an HBM budget; scenes, films and servers belong to one
+ * context. There is no global device state (the reference keeps the
+ * environment map in a __constant__ global, src/path_tracer.cu:333; here it
+ * is part of the scene).
+ *
+ * Memory: inputs passed as host pointers are copied into device memory the
+ * library owns; the caller keeps its arrays. Films (accumulation buffers)
+ * stay resident in HBM between render calls.
+ */
+#ifndef MCPT_H
+#define MCPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+#define MCPT_OK 0
+#define MCPT_DEFERRED 1          /* server: job accepted but waiting for HBM */
+#define MCPT_PENDING 2           /* server: job not finished yet */
+#define MCPT_ERR_INVALID (-1)    /* bad argument or state */
+#define MCPT_ERR_HIP (-2)        /* HIP runtime error */
+#define MCPT_ERR_BUDGET (-3)     /* request exceeds the context's HBM budget */
+#define MCPT_ERR_NODEVICE (-4)   /* no usable gfx950 device */
+
+/* ----------------------------------------------------------- scene records */
+/* Material kinds, in the order of MaterialType (src/path_tracer.cu:484). */
+#define MCPT_MAT_LAMBERTIAN 0
+#define MCPT_MAT_DIELECTRIC 1
+#define MCPT_MAT_CONDUCTOR 2
+#define MCPT_MAT_EMISSIVE 3
+#define MCPT_MAT_ISOTROPIC 4
+
+/* triangle_hot (src/path_tracer.cu:660-670): the 36-byte record read by
+ * every leaf test. */
+typedef struct mcpt_tri_hot {
+    float v0[3], v1[3], v2[3];
+} mcpt_tri_hot;
+
+/* triangle_cold (src/path_tracer.cu:672-680) without the inline Material and
+ * AABB: shading data read once per confirmed hit. `material` indexes the
+ * scene's material table. 64 bytes. */
+typedef struct mcpt_tri_cold {
+    float uv0[2], uv1[2], uv2[2];
+    float n0[3], n1[3], n2[3];
+    int32_t material;
+} mcpt_tri_cold;
+
+/* Material (src/path_tracer.cu:486-496) with a solid albedo that is already
+ * linear (make_solid_texture converts sRGB at construction, :207-209). */
+typedef struct mcpt_material {
+    int32_t type;
+    float albedo[3];
+    float emission[3];
+    float fuzz;
+    float ior;
+    float density;
+    float phase_g;
+    int32_t pad;
+} mcpt_material;
+
+/* Light (src/path_tracer.cu:942-947): an emissive triangle. 64 bytes. */
+typedef struct mcpt_light {
+    mcpt_tri_hot tri;
+    float emission[3];
+    float area;
+    float normal[3];
+} mcpt_light;
+
+/* Equirectangular environment map, RGBA FP32 rows top (v = 0, +y) first
+ * (the layout loadPfmAsRgba32f produces, src/path_tracer.cu:335-368). */
+typedef struct mcpt_env_desc {
+    int32_t width, height;
+    float intensity;
+    float rotation_radians;
+    const float* rgba; /* width*height*4 floats, host memory */
+} mcpt_env_desc;
+
+/* Ray for the trace hooks: origin, time, direction, tmax (32 bytes). */
+typedef struct mcpt_ray {
+    float origin[3];
+    float time;
+    float dir[3];
+    float tmax;
+} mcpt_ray;
+
+/* PackedHit (src/path_tracer.cu:991-996), the deferred 16-byte hit record. */
+typedef struct mcpt_packed_hit {
+    float t;
+    int32_t prim_type;  /* 1 = triangle, -1 = miss */
+    int32_t prim_index; /* original triangle index, -1 = miss */
+    int32_t did_hit;
+} mcpt_packed_hit;
+
+/* Camera as set up by Camera::initialize (src/path_tracer.cu:1610-1631). */
+typedef struct mcpt_camera_desc {
+    float look_from[3], look_at[3], up[3];
+    float vfov_degrees;
+    float aperture;
+    float focus_distance;
+    int32_t width, height;
+    float background[3];
+    float rr_cap; /* Russian-roulette cap, 0.95 in the reference (:1606) */
+} mcpt_camera_desc;
+
+typedef struct mcpt_camera {
+    float center[3], pixel00[3], delta_u[3], delta_v[3], u[3], v[3], w[3];
+    float lens_radius;
+    int32_t width, height;
+    float background[3];
+    float rr_cap;
+} mcpt_camera;
+
+/* Reference AABB (src/path_tracer.cu:612-625): min, pad, max, pad. */
+typedef struct mcpt_aabb {
+    float min[3];
+    float pad0;
+    float max[3];
+    float pad1;
+} mcpt_aabb;
+
+/* ------------------------------------------------------------- opaque ids */
+typedef struct mcpt_ctx_s* mcpt_ctx;
+typedef struct mcpt_scene_s* mcpt_scene;
+typedef struct mcpt_film_s* mcpt_film;
+
+const char* mcpt_last_error(void);
+const char* mcpt_version(void);
+
+/* ----------------------------------------------------------------- context */
+/* hbm_budget_bytes = 0 means "all free memory at creation". */
+int mcpt_ctx_create(int device, uint64_t hbm_budget_bytes, mcpt_ctx* out);
+int mcpt_ctx_destroy(mcpt_ctx ctx);
+/* bytes the context has allocated / may allocate */
+int mcpt_ctx_memory(mcpt_ctx ctx, uint64_t* used, uint64_t* budget);
+/* device name, CU count, clock, HBM size into a caller buffer (JSON) */
+int mcpt_ctx_describe(mcpt_ctx ctx, char* buf, size_t buflen);
+
+/* ------------------------------------------------------------------- scene */
+/* Copies triangles, materials and the optional environment map into HBM and
+ * builds the light list from emissive triangles, in triangle order
+ * (finalizeScene, src/path_tracer.cu:1973-1985). Does not build the BVH. */
+int mcpt_scene_create(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int32_t n_tris,
+                      const mcpt_material* materials, int32_t n_materials, const mcpt_env_desc* env,
+                      mcpt_scene* out);
+int mcpt_scene_destroy(mcpt_scene scene);
+int mcpt_scene_light_count(mcpt_scene scene, int32_t* n_lights);
+int mcpt_scene_lights(mcpt_scene scene, mcpt_light* out /* n_lights */);
+/* environment sampling tables (buildEnvironmentSamplingTables,
+ * src/path_tracer.cu:370-385): row_cdf[h+1], col_cdf[h*(w+1)], total */
+int mcpt_scene_env_tables(mcpt_scene scene, float* row_cdf, float* col_cdf, float* total_weight);
+
+/* Optional copy-back of every LBVH stage for parity tests. Each pointer may
+ * be NULL. Arrays are host memory sized as noted (n = triangle count). */
+typedef struct mcpt_lbvh_dump {
+    uint32_t* morton;       /* [n] per triangle, original order            */
+    uint32_t* sorted_keys;  /* [n]                                          */
+    int32_t* sorted_ids;    /* [n] stable (key, index) order                */
+    int32_t* left;          /* [n-1] internal node children                 */
+    int32_t* right;         /* [n-1]                                        */
+    uint8_t* left_is_leaf;  /* [n-1]                                        */
+    uint8_t* right_is_leaf; /* [n-1]                                        */
+    int32_t* parent;        /* [n-1] internal parents, root = -1            */
+    int32_t* leaf_parent;   /* [n]                                          */
+    mcpt_aabb* internal_bounds; /* [n-1] refit bounds                       */
+    mcpt_aabb* leaf_bounds; /* [n] triangle AABB padded by 1e-3, leaf order */
+    mcpt_aabb* scene_bounds; /* [1]                                         */
+    int32_t max_depth;      /* out: deepest leaf, in internal levels        */
+    float build_ms;         /* out: device time of the whole build          */
+} mcpt_lbvh_dump;
+
+/* Morton codes, stable radix sort, Karras radix tree and atomic refit, all
+ * on the GPU (BVH::generateHierarchy + the host part of finalizeScene and
+ * the host std::sort, src/path_tracer.cu:1930-1972, :4224-4245, :1063-1090). */
+int mcpt_lbvh_build(mcpt_scene scene, mcpt_lbvh_dump* dump /* may be NULL */);
+
+/* ------------------------------------------------------------ trace hooks */
+/* Closest hit for n rays (BVH::traverseRay / stepTraversal,
+ * src/path_tracer.cu:1092-1150, :1262-1321). `steps` and `leaves`, when not
+ * NULL, receive per-ray internal-node visits and leaf tests. Host arrays. */
+int mcpt_trace_closest(mcpt_scene scene, const mcpt_ray* rays, int32_t n, mcpt_packed_hit* hits, uint32_t* steps,
+                       uint32_t* leaves);
+/* Any hit in (1e-4, tmax) (BVH::traverseAnyHit, src/path_tracer.cu:1152-1207). */
+int mcpt_trace_any(mcpt_scene scene, const mcpt_ray* rays, int32_t n, uint8_t* occluded);
+
+/* ------------------------------------------------------------------ render */
+#define MCPT_MODE_MEGAKERNEL 0 /* trace_sample_kernel (:2900)                 */
+#define MCPT_MODE_WAVEFRONT 1  /* renderWithWavefront (:3757)                 */
+#define MCPT_MODE_PERSISTENT 2 /* renderWithWavefront + extendKernelPersistent */
+
+#define MCPT_TONEMAP_NONE 0
+#define MCPT_TONEMAP_REINHARD 1
+#define MCPT_TONEMAP_ACES 2
+
+/* flags */
+#define MCPT_RENDER_COUNT_TRAVERSAL 1u /* count extend steps/leaf tests      */
+#define MCPT_RENDER_TIME_KERNELS 2u    /* per-kernel HIP event timing        */
+
+typedef struct mcpt_render_opts {
+    int32_t mode;
+    int32_t spp;          /* samples rendered by this call                  */
+    int32_t sample_begin; /* index of the first sample (resume / sharding)  */
+    int32_t max_depth;
+    uint64_t seed;
+    int32_t refill_lanes; /* persistent extend: idle lanes before a refill;
+                             0 picks the default                           */
+    uint32_t flags;
+} mcpt_render_opts;
+
+typedef struct mcpt_render_stats {
+    double total_ms;      /* device time of the whole call                 */
+    double camera_ms, extend_ms, shade_ms, shadow_ms, finalize_ms;
+    uint64_t extend_launches;
+    uint64_t extend_rays; /* closest-hit rays traced                       */
+    uint64_t shadow_rays; /* any-hit rays traced                           */
+    uint64_t paths;       /* camera paths = W*H*spp                        */
+    uint64_t extend_steps;  /* with MCPT_RENDER_COUNT_TRAVERSAL            */
+    uint64_t extend_leaves; /* with MCPT_RENDER_COUNT_TRAVERSAL            */
+} mcpt_render_stats;
+
+/* Camera::initialize (src/path_tracer.cu:1610-1631) on the host. */
+int mcpt_camera_init(const mcpt_camera_desc* desc, mcpt_camera* out);
+
+/* Films: the linear accumulation contract. accum[p] holds W*H*3 FP32 sums of
+ * firefly-clamped samples (src/path_tracer.cu:3050-3057, :3740-3755). */
+int mcpt_film_create(mcpt_ctx ctx, int32_t width, int32_t height, mcpt_film* out);
+int mcpt_film_destroy(mcpt_film film);
+int mcpt_film_clear(mcpt_film film);
+int mcpt_film_read(mcpt_film film, float* rgb /* W*H*3 */);
+int mcpt_film_write(mcpt_film film, const float* rgb /* W*H*3 */);
+
+/* Renders opts->spp samples starting at opts->sample_begin and adds them to
+ * the film. stats may be NULL. The call returns when the device is done. */
+int mcpt_render(mcpt_scene scene, const mcpt_camera* camera, const mcpt_render_opts* opts, mcpt_film film,
+                mcpt_render_stats* stats);
+
+/* resolve_kernel (src/path_tracer.cu:3092-3100): accum / spp * exposure,
+ * tone map, sRGB OETF. out_rgb is host memory, W*H*3. */
+int mcpt_resolve(mcpt_film film, int32_t samples_so_far, int32_t tonemap, float exposure, float* out_rgb);
+
+/* ------------------------------------------------------- scene generators */
+#define MCPT_GEN_CORNELL 1 /* 36-triangle Cornell box (10 walls, 24 box, 2 light) */
+#define MCPT_GEN_SOUP 2    /* seeded incoherent triangle soup + 16 emitters      */
+#define MCPT_GEN_ICOSPHERE 3 /* displaced icosphere + random-triangle fill       */
+#define MCPT_GEN_MEDIUM 5  /* soup + homogeneous isotropic medium box            */
+
+typedef struct mcpt_gen_sizes {
+    int32_t n_tris;
+    int32_t n_materials;
+} mcpt_gen_sizes;
+
+/* Two-call pattern: pass NULL arrays to get sizes, then arrays of that size.
+ * param = soup/sphere triangle count (0 = the workload default). */
+int mcpt_gen_scene(int32_t kind, uint64_t seed, int64_t param, mcpt_tri_hot* hot, mcpt_tri_cold* cold,
+                   mcpt_material* materials, mcpt_gen_sizes* sizes);
+/* Procedural equirect sky: gradient + sun disk + glow (closed form of
+ * scripts/make_sky.py:5-27), width*height*4 floats, top row first. */
+int mcpt_gen_sky(int32_t width, int32_t height, float* rgba);
+/* Camera preset for a generated scene. */
+int mcpt_gen_camera(int32_t kind, int32_t width, int32_t height, mcpt_camera_desc* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCPT_H */

@@ -1,0 +1,362 @@
+"""Python binding of libmcpt.so, the gfx950 path-tracing core (include/mcpt.h).
+
+The library is loaded from this directory. There is no CPU fallback: if
+libmcpt.so is missing the import fails, and every call that reaches the GPU
+raises McptError when no gfx950 device is present.
+
+Typical use::
+
+    import importlib; mcpt = importlib.import_module("monocuda-pt_amd")
+    hot, cold, mats = mcpt.gen_scene(mcpt.GEN_CORNELL)
+    with mcpt.Context() as ctx:
+        scene = ctx.scene(hot, cold, mats)
+        scene.build()
+        cam = mcpt.camera_init(mcpt.gen_camera(mcpt.GEN_CORNELL, 256, 256))
+        film = ctx.film(256, 256)
+        stats = scene.render(cam, film, mode=mcpt.MODE_PERSISTENT, spp=16, max_depth=8)
+        rgb = film.read()
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import weakref
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmcpt.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is missing: build it with `make -C monocuda-pt_amd` or __graft_entry__.build()")
+_lib = C.CDLL(LIB_PATH)
+
+# ------------------------------------------------------------------ constants
+OK, DEFERRED, PENDING = 0, 1, 2
+ERR_INVALID, ERR_HIP, ERR_BUDGET, ERR_NODEVICE = -1, -2, -3, -4
+MAT_LAMBERTIAN, MAT_DIELECTRIC, MAT_CONDUCTOR, MAT_EMISSIVE, MAT_ISOTROPIC = 0, 1, 2, 3, 4
+MODE_MEGAKERNEL, MODE_WAVEFRONT, MODE_PERSISTENT = 0, 1, 2
+TONEMAP_NONE, TONEMAP_REINHARD, TONEMAP_ACES = 0, 1, 2
+RENDER_COUNT_TRAVERSAL, RENDER_TIME_KERNELS = 1, 2
+GEN_CORNELL, GEN_SOUP, GEN_ICOSPHERE, GEN_MEDIUM = 1, 2, 3, 5
+
+# numpy layouts of the C records (byte-identical)
+HOT_DTYPE = np.dtype([("v0", "<f4", 3), ("v1", "<f4", 3), ("v2", "<f4", 3)])
+COLD_DTYPE = np.dtype([("uv0", "<f4", 2), ("uv1", "<f4", 2), ("uv2", "<f4", 2), ("n0", "<f4", 3),
+                       ("n1", "<f4", 3), ("n2", "<f4", 3), ("material", "<i4")])
+MAT_DTYPE = np.dtype([("type", "<i4"), ("albedo", "<f4", 3), ("emission", "<f4", 3), ("fuzz", "<f4"),
+                      ("ior", "<f4"), ("density", "<f4"), ("phase_g", "<f4"), ("pad", "<i4")])
+LIGHT_DTYPE = np.dtype([("tri", HOT_DTYPE), ("emission", "<f4", 3), ("area", "<f4"), ("normal", "<f4", 3)])
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("time", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_type", "<i4"), ("prim_index", "<i4"), ("did_hit", "<i4")])
+AABB_DTYPE = np.dtype([("min", "<f4", 3), ("pad0", "<f4"), ("max", "<f4", 3), ("pad1", "<f4")])
+assert HOT_DTYPE.itemsize == 36 and COLD_DTYPE.itemsize == 64 and MAT_DTYPE.itemsize == 48
+assert LIGHT_DTYPE.itemsize == 64 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16
+assert AABB_DTYPE.itemsize == 32
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("look_from", C.c_float * 3), ("look_at", C.c_float * 3), ("up", C.c_float * 3),
+                ("vfov_degrees", C.c_float), ("aperture", C.c_float), ("focus_distance", C.c_float),
+                ("width", C.c_int32), ("height", C.c_int32), ("background", C.c_float * 3), ("rr_cap", C.c_float)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("pixel00", C.c_float * 3), ("delta_u", C.c_float * 3),
+                ("delta_v", C.c_float * 3), ("u", C.c_float * 3), ("v", C.c_float * 3), ("w", C.c_float * 3),
+                ("lens_radius", C.c_float), ("width", C.c_int32), ("height", C.c_int32),
+                ("background", C.c_float * 3), ("rr_cap", C.c_float)]
+
+
+class EnvDesc(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("intensity", C.c_float),
+                ("rotation_radians", C.c_float), ("rgba", C.c_void_p)]
+
+
+class LbvhDump(C.Structure):
+    _fields_ = [("morton", C.c_void_p), ("sorted_keys", C.c_void_p), ("sorted_ids", C.c_void_p),
+                ("left", C.c_void_p), ("right", C.c_void_p), ("left_is_leaf", C.c_void_p),
+                ("right_is_leaf", C.c_void_p), ("parent", C.c_void_p), ("leaf_parent", C.c_void_p),
+                ("internal_bounds", C.c_void_p), ("leaf_bounds", C.c_void_p), ("scene_bounds", C.c_void_p),
+                ("max_depth", C.c_int32), ("build_ms", C.c_float)]
+
+
+class RenderOpts(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("spp", C.c_int32), ("sample_begin", C.c_int32), ("max_depth", C.c_int32),
+                ("seed", C.c_uint64), ("refill_lanes", C.c_int32), ("flags", C.c_uint32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("camera_ms", C.c_double), ("extend_ms", C.c_double),
+                ("shade_ms", C.c_double), ("shadow_ms", C.c_double), ("finalize_ms", C.c_double),
+                ("extend_launches", C.c_uint64), ("extend_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("paths", C.c_uint64), ("extend_steps", C.c_uint64), ("extend_leaves", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class GenSizes(C.Structure):
+    _fields_ = [("n_tris", C.c_int32), ("n_materials", C.c_int32)]
+
+
+VP = C.c_void_p
+_SIGS = {
+    "mcpt_last_error": ([], C.c_char_p),
+    "mcpt_version": ([], C.c_char_p),
+    "mcpt_ctx_create": ([C.c_int, C.c_uint64, C.POINTER(VP)], C.c_int),
+    "mcpt_ctx_destroy": ([VP], C.c_int),
+    "mcpt_ctx_memory": ([VP, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
+    "mcpt_ctx_describe": ([VP, C.c_char_p, C.c_size_t], C.c_int),
+    "mcpt_scene_create": ([VP, VP, VP, C.c_int32, VP, C.c_int32, C.POINTER(EnvDesc), C.POINTER(VP)], C.c_int),
+    "mcpt_scene_destroy": ([VP], C.c_int),
+    "mcpt_scene_light_count": ([VP, C.POINTER(C.c_int32)], C.c_int),
+    "mcpt_scene_lights": ([VP, VP], C.c_int),
+    "mcpt_scene_env_tables": ([VP, VP, VP, C.POINTER(C.c_float)], C.c_int),
+    "mcpt_lbvh_build": ([VP, C.POINTER(LbvhDump)], C.c_int),
+    "mcpt_trace_closest": ([VP, VP, C.c_int32, VP, VP, VP], C.c_int),
+    "mcpt_trace_any": ([VP, VP, C.c_int32, VP], C.c_int),
+    "mcpt_camera_init": ([C.POINTER(CameraDesc), C.POINTER(Camera)], C.c_int),
+    "mcpt_film_create": ([VP, C.c_int32, C.c_int32, C.POINTER(VP)], C.c_int),
+    "mcpt_film_destroy": ([VP], C.c_int),
+    "mcpt_film_clear": ([VP], C.c_int),
+    "mcpt_film_read": ([VP, VP], C.c_int),
+    "mcpt_film_write": ([VP, VP], C.c_int),
+    "mcpt_render": ([VP, C.POINTER(Camera), C.POINTER(RenderOpts), VP, C.POINTER(RenderStats)], C.c_int),
+    "mcpt_resolve": ([VP, C.c_int32, C.c_int32, C.c_float, VP], C.c_int),
+    "mcpt_gen_scene": ([C.c_int32, C.c_uint64, C.c_int64, VP, VP, VP, C.POINTER(GenSizes)], C.c_int),
+    "mcpt_gen_sky": ([C.c_int32, C.c_int32, VP], C.c_int),
+    "mcpt_gen_camera": ([C.c_int32, C.c_int32, C.c_int32, C.POINTER(CameraDesc)], C.c_int),
+}
+EXPORTS = sorted(_SIGS)
+for _name, (_args, _res) in _SIGS.items():
+    _f = getattr(_lib, _name)
+    _f.argtypes = _args
+    _f.restype = _res
+
+
+class McptError(RuntimeError):
+    def __init__(self, code, where):
+        msg = _lib.mcpt_last_error().decode(errors="replace")
+        super().__init__(f"{where} failed ({code}): {msg}")
+        self.code = code
+
+
+def _check(code, where):
+    if code < 0:
+        raise McptError(code, where)
+    return code
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def version():
+    return _lib.mcpt_version().decode()
+
+
+# ------------------------------------------------------------ host helpers
+def gen_scene(kind, seed=1, param=0):
+    """Procedural scene -> (hot, cold, materials) numpy record arrays."""
+    sz = GenSizes()
+    _check(_lib.mcpt_gen_scene(kind, seed, param, None, None, None, C.byref(sz)), "mcpt_gen_scene")
+    hot = np.zeros(sz.n_tris, HOT_DTYPE)
+    cold = np.zeros(sz.n_tris, COLD_DTYPE)
+    mats = np.zeros(sz.n_materials, MAT_DTYPE)
+    _check(_lib.mcpt_gen_scene(kind, seed, param, _ptr(hot), _ptr(cold), _ptr(mats), None), "mcpt_gen_scene")
+    return hot, cold, mats
+
+
+def gen_sky(width=512, height=256):
+    rgba = np.zeros((height, width, 4), np.float32)
+    _check(_lib.mcpt_gen_sky(width, height, _ptr(rgba)), "mcpt_gen_sky")
+    return rgba
+
+
+def gen_camera(kind, width, height):
+    d = CameraDesc()
+    _check(_lib.mcpt_gen_camera(kind, width, height, C.byref(d)), "mcpt_gen_camera")
+    return d
+
+
+def camera_init(desc):
+    cam = Camera()
+    _check(_lib.mcpt_camera_init(C.byref(desc), C.byref(cam)), "mcpt_camera_init")
+    return cam
+
+
+def make_env(rgba, intensity=1.0, rotation_radians=0.0):
+    rgba = np.ascontiguousarray(rgba, np.float32)
+    e = EnvDesc(rgba.shape[1], rgba.shape[0], intensity, rotation_radians, rgba.ctypes.data)
+    e._keep = rgba
+    return e
+
+
+# ------------------------------------------------------------ device objects
+class Context:
+    """A device context. Closing it first closes every scene and film made
+    from it (the C library requires children to go before their context)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0):
+        self.h = VP()
+        self._children = weakref.WeakSet()
+        _check(_lib.mcpt_ctx_create(device, hbm_budget_bytes, C.byref(self.h)), "mcpt_ctx_create")
+
+    def close(self):
+        if self.h:
+            for child in list(self._children):
+                child.close()
+            _lib.mcpt_ctx_destroy(self.h)
+            self.h = VP()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def memory(self):
+        used, budget = C.c_uint64(), C.c_uint64()
+        _check(_lib.mcpt_ctx_memory(self.h, C.byref(used), C.byref(budget)), "mcpt_ctx_memory")
+        return used.value, budget.value
+
+    def describe(self):
+        buf = C.create_string_buffer(1024)
+        _check(_lib.mcpt_ctx_describe(self.h, buf, 1024), "mcpt_ctx_describe")
+        return buf.value.decode()
+
+    def scene(self, hot, cold, mats, env=None):
+        return Scene(self, hot, cold, mats, env)
+
+    def film(self, width, height):
+        return Film(self, width, height)
+
+
+class Scene:
+    def __init__(self, ctx, hot, cold, mats, env=None):
+        self.ctx = ctx
+        self.hot = np.ascontiguousarray(hot, HOT_DTYPE)
+        self.cold = np.ascontiguousarray(cold, COLD_DTYPE)
+        self.mats = np.ascontiguousarray(mats, MAT_DTYPE)
+        self.n = len(self.hot)
+        self.h = VP()
+        ctx._children.add(self)
+        envp = C.byref(env) if env is not None else None
+        _check(_lib.mcpt_scene_create(ctx.h, _ptr(self.hot), _ptr(self.cold), self.n, _ptr(self.mats),
+                                      len(self.mats), envp, C.byref(self.h)), "mcpt_scene_create")
+
+    def close(self):
+        if self.h:
+            if self.ctx.h:
+                _lib.mcpt_scene_destroy(self.h)
+            self.h = VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def build(self, dump=False):
+        """Run the GPU LBVH build; with dump=True return every stage as arrays."""
+        if not dump:
+            d = LbvhDump()
+            _check(_lib.mcpt_lbvh_build(self.h, C.byref(d)), "mcpt_lbvh_build")
+            return {"max_depth": d.max_depth, "build_ms": d.build_ms}
+        n, ni = self.n, max(self.n - 1, 0)
+        out = {
+            "morton": np.zeros(n, np.uint32), "sorted_keys": np.zeros(n, np.uint32),
+            "sorted_ids": np.zeros(n, np.int32), "left": np.zeros(ni, np.int32), "right": np.zeros(ni, np.int32),
+            "left_is_leaf": np.zeros(ni, np.uint8), "right_is_leaf": np.zeros(ni, np.uint8),
+            "parent": np.zeros(ni, np.int32), "leaf_parent": np.zeros(n, np.int32),
+            "internal_bounds": np.zeros(ni, AABB_DTYPE), "leaf_bounds": np.zeros(n, AABB_DTYPE),
+            "scene_bounds": np.zeros(1, AABB_DTYPE),
+        }
+        d = LbvhDump(*[_ptr(out[k]) if out[k].size else None for k in
+                       ("morton", "sorted_keys", "sorted_ids", "left", "right", "left_is_leaf", "right_is_leaf",
+                        "parent", "leaf_parent", "internal_bounds", "leaf_bounds", "scene_bounds")])
+        _check(_lib.mcpt_lbvh_build(self.h, C.byref(d)), "mcpt_lbvh_build")
+        out["max_depth"] = d.max_depth
+        out["build_ms"] = d.build_ms
+        return out
+
+    def lights(self):
+        n = C.c_int32()
+        _check(_lib.mcpt_scene_light_count(self.h, C.byref(n)), "mcpt_scene_light_count")
+        out = np.zeros(n.value, LIGHT_DTYPE)
+        if n.value:
+            _check(_lib.mcpt_scene_lights(self.h, _ptr(out)), "mcpt_scene_lights")
+        return out
+
+    def env_tables(self, width, height):
+        row = np.zeros(height + 1, np.float32)
+        col = np.zeros(height * (width + 1), np.float32)
+        tot = C.c_float()
+        _check(_lib.mcpt_scene_env_tables(self.h, _ptr(row), _ptr(col), C.byref(tot)), "mcpt_scene_env_tables")
+        return row, col, tot.value
+
+    def trace_closest(self, rays, counts=False):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        steps = np.zeros(len(rays), np.uint32) if counts else None
+        leaves = np.zeros(len(rays), np.uint32) if counts else None
+        _check(_lib.mcpt_trace_closest(self.h, _ptr(rays), len(rays), _ptr(hits), _ptr(steps), _ptr(leaves)),
+               "mcpt_trace_closest")
+        return (hits, steps, leaves) if counts else hits
+
+    def trace_any(self, rays):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        occ = np.zeros(len(rays), np.uint8)
+        _check(_lib.mcpt_trace_any(self.h, _ptr(rays), len(rays), _ptr(occ)), "mcpt_trace_any")
+        return occ
+
+    def render(self, camera, film, mode=MODE_PERSISTENT, spp=1, sample_begin=0, max_depth=8, seed=0,
+               refill_lanes=0, flags=0):
+        o = RenderOpts(mode, spp, sample_begin, max_depth, seed, refill_lanes, flags)
+        st = RenderStats()
+        _check(_lib.mcpt_render(self.h, C.byref(camera), C.byref(o), film.h, C.byref(st)), "mcpt_render")
+        return st.as_dict()
+
+
+class Film:
+    def __init__(self, ctx, width, height):
+        self.ctx, self.w, self.hgt = ctx, width, height
+        self.h = VP()
+        ctx._children.add(self)
+        _check(_lib.mcpt_film_create(ctx.h, width, height, C.byref(self.h)), "mcpt_film_create")
+
+    def close(self):
+        if self.h:
+            if self.ctx.h:
+                _lib.mcpt_film_destroy(self.h)
+            self.h = VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clear(self):
+        _check(_lib.mcpt_film_clear(self.h), "mcpt_film_clear")
+
+    def read(self):
+        out = np.zeros((self.hgt, self.w, 3), np.float32)
+        _check(_lib.mcpt_film_read(self.h, _ptr(out)), "mcpt_film_read")
+        return out
+
+    def write(self, rgb):
+        rgb = np.ascontiguousarray(rgb, np.float32).reshape(self.hgt, self.w, 3)
+        _check(_lib.mcpt_film_write(self.h, _ptr(rgb)), "mcpt_film_write")
+
+    def resolve(self, spp, tonemap=TONEMAP_NONE, exposure=1.0):
+        out = np.zeros((self.hgt, self.w, 3), np.float32)
+        _check(_lib.mcpt_resolve(self.h, spp, tonemap, exposure, _ptr(out)), "mcpt_resolve")
+        return out

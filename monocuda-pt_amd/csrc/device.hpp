@@ -1,0 +1,179 @@
+// device.hpp — types and per-ray arithmetic shared by the gfx950 kernels.
+//
+// Every floating-point expression keeps the operand order of the
+// reference's device functions (MonoCUDA-PT src/path_tracer.cu, cited per
+// function) and the library is compiled with -ffp-contract=off and IEEE
+// division/sqrt, so a ray produces the same bits here as in a host build of
+// the same formulas. Elementary functions come from mcpt_fmath.h, random
+// numbers from mcpt_pcg32.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mcpt.h"
+#include "mcpt_fmath.h"
+#include "mcpt_pcg32.h"
+
+#define MCPT_DEV __device__ __forceinline__
+#define MCPT_HDI __host__ __device__ __forceinline__
+
+namespace mcpt {
+
+// ------------------------------------------------------------------ vec3
+// src/path_tracer.cu:105-163
+struct vec3 {
+    float x, y, z;
+    MCPT_HDI vec3() : x(0), y(0), z(0) {}
+    MCPT_HDI vec3(float a, float b, float c) : x(a), y(b), z(c) {}
+    MCPT_HDI vec3 operator+(const vec3& o) const { return vec3(x + o.x, y + o.y, z + o.z); }
+    MCPT_HDI vec3 operator-(const vec3& o) const { return vec3(x - o.x, y - o.y, z - o.z); }
+    MCPT_HDI vec3 operator*(const vec3& o) const { return vec3(x * o.x, y * o.y, z * o.z); }
+    MCPT_HDI vec3 operator-() const { return vec3(-x, -y, -z); }
+    MCPT_HDI vec3 operator*(float n) const { return vec3(x * n, y * n, z * n); }
+    MCPT_HDI vec3 operator/(float n) const { return vec3(x / n, y / n, z / n); }
+    MCPT_HDI float length_squared() const { return x * x + y * y + z * z; }
+    MCPT_HDI float length() const { return __builtin_sqrtf(length_squared()); }
+    MCPT_HDI vec3 cross(const vec3& o) const { return vec3(y * o.z - z * o.y, z * o.x - x * o.z, x * o.y - y * o.x); }
+    MCPT_HDI float dot(const vec3& o) const { return x * o.x + y * o.y + z * o.z; }
+    MCPT_HDI vec3 normalize() const {
+        float l = __builtin_sqrtf(length_squared());
+        return vec3(x / l, y / l, z / l);
+    }
+    MCPT_HDI vec3 reflect(const vec3& n) const { return *this - n * (2.0f * dot(n)); }
+    // src/path_tracer.cu:146-163
+    MCPT_HDI vec3 refract(const vec3& n, float eta) const {
+        vec3 d = normalize();
+        float cos1 = fminf((-d).dot(n), 1.0f);
+        float theta1 = mf_acos(cos1);
+        float sin2 = mf_sin(theta1) * eta;
+        if (sin2 > 1.f) return d.reflect(n);
+        float theta2 = mf_asin(sin2);
+        vec3 tangent = d + n * cos1;
+        float tl = tangent.length();
+        if (tl > 1e-8f) tangent = tangent / tl;
+        return tangent * mf_sin(theta2) - n * mf_cos(theta2);
+    }
+};
+MCPT_HDI vec3 load3(const float* p) { return vec3(p[0], p[1], p[2]); }
+
+// ------------------------------------------------------------- BVH layout
+// One 64-byte record per internal node holding both children's boxes and
+// references, so a traversal step is four 16-byte loads from one 64-byte
+// segment. (The reference stores one box per node and reads the two
+// children's 64-byte BVH_Node records, src/path_tracer.cu:1006-1015.)
+// Child reference r >= 0: internal node r; r < 0: leaf ~r.
+struct alignas(16) BvhNode {
+    float4 a;  // left.min.xyz, left.max.x
+    float4 b;  // left.max.yz, right.min.xy
+    float4 c;  // right.min.z, right.max.xyz
+    int4 r;    // left ref, right ref, parent (node<<1 | side, root = -1), unused
+};
+static_assert(sizeof(BvhNode) == 64, "BVH node must be one 64-byte segment");
+
+struct Box {
+    vec3 mn, mx;
+};
+
+// hit_aabb (src/path_tracer.cu:1467-1488) with 1/d hoisted out of the loop
+// (the same IEEE division per ray). Evaluating all three slabs before the
+// single test gives the same answer as the per-axis early exit: tnear only
+// grows, tfar only shrinks, and fmaxf/fminf drop NaN operands.
+MCPT_DEV bool slab(float mnx, float mny, float mnz, float mxx, float mxy, float mxz, const vec3& o, const vec3& inv,
+                   float tfar) {
+    float tnear = 0.0001f;
+    float t0 = (mnx - o.x) * inv.x, t1 = (mxx - o.x) * inv.x;
+    if (inv.x < 0.0f) {
+        float s = t0;
+        t0 = t1;
+        t1 = s;
+    }
+    tnear = fmaxf(tnear, t0);
+    tfar = fminf(tfar, t1);
+    t0 = (mny - o.y) * inv.y;
+    t1 = (mxy - o.y) * inv.y;
+    if (inv.y < 0.0f) {
+        float s = t0;
+        t0 = t1;
+        t1 = s;
+    }
+    tnear = fmaxf(tnear, t0);
+    tfar = fminf(tfar, t1);
+    t0 = (mnz - o.z) * inv.z;
+    t1 = (mxz - o.z) * inv.z;
+    if (inv.z < 0.0f) {
+        float s = t0;
+        t0 = t1;
+        t1 = s;
+    }
+    tnear = fmaxf(tnear, t0);
+    tfar = fminf(tfar, t1);
+    return !(tfar <= tnear);
+}
+
+// hit_triangle (src/path_tracer.cu:715-741)
+MCPT_DEV bool hit_triangle(const vec3& v0, const vec3& v1, const vec3& v2, const vec3& o, const vec3& d, float& tout) {
+    vec3 e1 = v1 - v0;
+    vec3 e2 = v2 - v0;
+    vec3 n = e1.cross(e2).normalize();
+    float den = d.dot(n);
+    if (fabsf(den) < 1e-6f) return false;
+    float t = (v0 - o).dot(n) / den;
+    if (t < 0.0001f) return false;
+    vec3 p = o + d * t;
+    if (n.dot((v1 - v0).cross(p - v0)) < 0) return false;
+    if (n.dot((v2 - v1).cross(p - v1)) < 0) return false;
+    if (n.dot((v0 - v2).cross(p - v2)) < 0) return false;
+    tout = t;
+    return true;
+}
+
+// ------------------------------------------------------------ scene view
+struct EnvView {
+    const float4* texels = nullptr;
+    const float* row_cdf = nullptr;
+    const float* col_cdf = nullptr;
+    int w = 0, h = 0;
+    float intensity = 1.f, rot = 0.f, total = 0.f;
+    int enabled = 0;
+};
+
+// Everything a kernel needs to trace and shade, passed by value (no
+// __constant__ globals, so several scenes can be in flight at once).
+struct SceneView {
+    const BvhNode* nodes;
+    const float* leaf_hot;      // 9 floats per leaf, leaf (Morton) order
+    const int* leaf_prim;       // leaf -> original triangle index
+    const float* hot;           // 9 floats per triangle, original order
+    const mcpt_tri_cold* cold;  // original order
+    const mcpt_material* mats;
+    const mcpt_light* lights;
+    int n_lights;
+    int n_tris;
+    int has_medium;
+    int stack_depth;  // deepest leaf in internal levels (<= 64)
+    EnvView env;
+};
+
+// ------------------------------------------------------ wave64 utilities
+MCPT_DEV int lane_id() { return __lane_id(); }
+MCPT_DEV uint64_t ballot64(bool p) { return __ballot(p); }
+// number of set bits of m below the calling lane
+MCPT_DEV int rank_below(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+// One atomicAdd per wave for all lanes with `take`; returns each taking
+// lane's slot. Must be called by every active lane of the wave.
+MCPT_DEV int wave_push(int* counter, bool take) {
+    uint64_t m = ballot64(take);
+    int base = 0;
+    if (m) {
+        int leader = __builtin_ctzll(m);
+        if (lane_id() == leader) base = atomicAdd(counter, __popcll(m));
+        base = __shfl(base, leader);
+    }
+    return base + rank_below(m);
+}
+
+}  // namespace mcpt

@@ -1,0 +1,78 @@
+// host_common.hpp — host-side plumbing shared by the library's translation
+// units: error reporting, the budgeted device allocator, and the internal
+// objects behind the opaque handles of mcpt.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "mcpt.h"
+
+namespace mcpt {
+
+int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+#define MCPT_HIP(call)                                                                                       \
+    do {                                                                                                     \
+        hipError_t e_ = (call);                                                                              \
+        if (e_ != hipSuccess)                                                                                \
+            return ::mcpt::set_error(MCPT_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+    } while (0)
+
+#define MCPT_TRY(call)          \
+    do {                        \
+        int s_ = (call);        \
+        if (s_ < 0) return s_;  \
+    } while (0)
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    int cu_count = 0;
+    int clock_khz = 0;
+    char name[256] = {0};
+    // reusable wavefront workspace (see render.hip)
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    int64_t ws_capacity = 0;
+};
+
+// every device allocation of a context goes through these two calls, which
+// keep `used` within `budget`
+int dev_alloc(Ctx* ctx, void** p, size_t bytes);
+void dev_free(Ctx* ctx, void* p, size_t bytes);
+
+template <class T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t count = 0;
+    int alloc(Ctx* ctx, size_t n) {
+        release(ctx);
+        if (n == 0) return MCPT_OK;
+        void* p = nullptr;
+        int s = dev_alloc(ctx, &p, n * sizeof(T));
+        if (s < 0) return s;
+        ptr = (T*)p;
+        count = n;
+        return MCPT_OK;
+    }
+    void release(Ctx* ctx) {
+        if (ptr) dev_free(ctx, ptr, count * sizeof(T));
+        ptr = nullptr;
+        count = 0;
+    }
+    size_t bytes() const { return count * sizeof(T); }
+};
+
+// scene generators (scenegen.cpp)
+int gen_scene(int kind, uint64_t seed, int64_t param, mcpt_tri_hot* hot, mcpt_tri_cold* cold,
+              mcpt_material* materials, mcpt_gen_sizes* sizes);
+int gen_sky(int w, int h, float* rgba);
+int gen_camera(int kind, int w, int h, mcpt_camera_desc* d);
+
+}  // namespace mcpt

@@ -1,0 +1,491 @@
+// lbvh.hip — linear BVH build on gfx950.
+//
+// Stages (the reference does stages 1-3 on the host, src/path_tracer.cu
+// :1930-1972 and :4224-4245, and stages 4-5 on the GPU, :1063-1090):
+//   1. triangle boxes padded by 1e-3 and the scene bounds (two-level
+//      min/max reduction; exact, so order does not matter)
+//   2. 30-bit Morton codes of the centroids (expandBits/quantize/morton3D,
+//      :913-935)
+//   3. stable LSD radix sort of (code, index): 4 passes of 8 bits, each a
+//      tile histogram, one exclusive scan and an LDS-ranked scatter whose
+//      ranks come from wave64 ballots (8 ballots match the lanes holding
+//      the same digit, mbcnt ranks them)
+//   4. Karras radix tree (delta/determineRange/findSplit, :1337-1386,
+//      :1417-1447); each internal node records its children's references,
+//      and a leaf child's box is written straight into its parent's slot
+//   5. bottom-up refit: one thread per leaf climbs, an agent-scope acq_rel
+//      atomic per node lets the second arriver merge both child boxes
+//      (refitBoundsKernel, :1398-1415) and write the result into its own
+//      parent's slot.
+#include <algorithm>
+#include <cstring>
+
+#include "scene.hpp"
+
+namespace mcpt {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 16;  // keys per thread per radix tile
+constexpr int kTile = kBlock * kItems;
+
+__device__ __forceinline__ void tri_box(const float* t, float b[6]) {
+    const float eps = 1e-3f;  // triangle::calculate_bounding_box (:651-655)
+    b[0] = fminf(t[0], fminf(t[3], t[6])) - eps;
+    b[1] = fminf(t[1], fminf(t[4], t[7])) - eps;
+    b[2] = fminf(t[2], fminf(t[5], t[8])) - eps;
+    b[3] = fmaxf(t[0], fmaxf(t[3], t[6])) + eps;
+    b[4] = fmaxf(t[1], fmaxf(t[4], t[7])) + eps;
+    b[5] = fmaxf(t[2], fmaxf(t[5], t[8])) + eps;
+}
+
+__device__ __forceinline__ void merge6(float a[6], const float b[6]) {
+    for (int k = 0; k < 3; k++) a[k] = fminf(a[k], b[k]);
+    for (int k = 3; k < 6; k++) a[k] = fmaxf(a[k], b[k]);
+}
+
+// block-wide min/max of a 6-float box through LDS
+__device__ void block_reduce_box(float b[6], float* out) {
+    __shared__ float sh[6][kBlock];
+    for (int k = 0; k < 6; k++) sh[k][threadIdx.x] = b[k];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            for (int k = 0; k < 3; k++) sh[k][threadIdx.x] = fminf(sh[k][threadIdx.x], sh[k][threadIdx.x + s]);
+            for (int k = 3; k < 6; k++) sh[k][threadIdx.x] = fmaxf(sh[k][threadIdx.x], sh[k][threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; k++) out[k] = sh[k][0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_bounds_partial(const float* hot, int n, float* partial) {
+    float acc[6] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), -__builtin_inff(), -__builtin_inff(),
+                    -__builtin_inff()};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        float t[9];
+        for (int k = 0; k < 9; k++) t[k] = hot[(size_t)i * 9 + k];
+        float b[6];
+        tri_box(t, b);
+        merge6(acc, b);
+    }
+    block_reduce_box(acc, partial + 6 * blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bounds_final(const float* partial, int np, float* out) {
+    float acc[6] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), -__builtin_inff(), -__builtin_inff(),
+                    -__builtin_inff()};
+    for (int i = threadIdx.x; i < np; i += kBlock) merge6(acc, partial + 6 * i);
+    block_reduce_box(acc, out);
+}
+
+__device__ __forceinline__ uint32_t expand_bits(uint32_t p) {  // expandBits (:913-920)
+    uint32_t e = 0;
+    for (int i = 0; i < 10; i++) e |= ((p >> i) & 1u) << (3 * i);
+    return e;
+}
+
+__global__ __launch_bounds__(kBlock) void k_morton(const float* hot, int n, const float* sb, uint32_t* morton,
+                                                   uint32_t* keys, int* vals) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float* t = hot + (size_t)i * 9;
+    vec3 c = (vec3(t[0], t[1], t[2]) + vec3(t[3], t[4], t[5]) + vec3(t[6], t[7], t[8])) / 3.0f;
+    // quantize (:922-925): (v - min) / (max - min) * 1023, truncated
+    uint32_t qx = (uint32_t)((c.x - sb[0]) / (sb[3] - sb[0]) * 1023.0f);
+    uint32_t qy = (uint32_t)((c.y - sb[1]) / (sb[4] - sb[1]) * 1023.0f);
+    uint32_t qz = (uint32_t)((c.z - sb[2]) / (sb[5] - sb[2]) * 1023.0f);
+    uint32_t m = expand_bits(qx) | expand_bits(qy) << 1 | expand_bits(qz) << 2;
+    morton[i] = m;
+    keys[i] = m;
+    vals[i] = i;
+}
+
+// ------------------------------------------------------------ radix sort
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* keys, int n, int shift, int tiles,
+                                                       int* counts) {
+    __shared__ int hist[256];
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kTile;
+    for (int r = 0; r < kItems; r++) {
+        int i = base + r * kBlock + threadIdx.x;
+        if (i < n) atomicAdd(&hist[(keys[i] >> shift) & 255u], 1);
+    }
+    __syncthreads();
+    counts[threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
+}
+
+// exclusive scan of `total` ints in place, one block of 1024 threads
+__global__ __launch_bounds__(1024) void k_scan_exclusive(int* data, int total) {
+    __shared__ int sums[1024];
+    const int per = (total + 1023) / 1024;
+    const int b = threadIdx.x * per;
+    const int e = min(total, b + per);
+    int s = 0;
+    for (int i = b; i < e; i++) s += data[i];
+    sums[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+        int v = (int)threadIdx.x >= off ? sums[threadIdx.x - off] : 0;
+        __syncthreads();
+        sums[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = sums[threadIdx.x] - s;
+    for (int i = b; i < e; i++) {
+        int v = data[i];
+        data[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* kin, const int* vin, uint32_t* kout,
+                                                          int* vout, int n, int shift, int tiles, const int* offsets) {
+    __shared__ int running[256];
+    __shared__ int wave_cnt[kBlock / 64][256];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
+    running[tid] = offsets[tid * tiles + blockIdx.x];
+    for (int w = 0; w < kBlock / 64; w++) wave_cnt[w][tid] = 0;
+    __syncthreads();
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int base = blockIdx.x * kTile;
+    for (int r = 0; r < kItems; r++) {
+        int i = base + r * kBlock + tid;
+        bool valid = i < n;
+        uint32_t key = valid ? kin[i] : 0u;
+        int val = valid ? vin[i] : 0;
+        uint32_t digit = (key >> shift) & 255u;
+        // lanes of this wave holding the same digit
+        uint64_t peers = ballot64(valid);
+        for (int bit = 0; bit < 8; bit++) {
+            uint64_t b = ballot64((digit >> bit) & 1u);
+            peers &= ((digit >> bit) & 1u) ? b : ~b;
+        }
+        int wave_rank = __popcll(peers & below);
+        if (valid && wave_rank == 0) wave_cnt[wave][digit] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int pre = running[digit] + wave_rank;
+            for (int w = 0; w < wave; w++) pre += wave_cnt[w][digit];
+            kout[pre] = key;
+            vout[pre] = val;
+        }
+        __syncthreads();
+        int add = 0;
+        for (int w = 0; w < kBlock / 64; w++) {
+            add += wave_cnt[w][tid];
+            wave_cnt[w][tid] = 0;
+        }
+        running[tid] += add;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------- leaves
+__global__ __launch_bounds__(kBlock) void k_leaves(const float* hot, const int* ids, int n, float* leaf_hot,
+                                                   int* leaf_prim) {
+    int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    int p = ids[j];
+    leaf_prim[j] = p;
+    for (int k = 0; k < 9; k++) leaf_hot[(size_t)j * 9 + k] = hot[(size_t)p * 9 + k];
+}
+
+// --------------------------------------------------------- Karras tree
+__device__ __forceinline__ int delta(const uint32_t* k, int n, int i, int j) {  // :1337-1347
+    if (j < 0 || j >= n) return -1;
+    uint32_t a = k[i], b = k[j];
+    if (a == b) return 32 + __clz((uint32_t)(i ^ j));
+    return __clz(a ^ b);
+}
+
+__device__ __forceinline__ void store_slot(BvhNode* nodes, int node, int side, const float b[6]) {
+    float* f = reinterpret_cast<float*>(nodes + node);
+    const int o = side ? 6 : 0;
+    for (int k = 0; k < 6; k++) f[o + k] = b[k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* keys, int n, const float* leaf_hot, BvhNode* nodes,
+                                                   int* leaf_parent) {
+    int idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= n - 1) return;
+    // determineRange (:1349-1369)
+    int d = (delta(keys, n, idx, idx + 1) - delta(keys, n, idx, idx - 1)) >= 0 ? 1 : -1;
+    int dmin = delta(keys, n, idx, idx - d);
+    int lmax = 2;
+    while (delta(keys, n, idx, idx + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (delta(keys, n, idx, idx + (l + t) * d) > dmin) l += t;
+    int j = idx + l * d;
+    int first = min(idx, j), last = max(idx, j);
+    // findSplit (:1371-1386)
+    int cp = delta(keys, n, first, last);
+    int split = first, step = last - first;
+    do {
+        step = (step + 1) >> 1;
+        int ns = split + step;
+        if (ns < last && delta(keys, n, first, ns) > cp) split = ns;
+    } while (step > 1);
+
+    int* rf = reinterpret_cast<int*>(nodes + idx) + 12;
+    const int kids[2] = {split, split + 1};
+    const bool leafs[2] = {split == first, split + 1 == last};
+    for (int side = 0; side < 2; side++) {
+        int c = kids[side];
+        if (leafs[side]) {
+            rf[side] = ~c;
+            leaf_parent[c] = (idx << 1) | side;
+            float b[6];
+            const float* t = leaf_hot + (size_t)c * 9;
+            float tt[9];
+            for (int k = 0; k < 9; k++) tt[k] = t[k];
+            tri_box(tt, b);
+            store_slot(nodes, idx, side, b);
+        } else {
+            rf[side] = c;
+            reinterpret_cast<int*>(nodes + c)[14] = (idx << 1) | side;
+        }
+    }
+    if (idx == 0) rf[2] = -1;
+    rf[3] = 0;
+}
+
+// refitBoundsKernel (:1398-1415) on the two-box node layout
+__global__ __launch_bounds__(kBlock) void k_refit(BvhNode* nodes, const int* leaf_parent, int n, int* counters,
+                                                  float* root_box) {
+    int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    int p = leaf_parent[j] >> 1;
+    while (true) {
+        int old = __hip_atomic_fetch_add(&counters[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) return;
+        const float* f = reinterpret_cast<const float*>(nodes + p);
+        float b[6];
+        for (int k = 0; k < 3; k++) b[k] = fminf(f[k], f[6 + k]);
+        for (int k = 3; k < 6; k++) b[k] = fmaxf(f[k], f[6 + k]);
+        int pp = reinterpret_cast<const int*>(nodes + p)[14];
+        if (pp < 0) {
+            for (int k = 0; k < 6; k++) root_box[k] = b[k];
+            return;
+        }
+        store_slot(nodes, pp >> 1, pp & 1, b);
+        p = pp >> 1;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_depth(const BvhNode* nodes, const int* leaf_parent, int n, int* max_depth) {
+    int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    int d = 1;
+    int p = leaf_parent[j] >> 1;
+    while (true) {
+        int pp = reinterpret_cast<const int*>(nodes + p)[14];
+        if (pp < 0) break;
+        p = pp >> 1;
+        d++;
+    }
+    atomicMax(max_depth, d);
+}
+
+__global__ void k_fill_int(int* p, int n, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int blocks_for(int64_t n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+void to_aabb(const float* b, mcpt_aabb* o) {
+    for (int k = 0; k < 3; k++) {
+        o->min[k] = b[k];
+        o->max[k] = b[3 + k];
+    }
+    o->pad0 = o->pad1 = 0.f;
+}
+
+}  // namespace
+
+int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
+    Ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    const int n = s->n;
+    if (n < 1) return set_error(MCPT_ERR_INVALID, "empty scene");
+    hipEvent_t e0, e1;
+    MCPT_HIP(hipEventCreate(&e0));
+    MCPT_HIP(hipEventCreate(&e1));
+    MCPT_HIP(hipEventRecord(e0, st));
+
+    const int n_int = std::max(1, n - 1);
+    MCPT_TRY(s->nodes.alloc(ctx, n_int));
+    MCPT_TRY(s->leaf_hot.alloc(ctx, (size_t)n * 9));
+    MCPT_TRY(s->leaf_prim.alloc(ctx, n));
+    MCPT_TRY(s->root_box.alloc(ctx, 6));
+
+    const int tiles = blocks_for(n, kTile);
+    const int n_partial = std::min(1024, blocks_for(n));
+    DevBuf<float> partial, sb;
+    DevBuf<uint32_t> morton, keys_a, keys_b;
+    DevBuf<int> vals_a, vals_b, counts, leaf_parent, counters, depth;
+    int status = MCPT_OK;
+    auto fail = [&](int code) {
+        partial.release(ctx);
+        sb.release(ctx);
+        morton.release(ctx);
+        keys_a.release(ctx);
+        keys_b.release(ctx);
+        vals_a.release(ctx);
+        vals_b.release(ctx);
+        counts.release(ctx);
+        leaf_parent.release(ctx);
+        counters.release(ctx);
+        depth.release(ctx);
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return code;
+    };
+#define STEP(x)                       \
+    do {                              \
+        status = (x);                 \
+        if (status < 0) return fail(status); \
+    } while (0)
+#define HSTEP(x)                                                                                  \
+    do {                                                                                          \
+        hipError_t he_ = (x);                                                                     \
+        if (he_ != hipSuccess)                                                                    \
+            return fail(set_error(MCPT_ERR_HIP, "lbvh %s: %s", #x, hipGetErrorString(he_)));      \
+    } while (0)
+    STEP(partial.alloc(ctx, (size_t)n_partial * 6));
+    STEP(sb.alloc(ctx, 6));
+    STEP(morton.alloc(ctx, n));
+    STEP(keys_a.alloc(ctx, n));
+    STEP(keys_b.alloc(ctx, n));
+    STEP(vals_a.alloc(ctx, n));
+    STEP(vals_b.alloc(ctx, n));
+    STEP(counts.alloc(ctx, (size_t)256 * tiles));
+    STEP(leaf_parent.alloc(ctx, n));
+    STEP(counters.alloc(ctx, n_int));
+    STEP(depth.alloc(ctx, 1));
+
+    // 1-2. bounds and Morton codes
+    k_bounds_partial<<<n_partial, kBlock, 0, st>>>(s->hot.ptr, n, partial.ptr);
+    k_bounds_final<<<1, kBlock, 0, st>>>(partial.ptr, n_partial, sb.ptr);
+    k_morton<<<blocks_for(n), kBlock, 0, st>>>(s->hot.ptr, n, sb.ptr, morton.ptr, keys_a.ptr, vals_a.ptr);
+    HSTEP(hipGetLastError());
+    // 3. stable radix sort, 8 bits per pass over the 30 code bits
+    uint32_t* kin = keys_a.ptr;
+    uint32_t* kout = keys_b.ptr;
+    int* vin = vals_a.ptr;
+    int* vout = vals_b.ptr;
+    for (int shift = 0; shift < 30; shift += 8) {
+        k_radix_hist<<<tiles, kBlock, 0, st>>>(kin, n, shift, tiles, counts.ptr);
+        k_scan_exclusive<<<1, 1024, 0, st>>>(counts.ptr, 256 * tiles);
+        k_radix_scatter<<<tiles, kBlock, 0, st>>>(kin, vin, kout, vout, n, shift, tiles, counts.ptr);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    HSTEP(hipGetLastError());
+    // 4. leaves and the Karras tree
+    k_leaves<<<blocks_for(n), kBlock, 0, st>>>(s->hot.ptr, vin, n, s->leaf_hot.ptr, s->leaf_prim.ptr);
+    k_fill_int<<<blocks_for(n_int), kBlock, 0, st>>>(counters.ptr, n_int, 0);
+    k_fill_int<<<1, 64, 0, st>>>(depth.ptr, 1, 0);
+    if (n == 1) {
+        // One primitive: no internal node exists in the reference. The root
+        // here holds leaf 0 on both sides, so traversal tests it once per
+        // side and keeps the first (strict '<').
+        HSTEP(hipStreamSynchronize(st));
+        float t[9], b[6];
+        HSTEP(hipMemcpy(t, s->hot.ptr, sizeof(t), hipMemcpyDeviceToHost));
+        const float eps = 1e-3f;
+        b[0] = fminf(t[0], fminf(t[3], t[6])) - eps;
+        b[1] = fminf(t[1], fminf(t[4], t[7])) - eps;
+        b[2] = fminf(t[2], fminf(t[5], t[8])) - eps;
+        b[3] = fmaxf(t[0], fmaxf(t[3], t[6])) + eps;
+        b[4] = fmaxf(t[1], fmaxf(t[4], t[7])) + eps;
+        b[5] = fmaxf(t[2], fmaxf(t[5], t[8])) + eps;
+        BvhNode root;
+        std::memset(&root, 0, sizeof(root));
+        float* f = reinterpret_cast<float*>(&root);
+        for (int k = 0; k < 6; k++) f[k] = f[6 + k] = b[k];
+        root.r = make_int4(~0, ~0, -1, 0);
+        int lp = 0;
+        HSTEP(hipMemcpy(s->nodes.ptr, &root, sizeof(root), hipMemcpyHostToDevice));
+        HSTEP(hipMemcpy(s->root_box.ptr, b, sizeof(b), hipMemcpyHostToDevice));
+        HSTEP(hipMemcpy(leaf_parent.ptr, &lp, sizeof(int), hipMemcpyHostToDevice));
+        s->max_depth = 1;
+    } else {
+        k_karras<<<blocks_for(n - 1), kBlock, 0, st>>>(kin, n, s->leaf_hot.ptr, s->nodes.ptr, leaf_parent.ptr);
+        // 5. refit
+        k_refit<<<blocks_for(n), kBlock, 0, st>>>(s->nodes.ptr, leaf_parent.ptr, n, counters.ptr, s->root_box.ptr);
+        k_depth<<<blocks_for(n), kBlock, 0, st>>>(s->nodes.ptr, leaf_parent.ptr, n, depth.ptr);
+        HSTEP(hipGetLastError());
+        int md = 0;
+        HSTEP(hipMemcpyAsync(&md, depth.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+        HSTEP(hipStreamSynchronize(st));
+        s->max_depth = md;
+    }
+    HSTEP(hipEventRecord(e1, st));
+    HSTEP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HSTEP(hipEventElapsedTime(&ms, e0, e1));
+    s->last_build_ms = ms;
+    if (s->max_depth < 1 || s->max_depth > 64)
+        return fail(set_error(MCPT_ERR_INVALID, "BVH depth %d outside [1, 64]", s->max_depth));
+
+    if (dump) {
+        std::vector<BvhNode> nodes(n_int);
+        std::vector<int> lpar(n);
+        float root[6];
+        HSTEP(hipMemcpy(nodes.data(), s->nodes.ptr, n_int * sizeof(BvhNode), hipMemcpyDeviceToHost));
+        HSTEP(hipMemcpy(lpar.data(), leaf_parent.ptr, n * sizeof(int), hipMemcpyDeviceToHost));
+        HSTEP(hipMemcpy(root, s->root_box.ptr, sizeof(root), hipMemcpyDeviceToHost));
+        if (dump->morton) HSTEP(hipMemcpy(dump->morton, morton.ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (dump->sorted_keys) HSTEP(hipMemcpy(dump->sorted_keys, kin, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (dump->sorted_ids) HSTEP(hipMemcpy(dump->sorted_ids, vin, n * sizeof(int), hipMemcpyDeviceToHost));
+        if (dump->scene_bounds) {
+            float b[6];
+            HSTEP(hipMemcpy(b, sb.ptr, sizeof(b), hipMemcpyDeviceToHost));
+            to_aabb(b, dump->scene_bounds);
+        }
+        auto slot = [&](int parent_code, float out[6]) {
+            const float* f = reinterpret_cast<const float*>(&nodes[parent_code >> 1]);
+            const int o = (parent_code & 1) ? 6 : 0;
+            for (int k = 0; k < 6; k++) out[k] = f[o + k];
+        };
+        if (n > 1) {
+            for (int i = 0; i < n - 1; i++) {
+                const int4 r = nodes[i].r;
+                if (dump->left) dump->left[i] = r.x >= 0 ? r.x : ~r.x;
+                if (dump->right) dump->right[i] = r.y >= 0 ? r.y : ~r.y;
+                if (dump->left_is_leaf) dump->left_is_leaf[i] = r.x < 0;
+                if (dump->right_is_leaf) dump->right_is_leaf[i] = r.y < 0;
+                if (dump->parent) dump->parent[i] = r.z < 0 ? -1 : (r.z >> 1);
+                if (dump->internal_bounds) {
+                    float b[6];
+                    if (r.z < 0) std::memcpy(b, root, sizeof(b));
+                    else slot(r.z, b);
+                    to_aabb(b, &dump->internal_bounds[i]);
+                }
+            }
+        }
+        for (int j = 0; j < n; j++) {
+            if (dump->leaf_parent) dump->leaf_parent[j] = lpar[j] >> 1;
+            if (dump->leaf_bounds) {
+                float b[6];
+                slot(lpar[j], b);
+                to_aabb(b, &dump->leaf_bounds[j]);
+            }
+        }
+        dump->max_depth = s->max_depth;
+        dump->build_ms = ms;
+    }
+    s->built = true;
+    fail(MCPT_OK);
+#undef STEP
+#undef HSTEP
+    return MCPT_OK;
+}
+
+}  // namespace mcpt

@@ -1,0 +1,637 @@
+// render.hip — the integrators on gfx950.
+//
+//   megakernel   one lane per pixel path (trace_sample_kernel,
+//                src/path_tracer.cu:2900-3059)
+//   wavefront    camera -> [extend -> shade -> shadow] x depth -> finalize
+//                (renderWithWavefront, :3757-3840), with
+//                - ray queues of 32-byte records (origin+time, direction+path)
+//                  so extend reads each ray with two coalesced 16-byte loads,
+//                - the 16-byte PackedHit written per queue slot,
+//                - queue pushes compacted by wave64 ballots: one atomic per
+//                  wave and per queue (:3573, :3625, :3691 take one per lane),
+//                - queue lengths kept on the device: every kernel reads its
+//                  count from HBM, so a whole sample is enqueued without the
+//                  per-bounce host round trip of :3816.
+//   persistent   the extend kernel runs a fixed grid; a wave refills its idle
+//                lanes from a global work counter once `refill` lanes are
+//                idle (extendKernelPersistent, :3416-3468, on 64-lane waves).
+#include <algorithm>
+#include <cstring>
+
+#include "scene.hpp"
+#include "shade.hpp"
+
+namespace mcpt {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct alignas(16) RayItem {
+    float4 ot;  // origin xyz, time
+    float4 dp;  // direction xyz, path index (int bits)
+};
+struct alignas(16) ShadowItem {
+    float4 om;  // origin xyz, max distance
+    float4 dp;  // direction xyz, path index (int bits)
+    float4 c;   // contribution rgb, unused
+};
+
+// counters[] slots
+enum { C_Q0 = 0, C_Q1 = 1, C_SHADOW = 2, C_WORK = 3, C_EXT_TOTAL = 4, C_SHD_TOTAL = 6, C_STEPS = 8, C_LEAVES = 10 };
+
+struct Workspace {
+    RayItem* q[2];
+    mcpt_packed_hit* hits;
+    ShadowItem* sq;
+    float4* thr;  // throughput rgb, previous bsdf pdf
+    float4* smp;  // sample rgb, previous-bounce-specular flag
+    uint64_t* rng;
+    int* counters;  // int slots; the *_TOTAL and count slots at 4.. are u64
+};
+
+__device__ __forceinline__ uint64_t* u64slot(int* c, int i) { return reinterpret_cast<uint64_t*>(c + i); }
+
+// ------------------------------------------------------------ counters
+__global__ void k_counters_init(int* c, int cur, int count) {
+    if (threadIdx.x == 0) {
+        c[cur] = count;
+        c[1 - cur] = 0;
+        c[C_SHADOW] = 0;
+        c[C_WORK] = 0;
+    }
+}
+// start of a bounce: record the previous bounce's queue lengths, clear the
+// queues this bounce fills
+__global__ void k_counters_bounce(int* c, int cur) {
+    if (threadIdx.x == 0) {
+        c[1 - cur] = 0;
+        c[C_SHADOW] = 0;
+        c[C_WORK] = 0;
+        *u64slot(c, C_EXT_TOTAL) += (uint64_t)c[cur];
+    }
+}
+__global__ void k_counters_shadow_total(int* c) {
+    if (threadIdx.x == 0) *u64slot(c, C_SHD_TOTAL) += (uint64_t)c[C_SHADOW];
+}
+__global__ void k_counters_zero_totals(int* c) {
+    if (threadIdx.x < 16) c[threadIdx.x] = 0;
+}
+
+// ---------------------------------------------------------------- camera
+// generateCameraRaysKernel (:3342-3374)
+__global__ __launch_bounds__(kBlock) void k_camera(mcpt_camera cam, int sample, uint64_t seed, Workspace ws, int cur) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    const int W = cam.width, H = cam.height;
+    if (p >= W * H) return;
+    Rng rng;
+    mcpt_pcg32_for_path(&rng.g, (uint32_t)p, (uint32_t)sample, seed);
+    vec3 o, d;
+    float time;
+    camera_ray(cam, p % W, p / W, rng, o, d, time);
+    RayItem it;
+    it.ot = make_float4(o.x, o.y, o.z, time);
+    it.dp = make_float4(d.x, d.y, d.z, __int_as_float(p));
+    ws.q[cur][p] = it;
+    ws.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    ws.smp[p] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    ws.rng[p] = rng.g.state;
+}
+
+// ---------------------------------------------------------------- extend
+template <bool COUNT>
+__device__ __forceinline__ void count_flush(int* counters, uint32_t ns, uint32_t nl) {
+    if (!COUNT) return;
+    // wave-reduce, one atomic per wave
+    uint64_t s = ns, l = nl;
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off);
+        l += __shfl_xor(l, off);
+    }
+    if (lane_id() == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(u64slot(counters, C_STEPS)), (unsigned long long)s);
+        atomicAdd(reinterpret_cast<unsigned long long*>(u64slot(counters, C_LEAVES)), (unsigned long long)l);
+    }
+}
+
+__device__ __forceinline__ mcpt_packed_hit pack_hit(const SceneView& sv, const Trav& s) {
+    mcpt_packed_hit h;
+    h.t = s.t;
+    if (s.leaf >= 0) {
+        h.prim_type = 1;
+        h.prim_index = sv.leaf_prim[s.leaf];
+        h.did_hit = 1;
+    } else {
+        h.prim_type = -1;
+        h.prim_index = -1;
+        h.did_hit = 0;
+    }
+    return h;
+}
+
+// extendKernel (:3376-3397): one lane per queued ray
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_extend(SceneView sv, Workspace ws, int cur) {
+    extern __shared__ int lds_stack[];
+    const int slot = blockIdx.x * kBlock + threadIdx.x;
+    const int count = ws.counters[cur];
+    uint32_t ns = 0, nl = 0;
+    if (slot < count) {
+        const RayItem it = ws.q[cur][slot];
+        Trav s;
+        trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+        trace_closest<COUNT>(sv, s, lds_stack + threadIdx.x, kBlock, ns, nl);
+        ws.hits[slot] = pack_hit(sv, s);
+    }
+    count_flush<COUNT>(ws.counters, ns, nl);
+}
+
+// extendKernelPersistent (:3416-3468): lanes keep their traversal across
+// iterations; a wave with at least `refill` idle lanes takes that many queue
+// slots with one atomic and hands them out by ballot rank.
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Workspace ws, int cur, int refill) {
+    extern __shared__ int lds_stack[];
+    int* stk = lds_stack + threadIdx.x;
+    const int total = ws.counters[cur];
+    const int lane = lane_id();
+    Trav s;
+    bool has = false;
+    int slot = -1;
+    bool exhausted = false;  // uniform across the wave
+    uint32_t ns = 0, nl = 0;
+    while (true) {
+        if (!exhausted) {
+            const uint64_t idle = ballot64(!has);
+            const int nidle = __popcll(idle);
+            if (nidle >= refill) {
+                const int leader = __builtin_ctzll(idle);
+                int first = 0;
+                if (lane == leader) first = atomicAdd(&ws.counters[C_WORK], nidle);
+                first = __shfl(first, leader);
+                if (!has) {
+                    const int q = first + rank_below(idle);
+                    if (q < total) {
+                        const RayItem it = ws.q[cur][q];
+                        trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+                        slot = q;
+                        has = true;
+                    }
+                }
+                if (first + nidle >= total) exhausted = true;
+            }
+        }
+        if (exhausted && ballot64(has) == 0) break;
+        if (has) {
+            if (step_closest<COUNT>(sv, s, stk, kBlock, ns, nl)) {
+                ws.hits[slot] = pack_hit(sv, s);
+                has = false;
+            }
+        }
+    }
+    count_flush<COUNT>(ws.counters, ns, nl);
+}
+
+// ----------------------------------------------------------------- shade
+// shadeKernel (:3470-3696)
+__global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, Workspace ws, int cur, int depth, vec3 bg, float rr_cap,
+                                                  uint64_t seed) {
+    const int slot = blockIdx.x * kBlock + threadIdx.x;
+    const int count = ws.counters[cur];
+    const bool active = slot < count;
+    Bounce out;
+    out.next = false;
+    out.shadow = false;
+    int path = 0;
+    float time = 0.f;
+    if (active) {
+        const RayItem it = ws.q[cur][slot];
+        const mcpt_packed_hit hit = ws.hits[slot];
+        path = __float_as_int(it.dp.w);
+        time = it.ot.w;
+        const vec3 ro(it.ot.x, it.ot.y, it.ot.z), rd(it.dp.x, it.dp.y, it.dp.z);
+        const float4 thr = ws.thr[path], smp = ws.smp[path];
+        PathState ps;
+        ps.thr = vec3(thr.x, thr.y, thr.z);
+        ps.prev_pdf = thr.w;
+        ps.smp = vec3(smp.x, smp.y, smp.z);
+        ps.prev_spec = smp.w != 0.0f;
+        Rng rng;
+        rng.g.state = ws.rng[path];
+        rng.g.inc = ((uint64_t)(uint32_t)path << 1u) | 1u;
+        shade_bounce(sv, bg, rr_cap, depth, ro, rd, hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out);
+        ws.thr[path] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
+        ws.smp[path] = make_float4(ps.smp.x, ps.smp.y, ps.smp.z, ps.prev_spec ? 1.0f : 0.0f);
+        ws.rng[path] = rng.g.state;
+    }
+    // compacted pushes: every lane of the wave takes part
+    const int nslot = wave_push(&ws.counters[1 - cur], out.next);
+    if (out.next) {
+        RayItem it;
+        it.ot = make_float4(out.no.x, out.no.y, out.no.z, time);
+        it.dp = make_float4(out.nd.x, out.nd.y, out.nd.z, __int_as_float(path));
+        ws.q[1 - cur][nslot] = it;
+    }
+    const int sslot = wave_push(&ws.counters[C_SHADOW], out.shadow);
+    if (out.shadow) {
+        ShadowItem si;
+        si.om = make_float4(out.so.x, out.so.y, out.so.z, out.smax);
+        si.dp = make_float4(out.sd.x, out.sd.y, out.sd.z, __int_as_float(path));
+        si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, 0.0f);
+        ws.sq[sslot] = si;
+    }
+}
+
+// ---------------------------------------------------------------- shadow
+// shadowKernel (:3698-3728)
+__global__ __launch_bounds__(kBlock) void k_shadow(SceneView sv, Workspace ws) {
+    extern __shared__ int lds_stack[];
+    const int slot = blockIdx.x * kBlock + threadIdx.x;
+    if (slot >= ws.counters[C_SHADOW]) return;
+    const ShadowItem si = ws.sq[slot];
+    const int path = __float_as_int(si.dp.w);
+    const float tr = shadow_transmittance(sv, vec3(si.om.x, si.om.y, si.om.z), vec3(si.dp.x, si.dp.y, si.dp.z),
+                                          si.om.w, lds_stack + threadIdx.x, kBlock);
+    if (tr > 0.0f) {
+        float4 s = ws.smp[path];
+        s.x = s.x + si.c.x * tr;
+        s.y = s.y + si.c.y * tr;
+        s.z = s.z + si.c.z * tr;
+        ws.smp[path] = s;
+    }
+}
+
+// -------------------------------------------------------------- finalize
+// finalizeSampleKernel (:3740-3755)
+__global__ __launch_bounds__(kBlock) void k_finalize(Workspace ws, float* accum, int n) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    const float4 s = ws.smp[p];
+    const vec3 c = clamp_sample(vec3(s.x, s.y, s.z));
+    accum[3 * p] = accum[3 * p] + c.x;
+    accum[3 * p + 1] = accum[3 * p + 1] + c.y;
+    accum[3 * p + 2] = accum[3 * p + 2] + c.z;
+}
+
+// ------------------------------------------------------------ megakernel
+// trace_sample_kernel (:2900-3059); 16x16 pixel tiles, one lane per path
+__global__ __launch_bounds__(kBlock) void k_megakernel(SceneView sv, mcpt_camera cam, int sample, uint64_t seed,
+                                                       int max_depth, float* accum, int* counters) {
+    extern __shared__ int lds_stack[];
+    int* stk = lds_stack + threadIdx.x;
+    const int px = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int py = blockIdx.y * 16 + (threadIdx.x >> 4);
+    uint32_t n_ext = 0, n_shd = 0;
+    if (px < cam.width && py < cam.height) {
+        const int p = py * cam.width + px;
+        Rng rng;
+        mcpt_pcg32_for_path(&rng.g, (uint32_t)p, (uint32_t)sample, seed);
+        vec3 ro, rd;
+        float time;
+        camera_ray(cam, px, py, rng, ro, rd, time);
+        PathState ps;
+        ps.thr = vec3(1.0f, 1.0f, 1.0f);
+        ps.smp = vec3(0.0f, 0.0f, 0.0f);
+        ps.prev_pdf = 1.0f;
+        ps.prev_spec = true;
+        const vec3 bg = load3(cam.background);
+        for (int depth = 0; depth < max_depth; depth++) {
+            Trav s;
+            trav_begin(s, ro, rd, 1e30f);
+            uint32_t ns = 0, nl = 0;
+            trace_closest<false>(sv, s, stk, kBlock, ns, nl);
+            n_ext++;
+            const int prim = s.leaf >= 0 ? sv.leaf_prim[s.leaf] : -1;
+            Bounce out;
+            shade_bounce(sv, bg, cam.rr_cap, depth, ro, rd, s.leaf >= 0, s.t, prim, ps, rng, out);
+            if (out.shadow) {
+                n_shd++;
+                const float tr = shadow_transmittance(sv, out.so, out.sd, out.smax, stk, kBlock);
+                if (tr > 0.0f) ps.smp = ps.smp + out.contrib * tr;
+            }
+            if (!out.next) break;
+            ro = out.no;
+            rd = out.nd;
+        }
+        const vec3 c = clamp_sample(ps.smp);
+        accum[3 * p] = accum[3 * p] + c.x;
+        accum[3 * p + 1] = accum[3 * p + 1] + c.y;
+        accum[3 * p + 2] = accum[3 * p + 2] + c.z;
+    }
+    uint64_t e = n_ext, h = n_shd;
+    for (int off = 32; off > 0; off >>= 1) {
+        e += __shfl_xor(e, off);
+        h += __shfl_xor(h, off);
+    }
+    if (lane_id() == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(u64slot(counters, C_EXT_TOTAL)), (unsigned long long)e);
+        atomicAdd(reinterpret_cast<unsigned long long*>(u64slot(counters, C_SHD_TOTAL)), (unsigned long long)h);
+    }
+}
+
+// --------------------------------------------------------------- resolve
+// linearToSrgb / toneMapChannel / resolve_kernel (:3062-3100)
+__device__ float linear_to_srgb(float x) {
+    x = fmaxf(x, 0.0f);
+    if (x <= 0.0031308f) return 12.92f * x;
+    return 1.055f * mf_pow(x, 1.0f / 2.4f) - 0.055f;
+}
+__device__ float tonemap_channel(float x, int tm) {
+    x = fmaxf(x, 0.f);
+    if (tm == MCPT_TONEMAP_REINHARD) return x / (1.f + x);
+    if (tm == MCPT_TONEMAP_ACES) {
+        float m = (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
+        return fminf(fmaxf(m, 0.0f), 1.0f);
+    }
+    return x;
+}
+__global__ __launch_bounds__(kBlock) void k_resolve(const float* accum, float* out, int n, int spp, int tm,
+                                                    float exposure) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= 3 * n) return;
+    const float a = accum[i] / float(spp) * exposure;
+    out[i] = linear_to_srgb(tonemap_channel(a, tm));
+}
+
+// ------------------------------------------------------------ trace hooks
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const mcpt_ray* rays, int n,
+                                                          mcpt_packed_hit* hits, uint32_t* steps, uint32_t* leaves) {
+    extern __shared__ int lds_stack[];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const mcpt_ray r = rays[i];
+    Trav s;
+    trav_begin(s, load3(r.origin), load3(r.dir), r.tmax);
+    uint32_t ns = 0, nl = 0;
+    trace_closest<COUNT>(sv, s, lds_stack + threadIdx.x, kBlock, ns, nl);
+    hits[i] = pack_hit(sv, s);
+    if (COUNT) {
+        steps[i] = ns;
+        leaves[i] = nl;
+    }
+}
+__global__ __launch_bounds__(kBlock) void k_trace_any(SceneView sv, const mcpt_ray* rays, int n, uint8_t* occ) {
+    extern __shared__ int lds_stack[];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const mcpt_ray r = rays[i];
+    Trav s;
+    trav_begin(s, load3(r.origin), load3(r.dir), r.tmax);
+    occ[i] = trace_any(sv, s, lds_stack + threadIdx.x, kBlock) ? 1 : 0;
+}
+
+int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
+size_t stack_bytes(const mcpt_scene_s* s) { return (size_t)kBlock * std::max(1, s->max_depth) * sizeof(int); }
+
+// The wavefront workspace lives in the context and grows to the largest
+// path capacity requested.
+int ensure_workspace(Ctx* ctx, int64_t capacity, Workspace& ws) {
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t szq = align(sizeof(RayItem) * capacity), szh = align(sizeof(mcpt_packed_hit) * capacity),
+                 szs = align(sizeof(ShadowItem) * capacity), sz4 = align(sizeof(float4) * capacity),
+                 szr = align(sizeof(uint64_t) * capacity), szc = 256;
+    const size_t total = 2 * szq + szh + szs + 2 * sz4 + szr + szc;
+    if (ctx->ws_capacity < capacity) {
+        if (ctx->ws) dev_free(ctx, ctx->ws, ctx->ws_bytes);
+        ctx->ws = nullptr;
+        ctx->ws_bytes = 0;
+        ctx->ws_capacity = 0;
+        void* p = nullptr;
+        MCPT_TRY(dev_alloc(ctx, &p, total));
+        ctx->ws = p;
+        ctx->ws_bytes = total;
+        ctx->ws_capacity = capacity;
+    }
+    char* b = (char*)ctx->ws;
+    const size_t cap_q = align(sizeof(RayItem) * ctx->ws_capacity), cap_h = align(sizeof(mcpt_packed_hit) * ctx->ws_capacity),
+                 cap_s = align(sizeof(ShadowItem) * ctx->ws_capacity), cap_4 = align(sizeof(float4) * ctx->ws_capacity),
+                 cap_r = align(sizeof(uint64_t) * ctx->ws_capacity);
+    ws.q[0] = (RayItem*)b;
+    b += cap_q;
+    ws.q[1] = (RayItem*)b;
+    b += cap_q;
+    ws.hits = (mcpt_packed_hit*)b;
+    b += cap_h;
+    ws.sq = (ShadowItem*)b;
+    b += cap_s;
+    ws.thr = (float4*)b;
+    b += cap_4;
+    ws.smp = (float4*)b;
+    b += cap_4;
+    ws.rng = (uint64_t*)b;
+    b += cap_r;
+    ws.counters = (int*)b;
+    return MCPT_OK;
+}
+
+struct Timer {
+    bool on;
+    hipStream_t st;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind;
+    explicit Timer(bool enable, hipStream_t s) : on(enable), st(s) {}
+    int mark(int k) {  // records an event that closes a span of kind k
+        if (!on) return MCPT_OK;
+        hipEvent_t e;
+        MCPT_HIP(hipEventCreate(&e));
+        MCPT_HIP(hipEventRecord(e, st));
+        ev.push_back(e);
+        kind.push_back(k);
+        return MCPT_OK;
+    }
+    ~Timer() {
+        for (auto e : ev) hipEventDestroy(e);
+    }
+};
+
+}  // namespace
+
+// ------------------------------------------------------------ host driver
+int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, mcpt_film_s* film,
+           mcpt_render_stats* stats) {
+    Ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    if (!s->built) return set_error(MCPT_ERR_INVALID, "scene has no BVH (call mcpt_lbvh_build)");
+    if (film->w != cam->width || film->h != cam->height)
+        return set_error(MCPT_ERR_INVALID, "film %dx%d does not match camera %dx%d", film->w, film->h, cam->width,
+                         cam->height);
+    if (o->spp < 1 || o->max_depth < 1 || o->sample_begin < 0)
+        return set_error(MCPT_ERR_INVALID, "spp, max_depth must be >= 1 and sample_begin >= 0");
+    const int W = cam->width, H = cam->height, npix = W * H;
+    const SceneView sv = s->view();
+    const vec3 bg = load3(cam->background);
+    const size_t lds = stack_bytes(s);
+    const bool count = (o->flags & MCPT_RENDER_COUNT_TRAVERSAL) != 0;
+    const bool timed = (o->flags & MCPT_RENDER_TIME_KERNELS) != 0;
+    int refill = o->refill_lanes > 0 ? std::min(64, o->refill_lanes) : 16;
+
+    Workspace ws;
+    MCPT_TRY(ensure_workspace(ctx, o->mode == MCPT_MODE_MEGAKERNEL ? 1 : npix, ws));
+    enum { K_CAMERA, K_EXTEND, K_SHADE, K_SHADOW, K_FINALIZE, K_OTHER };
+    Timer tm(timed, st);
+    hipEvent_t e0, e1;
+    MCPT_HIP(hipEventCreate(&e0));
+    MCPT_HIP(hipEventCreate(&e1));
+    MCPT_HIP(hipEventRecord(e0, st));
+    k_counters_zero_totals<<<1, 64, 0, st>>>(ws.counters);
+    MCPT_TRY(tm.mark(K_OTHER));
+
+    int persistent_blocks = 0;
+    if (o->mode == MCPT_MODE_PERSISTENT) {
+        int per_cu = 0;
+        MCPT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_extend_persistent<false>, kBlock, lds));
+        persistent_blocks = std::max(1, per_cu) * ctx->cu_count;
+    }
+    uint64_t extend_launches = 0;
+    for (int k = 0; k < o->spp; k++) {
+        const int sample = o->sample_begin + k;
+        if (o->mode == MCPT_MODE_MEGAKERNEL) {
+            dim3 grid((W + 15) / 16, (H + 15) / 16);
+            k_megakernel<<<grid, kBlock, lds, st>>>(sv, *cam, sample, o->seed, o->max_depth, film->accum.ptr,
+                                                   ws.counters);
+            MCPT_TRY(tm.mark(K_EXTEND));
+            continue;
+        }
+        int cur = 0;
+        k_counters_init<<<1, 64, 0, st>>>(ws.counters, cur, npix);
+        k_camera<<<blocks_for(npix), kBlock, 0, st>>>(*cam, sample, o->seed, ws, cur);
+        MCPT_TRY(tm.mark(K_CAMERA));
+        for (int depth = 0; depth < o->max_depth; depth++) {
+            k_counters_bounce<<<1, 64, 0, st>>>(ws.counters, cur);
+            MCPT_TRY(tm.mark(K_OTHER));
+            if (o->mode == MCPT_MODE_PERSISTENT) {
+                const int blocks = std::min(blocks_for(npix), persistent_blocks);
+                if (count) k_extend_persistent<true><<<blocks, kBlock, lds, st>>>(sv, ws, cur, refill);
+                else k_extend_persistent<false><<<blocks, kBlock, lds, st>>>(sv, ws, cur, refill);
+            } else {
+                if (count) k_extend<true><<<blocks_for(npix), kBlock, lds, st>>>(sv, ws, cur);
+                else k_extend<false><<<blocks_for(npix), kBlock, lds, st>>>(sv, ws, cur);
+            }
+            extend_launches++;
+            MCPT_TRY(tm.mark(K_EXTEND));
+            k_shade<<<blocks_for(npix), kBlock, 0, st>>>(sv, ws, cur, depth, bg, cam->rr_cap, o->seed);
+            MCPT_TRY(tm.mark(K_SHADE));
+            k_counters_shadow_total<<<1, 64, 0, st>>>(ws.counters);
+            k_shadow<<<blocks_for(npix), kBlock, lds, st>>>(sv, ws);
+            MCPT_TRY(tm.mark(K_SHADOW));
+            cur = 1 - cur;
+        }
+        k_finalize<<<blocks_for(npix), kBlock, 0, st>>>(ws, film->accum.ptr, npix);
+        MCPT_TRY(tm.mark(K_FINALIZE));
+    }
+    MCPT_HIP(hipGetLastError());
+    MCPT_HIP(hipEventRecord(e1, st));
+    MCPT_HIP(hipEventSynchronize(e1));
+    float total_ms = 0.f;
+    MCPT_HIP(hipEventElapsedTime(&total_ms, e0, e1));
+    uint64_t totals[4] = {0, 0, 0, 0};  // extend rays, shadow rays, steps, leaves
+    MCPT_HIP(hipMemcpy(totals, ws.counters + C_EXT_TOTAL, sizeof(totals), hipMemcpyDeviceToHost));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->total_ms = total_ms;
+        stats->extend_launches = o->mode == MCPT_MODE_MEGAKERNEL ? (uint64_t)o->spp : extend_launches;
+        stats->extend_rays = totals[0];
+        stats->shadow_rays = totals[1];
+        stats->extend_steps = totals[2];
+        stats->extend_leaves = totals[3];
+        stats->paths = (uint64_t)npix * o->spp;
+        if (timed) {
+            double* acc[6] = {&stats->camera_ms, &stats->extend_ms, &stats->shade_ms, &stats->shadow_ms,
+                              &stats->finalize_ms, nullptr};
+            hipEvent_t prev = e0;
+            for (size_t i = 0; i < tm.ev.size(); i++) {
+                float ms = 0.f;
+                MCPT_HIP(hipEventElapsedTime(&ms, prev, tm.ev[i]));
+                if (acc[tm.kind[i]]) *acc[tm.kind[i]] += ms;
+                prev = tm.ev[i];
+            }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return MCPT_OK;
+}
+
+int resolve(mcpt_film_s* film, int spp, int tonemap, float exposure, float* out) {
+    Ctx* ctx = film->ctx;
+    const int n = film->w * film->h;
+    DevBuf<float> o;
+    MCPT_TRY(o.alloc(ctx, (size_t)n * 3));
+    k_resolve<<<blocks_for(3 * n), kBlock, 0, ctx->stream>>>(film->accum.ptr, o.ptr, n, spp, tonemap, exposure);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, o.ptr, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    o.release(ctx);
+    if (e != hipSuccess) return set_error(MCPT_ERR_HIP, "resolve: %s", hipGetErrorString(e));
+    return MCPT_OK;
+}
+
+int trace_closest_host(mcpt_scene_s* s, const mcpt_ray* rays, int n, mcpt_packed_hit* hits, uint32_t* steps,
+                       uint32_t* leaves) {
+    if (!s->built) return set_error(MCPT_ERR_INVALID, "scene has no BVH");
+    if (n < 0 || (n > 0 && (!rays || !hits))) return set_error(MCPT_ERR_INVALID, "bad trace arguments");
+    if (n == 0) return MCPT_OK;
+    Ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    const bool count = steps && leaves;
+    DevBuf<mcpt_ray> dr;
+    DevBuf<mcpt_packed_hit> dh;
+    DevBuf<uint32_t> dcount;
+    int status = dr.alloc(ctx, n);
+    if (status >= 0) status = dh.alloc(ctx, n);
+    if (status >= 0 && count) status = dcount.alloc(ctx, (size_t)2 * n);
+    hipError_t e = hipSuccess;
+    if (status >= 0) {
+        e = hipMemcpyAsync(dr.ptr, rays, sizeof(mcpt_ray) * n, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) {
+            if (count)
+                k_trace_closest<true><<<blocks_for(n), kBlock, stack_bytes(s), st>>>(s->view(), dr.ptr, n, dh.ptr,
+                                                                                    dcount.ptr, dcount.ptr + n);
+            else
+                k_trace_closest<false><<<blocks_for(n), kBlock, stack_bytes(s), st>>>(s->view(), dr.ptr, n, dh.ptr,
+                                                                                     nullptr, nullptr);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(hits, dh.ptr, sizeof(mcpt_packed_hit) * n, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && count) e = hipMemcpyAsync(steps, dcount.ptr, 4 * (size_t)n, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && count)
+            e = hipMemcpyAsync(leaves, dcount.ptr + n, 4 * (size_t)n, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    dr.release(ctx);
+    dh.release(ctx);
+    dcount.release(ctx);
+    if (status < 0) return status;
+    if (e != hipSuccess) return set_error(MCPT_ERR_HIP, "trace_closest: %s", hipGetErrorString(e));
+    return MCPT_OK;
+}
+
+int trace_any_host(mcpt_scene_s* s, const mcpt_ray* rays, int n, uint8_t* occluded) {
+    if (!s->built) return set_error(MCPT_ERR_INVALID, "scene has no BVH");
+    if (n < 0 || (n > 0 && (!rays || !occluded))) return set_error(MCPT_ERR_INVALID, "bad trace arguments");
+    if (n == 0) return MCPT_OK;
+    Ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    DevBuf<mcpt_ray> dr;
+    DevBuf<uint8_t> dout;
+    int status = dr.alloc(ctx, n);
+    if (status >= 0) status = dout.alloc(ctx, n);
+    hipError_t e = hipSuccess;
+    if (status >= 0) {
+        e = hipMemcpyAsync(dr.ptr, rays, sizeof(mcpt_ray) * n, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) {
+            k_trace_any<<<blocks_for(n), kBlock, stack_bytes(s), st>>>(s->view(), dr.ptr, n, dout.ptr);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(occluded, dout.ptr, n, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    dr.release(ctx);
+    dout.release(ctx);
+    if (status < 0) return status;
+    if (e != hipSuccess) return set_error(MCPT_ERR_HIP, "trace_any: %s", hipGetErrorString(e));
+    return MCPT_OK;
+}
+
+}  // namespace mcpt

@@ -1,0 +1,99 @@
+// scene.hpp — device-resident scene, film and the launchers each kernel
+// file provides.
+#pragma once
+
+#include "device.hpp"
+#include "host_common.hpp"
+
+struct mcpt_ctx_s {
+    mcpt::Ctx c;
+};
+
+struct mcpt_scene_s {
+    mcpt::Ctx* ctx = nullptr;
+    int n = 0;
+    int n_mats = 0;
+    bool has_medium = false;
+    // inputs (original triangle order)
+    mcpt::DevBuf<float> hot;  // 9 floats per triangle
+    mcpt::DevBuf<mcpt_tri_cold> cold;
+    mcpt::DevBuf<mcpt_material> mats;
+    mcpt::DevBuf<mcpt_light> lights;
+    std::vector<mcpt_light> host_lights;
+    // environment
+    bool env_enabled = false;
+    int env_w = 0, env_h = 0;
+    float env_intensity = 1.f, env_rot = 0.f, env_total = 0.f;
+    mcpt::DevBuf<float4> env_texels;
+    mcpt::DevBuf<float> env_row_cdf, env_col_cdf;
+    std::vector<float> host_row_cdf, host_col_cdf;
+    // BVH
+    bool built = false;
+    int max_depth = 0;
+    mcpt::DevBuf<mcpt::BvhNode> nodes;
+    mcpt::DevBuf<float> leaf_hot;
+    mcpt::DevBuf<int> leaf_prim;
+    mcpt::DevBuf<float> root_box;  // 6 floats: min xyz, max xyz
+    float last_build_ms = 0.f;
+
+    mcpt::SceneView view() const {
+        mcpt::SceneView v;
+        v.nodes = nodes.ptr;
+        v.leaf_hot = leaf_hot.ptr;
+        v.leaf_prim = leaf_prim.ptr;
+        v.hot = hot.ptr;
+        v.cold = cold.ptr;
+        v.mats = mats.ptr;
+        v.lights = lights.ptr;
+        v.n_lights = (int)host_lights.size();
+        v.n_tris = n;
+        v.has_medium = has_medium ? 1 : 0;
+        v.stack_depth = max_depth;
+        v.env.texels = env_texels.ptr;
+        v.env.row_cdf = env_row_cdf.ptr;
+        v.env.col_cdf = env_col_cdf.ptr;
+        v.env.w = env_w;
+        v.env.h = env_h;
+        v.env.intensity = env_intensity;
+        v.env.rot = env_rot;
+        v.env.total = env_total;
+        v.env.enabled = env_enabled ? 1 : 0;
+        return v;
+    }
+    void release() {
+        hot.release(ctx);
+        cold.release(ctx);
+        mats.release(ctx);
+        lights.release(ctx);
+        env_texels.release(ctx);
+        env_row_cdf.release(ctx);
+        env_col_cdf.release(ctx);
+        nodes.release(ctx);
+        leaf_hot.release(ctx);
+        leaf_prim.release(ctx);
+        root_box.release(ctx);
+    }
+};
+
+struct mcpt_film_s {
+    mcpt::Ctx* ctx = nullptr;
+    int w = 0, h = 0;
+    mcpt::DevBuf<float> accum;  // W*H*3
+};
+
+namespace mcpt {
+
+// lbvh.hip
+int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump);
+
+// trace.hip
+int trace_closest_host(mcpt_scene_s* s, const mcpt_ray* rays, int n, mcpt_packed_hit* hits, uint32_t* steps,
+                       uint32_t* leaves);
+int trace_any_host(mcpt_scene_s* s, const mcpt_ray* rays, int n, uint8_t* occluded);
+
+// render.hip
+int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, mcpt_film_s* film,
+           mcpt_render_stats* stats);
+int resolve(mcpt_film_s* film, int spp, int tonemap, float exposure, float* out);
+
+}  // namespace mcpt

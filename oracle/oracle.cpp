@@ -1,0 +1,1134 @@
+// oracle.cpp — CPU checker for the path-tracing core. TEST INFRASTRUCTURE:
+// only the tests, the smoke check and the benchmark's CPU-baseline leg use
+// it. It restates MonoCUDA-PT's device functions (src/path_tracer.cu, cited
+// per function) with the same operation order, in host C++17 built with
+// -ffp-contract=off. cuRAND is replaced by PCG32 keyed by (pixel, sample)
+// and CUDA's hardware bilinear texture fetch by a software bilinear filter.
+#include "oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "mcpt_fmath.h"
+#include "mcpt_pcg32.h"
+
+namespace {
+
+// ---------------------------------------------------------------- vec3
+// src/path_tracer.cu:105-163
+struct V3 {
+    float x, y, z;
+    V3() : x(0), y(0), z(0) {}
+    V3(float a, float b, float c) : x(a), y(b), z(c) {}
+    V3 operator+(const V3& o) const { return V3(x + o.x, y + o.y, z + o.z); }
+    V3 operator-(const V3& o) const { return V3(x - o.x, y - o.y, z - o.z); }
+    V3 operator*(const V3& o) const { return V3(x * o.x, y * o.y, z * o.z); }
+    V3 operator-() const { return V3(-x, -y, -z); }
+    V3 operator*(float n) const { return V3(x * n, y * n, z * n); }
+    V3 operator/(float n) const { return V3(x / n, y / n, z / n); }
+    float length_squared() const { return x * x + y * y + z * z; }
+    float length() const { return sqrtf(length_squared()); }
+    V3 cross(const V3& o) const { return V3(y * o.z - z * o.y, z * o.x - x * o.z, x * o.y - y * o.x); }
+    float dot(const V3& o) const { return x * o.x + y * o.y + z * o.z; }
+    V3 normalize() const {
+        float l = sqrtf(length_squared());
+        return V3(x / l, y / l, z / l);
+    }
+    V3 reflect(const V3& n) const { return *this - n * (2.0f * dot(n)); }
+    // src/path_tracer.cu:146-163
+    V3 refract(const V3& n, float eta) const {
+        V3 d = normalize();
+        float cos1 = fminf((-d).dot(n), 1.0f);
+        float theta1 = mf_acos(cos1);
+        float sin2 = mf_sin(theta1) * eta;
+        if (sin2 > 1.f) return d.reflect(n);
+        float theta2 = mf_asin(sin2);
+        V3 tangent = d + n * cos1;
+        float tl = tangent.length();
+        if (tl > 1e-8f) tangent = tangent / tl;
+        return tangent * mf_sin(theta2) - n * mf_cos(theta2);
+    }
+};
+V3 v3(const float* p) { return V3(p[0], p[1], p[2]); }
+
+struct AABB {
+    V3 mn, mx;
+};
+
+// merge (src/path_tracer.cu:1456-1465)
+AABB merge(const AABB& a, const AABB& b) {
+    AABB m;
+    m.mx.x = fmaxf(a.mx.x, b.mx.x);
+    m.mx.y = fmaxf(a.mx.y, b.mx.y);
+    m.mx.z = fmaxf(a.mx.z, b.mx.z);
+    m.mn.x = fminf(a.mn.x, b.mn.x);
+    m.mn.y = fminf(a.mn.y, b.mn.y);
+    m.mn.z = fminf(a.mn.z, b.mn.z);
+    return m;
+}
+
+// triangle::calculate_bounding_box (src/path_tracer.cu:651-655)
+AABB tri_box(const mcpt_tri_hot& t) {
+    const float eps = 1e-3f;
+    AABB b;
+    b.mn = V3(fminf(t.v0[0], fminf(t.v1[0], t.v2[0])) - eps, fminf(t.v0[1], fminf(t.v1[1], t.v2[1])) - eps,
+              fminf(t.v0[2], fminf(t.v1[2], t.v2[2])) - eps);
+    b.mx = V3(fmaxf(t.v0[0], fmaxf(t.v1[0], t.v2[0])) + eps, fmaxf(t.v0[1], fmaxf(t.v1[1], t.v2[1])) + eps,
+              fmaxf(t.v0[2], fmaxf(t.v1[2], t.v2[2])) + eps);
+    return b;
+}
+
+// hit_aabb (src/path_tracer.cu:1467-1488)
+bool hit_aabb(const AABB& box, const V3& o, const V3& d, float& tnear, float& tfar) {
+    const float os[3] = {o.x, o.y, o.z}, ds[3] = {d.x, d.y, d.z};
+    const float mn[3] = {box.mn.x, box.mn.y, box.mn.z}, mx[3] = {box.mx.x, box.mx.y, box.mx.z};
+    for (int a = 0; a < 3; a++) {
+        float inv = 1.0f / ds[a];
+        float t0 = (mn[a] - os[a]) * inv;
+        float t1 = (mx[a] - os[a]) * inv;
+        if (inv < 0.0f) std::swap(t0, t1);
+        tnear = fmaxf(tnear, t0);
+        tfar = fminf(tfar, t1);
+        if (tfar <= tnear) return false;
+    }
+    return true;
+}
+
+// hit_triangle (src/path_tracer.cu:715-741)
+bool hit_triangle(const mcpt_tri_hot& tri, const V3& o, const V3& d, float& tout) {
+    V3 v0 = v3(tri.v0), v1 = v3(tri.v1), v2 = v3(tri.v2);
+    V3 e1 = v1 - v0;
+    V3 e2 = v2 - v0;
+    V3 n = e1.cross(e2).normalize();
+    float den = d.dot(n);
+    if (fabsf(den) < 1e-6f) return false;
+    float t = (v0 - o).dot(n) / den;
+    if (t < 0.0001f) return false;
+    V3 p = o + d * t;
+    if (n.dot((v1 - v0).cross(p - v0)) < 0) return false;
+    if (n.dot((v2 - v1).cross(p - v1)) < 0) return false;
+    if (n.dot((v0 - v2).cross(p - v2)) < 0) return false;
+    tout = t;
+    return true;
+}
+
+// Morton (src/path_tracer.cu:913-935)
+uint32_t expand_bits(uint32_t p) {
+    uint32_t e = 0;
+    for (int i = 0; i < 10; i++) e |= ((p >> i) & 1u) << (3 * i);
+    return e;
+}
+uint32_t quantize(float v, float lo, float hi) {
+    float nrm = (v - lo) / (hi - lo);
+    return (uint32_t)(nrm * 1023.0f);
+}
+
+// delta (src/path_tracer.cu:1337-1347)
+int delta(const uint32_t* k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    if (k[i] == k[j]) return 32 + __builtin_clz((uint32_t)(i ^ j));
+    return __builtin_clz(k[i] ^ k[j]);
+}
+
+struct Node {  // internal node of the Karras tree (BVH_Node, :1006-1015)
+    int left, right;
+    bool left_leaf, right_leaf;
+    int parent;
+    AABB box;
+};
+
+struct Env {
+    bool enabled = false;
+    int w = 0, h = 0;
+    float intensity = 1.f, rot = 0.f, total = 0.f;
+    std::vector<float> rgba, row_cdf, col_cdf;
+};
+
+}  // namespace
+
+struct oracle_scene_s {
+    std::vector<mcpt_tri_hot> hot;
+    std::vector<mcpt_tri_cold> cold;
+    std::vector<mcpt_material> mats;
+    std::vector<mcpt_light> lights;
+    Env env;
+    // BVH
+    int n = 0;
+    std::vector<uint32_t> morton, keys;
+    std::vector<int32_t> ids;
+    std::vector<AABB> tri_boxes;  // original order
+    AABB bounds;
+    std::vector<Node> nodes;       // n-1 (or 1 for n == 1)
+    std::vector<int> leaf_parent;  // n
+    std::vector<int> leaf_prim;    // n
+    std::vector<AABB> leaf_box;    // n
+    int max_depth = 0;
+};
+
+namespace {
+
+// ---------------------------------------------------------------- traversal
+struct Closest {
+    float t;
+    int prim;
+};
+
+// BVH::traverseRay / stepTraversal / testPrimitiveClosest
+// (src/path_tracer.cu:1092-1150, :1262-1321, :1537-1551). Returns the closest
+// candidate; counts internal-node iterations and leaf tests.
+Closest traverse_closest(const oracle_scene_s& s, const V3& o, const V3& d, float tmax, uint32_t* steps,
+                         uint32_t* leaves) {
+    Closest c{tmax, -1};
+    int stack[64];
+    int sp = 0;
+    int node = 0;
+    uint32_t ns = 0, nl = 0;
+    while (true) {
+        const Node& nd = s.nodes[node];
+        ns++;
+        const AABB& bl = nd.left_leaf ? s.leaf_box[nd.left] : s.nodes[nd.left].box;
+        const AABB& br = nd.right_leaf ? s.leaf_box[nd.right] : s.nodes[nd.right].box;
+        float nL = 0.0001f, fL = c.t;
+        bool oL = hit_aabb(bl, o, d, nL, fL);
+        float nR = 0.0001f, fR = c.t;
+        bool oR = hit_aabb(br, o, d, nR, fR);
+        if (oL && nd.left_leaf) {
+            nl++;
+            int p = s.leaf_prim[nd.left];
+            float t;
+            if (hit_triangle(s.hot[p], o, d, t) && t < c.t && t > 0.0001f) c = Closest{t, p};
+        }
+        if (oR && nd.right_leaf) {
+            nl++;
+            int p = s.leaf_prim[nd.right];
+            float t;
+            if (hit_triangle(s.hot[p], o, d, t) && t < c.t && t > 0.0001f) c = Closest{t, p};
+        }
+        bool tL = oL && !nd.left_leaf;
+        bool tR = oR && !nd.right_leaf;
+        if (!tL && !tR) {
+            if (sp == 0) break;
+            node = stack[--sp];
+        } else {
+            if (tL && tR) stack[sp++] = nd.right;
+            node = tL ? nd.left : nd.right;
+        }
+    }
+    if (steps) *steps = ns;
+    if (leaves) *leaves = nl;
+    return c;
+}
+
+// BVH::traverseAnyHit + doesPrimitiveBlockRay (src/path_tracer.cu:1152-1207, :1553-1563)
+bool traverse_any(const oracle_scene_s& s, const V3& o, const V3& d, float tmax) {
+    int stack[64];
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        const Node& nd = s.nodes[node];
+        const AABB& bl = nd.left_leaf ? s.leaf_box[nd.left] : s.nodes[nd.left].box;
+        const AABB& br = nd.right_leaf ? s.leaf_box[nd.right] : s.nodes[nd.right].box;
+        float nL = 0.0001f, fL = tmax;
+        bool oL = hit_aabb(bl, o, d, nL, fL);
+        float nR = 0.0001f, fR = tmax;
+        bool oR = hit_aabb(br, o, d, nR, fR);
+        float t;
+        if (oL && nd.left_leaf && hit_triangle(s.hot[s.leaf_prim[nd.left]], o, d, t) && t > 0.0001f && t < tmax)
+            return true;
+        if (oR && nd.right_leaf && hit_triangle(s.hot[s.leaf_prim[nd.right]], o, d, t) && t > 0.0001f && t < tmax)
+            return true;
+        bool tL = oL && !nd.left_leaf;
+        bool tR = oR && !nd.right_leaf;
+        if (!tL && !tR) {
+            if (sp == 0) break;
+            node = stack[--sp];
+        } else {
+            if (tL && tR) stack[sp++] = nd.right;
+            node = tL ? nd.left : nd.right;
+        }
+    }
+    return false;
+}
+
+// --------------------------------------------------------- hit reconstruction
+struct Hit {
+    bool did = false;
+    float t = 1e30f;
+    V3 p, normal;
+    bool front = true;
+    int prim = -1;
+    float u = 0, v = 0;
+    const mcpt_material* mat = nullptr;
+};
+
+// computeTriangleUvFromHitPoint (src/path_tracer.cu:687-696)
+void tri_uv(const mcpt_tri_hot& h, const mcpt_tri_cold& c, const V3& p, float& u, float& v) {
+    V3 v0 = v3(h.v0), v1 = v3(h.v1), v2 = v3(h.v2);
+    float a2 = (v1 - v0).cross(v2 - v0).length();
+    float w0 = (v1 - p).cross(v2 - p).length() / a2;
+    float w1 = (v0 - p).cross(v2 - p).length() / a2;
+    float w2 = (v1 - p).cross(v0 - p).length() / a2;
+    u = ((c.uv0[0] * w0) + (c.uv1[0] * w1)) + (c.uv2[0] * w2);
+    v = ((c.uv0[1] * w0) + (c.uv1[1] * w1)) + (c.uv2[1] * w2);
+}
+
+// computeTriangleShadingNormal (src/path_tracer.cu:698-713)
+V3 tri_shading_normal(const mcpt_tri_hot& h, const mcpt_tri_cold& c, const V3& p, const V3& face) {
+    V3 n0 = v3(c.n0), n1 = v3(c.n1), n2 = v3(c.n2);
+    if (n0.length_squared() == 0.0f) return face;
+    V3 v0 = v3(h.v0), v1 = v3(h.v1), v2 = v3(h.v2);
+    float a2 = (v1 - v0).cross(v2 - v0).length();
+    float w0 = (v1 - p).cross(v2 - p).length() / a2;
+    float w1 = (v0 - p).cross(v2 - p).length() / a2;
+    float w2 = (v1 - p).cross(v0 - p).length() / a2;
+    float ws = w0 + w1 + w2;
+    if (!(ws > 0.0f) || ws > 1.01f) return face;
+    V3 sn = n0 * w0 + n1 * w1 + n2 * w2;
+    if (sn.length_squared() < 1e-12f) return face;
+    sn = sn.normalize();
+    if (sn.dot(face) < 0.0f) sn = -sn;
+    return sn;
+}
+
+// the hitRecord fields testPrimitiveHit fills (src/path_tracer.cu:1511-1533)
+Hit make_hit(const oracle_scene_s& s, const V3& o, const V3& d, const Closest& c) {
+    Hit h;
+    if (c.prim < 0) {
+        h.t = c.t;
+        return h;
+    }
+    const mcpt_tri_hot& th = s.hot[c.prim];
+    const mcpt_tri_cold& tc = s.cold[c.prim];
+    h.did = true;
+    h.t = c.t;
+    h.prim = c.prim;
+    h.p = o + d * c.t;
+    h.mat = &s.mats[tc.material];
+    V3 n = (v3(th.v1) - v3(th.v0)).cross(v3(th.v2) - v3(th.v0)).normalize();
+    h.front = d.dot(n) < 0.0f;
+    V3 sn = tri_shading_normal(th, tc, h.p, n);
+    h.normal = h.front ? sn : -sn;
+    tri_uv(th, tc, h.p, h.u, h.v);
+    return h;
+}
+
+Hit trace_ray(const oracle_scene_s& s, const V3& o, const V3& d, float tmax, uint32_t* steps, uint32_t* leaves) {
+    Closest c = traverse_closest(s, o, d, tmax, steps, leaves);
+    return make_hit(s, o, d, c);
+}
+
+// ---------------------------------------------------------------- sampling
+struct Rng {
+    mcpt_pcg32 g;
+    float u() { return mcpt_pcg32_uniform(&g); }
+};
+
+// random_unit_vector (src/path_tracer.cu:498-509)
+V3 random_unit_vector(Rng& r) {
+    while (true) {
+        float x = r.u() * 2.f - 1.f;
+        float y = r.u() * 2.f - 1.f;
+        float z = r.u() * 2.f - 1.f;
+        V3 v(x, y, z);
+        if (v.length_squared() < 1.0f) return v.normalize();
+    }
+}
+// lambertian_scatter_direction (:511-518)
+V3 lambertian_dir(const V3& n, Rng& r) {
+    V3 d = n + random_unit_vector(r);
+    if (d.length_squared() < 1e-8f) return n;
+    return d.normalize();
+}
+// schlick_approx (:476-480)
+float schlick(float c, float eta) {
+    float r0 = (1.f - eta) / (1.f + eta);
+    r0 = r0 * r0;
+    return r0 + (1.f - r0) * mf_pow((1 - c), 5.f);
+}
+// dielectric_scatter_direction (:524-540)
+V3 dielectric_dir(const V3& in, const V3& n, float eta, Rng& r) {
+    V3 ud = in.normalize();
+    float c = fminf((-ud).dot(n), 1.0f);
+    float sn = sqrtf(1.0f - c * c);
+    bool cannot = (eta * sn) > 1.0f;
+    float pr = cannot ? 1.0f : schlick(c, eta);
+    if (r.u() < pr) return ud.reflect(n);
+    return ud.refract(n, eta);
+}
+// isotropic_scatter_direction (:542-551)
+V3 isotropic_dir(Rng& r) {
+    float u1 = r.u();
+    float u2 = r.u();
+    float z = 1.0f - 2.0f * u1;
+    float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    float phi = 2.0f * MCPT_PI_F * u2;
+    return V3(rr * mf_cos(phi), rr * mf_sin(phi), z);
+}
+// henyeyGreenstein_scatter_direction (:556-578)
+V3 hg_dir(const V3& in, float g, Rng& r) {
+    if (fabsf(g) < 1e-3f) return isotropic_dir(r);
+    float u1 = r.u();
+    float u2 = r.u();
+    float sq = (1.0f - g * g) / (1.0f + g - 2.0f * g * u1);
+    float ct = (1.0f + g * g - sq * sq) / (2.0f * g);
+    ct = fminf(fmaxf(ct, -1.0f), 1.0f);
+    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    float phi = 2.0f * MCPT_PI_F * u2;
+    V3 fw = in.normalize();
+    V3 helper = (fabsf(fw.x) > 0.9f) ? V3(0.f, 1.f, 0.f) : V3(1.f, 0.f, 0.f);
+    V3 tg = helper.cross(fw).normalize();
+    V3 bt = fw.cross(tg);
+    return tg * (st * mf_cos(phi)) + bt * (st * mf_sin(phi)) + fw * ct;
+}
+// sample_triangle (:744-755)
+V3 sample_triangle(const mcpt_tri_hot& t, Rng& r) {
+    float r1 = r.u();
+    float r2 = r.u();
+    float s1 = sqrtf(r1);
+    float w0 = 1.0f - s1;
+    float w1 = r2 * s1;
+    float w2 = 1.0f - w0 - w1;
+    return v3(t.v0) * w0 + v3(t.v1) * w1 + v3(t.v2) * w2;
+}
+float power_heuristic(float a, float b) {  // :757-761
+    float a2 = a * a, b2 = b * b;
+    return a2 / (a2 + b2);
+}
+float lambert_pdf(float c) { return c / MCPT_PI_F; }  // :763-765
+float hg_pdf(float ct, float g) {                     // :775-780
+    float g2 = g * g;
+    float den = fmaxf(1.0f + g2 - 2.0f * g * ct, 1e-6f);
+    return (1.0f - g2) / (4.0f * MCPT_PI_F * den * sqrtf(den));
+}
+float tri_area(const mcpt_tri_hot& t) {  // :837-839
+    return 0.5f * (v3(t.v1) - v3(t.v0)).cross(v3(t.v2) - v3(t.v0)).length();
+}
+
+// ------------------------------------------------------------ environment
+// environmentUv / environmentDirection (src/path_tracer.cu:408-420)
+void env_uv(const Env& e, const V3& ud, float& u, float& v) {
+    float uu = mf_atan2(ud.z, ud.x) / (2.0f * MCPT_PI_F) + 0.5f + e.rot / (2.0f * MCPT_PI_F);
+    v = mf_acos(fminf(fmaxf(ud.y, -1.0f), 1.0f)) / MCPT_PI_F;
+    u = uu - floorf(uu);
+}
+V3 env_direction(const Env& e, float u, float v) {
+    float phi = (u - 0.5f) * 2.0f * MCPT_PI_F - e.rot;
+    float theta = v * MCPT_PI_F;
+    float st = mf_sin(theta);
+    return V3(mf_cos(phi) * st, mf_cos(theta), mf_sin(phi) * st);
+}
+// Software replacement of tex2D with linear filtering, wrap in u, clamp in
+// v (the texture set up at src/path_tracer.cu:387-406): full FP32 weights.
+V3 env_fetch(const Env& e, float u, float v) {
+    float x = u * (float)e.w - 0.5f;
+    float y = v * (float)e.h - 0.5f;
+    float x0 = floorf(x), y0 = floorf(y);
+    float fx = x - x0, fy = y - y0;
+    int ix0 = (int)x0, iy0 = (int)y0;
+    int ix1 = ix0 + 1, iy1 = iy0 + 1;
+    ix0 = ((ix0 % e.w) + e.w) % e.w;
+    ix1 = ((ix1 % e.w) + e.w) % e.w;
+    iy0 = std::min(std::max(iy0, 0), e.h - 1);
+    iy1 = std::min(std::max(iy1, 0), e.h - 1);
+    const float* t00 = &e.rgba[((size_t)iy0 * e.w + ix0) * 4];
+    const float* t10 = &e.rgba[((size_t)iy0 * e.w + ix1) * 4];
+    const float* t01 = &e.rgba[((size_t)iy1 * e.w + ix0) * 4];
+    const float* t11 = &e.rgba[((size_t)iy1 * e.w + ix1) * 4];
+    float gx = 1.0f - fx, gy = 1.0f - fy;
+    float c[3];
+    for (int k = 0; k < 3; k++) c[k] = (t00[k] * gx + t10[k] * fx) * gy + (t01[k] * gx + t11[k] * fx) * fy;
+    return V3(c[0], c[1], c[2]);
+}
+// environmentRadiance (:422-427)
+V3 env_radiance(const Env& e, const V3& d, const V3& fallback) {
+    if (!e.enabled) return fallback;
+    float u, v;
+    env_uv(e, d.normalize(), u, v);
+    return env_fetch(e, u, v) * e.intensity;
+}
+float env_select_prob(const Env& e, int nl) {  // :429-432
+    if (!e.enabled || e.total <= 0.0f) return 0.0f;
+    return (nl > 0) ? 0.5f : 1.0f;
+}
+int find_cdf_interval(const float* cdf, int count, float value) {  // :434-443
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cdf[mid + 1] <= value) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+float env_texel_pdf(const Env& e, int row, int col, float st) {  // :445-450
+    if (st < 1e-6f) return 0.0f;
+    const float* cdf = e.col_cdf.data() + (size_t)row * (e.w + 1);
+    float tw = cdf[col + 1] - cdf[col];
+    return tw * e.w * e.h / (e.total * 2.0f * MCPT_PI_F * MCPT_PI_F * st);
+}
+float env_dir_pdf(const Env& e, const V3& d) {  // :452-458
+    V3 ud = d.normalize();
+    float u, v;
+    env_uv(e, ud, u, v);
+    int col = std::min((int)(u * e.w), e.w - 1);
+    int row = std::min((int)(v * e.h), e.h - 1);
+    return env_texel_pdf(e, row, col, sqrtf(fmaxf(1.0f - ud.y * ud.y, 0.0f)));
+}
+V3 env_sample(const Env& e, Rng& r, float& pdf) {  // :460-467
+    int row = find_cdf_interval(e.row_cdf.data(), e.h, r.u() * e.total);
+    const float* cdf = e.col_cdf.data() + (size_t)row * (e.w + 1);
+    int col = find_cdf_interval(cdf, e.w, r.u() * cdf[e.w]);
+    float u = (col + r.u()) / e.w;
+    float v = (row + r.u()) / e.h;
+    pdf = env_texel_pdf(e, row, col, mf_sin(v * MCPT_PI_F));
+    return env_direction(e, u, v);
+}
+
+// --------------------------------------------------------------- integrator
+struct PathCounts {
+    uint64_t ext = 0, shd = 0, steps = 0, leaves = 0;
+};
+
+// mediumTransmittance (src/path_tracer.cu:2874-2898), homogeneous media
+float medium_transmittance(const oracle_scene_s& s, V3 o, const V3& d, float maxd, PathCounts& pc) {
+    float tr = 1.0f;
+    float travelled = 0.0f;
+    for (int k = 0; k < 4; k++) {
+        float remaining = maxd - travelled;
+        if (remaining <= 0.0f) break;
+        pc.shd++;
+        Hit b = trace_ray(s, o, d, remaining, nullptr, nullptr);
+        if (!b.did) break;
+        if (b.mat->type != MCPT_MAT_ISOTROPIC) return 0.0f;
+        if (!b.front) tr = tr * mf_exp(-b.mat->density * b.t);
+        travelled = travelled + b.t;
+        o = b.p;
+    }
+    return tr;
+}
+
+// trace_sample_kernel body (src/path_tracer.cu:2900-3059) for one path.
+V3 trace_path(const oracle_scene_s& s, const mcpt_camera& cam, int px, int py, Rng& rng, int max_depth,
+              PathCounts& pc) {
+    // Camera::getRay (src/path_tracer.cu:1633-1658)
+    V3 pc_ = v3(cam.pixel00) + (v3(cam.delta_u) * float(px)) + (v3(cam.delta_v) * float(py));
+    V3 lens;
+    while (true) {
+        float rx = rng.u() * 2.f - 1.f;
+        float ry = rng.u() * 2.f - 1.f;
+        V3 cand(rx, ry, 0.0f);
+        if (cand.length_squared() < 1.0f) {
+            lens = cand;
+            break;
+        }
+    }
+    lens = lens * cam.lens_radius;
+    V3 off = v3(cam.u) * lens.x + v3(cam.v) * lens.y;
+    V3 ro = v3(cam.center) + off;
+    (void)rng.u();  // ray time
+    V3 rd = (pc_ - ro).normalize();
+
+    const int nl = (int)s.lights.size();
+    const V3 bg = v3(cam.background);
+    V3 att(1.0f, 1.0f, 1.0f);
+    V3 col(0.0f, 0.0f, 0.0f);
+    bool prev_spec = true;
+    float prev_pdf = 1.0f;
+    for (int depth = 0; depth < max_depth; depth++) {
+        pc.ext++;
+        uint32_t st = 0, lf = 0;
+        Hit hit = trace_ray(s, ro, rd, 1e30f, &st, &lf);
+        pc.steps += st;
+        pc.leaves += lf;
+        if (!hit.did) {
+            float w = 1.0f;
+            float pe = env_select_prob(s.env, nl);
+            if (!(depth == 0 || prev_spec) && pe > 0.0f) w = power_heuristic(prev_pdf, pe * env_dir_pdf(s.env, rd));
+            col = col + att * env_radiance(s.env, rd, bg) * w;
+            break;
+        }
+        const mcpt_material& m = *hit.mat;
+        float ew = 1.0f;
+        bool from_bsdf = !(depth == 0 || prev_spec);
+        if (from_bsdf && m.type == MCPT_MAT_EMISSIVE) {
+            float cl = fmaxf(fabsf(rd.dot(hit.normal)), 1e-4f);
+            float area = tri_area(s.hot[hit.prim]);
+            float lp = ((1.0f - env_select_prob(s.env, nl)) / float(nl)) * (hit.t * hit.t) / (cl * area);
+            ew = power_heuristic(prev_pdf, lp);
+        }
+        col = col + att * v3(m.emission) * ew;
+        if (m.type == MCPT_MAT_EMISSIVE) break;
+
+        V3 so = hit.p;
+        if (m.type == MCPT_MAT_ISOTROPIC) {
+            float sd;
+            if (hit.front) sd = hit.t;
+            else sd = -mf_log(1.0f - rng.u()) / m.density;
+            if (sd >= hit.t) {
+                ro = hit.p;
+                continue;
+            }
+            so = ro + rd * sd;
+        }
+        att = att * v3(m.albedo);
+
+        // next-event estimation (src/path_tracer.cu:2962-3007)
+        float pe = env_select_prob(s.env, nl);
+        if ((m.type == MCPT_MAT_LAMBERTIAN || m.type == MCPT_MAT_ISOTROPIC) && (nl > 0 || pe > 0.0f)) {
+            V3 dl, lc;
+            float dist, sapdf;
+            if (pe > 0.0f && (nl == 0 || rng.u() < pe)) {
+                float dp;
+                dl = env_sample(s.env, rng, dp);
+                lc = env_radiance(s.env, dl, V3(0.f, 0.f, 0.f));
+                dist = 1e30f;
+                sapdf = pe * dp;
+            } else {
+                int li = std::min((int)(rng.u() * nl), nl - 1);
+                const mcpt_light& L = s.lights[li];
+                float lsp = (1.0f - pe) / float(nl);
+                V3 pl = sample_triangle(L.tri, rng);
+                V3 tl = pl - so;
+                dist = tl.length();
+                dl = tl / dist;
+                lc = v3(L.emission);
+                float cl = fabsf(v3(L.normal).dot(dl));
+                sapdf = (cl > 1e-4f) ? lsp * (dist * dist) / (cl * L.area) : 0.0f;
+            }
+            bool iso = (m.type == MCPT_MAT_ISOTROPIC);
+            float cs = iso ? 1.0f : hit.normal.dot(dl);
+            if (cs > 0.0f && sapdf > 0.0f) {
+                float tr = medium_transmittance(s, so, dl, dist * 0.999f, pc);
+                if (tr > 0.0f) {
+                    float php = hg_pdf(rd.dot(dl), m.phase_g);
+                    float spdf = iso ? php : lambert_pdf(cs);
+                    float sval = iso ? php : cs / MCPT_PI_F;
+                    float nw = power_heuristic(sapdf, spdf);
+                    col = col + att * lc * (sval / sapdf) * nw * tr;
+                }
+            }
+        }
+        // Russian roulette (src/path_tracer.cu:3011-3017)
+        if (depth > 3) {
+            float mc = fmaxf(att.x, fmaxf(att.y, att.z));
+            float cp = fminf(mc, cam.rr_cap);
+            if (rng.u() > cp) break;
+            att = att / cp;
+        }
+        V3 nd;
+        switch (m.type) {
+            case MCPT_MAT_CONDUCTOR: {
+                V3 refl = rd.normalize().reflect(hit.normal);
+                nd = (refl + random_unit_vector(rng) * m.fuzz).normalize();
+                break;
+            }
+            case MCPT_MAT_DIELECTRIC: {
+                float eta = hit.front ? 1.0f / m.ior : m.ior;
+                nd = dielectric_dir(rd, hit.normal, eta, rng);
+                break;
+            }
+            case MCPT_MAT_ISOTROPIC:
+                nd = hg_dir(rd, m.phase_g, rng);
+                break;
+            default:
+                nd = lambertian_dir(hit.normal, rng);
+                break;
+        }
+        prev_spec = (m.type != MCPT_MAT_LAMBERTIAN && m.type != MCPT_MAT_ISOTROPIC);
+        prev_pdf = (m.type == MCPT_MAT_ISOTROPIC) ? hg_pdf(rd.dot(nd), m.phase_g)
+                                                  : lambert_pdf(fmaxf(nd.dot(hit.normal), 0.0f));
+        ro = so;
+        rd = nd;
+    }
+    // firefly clamp (src/path_tracer.cu:3050-3055)
+    float b = fmaxf(col.x, fmaxf(col.y, col.z));
+    if (b > 10.0f) col = col * (10.0f / b);
+    return col;
+}
+
+int hw_threads(int t) {
+    if (t > 0) return t;
+    unsigned h = std::thread::hardware_concurrency();
+    return h ? (int)h : 1;
+}
+
+template <class F>
+void parallel_for(int n, int chunk, int threads, F f) {
+    threads = hw_threads(threads);
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        while (true) {
+            int b = next.fetch_add(chunk);
+            if (b >= n) break;
+            int e = std::min(n, b + chunk);
+            for (int i = b; i < e; i++) f(i);
+        }
+    };
+    if (threads == 1) {
+        work();
+        return;
+    }
+    std::vector<std::thread> ts;
+    for (int i = 0; i < threads; i++) ts.emplace_back(work);
+    for (auto& t : ts) t.join();
+}
+
+}  // namespace
+
+// ============================================================== C interface
+extern "C" {
+
+int oracle_camera_init(const mcpt_camera_desc* d, mcpt_camera* c) {
+    // Camera::initialize (src/path_tracer.cu:1610-1631)
+    float aspect = d->width / float(d->height);
+    int width = d->width;
+    int height = int(width / aspect);
+    if (height != d->height) return -1;
+    V3 from = v3(d->look_from), at = v3(d->look_at), up = v3(d->up);
+    V3 w = (from - at).normalize();
+    V3 u = up.cross(w).normalize();
+    V3 v = w.cross(u);
+    float lens_radius = d->aperture / 2.;
+    V3 center = from;
+    float theta = d->vfov_degrees * (3.14159265358979323846f / 180.0f);
+    float h = tanf(theta / 2.0f);
+    float vh = 2.0f * h * d->focus_distance;
+    float vw = vh * (float(width) / height);
+    V3 vu = u * vw;
+    V3 vv = v * -vh;
+    V3 du = vu / float(width);
+    V3 dv = vv / float(height);
+    V3 ul = center - (w * d->focus_distance) - vu / 2.0f - vv / 2.0f;
+    V3 p00 = ul + (du + dv) * 0.5f;
+    auto put = [](float* o, const V3& x) {
+        o[0] = x.x;
+        o[1] = x.y;
+        o[2] = x.z;
+    };
+    std::memset(c, 0, sizeof(*c));
+    put(c->center, center);
+    put(c->pixel00, p00);
+    put(c->delta_u, du);
+    put(c->delta_v, dv);
+    put(c->u, u);
+    put(c->v, v);
+    put(c->w, w);
+    c->lens_radius = lens_radius;
+    c->width = width;
+    c->height = height;
+    for (int k = 0; k < 3; k++) c->background[k] = d->background[k];
+    c->rr_cap = d->rr_cap;
+    return 0;
+}
+
+void oracle_env_tables(const float* rgba, int w, int h, float* row_cdf, float* col_cdf, float* total_weight) {
+    // buildEnvironmentSamplingTables (src/path_tracer.cu:370-385)
+    for (int i = 0; i <= h; i++) row_cdf[i] = 0.0f;
+    for (size_t i = 0; i < (size_t)h * (w + 1); i++) col_cdf[i] = 0.0f;
+    for (int row = 0; row < h; row++) {
+        float st = mf_sin(MCPT_PI_F * (row + 0.5f) / h);
+        float* cdf = &col_cdf[(size_t)row * (w + 1)];
+        cdf[0] = 0.0f;
+        for (int col = 0; col < w; col++) {
+            const float* t = &rgba[((size_t)row * w + col) * 4];
+            float lum = 0.2126f * t[0] + 0.7152f * t[1] + 0.0722f * t[2];
+            cdf[col + 1] = cdf[col] + fmaxf(lum, 0.0f) * st;
+        }
+        row_cdf[row + 1] = row_cdf[row] + cdf[w];
+    }
+    *total_weight = row_cdf[h];
+}
+
+uint32_t oracle_expand_bits(uint32_t v) { return expand_bits(v); }
+uint32_t oracle_quantize(float v, float lo, float hi) { return quantize(v, lo, hi); }
+
+void oracle_finalize(const mcpt_tri_hot* hot, int n, mcpt_aabb* tri_bounds, mcpt_aabb* scene_bounds, float* centroids,
+                     uint32_t* morton) {
+    // finalizeScene (src/path_tracer.cu:1930-1972), triangles only
+    std::vector<AABB> boxes(n);
+    AABB sb;
+    for (int i = 0; i < n; i++) {
+        boxes[i] = tri_box(hot[i]);
+        sb = (i == 0) ? boxes[i] : merge(sb, boxes[i]);
+    }
+    for (int i = 0; i < n; i++) {
+        V3 c = (v3(hot[i].v0) + v3(hot[i].v1) + v3(hot[i].v2)) / 3.;
+        uint32_t qx = quantize(c.x, sb.mn.x, sb.mx.x);
+        uint32_t qy = quantize(c.y, sb.mn.y, sb.mx.y);
+        uint32_t qz = quantize(c.z, sb.mn.z, sb.mx.z);
+        if (morton) morton[i] = expand_bits(qx) | expand_bits(qy) << 1 | expand_bits(qz) << 2;
+        if (centroids) {
+            centroids[3 * i] = c.x;
+            centroids[3 * i + 1] = c.y;
+            centroids[3 * i + 2] = c.z;
+        }
+        if (tri_bounds) {
+            mcpt_aabb& b = tri_bounds[i];
+            b.min[0] = boxes[i].mn.x;
+            b.min[1] = boxes[i].mn.y;
+            b.min[2] = boxes[i].mn.z;
+            b.max[0] = boxes[i].mx.x;
+            b.max[1] = boxes[i].mx.y;
+            b.max[2] = boxes[i].mx.z;
+            b.pad0 = b.pad1 = 0.f;
+        }
+    }
+    if (scene_bounds && n > 0) {
+        scene_bounds->min[0] = sb.mn.x;
+        scene_bounds->min[1] = sb.mn.y;
+        scene_bounds->min[2] = sb.mn.z;
+        scene_bounds->max[0] = sb.mx.x;
+        scene_bounds->max[1] = sb.mx.y;
+        scene_bounds->max[2] = sb.mx.z;
+        scene_bounds->pad0 = scene_bounds->pad1 = 0.f;
+    }
+}
+
+void oracle_sort_stable(const uint32_t* morton, int n, uint32_t* keys, int32_t* ids) {
+    std::vector<int32_t> idx(n);
+    for (int i = 0; i < n; i++) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return morton[a] < morton[b]; });
+    for (int i = 0; i < n; i++) {
+        ids[i] = idx[i];
+        keys[i] = morton[idx[i]];
+    }
+}
+
+void oracle_sort_reference(const uint32_t* morton, int n, uint32_t* keys, int32_t* ids) {
+    struct Ref {
+        uint32_t code;
+        int32_t idx;
+    };
+    std::vector<Ref> v(n);
+    for (int i = 0; i < n; i++) v[i] = Ref{morton[i], i};
+    std::sort(v.begin(), v.end(), [](const Ref& a, const Ref& b) { return a.code < b.code; });
+    for (int i = 0; i < n; i++) {
+        keys[i] = v[i].code;
+        ids[i] = v[i].idx;
+    }
+}
+
+void oracle_karras(const uint32_t* k, int n, int32_t* left, int32_t* right, uint8_t* left_leaf, uint8_t* right_leaf,
+                   int32_t* parent, int32_t* leaf_parent) {
+    if (n > 1) parent[0] = -1;
+    for (int idx = 0; idx < n - 1; idx++) {
+        // determineRange (src/path_tracer.cu:1349-1369)
+        int d = (delta(k, n, idx, idx + 1) - delta(k, n, idx, idx - 1)) >= 0 ? 1 : -1;
+        int dmin = delta(k, n, idx, idx - d);
+        int lmax = 2;
+        while (delta(k, n, idx, idx + lmax * d) > dmin) lmax *= 2;
+        int l = 0;
+        for (int t = lmax / 2; t >= 1; t /= 2)
+            if (delta(k, n, idx, idx + (l + t) * d) > dmin) l += t;
+        int j = idx + l * d;
+        int first = std::min(idx, j), last = std::max(idx, j);
+        // findSplit (src/path_tracer.cu:1371-1386)
+        int cp = delta(k, n, first, last);
+        int split = first, step = last - first;
+        do {
+            step = (step + 1) >> 1;
+            int ns = split + step;
+            if (ns < last && delta(k, n, first, ns) > cp) split = ns;
+        } while (step > 1);
+        // buildInternalNodesKernel child wiring (src/path_tracer.cu:1425-1446)
+        left[idx] = split;
+        left_leaf[idx] = (split == first);
+        if (split == first) leaf_parent[split] = idx;
+        else parent[split] = idx;
+        right[idx] = split + 1;
+        right_leaf[idx] = (split + 1 == last);
+        if (split + 1 == last) leaf_parent[split + 1] = idx;
+        else parent[split + 1] = idx;
+    }
+}
+
+int oracle_scene_create(const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int n, const mcpt_material* mats,
+                        int n_mats, const mcpt_env_desc* env, oracle_scene* out) {
+    if (n < 1 || !hot || !cold || !mats || n_mats < 1) return -1;
+    for (int i = 0; i < n; i++)
+        if (cold[i].material < 0 || cold[i].material >= n_mats) return -1;
+    oracle_scene_s* s = new oracle_scene_s();
+    s->hot.assign(hot, hot + n);
+    s->cold.assign(cold, cold + n);
+    s->mats.assign(mats, mats + n_mats);
+    s->n = n;
+    // lights (finalizeScene, src/path_tracer.cu:1973-1985)
+    for (int i = 0; i < n; i++) {
+        if (mats[cold[i].material].type != MCPT_MAT_EMISSIVE) continue;
+        V3 e1 = v3(hot[i].v1) - v3(hot[i].v0);
+        V3 e2 = v3(hot[i].v2) - v3(hot[i].v0);
+        V3 av = e1.cross(e2);
+        mcpt_light L;
+        L.tri = hot[i];
+        for (int k = 0; k < 3; k++) L.emission[k] = mats[cold[i].material].emission[k];
+        L.area = 0.5f * av.length();
+        V3 nn = av.normalize();
+        L.normal[0] = nn.x;
+        L.normal[1] = nn.y;
+        L.normal[2] = nn.z;
+        s->lights.push_back(L);
+    }
+    if (env && env->rgba && env->width > 0 && env->height > 0) {
+        Env& e = s->env;
+        e.enabled = true;
+        e.w = env->width;
+        e.h = env->height;
+        e.intensity = env->intensity;
+        e.rot = env->rotation_radians;
+        e.rgba.assign(env->rgba, env->rgba + (size_t)e.w * e.h * 4);
+        e.row_cdf.resize(e.h + 1);
+        e.col_cdf.resize((size_t)e.h * (e.w + 1));
+        oracle_env_tables(e.rgba.data(), e.w, e.h, e.row_cdf.data(), e.col_cdf.data(), &e.total);
+    }
+    // BVH
+    s->morton.resize(n);
+    std::vector<mcpt_aabb> tb(n);
+    mcpt_aabb sb;
+    oracle_finalize(hot, n, tb.data(), &sb, nullptr, s->morton.data());
+    s->tri_boxes.resize(n);
+    for (int i = 0; i < n; i++) {
+        s->tri_boxes[i].mn = v3(tb[i].min);
+        s->tri_boxes[i].mx = v3(tb[i].max);
+    }
+    s->bounds.mn = v3(sb.min);
+    s->bounds.mx = v3(sb.max);
+    s->keys.resize(n);
+    s->ids.resize(n);
+    oracle_sort_stable(s->morton.data(), n, s->keys.data(), s->ids.data());
+    s->leaf_prim.assign(s->ids.begin(), s->ids.end());
+    s->leaf_box.resize(n);
+    for (int i = 0; i < n; i++) s->leaf_box[i] = s->tri_boxes[s->ids[i]];
+    s->leaf_parent.assign(n, -1);
+    if (n == 1) {
+        // A single primitive has no internal node in the reference (its
+        // traversal would read d_internalNodes[0] out of bounds). Here the
+        // root holds the leaf on the left and an empty box on the right.
+        Node r;
+        r.left = 0;
+        r.left_leaf = true;
+        r.right = 0;
+        r.right_leaf = true;
+        r.parent = -1;
+        r.box = s->leaf_box[0];
+        s->nodes.push_back(r);
+        s->leaf_parent[0] = 0;
+        s->max_depth = 1;
+        *out = s;
+        return 0;
+    }
+    std::vector<int32_t> L(n - 1), R(n - 1), P(n - 1);
+    std::vector<uint8_t> LL(n - 1), RL(n - 1);
+    oracle_karras(s->keys.data(), n, L.data(), R.data(), LL.data(), RL.data(), P.data(), s->leaf_parent.data());
+    s->nodes.resize(n - 1);
+    for (int i = 0; i < n - 1; i++) {
+        s->nodes[i].left = L[i];
+        s->nodes[i].right = R[i];
+        s->nodes[i].left_leaf = LL[i];
+        s->nodes[i].right_leaf = RL[i];
+        s->nodes[i].parent = P[i];
+    }
+    // refit (refitBoundsKernel, src/path_tracer.cu:1398-1415): a post-order
+    // walk computes the same fmin/fmax merges the atomic climb does.
+    std::vector<int> order;
+    order.reserve(n - 1);
+    std::vector<std::pair<int, int>> st;  // node, depth
+    st.push_back({0, 1});
+    int maxd = 0;
+    while (!st.empty()) {
+        auto [x, dep] = st.back();
+        st.pop_back();
+        order.push_back(x);
+        const Node& nd = s->nodes[x];
+        if (nd.left_leaf || nd.right_leaf) maxd = std::max(maxd, dep);
+        if (!nd.left_leaf) st.push_back({nd.left, dep + 1});
+        if (!nd.right_leaf) st.push_back({nd.right, dep + 1});
+    }
+    s->max_depth = maxd;
+    for (int i = (int)order.size() - 1; i >= 0; i--) {
+        Node& nd = s->nodes[order[i]];
+        const AABB& a = nd.left_leaf ? s->leaf_box[nd.left] : s->nodes[nd.left].box;
+        const AABB& b = nd.right_leaf ? s->leaf_box[nd.right] : s->nodes[nd.right].box;
+        nd.box = merge(a, b);
+    }
+    *out = s;
+    return 0;
+}
+
+void oracle_scene_destroy(oracle_scene s) { delete s; }
+
+int oracle_scene_lights(oracle_scene s, mcpt_light* out, int cap) {
+    int nl = (int)s->lights.size();
+    for (int i = 0; i < nl && i < cap; i++) out[i] = s->lights[i];
+    return nl;
+}
+
+int oracle_scene_dump(oracle_scene s, mcpt_lbvh_dump* d) {
+    int n = s->n;
+    auto box = [](mcpt_aabb& o, const AABB& b) {
+        o.min[0] = b.mn.x;
+        o.min[1] = b.mn.y;
+        o.min[2] = b.mn.z;
+        o.max[0] = b.mx.x;
+        o.max[1] = b.mx.y;
+        o.max[2] = b.mx.z;
+        o.pad0 = o.pad1 = 0.f;
+    };
+    if (d->morton) std::copy(s->morton.begin(), s->morton.end(), d->morton);
+    if (d->sorted_keys) std::copy(s->keys.begin(), s->keys.end(), d->sorted_keys);
+    if (d->sorted_ids) std::copy(s->ids.begin(), s->ids.end(), d->sorted_ids);
+    int ni = n - 1;
+    for (int i = 0; i < ni; i++) {
+        const Node& nd = s->nodes[i];
+        if (d->left) d->left[i] = nd.left;
+        if (d->right) d->right[i] = nd.right;
+        if (d->left_is_leaf) d->left_is_leaf[i] = nd.left_leaf;
+        if (d->right_is_leaf) d->right_is_leaf[i] = nd.right_leaf;
+        if (d->parent) d->parent[i] = nd.parent;
+        if (d->internal_bounds) box(d->internal_bounds[i], nd.box);
+    }
+    for (int i = 0; i < n; i++) {
+        if (d->leaf_parent) d->leaf_parent[i] = s->leaf_parent[i];
+        if (d->leaf_bounds) box(d->leaf_bounds[i], s->leaf_box[i]);
+    }
+    if (d->scene_bounds) box(*d->scene_bounds, s->bounds);
+    d->max_depth = s->max_depth;
+    d->build_ms = 0.f;
+    return 0;
+}
+
+void oracle_trace_closest(oracle_scene s, const mcpt_ray* rays, int n, mcpt_packed_hit* hits, uint32_t* steps,
+                          uint32_t* leaves, int threads) {
+    parallel_for(n, 256, threads, [&](int i) {
+        const mcpt_ray& r = rays[i];
+        Closest c = traverse_closest(*s, v3(r.origin), v3(r.dir), r.tmax, steps ? &steps[i] : nullptr,
+                                     leaves ? &leaves[i] : nullptr);
+        hits[i].t = c.t;
+        hits[i].prim_index = c.prim;
+        hits[i].prim_type = c.prim >= 0 ? 1 : -1;
+        hits[i].did_hit = c.prim >= 0 ? 1 : 0;
+    });
+}
+
+void oracle_trace_any(oracle_scene s, const mcpt_ray* rays, int n, uint8_t* occluded, int threads) {
+    parallel_for(n, 256, threads, [&](int i) {
+        const mcpt_ray& r = rays[i];
+        occluded[i] = traverse_any(*s, v3(r.origin), v3(r.dir), r.tmax) ? 1 : 0;
+    });
+}
+
+void oracle_brute_closest(const mcpt_tri_hot* hot, int n_tris, const mcpt_ray* rays, int n, mcpt_packed_hit* hits) {
+    parallel_for(n, 64, 0, [&](int i) {
+        const mcpt_ray& r = rays[i];
+        V3 o = v3(r.origin), d = v3(r.dir);
+        Closest c{r.tmax, -1};
+        for (int k = 0; k < n_tris; k++) {
+            float t;
+            if (hit_triangle(hot[k], o, d, t) && t < c.t && t > 0.0001f) c = Closest{t, k};
+        }
+        hits[i].t = c.t;
+        hits[i].prim_index = c.prim;
+        hits[i].prim_type = c.prim >= 0 ? 1 : -1;
+        hits[i].did_hit = c.prim >= 0 ? 1 : 0;
+    });
+}
+
+int oracle_render(oracle_scene s, const mcpt_camera* cam, int spp, int sample_begin, int max_depth, uint64_t seed,
+                  int row_begin, int row_end, float* accum, double* accum64, int threads,
+                  oracle_render_stats* stats) {
+    const int W = cam->width, H = cam->height;
+    if (row_begin < 0) row_begin = 0;
+    if (row_end <= 0 || row_end > H) row_end = H;
+    if (row_end <= row_begin || spp < 1) return -1;
+    const int tiles_x = (W + 15) / 16;
+    const int tiles_y = (row_end - row_begin + 15) / 16;
+    std::atomic<uint64_t> ext{0}, shd{0}, stp{0}, lvs{0};
+    auto t0 = std::chrono::steady_clock::now();
+    parallel_for(tiles_x * tiles_y, 1, threads, [&](int tile) {
+        int tx = tile % tiles_x, ty = tile / tiles_x;
+        PathCounts pc;
+        for (int y = row_begin + ty * 16; y < std::min(row_end, row_begin + ty * 16 + 16); y++) {
+            for (int x = tx * 16; x < std::min(W, tx * 16 + 16); x++) {
+                int p = y * W + x;
+                for (int k = 0; k < spp; k++) {
+                    Rng rng;
+                    mcpt_pcg32_for_path(&rng.g, (uint32_t)p, (uint32_t)(sample_begin + k), seed);
+                    V3 c = trace_path(*s, *cam, x, y, rng, max_depth, pc);
+                    accum[3 * p] = accum[3 * p] + c.x;
+                    accum[3 * p + 1] = accum[3 * p + 1] + c.y;
+                    accum[3 * p + 2] = accum[3 * p + 2] + c.z;
+                    if (accum64) {
+                        accum64[3 * p] += c.x;
+                        accum64[3 * p + 1] += c.y;
+                        accum64[3 * p + 2] += c.z;
+                    }
+                }
+            }
+        }
+        ext += pc.ext;
+        shd += pc.shd;
+        stp += pc.steps;
+        lvs += pc.leaves;
+    });
+    auto t1 = std::chrono::steady_clock::now();
+    if (stats) {
+        stats->extend_rays = ext;
+        stats->shadow_rays = shd;
+        stats->extend_steps = stp;
+        stats->extend_leaves = lvs;
+        stats->paths = (uint64_t)W * (row_end - row_begin) * spp;
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+        stats->threads = hw_threads(threads);
+    }
+    return 0;
+}
+
+// linearToSrgb / toneMapChannel / resolve_kernel (src/path_tracer.cu:3062-3100)
+static float linear_to_srgb(float x) {
+    x = fmaxf(x, 0.0f);
+    if (x <= 0.0031308f) return 12.92f * x;
+    return 1.055f * mf_pow(x, 1.0f / 2.4f) - 0.055f;
+}
+static float tonemap_channel(float x, int tm) {
+    x = fmaxf(x, 0.f);
+    if (tm == MCPT_TONEMAP_REINHARD) return x / (1.f + x);
+    if (tm == MCPT_TONEMAP_ACES) {
+        float m = (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
+        return fminf(fmaxf(m, 0.0f), 1.0f);
+    }
+    return x;
+}
+void oracle_resolve(const float* accum, int w, int h, int spp, int tonemap, float exposure, float* out) {
+    for (int p = 0; p < w * h; p++) {
+        for (int k = 0; k < 3; k++) {
+            float a = accum[3 * p + k] / float(spp) * exposure;
+            out[3 * p + k] = linear_to_srgb(tonemap_channel(a, tonemap));
+        }
+    }
+}
+
+void oracle_fmath(int fn, const float* x, const float* y, float* out, int n) {
+    for (int i = 0; i < n; i++) {
+        switch (fn) {
+            case 0: out[i] = mf_sin(x[i]); break;
+            case 1: out[i] = mf_cos(x[i]); break;
+            case 2: out[i] = mf_asin(x[i]); break;
+            case 3: out[i] = mf_acos(x[i]); break;
+            case 4: out[i] = mf_atan2(x[i], y[i]); break;
+            case 5: out[i] = mf_log(x[i]); break;
+            case 6: out[i] = mf_exp(x[i]); break;
+            default: out[i] = mf_pow(x[i], y[i]); break;
+        }
+    }
+}
+
+void oracle_pcg32(uint32_t pixel, uint32_t sample, uint64_t seed, int n, uint32_t* out) {
+    mcpt_pcg32 g;
+    mcpt_pcg32_for_path(&g, pixel, sample, seed);
+    for (int i = 0; i < n; i++) out[i] = mcpt_pcg32_next(&g);
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+/*
+ * oracle.h — CPU checker for the path-tracing core. TEST INFRASTRUCTURE ONLY:
+ * the test suite, the smoke check and the CPU-baseline leg of the benchmark
+ * call it as the thing results are compared against. The product library
+ * never links it.
+ *
+ * It restates the reference's device functions (MonoCUDA-PT
+ * src/path_tracer.cu, cited per function in oracle.cpp) in host C++17,
+ * built with g++ -O2 -ffp-contract=off and no fast-math. Elementary
+ * functions come from include/mcpt_fmath.h and random numbers from
+ * include/mcpt_pcg32.h; both are pinned independently (libm error bounds,
+ * an independent PCG32 restatement) by the tests.
+ */
+#ifndef MCPT_ORACLE_H
+#define MCPT_ORACLE_H
+
+#include "mcpt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_scene_s* oracle_scene;
+
+/* Camera::initialize (src/path_tracer.cu:1610-1631) */
+int oracle_camera_init(const mcpt_camera_desc* desc, mcpt_camera* out);
+
+/* buildEnvironmentSamplingTables (src/path_tracer.cu:370-385) */
+void oracle_env_tables(const float* rgba, int w, int h, float* row_cdf, float* col_cdf, float* total_weight);
+
+/* finalizeScene (src/path_tracer.cu:1930-1987), triangles only: padded
+ * boxes, scene bounds, centroids and Morton codes. centroids may be NULL. */
+void oracle_finalize(const mcpt_tri_hot* hot, int n, mcpt_aabb* tri_bounds, mcpt_aabb* scene_bounds, float* centroids,
+                     uint32_t* morton);
+/* expandBits / quantize / morton3D (src/path_tracer.cu:913-935) */
+uint32_t oracle_expand_bits(uint32_t v);
+uint32_t oracle_quantize(float v, float lo, float hi);
+
+/* stable (key, index) sort: the tie rule of this build */
+void oracle_sort_stable(const uint32_t* morton, int n, uint32_t* keys, int32_t* ids);
+/* std::sort with the reference comparator (src/path_tracer.cu:4224-4226) */
+void oracle_sort_reference(const uint32_t* morton, int n, uint32_t* keys, int32_t* ids);
+
+/* buildInternalNodesKernel (src/path_tracer.cu:1337-1386, :1417-1447),
+ * sequential. Arrays sized n-1 (internal) and n (leaf_parent). */
+void oracle_karras(const uint32_t* keys, int n, int32_t* left, int32_t* right, uint8_t* left_leaf, uint8_t* right_leaf,
+                   int32_t* parent, int32_t* leaf_parent);
+
+/* Whole scene: BVH (finalize + stable sort + Karras + refit), light list,
+ * environment (env may be NULL). */
+int oracle_scene_create(const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int n, const mcpt_material* mats,
+                        int n_mats, const mcpt_env_desc* env, oracle_scene* out);
+void oracle_scene_destroy(oracle_scene s);
+int oracle_scene_dump(oracle_scene s, mcpt_lbvh_dump* dump);
+int oracle_scene_lights(oracle_scene s, mcpt_light* out, int cap);
+
+/* traverseRay (closest) / traverseAnyHit, with per-ray visit counts. */
+void oracle_trace_closest(oracle_scene s, const mcpt_ray* rays, int n, mcpt_packed_hit* hits, uint32_t* steps,
+                          uint32_t* leaves, int threads);
+void oracle_trace_any(oracle_scene s, const mcpt_ray* rays, int n, uint8_t* occluded, int threads);
+/* every triangle in index order, strict '<' (brute force) */
+void oracle_brute_closest(const mcpt_tri_hot* hot, int n_tris, const mcpt_ray* rays, int n, mcpt_packed_hit* hits);
+
+typedef struct oracle_render_stats {
+    uint64_t extend_rays, shadow_rays, paths;
+    uint64_t extend_steps, extend_leaves;
+    double seconds;
+    int threads;
+} oracle_render_stats;
+
+/* trace_sample_kernel (src/path_tracer.cu:2900-3059) for samples
+ * [sample_begin, sample_begin + spp) of rows [row_begin, row_end).
+ * accum (W*H*3 FP32) is added to in sample order; accum64 (may be NULL)
+ * receives the same sums in FP64. */
+int oracle_render(oracle_scene s, const mcpt_camera* cam, int spp, int sample_begin, int max_depth, uint64_t seed,
+                  int row_begin, int row_end, float* accum, double* accum64, int threads, oracle_render_stats* stats);
+
+/* Elementary functions of mcpt_fmath.h over arrays (for accuracy tests):
+ * fn 0 sin, 1 cos, 2 asin, 3 acos, 4 atan2(x, y), 5 log, 6 exp, 7 pow(x, y) */
+void oracle_fmath(int fn, const float* x, const float* y, float* out, int n);
+/* first n outputs of the PCG32 stream of (pixel, sample, seed) */
+void oracle_pcg32(uint32_t pixel, uint32_t sample, uint64_t seed, int n, uint32_t* out);
+
+/* resolve_kernel (src/path_tracer.cu:3062-3100) */
+void oracle_resolve(const float* accum, int w, int h, int spp, int tonemap, float exposure, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,262 @@
+"""ctypes binding of liboracle.so, the CPU checker (test infrastructure only).
+
+Only the test suite, the smoke check and the benchmark's CPU-baseline leg
+import this module; the product library never does. Record layouts are the
+ones of include/mcpt.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+HOT_DTYPE = np.dtype([("v0", "<f4", 3), ("v1", "<f4", 3), ("v2", "<f4", 3)])
+COLD_DTYPE = np.dtype([("uv0", "<f4", 2), ("uv1", "<f4", 2), ("uv2", "<f4", 2), ("n0", "<f4", 3),
+                       ("n1", "<f4", 3), ("n2", "<f4", 3), ("material", "<i4")])
+MAT_DTYPE = np.dtype([("type", "<i4"), ("albedo", "<f4", 3), ("emission", "<f4", 3), ("fuzz", "<f4"),
+                      ("ior", "<f4"), ("density", "<f4"), ("phase_g", "<f4"), ("pad", "<i4")])
+LIGHT_DTYPE = np.dtype([("tri", HOT_DTYPE), ("emission", "<f4", 3), ("area", "<f4"), ("normal", "<f4", 3)])
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("time", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_type", "<i4"), ("prim_index", "<i4"), ("did_hit", "<i4")])
+AABB_DTYPE = np.dtype([("min", "<f4", 3), ("pad0", "<f4"), ("max", "<f4", 3), ("pad1", "<f4")])
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        VP = C.c_void_p
+        sigs = {
+            "oracle_camera_init": ([VP, VP], C.c_int),
+            "oracle_env_tables": ([VP, C.c_int, C.c_int, VP, VP, C.POINTER(C.c_float)], None),
+            "oracle_finalize": ([VP, C.c_int, VP, VP, VP, VP], None),
+            "oracle_expand_bits": ([C.c_uint32], C.c_uint32),
+            "oracle_quantize": ([C.c_float, C.c_float, C.c_float], C.c_uint32),
+            "oracle_sort_stable": ([VP, C.c_int, VP, VP], None),
+            "oracle_sort_reference": ([VP, C.c_int, VP, VP], None),
+            "oracle_karras": ([VP, C.c_int, VP, VP, VP, VP, VP, VP], None),
+            "oracle_scene_create": ([VP, VP, C.c_int, VP, C.c_int, VP, C.POINTER(VP)], C.c_int),
+            "oracle_scene_destroy": ([VP], None),
+            "oracle_scene_dump": ([VP, VP], C.c_int),
+            "oracle_scene_lights": ([VP, VP, C.c_int], C.c_int),
+            "oracle_trace_closest": ([VP, VP, C.c_int, VP, VP, VP, C.c_int], None),
+            "oracle_trace_any": ([VP, VP, C.c_int, VP, C.c_int], None),
+            "oracle_brute_closest": ([VP, C.c_int, VP, C.c_int, VP], None),
+            "oracle_render": ([VP, VP, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int, VP, VP, C.c_int,
+                               VP], C.c_int),
+            "oracle_resolve": ([VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, VP], None),
+            "oracle_fmath": ([C.c_int, VP, VP, VP, C.c_int], None),
+            "oracle_pcg32": ([C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, VP], None),
+        }
+        for name, (args, res) in sigs.items():
+            f = getattr(_lib, name)
+            f.argtypes = args
+            f.restype = res
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class EnvDesc(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("intensity", C.c_float),
+                ("rotation_radians", C.c_float), ("rgba", C.c_void_p)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("center", C.c_float * 3), ("pixel00", C.c_float * 3), ("delta_u", C.c_float * 3),
+                ("delta_v", C.c_float * 3), ("u", C.c_float * 3), ("v", C.c_float * 3), ("w", C.c_float * 3),
+                ("lens_radius", C.c_float), ("width", C.c_int32), ("height", C.c_int32),
+                ("background", C.c_float * 3), ("rr_cap", C.c_float)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("extend_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("paths", C.c_uint64),
+                ("extend_steps", C.c_uint64), ("extend_leaves", C.c_uint64), ("seconds", C.c_double),
+                ("threads", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def camera_init(desc):
+    """desc: a ctypes structure laid out as mcpt_camera_desc."""
+    cam = Camera()
+    if lib().oracle_camera_init(C.addressof(desc), C.addressof(cam)) != 0:
+        raise ValueError("camera size not reproducible")
+    return cam
+
+
+def env_tables(rgba):
+    rgba = np.ascontiguousarray(rgba, np.float32)
+    h, w = rgba.shape[:2]
+    row = np.zeros(h + 1, np.float32)
+    col = np.zeros(h * (w + 1), np.float32)
+    tot = C.c_float()
+    lib().oracle_env_tables(_p(rgba), w, h, _p(row), _p(col), C.byref(tot))
+    return row, col, tot.value
+
+
+def finalize(hot):
+    hot = np.ascontiguousarray(hot, HOT_DTYPE)
+    n = len(hot)
+    boxes = np.zeros(n, AABB_DTYPE)
+    sb = np.zeros(1, AABB_DTYPE)
+    cen = np.zeros((n, 3), np.float32)
+    mort = np.zeros(n, np.uint32)
+    lib().oracle_finalize(_p(hot), n, _p(boxes), _p(sb), _p(cen), _p(mort))
+    return boxes, sb[0], cen, mort
+
+
+def sort_stable(morton):
+    morton = np.ascontiguousarray(morton, np.uint32)
+    k = np.zeros_like(morton)
+    i = np.zeros(len(morton), np.int32)
+    lib().oracle_sort_stable(_p(morton), len(morton), _p(k), _p(i))
+    return k, i
+
+
+def sort_reference(morton):
+    morton = np.ascontiguousarray(morton, np.uint32)
+    k = np.zeros_like(morton)
+    i = np.zeros(len(morton), np.int32)
+    lib().oracle_sort_reference(_p(morton), len(morton), _p(k), _p(i))
+    return k, i
+
+
+def karras(keys):
+    keys = np.ascontiguousarray(keys, np.uint32)
+    n = len(keys)
+    ni = max(n - 1, 0)
+    L, R, P = (np.zeros(ni, np.int32) for _ in range(3))
+    LL, RL = np.zeros(ni, np.uint8), np.zeros(ni, np.uint8)
+    LP = np.full(n, -1, np.int32)
+    lib().oracle_karras(_p(keys), n, _p(L), _p(R), _p(LL), _p(RL), _p(P), _p(LP))
+    return {"left": L, "right": R, "left_is_leaf": LL, "right_is_leaf": RL, "parent": P, "leaf_parent": LP}
+
+
+class Scene:
+    def __init__(self, hot, cold, mats, sky=None, intensity=1.0, rotation=0.0):
+        self.hot = np.ascontiguousarray(hot, HOT_DTYPE)
+        self.cold = np.ascontiguousarray(cold, COLD_DTYPE)
+        self.mats = np.ascontiguousarray(mats, MAT_DTYPE)
+        self.n = len(self.hot)
+        env = None
+        if sky is not None:
+            self._sky = np.ascontiguousarray(sky, np.float32)
+            env = EnvDesc(self._sky.shape[1], self._sky.shape[0], intensity, rotation, self._sky.ctypes.data)
+        self.h = C.c_void_p()
+        rc = lib().oracle_scene_create(_p(self.hot), _p(self.cold), self.n, _p(self.mats), len(self.mats),
+                                       C.byref(env) if env is not None else None, C.byref(self.h))
+        if rc != 0:
+            raise ValueError("oracle_scene_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_scene_destroy(self.h)
+            self.h = None
+
+    def dump(self):
+        n, ni = self.n, max(self.n - 1, 0)
+        out = {
+            "morton": np.zeros(n, np.uint32), "sorted_keys": np.zeros(n, np.uint32),
+            "sorted_ids": np.zeros(n, np.int32), "left": np.zeros(ni, np.int32), "right": np.zeros(ni, np.int32),
+            "left_is_leaf": np.zeros(ni, np.uint8), "right_is_leaf": np.zeros(ni, np.uint8),
+            "parent": np.zeros(ni, np.int32), "leaf_parent": np.zeros(n, np.int32),
+            "internal_bounds": np.zeros(ni, AABB_DTYPE), "leaf_bounds": np.zeros(n, AABB_DTYPE),
+            "scene_bounds": np.zeros(1, AABB_DTYPE),
+        }
+
+        class Dump(C.Structure):
+            _fields_ = [(k, C.c_void_p) for k in ("morton", "sorted_keys", "sorted_ids", "left", "right",
+                                                   "left_is_leaf", "right_is_leaf", "parent", "leaf_parent",
+                                                   "internal_bounds", "leaf_bounds", "scene_bounds")] + \
+                       [("max_depth", C.c_int32), ("build_ms", C.c_float)]
+
+        d = Dump(*[_p(out[k]) if out[k].size else None for k, _ in Dump._fields_[:12]])
+        lib().oracle_scene_dump(self.h, C.byref(d))
+        out["max_depth"] = d.max_depth
+        return out
+
+    def lights(self):
+        n = lib().oracle_scene_lights(self.h, None, 0)
+        out = np.zeros(n, LIGHT_DTYPE)
+        lib().oracle_scene_lights(self.h, _p(out), n)
+        return out
+
+    def trace_closest(self, rays, threads=0):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        steps = np.zeros(len(rays), np.uint32)
+        leaves = np.zeros(len(rays), np.uint32)
+        lib().oracle_trace_closest(self.h, _p(rays), len(rays), _p(hits), _p(steps), _p(leaves), threads)
+        return hits, steps, leaves
+
+    def trace_any(self, rays, threads=0):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        occ = np.zeros(len(rays), np.uint8)
+        lib().oracle_trace_any(self.h, _p(rays), len(rays), _p(occ), threads)
+        return occ
+
+    def render(self, cam, spp, max_depth, seed=0, sample_begin=0, rows=None, accum=None, accum64=False,
+               threads=0):
+        W, H = cam.width, cam.height
+        if accum is None:
+            accum = np.zeros((H, W, 3), np.float32)
+        a64 = np.zeros((H, W, 3), np.float64) if accum64 else None
+        r0, r1 = rows if rows else (0, H)
+        st = RenderStats()
+        rc = lib().oracle_render(self.h, C.addressof(cam), spp, sample_begin, max_depth, seed, r0, r1, _p(accum),
+                                 _p(a64), threads, C.byref(st))
+        if rc != 0:
+            raise ValueError("oracle_render failed")
+        return accum, a64, st.as_dict()
+
+
+FMATH = {"sin": 0, "cos": 1, "asin": 2, "acos": 3, "atan2": 4, "log": 5, "exp": 6, "pow": 7}
+
+
+def fmath(name, x, y=None):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y if y is not None else np.zeros_like(x), np.float32)
+    out = np.zeros_like(x)
+    lib().oracle_fmath(FMATH[name], _p(x), _p(y), _p(out), len(x))
+    return out
+
+
+def pcg32(pixel, sample, seed, n):
+    out = np.zeros(n, np.uint32)
+    lib().oracle_pcg32(pixel, sample, seed, n, _p(out))
+    return out
+
+
+def brute_closest(hot, rays):
+    hot = np.ascontiguousarray(hot, HOT_DTYPE)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    hits = np.zeros(len(rays), HIT_DTYPE)
+    lib().oracle_brute_closest(_p(hot), len(hot), _p(rays), len(rays), _p(hits))
+    return hits
+
+
+def resolve(accum, spp, tonemap=0, exposure=1.0):
+    accum = np.ascontiguousarray(accum, np.float32)
+    H, W = accum.shape[:2]
+    out = np.zeros_like(accum)
+    lib().oracle_resolve(_p(accum), W, H, spp, tonemap, exposure, _p(out))
+    return out
