@@ -22,7 +22,8 @@
  *             s = (m-1)/(m+1), ln 2 split 9 + 24 bits.
  *   exp       k = round(x log2 e), Cody-Waite r = x - k ln2, polynomial on
  *             [-ln2/2, ln2/2], scale by 2^k through the exponent field.
- *   pow       exp(y log x) for x > 0.
+ *   pow       exp(y log x) for x > 0, without extra precision: the error
+ *             grows with |y log x| (bound 2 + 2|y log x| ulp).
  * Polynomial coefficients are weighted least-squares fits at 4000 Chebyshev
  * nodes of the exact Taylor remainder, rounded to binary32.
  * Accuracy against libm (double-rounded reference) is checked by the test
