@@ -112,11 +112,17 @@ MCPT_DEV bool slab(float mnx, float mny, float mnz, float mxx, float mxy, float 
     return !(tfar <= tnear);
 }
 
-// hit_triangle (src/path_tracer.cu:715-741)
-MCPT_DEV bool hit_triangle(const vec3& v0, const vec3& v1, const vec3& v2, const vec3& o, const vec3& d, float& tout) {
+// unit normal exactly as hit_triangle forms it (src/path_tracer.cu:716-718)
+MCPT_HDI vec3 triangle_normal(const vec3& v0, const vec3& v1, const vec3& v2) {
     vec3 e1 = v1 - v0;
     vec3 e2 = v2 - v0;
-    vec3 n = e1.cross(e2).normalize();
+    return e1.cross(e2).normalize();
+}
+
+// hit_triangle (src/path_tracer.cu:715-741) with the normal, a pure function
+// of the vertices, precomputed once per leaf by the BVH build.
+MCPT_DEV bool hit_triangle_n(const vec3& v0, const vec3& v1, const vec3& v2, const vec3& n, const vec3& o,
+                             const vec3& d, float& tout) {
     float den = d.dot(n);
     if (fabsf(den) < 1e-6f) return false;
     float t = (v0 - o).dot(n) / den;
@@ -141,9 +147,19 @@ struct EnvView {
 
 // Everything a kernel needs to trace and shade, passed by value (no
 // __constant__ globals, so several scenes can be in flight at once).
+// Leaf record in Morton (leaf) order: the triangle's 36-byte hot vertices
+// followed by the unit normal hit_triangle would compute, 48 bytes, read as
+// three 16-byte loads.
+struct alignas(16) LeafTri {
+    float4 a;  // v0.xyz, v1.x
+    float4 b;  // v1.yz, v2.xy
+    float4 c;  // v2.z, n.xyz
+};
+static_assert(sizeof(LeafTri) == 48, "leaf record is 48 bytes");
+
 struct SceneView {
     const BvhNode* nodes;
-    const float* leaf_hot;      // 9 floats per leaf, leaf (Morton) order
+    const LeafTri* leaf_tri;    // leaf (Morton) order
     const int* leaf_prim;       // leaf -> original triangle index
     const float* hot;           // 9 floats per triangle, original order
     const mcpt_tri_cold* cold;  // original order

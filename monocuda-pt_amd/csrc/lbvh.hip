@@ -185,13 +185,22 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* kin, c
 }
 
 // ------------------------------------------------------------- leaves
-__global__ __launch_bounds__(kBlock) void k_leaves(const float* hot, const int* ids, int n, float* leaf_hot,
+// leaf records in Morton order: vertices plus the unit normal hit_triangle
+// forms from them (same operations, so the same bits)
+__global__ __launch_bounds__(kBlock) void k_leaves(const float* hot, const int* ids, int n, LeafTri* leaf_tri,
                                                    int* leaf_prim) {
     int j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= n) return;
     int p = ids[j];
     leaf_prim[j] = p;
-    for (int k = 0; k < 9; k++) leaf_hot[(size_t)j * 9 + k] = hot[(size_t)p * 9 + k];
+    const float* t = hot + (size_t)p * 9;
+    const vec3 v0(t[0], t[1], t[2]), v1(t[3], t[4], t[5]), v2(t[6], t[7], t[8]);
+    const vec3 nn = triangle_normal(v0, v1, v2);
+    LeafTri L;
+    L.a = make_float4(v0.x, v0.y, v0.z, v1.x);
+    L.b = make_float4(v1.y, v1.z, v2.x, v2.y);
+    L.c = make_float4(v2.z, nn.x, nn.y, nn.z);
+    leaf_tri[j] = L;
 }
 
 // --------------------------------------------------------- Karras tree
@@ -208,7 +217,7 @@ __device__ __forceinline__ void store_slot(BvhNode* nodes, int node, int side, c
     for (int k = 0; k < 6; k++) f[o + k] = b[k];
 }
 
-__global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* keys, int n, const float* leaf_hot, BvhNode* nodes,
+__global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* keys, int n, const LeafTri* leaf_tri, BvhNode* nodes,
                                                    int* leaf_parent) {
     int idx = blockIdx.x * kBlock + threadIdx.x;
     if (idx >= n - 1) return;
@@ -240,9 +249,8 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* keys, int n, 
             rf[side] = ~c;
             leaf_parent[c] = (idx << 1) | side;
             float b[6];
-            const float* t = leaf_hot + (size_t)c * 9;
-            float tt[9];
-            for (int k = 0; k < 9; k++) tt[k] = t[k];
+            const LeafTri L = leaf_tri[c];
+            const float tt[9] = {L.a.x, L.a.y, L.a.z, L.a.w, L.b.x, L.b.y, L.b.z, L.b.w, L.c.x};
             tri_box(tt, b);
             store_slot(nodes, idx, side, b);
         } else {
@@ -320,7 +328,7 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
 
     const int n_int = std::max(1, n - 1);
     MCPT_TRY(s->nodes.alloc(ctx, n_int));
-    MCPT_TRY(s->leaf_hot.alloc(ctx, (size_t)n * 9));
+    MCPT_TRY(s->leaf_tri.alloc(ctx, n));
     MCPT_TRY(s->leaf_prim.alloc(ctx, n));
     MCPT_TRY(s->root_box.alloc(ctx, 6));
 
@@ -388,7 +396,7 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
     }
     HSTEP(hipGetLastError());
     // 4. leaves and the Karras tree
-    k_leaves<<<blocks_for(n), kBlock, 0, st>>>(s->hot.ptr, vin, n, s->leaf_hot.ptr, s->leaf_prim.ptr);
+    k_leaves<<<blocks_for(n), kBlock, 0, st>>>(s->hot.ptr, vin, n, s->leaf_tri.ptr, s->leaf_prim.ptr);
     k_fill_int<<<blocks_for(n_int), kBlock, 0, st>>>(counters.ptr, n_int, 0);
     k_fill_int<<<1, 64, 0, st>>>(depth.ptr, 1, 0);
     if (n == 1) {
@@ -416,7 +424,7 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
         HSTEP(hipMemcpy(leaf_parent.ptr, &lp, sizeof(int), hipMemcpyHostToDevice));
         s->max_depth = 1;
     } else {
-        k_karras<<<blocks_for(n - 1), kBlock, 0, st>>>(kin, n, s->leaf_hot.ptr, s->nodes.ptr, leaf_parent.ptr);
+        k_karras<<<blocks_for(n - 1), kBlock, 0, st>>>(kin, n, s->leaf_tri.ptr, s->nodes.ptr, leaf_parent.ptr);
         // 5. refit
         k_refit<<<blocks_for(n), kBlock, 0, st>>>(s->nodes.ptr, leaf_parent.ptr, n, counters.ptr, s->root_box.ptr);
         k_depth<<<blocks_for(n), kBlock, 0, st>>>(s->nodes.ptr, leaf_parent.ptr, n, depth.ptr);

@@ -31,7 +31,7 @@ struct mcpt_scene_s {
     bool built = false;
     int max_depth = 0;
     mcpt::DevBuf<mcpt::BvhNode> nodes;
-    mcpt::DevBuf<float> leaf_hot;
+    mcpt::DevBuf<mcpt::LeafTri> leaf_tri;
     mcpt::DevBuf<int> leaf_prim;
     mcpt::DevBuf<float> root_box;  // 6 floats: min xyz, max xyz
     float last_build_ms = 0.f;
@@ -39,7 +39,7 @@ struct mcpt_scene_s {
     mcpt::SceneView view() const {
         mcpt::SceneView v;
         v.nodes = nodes.ptr;
-        v.leaf_hot = leaf_hot.ptr;
+        v.leaf_tri = leaf_tri.ptr;
         v.leaf_prim = leaf_prim.ptr;
         v.hot = hot.ptr;
         v.cold = cold.ptr;
@@ -69,7 +69,7 @@ struct mcpt_scene_s {
         env_row_cdf.release(ctx);
         env_col_cdf.release(ctx);
         nodes.release(ctx);
-        leaf_hot.release(ctx);
+        leaf_tri.release(ctx);
         leaf_prim.release(ctx);
         root_box.release(ctx);
     }

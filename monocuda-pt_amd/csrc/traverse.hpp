@@ -37,6 +37,7 @@ MCPT_DEV void trav_begin(Trav& s, const vec3& o, const vec3& d, float tmax) {
     s.sp = 0;
 }
 
+// original-order hot triangle (9 floats), used when shading
 MCPT_DEV void load_tri(const float* base, int i, vec3& v0, vec3& v1, vec3& v2) {
     const float* t = base + (size_t)i * 9;
     v0 = vec3(t[0], t[1], t[2]);
@@ -44,12 +45,21 @@ MCPT_DEV void load_tri(const float* base, int i, vec3& v0, vec3& v1, vec3& v2) {
     v2 = vec3(t[6], t[7], t[8]);
 }
 
+MCPT_DEV void load_leaf(const LeafTri* base, int i, vec3& v0, vec3& v1, vec3& v2, vec3& n) {
+    const LeafTri* p = base + i;
+    const float4 a = p->a, b = p->b, c = p->c;
+    v0 = vec3(a.x, a.y, a.z);
+    v1 = vec3(a.w, b.x, b.y);
+    v2 = vec3(b.z, b.w, c.x);
+    n = vec3(c.y, c.z, c.w);
+}
+
 // testPrimitiveClosest (src/path_tracer.cu:1537-1551)
 MCPT_DEV void test_leaf_closest(const SceneView& sv, Trav& s, int leaf) {
-    vec3 v0, v1, v2;
-    load_tri(sv.leaf_hot, leaf, v0, v1, v2);
+    vec3 v0, v1, v2, n;
+    load_leaf(sv.leaf_tri, leaf, v0, v1, v2, n);
     float t;
-    if (hit_triangle(v0, v1, v2, s.o, s.d, t) && t < s.t && t > 0.0001f) {
+    if (hit_triangle_n(v0, v1, v2, n, s.o, s.d, t) && t < s.t && t > 0.0001f) {
         s.t = t;
         s.leaf = leaf;
     }
@@ -57,13 +67,16 @@ MCPT_DEV void test_leaf_closest(const SceneView& sv, Trav& s, int leaf) {
 
 // doesPrimitiveBlockRay (src/path_tracer.cu:1553-1563)
 MCPT_DEV bool test_leaf_any(const SceneView& sv, const Trav& s, int leaf) {
-    vec3 v0, v1, v2;
-    load_tri(sv.leaf_hot, leaf, v0, v1, v2);
+    vec3 v0, v1, v2, n;
+    load_leaf(sv.leaf_tri, leaf, v0, v1, v2, n);
     float t;
-    return hit_triangle(v0, v1, v2, s.o, s.d, t) && t > 0.0001f && t < s.t;
+    return hit_triangle_n(v0, v1, v2, n, s.o, s.d, t) && t > 0.0001f && t < s.t;
 }
 
-// One loop iteration; returns true when the traversal is finished.
+// One loop iteration; returns true when the traversal is finished. The up
+// to two leaf children are tested by one loop (left first, as in the
+// reference), so a wave runs the triangle test code once per step rather
+// than once per side.
 template <bool COUNT>
 MCPT_DEV bool step_closest(const SceneView& sv, Trav& s, int* stk, int stride, uint32_t& ns, uint32_t& nl) {
     const BvhNode* np = sv.nodes + s.node;
@@ -73,13 +86,17 @@ MCPT_DEV bool step_closest(const SceneView& sv, Trav& s, int* stk, int stride, u
     const bool oR = slab(b.z, b.w, c.x, c.y, c.z, c.w, s.o, s.inv, s.t);
     const bool lL = r.x < 0, rL = r.y < 0;
     if (COUNT) ns++;
-    if (oL && lL) {
-        if (COUNT) nl++;
-        test_leaf_closest(sv, s, ~r.x);
-    }
+    int first = -1, second = -1;
+    if (oL && lL) first = ~r.x;
     if (oR && rL) {
+        if (first < 0) first = ~r.y;
+        else second = ~r.y;
+    }
+    while (first >= 0) {
         if (COUNT) nl++;
-        test_leaf_closest(sv, s, ~r.y);
+        test_leaf_closest(sv, s, first);
+        first = second;
+        second = -1;
     }
     const bool tL = oL && !lL, tR = oR && !rL;
     if (!tL && !tR) {
@@ -112,8 +129,19 @@ MCPT_DEV bool trace_any(const SceneView& sv, Trav& s, int* stk, int stride) {
         const bool oL = slab(a.x, a.y, a.z, a.w, b.x, b.y, s.o, s.inv, s.t);
         const bool oR = slab(b.z, b.w, c.x, c.y, c.z, c.w, s.o, s.inv, s.t);
         const bool lL = r.x < 0, rL = r.y < 0;
-        if (oL && lL && test_leaf_any(sv, s, ~r.x)) return true;
-        if (oR && rL && test_leaf_any(sv, s, ~r.y)) return true;
+        int first = -1, second = -1;
+        if (oL && lL) first = ~r.x;
+        if (oR && rL) {
+            if (first < 0) first = ~r.y;
+            else second = ~r.y;
+        }
+        bool blocked = false;
+        while (first >= 0 && !blocked) {
+            blocked = test_leaf_any(sv, s, first);
+            first = second;
+            second = -1;
+        }
+        if (blocked) return true;
         const bool tL = oL && !lL, tR = oR && !rL;
         if (!tL && !tR) {
             if (s.sp == 0) return false;
