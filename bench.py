@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--tris", type=int, default=1048576)
     ap.add_argument("--mode", default="persistent", choices=["megakernel", "wavefront", "persistent"])
     ap.add_argument("--refill", type=int, default=0)
+    ap.add_argument("--persistent-shadow", action="store_true", help="shadow rays on the persistent scheme too")
+    ap.add_argument("--batch", type=int, default=0, help="samples in flight per wavefront pass (0 = auto)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the traversal-counting pass")
@@ -95,6 +97,7 @@ def main():
     mode = {"megakernel": mcpt.MODE_MEGAKERNEL, "wavefront": mcpt.MODE_WAVEFRONT,
             "persistent": mcpt.MODE_PERSISTENT}[args.mode]
     W, H, K = args.width, args.height, args.steps
+    xf = mcpt.RENDER_PERSISTENT_SHADOW if args.persistent_shadow else 0
 
     t0 = time.time()
     hot, cold, mats = mcpt.gen_scene(mcpt.GEN_SOUP, seed=args.seed, param=args.tris)
@@ -112,20 +115,27 @@ def main():
         scratch = ctx.film(W, H)
         if args.warmup > 0:
             scene.render(cam, scratch, mode=mode, spp=args.warmup, sample_begin=K, max_depth=args.depth,
-                         seed=args.seed, refill_lanes=args.refill)
+                         seed=args.seed, refill_lanes=args.refill, batch=args.batch, flags=xf)
         log(f"bench: warmup {args.warmup} done")
         t1 = time.perf_counter()
         st = scene.render(cam, film, mode=mode, spp=K, sample_begin=0, max_depth=args.depth, seed=args.seed,
-                          refill_lanes=args.refill, flags=mcpt.RENDER_TIME_KERNELS)
+                          refill_lanes=args.refill, batch=args.batch, flags=mcpt.RENDER_TIME_KERNELS | xf)
         wall_s = time.perf_counter() - t1
         log(f"bench: timed {K} steps in {wall_s:.3f} s (device {st['total_ms']:.1f} ms)")
-        counted = None
+        counted = ref_counted = None
         if not args.no_count:
+            # the timed kernel's own visit counts (near-first order), then the
+            # reference's visit order over the same rays
             scratch.clear()
             counted = scene.render(cam, scratch, mode=mode, spp=K, sample_begin=0, max_depth=args.depth,
-                                   seed=args.seed, refill_lanes=args.refill, flags=mcpt.RENDER_COUNT_TRAVERSAL)
+                                   seed=args.seed, refill_lanes=args.refill, batch=args.batch, flags=mcpt.RENDER_COUNT_TRAVERSAL | xf)
             assert counted["extend_rays"] == st["extend_rays"], "counting pass traced different rays"
-            log("bench: counting pass done")
+            scratch.clear()
+            ref_counted = scene.render(cam, scratch, mode=mode, spp=K, sample_begin=0, max_depth=args.depth,
+                                       seed=args.seed, refill_lanes=args.refill,
+                                       flags=mcpt.RENDER_COUNT_TRAVERSAL | mcpt.RENDER_REFERENCE_ORDER | xf)
+            assert ref_counted["extend_rays"] == st["extend_rays"]
+            log("bench: counting passes done")
         img = film.read()
         film.close()
         scratch.close()
@@ -169,6 +179,12 @@ def main():
             "kernel": "k_extend_persistent" if mode == mcpt.MODE_PERSISTENT else "k_extend",
             "launches": launches, "avg_launch_ms": avg_ms, "alg_bytes_per_launch": alg_bytes / launches,
             "steps_per_ray": counted["extend_steps"] / n, "leaves_per_ray": counted["extend_leaves"] / n,
+        }
+        ref_bytes = 48 * n + 80 * ref_counted["extend_steps"] + 44 * ref_counted["extend_leaves"]
+        result["roofline"]["reference_order"] = {
+            "steps_per_ray": ref_counted["extend_steps"] / n, "leaves_per_ray": ref_counted["extend_leaves"] / n,
+            "equivalent_GBps": ref_bytes / (st["extend_ms"] / 1e3) / 1e9,
+            "note": "the reference's left-first visit order does this many bytes of work for the same hits",
         }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

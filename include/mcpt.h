@@ -213,6 +213,13 @@ int mcpt_trace_any(mcpt_scene scene, const mcpt_ray* rays, int32_t n, uint8_t* o
 /* flags */
 #define MCPT_RENDER_COUNT_TRAVERSAL 1u /* count extend steps/leaf tests      */
 #define MCPT_RENDER_TIME_KERNELS 2u    /* per-kernel HIP event timing        */
+#define MCPT_RENDER_REFERENCE_ORDER 4u /* extend in the reference's visit
+                                          order (left child first) instead of
+                                          near child first; same hits, the
+                                          reference's step/leaf counts       */
+#define MCPT_RENDER_PERSISTENT_SHADOW 8u /* persistent mode: shadow rays on the
+                                            persistent scheme too (measured
+                                            slower than one lane per ray)    */
 
 typedef struct mcpt_render_opts {
     int32_t mode;
@@ -223,6 +230,9 @@ typedef struct mcpt_render_opts {
     int32_t refill_lanes; /* persistent extend: idle lanes before a refill;
                              0 picks the default                           */
     uint32_t flags;
+    int32_t batch;        /* wavefront modes: samples in flight per pass
+                             (paths = W*H*batch); 0 picks about 4 Mi paths */
+    int32_t reserved;
 } mcpt_render_opts;
 
 typedef struct mcpt_render_stats {

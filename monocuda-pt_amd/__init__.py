@@ -37,7 +37,7 @@ ERR_INVALID, ERR_HIP, ERR_BUDGET, ERR_NODEVICE = -1, -2, -3, -4
 MAT_LAMBERTIAN, MAT_DIELECTRIC, MAT_CONDUCTOR, MAT_EMISSIVE, MAT_ISOTROPIC = 0, 1, 2, 3, 4
 MODE_MEGAKERNEL, MODE_WAVEFRONT, MODE_PERSISTENT = 0, 1, 2
 TONEMAP_NONE, TONEMAP_REINHARD, TONEMAP_ACES = 0, 1, 2
-RENDER_COUNT_TRAVERSAL, RENDER_TIME_KERNELS = 1, 2
+RENDER_COUNT_TRAVERSAL, RENDER_TIME_KERNELS, RENDER_REFERENCE_ORDER, RENDER_PERSISTENT_SHADOW = 1, 2, 4, 8
 GEN_CORNELL, GEN_SOUP, GEN_ICOSPHERE, GEN_MEDIUM = 1, 2, 3, 5
 
 # numpy layouts of the C records (byte-identical)
@@ -83,7 +83,8 @@ class LbvhDump(C.Structure):
 
 class RenderOpts(C.Structure):
     _fields_ = [("mode", C.c_int32), ("spp", C.c_int32), ("sample_begin", C.c_int32), ("max_depth", C.c_int32),
-                ("seed", C.c_uint64), ("refill_lanes", C.c_int32), ("flags", C.c_uint32)]
+                ("seed", C.c_uint64), ("refill_lanes", C.c_int32), ("flags", C.c_uint32), ("batch", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class RenderStats(C.Structure):
@@ -318,8 +319,8 @@ class Scene:
         return occ
 
     def render(self, camera, film, mode=MODE_PERSISTENT, spp=1, sample_begin=0, max_depth=8, seed=0,
-               refill_lanes=0, flags=0):
-        o = RenderOpts(mode, spp, sample_begin, max_depth, seed, refill_lanes, flags)
+               refill_lanes=0, flags=0, batch=0):
+        o = RenderOpts(mode, spp, sample_begin, max_depth, seed, refill_lanes, flags, batch, 0)
         st = RenderStats()
         _check(_lib.mcpt_render(self.h, C.byref(camera), C.byref(o), film.h, C.byref(st)), "mcpt_render")
         return st.as_dict()

@@ -68,7 +68,9 @@ struct alignas(16) BvhNode {
     float4 a;  // left.min.xyz, left.max.x
     float4 b;  // left.max.yz, right.min.xy
     float4 c;  // right.min.z, right.max.xyz
-    int4 r;    // left ref, right ref, parent (node<<1 | side, root = -1), unused
+    int4 r;    // left ref, right ref, parent (node<<1 | side, root = -1),
+               // the other end of the node's leaf range (Karras: one end is
+               // the node index itself)
 };
 static_assert(sizeof(BvhNode) == 64, "BVH node must be one 64-byte segment");
 
@@ -111,6 +113,37 @@ MCPT_DEV bool slab(float mnx, float mny, float mnz, float mxx, float mxy, float 
     tfar = fminf(tfar, t1);
     return !(tfar <= tnear);
 }
+
+// Box test of the fast traversal orders: (b - o) / d as fma(b, 1/d, -o/d).
+// It may round the entry and exit distances differently from the reference
+// by an ulp, which only matters for boxes the ray barely touches; every
+// triangle sits at least 1e-3 inside its boxes, so such a box cannot hold a
+// hit, and the exact triangle test alone decides t and the primitive. An
+// axis-parallel ray can turn an axis into NaN, which fmaxf/fminf drop: the
+// test then passes more boxes, never fewer.
+MCPT_DEV bool slab_fma(float mnx, float mny, float mnz, float mxx, float mxy, float mxz, const vec3& oi,
+                       const vec3& inv, float tfar, float& tnear_out) {
+    float tnear = 0.0001f;
+    float t0 = __builtin_fmaf(mnx, inv.x, -oi.x), t1 = __builtin_fmaf(mxx, inv.x, -oi.x);
+    float lo = inv.x < 0.0f ? t1 : t0, hi = inv.x < 0.0f ? t0 : t1;
+    tnear = fmaxf(tnear, lo);
+    tfar = fminf(tfar, hi);
+    t0 = __builtin_fmaf(mny, inv.y, -oi.y);
+    t1 = __builtin_fmaf(mxy, inv.y, -oi.y);
+    lo = inv.y < 0.0f ? t1 : t0;
+    hi = inv.y < 0.0f ? t0 : t1;
+    tnear = fmaxf(tnear, lo);
+    tfar = fminf(tfar, hi);
+    t0 = __builtin_fmaf(mnz, inv.z, -oi.z);
+    t1 = __builtin_fmaf(mxz, inv.z, -oi.z);
+    lo = inv.z < 0.0f ? t1 : t0;
+    hi = inv.z < 0.0f ? t0 : t1;
+    tnear = fmaxf(tnear, lo);
+    tfar = fminf(tfar, hi);
+    tnear_out = tnear;
+    return !(tfar <= tnear);
+}
+
 
 // unit normal exactly as hit_triangle forms it (src/path_tracer.cu:716-718)
 MCPT_HDI vec3 triangle_normal(const vec3& v0, const vec3& v1, const vec3& v2) {
@@ -157,10 +190,16 @@ struct alignas(16) LeafTri {
 };
 static_assert(sizeof(LeafTri) == 48, "leaf record is 48 bytes");
 
-struct SceneView {
+// What traversal needs of a scene (24 bytes, loaded per ray when the rays of
+// several scenes share one queue).
+struct BvhView {
     const BvhNode* nodes;
-    const LeafTri* leaf_tri;    // leaf (Morton) order
-    const int* leaf_prim;       // leaf -> original triangle index
+    const LeafTri* leaf_tri;  // leaf (Morton) order
+    const int* leaf_prim;     // leaf -> original triangle index
+};
+
+struct SceneView {
+    BvhView bvh;
     const float* hot;           // 9 floats per triangle, original order
     const mcpt_tri_cold* cold;  // original order
     const mcpt_material* mats;

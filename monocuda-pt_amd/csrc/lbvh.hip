@@ -259,7 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* keys, int n, 
         }
     }
     if (idx == 0) rf[2] = -1;
-    rf[3] = 0;
+    rf[3] = j;  // the node covers leaves [min(idx, j), max(idx, j)]
 }
 
 // refitBoundsKernel (:1398-1415) on the two-box node layout

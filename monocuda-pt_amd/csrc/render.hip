@@ -37,8 +37,18 @@ struct alignas(16) ShadowItem {
     float4 c;   // contribution rgb, unused
 };
 
-// counters[] slots
-enum { C_Q0 = 0, C_Q1 = 1, C_SHADOW = 2, C_WORK = 3, C_EXT_TOTAL = 4, C_SHD_TOTAL = 6, C_STEPS = 8, C_LEAVES = 10 };
+// counters[] slots (ints; the *_TOTAL, STEPS and LEAVES slots hold u64)
+enum {
+    C_Q0 = 0,
+    C_Q1 = 1,
+    C_SHADOW = 2,
+    C_WORK = 3,
+    C_EXT_TOTAL = 4,
+    C_SHD_TOTAL = 6,
+    C_STEPS = 8,
+    C_LEAVES = 10,
+    C_WORK_SHADOW = 12
+};
 
 struct Workspace {
     RayItem* q[2];
@@ -59,6 +69,7 @@ __global__ void k_counters_init(int* c, int cur, int count) {
         c[1 - cur] = 0;
         c[C_SHADOW] = 0;
         c[C_WORK] = 0;
+        c[C_WORK_SHADOW] = 0;
     }
 }
 // start of a bounce: record the previous bounce's queue lengths, clear the
@@ -68,6 +79,7 @@ __global__ void k_counters_bounce(int* c, int cur) {
         c[1 - cur] = 0;
         c[C_SHADOW] = 0;
         c[C_WORK] = 0;
+        c[C_WORK_SHADOW] = 0;
         *u64slot(c, C_EXT_TOTAL) += (uint64_t)c[cur];
     }
 }
@@ -79,16 +91,19 @@ __global__ void k_counters_zero_totals(int* c) {
 }
 
 // ---------------------------------------------------------------- camera
-// generateCameraRaysKernel (:3342-3374)
-__global__ __launch_bounds__(kBlock) void k_camera(mcpt_camera cam, int sample, uint64_t seed, Workspace ws, int cur) {
+// generateCameraRaysKernel (:3342-3374). A pass carries `batch` samples:
+// path = k * W*H + pixel renders sample `sample0 + k` of `pixel`.
+__global__ __launch_bounds__(kBlock) void k_camera(mcpt_camera cam, int sample0, int batch, uint64_t seed,
+                                                   Workspace ws, int cur) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
-    const int W = cam.width, H = cam.height;
-    if (p >= W * H) return;
+    const int W = cam.width, npix = cam.width * cam.height;
+    if (p >= npix * batch) return;
+    const int pixel = p % npix, k = p / npix;
     Rng rng;
-    mcpt_pcg32_for_path(&rng.g, (uint32_t)p, (uint32_t)sample, seed);
+    mcpt_pcg32_for_path(&rng.g, (uint32_t)pixel, (uint32_t)(sample0 + k), seed);
     vec3 o, d;
     float time;
-    camera_ray(cam, p % W, p / W, rng, o, d, time);
+    camera_ray(cam, pixel % W, pixel / W, rng, o, d, time);
     RayItem it;
     it.ot = make_float4(o.x, o.y, o.z, time);
     it.dp = make_float4(d.x, d.y, d.z, __int_as_float(p));
@@ -114,12 +129,12 @@ __device__ __forceinline__ void count_flush(int* counters, uint32_t ns, uint32_t
     }
 }
 
-__device__ __forceinline__ mcpt_packed_hit pack_hit(const SceneView& sv, const Trav& s) {
+__device__ __forceinline__ mcpt_packed_hit pack_hit(const BvhView& bv, const Trav& s) {
     mcpt_packed_hit h;
     h.t = s.t;
     if (s.leaf >= 0) {
         h.prim_type = 1;
-        h.prim_index = sv.leaf_prim[s.leaf];
+        h.prim_index = bv.leaf_prim[s.leaf];
         h.did_hit = 1;
     } else {
         h.prim_type = -1;
@@ -130,7 +145,7 @@ __device__ __forceinline__ mcpt_packed_hit pack_hit(const SceneView& sv, const T
 }
 
 // extendKernel (:3376-3397): one lane per queued ray
-template <bool COUNT>
+template <bool COUNT, bool REF>
 __global__ __launch_bounds__(kBlock) void k_extend(SceneView sv, Workspace ws, int cur) {
     extern __shared__ int lds_stack[];
     const int slot = blockIdx.x * kBlock + threadIdx.x;
@@ -140,8 +155,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneView sv, Workspace ws, i
         const RayItem it = ws.q[cur][slot];
         Trav s;
         trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
-        trace_closest<COUNT>(sv, s, lds_stack + threadIdx.x, kBlock, ns, nl);
-        ws.hits[slot] = pack_hit(sv, s);
+        trace_closest<COUNT, REF>(sv.bvh, s, lds_stack + threadIdx.x, kBlock, ns, nl);
+        ws.hits[slot] = pack_hit(sv.bvh, s);
     }
     count_flush<COUNT>(ws.counters, ns, nl);
 }
@@ -149,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneView sv, Workspace ws, i
 // extendKernelPersistent (:3416-3468): lanes keep their traversal across
 // iterations; a wave with at least `refill` idle lanes takes that many queue
 // slots with one atomic and hands them out by ballot rank.
-template <bool COUNT>
+template <bool COUNT, bool REF>
 __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Workspace ws, int cur, int refill) {
     extern __shared__ int lds_stack[];
     int* stk = lds_stack + threadIdx.x;
@@ -183,8 +198,8 @@ __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Work
         }
         if (exhausted && ballot64(has) == 0) break;
         if (has) {
-            if (step_closest<COUNT>(sv, s, stk, kBlock, ns, nl)) {
-                ws.hits[slot] = pack_hit(sv, s);
+            if (REF ? step_closest<COUNT>(sv.bvh, s, stk, kBlock, ns, nl) : step_ordered<COUNT>(sv.bvh, s, stk, kBlock, ns, nl)) {
+                ws.hits[slot] = pack_hit(sv.bvh, s);
                 has = false;
             }
         }
@@ -195,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Work
 // ----------------------------------------------------------------- shade
 // shadeKernel (:3470-3696)
 __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, Workspace ws, int cur, int depth, vec3 bg, float rr_cap,
-                                                  uint64_t seed) {
+                                                  int npix) {
     const int slot = blockIdx.x * kBlock + threadIdx.x;
     const int count = ws.counters[cur];
     const bool active = slot < count;
@@ -218,7 +233,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, Workspace ws, in
         ps.prev_spec = smp.w != 0.0f;
         Rng rng;
         rng.g.state = ws.rng[path];
-        rng.g.inc = ((uint64_t)(uint32_t)path << 1u) | 1u;
+        rng.g.inc = ((uint64_t)(uint32_t)(path % npix) << 1u) | 1u;  // the stream of the path's pixel
         shade_bounce(sv, bg, rr_cap, depth, ro, rd, hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out);
         ws.thr[path] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
         ws.smp[path] = make_float4(ps.smp.x, ps.smp.y, ps.smp.z, ps.prev_spec ? 1.0f : 0.0f);
@@ -261,16 +276,78 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneView sv, Workspace ws) {
     }
 }
 
+// Shadow rays on the persistent scheme of the extend kernel (scenes without
+// a medium: a plain visibility test). The reference tried this on 32-lane
+// warps and reverted it (README.md:170); on 64-lane waves it is measured
+// against k_shadow by the benchmark.
+__global__ __launch_bounds__(kBlock) void k_shadow_persistent(SceneView sv, Workspace ws, int refill) {
+    extern __shared__ int lds_stack[];
+    int* stk = lds_stack + threadIdx.x;
+    const int total = ws.counters[C_SHADOW];
+    const int lane = lane_id();
+    Trav s;
+    bool has = false;
+    int slot = -1;
+    bool exhausted = false;  // uniform across the wave
+    while (true) {
+        if (!exhausted) {
+            const uint64_t idle = ballot64(!has);
+            const int nidle = __popcll(idle);
+            if (nidle >= refill) {
+                const int leader = __builtin_ctzll(idle);
+                int first = 0;
+                if (lane == leader) first = atomicAdd(&ws.counters[C_WORK_SHADOW], nidle);
+                first = __shfl(first, leader);
+                if (!has) {
+                    const int q = first + rank_below(idle);
+                    if (q < total) {
+                        const ShadowItem si = ws.sq[q];
+                        trav_begin(s, vec3(si.om.x, si.om.y, si.om.z), vec3(si.dp.x, si.dp.y, si.dp.z), si.om.w);
+                        slot = q;
+                        has = true;
+                    }
+                }
+                if (first + nidle >= total) exhausted = true;
+            }
+        }
+        if (exhausted && ballot64(has) == 0) break;
+        if (has) {
+            const int st = step_any(sv.bvh, s, stk, kBlock);
+            if (st != 0) {
+                if (st == 1) {
+                    const ShadowItem si = ws.sq[slot];
+                    const int path = __float_as_int(si.dp.w);
+                    const float tr = 1.0f;
+                    float4 v = ws.smp[path];
+                    v.x = v.x + si.c.x * tr;
+                    v.y = v.y + si.c.y * tr;
+                    v.z = v.z + si.c.z * tr;
+                    ws.smp[path] = v;
+                }
+                has = false;
+            }
+        }
+    }
+}
+
 // -------------------------------------------------------------- finalize
-// finalizeSampleKernel (:3740-3755)
-__global__ __launch_bounds__(kBlock) void k_finalize(Workspace ws, float* accum, int n) {
+// finalizeSampleKernel (:3740-3755): the pass's samples of a pixel are added
+// in sample order, so the film holds the same FP32 sums however many
+// samples a pass carries.
+__global__ __launch_bounds__(kBlock) void k_finalize(Workspace ws, float* accum, int npix, int batch) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= n) return;
-    const float4 s = ws.smp[p];
-    const vec3 c = clamp_sample(vec3(s.x, s.y, s.z));
-    accum[3 * p] = accum[3 * p] + c.x;
-    accum[3 * p + 1] = accum[3 * p + 1] + c.y;
-    accum[3 * p + 2] = accum[3 * p + 2] + c.z;
+    if (p >= npix) return;
+    float r = accum[3 * p], g = accum[3 * p + 1], b = accum[3 * p + 2];
+    for (int k = 0; k < batch; k++) {
+        const float4 s = ws.smp[(size_t)k * npix + p];
+        const vec3 c = clamp_sample(vec3(s.x, s.y, s.z));
+        r = r + c.x;
+        g = g + c.y;
+        b = b + c.z;
+    }
+    accum[3 * p] = r;
+    accum[3 * p + 1] = g;
+    accum[3 * p + 2] = b;
 }
 
 // ------------------------------------------------------------ megakernel
@@ -299,9 +376,9 @@ __global__ __launch_bounds__(kBlock) void k_megakernel(SceneView sv, mcpt_camera
             Trav s;
             trav_begin(s, ro, rd, 1e30f);
             uint32_t ns = 0, nl = 0;
-            trace_closest<false>(sv, s, stk, kBlock, ns, nl);
+            trace_closest<false>(sv.bvh, s, stk, kBlock, ns, nl);
             n_ext++;
-            const int prim = s.leaf >= 0 ? sv.leaf_prim[s.leaf] : -1;
+            const int prim = s.leaf >= 0 ? sv.bvh.leaf_prim[s.leaf] : -1;
             Bounce out;
             shade_bounce(sv, bg, cam.rr_cap, depth, ro, rd, s.leaf >= 0, s.t, prim, ps, rng, out);
             if (out.shadow) {
@@ -354,7 +431,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const float* accum, float* o
 }
 
 // ------------------------------------------------------------ trace hooks
-template <bool COUNT>
+template <bool COUNT, bool REF>
 __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const mcpt_ray* rays, int n,
                                                           mcpt_packed_hit* hits, uint32_t* steps, uint32_t* leaves) {
     extern __shared__ int lds_stack[];
@@ -364,8 +441,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const mc
     Trav s;
     trav_begin(s, load3(r.origin), load3(r.dir), r.tmax);
     uint32_t ns = 0, nl = 0;
-    trace_closest<COUNT>(sv, s, lds_stack + threadIdx.x, kBlock, ns, nl);
-    hits[i] = pack_hit(sv, s);
+    trace_closest<COUNT, REF>(sv.bvh, s, lds_stack + threadIdx.x, kBlock, ns, nl);
+    hits[i] = pack_hit(sv.bvh, s);
     if (COUNT) {
         steps[i] = ns;
         leaves[i] = nl;
@@ -378,7 +455,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(SceneView sv, const mcpt_r
     const mcpt_ray r = rays[i];
     Trav s;
     trav_begin(s, load3(r.origin), load3(r.dir), r.tmax);
-    occ[i] = trace_any(sv, s, lds_stack + threadIdx.x, kBlock) ? 1 : 0;
+    occ[i] = trace_any(sv.bvh, s, lds_stack + threadIdx.x, kBlock) ? 1 : 0;
 }
 
 int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
@@ -464,10 +541,22 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
     const size_t lds = stack_bytes(s);
     const bool count = (o->flags & MCPT_RENDER_COUNT_TRAVERSAL) != 0;
     const bool timed = (o->flags & MCPT_RENDER_TIME_KERNELS) != 0;
-    int refill = o->refill_lanes > 0 ? std::min(64, o->refill_lanes) : 16;
+    const bool ref = (o->flags & MCPT_RENDER_REFERENCE_ORDER) != 0;
+    int refill = o->refill_lanes > 0 ? std::min(64, o->refill_lanes) : 32;
+
+    // Samples per wavefront pass. Every bounce ends with the slowest rays of
+    // its launch, so the late, thin bounces of one sample leave most of the
+    // GPU idle; carrying many samples per pass keeps them full. Default: up
+    // to 64 Mi paths (about 10.5 GB of queues and path state at 168 B per
+    // path), at most a quarter of the context's HBM budget.
+    const int64_t per_path = 2 * 32 + 16 + 48 + 2 * 16 + 8;
+    int64_t cap_paths = std::min<int64_t>(int64_t(64) << 20, (int64_t)(ctx->budget / 4) / per_path);
+    int batch = o->batch > 0 ? o->batch : (int)std::max<int64_t>(1, cap_paths / npix);
+    batch = std::max(1, std::min(batch, o->spp));
+    if ((int64_t)npix * batch > (int64_t(1) << 30)) batch = std::max(1, (1 << 30) / npix);
 
     Workspace ws;
-    MCPT_TRY(ensure_workspace(ctx, o->mode == MCPT_MODE_MEGAKERNEL ? 1 : npix, ws));
+    MCPT_TRY(ensure_workspace(ctx, o->mode == MCPT_MODE_MEGAKERNEL ? 1 : (int64_t)npix * batch, ws));
     enum { K_CAMERA, K_EXTEND, K_SHADE, K_SHADOW, K_FINALIZE, K_OTHER };
     Timer tm(timed, st);
     hipEvent_t e0, e1;
@@ -477,47 +566,59 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
     k_counters_zero_totals<<<1, 64, 0, st>>>(ws.counters);
     MCPT_TRY(tm.mark(K_OTHER));
 
-    int persistent_blocks = 0;
+    int persistent_blocks = 0, shadow_blocks = 0;
+    const bool persistent_shadow =
+        o->mode == MCPT_MODE_PERSISTENT && !s->has_medium && (o->flags & MCPT_RENDER_PERSISTENT_SHADOW);
     if (o->mode == MCPT_MODE_PERSISTENT) {
         int per_cu = 0;
-        MCPT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_extend_persistent<false>, kBlock, lds));
+        MCPT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_extend_persistent<false, false>, kBlock, lds));
         persistent_blocks = std::max(1, per_cu) * ctx->cu_count;
+        MCPT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_shadow_persistent, kBlock, lds));
+        shadow_blocks = std::max(1, per_cu) * ctx->cu_count;
     }
     uint64_t extend_launches = 0;
-    for (int k = 0; k < o->spp; k++) {
-        const int sample = o->sample_begin + k;
-        if (o->mode == MCPT_MODE_MEGAKERNEL) {
+    if (o->mode == MCPT_MODE_MEGAKERNEL) {
+        for (int k = 0; k < o->spp; k++) {
             dim3 grid((W + 15) / 16, (H + 15) / 16);
-            k_megakernel<<<grid, kBlock, lds, st>>>(sv, *cam, sample, o->seed, o->max_depth, film->accum.ptr,
-                                                   ws.counters);
+            k_megakernel<<<grid, kBlock, lds, st>>>(sv, *cam, o->sample_begin + k, o->seed, o->max_depth,
+                                                   film->accum.ptr, ws.counters);
             MCPT_TRY(tm.mark(K_EXTEND));
-            continue;
         }
+    }
+    for (int k0 = 0; o->mode != MCPT_MODE_MEGAKERNEL && k0 < o->spp; k0 += batch) {
+        const int nb = std::min(batch, o->spp - k0);
+        const int npath = npix * nb;
+        const int grid = blocks_for(npath);
         int cur = 0;
-        k_counters_init<<<1, 64, 0, st>>>(ws.counters, cur, npix);
-        k_camera<<<blocks_for(npix), kBlock, 0, st>>>(*cam, sample, o->seed, ws, cur);
+        k_counters_init<<<1, 64, 0, st>>>(ws.counters, cur, npath);
+        k_camera<<<grid, kBlock, 0, st>>>(*cam, o->sample_begin + k0, nb, o->seed, ws, cur);
         MCPT_TRY(tm.mark(K_CAMERA));
         for (int depth = 0; depth < o->max_depth; depth++) {
             k_counters_bounce<<<1, 64, 0, st>>>(ws.counters, cur);
             MCPT_TRY(tm.mark(K_OTHER));
             if (o->mode == MCPT_MODE_PERSISTENT) {
-                const int blocks = std::min(blocks_for(npix), persistent_blocks);
-                if (count) k_extend_persistent<true><<<blocks, kBlock, lds, st>>>(sv, ws, cur, refill);
-                else k_extend_persistent<false><<<blocks, kBlock, lds, st>>>(sv, ws, cur, refill);
+                const int blocks = std::min(grid, persistent_blocks);
+                auto kern = count ? (ref ? k_extend_persistent<true, true> : k_extend_persistent<true, false>)
+                                  : (ref ? k_extend_persistent<false, true> : k_extend_persistent<false, false>);
+                kern<<<blocks, kBlock, lds, st>>>(sv, ws, cur, refill);
             } else {
-                if (count) k_extend<true><<<blocks_for(npix), kBlock, lds, st>>>(sv, ws, cur);
-                else k_extend<false><<<blocks_for(npix), kBlock, lds, st>>>(sv, ws, cur);
+                auto kern = count ? (ref ? k_extend<true, true> : k_extend<true, false>)
+                                  : (ref ? k_extend<false, true> : k_extend<false, false>);
+                kern<<<grid, kBlock, lds, st>>>(sv, ws, cur);
             }
             extend_launches++;
             MCPT_TRY(tm.mark(K_EXTEND));
-            k_shade<<<blocks_for(npix), kBlock, 0, st>>>(sv, ws, cur, depth, bg, cam->rr_cap, o->seed);
+            k_shade<<<grid, kBlock, 0, st>>>(sv, ws, cur, depth, bg, cam->rr_cap, npix);
             MCPT_TRY(tm.mark(K_SHADE));
             k_counters_shadow_total<<<1, 64, 0, st>>>(ws.counters);
-            k_shadow<<<blocks_for(npix), kBlock, lds, st>>>(sv, ws);
+            if (persistent_shadow)
+                k_shadow_persistent<<<std::min(grid, shadow_blocks), kBlock, lds, st>>>(sv, ws, refill);
+            else
+                k_shadow<<<grid, kBlock, lds, st>>>(sv, ws);
             MCPT_TRY(tm.mark(K_SHADOW));
             cur = 1 - cur;
         }
-        k_finalize<<<blocks_for(npix), kBlock, 0, st>>>(ws, film->accum.ptr, npix);
+        k_finalize<<<blocks_for(npix), kBlock, 0, st>>>(ws, film->accum.ptr, npix, nb);
         MCPT_TRY(tm.mark(K_FINALIZE));
     }
     MCPT_HIP(hipGetLastError());
@@ -585,12 +686,14 @@ int trace_closest_host(mcpt_scene_s* s, const mcpt_ray* rays, int n, mcpt_packed
     if (status >= 0) {
         e = hipMemcpyAsync(dr.ptr, rays, sizeof(mcpt_ray) * n, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) {
+            // with counts: the reference's visit order, so the counts are the
+            // reference's; without: near-first order (same hits)
             if (count)
-                k_trace_closest<true><<<blocks_for(n), kBlock, stack_bytes(s), st>>>(s->view(), dr.ptr, n, dh.ptr,
-                                                                                    dcount.ptr, dcount.ptr + n);
+                k_trace_closest<true, true><<<blocks_for(n), kBlock, stack_bytes(s), st>>>(
+                    s->view(), dr.ptr, n, dh.ptr, dcount.ptr, dcount.ptr + n);
             else
-                k_trace_closest<false><<<blocks_for(n), kBlock, stack_bytes(s), st>>>(s->view(), dr.ptr, n, dh.ptr,
-                                                                                     nullptr, nullptr);
+                k_trace_closest<false, false><<<blocks_for(n), kBlock, stack_bytes(s), st>>>(
+                    s->view(), dr.ptr, n, dh.ptr, nullptr, nullptr);
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipMemcpyAsync(hits, dh.ptr, sizeof(mcpt_packed_hit) * n, hipMemcpyDeviceToHost, st);

@@ -38,9 +38,9 @@ struct mcpt_scene_s {
 
     mcpt::SceneView view() const {
         mcpt::SceneView v;
-        v.nodes = nodes.ptr;
-        v.leaf_tri = leaf_tri.ptr;
-        v.leaf_prim = leaf_prim.ptr;
+        v.bvh.nodes = nodes.ptr;
+        v.bvh.leaf_tri = leaf_tri.ptr;
+        v.bvh.leaf_prim = leaf_prim.ptr;
         v.hot = hot.ptr;
         v.cold = cold.ptr;
         v.mats = mats.ptr;

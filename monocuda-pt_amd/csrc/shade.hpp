@@ -218,9 +218,9 @@ MCPT_DEV float medium_transmittance(const SceneView& sv, vec3 o, const vec3& d, 
         Trav s;
         trav_begin(s, o, d, remaining);
         uint32_t ns = 0, nl = 0;
-        trace_closest<false>(sv, s, stk, stride, ns, nl);
+        trace_closest<false>(sv.bvh, s, stk, stride, ns, nl);
         if (s.leaf < 0) break;
-        int prim = sv.leaf_prim[s.leaf];
+        int prim = sv.bvh.leaf_prim[s.leaf];
         const mcpt_material& m = sv.mats[sv.cold[prim].material];
         if (m.type != MCPT_MAT_ISOTROPIC) return 0.0f;
         vec3 v0, v1, v2;
@@ -241,7 +241,7 @@ MCPT_DEV float shadow_transmittance(const SceneView& sv, const vec3& o, const ve
     if (sv.has_medium) return medium_transmittance(sv, o, d, maxd, stk, stride);
     Trav s;
     trav_begin(s, o, d, maxd);
-    return trace_any(sv, s, stk, stride) ? 0.0f : 1.0f;
+    return trace_any(sv.bvh, s, stk, stride) ? 0.0f : 1.0f;
 }
 
 // ------------------------------------------------------------ one bounce

@@ -31,7 +31,7 @@ def test_record_layouts(mcpt):
     import ctypes as C
     # sizes printed by gcc for the structs of include/mcpt.h
     assert C.sizeof(mcpt.CameraDesc) == 72 and C.sizeof(mcpt.Camera) == 112
-    assert C.sizeof(mcpt.LbvhDump) == 104 and C.sizeof(mcpt.RenderOpts) == 32
+    assert C.sizeof(mcpt.LbvhDump) == 104 and C.sizeof(mcpt.RenderOpts) == 40
     assert C.sizeof(mcpt.RenderStats) == 96 and C.sizeof(mcpt.EnvDesc) == 24
     assert mcpt.HOT_DTYPE.itemsize == 36 and mcpt.COLD_DTYPE.itemsize == 64
 

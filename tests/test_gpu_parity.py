@@ -88,12 +88,16 @@ def test_trace_closest_and_any(mcpt, ctx, oracle, n_tris):
     ref = oracle.Scene(hot, cold, mats)
     rays = _rays(1 << 16, n_tris, (-0.2, 1.2))
     print(f"trace parity n_tris={n_tris}", flush=True)
+    # with counts: the reference's visit order, counts must match too
     hits, steps, leaves = scene.trace_closest(rays, counts=True)
     rh, rs, rl = ref.trace_closest(rays)
     assert np.array_equal(hits["prim_index"], rh["prim_index"])
     assert np.array_equal(hits["t"].view(np.uint32), rh["t"].view(np.uint32))
     assert np.array_equal(hits["did_hit"], rh["did_hit"])
     assert np.array_equal(steps, rs) and np.array_equal(leaves, rl)
+    # without: near-first order, same hits bit for bit
+    fast = scene.trace_closest(rays)
+    assert np.array_equal(fast, hits)
     assert hits["did_hit"].mean() > (0.05 if n_tris >= 65536 else 0.005)
     # any-hit with a finite limit
     rays2 = rays.copy()
@@ -101,6 +105,44 @@ def test_trace_closest_and_any(mcpt, ctx, oracle, n_tris):
     occ = scene.trace_any(rays2)
     assert np.array_equal(occ, ref.trace_any(rays2))
     assert 0 < occ.mean() < 1
+    scene.close()
+
+
+def test_trace_exact_ties(mcpt, ctx, oracle):
+    """A flat grid of 2-triangle quads hit exactly on shared edges and
+    vertices: many triangles share the same t, so the winner is decided by
+    the reference's test order alone (near-first traversal must reproduce
+    it)."""
+    g = 24
+    tris = []
+    for i in range(g):
+        for j in range(g):
+            x0, x1, y0, y1 = i / g, (i + 1) / g, j / g, (j + 1) / g
+            if (i + j) % 2:
+                tris += [((x0, y0, 0), (x1, y0, 0), (x1, y1, 0)), ((x0, y0, 0), (x1, y1, 0), (x0, y1, 0))]
+            else:
+                tris += [((x0, y0, 0), (x1, y0, 0), (x0, y1, 0)), ((x1, y0, 0), (x1, y1, 0), (x0, y1, 0))]
+    hot = np.zeros(len(tris), mcpt.HOT_DTYPE)
+    for k, t in enumerate(tris):
+        hot[k]["v0"], hot[k]["v1"], hot[k]["v2"] = t
+    cold = np.zeros(len(tris), mcpt.COLD_DTYPE)
+    mats = np.zeros(1, mcpt.MAT_DTYPE)
+    pts = [(i / g, j / g) for i in range(g + 1) for j in range(g + 1)]            # vertices
+    pts += [((i + 0.5) / g, j / g) for i in range(g) for j in range(g + 1)]       # edges
+    pts += [((i + 0.5) / g, (j + 0.5) / g) for i in range(g) for j in range(g)]   # diagonals
+    rays = np.zeros(len(pts), mcpt_ray_dtype())
+    rays["origin"] = [(x, y, 1.0) for x, y in pts]
+    rays["dir"] = (0.0, 0.0, -1.0)
+    rays["tmax"] = 1e30
+    scene = ctx.scene(hot, cold, mats)
+    scene.build()
+    rh, _, _ = oracle.Scene(hot, cold, mats).trace_closest(rays)
+    assert rh["did_hit"].sum() > len(pts) // 2
+    for counts in (True, False):
+        got = scene.trace_closest(rays, counts=counts)
+        got = got[0] if counts else got
+        assert np.array_equal(got["prim_index"], rh["prim_index"]), counts
+        assert np.array_equal(got["t"].view(np.uint32), rh["t"].view(np.uint32))
     scene.close()
 
 
@@ -131,6 +173,21 @@ def test_render_cornell(mcpt, ctx, oracle):
         assert np.array_equal(img, base), f"mode {mode} differs from the megakernel"
         assert rel_mse(img, ref_img) < 1e-4
         assert st["extend_rays"] == ref_st["extend_rays"]
+    # samples in flight per pass (4 samples as 3 + 1, and 1 + 1 + 1 + 1): same film
+    for batch in (3, 1):
+        film = ctx.film(W, H)
+        scene.render(cam, film, mode=mcpt.MODE_PERSISTENT, spp=spp, max_depth=depth, seed=seed, batch=batch)
+        assert np.array_equal(film.read(), base), batch
+        film.close()
+    # reference visit order: same image, and the extend kernels' step and
+    # leaf-test totals equal the checker's
+    for mode in (mcpt.MODE_WAVEFRONT, mcpt.MODE_PERSISTENT):
+        film = ctx.film(W, H)
+        st = scene.render(cam, film, mode=mode, spp=spp, max_depth=depth, seed=seed,
+                          flags=mcpt.RENDER_COUNT_TRAVERSAL | mcpt.RENDER_REFERENCE_ORDER)
+        assert np.array_equal(film.read(), base)
+        assert st["extend_steps"] == ref_st["extend_steps"] and st["extend_leaves"] == ref_st["extend_leaves"]
+        film.close()
     scene.close()
 
 
