@@ -69,7 +69,27 @@ def cpu_baseline(hot, cold, mats, sky, desc, depth, seed, budget_s, threads):
     }
 
 
+def copy_calibration(nbytes=1 << 30):
+    """One device-to-device copy of `nbytes`, for scaling rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters: a copy kernel of known size reads and
+    writes exactly nbytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
+
+
 def main():
+    if "--copy-calibration" in sys.argv:
+        copy_calibration()
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64, help="timed sample passes (spp)")
@@ -180,6 +200,17 @@ def main():
             "launches": launches, "avg_launch_ms": avg_ms, "alg_bytes_per_launch": alg_bytes / launches,
             "steps_per_ray": counted["extend_steps"] / n, "leaves_per_ray": counted["extend_leaves"] / n,
         }
+        # HBM-side traffic per launch, measured by rocprofv3 --pmc FETCH_SIZE /
+        # WRITE_SIZE (copy-kernel scaled) for exactly this configuration
+        prof = os.path.join(ROOT, "profiles", "extend_soup1M_64spp.json")
+        if os.path.exists(prof):
+            with open(prof) as f:
+                p = json.load(f)
+            wl = p["workload"]
+            if (wl["width"], wl["height"], wl["spp"], wl["max_depth"], wl["triangles"], wl["mode"], wl["launches"]) == \
+                    (W, H, K, args.depth, len(hot), args.mode, launches) and mode == mcpt.MODE_PERSISTENT:
+                result["roofline"]["traffic"] = p["k_extend_persistent"]["traffic_bytes_per_launch"]
+                result["roofline"]["traffic_source"] = "profiles/extend_soup1M_64spp.json (rocprofv3 --pmc)"
         ref_bytes = 48 * n + 80 * ref_counted["extend_steps"] + 44 * ref_counted["extend_leaves"]
         result["roofline"]["reference_order"] = {
             "steps_per_ray": ref_counted["extend_steps"] / n, "leaves_per_ray": ref_counted["extend_leaves"] / n,
