@@ -266,6 +266,66 @@ int mcpt_render(mcpt_scene scene, const mcpt_camera* camera, const mcpt_render_o
  * tone map, sRGB OETF. out_rgb is host memory, W*H*3. */
 int mcpt_resolve(mcpt_film film, int32_t samples_so_far, int32_t tonemap, float exposure, float* out_rgb);
 
+/* ------------------------------------------------------------ render server
+ * Continuous batching. The reference renders one image per process run;
+ * these entry points have no counterpart in it. A server owns a pool of
+ * paths in flight shared by every admitted job; each bounce step of the
+ * pool refills freed slots with camera paths of admitted jobs, so work of
+ * jobs that arrive while others run overlaps on the device. A job is
+ * admitted when a job slot is free and its film (plus its frame buffers)
+ * fits the server's HBM budget; otherwise it waits (MCPT_DEFERRED) and is
+ * admitted as memory is released. Each job's film is bit-identical to
+ * mcpt_render of the same job alone (samples of a pixel are summed in
+ * sample order). Scenes stay resident; a server runs its own host thread
+ * and HIP stream. */
+typedef struct mcpt_server_s* mcpt_server;
+
+typedef struct mcpt_server_opts {
+    int32_t path_capacity;  /* paths in flight (0 = 8 Mi)                   */
+    int32_t max_jobs;       /* admitted jobs at once (0 = 64)               */
+    int32_t frames_per_job; /* samples of one job in flight (0 = auto)      */
+    int32_t refill_lanes;   /* persistent extend refill threshold (0 = 32)  */
+    uint64_t film_budget_bytes; /* HBM for films of admitted jobs (0 = half
+                                   of the context's free budget)            */
+} mcpt_server_opts;
+
+typedef struct mcpt_job {
+    int32_t scene;     /* index into the server's scene list */
+    int32_t spp;
+    int32_t max_depth;
+    int32_t reserved;
+    uint64_t seed;
+    mcpt_camera camera;
+} mcpt_job;
+
+typedef struct mcpt_job_result {
+    int32_t status; /* MCPT_OK done, MCPT_PENDING running, MCPT_DEFERRED waiting */
+    int32_t width, height, spp;
+    double submit_ms, start_ms, finish_ms; /* server clock                */
+    uint64_t paths;
+} mcpt_job_result;
+
+typedef struct mcpt_server_stats {
+    uint64_t iterations;
+    uint64_t jobs_submitted, jobs_done, jobs_waiting, jobs_running;
+    uint64_t paths_started, extend_rays, shadow_rays;
+    uint64_t film_bytes_in_use, film_budget;
+    double busy_ms; /* device time of the bounce steps */
+} mcpt_server_stats;
+
+int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes, const mcpt_server_opts* opts,
+                       mcpt_server* out);
+/* returns MCPT_OK when admitted at once, MCPT_DEFERRED when queued */
+int mcpt_server_submit(mcpt_server srv, const mcpt_job* job, uint64_t* job_id);
+/* MCPT_OK when done (result filled), MCPT_PENDING / MCPT_DEFERRED otherwise */
+int mcpt_server_poll(mcpt_server srv, uint64_t job_id, mcpt_job_result* result);
+/* blocks until the job is done; copies its film (W*H*3 sums) when rgb != NULL */
+int mcpt_server_wait(mcpt_server srv, uint64_t job_id, mcpt_job_result* result, float* rgb);
+/* frees a finished job's film so waiting jobs can be admitted */
+int mcpt_server_release(mcpt_server srv, uint64_t job_id);
+int mcpt_server_stats_get(mcpt_server srv, mcpt_server_stats* stats);
+int mcpt_server_destroy(mcpt_server srv);
+
 /* ------------------------------------------------------- scene generators */
 #define MCPT_GEN_CORNELL 1 /* 36-triangle Cornell box (10 walls, 24 box, 2 light) */
 #define MCPT_GEN_SOUP 2    /* seeded incoherent triangle soup + 16 emitters      */

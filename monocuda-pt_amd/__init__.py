@@ -101,6 +101,35 @@ class GenSizes(C.Structure):
     _fields_ = [("n_tris", C.c_int32), ("n_materials", C.c_int32)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("path_capacity", C.c_int32), ("max_jobs", C.c_int32), ("frames_per_job", C.c_int32),
+                ("refill_lanes", C.c_int32), ("film_budget_bytes", C.c_uint64)]
+
+
+class Job(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32), ("reserved", C.c_int32),
+                ("seed", C.c_uint64), ("camera", Camera)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
+                ("submit_ms", C.c_double), ("start_ms", C.c_double), ("finish_ms", C.c_double),
+                ("paths", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("iterations", C.c_uint64), ("jobs_submitted", C.c_uint64), ("jobs_done", C.c_uint64),
+                ("jobs_waiting", C.c_uint64), ("jobs_running", C.c_uint64), ("paths_started", C.c_uint64),
+                ("extend_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("film_bytes_in_use", C.c_uint64),
+                ("film_budget", C.c_uint64), ("busy_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 VP = C.c_void_p
 _SIGS = {
     "mcpt_last_error": ([], C.c_char_p),
@@ -125,6 +154,13 @@ _SIGS = {
     "mcpt_film_write": ([VP, VP], C.c_int),
     "mcpt_render": ([VP, C.POINTER(Camera), C.POINTER(RenderOpts), VP, C.POINTER(RenderStats)], C.c_int),
     "mcpt_resolve": ([VP, C.c_int32, C.c_int32, C.c_float, VP], C.c_int),
+    "mcpt_server_create": ([VP, VP, C.c_int32, C.POINTER(ServerOpts), C.POINTER(VP)], C.c_int),
+    "mcpt_server_submit": ([VP, C.POINTER(Job), C.POINTER(C.c_uint64)], C.c_int),
+    "mcpt_server_poll": ([VP, C.c_uint64, C.POINTER(JobResult)], C.c_int),
+    "mcpt_server_wait": ([VP, C.c_uint64, C.POINTER(JobResult), VP], C.c_int),
+    "mcpt_server_release": ([VP, C.c_uint64], C.c_int),
+    "mcpt_server_stats_get": ([VP, C.POINTER(ServerStats)], C.c_int),
+    "mcpt_server_destroy": ([VP], C.c_int),
     "mcpt_gen_scene": ([C.c_int32, C.c_uint64, C.c_int64, VP, VP, VP, C.POINTER(GenSizes)], C.c_int),
     "mcpt_gen_sky": ([C.c_int32, C.c_int32, VP], C.c_int),
     "mcpt_gen_camera": ([C.c_int32, C.c_int32, C.c_int32, C.POINTER(CameraDesc)], C.c_int),
@@ -206,7 +242,8 @@ class Context:
 
     def close(self):
         if self.h:
-            for child in list(self._children):
+            # servers first: they hold the scenes they render
+            for child in sorted(self._children, key=lambda c: not isinstance(c, Server)):
                 child.close()
             _lib.mcpt_ctx_destroy(self.h)
             self.h = VP()
@@ -238,6 +275,9 @@ class Context:
 
     def film(self, width, height):
         return Film(self, width, height)
+
+    def server(self, scenes, **opts):
+        return Server(self, scenes, **opts)
 
 
 class Scene:
@@ -361,3 +401,62 @@ class Film:
         out = np.zeros((self.hgt, self.w, 3), np.float32)
         _check(_lib.mcpt_resolve(self.h, spp, tonemap, exposure, _ptr(out)), "mcpt_resolve")
         return out
+
+
+class Server:
+    """Continuous-batching render server over resident scenes
+    (mcpt_server_*). Jobs render into their own films; a finished job's
+    film stays on the device until release()."""
+
+    def __init__(self, ctx, scenes, path_capacity=0, max_jobs=0, frames_per_job=0, refill_lanes=0,
+                 film_budget_bytes=0):
+        self.ctx = ctx
+        self.scenes = list(scenes)
+        self.h = VP()
+        ctx._children.add(self)
+        arr = (VP * len(self.scenes))(*[sc.h for sc in self.scenes])
+        o = ServerOpts(path_capacity, max_jobs, frames_per_job, refill_lanes, film_budget_bytes)
+        _check(_lib.mcpt_server_create(ctx.h, arr, len(self.scenes), C.byref(o), C.byref(self.h)),
+               "mcpt_server_create")
+
+    def close(self):
+        if self.h:
+            if self.ctx.h:
+                _lib.mcpt_server_destroy(self.h)
+            self.h = VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, scene, camera, spp, max_depth=8, seed=0):
+        """Returns (job_id, admitted_now)."""
+        j = Job(scene, spp, max_depth, 0, seed, camera)
+        jid = C.c_uint64()
+        code = _check(_lib.mcpt_server_submit(self.h, C.byref(j), C.byref(jid)), "mcpt_server_submit")
+        return jid.value, code == OK
+
+    def poll(self, job_id):
+        r = JobResult()
+        _check(_lib.mcpt_server_poll(self.h, job_id, C.byref(r)), "mcpt_server_poll")
+        return r.as_dict()
+
+    def wait(self, job_id, read=True):
+        """Block until the job is done; returns (result, film rgb or None)."""
+        r = JobResult()
+        rgb = None
+        if read:
+            info = self.poll(job_id)
+            rgb = np.zeros((info["height"], info["width"], 3), np.float32)
+        _check(_lib.mcpt_server_wait(self.h, job_id, C.byref(r), _ptr(rgb)), "mcpt_server_wait")
+        return r.as_dict(), rgb
+
+    def release(self, job_id):
+        _check(_lib.mcpt_server_release(self.h, job_id), "mcpt_server_release")
+
+    def stats(self):
+        st = ServerStats()
+        _check(_lib.mcpt_server_stats_get(self.h, C.byref(st)), "mcpt_server_stats_get")
+        return st.as_dict()

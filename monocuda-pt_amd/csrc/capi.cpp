@@ -28,6 +28,7 @@ void clear_error() { g_last_error.clear(); }
 int dev_alloc(Ctx* ctx, void** p, size_t bytes) {
     *p = nullptr;
     if (bytes == 0) return MCPT_OK;
+    std::lock_guard<std::mutex> lock(ctx->alloc_mu);
     if (ctx->used + bytes > ctx->budget)
         return set_error(MCPT_ERR_BUDGET, "allocation of %zu bytes exceeds the HBM budget (%llu of %llu used)", bytes,
                          (unsigned long long)ctx->used, (unsigned long long)ctx->budget);
@@ -42,6 +43,7 @@ int dev_alloc(Ctx* ctx, void** p, size_t bytes) {
 
 void dev_free(Ctx* ctx, void* p, size_t bytes) {
     if (!p) return;
+    std::lock_guard<std::mutex> lock(ctx->alloc_mu);
     (void)hipFree(p);
     ctx->used -= bytes;
 }

@@ -211,6 +211,17 @@ struct SceneView {
     EnvView env;
 };
 
+// ------------------------------------------------------- queue records
+struct alignas(16) RayItem {  // 32 bytes: two 16-byte loads per ray
+    float4 ot;                // origin xyz, time
+    float4 dp;                // direction xyz, path index (int bits)
+};
+struct alignas(16) ShadowItem {
+    float4 om;  // origin xyz, max distance
+    float4 dp;  // direction xyz, path index (int bits)
+    float4 c;   // contribution rgb, unused
+};
+
 // ------------------------------------------------------ wave64 utilities
 MCPT_DEV int lane_id() { return __lane_id(); }
 MCPT_DEV uint64_t ballot64(bool p) { return __ballot(p); }

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "mcpt.h"
@@ -31,6 +32,7 @@ void clear_error();
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    std::mutex alloc_mu;  // guards `used` (a server thread allocates too)
     uint64_t budget = 0;
     uint64_t used = 0;
     int cu_count = 0;

@@ -27,16 +27,6 @@ namespace {
 
 constexpr int kBlock = 256;
 
-struct alignas(16) RayItem {
-    float4 ot;  // origin xyz, time
-    float4 dp;  // direction xyz, path index (int bits)
-};
-struct alignas(16) ShadowItem {
-    float4 om;  // origin xyz, max distance
-    float4 dp;  // direction xyz, path index (int bits)
-    float4 c;   // contribution rgb, unused
-};
-
 // counters[] slots (ints; the *_TOTAL, STEPS and LEAVES slots hold u64)
 enum {
     C_Q0 = 0,
