@@ -69,6 +69,198 @@ def cpu_baseline(hot, cold, mats, sky, desc, depth, seed, budget_s, threads):
     }
 
 
+# ---------------------------------------------------------------- serving
+SERVE_SPP = (1, 4, 16, 64)
+
+
+def serving_workload(mcpt, n_scenes, n_jobs, seed):
+    """Resident scenes with triangle counts log-uniform in 10k-2M (the
+    incoherent-mesh generator) and a seeded job list: scene uniform,
+    square resolution log-uniform in 128^2-1024^2, spp uniform in
+    {1, 4, 16, 64}."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    tris = np.exp(rng.uniform(np.log(1e4), np.log(2e6), n_scenes)).astype(np.int64)
+    jobs = []
+    for i in range(n_jobs):
+        side = int(round(np.exp(rng.uniform(np.log(128), np.log(1024)))))
+        jobs.append({"scene": int(rng.integers(n_scenes)), "side": side, "spp": int(rng.choice(SERVE_SPP)),
+                     "seed": 1000 + i})
+    gaps = rng.exponential(1.0, n_jobs)  # unit-rate Poisson inter-arrival gaps, scaled per load
+    return tris, jobs, gaps
+
+
+def _pct(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, np.float64), q)) if len(xs) else None
+
+
+def _plain_fifo(service_s, arrivals_s):
+    """Latencies of one-job-at-a-time FIFO service with known service times
+    (Lindley's recursion)."""
+    free_at, lat = 0.0, []
+    for a, s in zip(arrivals_s, service_s):
+        start = max(a, free_at)
+        free_at = start + s
+        lat.append(free_at - a)
+    return lat
+
+
+def _run_server(mcpt, srv, jobs, cams, depth, arrivals_s=None):
+    """Submit `jobs` (all at once, or at `arrivals_s` seconds), release each
+    as it finishes; returns per-job latency (s) and the makespan (s)."""
+    import threading
+    ids = {}
+    results = {}
+    done = threading.Event()
+    lock = threading.Lock()
+
+    def reaper():
+        while len(results) < len(jobs):
+            with lock:
+                pending = [(i, jid) for i, jid in ids.items() if i not in results]
+            for i, jid in pending:
+                r = srv.poll(jid)
+                if r["status"] == mcpt.OK:
+                    results[i] = r
+                    srv.release(jid)
+            time.sleep(0.0005)
+        done.set()
+
+    th = threading.Thread(target=reaper, daemon=True)
+    th.start()
+    t0 = time.perf_counter()
+    for i, j in enumerate(jobs):
+        if arrivals_s is not None:
+            wait = t0 + arrivals_s[i] - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+        jid, _ = srv.submit(j["scene"], cams[j["side"]], j["spp"], max_depth=depth, seed=j["seed"])
+        with lock:
+            ids[i] = jid
+    done.wait()
+    th.join()
+    lat = [(results[i]["finish_ms"] - results[i]["submit_ms"]) / 1e3 for i in range(len(jobs))]
+    first = min(r["submit_ms"] for r in results.values())
+    last = max(r["finish_ms"] for r in results.values())
+    return lat, (last - first) / 1e3
+
+
+def _run_plain(mcpt, ctx, scenes, jobs, cams, depth, arrivals_s=None):
+    """One job at a time with mcpt_render (persistent mode), FIFO; a film
+    per job. Returns per-job latency and service time (s) and the makespan."""
+    lat, svc = [], []
+    totals = {"rays": 0, "device_ms": 0.0}
+    t0 = time.perf_counter()
+    for i, j in enumerate(jobs):
+        if arrivals_s is not None:
+            wait = t0 + arrivals_s[i] - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+        ts = time.perf_counter()
+        film = ctx.film(j["side"], j["side"])
+        st = scenes[j["scene"]].render(cams[j["side"]], film, mode=mcpt.MODE_PERSISTENT, spp=j["spp"],
+                                       max_depth=depth, seed=j["seed"])
+        film.close()
+        te = time.perf_counter()
+        svc.append(te - ts)
+        lat.append(te - t0 - (arrivals_s[i] if arrivals_s is not None else 0.0))
+        totals["rays"] += st["extend_rays"] + st["shadow_rays"]
+        totals["device_ms"] += st["total_ms"]
+    _run_plain.totals = totals
+    return lat, svc, time.perf_counter() - t0
+
+
+def serving_bench(mcpt, ctx, args):
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    tris, jobs, gaps = serving_workload(mcpt, args.serve_scenes, args.serve_jobs, args.serve_seed)
+    t0 = time.time()
+    with ThreadPoolExecutor(8) as ex:
+        gen = list(ex.map(lambda a: mcpt.gen_scene(mcpt.GEN_SOUP, seed=a[0], param=int(a[1])),
+                          enumerate(tris, start=100)))
+    sky = mcpt.gen_sky(512, 256)
+    scenes = []
+    for hot, cold, mats in gen:
+        sc = ctx.scene(hot, cold, mats, env=mcpt.make_env(sky))
+        sc.build()
+        scenes.append(sc)
+    del gen
+    log(f"serving: {len(scenes)} resident scenes, {int(tris.sum())} triangles, ready in {time.time() - t0:.1f} s")
+    cams = {j["side"]: mcpt.camera_init(mcpt.gen_camera(mcpt.GEN_SOUP, j["side"], j["side"])) for j in jobs}
+    depth = args.serve_depth
+    paths = [j["side"] ** 2 * j["spp"] for j in jobs]
+
+    # one job at a time, back to back: service times and plain capacity
+    big = max(range(len(jobs)), key=lambda i: paths[i])
+    _run_plain(mcpt, ctx, scenes, [jobs[big]] + jobs[:4], cams, depth)  # warm-up (workspace allocation)
+    _, svc, plain_span = _run_plain(mcpt, ctx, scenes, jobs, cams, depth)
+    plain_cap = len(jobs) / sum(svc)
+    pt = _run_plain.totals
+    plain_dev = {"Mrays_per_device_s": pt["rays"] / pt["device_ms"] / 1e3, "device_s": pt["device_ms"] / 1e3,
+                 "wall_s": sum(svc)}
+    log(f"serving: plain back-to-back {plain_cap:.2f} jobs/s {plain_dev}")
+
+    srv = ctx.server(scenes, path_capacity=args.serve_pool, policy=args.serve_policy)
+    _run_server(mcpt, srv, jobs[:8], cams, depth)  # warm-up
+    s0 = srv.stats()
+    _, span = _run_server(mcpt, srv, jobs, cams, depth)
+    s1 = srv.stats()
+    srv_cap = len(jobs) / span
+    srv_dev = {"Mrays_per_device_s": (s1["extend_rays"] + s1["shadow_rays"] - s0["extend_rays"] - s0["shadow_rays"])
+               / (s1["busy_ms"] - s0["busy_ms"]) / 1e3, "device_s": (s1["busy_ms"] - s0["busy_ms"]) / 1e3,
+               "wall_s": span, "steps": s1["iterations"] - s0["iterations"]}
+    log(f"serving: server saturated {srv_cap:.2f} jobs/s {srv_dev}")
+
+    out = {
+        "workload": f"{len(scenes)} resident incoherent-mesh scenes ({int(tris.min())}-{int(tris.max())} tris, "
+                    f"{int(tris.sum())} total) + 512x256 sky; {len(jobs)} jobs, square resolution log-uniform "
+                    f"128^2-1024^2, spp uniform {{1,4,16,64}}, depth {depth}; Poisson arrivals",
+        "jobs": len(jobs), "mean_paths_per_job": float(np.mean(paths)),
+        "pool_paths": args.serve_pool or 8 << 20,
+        "policy": "shortest-first" if args.serve_policy == mcpt.SCHED_SHORTEST_FIRST else "fifo",
+        "capacity_jobs_per_s": {"server": srv_cap, "plain": plain_cap},
+        "back_to_back": {"server": srv_dev, "plain": plain_dev},
+        "loads": [],
+    }
+    for load in args.serve_loads:
+        lam = load * srv_cap
+        arr = np.cumsum(gaps) / lam
+        arr -= arr[0]
+        lat_s, span_s = _run_server(mcpt, srv, jobs, cams, depth, arr)
+        lat_p, _, span_p = _run_plain(mcpt, ctx, scenes, jobs, cams, depth, arr)
+        p99_s = _pct(lat_s, 99)
+        # the plain path's sustainable rate at the server's p99: Lindley
+        # replay of the measured back-to-back service times on the same
+        # arrival pattern, bisected over the rate
+        lo, hi = 1e-3, plain_cap
+        for _ in range(40):
+            mid = 0.5 * (lo + hi)
+            a = np.cumsum(gaps) / mid
+            if _pct(_plain_fifo(svc, a - a[0]), 99) <= p99_s:
+                lo = mid
+            else:
+                hi = mid
+        rec = {
+            "load": load, "arrival_jobs_per_s": lam,
+            "server": {"jobs_per_s": len(jobs) / span_s, "p50_ms": _pct(lat_s, 50) * 1e3,
+                       "p99_ms": p99_s * 1e3},
+            "plain": {"jobs_per_s": len(jobs) / span_p, "p50_ms": _pct(lat_p, 50) * 1e3,
+                      "p99_ms": _pct(lat_p, 99) * 1e3,
+                      "replay_p99_ms": _pct(_plain_fifo(svc, arr), 99) * 1e3},
+            "plain_jobs_per_s_at_server_p99": lo,
+            "jobs_per_s_ratio_at_equal_p99": lam / lo,
+        }
+        out["loads"].append(rec)
+        log(f"serving: load {load}: {json.dumps(rec)}")
+    st = srv.stats()
+    out["server_stats"] = {k: st[k] for k in ("iterations", "paths_started", "extend_rays", "shadow_rays", "busy_ms")}
+    srv.close()
+    for sc in scenes:
+        sc.close()
+    return out
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device copy of `nbytes`, for scaling rocprofv3's
     FETCH_SIZE / WRITE_SIZE counters: a copy kernel of known size reads and
@@ -106,6 +298,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the traversal-counting pass")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--workload", default="all", choices=["all", "extend", "serving"])
+    ap.add_argument("--serve-jobs", type=int, default=256)
+    ap.add_argument("--serve-scenes", type=int, default=32)
+    ap.add_argument("--serve-seed", type=int, default=7)
+    ap.add_argument("--serve-depth", type=int, default=16)
+    ap.add_argument("--serve-pool", type=int, default=0, help="server path pool (0 = library default)")
+    ap.add_argument("--serve-policy", type=int, default=0, help="0 shortest-first, 1 FIFO")
+    ap.add_argument("--serve-loads", type=float, nargs="*", default=[0.5, 0.8])
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -114,6 +314,15 @@ def main():
         sys.exit(2)
 
     mcpt = importlib.import_module("monocuda-pt_amd")
+    if args.workload == "serving":
+        with mcpt.Context() as ctx:
+            sv = serving_bench(mcpt, ctx, args)
+        print(json.dumps({
+            "metric": "jobs/s and p50/p99 job latency under Poisson load", "value": sv["capacity_jobs_per_s"]["server"],
+            "unit": "jobs/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic (seeded procedural scenes and job stream)",
+            "config": {"workload": sv["workload"]}, "serving": sv}), flush=True)
+        return
     mode = {"megakernel": mcpt.MODE_MEGAKERNEL, "wavefront": mcpt.MODE_WAVEFRONT,
             "persistent": mcpt.MODE_PERSISTENT}[args.mode]
     W, H, K = args.width, args.height, args.steps
@@ -221,6 +430,10 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline(hot, cold, mats, sky, desc, args.depth, args.seed, args.cpu_seconds,
                                               threads)
+    if args.workload == "all":
+        del hot, cold, mats
+        with mcpt.Context() as ctx:
+            result["serving"] = serving_bench(mcpt, ctx, args)
     print(json.dumps(result), flush=True)
 
 

@@ -287,7 +287,13 @@ typedef struct mcpt_server_opts {
     int32_t refill_lanes;   /* persistent extend refill threshold (0 = 32)  */
     uint64_t film_budget_bytes; /* HBM for films of admitted jobs (0 = half
                                    of the context's free budget)            */
+    int32_t policy;         /* which admitted job's paths enter the pool
+                               first: MCPT_SCHED_* (0 = shortest first)     */
+    int32_t reserved;
 } mcpt_server_opts;
+
+#define MCPT_SCHED_SHORTEST_FIRST 0 /* least remaining paths to start first */
+#define MCPT_SCHED_FIFO 1           /* admission order                      */
 
 typedef struct mcpt_job {
     int32_t scene;     /* index into the server's scene list */

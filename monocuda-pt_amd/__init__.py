@@ -39,6 +39,7 @@ MODE_MEGAKERNEL, MODE_WAVEFRONT, MODE_PERSISTENT = 0, 1, 2
 TONEMAP_NONE, TONEMAP_REINHARD, TONEMAP_ACES = 0, 1, 2
 RENDER_COUNT_TRAVERSAL, RENDER_TIME_KERNELS, RENDER_REFERENCE_ORDER, RENDER_PERSISTENT_SHADOW = 1, 2, 4, 8
 GEN_CORNELL, GEN_SOUP, GEN_ICOSPHERE, GEN_MEDIUM = 1, 2, 3, 5
+SCHED_SHORTEST_FIRST, SCHED_FIFO = 0, 1
 
 # numpy layouts of the C records (byte-identical)
 HOT_DTYPE = np.dtype([("v0", "<f4", 3), ("v1", "<f4", 3), ("v2", "<f4", 3)])
@@ -103,7 +104,8 @@ class GenSizes(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("path_capacity", C.c_int32), ("max_jobs", C.c_int32), ("frames_per_job", C.c_int32),
-                ("refill_lanes", C.c_int32), ("film_budget_bytes", C.c_uint64)]
+                ("refill_lanes", C.c_int32), ("film_budget_bytes", C.c_uint64), ("policy", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class Job(C.Structure):
@@ -409,13 +411,13 @@ class Server:
     film stays on the device until release()."""
 
     def __init__(self, ctx, scenes, path_capacity=0, max_jobs=0, frames_per_job=0, refill_lanes=0,
-                 film_budget_bytes=0):
+                 film_budget_bytes=0, policy=0):
         self.ctx = ctx
         self.scenes = list(scenes)
         self.h = VP()
         ctx._children.add(self)
         arr = (VP * len(self.scenes))(*[sc.h for sc in self.scenes])
-        o = ServerOpts(path_capacity, max_jobs, frames_per_job, refill_lanes, film_budget_bytes)
+        o = ServerOpts(path_capacity, max_jobs, frames_per_job, refill_lanes, film_budget_bytes, policy, 0)
         _check(_lib.mcpt_server_create(ctx.h, arr, len(self.scenes), C.byref(o), C.byref(self.h)),
                "mcpt_server_create")
 

@@ -241,5 +241,20 @@ MCPT_DEV int wave_push(int* counter, bool take) {
     }
     return base + rank_below(m);
 }
+// Two pushes with one atomic: the counter holds two 32-bit counts (a in the
+// low half, b in the high half), so lanes taking a slot in either list cost
+// the wave a single round trip.
+MCPT_DEV void wave_push2(unsigned long long* counter, bool take_a, bool take_b, int& slot_a, int& slot_b) {
+    const uint64_t ma = ballot64(take_a), mb = ballot64(take_b);
+    unsigned long long base = 0;
+    if (ma | mb) {
+        const int leader = __builtin_ctzll(ma | mb);
+        if (lane_id() == leader)
+            base = atomicAdd(counter, (unsigned long long)__popcll(ma) + ((unsigned long long)__popcll(mb) << 32));
+        base = __shfl(base, leader);
+    }
+    slot_a = (int)(uint32_t)base + rank_below(ma);
+    slot_b = (int)(uint32_t)(base >> 32) + rank_below(mb);
+}
 
 }  // namespace mcpt

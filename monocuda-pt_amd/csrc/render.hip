@@ -27,17 +27,22 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// counters[] slots (ints; the *_TOTAL, STEPS and LEAVES slots hold u64)
+// counters[] slots, one 128-byte line each: every wave of a launch takes
+// slots with atomics on them, and atomics on one line serialize in one L2
+// channel. C_NS0 / C_NS1 are u64 per ray queue: the queue's length (low 32
+// bits) and the length of the shadow queue filled in the same bounce (high
+// 32 bits), so shade takes both slots with one atomic. The *_TOTAL, STEPS
+// and LEAVES slots hold u64.
 enum {
-    C_Q0 = 0,
-    C_Q1 = 1,
-    C_SHADOW = 2,
-    C_WORK = 3,
-    C_EXT_TOTAL = 4,
-    C_SHD_TOTAL = 6,
-    C_STEPS = 8,
-    C_LEAVES = 10,
-    C_WORK_SHADOW = 12
+    C_NS0 = 0,
+    C_NS1 = 32,
+    C_WORK = 64,
+    C_WORK_SHADOW = 96,
+    C_EXT_TOTAL = 128,
+    C_SHD_TOTAL = 160,
+    C_STEPS = 192,
+    C_LEAVES = 224,
+    C_NCOUNTERS = 256
 };
 
 struct Workspace {
@@ -51,13 +56,17 @@ struct Workspace {
 };
 
 __device__ __forceinline__ uint64_t* u64slot(int* c, int i) { return reinterpret_cast<uint64_t*>(c + i); }
+__device__ __forceinline__ unsigned long long* ns_slot(int* c, int q) {
+    return reinterpret_cast<unsigned long long*>(c + (q ? C_NS1 : C_NS0));
+}
+__device__ __forceinline__ int queue_len(const int* c, int q) { return c[q ? C_NS1 : C_NS0]; }
+__device__ __forceinline__ int shadow_len(const int* c, int q) { return c[(q ? C_NS1 : C_NS0) + 1]; }
 
 // ------------------------------------------------------------ counters
 __global__ void k_counters_init(int* c, int cur, int count) {
     if (threadIdx.x == 0) {
-        c[cur] = count;
-        c[1 - cur] = 0;
-        c[C_SHADOW] = 0;
+        *ns_slot(c, cur) = (unsigned long long)(uint32_t)count;
+        *ns_slot(c, 1 - cur) = 0;
         c[C_WORK] = 0;
         c[C_WORK_SHADOW] = 0;
     }
@@ -66,18 +75,17 @@ __global__ void k_counters_init(int* c, int cur, int count) {
 // queues this bounce fills
 __global__ void k_counters_bounce(int* c, int cur) {
     if (threadIdx.x == 0) {
-        c[1 - cur] = 0;
-        c[C_SHADOW] = 0;
+        *ns_slot(c, 1 - cur) = 0;
         c[C_WORK] = 0;
         c[C_WORK_SHADOW] = 0;
-        *u64slot(c, C_EXT_TOTAL) += (uint64_t)c[cur];
+        *u64slot(c, C_EXT_TOTAL) += (uint64_t)queue_len(c, cur);
     }
 }
-__global__ void k_counters_shadow_total(int* c) {
-    if (threadIdx.x == 0) *u64slot(c, C_SHD_TOTAL) += (uint64_t)c[C_SHADOW];
+__global__ void k_counters_shadow_total(int* c, int cur) {
+    if (threadIdx.x == 0) *u64slot(c, C_SHD_TOTAL) += (uint64_t)shadow_len(c, 1 - cur);
 }
 __global__ void k_counters_zero_totals(int* c) {
-    if (threadIdx.x < 16) c[threadIdx.x] = 0;
+    if (threadIdx.x < C_NCOUNTERS) c[threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------- camera
@@ -139,7 +147,7 @@ template <bool COUNT, bool REF>
 __global__ __launch_bounds__(kBlock) void k_extend(SceneView sv, Workspace ws, int cur) {
     extern __shared__ int lds_stack[];
     const int slot = blockIdx.x * kBlock + threadIdx.x;
-    const int count = ws.counters[cur];
+    const int count = queue_len(ws.counters, cur);
     uint32_t ns = 0, nl = 0;
     if (slot < count) {
         const RayItem it = ws.q[cur][slot];
@@ -158,7 +166,7 @@ template <bool COUNT, bool REF>
 __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Workspace ws, int cur, int refill) {
     extern __shared__ int lds_stack[];
     int* stk = lds_stack + threadIdx.x;
-    const int total = ws.counters[cur];
+    const int total = queue_len(ws.counters, cur);
     const int lane = lane_id();
     Trav s;
     bool has = false;
@@ -202,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Work
 __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, Workspace ws, int cur, int depth, vec3 bg, float rr_cap,
                                                   int npix) {
     const int slot = blockIdx.x * kBlock + threadIdx.x;
-    const int count = ws.counters[cur];
+    const int count = queue_len(ws.counters, cur);
     const bool active = slot < count;
     Bounce out;
     out.next = false;
@@ -230,14 +238,15 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, Workspace ws, in
         ws.rng[path] = rng.g.state;
     }
     // compacted pushes: every lane of the wave takes part
-    const int nslot = wave_push(&ws.counters[1 - cur], out.next);
+    int nslot, sslot;
+    wave_push2(ns_slot(ws.counters, 1 - cur), out.next, out.shadow, nslot, sslot);
     if (out.next) {
         RayItem it;
         it.ot = make_float4(out.no.x, out.no.y, out.no.z, time);
         it.dp = make_float4(out.nd.x, out.nd.y, out.nd.z, __int_as_float(path));
         ws.q[1 - cur][nslot] = it;
     }
-    const int sslot = wave_push(&ws.counters[C_SHADOW], out.shadow);
+
     if (out.shadow) {
         ShadowItem si;
         si.om = make_float4(out.so.x, out.so.y, out.so.z, out.smax);
@@ -249,10 +258,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, Workspace ws, in
 
 // ---------------------------------------------------------------- shadow
 // shadowKernel (:3698-3728)
-__global__ __launch_bounds__(kBlock) void k_shadow(SceneView sv, Workspace ws) {
+__global__ __launch_bounds__(kBlock) void k_shadow(SceneView sv, Workspace ws, int cur) {
     extern __shared__ int lds_stack[];
     const int slot = blockIdx.x * kBlock + threadIdx.x;
-    if (slot >= ws.counters[C_SHADOW]) return;
+    if (slot >= shadow_len(ws.counters, 1 - cur)) return;
     const ShadowItem si = ws.sq[slot];
     const int path = __float_as_int(si.dp.w);
     const float tr = shadow_transmittance(sv, vec3(si.om.x, si.om.y, si.om.z), vec3(si.dp.x, si.dp.y, si.dp.z),
@@ -270,10 +279,10 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneView sv, Workspace ws) {
 // a medium: a plain visibility test). The reference tried this on 32-lane
 // warps and reverted it (README.md:170); on 64-lane waves it is measured
 // against k_shadow by the benchmark.
-__global__ __launch_bounds__(kBlock) void k_shadow_persistent(SceneView sv, Workspace ws, int refill) {
+__global__ __launch_bounds__(kBlock) void k_shadow_persistent(SceneView sv, Workspace ws, int cur, int refill) {
     extern __shared__ int lds_stack[];
     int* stk = lds_stack + threadIdx.x;
-    const int total = ws.counters[C_SHADOW];
+    const int total = shadow_len(ws.counters, 1 - cur);
     const int lane = lane_id();
     Trav s;
     bool has = false;
@@ -457,7 +466,7 @@ int ensure_workspace(Ctx* ctx, int64_t capacity, Workspace& ws) {
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t szq = align(sizeof(RayItem) * capacity), szh = align(sizeof(mcpt_packed_hit) * capacity),
                  szs = align(sizeof(ShadowItem) * capacity), sz4 = align(sizeof(float4) * capacity),
-                 szr = align(sizeof(uint64_t) * capacity), szc = 256;
+                 szr = align(sizeof(uint64_t) * capacity), szc = C_NCOUNTERS * sizeof(int);
     const size_t total = 2 * szq + szh + szs + 2 * sz4 + szr + szc;
     if (ctx->ws_capacity < capacity) {
         if (ctx->ws) dev_free(ctx, ctx->ws, ctx->ws_bytes);
@@ -553,7 +562,7 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
     MCPT_HIP(hipEventCreate(&e0));
     MCPT_HIP(hipEventCreate(&e1));
     MCPT_HIP(hipEventRecord(e0, st));
-    k_counters_zero_totals<<<1, 64, 0, st>>>(ws.counters);
+    k_counters_zero_totals<<<1, C_NCOUNTERS, 0, st>>>(ws.counters);
     MCPT_TRY(tm.mark(K_OTHER));
 
     int persistent_blocks = 0, shadow_blocks = 0;
@@ -600,11 +609,11 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
             MCPT_TRY(tm.mark(K_EXTEND));
             k_shade<<<grid, kBlock, 0, st>>>(sv, ws, cur, depth, bg, cam->rr_cap, npix);
             MCPT_TRY(tm.mark(K_SHADE));
-            k_counters_shadow_total<<<1, 64, 0, st>>>(ws.counters);
+            k_counters_shadow_total<<<1, 64, 0, st>>>(ws.counters, cur);
             if (persistent_shadow)
-                k_shadow_persistent<<<std::min(grid, shadow_blocks), kBlock, lds, st>>>(sv, ws, refill);
+                k_shadow_persistent<<<std::min(grid, shadow_blocks), kBlock, lds, st>>>(sv, ws, cur, refill);
             else
-                k_shadow<<<grid, kBlock, lds, st>>>(sv, ws);
+                k_shadow<<<grid, kBlock, lds, st>>>(sv, ws, cur);
             MCPT_TRY(tm.mark(K_SHADOW));
             cur = 1 - cur;
         }
@@ -616,8 +625,11 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
     MCPT_HIP(hipEventSynchronize(e1));
     float total_ms = 0.f;
     MCPT_HIP(hipEventElapsedTime(&total_ms, e0, e1));
-    uint64_t totals[4] = {0, 0, 0, 0};  // extend rays, shadow rays, steps, leaves
-    MCPT_HIP(hipMemcpy(totals, ws.counters + C_EXT_TOTAL, sizeof(totals), hipMemcpyDeviceToHost));
+    int ctr[C_NCOUNTERS];
+    MCPT_HIP(hipMemcpy(ctr, ws.counters, sizeof(ctr), hipMemcpyDeviceToHost));
+    uint64_t totals[4];  // extend rays, shadow rays, steps, leaves
+    const int at[4] = {C_EXT_TOTAL, C_SHD_TOTAL, C_STEPS, C_LEAVES};
+    for (int i = 0; i < 4; i++) std::memcpy(&totals[i], ctr + at[i], sizeof(uint64_t));
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->total_ms = total_ms;

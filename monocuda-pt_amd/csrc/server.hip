@@ -51,19 +51,31 @@ struct Quota {  // one run of new camera paths: pixels [pixel0, pixel0+count) of
     int job, sample, pixel0, count, prefix;
 };
 
-enum { S_Q0 = 0, S_Q1 = 1, S_SHADOW = 2, S_DONE = 3, S_WORK = 4, S_NCOUNTERS = 16 };
+// Step counters, each in its own 128-byte line: every wave of a step takes
+// slots with atomics on them, and atomics on one line serialize in one L2
+// channel. S_ND holds the next-queue count (low 32 bits) and the done count
+// (high 32 bits) for one combined push.
+enum { S_QCOUNT = 0, S_ND = 32, S_SHADOW = 64, S_WORK = 96, S_NCOUNTERS = 128 };
 
+__device__ __forceinline__ unsigned long long* nd_counter(int* c) {
+    return reinterpret_cast<unsigned long long*>(c + S_ND);
+}
+
+// Path state lives beside its ray: slot i of queue c owns thr[c][i],
+// smp[c][i], rng[c][i] and meta[c][i], and shade moves the state with the
+// ray when it compacts the survivors into the other queue. Every state
+// access of a wave is then one coalesced run, where ids handed out by a
+// free list would turn into a random permutation within a few steps.
 struct Pool {
     RayItem* q[2];
+    float4* thr[2];   // throughput rgb, previous bsdf pdf
+    float4* smp[2];   // sample rgb, previous-bounce-specular flag
+    uint64_t* rng[2];
+    int4* meta[2];    // job slot, pixel, sample, depth
     mcpt_packed_hit* hits;
-    ShadowItem* sq;
-    float4* thr;
-    float4* smp;
-    uint64_t* rng;
-    int4* meta;  // job slot, pixel, sample, depth
-    int* pscene;
-    int* done;
-    int* free_list;
+    ShadowItem* sq;   // dp.w: target state slot (>= 0 next queue, < 0 ~done slot)
+    float4* done_smp; // terminated paths of the step
+    int4* done_meta;
     int* counters;
     int* frame_done;  // max_jobs * max_frames
     JobDev* jobs;
@@ -74,20 +86,19 @@ struct Pool {
 };
 
 // ------------------------------------------------------------------ kernels
-__global__ void k_srv_begin(int* c, int cur) {
+__global__ void k_srv_begin(int* c) {
     if (threadIdx.x == 0) {
-        c[1 - cur] = 0;
+        *nd_counter(c) = 0;
         c[S_SHADOW] = 0;
-        c[S_DONE] = 0;
         c[S_WORK] = 0;
     }
 }
 
-// camera paths for the quota runs; slot = base + g in the current queue,
-// path ids popped from the top of the free list
-__global__ __launch_bounds__(kBlock) void k_srv_generate(Pool p, int cur, int total, int nq, int free_top, int base) {
+// camera paths for the quota runs, appended to the current queue at
+// base + g (mcpt_render's camera kernel per path)
+__global__ __launch_bounds__(kBlock) void k_srv_generate(Pool p, int cur, int total, int nq, int base) {
     const int g = blockIdx.x * kBlock + threadIdx.x;
-    if (g == 0) p.counters[cur] = base + total;
+    if (g == 0) p.counters[S_QCOUNT] = base + total;
     if (g >= total) return;
     int lo = 0, hi = nq - 1;  // last quota with prefix <= g
     while (lo < hi) {
@@ -98,31 +109,33 @@ __global__ __launch_bounds__(kBlock) void k_srv_generate(Pool p, int cur, int to
     const Quota q = p.quotas[lo];
     const JobDev& J = p.jobs[q.job];
     const int pixel = q.pixel0 + (g - q.prefix);
-    const int path = p.free_list[free_top - 1 - g];
     Rng rng;
     mcpt_pcg32_for_path(&rng.g, (uint32_t)pixel, (uint32_t)q.sample, J.seed);
     vec3 o, d;
     float time;
     camera_ray(J.cam, pixel % J.cam.width, pixel / J.cam.width, rng, o, d, time);
+    (void)time;  // camera time: static scenes, no kernel reads it
+    const int slot = base + g;
     RayItem it;
-    it.ot = make_float4(o.x, o.y, o.z, time);
-    it.dp = make_float4(d.x, d.y, d.z, __int_as_float(path));
-    p.q[cur][base + g] = it;
-    p.thr[path] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    p.smp[path] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-    p.rng[path] = rng.g.state;
-    p.meta[path] = make_int4(q.job, pixel, q.sample, 0);
-    p.pscene[path] = J.scene;
+    it.ot = make_float4(o.x, o.y, o.z, __int_as_float(J.scene));
+    it.dp = make_float4(d.x, d.y, d.z, 0.0f);
+    p.q[cur][slot] = it;
+    p.thr[cur][slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    p.smp[cur][slot] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    p.rng[cur][slot] = rng.g.state;
+    p.meta[cur][slot] = make_int4(q.job, pixel, q.sample, 0);
 }
 
-// persistent extend over rays of mixed scenes
+// persistent extend over rays of mixed scenes; the scene index rides in
+// the ray record (ot.w)
 __global__ __launch_bounds__(kBlock) void k_srv_extend(Pool p, int cur, int refill) {
     extern __shared__ int lds_stack[];
     int* stk = lds_stack + threadIdx.x;
-    const int total = p.counters[cur];
+    const int total = p.counters[S_QCOUNT];
     const int lane = lane_id();
     Trav s;
     BvhView bv;
+    int bv_scene = -1;  // scene whose view `bv` holds: refills rarely change it
     bool has = false;
     int slot = -1;
     bool exhausted = false;
@@ -140,7 +153,11 @@ __global__ __launch_bounds__(kBlock) void k_srv_extend(Pool p, int cur, int refi
                     const int q = first + rank_below(idle);
                     if (q < total) {
                         const RayItem it = p.q[cur][q];
-                        bv = p.bvhs[p.pscene[__float_as_int(it.dp.w)]];
+                        const int sc = __float_as_int(it.ot.w);
+                        if (sc != bv_scene) {
+                            bv = p.bvhs[sc];
+                            bv_scene = sc;
+                        }
                         trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
                         slot = q;
                         has = true;
@@ -162,97 +179,124 @@ __global__ __launch_bounds__(kBlock) void k_srv_extend(Pool p, int cur, int refi
     }
 }
 
+// shadeKernel (src/path_tracer.cu:3470-3696) per queue slot; survivors
+// move to the other queue with their state, terminated paths to the done
+// list
 __global__ __launch_bounds__(kBlock) void k_srv_shade(Pool p, int cur) {
     const int slot = blockIdx.x * kBlock + threadIdx.x;
-    const bool active = slot < p.counters[cur];
+    const bool active = slot < p.counters[S_QCOUNT];
     Bounce out;
     out.next = false;
     out.shadow = false;
     bool finished = false;
-    int path = 0;
-    float time = 0.f;
+    int scene = 0;
+    PathState ps;
+    Rng rng;
+    int4 m = make_int4(0, 0, 0, 0);
     if (active) {
         const RayItem it = p.q[cur][slot];
         const mcpt_packed_hit hit = p.hits[slot];
-        path = __float_as_int(it.dp.w);
-        time = it.ot.w;
-        const int4 m = p.meta[path];
+        scene = __float_as_int(it.ot.w);
+        m = p.meta[cur][slot];
         const JobDev& J = p.jobs[m.x];
-        const SceneView& sv = p.scenes[p.pscene[path]];
-        const vec3 ro(it.ot.x, it.ot.y, it.ot.z), rd(it.dp.x, it.dp.y, it.dp.z);
-        const float4 thr = p.thr[path], smp = p.smp[path];
-        PathState ps;
+        const SceneView& sv = p.scenes[scene];
+        const float4 thr = p.thr[cur][slot], smp = p.smp[cur][slot];
         ps.thr = vec3(thr.x, thr.y, thr.z);
         ps.prev_pdf = thr.w;
         ps.smp = vec3(smp.x, smp.y, smp.z);
         ps.prev_spec = smp.w != 0.0f;
-        Rng rng;
-        rng.g.state = p.rng[path];
+        rng.g.state = p.rng[cur][slot];
         rng.g.inc = ((uint64_t)(uint32_t)m.y << 1u) | 1u;
-        shade_bounce(sv, load3(J.cam.background), J.cam.rr_cap, m.w, ro, rd, hit.did_hit != 0, hit.t, hit.prim_index,
-                     ps, rng, out);
+        shade_bounce(sv, load3(J.cam.background), J.cam.rr_cap, m.w, vec3(it.ot.x, it.ot.y, it.ot.z),
+                     vec3(it.dp.x, it.dp.y, it.dp.z), hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out);
         // the reference's depth loop ends after shading bounce max_depth-1
         if (out.next && m.w + 1 >= J.max_depth) out.next = false;
         finished = !out.next;
-        p.thr[path] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
-        p.smp[path] = make_float4(ps.smp.x, ps.smp.y, ps.smp.z, ps.prev_spec ? 1.0f : 0.0f);
-        p.rng[path] = rng.g.state;
-        if (out.next) p.meta[path].w = m.w + 1;
     }
-    const int nslot = wave_push(&p.counters[1 - cur], out.next);
-    if (out.next) {
-        RayItem it;
-        it.ot = make_float4(out.no.x, out.no.y, out.no.z, time);
-        it.dp = make_float4(out.nd.x, out.nd.y, out.nd.z, __int_as_float(path));
-        p.q[1 - cur][nslot] = it;
-    }
+    // compacted pushes: every lane of the wave takes part
+    int nslot, dslot;
+    wave_push2(nd_counter(p.counters), out.next, finished, nslot, dslot);
     const int sslot = wave_push(&p.counters[S_SHADOW], out.shadow);
+    const float4 smp_out = make_float4(ps.smp.x, ps.smp.y, ps.smp.z, ps.prev_spec ? 1.0f : 0.0f);
+    if (out.next) {
+        const int n = 1 - cur;
+        RayItem it;
+        it.ot = make_float4(out.no.x, out.no.y, out.no.z, __int_as_float(scene));
+        it.dp = make_float4(out.nd.x, out.nd.y, out.nd.z, 0.0f);
+        p.q[n][nslot] = it;
+        p.thr[n][nslot] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
+        p.smp[n][nslot] = smp_out;
+        p.rng[n][nslot] = rng.g.state;
+        p.meta[n][nslot] = make_int4(m.x, m.y, m.z, m.w + 1);
+    }
+    if (finished) {
+        p.done_smp[dslot] = smp_out;
+        p.done_meta[dslot] = m;
+    }
     if (out.shadow) {
         ShadowItem si;
         si.om = make_float4(out.so.x, out.so.y, out.so.z, out.smax);
-        si.dp = make_float4(out.sd.x, out.sd.y, out.sd.z, __int_as_float(path));
-        si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, 0.0f);
+        si.dp = make_float4(out.sd.x, out.sd.y, out.sd.z, __int_as_float(out.next ? nslot : ~dslot));
+        si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, __int_as_float(scene));
         p.sq[sslot] = si;
     }
-    const int dslot = wave_push(&p.counters[S_DONE], finished);
-    if (finished) p.done[dslot] = path;
 }
 
-__global__ __launch_bounds__(kBlock) void k_srv_shadow(Pool p) {
+// shadowKernel (src/path_tracer.cu:3698-3728): the light sample is added
+// to the path's state wherever shade moved it
+__global__ __launch_bounds__(kBlock) void k_srv_shadow(Pool p, int cur) {
     extern __shared__ int lds_stack[];
     const int slot = blockIdx.x * kBlock + threadIdx.x;
-    if (slot >= p.counters[S_SHADOW]) return;
-    const ShadowItem si = p.sq[slot];
-    const int path = __float_as_int(si.dp.w);
-    const SceneView& sv = p.scenes[p.pscene[path]];
-    const float tr = shadow_transmittance(sv, vec3(si.om.x, si.om.y, si.om.z), vec3(si.dp.x, si.dp.y, si.dp.z),
-                                          si.om.w, lds_stack + threadIdx.x, kBlock);
-    if (tr > 0.0f) {
-        float4 v = p.smp[path];
+    const bool active = slot < p.counters[S_SHADOW];
+    ShadowItem si;
+    int scene = -1;
+    if (active) {
+        si = p.sq[slot];
+        scene = __float_as_int(si.c.w);
+    }
+    float tr = 0.0f;
+    if (active)
+        tr = shadow_transmittance(p.scenes[scene], vec3(si.om.x, si.om.y, si.om.z), vec3(si.dp.x, si.dp.y, si.dp.z),
+                                  si.om.w, lds_stack + threadIdx.x, kBlock);
+    const int target = __float_as_int(si.dp.w);
+    if (active && tr > 0.0f) {
+        float4* dst = target >= 0 ? p.smp[1 - cur] + target : p.done_smp + ~target;
+        float4 v = *dst;
         v.x = v.x + si.c.x * tr;
         v.y = v.y + si.c.y * tr;
         v.z = v.z + si.c.z * tr;
-        p.smp[path] = v;
+        *dst = v;
     }
 }
 
-// finished paths: clamped sample into the job's frame buffer, path id back
-// onto the free list at free_base + d
-__global__ __launch_bounds__(kBlock) void k_srv_finish(Pool p, int free_base) {
+// terminated paths: clamped sample into the frame buffer of the path's
+// sample (finalizeSampleKernel's clamp, :3740-3755)
+__global__ __launch_bounds__(kBlock) void k_srv_finish(Pool p) {
     const int d = blockIdx.x * kBlock + threadIdx.x;
-    if (d >= p.counters[S_DONE]) return;
-    const int path = p.done[d];
-    const int4 m = p.meta[path];
-    const JobDev& J = p.jobs[m.x];
-    const int fslot = m.z % J.nframes;
-    const float4 s = p.smp[path];
-    const vec3 c = clamp_sample(vec3(s.x, s.y, s.z));
-    float* f = J.frames + ((size_t)fslot * J.npix + m.y) * 3;
-    f[0] = c.x;
-    f[1] = c.y;
-    f[2] = c.z;
-    atomicAdd(&p.frame_done[m.x * p.max_frames + fslot], 1);
-    p.free_list[free_base + d] = path;
+    const bool valid = d < (int)(*nd_counter(p.counters) >> 32);
+    int key = -1;
+    if (valid) {
+        const int4 m = p.done_meta[d];
+        const JobDev& J = p.jobs[m.x];
+        const int fslot = m.z % J.nframes;
+        const float4 s = p.done_smp[d];
+        const vec3 c = clamp_sample(vec3(s.x, s.y, s.z));
+        float* f = J.frames + ((size_t)fslot * J.npix + m.y) * 3;
+        f[0] = c.x;
+        f[1] = c.y;
+        f[2] = c.z;
+        key = m.x * p.max_frames + fslot;
+    }
+    // one atomic per distinct frame in the wave: the terminated paths of a
+    // wave mostly belong to one or two frames
+    uint64_t todo = ballot64(valid);
+    while (todo) {
+        const int leader = __builtin_ctzll(todo);
+        const int k = __shfl(key, leader);
+        const uint64_t same = ballot64(valid && key == k);
+        if (lane_id() == leader) atomicAdd(&p.frame_done[k], __popcll(same));
+        todo &= ~same;
+    }
 }
 
 // film += frame (the sample order is the host's commit order)
@@ -268,10 +312,6 @@ __global__ void k_srv_fill_f(float* p, size_t n, float v) {
 __global__ void k_srv_fill_i(int* p, int n, int v) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) p[i] = v;
-}
-__global__ void k_srv_iota(int* p, int n) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) p[i] = i;
 }
 
 int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
@@ -300,6 +340,7 @@ struct JobHost {
     int opened = 0;     // samples whose frame slot is in use
     int gen_sample = 0, gen_pixel = 0;
     uint64_t paths = 0;
+    int64_t unstarted() const { return (int64_t)(job.spp - gen_sample) * npix - gen_pixel; }
 };
 
 struct mcpt_server_s {
@@ -308,7 +349,10 @@ struct mcpt_server_s {
     hipStream_t stream = nullptr;
     std::chrono::steady_clock::time_point t0;
     // configuration
-    int capacity = 0, max_jobs = 0, frames_opt = 0, refill = 32, max_frames = 64;
+    int capacity = 0, max_jobs = 0, frames_opt = 0, refill = 32, max_frames = 64, policy = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int extend_blocks = 1;  // resident blocks of the persistent extend
+    std::vector<JobHost*> order;
     uint64_t film_budget = 0, film_in_use = 0;
     size_t lds = 0;
     // device memory
@@ -337,7 +381,7 @@ struct mcpt_server_s {
     std::string error_msg;
     std::thread worker;
     // stream state
-    int cur = 0, cur_count = 0, free_top = 0;
+    int cur = 0, cur_count = 0;  // current queue and its length (paths in flight)
     mcpt_server_stats st{};
 };
 
@@ -345,10 +389,18 @@ namespace {
 
 int srv_admit_locked(mcpt_server_s* s);
 
-// frames of one job in flight: enough to fill the pool alone, at most spp
+// Samples of one job in flight. A sample's frame is committed only when its
+// slowest path has ended, while most paths end within a bounce or two, so
+// the ring has to run far ahead of the oldest open sample to keep the pool
+// full: all of the job's samples up to 64, as many as the film budget
+// holds next to the film.
 int frames_for(const mcpt_server_s* s, int npix, int spp) {
-    int f = s->frames_opt > 0 ? s->frames_opt : std::max(1, s->capacity / std::max(1, npix));
-    return std::max(1, std::min(std::min(f, spp), s->max_frames));
+    int f = s->frames_opt > 0 ? s->frames_opt : s->max_frames;
+    f = std::max(1, std::min(std::min(f, spp), s->max_frames));
+    const uint64_t film = (uint64_t)npix * 3 * sizeof(float);
+    const uint64_t fit = s->film_budget / film;  // film + frames
+    if (fit >= 2 && (uint64_t)f + 1 > fit) f = (int)(fit - 1);
+    return f;
 }
 
 void srv_fail(mcpt_server_s* s, const char* what, hipError_t e) {
@@ -409,7 +461,6 @@ int srv_admit_locked(mcpt_server_s* s) {
         k_srv_fill_f<<<blocks_for((int64_t)j.npix * 3), kBlock, 0, s->stream>>>(j.film, (size_t)j.npix * 3, 0.0f);
         k_srv_fill_i<<<1, kBlock, 0, s->stream>>>(s->pool.frame_done + j.slot * s->max_frames, s->max_frames, 0);
     }
-    if (!s->waiting.empty()) s->st.jobs_waiting = s->waiting.size();
     return admitted;
 }
 
@@ -417,13 +468,20 @@ int srv_admit_locked(mcpt_server_s* s) {
 void srv_step(mcpt_server_s* s) {
     Pool& p = s->pool;
     hipStream_t st = s->stream;
-    // ---- plan generation (FIFO by admission)
-    int free = s->free_top;
+    // ---- plan generation into the free part of the current queue
+    int free = s->capacity - s->cur_count;
     int nq = 0, total = 0;
     {
         std::lock_guard<std::mutex> lock(s->mu);
-        for (uint64_t id : s->running) {
-            JobHost& j = s->jobs[id];
+        std::vector<JobHost*>& order = s->order;
+        order.clear();
+        for (uint64_t id : s->running) order.push_back(&s->jobs[id]);
+        if (s->policy == MCPT_SCHED_SHORTEST_FIRST)  // least paths left to start first, ties by admission
+            std::stable_sort(order.begin(), order.end(), [](const JobHost* a, const JobHost* b) {
+                return a->unstarted() < b->unstarted();
+            });
+        for (JobHost* jp : order) {
+            JobHost& j = *jp;
             // open frames
             while (j.opened < j.job.spp && j.opened - j.committed < j.nframes) j.opened++;
             while (free > 0 && nq < kMaxQuotas && j.gen_sample < j.opened) {
@@ -443,27 +501,17 @@ void srv_step(mcpt_server_s* s) {
     }
     if (s->cur_count == 0 && total == 0) return;
     const int cur = s->cur;
-    auto ev_ms = [&](hipEvent_t a, hipEvent_t b) {
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, a, b);
-        return (double)ms;
-    };
-    hipEvent_t e0, e1;
-    SRV_HIP(hipEventCreate(&e0));
-    SRV_HIP(hipEventCreate(&e1));
+    hipEvent_t e0 = s->ev[0], e1 = s->ev[1];
     SRV_HIP(hipEventRecord(e0, st));
-    k_srv_begin<<<1, 64, 0, st>>>(p.counters, cur);
+    k_srv_begin<<<1, 64, 0, st>>>(p.counters);
     if (nq) SRV_HIP(hipMemcpyAsync(p.quotas, s->h_quotas, nq * sizeof(Quota), hipMemcpyHostToDevice, st));
-    k_srv_generate<<<std::max(1, blocks_for(total)), kBlock, 0, st>>>(p, cur, total, nq, s->free_top, s->cur_count);
+    k_srv_generate<<<std::max(1, blocks_for(total)), kBlock, 0, st>>>(p, cur, total, nq, s->cur_count);
     const int inflight = s->cur_count + total;
     const int grid = blocks_for(inflight);
-    int per_cu = 0;
-    SRV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_srv_extend, kBlock, s->lds));
-    k_srv_extend<<<std::min(grid, std::max(1, per_cu) * s->ctx->cu_count), kBlock, s->lds, st>>>(p, cur, s->refill);
+    k_srv_extend<<<std::min(grid, s->extend_blocks), kBlock, s->lds, st>>>(p, cur, s->refill);
     k_srv_shade<<<grid, kBlock, 0, st>>>(p, cur);
-    k_srv_shadow<<<grid, kBlock, s->lds, st>>>(p);
-    const int free_base = s->free_top - total;
-    k_srv_finish<<<grid, kBlock, 0, st>>>(p, free_base);
+    k_srv_shadow<<<grid, kBlock, s->lds, st>>>(p, cur);
+    k_srv_finish<<<grid, kBlock, 0, st>>>(p);
     SRV_HIP(hipGetLastError());
     const size_t status_bytes = (S_NCOUNTERS + (size_t)s->max_jobs * s->max_frames) * sizeof(int);
     SRV_HIP(hipMemcpyAsync(s->h_status, p.counters, S_NCOUNTERS * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -471,11 +519,9 @@ void srv_step(mcpt_server_s* s) {
                            hipMemcpyDeviceToHost, st));
     SRV_HIP(hipEventRecord(e1, st));
     SRV_HIP(hipEventSynchronize(e1));
-    const double busy = ev_ms(e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    const int next_count = s->h_status[1 - cur];
-    const int done = s->h_status[S_DONE];
+    float busy = 0.f;
+    SRV_HIP(hipEventElapsedTime(&busy, e0, e1));
+    const int next_count = s->h_status[S_ND];  // low half of the combined next/done counter
     // ---- bookkeeping
     std::vector<uint64_t> finished;
     {
@@ -485,7 +531,6 @@ void srv_step(mcpt_server_s* s) {
         s->st.paths_started += total;
         s->st.extend_rays += inflight;
         s->st.shadow_rays += s->h_status[S_SHADOW];
-        s->free_top = free_base + done;
         s->cur_count = next_count;
         s->cur = 1 - cur;
         const int* fd = s->h_status + S_NCOUNTERS;
@@ -558,6 +603,11 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
     s->max_jobs = (opts && opts->max_jobs > 0) ? opts->max_jobs : 64;
     s->frames_opt = opts ? opts->frames_per_job : 0;
     s->refill = (opts && opts->refill_lanes > 0) ? std::min(64, opts->refill_lanes) : 32;
+    s->policy = opts ? opts->policy : MCPT_SCHED_SHORTEST_FIRST;
+    if (s->policy != MCPT_SCHED_SHORTEST_FIRST && s->policy != MCPT_SCHED_FIFO) {
+        delete s;
+        return set_error(MCPT_ERR_INVALID, "unknown scheduling policy %d", opts->policy);
+    }
     s->n_scenes = n_scenes;
     s->scenes.assign(scenes, scenes + n_scenes);
     for (int i = 0; i < n_scenes; i++) s->max_depth_scene = std::max(s->max_depth_scene, scenes[i]->max_depth);
@@ -565,10 +615,11 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
     // device memory: pool + tables
     const size_t C = s->capacity;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t parts[] = {al(C * sizeof(RayItem)),        al(C * sizeof(RayItem)),  al(C * sizeof(mcpt_packed_hit)),
-                            al(C * sizeof(ShadowItem)),     al(C * sizeof(float4)),   al(C * sizeof(float4)),
-                            al(C * sizeof(uint64_t)),       al(C * sizeof(int4)),     al(C * sizeof(int)),
-                            al(C * sizeof(int)),            al(C * sizeof(int)),      al(S_NCOUNTERS * sizeof(int)),
+    const size_t parts[] = {al(C * sizeof(RayItem)),    al(C * sizeof(RayItem)),  al(C * sizeof(float4)),
+                            al(C * sizeof(float4)),     al(C * sizeof(float4)),   al(C * sizeof(float4)),
+                            al(C * sizeof(uint64_t)),   al(C * sizeof(uint64_t)), al(C * sizeof(int4)),
+                            al(C * sizeof(int4)),       al(C * sizeof(mcpt_packed_hit)), al(C * sizeof(ShadowItem)),
+                            al(C * sizeof(float4)),     al(C * sizeof(int4)),     al(S_NCOUNTERS * sizeof(int)),
                             al((size_t)s->max_jobs * s->max_frames * sizeof(int)), al(s->max_jobs * sizeof(JobDev)),
                             al(kMaxQuotas * sizeof(Quota)), al(n_scenes * sizeof(SceneView)),
                             al(n_scenes * sizeof(BvhView))};
@@ -585,25 +636,34 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
     Pool& p = s->pool;
     p.q[0] = (RayItem*)b, b += parts[k++];
     p.q[1] = (RayItem*)b, b += parts[k++];
+    p.thr[0] = (float4*)b, b += parts[k++];
+    p.thr[1] = (float4*)b, b += parts[k++];
+    p.smp[0] = (float4*)b, b += parts[k++];
+    p.smp[1] = (float4*)b, b += parts[k++];
+    p.rng[0] = (uint64_t*)b, b += parts[k++];
+    p.rng[1] = (uint64_t*)b, b += parts[k++];
+    p.meta[0] = (int4*)b, b += parts[k++];
+    p.meta[1] = (int4*)b, b += parts[k++];
     p.hits = (mcpt_packed_hit*)b, b += parts[k++];
     p.sq = (ShadowItem*)b, b += parts[k++];
-    p.thr = (float4*)b, b += parts[k++];
-    p.smp = (float4*)b, b += parts[k++];
-    p.rng = (uint64_t*)b, b += parts[k++];
-    p.meta = (int4*)b, b += parts[k++];
-    p.pscene = (int*)b, b += parts[k++];
-    p.done = (int*)b, b += parts[k++];
-    p.free_list = (int*)b, b += parts[k++];
+    p.done_smp = (float4*)b, b += parts[k++];
+    p.done_meta = (int4*)b, b += parts[k++];
     p.counters = (int*)b, b += parts[k++];
     p.frame_done = (int*)b, b += parts[k++];
     p.jobs = (JobDev*)b, b += parts[k++];
     p.quotas = (Quota*)b, b += parts[k++];
     s->scene_tab = (SceneView*)b, b += parts[k++];
     s->bvh_tab = (BvhView*)b, b += parts[k++];
+    if (k != (int)(sizeof(parts) / sizeof(parts[0]))) std::abort();  // every part placed once
     p.scenes = s->scene_tab;
     p.bvhs = s->bvh_tab;
     p.max_frames = s->max_frames;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&s->ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&s->ev[1]);
+    int per_cu = 0;
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_srv_extend, kBlock, s->lds);
+    s->extend_blocks = std::max(1, per_cu) * c->cu_count;
     std::vector<SceneView> sv(n_scenes);
     std::vector<BvhView> bv(n_scenes);
     for (int i = 0; i < n_scenes; i++) {
@@ -618,7 +678,6 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
                           hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_job, s->max_jobs * sizeof(JobDev), hipHostMallocDefault);
     if (e == hipSuccess) {
-        k_srv_iota<<<blocks_for(s->capacity), kBlock, 0, s->stream>>>(p.free_list, s->capacity);
         k_srv_fill_i<<<1, 64, 0, s->stream>>>(p.counters, S_NCOUNTERS, 0);
         k_srv_fill_i<<<blocks_for(s->max_jobs * s->max_frames), kBlock, 0, s->stream>>>(p.frame_done,
                                                                                          s->max_jobs * s->max_frames, 0);
@@ -628,12 +687,13 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
         if (s->h_quotas) (void)hipHostFree(s->h_quotas);
         if (s->h_status) (void)hipHostFree(s->h_status);
         if (s->h_job) (void)hipHostFree(s->h_job);
+        for (hipEvent_t ev : s->ev)
+            if (ev) (void)hipEventDestroy(ev);
         if (s->stream) (void)hipStreamDestroy(s->stream);
         dev_free(c, s->pool_mem, s->pool_bytes);
         delete s;
         return set_error(MCPT_ERR_HIP, "server set-up: %s", hipGetErrorString(e));
     }
-    s->free_top = s->capacity;
     for (int i = s->max_jobs - 1; i >= 0; i--) s->free_slots.push_back(i);
     {
         std::lock_guard<std::mutex> lock(c->alloc_mu);
@@ -767,6 +827,8 @@ int mcpt_server_destroy(mcpt_server s) {
     (void)hipHostFree(s->h_quotas);
     (void)hipHostFree(s->h_status);
     (void)hipHostFree(s->h_job);
+    (void)hipEventDestroy(s->ev[0]);
+    (void)hipEventDestroy(s->ev[1]);
     (void)hipStreamDestroy(s->stream);
     dev_free(s->ctx, s->pool_mem, s->pool_bytes);
     delete s;
