@@ -118,9 +118,8 @@ MCPT_DEV bool slab(float mnx, float mny, float mnz, float mxx, float mxy, float 
 // It may round the entry and exit distances differently from the reference
 // by an ulp, which only matters for boxes the ray barely touches; every
 // triangle sits at least 1e-3 inside its boxes, so such a box cannot hold a
-// hit, and the exact triangle test alone decides t and the primitive. An
-// axis-parallel ray can turn an axis into NaN, which fmaxf/fminf drop: the
-// test then passes more boxes, never fewer.
+// hit, and the exact triangle test alone decides t and the primitive. The
+// reciprocals are finite (trav_begin clamps them), so no axis turns NaN.
 MCPT_DEV bool slab_fma(float mnx, float mny, float mnz, float mxx, float mxy, float mxz, const vec3& oi,
                        const vec3& inv, float tfar, float& tnear_out) {
     float tnear = 0.0001f;

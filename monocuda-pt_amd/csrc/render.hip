@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneView sv, Workspace ws, i
     if (slot < count) {
         const RayItem it = ws.q[cur][slot];
         Trav s;
-        trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+        trav_begin<REF>(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
         trace_closest<COUNT, REF>(sv.bvh, s, lds_stack + threadIdx.x, kBlock, ns, nl);
         ws.hits[slot] = pack_hit(sv.bvh, s);
     }
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Work
                     const int q = first + rank_below(idle);
                     if (q < total) {
                         const RayItem it = ws.q[cur][q];
-                        trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+                        trav_begin<REF>(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
                         slot = q;
                         has = true;
                     }
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const mc
     if (i >= n) return;
     const mcpt_ray r = rays[i];
     Trav s;
-    trav_begin(s, load3(r.origin), load3(r.dir), r.tmax);
+    trav_begin<REF>(s, load3(r.origin), load3(r.dir), r.tmax);
     uint32_t ns = 0, nl = 0;
     trace_closest<COUNT, REF>(sv.bvh, s, lds_stack + threadIdx.x, kBlock, ns, nl);
     hits[i] = pack_hit(sv.bvh, s);

@@ -270,9 +270,20 @@ __global__ __launch_bounds__(kBlock) void k_srv_shadow(Pool p, int cur) {
 }
 
 // terminated paths: clamped sample into the frame buffer of the path's
-// sample (finalizeSampleKernel's clamp, :3740-3755)
-__global__ __launch_bounds__(kBlock) void k_srv_finish(Pool p) {
-    const int d = blockIdx.x * kBlock + threadIdx.x;
+// sample (finalizeSampleKernel's clamp, :3740-3755), and the count of
+// finished paths per frame. The terminated paths of a block nearly always
+// belong to one or two frames: wave leaders add their group counts into a
+// small LDS table and the block adds each table entry with one atomic.
+constexpr int kFinishBlock = 1024;
+constexpr int kFinishKeys = 32;
+__global__ __launch_bounds__(kFinishBlock) void k_srv_finish(Pool p) {
+    __shared__ int lkey[kFinishKeys], lcnt[kFinishKeys];
+    if (threadIdx.x < kFinishKeys) {
+        lkey[threadIdx.x] = -1;
+        lcnt[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const int d = blockIdx.x * kFinishBlock + threadIdx.x;
     const bool valid = d < (int)(*nd_counter(p.counters) >> 32);
     int key = -1;
     if (valid) {
@@ -287,16 +298,28 @@ __global__ __launch_bounds__(kBlock) void k_srv_finish(Pool p) {
         f[2] = c.z;
         key = m.x * p.max_frames + fslot;
     }
-    // one atomic per distinct frame in the wave: the terminated paths of a
-    // wave mostly belong to one or two frames
     uint64_t todo = ballot64(valid);
-    while (todo) {
+    while (todo) {  // one group per distinct key in the wave
         const int leader = __builtin_ctzll(todo);
         const int k = __shfl(key, leader);
         const uint64_t same = ballot64(valid && key == k);
-        if (lane_id() == leader) atomicAdd(&p.frame_done[k], __popcll(same));
+        if (lane_id() == leader) {
+            const int n = __popcll(same);
+            bool placed = false;
+            for (int i = 0; i < kFinishKeys && !placed; i++) {
+                const int e = (k + i) & (kFinishKeys - 1);
+                const int prev = atomicCAS(&lkey[e], -1, k);
+                if (prev == -1 || prev == k) {
+                    atomicAdd(&lcnt[e], n);
+                    placed = true;
+                }
+            }
+            if (!placed) atomicAdd(&p.frame_done[k], n);  // table full
+        }
         todo &= ~same;
     }
+    __syncthreads();
+    if (threadIdx.x < kFinishKeys && lcnt[threadIdx.x] > 0) atomicAdd(&p.frame_done[lkey[threadIdx.x]], lcnt[threadIdx.x]);
 }
 
 // film += frame (the sample order is the host's commit order)
@@ -511,7 +534,7 @@ void srv_step(mcpt_server_s* s) {
     k_srv_extend<<<std::min(grid, s->extend_blocks), kBlock, s->lds, st>>>(p, cur, s->refill);
     k_srv_shade<<<grid, kBlock, 0, st>>>(p, cur);
     k_srv_shadow<<<grid, kBlock, s->lds, st>>>(p, cur);
-    k_srv_finish<<<grid, kBlock, 0, st>>>(p);
+    k_srv_finish<<<(inflight + kFinishBlock - 1) / kFinishBlock, kFinishBlock, 0, st>>>(p);
     SRV_HIP(hipGetLastError());
     const size_t status_bytes = (S_NCOUNTERS + (size_t)s->max_jobs * s->max_frames) * sizeof(int);
     SRV_HIP(hipMemcpyAsync(s->h_status, p.counters, S_NCOUNTERS * sizeof(int), hipMemcpyDeviceToHost, st));

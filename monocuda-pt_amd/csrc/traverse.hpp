@@ -33,10 +33,21 @@ struct Trav {
     int sp;
 };
 
+// EXACT keeps the IEEE reciprocals that the reference's slab test uses
+// (step_closest). The fast orders clamp them to +-1e30: with an infinite
+// reciprocal (a zero direction component) the fused box test would form
+// inf - inf = NaN on that axis for every box, drop the axis, and walk most
+// of the tree. Clamping only moves the slab ends of axes along which the
+// ray advances less than 1e-26 across any scene, and every triangle sits
+// 1e-3 inside its boxes, so no box that holds a reachable hit is culled.
+template <bool EXACT = false>
 MCPT_DEV void trav_begin(Trav& s, const vec3& o, const vec3& d, float tmax) {
     s.o = o;
     s.d = d;
     s.inv = vec3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    if (!EXACT)
+        s.inv = vec3(fminf(fmaxf(s.inv.x, -1e30f), 1e30f), fminf(fmaxf(s.inv.y, -1e30f), 1e30f),
+                     fminf(fmaxf(s.inv.z, -1e30f), 1e30f));
     s.oi = vec3(o.x * s.inv.x, o.y * s.inv.y, o.z * s.inv.z);
     s.t = tmax;
     s.leaf = -1;

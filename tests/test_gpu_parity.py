@@ -146,6 +146,40 @@ def test_trace_exact_ties(mcpt, ctx, oracle):
     scene.close()
 
 
+def test_trace_axis_parallel(mcpt, ctx, oracle):
+    """Directions with zero or denormal components (infinite or huge
+    reciprocals, e.g. the centre pixel of an odd-sized image without jitter):
+    both visit orders must return the reference's hits."""
+    hot, cold, mats = mcpt.gen_scene(mcpt.GEN_SOUP, seed=5, param=20000)
+    rng = np.random.default_rng(3)
+    n = 6000
+    rays = np.zeros(n, mcpt_ray_dtype())
+    rays["origin"] = rng.uniform(-0.1, 1.1, (n, 3)).astype(np.float32)
+    d = np.zeros((n, 3))
+    axis = rng.integers(3, size=n)
+    d[np.arange(n), axis] = rng.choice([-1.0, 1.0], n)
+    two = np.arange(n) % 3 == 1  # one zero component
+    d[two] = rng.normal(size=(two.sum(), 3))
+    d[two, axis[two]] = 0.0
+    tiny = np.arange(n) % 3 == 2  # denormal components
+    d[tiny, (axis[tiny] + 1) % 3] = 1e-39
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays["dir"] = d.astype(np.float32)
+    rays["tmax"] = 1e30
+    scene = ctx.scene(hot, cold, mats)
+    scene.build()
+    rh, _, _ = oracle.Scene(hot, cold, mats).trace_closest(rays)
+    assert rh["did_hit"].sum() > 50
+    for counts in (True, False):
+        got = scene.trace_closest(rays, counts=counts)
+        got = got[0] if counts else got
+        assert np.array_equal(got["prim_index"], rh["prim_index"]), counts
+        assert np.array_equal(got["t"].view(np.uint32), rh["t"].view(np.uint32))
+    rays["tmax"] = 0.3
+    assert np.array_equal(scene.trace_any(rays), oracle.Scene(hot, cold, mats).trace_any(rays))
+    scene.close()
+
+
 def _render_modes(mcpt, ctx, scene, cam, spp, depth, seed):
     out = {}
     for mode in (mcpt.MODE_MEGAKERNEL, mcpt.MODE_WAVEFRONT, mcpt.MODE_PERSISTENT):
