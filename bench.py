@@ -146,9 +146,12 @@ def _run_server(mcpt, srv, jobs, cams, depth, arrivals_s=None):
     return lat, (last - first) / 1e3
 
 
-def _run_plain(mcpt, ctx, scenes, jobs, cams, depth, arrivals_s=None):
+def _run_plain(mcpt, ctx, scenes, jobs, cams, depth, arrivals_s=None, batch=0):
     """One job at a time with mcpt_render (persistent mode), FIFO; a film
-    per job. Returns per-job latency and service time (s) and the makespan."""
+    per job. batch=0 puts all of a job's samples in flight at once; batch=1
+    renders one sample per wavefront pass as the reference's
+    renderWithWavefront does. Returns per-job latency and service time (s)
+    and the makespan."""
     lat, svc = [], []
     totals = {"rays": 0, "device_ms": 0.0}
     t0 = time.perf_counter()
@@ -160,7 +163,7 @@ def _run_plain(mcpt, ctx, scenes, jobs, cams, depth, arrivals_s=None):
         ts = time.perf_counter()
         film = ctx.film(j["side"], j["side"])
         st = scenes[j["scene"]].render(cams[j["side"]], film, mode=mcpt.MODE_PERSISTENT, spp=j["spp"],
-                                       max_depth=depth, seed=j["seed"])
+                                       max_depth=depth, seed=j["seed"], batch=batch)
         film.close()
         te = time.perf_counter()
         svc.append(te - ts)
@@ -201,6 +204,11 @@ def serving_bench(mcpt, ctx, args):
                  "wall_s": sum(svc)}
     log(f"serving: plain back-to-back {plain_cap:.2f} jobs/s {plain_dev}")
 
+    # the reference's structure, one sample per pass, back to back
+    _, svc1, _ = _run_plain(mcpt, ctx, scenes, jobs, cams, depth, batch=1)
+    plain1_cap = len(jobs) / sum(svc1)
+    log(f"serving: plain one sample per pass {plain1_cap:.2f} jobs/s")
+
     srv = ctx.server(scenes, path_capacity=args.serve_pool, policy=args.serve_policy)
     _run_server(mcpt, srv, jobs[:8], cams, depth)  # warm-up
     s0 = srv.stats()
@@ -219,7 +227,7 @@ def serving_bench(mcpt, ctx, args):
         "jobs": len(jobs), "mean_paths_per_job": float(np.mean(paths)),
         "pool_paths": args.serve_pool or 8 << 20,
         "policy": "shortest-first" if args.serve_policy == mcpt.SCHED_SHORTEST_FIRST else "fifo",
-        "capacity_jobs_per_s": {"server": srv_cap, "plain": plain_cap},
+        "capacity_jobs_per_s": {"server": srv_cap, "plain": plain_cap, "plain_one_sample_per_pass": plain1_cap},
         "back_to_back": {"server": srv_dev, "plain": plain_dev},
         "loads": [],
     }
@@ -233,14 +241,18 @@ def serving_bench(mcpt, ctx, args):
         # the plain path's sustainable rate at the server's p99: Lindley
         # replay of the measured back-to-back service times on the same
         # arrival pattern, bisected over the rate
-        lo, hi = 1e-3, plain_cap
-        for _ in range(40):
-            mid = 0.5 * (lo + hi)
-            a = np.cumsum(gaps) / mid
-            if _pct(_plain_fifo(svc, a - a[0]), 99) <= p99_s:
-                lo = mid
-            else:
-                hi = mid
+        def rate_at(service, cap):
+            lo, hi = 1e-3, cap
+            for _ in range(40):
+                mid = 0.5 * (lo + hi)
+                a = np.cumsum(gaps) / mid
+                if _pct(_plain_fifo(service, a - a[0]), 99) <= p99_s:
+                    lo = mid
+                else:
+                    hi = mid
+            return lo
+        lo = rate_at(svc, plain_cap)
+        lo1 = rate_at(svc1, plain1_cap)
         rec = {
             "load": load, "arrival_jobs_per_s": lam,
             "server": {"jobs_per_s": len(jobs) / span_s, "p50_ms": _pct(lat_s, 50) * 1e3,
@@ -250,6 +262,9 @@ def serving_bench(mcpt, ctx, args):
                       "replay_p99_ms": _pct(_plain_fifo(svc, arr), 99) * 1e3},
             "plain_jobs_per_s_at_server_p99": lo,
             "jobs_per_s_ratio_at_equal_p99": lam / lo,
+            "plain_one_sample_per_pass": {"replay_p99_ms": _pct(_plain_fifo(svc1, arr), 99) * 1e3,
+                                          "jobs_per_s_at_server_p99": lo1,
+                                          "jobs_per_s_ratio_at_equal_p99": lam / lo1},
         }
         out["loads"].append(rec)
         log(f"serving: load {load}: {json.dumps(rec)}")
