@@ -332,6 +332,77 @@ int mcpt_server_release(mcpt_server srv, uint64_t job_id);
 int mcpt_server_stats_get(mcpt_server srv, mcpt_server_stats* stats);
 int mcpt_server_destroy(mcpt_server srv);
 
+/* --------------------------------------------------------- radiance cache
+ * No counterpart in the reference. An MLP trained online on the tracer's
+ * own paths predicts the radiance scattered at a path vertex (next-event
+ * light plus everything the continuation would gather). Features, 11
+ * floats per vertex: position normalised to the scene box (3), outgoing
+ * direction and normal as (theta/pi, phi/(2pi) + 1/2) (2 + 2), roughness
+ * (1), albedo (3). They are encoded into 64 inputs (sin/cos at 6 octaves
+ * for the position, 2 for the angles, the raw values and a constant 1),
+ * then 4 hidden layers of 64 ReLU units and a linear RGB output; no
+ * biases. Weights and activations are BF16 on the matrix cores with FP32
+ * accumulation; master weights and Adam moments are FP32. */
+typedef struct mcpt_mlp_s* mcpt_mlp;
+
+#define MCPT_MLP_FEATURES 11
+#define MCPT_MLP_INPUTS 64
+#define MCPT_MLP_WIDTH 64
+#define MCPT_MLP_HIDDEN 4
+#define MCPT_MLP_OUTPUTS 3
+/* W0..W3 as [64 out][64 in], then W4 as [3 out][64 in], row-major */
+#define MCPT_MLP_PARAMS (4 * 64 * 64 + 3 * 64)
+
+/* He-uniform initial weights from a PCG32 stream keyed by seed */
+int mcpt_mlp_create(mcpt_ctx ctx, uint64_t seed, mcpt_mlp* out);
+int mcpt_mlp_destroy(mcpt_mlp mlp);
+int mcpt_mlp_get_params(mcpt_mlp mlp, float* params /* MCPT_MLP_PARAMS */);
+/* replaces the weights and resets the optimiser */
+int mcpt_mlp_set_params(mcpt_mlp mlp, const float* params);
+/* host arrays: features n*11, rgb n*3 */
+int mcpt_mlp_forward(mcpt_mlp mlp, const float* features, int32_t n, float* rgb);
+/* Relative L2 loss of Mueller et al. 2021, mean over samples and channels
+ * of (y - t)^2 / (y^2 + 0.01) with the denominator held constant, and its
+ * gradient with respect to the parameters (MCPT_MLP_PARAMS floats). */
+int mcpt_mlp_gradient(mcpt_mlp mlp, const float* features, const float* targets, int32_t n, float* loss,
+                      float* grad);
+/* Adam (beta1 0.9, beta2 0.99, eps 1e-8, bias-corrected) over the batch in
+ * consecutive mini-batches of `batch` samples; loss = the last step's */
+int mcpt_mlp_train(mcpt_mlp mlp, const float* features, const float* targets, int32_t n, int32_t batch, float lr,
+                   float* loss);
+
+typedef struct mcpt_cache_opts {
+    int32_t train_paths;     /* unbiased training paths per frame (0 = 65536) */
+    int32_t train_batch;     /* samples per Adam step (0 = 16384)             */
+    int32_t train_steps;     /* at most this many steps per frame (0 = 4)     */
+    int32_t terminate_depth; /* diffuse hits at this bounce or later end in
+                                the cache (0 = 1: the first secondary hit)   */
+    float lr;                /* Adam step size (0 = 0.005)                    */
+    int32_t reserved;
+} mcpt_cache_opts;
+
+typedef struct mcpt_cache_stats {
+    double total_ms;       /* device time of the call                        */
+    double train_trace_ms; /* tracing the training paths                     */
+    double train_ms;       /* MLP forward + backward + Adam                  */
+    double render_ms;      /* tracing the image paths                        */
+    double infer_ms;       /* MLP queries of the image paths                 */
+    uint64_t train_samples, queries, paths;
+    float loss;            /* last training step                             */
+    int32_t reserved;
+} mcpt_cache_stats;
+
+/* Renders opts->spp frames (samples sample_begin ...). A frame first traces
+ * cache_opts->train_paths unbiased paths from pixels drawn at random, turns
+ * every diffuse vertex of them into a training sample (target: the radiance
+ * the path gathered after the vertex divided by its throughput there) and
+ * takes Adam steps; then it renders one sample per pixel whose paths end in
+ * the cache at their first diffuse hit from terminate_depth on. The film
+ * receives the same clamped linear sums as mcpt_render. Requires
+ * MCPT_MODE_PERSISTENT or MCPT_MODE_WAVEFRONT. */
+int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* camera, const mcpt_render_opts* opts, mcpt_film film,
+                       mcpt_mlp mlp, const mcpt_cache_opts* cache_opts, mcpt_cache_stats* stats);
+
 /* ------------------------------------------------------- scene generators */
 #define MCPT_GEN_CORNELL 1 /* 36-triangle Cornell box (10 walls, 24 box, 2 light) */
 #define MCPT_GEN_SOUP 2    /* seeded incoherent triangle soup + 16 emitters      */

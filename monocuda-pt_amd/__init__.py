@@ -39,6 +39,8 @@ MODE_MEGAKERNEL, MODE_WAVEFRONT, MODE_PERSISTENT = 0, 1, 2
 TONEMAP_NONE, TONEMAP_REINHARD, TONEMAP_ACES = 0, 1, 2
 RENDER_COUNT_TRAVERSAL, RENDER_TIME_KERNELS, RENDER_REFERENCE_ORDER, RENDER_PERSISTENT_SHADOW = 1, 2, 4, 8
 GEN_CORNELL, GEN_SOUP, GEN_ICOSPHERE, GEN_MEDIUM = 1, 2, 3, 5
+MLP_FEATURES, MLP_INPUTS, MLP_WIDTH, MLP_HIDDEN, MLP_OUTPUTS = 11, 64, 64, 4, 3
+MLP_PARAMS = 4 * 64 * 64 + 3 * 64
 SCHED_SHORTEST_FIRST, SCHED_FIFO = 0, 1
 
 # numpy layouts of the C records (byte-identical)
@@ -102,6 +104,20 @@ class GenSizes(C.Structure):
     _fields_ = [("n_tris", C.c_int32), ("n_materials", C.c_int32)]
 
 
+class CacheOpts(C.Structure):
+    _fields_ = [("train_paths", C.c_int32), ("train_batch", C.c_int32), ("train_steps", C.c_int32),
+                ("terminate_depth", C.c_int32), ("lr", C.c_float), ("reserved", C.c_int32)]
+
+
+class CacheStats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("train_trace_ms", C.c_double), ("train_ms", C.c_double),
+                ("render_ms", C.c_double), ("infer_ms", C.c_double), ("train_samples", C.c_uint64),
+                ("queries", C.c_uint64), ("paths", C.c_uint64), ("loss", C.c_float), ("reserved", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class ServerOpts(C.Structure):
     _fields_ = [("path_capacity", C.c_int32), ("max_jobs", C.c_int32), ("frames_per_job", C.c_int32),
                 ("refill_lanes", C.c_int32), ("film_budget_bytes", C.c_uint64), ("policy", C.c_int32),
@@ -163,6 +179,15 @@ _SIGS = {
     "mcpt_server_release": ([VP, C.c_uint64], C.c_int),
     "mcpt_server_stats_get": ([VP, C.POINTER(ServerStats)], C.c_int),
     "mcpt_server_destroy": ([VP], C.c_int),
+    "mcpt_mlp_create": ([VP, C.c_uint64, C.POINTER(VP)], C.c_int),
+    "mcpt_mlp_destroy": ([VP], C.c_int),
+    "mcpt_mlp_get_params": ([VP, VP], C.c_int),
+    "mcpt_mlp_set_params": ([VP, VP], C.c_int),
+    "mcpt_mlp_forward": ([VP, VP, C.c_int32, VP], C.c_int),
+    "mcpt_mlp_gradient": ([VP, VP, VP, C.c_int32, C.POINTER(C.c_float), VP], C.c_int),
+    "mcpt_mlp_train": ([VP, VP, VP, C.c_int32, C.c_int32, C.c_float, C.POINTER(C.c_float)], C.c_int),
+    "mcpt_render_cached": ([VP, C.POINTER(Camera), C.POINTER(RenderOpts), VP, VP, C.POINTER(CacheOpts),
+                            C.POINTER(CacheStats)], C.c_int),
     "mcpt_gen_scene": ([C.c_int32, C.c_uint64, C.c_int64, VP, VP, VP, C.POINTER(GenSizes)], C.c_int),
     "mcpt_gen_sky": ([C.c_int32, C.c_int32, VP], C.c_int),
     "mcpt_gen_camera": ([C.c_int32, C.c_int32, C.c_int32, C.POINTER(CameraDesc)], C.c_int),
@@ -281,6 +306,9 @@ class Context:
     def server(self, scenes, **opts):
         return Server(self, scenes, **opts)
 
+    def mlp(self, seed=1):
+        return Mlp(self, seed)
+
 
 class Scene:
     def __init__(self, ctx, hot, cold, mats, env=None):
@@ -365,6 +393,15 @@ class Scene:
         o = RenderOpts(mode, spp, sample_begin, max_depth, seed, refill_lanes, flags, batch, 0)
         st = RenderStats()
         _check(_lib.mcpt_render(self.h, C.byref(camera), C.byref(o), film.h, C.byref(st)), "mcpt_render")
+        return st.as_dict()
+
+    def render_cached(self, camera, film, mlp, spp=1, sample_begin=0, max_depth=8, seed=0, refill_lanes=0,
+                      train_paths=0, train_batch=0, train_steps=0, terminate_depth=0, lr=0.0):
+        o = RenderOpts(MODE_PERSISTENT, spp, sample_begin, max_depth, seed, refill_lanes, 0, 0, 0)
+        co = CacheOpts(train_paths, train_batch, train_steps, terminate_depth, lr, 0)
+        st = CacheStats()
+        _check(_lib.mcpt_render_cached(self.h, C.byref(camera), C.byref(o), film.h, mlp.h, C.byref(co),
+                                       C.byref(st)), "mcpt_render_cached")
         return st.as_dict()
 
 
@@ -462,3 +499,57 @@ class Server:
         st = ServerStats()
         _check(_lib.mcpt_server_stats_get(self.h, C.byref(st)), "mcpt_server_stats_get")
         return st.as_dict()
+
+
+class Mlp:
+    """The radiance-cache network (mcpt_mlp_*)."""
+
+    def __init__(self, ctx, seed=1):
+        self.ctx = ctx
+        self.h = VP()
+        ctx._children.add(self)
+        _check(_lib.mcpt_mlp_create(ctx.h, seed, C.byref(self.h)), "mcpt_mlp_create")
+
+    def close(self):
+        if self.h:
+            if self.ctx.h:
+                _lib.mcpt_mlp_destroy(self.h)
+            self.h = VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(MLP_PARAMS, np.float32)
+        _check(_lib.mcpt_mlp_get_params(self.h, _ptr(out)), "mcpt_mlp_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == MLP_PARAMS
+        _check(_lib.mcpt_mlp_set_params(self.h, _ptr(p)), "mcpt_mlp_set_params")
+
+    def forward(self, features):
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, MLP_FEATURES)
+        out = np.zeros((len(f), 3), np.float32)
+        _check(_lib.mcpt_mlp_forward(self.h, _ptr(f), len(f), _ptr(out)), "mcpt_mlp_forward")
+        return out
+
+    def gradient(self, features, targets):
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, MLP_FEATURES)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        _check(_lib.mcpt_mlp_gradient(self.h, _ptr(f), _ptr(t), len(f), C.byref(loss), _ptr(g)), "mcpt_mlp_gradient")
+        return loss.value, g
+
+    def train(self, features, targets, batch=16384, lr=0.005):
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, MLP_FEATURES)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        loss = C.c_float()
+        _check(_lib.mcpt_mlp_train(self.h, _ptr(f), _ptr(t), len(f), batch, lr, C.byref(loss)), "mcpt_mlp_train")
+        return loss.value
+

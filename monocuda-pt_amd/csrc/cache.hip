@@ -1,0 +1,1030 @@
+// cache.hip — the radiance cache: a BF16 MLP on the matrix cores, trained
+// online with Adam on the tracer's own paths, and the wavefront render whose
+// paths end in it. The reference has no learned component; the network and
+// the loss are the ones include/mcpt.h declares.
+//
+// Matrix products use v_mfma_f32_16x16x32_bf16 (gfx950): D(16x16, f32) +=
+// A(16x32, bf16) * B(32x16, bf16), with lane l = r + 16 g (r < 16, g < 4)
+// holding A[r][8g..8g+7], B[8g..8g+7][r] and D[4g..4g+3][r]. Layers are
+// evaluated transposed, X_{l+1}^T = relu(W_l X_l^T): A is a 16-row block of
+// W_l, B a block of 16 samples' inputs, so both operands are rows of
+// row-major [out][in] / [sample][feature] arrays in LDS and no operand needs
+// a transpose. A block of 4 waves runs a tile of 64 samples: wave w owns
+// output rows 16w..16w+15 of every hidden layer.
+//
+// Training (mcpt_mlp_train, and every frame of mcpt_render_cached): one
+// kernel runs forward and backward per tile and keeps the weight gradients
+// of its tiles in registers; each block writes its partial sums and a
+// second kernel adds the blocks in a fixed order, so gradients do not
+// depend on scheduling. Adam then updates FP32 master weights and writes
+// their BF16 image.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "scene.hpp"
+#include "shade.hpp"
+
+namespace mcpt {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTile = 64;                      // samples per block iteration
+constexpr int kIn = MCPT_MLP_INPUTS;           // 64
+constexpr int kW = MCPT_MLP_WIDTH;             // 64
+constexpr int kRow = 72;                       // bf16 row stride in LDS and in the weight image (144 B)
+constexpr int kOutRows = 16;                   // output layer rows padded to one MFMA tile
+constexpr int kImgRows = MCPT_MLP_HIDDEN * kW + kOutRows;
+constexpr int kImg = kImgRows * kRow;          // bf16 elements of the weight image
+constexpr int kParams = MCPT_MLP_PARAMS;
+constexpr int kMaxGradBlocks = 256;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+static_assert(kIn == 64 && kW == 64, "the tiling assumes 64-wide layers");
+
+// row of parameter (layer, out) in the weight image, and the flat index
+MCPT_HDI int img_row(int layer, int o) { return layer * kW + o; }
+MCPT_HDI int param_index(int layer, int o, int in) { return layer * kW * kIn + o * kIn + in; }
+
+struct Query {  // one cache lookup (64 bytes)
+    float f[MCPT_MLP_FEATURES];
+    float thr[3];
+    int path;
+    float pad;
+};
+static_assert(sizeof(Query) == 64, "query record is 64 bytes");
+
+// ------------------------------------------------------------ encoding
+// input f of the 64 (order as in include/mcpt.h and oracle/mlp.py)
+MCPT_DEV float enc_feature(int f, const float* x) {
+    if (f < 36) {
+        const int a = f / 12, r = f % 12, k = r >> 1;
+        const float arg = float(1 << k) * MCPT_PI_F * x[a];
+        return (r & 1) ? mf_cos(arg) : mf_sin(arg);
+    }
+    if (f < 52) {
+        const bool dir = f < 44;
+        const int r = f - (dir ? 36 : 44), c = r >> 2, k = (r >> 1) & 1;
+        const float arg = float(1 << k) * MCPT_PI_F * x[(dir ? 3 : 5) + c];
+        return (r & 1) ? mf_cos(arg) : mf_sin(arg);
+    }
+    if (f == 52) return x[7];
+    if (f < 56) return x[8 + (f - 53)];
+    if (f == 56) return 1.0f;
+    if (f < 60) return x[f - 57];
+    if (f < 62) return x[3 + (f - 60)];
+    return x[5 + (f - 62)];
+}
+
+// unit vector -> (theta/pi, phi/(2 pi) + 1/2)
+MCPT_DEV void sph2(const vec3& d, float* out) {
+    const float z = fminf(fmaxf(d.z, -1.0f), 1.0f);
+    out[0] = mf_acos(z) / MCPT_PI_F;
+    out[1] = mf_atan2(d.y, d.x) / (2.0f * MCPT_PI_F) + 0.5f;
+}
+
+// ------------------------------------------------------------ MLP tiles
+// copy the weight image into LDS (16-byte pieces)
+MCPT_DEV void load_weights(const __bf16* wimg, __bf16* w) {
+    const uint4* src = reinterpret_cast<const uint4*>(wimg);
+    uint4* dst = reinterpret_cast<uint4*>(w);
+    for (int i = threadIdx.x; i < kImg / 8; i += kBlock) dst[i] = src[i];
+}
+
+// the 64 encoded inputs of the tile's samples; thread t writes inputs
+// 16 (t % 4) .. +15 of sample t / 4
+MCPT_DEV void encode_tile(const float* feat, bool valid, __bf16* x) {
+    const int s = threadIdx.x >> 2, q = threadIdx.x & 3;
+    float f[MCPT_MLP_FEATURES];
+    for (int i = 0; i < MCPT_MLP_FEATURES; i++) f[i] = valid ? feat[i] : 0.0f;
+    for (int j = 0; j < 16; j++) {
+        const int k = 16 * q + j;
+        x[s * kRow + k] = (__bf16)(valid ? enc_feature(k, f) : 0.0f);
+    }
+}
+
+// hidden layer: xout = relu(W xin), W rows 16w.. of this wave
+MCPT_DEV void hidden_forward(const __bf16* W, const __bf16* xin, __bf16* xout) {
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    f32x4 acc[4];
+    for (int nt = 0; nt < 4; nt++) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < 2; kc++) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(W + (16 * wave + r) * kRow + 32 * kc + 8 * g);
+        for (int nt = 0; nt < 4; nt++) {
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(xin + (16 * nt + r) * kRow + 32 * kc + 8 * g);
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nt], 0, 0, 0);
+        }
+    }
+    for (int nt = 0; nt < 4; nt++)
+        for (int e = 0; e < 4; e++)
+            xout[(16 * nt + r) * kRow + 16 * wave + 4 * g + e] = (__bf16)fmaxf(acc[nt][e], 0.0f);
+}
+
+// output layer for samples 16w..16w+15: lanes with g == 0 get y[0..2]
+MCPT_DEV f32x4 output_forward(const __bf16* W4, const __bf16* xin) {
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < 2; kc++) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(W4 + r * kRow + 32 * kc + 8 * g);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(xin + (16 * wave + r) * kRow + 32 * kc + 8 * g);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// ------------------------------------------------------------- kernels
+// Inference over queries. yout != nullptr: write y (n*3); otherwise add
+// thr * max(y, 0) to the path's sample (the cached render).
+__global__ __launch_bounds__(kBlock) void k_mlp_infer(const __bf16* wimg, const Query* q, const int* count_ptr,
+                                                     int count, float* yout, float4* smp) {
+    __shared__ __attribute__((aligned(16))) __bf16 w[kImg];
+    __shared__ __attribute__((aligned(16))) __bf16 x[2][kTile * kRow];
+    const int n = count_ptr ? *count_ptr : count;
+    load_weights(wimg, w);
+    const int ntiles = (n + kTile - 1) / kTile;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int s = tile * kTile + (threadIdx.x >> 2);
+        __syncthreads();
+        encode_tile(q[min(s, n - 1)].f, s < n, x[0]);
+        __syncthreads();
+        int c = 0;
+        for (int l = 0; l < MCPT_MLP_HIDDEN; l++) {
+            hidden_forward(w + img_row(l, 0) * kRow, x[c], x[1 - c]);
+            __syncthreads();
+            c = 1 - c;
+        }
+        const f32x4 y = output_forward(w + img_row(MCPT_MLP_HIDDEN, 0) * kRow, x[c]);
+        const int lane = lane_id();
+        const int so = tile * kTile + 16 * (threadIdx.x >> 6) + (lane & 15);
+        if ((lane >> 4) == 0 && so < n) {
+            if (yout) {
+                yout[3 * so] = y[0];
+                yout[3 * so + 1] = y[1];
+                yout[3 * so + 2] = y[2];
+            } else {
+                const Query& qq = q[so];
+                float4 v = smp[qq.path];
+                v.x = v.x + qq.thr[0] * fmaxf(y[0], 0.0f);
+                v.y = v.y + qq.thr[1] * fmaxf(y[1], 0.0f);
+                v.z = v.z + qq.thr[2] * fmaxf(y[2], 0.0f);
+                smp[qq.path] = v;
+            }
+        }
+    }
+}
+
+// Forward + backward over samples [first, first + n) of (feat, targ),
+// tiles block-strided; per-block weight gradients and loss sums.
+__global__ __launch_bounds__(kBlock) void k_mlp_grad(const __bf16* wimg, const float* feat, const float* targ, int first,
+                                                    int n, int n_total_for_mean, float* partial, float* partial_loss) {
+    __shared__ __attribute__((aligned(16))) __bf16 w[kImg];
+    __shared__ __attribute__((aligned(16))) __bf16 x[MCPT_MLP_HIDDEN + 1][kTile * kRow];
+    __shared__ __attribute__((aligned(16))) __bf16 dy[2][kTile * kRow];
+    __shared__ float lsum[kBlock / 64];
+    load_weights(wimg, w);
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const float inv_count = 1.0f / float(3 * n_total_for_mean);
+    f32x4 gh[MCPT_MLP_HIDDEN][4];  // hidden dW: rows 16w+4g+e, columns 16nt+r
+    for (int l = 0; l < MCPT_MLP_HIDDEN; l++)
+        for (int nt = 0; nt < 4; nt++) gh[l][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    f32x4 go = f32x4{0.f, 0.f, 0.f, 0.f};  // output dW: rows 4g+e, columns 16w+r
+    float loss = 0.0f;
+    const __bf16* W4 = w + img_row(MCPT_MLP_HIDDEN, 0) * kRow;
+    const int ntiles = (n + kTile - 1) / kTile;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int s = tile * kTile + (threadIdx.x >> 2);
+        __syncthreads();
+        encode_tile(feat + (size_t)(first + min(s, n - 1)) * MCPT_MLP_FEATURES, s < n, x[0]);
+        __syncthreads();
+        for (int l = 0; l < MCPT_MLP_HIDDEN; l++) {
+            hidden_forward(w + img_row(l, 0) * kRow, x[l], x[l + 1]);
+            __syncthreads();
+        }
+        // output, loss and its gradient dy (columns 0..2 of dy[0], 3..31 zero)
+        const f32x4 y = output_forward(W4, x[MCPT_MLP_HIDDEN]);
+        const int so = tile * kTile + 16 * wave + r;
+        if (g == 0) {
+            for (int e = 0; e < 3; e++) {
+                float d = 0.0f;
+                if (so < n) {
+                    const float t = targ[(size_t)(first + so) * 3 + e];
+                    const float den = y[e] * y[e] + 0.01f;
+                    const float diff = y[e] - t;
+                    loss += diff * diff / den;
+                    d = 2.0f * diff / den * inv_count;
+                }
+                dy[0][(16 * wave + r) * kRow + e] = (__bf16)d;
+            }
+        }
+        for (int j = threadIdx.x; j < kTile * 29; j += kBlock) dy[0][(j / 29) * kRow + 3 + (j % 29)] = (__bf16)0.0f;
+        __syncthreads();
+        // output layer: dW4 (wave w: input columns 16w..) and dX4 -> dy[1]
+        {
+            f32x4 acc = go;
+            for (int kc = 0; kc < 2; kc++) {
+                bf16x8 a, b;
+                for (int e = 0; e < 8; e++) {
+                    a[e] = dy[0][(32 * kc + 8 * g + e) * kRow + r];  // A[o=r][s]; o >= 3 reads zeros
+                    b[e] = x[MCPT_MLP_HIDDEN][(32 * kc + 8 * g + e) * kRow + 16 * wave + r];
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+            }
+            go = acc;
+            f32x4 dx[4];
+            for (int nt = 0; nt < 4; nt++) dx[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+            bf16x8 a;  // A[i=in][k=o] = W4[o][in], o = 8g+e < 16 lives in the image
+            for (int e = 0; e < 8; e++) a[e] = (g < 2) ? W4[(8 * g + e) * kRow + 16 * wave + r] : (__bf16)0.0f;
+            for (int nt = 0; nt < 4; nt++) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8*>(&dy[0][(16 * nt + r) * kRow + 8 * g]);
+                dx[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, dx[nt], 0, 0, 0);
+            }
+            for (int nt = 0; nt < 4; nt++)
+                for (int e = 0; e < 4; e++) {
+                    const int si = 16 * nt + r, ii = 16 * wave + 4 * g + e;
+                    const bool on = (float)x[MCPT_MLP_HIDDEN][si * kRow + ii] > 0.0f;
+                    dy[1][si * kRow + ii] = (__bf16)(on ? dx[nt][e] : 0.0f);
+                }
+        }
+        __syncthreads();
+        // hidden layers, last to first; dy[c] holds the gradient at layer l's output
+        int c = 1;
+        for (int l = MCPT_MLP_HIDDEN - 1; l >= 0; l--) {
+            const __bf16* Wl = w + img_row(l, 0) * kRow;
+            // dW_l[o][in] += sum_s dy[s][o] x_l[s][in]: A[o][s], B[s][in]
+            for (int kc = 0; kc < 2; kc++) {
+                bf16x8 a;
+                for (int e = 0; e < 8; e++) a[e] = dy[c][(32 * kc + 8 * g + e) * kRow + 16 * wave + r];
+                for (int nt = 0; nt < 4; nt++) {
+                    bf16x8 b;
+                    for (int e = 0; e < 8; e++) b[e] = x[l][(32 * kc + 8 * g + e) * kRow + 16 * nt + r];
+                    gh[l][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, gh[l][nt], 0, 0, 0);
+                }
+            }
+            if (l > 0) {
+                // dx_l[in][s] = sum_o W_l[o][in] dy[s][o], masked by x_l > 0
+                f32x4 dx[4];
+                for (int nt = 0; nt < 4; nt++) dx[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+                for (int kc = 0; kc < 2; kc++) {
+                    bf16x8 a;
+                    for (int e = 0; e < 8; e++) a[e] = Wl[(32 * kc + 8 * g + e) * kRow + 16 * wave + r];
+                    for (int nt = 0; nt < 4; nt++) {
+                        const bf16x8 b =
+                            *reinterpret_cast<const bf16x8*>(&dy[c][(16 * nt + r) * kRow + 32 * kc + 8 * g]);
+                        dx[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, dx[nt], 0, 0, 0);
+                    }
+                }
+                for (int nt = 0; nt < 4; nt++)
+                    for (int e = 0; e < 4; e++) {
+                        const int si = 16 * nt + r, ii = 16 * wave + 4 * g + e;
+                        const bool on = (float)x[l][si * kRow + ii] > 0.0f;
+                        dy[1 - c][si * kRow + ii] = (__bf16)(on ? dx[nt][e] : 0.0f);
+                    }
+            }
+            __syncthreads();
+            c = 1 - c;
+        }
+    }
+    // partial sums of this block, in the flat parameter order
+    float* out = partial + (size_t)blockIdx.x * kParams;
+    for (int l = 0; l < MCPT_MLP_HIDDEN; l++)
+        for (int nt = 0; nt < 4; nt++)
+            for (int e = 0; e < 4; e++) out[param_index(l, 16 * wave + 4 * g + e, 16 * nt + r)] = gh[l][nt][e];
+    if (g == 0)
+        for (int e = 0; e < 3; e++) out[param_index(MCPT_MLP_HIDDEN, e, 16 * wave + r)] = go[e];
+    // loss: lanes r of g == 0 hold per-sample sums
+    float ls = loss;
+    for (int off = 32; off > 0; off >>= 1) ls += __shfl_xor(ls, off);
+    if (lane == 0) lsum[wave] = ls;
+    __syncthreads();
+    if (threadIdx.x == 0) partial_loss[blockIdx.x] = (lsum[0] + lsum[1]) + (lsum[2] + lsum[3]);
+}
+
+// grad[p] = sum over blocks in block order; loss likewise
+__global__ __launch_bounds__(kBlock) void k_mlp_reduce(const float* partial, const float* partial_loss, int nblk,
+                                                      float* grad, float* loss_out, float loss_scale) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p < kParams) {
+        float s = 0.0f;
+        for (int b = 0; b < nblk; b++) s += partial[(size_t)b * kParams + p];
+        grad[p] = s;
+    }
+    if (p == 0) {
+        float s = 0.0f;
+        for (int b = 0; b < nblk; b++) s += partial_loss[b];
+        loss_out[0] = s * loss_scale;
+    }
+}
+
+// Adam on the FP32 master weights, then their BF16 image
+__global__ __launch_bounds__(kBlock) void k_adam(float* master, float* m, float* v, const float* grad, __bf16* wimg,
+                                                float lr, float bc1, float bc2) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= kParams) return;
+    const float g = grad[p];
+    const float mm = 0.9f * m[p] + 0.1f * g;
+    const float vv = 0.99f * v[p] + 0.01f * g * g;
+    m[p] = mm;
+    v[p] = vv;
+    const float wv = master[p] - lr * (mm / bc1) / (sqrtf(vv / bc2) + 1e-8f);
+    master[p] = wv;
+    const int layer = p / (kW * kIn), o = (p % (kW * kIn)) / kIn, in = p % kIn;
+    wimg[img_row(layer, o) * kRow + in] = (__bf16)wv;
+}
+
+// BF16 image of the master weights (padding stays zero)
+__global__ __launch_bounds__(kBlock) void k_pack(const float* master, __bf16* wimg) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= kImg) return;
+    const int row = i / kRow, col = i % kRow;
+    const int layer = row / kW, o = row % kW;
+    float val = 0.0f;
+    if (col < kIn && (layer < MCPT_MLP_HIDDEN || o < MCPT_MLP_OUTPUTS)) val = master[param_index(layer, o, col)];
+    wimg[i] = (__bf16)val;
+}
+
+int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
+
+// ----------------------------------------------------- cached wavefront
+// Counters, one 128-byte line each (see render.hip): C_NS0/C_NS1 are u64
+// per ray queue, queue length in the low half and the length of the shadow
+// queue filled in the same bounce in the high half.
+enum { C_NS0 = 0, C_NS1 = 32, C_WORK = 64, C_QCOUNT = 96, C_NCOUNTERS = 128 };
+
+MCPT_DEV unsigned long long* ns_slot(int* c, int q) {
+    return reinterpret_cast<unsigned long long*>(c + (q ? C_NS1 : C_NS0));
+}
+MCPT_DEV int queue_len(const int* c, int q) { return c[q ? C_NS1 : C_NS0]; }
+MCPT_DEV int shadow_len(const int* c, int q) { return c[(q ? C_NS1 : C_NS0) + 1]; }
+
+struct alignas(16) VertexRec {  // a diffuse vertex of a training path (80 bytes)
+    float f[MCPT_MLP_FEATURES];
+    float s[3];  // sample before the vertex
+    float t[3];  // throughput arriving at the vertex
+    float pad[3];
+};
+
+struct CacheWs {
+    RayItem* q[2];
+    mcpt_packed_hit* hits;
+    ShadowItem* sq;
+    float4* thr;
+    float4* smp;
+    uint64_t* rng;
+    int* pix;  // pixel of each path
+    Query* queries;
+    VertexRec* vrec;  // train paths * max_depth
+    int* nv;          // vertices recorded per training path
+    int* offs;        // exclusive prefix of nv (train paths + 1)
+    float* tfeat;     // compacted training batch
+    float* ttarg;
+    int* counters;
+};
+
+struct Frame {  // the per-frame constants of the shade kernels
+    float bmin[3], bext[3];  // scene box for position features
+    int max_depth;
+    int depth;
+    int term_depth;
+};
+
+MCPT_DEV void vertex_features(const Frame& fr, const vec3& p, const vec3& rd, const vec3& n, const vec3& albedo,
+                              float* f) {
+    f[0] = (p.x - fr.bmin[0]) / fr.bext[0];
+    f[1] = (p.y - fr.bmin[1]) / fr.bext[1];
+    f[2] = (p.z - fr.bmin[2]) / fr.bext[2];
+    sph2(-rd, f + 3);
+    sph2(n, f + 5);
+    f[7] = 1.0f;  // diffuse
+    f[8] = albedo.x;
+    f[9] = albedo.y;
+    f[10] = albedo.z;
+}
+
+// image paths: one per pixel (generateCameraRaysKernel, :3342-3374)
+__global__ __launch_bounds__(kBlock) void k_cc_camera(mcpt_camera cam, uint32_t sample, uint64_t seed, CacheWs ws) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    const int W = cam.width, npix = cam.width * cam.height;
+    if (p == 0) {
+        *ns_slot(ws.counters, 0) = (unsigned long long)(uint32_t)npix;
+        ws.counters[C_QCOUNT] = 0;
+    }
+    if (p >= npix) return;
+    Rng rng;
+    mcpt_pcg32_for_path(&rng.g, (uint32_t)p, sample, seed);
+    vec3 o, d;
+    float time;
+    camera_ray(cam, p % W, p / W, rng, o, d, time);
+    RayItem it;
+    it.ot = make_float4(o.x, o.y, o.z, time);
+    it.dp = make_float4(d.x, d.y, d.z, __int_as_float(p));
+    ws.q[0][p] = it;
+    ws.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    ws.smp[p] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    ws.rng[p] = rng.g.state;
+    ws.pix[p] = p;
+}
+
+// training paths from pixels drawn at random; their RNG streams are keyed by
+// sample indices with bit 30 set, disjoint from the image samples
+__global__ __launch_bounds__(kBlock) void k_cc_train_camera(mcpt_camera cam, uint32_t frame, uint64_t seed, int n,
+                                                           CacheWs ws) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int W = cam.width, npix = cam.width * cam.height;
+    if (i == 0) *ns_slot(ws.counters, 0) = (unsigned long long)(uint32_t)n;
+    if (i >= n) return;
+    mcpt_pcg32 pick;
+    mcpt_pcg32_for_path(&pick, (uint32_t)i, frame, seed ^ 0x5851F42D4C957F2DULL);
+    const int p = (int)(mcpt_pcg32_next(&pick) % (uint32_t)npix);
+    Rng rng;
+    mcpt_pcg32_for_path(&rng.g, (uint32_t)p, frame | 0x40000000u, seed);
+    vec3 o, d;
+    float time;
+    camera_ray(cam, p % W, p / W, rng, o, d, time);
+    RayItem it;
+    it.ot = make_float4(o.x, o.y, o.z, time);
+    it.dp = make_float4(d.x, d.y, d.z, __int_as_float(i));
+    ws.q[0][i] = it;
+    ws.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    ws.smp[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    ws.rng[i] = rng.g.state;
+    ws.pix[i] = p;
+    ws.nv[i] = 0;
+}
+
+__global__ void k_cc_bounce(int* c, int cur) {
+    if (threadIdx.x == 0) {
+        *ns_slot(c, 1 - cur) = 0;
+        c[C_WORK] = 0;
+    }
+}
+
+// persistent extend (extendKernelPersistent, :3416-3468)
+__global__ __launch_bounds__(kBlock) void k_cc_extend(SceneView sv, CacheWs ws, int cur, int refill) {
+    extern __shared__ int lds_stack[];
+    int* stk = lds_stack + threadIdx.x;
+    const int total = queue_len(ws.counters, cur);
+    const int lane = lane_id();
+    Trav s;
+    bool has = false;
+    int slot = -1;
+    bool exhausted = false;
+    uint32_t ns = 0, nl = 0;
+    while (true) {
+        if (!exhausted) {
+            const uint64_t idle = ballot64(!has);
+            const int nidle = __popcll(idle);
+            if (nidle >= refill) {
+                const int leader = __builtin_ctzll(idle);
+                int first = 0;
+                if (lane == leader) first = atomicAdd(&ws.counters[C_WORK], nidle);
+                first = __shfl(first, leader);
+                if (!has) {
+                    const int q = first + rank_below(idle);
+                    if (q < total) {
+                        const RayItem it = ws.q[cur][q];
+                        trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+                        slot = q;
+                        has = true;
+                    }
+                }
+                if (first + nidle >= total) exhausted = true;
+            }
+        }
+        if (exhausted && ballot64(has) == 0) break;
+        if (has && step_ordered<false>(sv.bvh, s, stk, kBlock, ns, nl)) {
+            mcpt_packed_hit h;
+            h.t = s.t;
+            h.prim_type = s.leaf >= 0 ? 1 : -1;
+            h.prim_index = s.leaf >= 0 ? sv.bvh.leaf_prim[s.leaf] : -1;
+            h.did_hit = s.leaf >= 0 ? 1 : 0;
+            ws.hits[slot] = h;
+            has = false;
+        }
+    }
+}
+
+// shadeKernel (:3470-3696). TRAIN: record every diffuse vertex (features,
+// sample and throughput on arrival) before shading it. Otherwise a diffuse
+// hit at depth >= term_depth ends the path in a cache query.
+template <bool TRAIN>
+__global__ __launch_bounds__(kBlock) void k_cc_shade(SceneView sv, CacheWs ws, int cur, Frame fr, vec3 bg,
+                                                    float rr_cap) {
+    const int slot = blockIdx.x * kBlock + threadIdx.x;
+    const bool active = slot < queue_len(ws.counters, cur);
+    Bounce out;
+    out.next = false;
+    out.shadow = false;
+    bool query = false;
+    Query qr;
+    int path = 0;
+    float time = 0.f;
+    if (active) {
+        const RayItem it = ws.q[cur][slot];
+        const mcpt_packed_hit hit = ws.hits[slot];
+        path = __float_as_int(it.dp.w);
+        time = it.ot.w;
+        const vec3 ro(it.ot.x, it.ot.y, it.ot.z), rd(it.dp.x, it.dp.y, it.dp.z);
+        const float4 thr = ws.thr[path], smp = ws.smp[path];
+        PathState ps;
+        ps.thr = vec3(thr.x, thr.y, thr.z);
+        ps.prev_pdf = thr.w;
+        ps.smp = vec3(smp.x, smp.y, smp.z);
+        ps.prev_spec = smp.w != 0.0f;
+        bool diffuse = false;
+        HitInfo h;
+        vec3 albedo;
+        if (hit.did_hit) {
+            const mcpt_material& m = sv.mats[sv.cold[hit.prim_index].material];
+            diffuse = m.type == MCPT_MAT_LAMBERTIAN;
+            if (diffuse) {
+                rebuild_hit(sv, hit.prim_index, ro, rd, hit.t, h);
+                albedo = load3(m.albedo);
+            }
+        }
+        if (TRAIN && diffuse) {
+            const int k = ws.nv[path];
+            VertexRec& v = ws.vrec[(size_t)path * fr.max_depth + k];
+            vertex_features(fr, h.p, rd, h.normal, albedo, v.f);
+            v.s[0] = ps.smp.x;
+            v.s[1] = ps.smp.y;
+            v.s[2] = ps.smp.z;
+            v.t[0] = ps.thr.x;
+            v.t[1] = ps.thr.y;
+            v.t[2] = ps.thr.z;
+            ws.nv[path] = k + 1;
+        }
+        if (!TRAIN && diffuse && fr.depth >= fr.term_depth) {
+            query = true;
+            vertex_features(fr, h.p, rd, h.normal, albedo, qr.f);
+            qr.thr[0] = ps.thr.x;
+            qr.thr[1] = ps.thr.y;
+            qr.thr[2] = ps.thr.z;
+            qr.path = path;
+            qr.pad = 0.0f;
+        } else {
+            Rng rng;
+            rng.g.state = ws.rng[path];
+            rng.g.inc = ((uint64_t)(uint32_t)ws.pix[path] << 1u) | 1u;
+            shade_bounce(sv, bg, rr_cap, fr.depth, ro, rd, hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out);
+            ws.thr[path] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
+            ws.smp[path] = make_float4(ps.smp.x, ps.smp.y, ps.smp.z, ps.prev_spec ? 1.0f : 0.0f);
+            ws.rng[path] = rng.g.state;
+        }
+    }
+    int nslot, sslot;
+    wave_push2(ns_slot(ws.counters, 1 - cur), out.next, out.shadow, nslot, sslot);
+    if (out.next) {
+        RayItem it;
+        it.ot = make_float4(out.no.x, out.no.y, out.no.z, time);
+        it.dp = make_float4(out.nd.x, out.nd.y, out.nd.z, __int_as_float(path));
+        ws.q[1 - cur][nslot] = it;
+    }
+    if (out.shadow) {
+        ShadowItem si;
+        si.om = make_float4(out.so.x, out.so.y, out.so.z, out.smax);
+        si.dp = make_float4(out.sd.x, out.sd.y, out.sd.z, __int_as_float(path));
+        si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, 0.0f);
+        ws.sq[sslot] = si;
+    }
+    if (!TRAIN) {
+        const int qslot = wave_push(&ws.counters[C_QCOUNT], query);
+        if (query) ws.queries[qslot] = qr;
+    }
+}
+
+// shadowKernel (:3698-3728)
+__global__ __launch_bounds__(kBlock) void k_cc_shadow(SceneView sv, CacheWs ws, int cur) {
+    extern __shared__ int lds_stack[];
+    const int slot = blockIdx.x * kBlock + threadIdx.x;
+    if (slot >= shadow_len(ws.counters, 1 - cur)) return;
+    const ShadowItem si = ws.sq[slot];
+    const int path = __float_as_int(si.dp.w);
+    const float tr = shadow_transmittance(sv, vec3(si.om.x, si.om.y, si.om.z), vec3(si.dp.x, si.dp.y, si.dp.z),
+                                          si.om.w, lds_stack + threadIdx.x, kBlock);
+    if (tr > 0.0f) {
+        float4 v = ws.smp[path];
+        v.x = v.x + si.c.x * tr;
+        v.y = v.y + si.c.y * tr;
+        v.z = v.z + si.c.z * tr;
+        ws.smp[path] = v;
+    }
+}
+
+// exclusive prefix of nv into offs (one block; offs[n] = total)
+__global__ __launch_bounds__(1024) void k_cc_scan(const int* nv, int n, int* offs) {
+    __shared__ int part[1024];
+    const int per = (n + 1023) / 1024;
+    const int b = threadIdx.x * per, e = min(n, b + per);
+    int s = 0;
+    for (int i = b; i < e; i++) s += nv[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+        const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = part[threadIdx.x] - s;
+    for (int i = b; i < e; i++) {
+        offs[i] = run;
+        run += nv[i];
+    }
+    if (threadIdx.x == 1023) offs[n] = part[1023];
+}
+
+// training batch: features and target (radiance gathered after the vertex
+// over the throughput that reached it) of every recorded vertex, in path
+// order
+__global__ __launch_bounds__(kBlock) void k_cc_targets(CacheWs ws, int n, int max_depth) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    const float4 fin = ws.smp[p];
+    const int k = ws.nv[p], base = ws.offs[p];
+    for (int v = 0; v < k; v++) {
+        const VertexRec r = ws.vrec[(size_t)p * max_depth + v];
+        float* f = ws.tfeat + (size_t)(base + v) * MCPT_MLP_FEATURES;
+        for (int i = 0; i < MCPT_MLP_FEATURES; i++) f[i] = r.f[i];
+        const float fs[3] = {fin.x, fin.y, fin.z};
+        float* t = ws.ttarg + (size_t)(base + v) * 3;
+        for (int c = 0; c < 3; c++) t[c] = r.t[c] > 0.0f ? (fs[c] - r.s[c]) / r.t[c] : 0.0f;
+    }
+}
+
+// finalizeSampleKernel (:3740-3755): one sample per pixel this frame
+__global__ __launch_bounds__(kBlock) void k_cc_finalize(const float4* smp, float* accum, int npix) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= npix) return;
+    const float4 s = smp[p];
+    const vec3 c = clamp_sample(vec3(s.x, s.y, s.z));
+    accum[3 * p] = accum[3 * p] + c.x;
+    accum[3 * p + 1] = accum[3 * p + 1] + c.y;
+    accum[3 * p + 2] = accum[3 * p + 2] + c.z;
+}
+
+}  // namespace
+}  // namespace mcpt
+
+using namespace mcpt;
+
+struct mcpt_mlp_s {
+    Ctx* ctx = nullptr;
+    DevBuf<float> master, m, v, grad, partial, partial_loss, loss;
+    DevBuf<__bf16> wimg;
+    DevBuf<float> io;  // staging for the host-array entry points
+    int step = 0;
+    // cached-render workspace, grown on demand
+    void* ws_mem = nullptr;
+    size_t ws_bytes = 0;
+    int64_t ws_paths = 0, ws_train = 0;
+    int ws_depth = 0;
+};
+
+namespace mcpt {
+namespace {
+
+// He-uniform initial weights: w = (2u - 1) sqrt(6 / fan_in), u from the
+// PCG32 stream keyed by (layer, 0, seed)
+void init_params(uint64_t seed, std::vector<float>& p) {
+    p.assign(kParams, 0.0f);
+    int k = 0;
+    for (int l = 0; l <= MCPT_MLP_HIDDEN; l++) {
+        const int rows = l < MCPT_MLP_HIDDEN ? kW : MCPT_MLP_OUTPUTS, cols = l == 0 ? kIn : kW;
+        mcpt_pcg32 rng;
+        mcpt_pcg32_for_path(&rng, (uint32_t)l, 0u, seed);
+        const double scale = std::sqrt(6.0 / cols);
+        for (int i = 0; i < rows * cols; i++) {
+            const float u = (float)((double)mcpt_pcg32_next(&rng) * 0x1p-32);
+            p[k++] = (float)((2.0 * (double)u - 1.0) * scale);
+        }
+    }
+}
+
+// gradient of the mean loss over samples [first, first + n) of device
+// arrays; result in mlp->grad and mlp->loss
+int mlp_gradient_dev(mcpt_mlp_s* mlp, const float* feat, const float* targ, int first, int n, hipStream_t st) {
+    const int ntiles = (n + kTile - 1) / kTile;
+    const int nblk = std::max(1, std::min(ntiles, kMaxGradBlocks));
+    k_mlp_grad<<<nblk, kBlock, 0, st>>>(mlp->wimg.ptr, feat, targ, first, n, n, mlp->partial.ptr,
+                                        mlp->partial_loss.ptr);
+    k_mlp_reduce<<<blocks_for(kParams), kBlock, 0, st>>>(mlp->partial.ptr, mlp->partial_loss.ptr, nblk,
+                                                         mlp->grad.ptr, mlp->loss.ptr, 1.0f / float(3 * n));
+    MCPT_HIP(hipGetLastError());
+    return MCPT_OK;
+}
+
+int mlp_adam(mcpt_mlp_s* mlp, float lr, hipStream_t st) {
+    mlp->step++;
+    const float bc1 = (float)(1.0 - std::pow(0.9, mlp->step)), bc2 = (float)(1.0 - std::pow(0.99, mlp->step));
+    k_adam<<<blocks_for(kParams), kBlock, 0, st>>>(mlp->master.ptr, mlp->m.ptr, mlp->v.ptr, mlp->grad.ptr,
+                                                   mlp->wimg.ptr, lr, bc1, bc2);
+    MCPT_HIP(hipGetLastError());
+    return MCPT_OK;
+}
+
+int ensure_io(mcpt_mlp_s* mlp, size_t floats) {
+    if (mlp->io.count >= floats) return MCPT_OK;
+    return mlp->io.alloc(mlp->ctx, floats);
+}
+
+// carve the cached-render workspace for `paths` image paths, `train`
+// training paths of up to `depth` bounces
+int ensure_cache_ws(mcpt_mlp_s* mlp, int64_t paths, int64_t train, int depth, CacheWs& ws) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const int64_t cap = std::max(paths, train);
+    const int64_t nrec = train * depth;
+    const size_t parts[] = {al(cap * sizeof(RayItem)),      al(cap * sizeof(RayItem)),   al(cap * sizeof(mcpt_packed_hit)),
+                            al(cap * sizeof(ShadowItem)),   al(cap * sizeof(float4)),    al(cap * sizeof(float4)),
+                            al(cap * sizeof(uint64_t)),     al(cap * sizeof(int)),       al(paths * sizeof(Query)),
+                            al(nrec * sizeof(VertexRec)),   al(train * sizeof(int)),     al((train + 1) * sizeof(int)),
+                            al(nrec * MCPT_MLP_FEATURES * sizeof(float)), al(nrec * 3 * sizeof(float)),
+                            al(C_NCOUNTERS * sizeof(int))};
+    size_t total = 0;
+    for (size_t x : parts) total += x;
+    if (mlp->ws_paths < paths || mlp->ws_train < train || mlp->ws_depth < depth) {
+        if (mlp->ws_mem) dev_free(mlp->ctx, mlp->ws_mem, mlp->ws_bytes);
+        mlp->ws_mem = nullptr;
+        mlp->ws_bytes = 0;
+        mlp->ws_paths = mlp->ws_train = 0;
+        mlp->ws_depth = 0;
+        MCPT_TRY(dev_alloc(mlp->ctx, &mlp->ws_mem, total));
+        mlp->ws_bytes = total;
+        mlp->ws_paths = paths;
+        mlp->ws_train = train;
+        mlp->ws_depth = depth;
+    }
+    // carve with the allocated capacities (the layout only depends on them)
+    return MCPT_OK;
+}
+
+void carve_cache_ws(mcpt_mlp_s* mlp, CacheWs& ws) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const int64_t paths = mlp->ws_paths, train = mlp->ws_train;
+    const int64_t cap = std::max(paths, train), nrec = train * mlp->ws_depth;
+    char* b = (char*)mlp->ws_mem;
+    ws.q[0] = (RayItem*)b, b += al(cap * sizeof(RayItem));
+    ws.q[1] = (RayItem*)b, b += al(cap * sizeof(RayItem));
+    ws.hits = (mcpt_packed_hit*)b, b += al(cap * sizeof(mcpt_packed_hit));
+    ws.sq = (ShadowItem*)b, b += al(cap * sizeof(ShadowItem));
+    ws.thr = (float4*)b, b += al(cap * sizeof(float4));
+    ws.smp = (float4*)b, b += al(cap * sizeof(float4));
+    ws.rng = (uint64_t*)b, b += al(cap * sizeof(uint64_t));
+    ws.pix = (int*)b, b += al(cap * sizeof(int));
+    ws.queries = (Query*)b, b += al(paths * sizeof(Query));
+    ws.vrec = (VertexRec*)b, b += al(nrec * sizeof(VertexRec));
+    ws.nv = (int*)b, b += al(train * sizeof(int));
+    ws.offs = (int*)b, b += al((train + 1) * sizeof(int));
+    ws.tfeat = (float*)b, b += al(nrec * MCPT_MLP_FEATURES * sizeof(float));
+    ws.ttarg = (float*)b, b += al(nrec * 3 * sizeof(float));
+    ws.counters = (int*)b;
+}
+
+}  // namespace
+}  // namespace mcpt
+
+extern "C" {
+
+int mcpt_mlp_create(mcpt_ctx ctx, uint64_t seed, mcpt_mlp* out) {
+    if (!ctx || !out) return set_error(MCPT_ERR_INVALID, "bad arguments");
+    *out = nullptr;
+    Ctx* c = &ctx->c;
+    MCPT_HIP(hipSetDevice(c->device));
+    auto* mlp = new mcpt_mlp_s();
+    mlp->ctx = c;
+    int s = MCPT_OK;
+    if (s >= 0) s = mlp->master.alloc(c, kParams);
+    if (s >= 0) s = mlp->m.alloc(c, kParams);
+    if (s >= 0) s = mlp->v.alloc(c, kParams);
+    if (s >= 0) s = mlp->grad.alloc(c, kParams);
+    if (s >= 0) s = mlp->partial.alloc(c, (size_t)kMaxGradBlocks * kParams);
+    if (s >= 0) s = mlp->partial_loss.alloc(c, kMaxGradBlocks);
+    if (s >= 0) s = mlp->loss.alloc(c, 4);
+    if (s >= 0) s = mlp->wimg.alloc(c, kImg);
+    if (s < 0) {
+        mcpt_mlp_destroy(mlp);
+        return s;
+    }
+    std::vector<float> p;
+    init_params(seed, p);
+    *out = mlp;
+    return mcpt_mlp_set_params(mlp, p.data());
+}
+
+int mcpt_mlp_destroy(mcpt_mlp mlp) {
+    if (!mlp) return MCPT_OK;
+    Ctx* c = mlp->ctx;
+    for (DevBuf<float>* b : {&mlp->master, &mlp->m, &mlp->v, &mlp->grad, &mlp->partial, &mlp->partial_loss,
+                             &mlp->loss, &mlp->io})
+        b->release(c);
+    mlp->wimg.release(c);
+    if (mlp->ws_mem) dev_free(c, mlp->ws_mem, mlp->ws_bytes);
+    delete mlp;
+    return MCPT_OK;
+}
+
+int mcpt_mlp_get_params(mcpt_mlp mlp, float* params) {
+    if (!mlp || !params) return set_error(MCPT_ERR_INVALID, "bad arguments");
+    MCPT_HIP(hipStreamSynchronize(mlp->ctx->stream));
+    MCPT_HIP(hipMemcpy(params, mlp->master.ptr, kParams * sizeof(float), hipMemcpyDeviceToHost));
+    return MCPT_OK;
+}
+
+int mcpt_mlp_set_params(mcpt_mlp mlp, const float* params) {
+    if (!mlp || !params) return set_error(MCPT_ERR_INVALID, "bad arguments");
+    hipStream_t st = mlp->ctx->stream;
+    MCPT_HIP(hipMemcpyAsync(mlp->master.ptr, params, kParams * sizeof(float), hipMemcpyHostToDevice, st));
+    MCPT_HIP(hipMemsetAsync(mlp->m.ptr, 0, kParams * sizeof(float), st));
+    MCPT_HIP(hipMemsetAsync(mlp->v.ptr, 0, kParams * sizeof(float), st));
+    k_pack<<<blocks_for(kImg), kBlock, 0, st>>>(mlp->master.ptr, mlp->wimg.ptr);
+    MCPT_HIP(hipGetLastError());
+    MCPT_HIP(hipStreamSynchronize(st));
+    mlp->step = 0;
+    return MCPT_OK;
+}
+
+int mcpt_mlp_forward(mcpt_mlp mlp, const float* features, int32_t n, float* rgb) {
+    if (!mlp || n < 0 || (n > 0 && (!features || !rgb))) return set_error(MCPT_ERR_INVALID, "bad arguments");
+    if (n == 0) return MCPT_OK;
+    hipStream_t st = mlp->ctx->stream;
+    // queries (16 floats each) then outputs (3 each)
+    MCPT_TRY(ensure_io(mlp, (size_t)n * 19));
+    std::vector<Query> q(n);
+    for (int i = 0; i < n; i++) {
+        std::memset(&q[i], 0, sizeof(Query));
+        std::memcpy(q[i].f, features + (size_t)i * MCPT_MLP_FEATURES, sizeof(q[i].f));
+    }
+    Query* dq = reinterpret_cast<Query*>(mlp->io.ptr);
+    float* dy = mlp->io.ptr + (size_t)n * 16;
+    MCPT_HIP(hipMemcpyAsync(dq, q.data(), (size_t)n * sizeof(Query), hipMemcpyHostToDevice, st));
+    const int ntiles = (n + kTile - 1) / kTile;
+    k_mlp_infer<<<std::min(ntiles, 4096), kBlock, 0, st>>>(mlp->wimg.ptr, dq, nullptr, n, dy, nullptr);
+    MCPT_HIP(hipGetLastError());
+    MCPT_HIP(hipMemcpyAsync(rgb, dy, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost, st));
+    MCPT_HIP(hipStreamSynchronize(st));
+    return MCPT_OK;
+}
+
+int mcpt_mlp_gradient(mcpt_mlp mlp, const float* features, const float* targets, int32_t n, float* loss,
+                      float* grad) {
+    if (!mlp || n < 1 || !features || !targets) return set_error(MCPT_ERR_INVALID, "bad arguments");
+    hipStream_t st = mlp->ctx->stream;
+    MCPT_TRY(ensure_io(mlp, (size_t)n * 14));
+    float* df = mlp->io.ptr;
+    float* dt = df + (size_t)n * MCPT_MLP_FEATURES;
+    MCPT_HIP(hipMemcpyAsync(df, features, (size_t)n * MCPT_MLP_FEATURES * sizeof(float), hipMemcpyHostToDevice, st));
+    MCPT_HIP(hipMemcpyAsync(dt, targets, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, st));
+    MCPT_TRY(mlp_gradient_dev(mlp, df, dt, 0, n, st));
+    if (grad) MCPT_HIP(hipMemcpyAsync(grad, mlp->grad.ptr, kParams * sizeof(float), hipMemcpyDeviceToHost, st));
+    float l = 0.0f;
+    MCPT_HIP(hipMemcpyAsync(&l, mlp->loss.ptr, sizeof(float), hipMemcpyDeviceToHost, st));
+    MCPT_HIP(hipStreamSynchronize(st));
+    if (loss) *loss = l;
+    return MCPT_OK;
+}
+
+int mcpt_mlp_train(mcpt_mlp mlp, const float* features, const float* targets, int32_t n, int32_t batch, float lr,
+                   float* loss) {
+    if (!mlp || n < 1 || batch < 1 || !features || !targets || !(lr > 0.0f))
+        return set_error(MCPT_ERR_INVALID, "bad arguments");
+    hipStream_t st = mlp->ctx->stream;
+    MCPT_TRY(ensure_io(mlp, (size_t)n * 14));
+    float* df = mlp->io.ptr;
+    float* dt = df + (size_t)n * MCPT_MLP_FEATURES;
+    MCPT_HIP(hipMemcpyAsync(df, features, (size_t)n * MCPT_MLP_FEATURES * sizeof(float), hipMemcpyHostToDevice, st));
+    MCPT_HIP(hipMemcpyAsync(dt, targets, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, st));
+    for (int first = 0; first < n; first += batch) {
+        const int cnt = std::min(batch, n - first);
+        MCPT_TRY(mlp_gradient_dev(mlp, df, dt, first, cnt, st));
+        MCPT_TRY(mlp_adam(mlp, lr, st));
+    }
+    float l = 0.0f;
+    MCPT_HIP(hipMemcpyAsync(&l, mlp->loss.ptr, sizeof(float), hipMemcpyDeviceToHost, st));
+    MCPT_HIP(hipStreamSynchronize(st));
+    if (loss) *loss = l;
+    return MCPT_OK;
+}
+
+int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* cam, const mcpt_render_opts* o, mcpt_film film,
+                       mcpt_mlp mlp, const mcpt_cache_opts* co, mcpt_cache_stats* stats) {
+    if (!scene || !cam || !o || !film || !mlp) return set_error(MCPT_ERR_INVALID, "bad arguments");
+    if (!scene->built) return set_error(MCPT_ERR_INVALID, "scene has no BVH (call mcpt_lbvh_build)");
+    if (film->w != cam->width || film->h != cam->height)
+        return set_error(MCPT_ERR_INVALID, "film %dx%d does not match camera %dx%d", film->w, film->h, cam->width,
+                         cam->height);
+    if (o->mode == MCPT_MODE_MEGAKERNEL) return set_error(MCPT_ERR_INVALID, "the cached render is wavefront only");
+    if (o->spp < 1 || o->max_depth < 1 || o->sample_begin < 0 || o->sample_begin + o->spp > 0x40000000)
+        return set_error(MCPT_ERR_INVALID, "spp, max_depth or sample_begin out of range");
+    if (scene->ctx != mlp->ctx) return set_error(MCPT_ERR_INVALID, "scene and MLP belong to different contexts");
+    const int train_paths = (co && co->train_paths > 0) ? co->train_paths : 65536;
+    const int batch = (co && co->train_batch > 0) ? co->train_batch : 16384;
+    const int max_steps = (co && co->train_steps > 0) ? co->train_steps : 4;
+    const int term_depth = (co && co->terminate_depth > 0) ? co->terminate_depth : 1;
+    const float lr = (co && co->lr > 0.0f) ? co->lr : 0.005f;
+    if (train_paths > (1 << 24) || batch > (1 << 24))
+        return set_error(MCPT_ERR_INVALID, "train_paths and train_batch must be <= 16 Mi");
+    Ctx* ctx = scene->ctx;
+    hipStream_t st = ctx->stream;
+    const int npix = cam->width * cam->height, depth = o->max_depth;
+    CacheWs ws;
+    MCPT_TRY(ensure_cache_ws(mlp, npix, train_paths, depth, ws));
+    carve_cache_ws(mlp, ws);
+    const SceneView sv = scene->view();
+    const size_t lds = (size_t)kBlock * std::max(1, scene->max_depth) * sizeof(int);
+    const int refill = o->refill_lanes > 0 ? std::min(64, o->refill_lanes) : 32;
+    int per_cu = 0;
+    MCPT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cc_extend, kBlock, lds));
+    const int persistent_blocks = std::max(1, per_cu) * ctx->cu_count;
+    Frame fr;
+    {
+        float b[6];
+        MCPT_HIP(hipMemcpy(b, scene->root_box.ptr, sizeof(b), hipMemcpyDeviceToHost));
+        for (int k = 0; k < 3; k++) {
+            fr.bmin[k] = b[k];
+            fr.bext[k] = b[3 + k] - b[k] > 0.0f ? b[3 + k] - b[k] : 1.0f;
+        }
+    }
+    fr.max_depth = depth;
+    fr.term_depth = term_depth;
+    const vec3 bg = load3(cam->background);
+    mcpt_cache_stats acc;
+    std::memset(&acc, 0, sizeof(acc));
+    hipEvent_t ev[6];
+    for (auto& e : ev) MCPT_HIP(hipEventCreate(&e));
+    auto elapsed = [&](hipEvent_t a, hipEvent_t b) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        return (double)ms;
+    };
+    // one bounce loop over the current queue contents
+    auto trace = [&](bool train, int n) {
+        int cur = 0;
+        const int grid = blocks_for(n);
+        for (int d = 0; d < depth; d++) {
+            fr.depth = d;
+            k_cc_bounce<<<1, 64, 0, st>>>(ws.counters, cur);
+            k_cc_extend<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(sv, ws, cur, refill);
+            if (train)
+                k_cc_shade<true><<<grid, kBlock, 0, st>>>(sv, ws, cur, fr, bg, cam->rr_cap);
+            else
+                k_cc_shade<false><<<grid, kBlock, 0, st>>>(sv, ws, cur, fr, bg, cam->rr_cap);
+            k_cc_shadow<<<grid, kBlock, lds, st>>>(sv, ws, cur);
+            cur = 1 - cur;
+        }
+    };
+    for (int f = 0; f < o->spp; f++) {
+        const uint32_t sample = (uint32_t)(o->sample_begin + f);
+        // -- training paths, their samples, Adam steps
+        MCPT_HIP(hipEventRecord(ev[0], st));
+        k_cc_train_camera<<<blocks_for(train_paths), kBlock, 0, st>>>(*cam, sample, o->seed, train_paths, ws);
+        trace(true, train_paths);
+        k_cc_scan<<<1, 1024, 0, st>>>(ws.nv, train_paths, ws.offs);
+        k_cc_targets<<<blocks_for(train_paths), kBlock, 0, st>>>(ws, train_paths, depth);
+        MCPT_HIP(hipGetLastError());
+        int n_train = 0;
+        MCPT_HIP(hipMemcpyAsync(&n_train, ws.offs + train_paths, sizeof(int), hipMemcpyDeviceToHost, st));
+        MCPT_HIP(hipEventRecord(ev[1], st));
+        MCPT_HIP(hipEventSynchronize(ev[1]));
+        const int steps = std::min(max_steps, (n_train + batch - 1) / batch);
+        for (int k = 0; k < steps; k++) {
+            const int first = k * batch, cnt = std::min(batch, n_train - first);
+            MCPT_TRY(mlp_gradient_dev(mlp, ws.tfeat, ws.ttarg, first, cnt, st));
+            MCPT_TRY(mlp_adam(mlp, lr, st));
+        }
+        MCPT_HIP(hipEventRecord(ev[2], st));
+        // -- image paths ending in the cache
+        k_cc_camera<<<blocks_for(npix), kBlock, 0, st>>>(*cam, sample, o->seed, ws);
+        trace(false, npix);
+        MCPT_HIP(hipEventRecord(ev[3], st));
+        k_mlp_infer<<<std::max(1, std::min((npix + kTile - 1) / kTile, 2 * ctx->cu_count)), kBlock, 0, st>>>(
+            mlp->wimg.ptr, ws.queries, ws.counters + C_QCOUNT, 0, nullptr, ws.smp);
+        MCPT_HIP(hipEventRecord(ev[4], st));
+        k_cc_finalize<<<blocks_for(npix), kBlock, 0, st>>>(ws.smp, film->accum.ptr, npix);
+        MCPT_HIP(hipGetLastError());
+        MCPT_HIP(hipEventRecord(ev[5], st));
+        MCPT_HIP(hipEventSynchronize(ev[5]));
+        int nq = 0;
+        MCPT_HIP(hipMemcpy(&nq, ws.counters + C_QCOUNT, sizeof(int), hipMemcpyDeviceToHost));
+        float l = 0.f;
+        if (steps) MCPT_HIP(hipMemcpy(&l, mlp->loss.ptr, sizeof(float), hipMemcpyDeviceToHost));
+        acc.train_trace_ms += elapsed(ev[0], ev[1]);
+        acc.train_ms += elapsed(ev[1], ev[2]);
+        acc.render_ms += elapsed(ev[2], ev[3]) + elapsed(ev[4], ev[5]);
+        acc.infer_ms += elapsed(ev[3], ev[4]);
+        acc.total_ms += elapsed(ev[0], ev[5]);
+        acc.train_samples += (uint64_t)std::min(n_train, steps * batch);
+        acc.queries += (uint64_t)nq;
+        acc.paths += (uint64_t)npix;
+        if (steps) acc.loss = l;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (stats) *stats = acc;
+    return MCPT_OK;
+}
+
+}  // extern "C"
