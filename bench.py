@@ -276,6 +276,131 @@ def serving_bench(mcpt, ctx, args):
     return out
 
 
+# ---------------------------------------------------------- radiance cache
+MLP_FLOPS_FWD = 2 * (4 * 64 * 64 + 3 * 64)  # multiply-adds x 2 per query
+MLP_FLOPS_TRAIN = 3 * MLP_FLOPS_FWD         # forward, input gradient, weight gradient
+MFMA_BF16_PEAK = 2.5e15
+
+
+def relmse(img, ref, eps=0.01):
+    import numpy as np
+    d = img.astype(np.float64) - ref
+    return float(np.mean(d * d / (ref * ref + eps)))
+
+
+def cache_bench(mcpt, ctx, args):
+    """Time to relMSE 0.01 against an FP64-accumulated high-spp plain render,
+    plain tracer vs the radiance-cached tracer (training included)."""
+    import numpy as np
+    W, H, depth = args.cache_width, args.cache_height, args.cache_depth
+    t0 = time.time()
+    hot, cold, mats = mcpt.gen_scene(mcpt.GEN_MEDIUM, seed=args.seed, param=args.cache_tris)
+    sky = mcpt.gen_sky(512, 256)
+    scene = ctx.scene(hot, cold, mats, env=mcpt.make_env(sky))
+    scene.build()
+    cam = mcpt.camera_init(mcpt.gen_camera(mcpt.GEN_SOUP, W, H))
+    log(f"cache: {len(hot)} triangles + medium box, {W}x{H}, depth {depth}, ready in {time.time() - t0:.1f} s")
+    # reference: chunks of FP32 sums added in float64
+    film = ctx.film(W, H)
+    ref = np.zeros((H, W, 3), np.float64)
+    chunk, done, ref_ms = 16, 0, 0.0
+    while done < args.cache_ref_spp:
+        film.clear()
+        st = scene.render(film=film, camera=cam, spp=chunk, sample_begin=done, max_depth=depth, seed=args.seed + 1)
+        ref += film.read()
+        ref_ms += st["total_ms"]
+        done += chunk
+    ref /= done
+    log(f"cache: reference {done} spp in {ref_ms / 1e3:.1f} s")
+
+    checkpoints = [1 << k for k in range(0, 30) if (1 << k) <= args.cache_max_spp]
+
+    def converge(render_one):
+        film.clear()
+        curve, t_ms, spp = [], 0.0, 0
+        for cp in checkpoints:
+            while spp < cp:
+                t_ms += render_one(spp)
+                spp += 1
+            e = relmse(film.read() / spp, ref)
+            curve.append({"spp": spp, "ms": t_ms, "relmse": e})
+            log(f"cache:   spp {spp:5d}  {t_ms:9.1f} ms  relMSE {e:.5f}")
+        return curve
+
+    def time_to(curve, target):
+        prev = None
+        for c in curve:
+            if c["relmse"] <= target:
+                if prev is None:
+                    return c["ms"]
+                # relMSE ~ 1/time between checkpoints: interpolate in log-log
+                import math
+                a, b = prev, c
+                f = (math.log(a["relmse"]) - math.log(target)) / (math.log(a["relmse"]) - math.log(b["relmse"]))
+                return math.exp(math.log(a["ms"]) + f * (math.log(b["ms"]) - math.log(a["ms"])))
+            prev = c
+        return None
+
+    log("cache: plain")
+    plain = [] if args.cache_only else converge(
+        lambda k: scene.render(cam, film, spp=1, sample_begin=k, max_depth=depth, seed=args.seed)["total_ms"])
+    net = ctx.mlp(seed=1)
+    agg = {"train_ms": 0.0, "infer_ms": 0.0, "train_samples": 0, "queries": 0, "train_trace_ms": 0.0,
+           "render_ms": 0.0}
+
+    def cached_one(k):
+        st = scene.render_cached(cam, film, net, spp=1, sample_begin=k, max_depth=depth, seed=args.seed,
+                                 train_paths=args.cache_train_paths, terminate_depth=args.cache_terminate,
+                                 lr=args.cache_lr, train_steps=args.cache_steps, ema_decay=args.cache_ema)
+        for key in agg:
+            agg[key] += st[key]
+        agg["loss"] = st["loss"]
+        return st["total_ms"]
+
+    log("cache: cached")
+    warm_ms = 0.0
+    if args.cache_warmup:
+        scratch = ctx.film(W, H)
+        st = scene.render_cached(cam, scratch, net, spp=args.cache_warmup, sample_begin=1 << 20, max_depth=depth,
+                                 seed=args.seed, train_paths=args.cache_train_paths,
+                                 terminate_depth=args.cache_terminate, lr=args.cache_lr,
+                                 train_steps=args.cache_steps, ema_decay=args.cache_ema)
+        warm_ms = st["total_ms"]
+        scratch.close()
+        log(f"cache: warm-up {args.cache_warmup} frames {warm_ms:.1f} ms")
+    cached = converge(cached_one)
+    for c in cached:
+        c["ms"] += warm_ms
+    target = 0.01
+    tp, tc = time_to(plain, target), time_to(cached, target)
+    infer_tflops = agg["queries"] * MLP_FLOPS_FWD / (agg["infer_ms"] / 1e3) / 1e12 if agg["infer_ms"] else None
+    train_tflops = agg["train_samples"] * MLP_FLOPS_TRAIN / (agg["train_ms"] / 1e3) / 1e12 if agg["train_ms"] else None
+    out = {
+        "workload": f"{len(hot)} tris incoherent mesh + medium box (sigma_t 0.5, albedo 0.8, g 0.3) + 512x256 sky, "
+                    f"{W}x{H}, depth {depth}; reference {done} spp FP64-accumulated",
+        "relmse_eps": 0.01,
+        "time_to_relmse_ms": {"plain": tp, "cached": tc, "target": target},
+        "speedup": (tp / tc) if (tp and tc) else None,
+        "relmse_at_16spp": {"plain": next((c["relmse"] for c in plain if c["spp"] == 16), None),
+                            "cached": next((c["relmse"] for c in cached if c["spp"] == 16), None)},
+        "curves": {"plain": plain, "cached": cached},
+        "mlp": {"train_samples": agg["train_samples"], "queries": agg["queries"], "train_ms": agg["train_ms"],
+                "infer_ms": agg["infer_ms"], "train_trace_ms": agg["train_trace_ms"], "render_ms": agg["render_ms"],
+                "last_loss": agg.get("loss"),
+                "mfma": {"bound": "mfma", "unit": "TFLOP/s", "peak": MFMA_BF16_PEAK / 1e12,
+                         "infer_achieved": infer_tflops, "train_achieved": train_tflops,
+                         "infer_frac": infer_tflops / (MFMA_BF16_PEAK / 1e12) if infer_tflops else None,
+                         "flops_per_query": MLP_FLOPS_FWD, "flops_per_train_sample": MLP_FLOPS_TRAIN}},
+        "cache_opts": {"train_paths": args.cache_train_paths, "terminate_depth": args.cache_terminate,
+                       "warmup_frames": args.cache_warmup, "warmup_ms": warm_ms,
+                       "lr": args.cache_lr, "train_steps": args.cache_steps, "ema_decay": args.cache_ema},
+    }
+    film.close()
+    net.close()
+    scene.close()
+    return out
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device copy of `nbytes`, for scaling rocprofv3's
     FETCH_SIZE / WRITE_SIZE counters: a copy kernel of known size reads and
@@ -313,7 +438,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the traversal-counting pass")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", default="all", choices=["all", "extend", "serving"])
+    ap.add_argument("--workload", default="all", choices=["all", "extend", "serving", "cache"])
+    ap.add_argument("--cache-width", type=int, default=1920)
+    ap.add_argument("--cache-height", type=int, default=1080)
+    ap.add_argument("--cache-depth", type=int, default=16)
+    ap.add_argument("--cache-tris", type=int, default=2097152)
+    ap.add_argument("--cache-ref-spp", type=int, default=4096)
+    ap.add_argument("--cache-max-spp", type=int, default=256)
+    ap.add_argument("--cache-train-paths", type=int, default=65536)
+    ap.add_argument("--cache-terminate", type=int, default=1)
+    ap.add_argument("--cache-lr", type=float, default=0.005)
+    ap.add_argument("--cache-steps", type=int, default=4)
+    ap.add_argument("--cache-ema", type=float, default=0.0)
+    ap.add_argument("--cache-only", action="store_true", help="skip the plain curve (tuning runs)")
+    ap.add_argument("--cache-warmup", type=int, default=0,
+                    help="cached frames traced first to train the network; their time counts, their samples "
+                         "are not kept")
     ap.add_argument("--serve-jobs", type=int, default=256)
     ap.add_argument("--serve-scenes", type=int, default=32)
     ap.add_argument("--serve-seed", type=int, default=7)
@@ -329,6 +469,15 @@ def main():
         sys.exit(2)
 
     mcpt = importlib.import_module("monocuda-pt_amd")
+    if args.workload == "cache":
+        with mcpt.Context() as ctx:
+            cb = cache_bench(mcpt, ctx, args)
+        print(json.dumps({
+            "metric": "time to relMSE 0.01, plain / radiance-cached", "value": cb["speedup"], "unit": "x",
+            "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16",
+            "data": "synthetic (seeded procedural scene)", "config": {"workload": cb["workload"]}, "cache": cb}),
+            flush=True)
+        return
     if args.workload == "serving":
         with mcpt.Context() as ctx:
             sv = serving_bench(mcpt, ctx, args)

@@ -372,34 +372,40 @@ int mcpt_mlp_train(mcpt_mlp mlp, const float* features, const float* targets, in
                    float* loss);
 
 typedef struct mcpt_cache_opts {
-    int32_t train_paths;     /* unbiased training paths per frame (0 = 65536) */
+    int32_t train_paths;     /* image paths per frame traced without the
+                                cache and recorded for training (0 = 65536) */
     int32_t train_batch;     /* samples per Adam step (0 = 16384)             */
     int32_t train_steps;     /* at most this many steps per frame (0 = 4)     */
     int32_t terminate_depth; /* diffuse hits at this bounce or later end in
                                 the cache (0 = 1: the first secondary hit)   */
     float lr;                /* Adam step size (0 = 0.005)                    */
-    int32_t reserved;
+    float ema_decay;         /* queries use an exponential average of the
+                                weights, this decay per Adam step (0 = the
+                                latest weights)                              */
 } mcpt_cache_opts;
 
 typedef struct mcpt_cache_stats {
     double total_ms;       /* device time of the call                        */
-    double train_trace_ms; /* tracing the training paths                     */
+    double train_trace_ms; /* film update and the training batch          */
     double train_ms;       /* MLP forward + backward + Adam                  */
-    double render_ms;      /* tracing the image paths                        */
+    double render_ms;      /* tracing the frame's paths                      */
     double infer_ms;       /* MLP queries of the image paths                 */
     uint64_t train_samples, queries, paths;
     float loss;            /* last training step                             */
     int32_t reserved;
 } mcpt_cache_stats;
 
-/* Renders opts->spp frames (samples sample_begin ...). A frame first traces
- * cache_opts->train_paths unbiased paths from pixels drawn at random, turns
- * every diffuse vertex of them into a training sample (target: the radiance
- * the path gathered after the vertex divided by its throughput there) and
- * takes Adam steps; then it renders one sample per pixel whose paths end in
- * the cache at their first diffuse hit from terminate_depth on. The film
- * receives the same clamped linear sums as mcpt_render. Requires
- * MCPT_MODE_PERSISTENT or MCPT_MODE_WAVEFRONT. */
+/* Renders opts->spp frames (samples sample_begin ...), one sample per pixel
+ * each, as Mueller et al. 2021 train a neural radiance cache: the paths of
+ * cache_opts->train_paths pixels drawn at random are traced without the
+ * cache (unbiased samples of the image like any other) and every diffuse
+ * vertex on them becomes a training sample (target: the radiance the path
+ * gathered after the vertex divided by its throughput there); every other
+ * path ends in a cache query at its first diffuse hit from terminate_depth
+ * on. After the frame the network takes up to train_steps Adam steps on the
+ * frame's samples. Until the network has taken a step, frames are traced
+ * without the cache. The film receives the same clamped linear sums as
+ * mcpt_render. Requires MCPT_MODE_PERSISTENT or MCPT_MODE_WAVEFRONT. */
 int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* camera, const mcpt_render_opts* opts, mcpt_film film,
                        mcpt_mlp mlp, const mcpt_cache_opts* cache_opts, mcpt_cache_stats* stats);
 

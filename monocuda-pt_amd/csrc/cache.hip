@@ -321,9 +321,11 @@ __global__ __launch_bounds__(kBlock) void k_mlp_reduce(const float* partial, con
     }
 }
 
-// Adam on the FP32 master weights, then their BF16 image
+// Adam on the FP32 master weights, then their BF16 image; with decay > 0
+// also the exponential average of the weights and its image
 __global__ __launch_bounds__(kBlock) void k_adam(float* master, float* m, float* v, const float* grad, __bf16* wimg,
-                                                float lr, float bc1, float bc2) {
+                                                float lr, float bc1, float bc2, float* ema, __bf16* wimg_ema,
+                                                float decay) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= kParams) return;
     const float g = grad[p];
@@ -335,6 +337,11 @@ __global__ __launch_bounds__(kBlock) void k_adam(float* master, float* m, float*
     master[p] = wv;
     const int layer = p / (kW * kIn), o = (p % (kW * kIn)) / kIn, in = p % kIn;
     wimg[img_row(layer, o) * kRow + in] = (__bf16)wv;
+    if (decay > 0.0f) {
+        const float a = decay * ema[p] + (1.0f - decay) * wv;
+        ema[p] = a;
+        wimg_ema[img_row(layer, o) * kRow + in] = (__bf16)a;
+    }
 }
 
 // BF16 image of the master weights (padding stays zero)
@@ -376,7 +383,9 @@ struct CacheWs {
     float4* thr;
     float4* smp;
     uint64_t* rng;
-    int* pix;  // pixel of each path
+    int* pix;    // pixel of each path
+    int* tslot;  // training slot of an image path, -1 for none
+    int* tpix;   // pixel of a training slot
     Query* queries;
     VertexRec* vrec;  // train paths * max_depth
     int* nv;          // vertices recorded per training path
@@ -428,33 +437,20 @@ __global__ __launch_bounds__(kBlock) void k_cc_camera(mcpt_camera cam, uint32_t 
     ws.smp[p] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
     ws.rng[p] = rng.g.state;
     ws.pix[p] = p;
+    ws.tslot[p] = -1;
 }
 
-// training paths from pixels drawn at random; their RNG streams are keyed by
-// sample indices with bit 30 set, disjoint from the image samples
-__global__ __launch_bounds__(kBlock) void k_cc_train_camera(mcpt_camera cam, uint32_t frame, uint64_t seed, int n,
-                                                           CacheWs ws) {
+// training slots: pixels drawn at random whose paths are traced without the
+// cache and recorded; a pixel drawn twice keeps its larger slot
+__global__ __launch_bounds__(kBlock) void k_cc_pick(uint32_t frame, uint64_t seed, int n, int npix, CacheWs ws) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    const int W = cam.width, npix = cam.width * cam.height;
-    if (i == 0) *ns_slot(ws.counters, 0) = (unsigned long long)(uint32_t)n;
     if (i >= n) return;
     mcpt_pcg32 pick;
     mcpt_pcg32_for_path(&pick, (uint32_t)i, frame, seed ^ 0x5851F42D4C957F2DULL);
     const int p = (int)(mcpt_pcg32_next(&pick) % (uint32_t)npix);
-    Rng rng;
-    mcpt_pcg32_for_path(&rng.g, (uint32_t)p, frame | 0x40000000u, seed);
-    vec3 o, d;
-    float time;
-    camera_ray(cam, p % W, p / W, rng, o, d, time);
-    RayItem it;
-    it.ot = make_float4(o.x, o.y, o.z, time);
-    it.dp = make_float4(d.x, d.y, d.z, __int_as_float(i));
-    ws.q[0][i] = it;
-    ws.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    ws.smp[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-    ws.rng[i] = rng.g.state;
-    ws.pix[i] = p;
+    ws.tpix[i] = p;
     ws.nv[i] = 0;
+    atomicMax(&ws.tslot[p], i);
 }
 
 __global__ void k_cc_bounce(int* c, int cur) {
@@ -509,10 +505,10 @@ __global__ __launch_bounds__(kBlock) void k_cc_extend(SceneView sv, CacheWs ws, 
     }
 }
 
-// shadeKernel (:3470-3696). TRAIN: record every diffuse vertex (features,
-// sample and throughput on arrival) before shading it. Otherwise a diffuse
-// hit at depth >= term_depth ends the path in a cache query.
-template <bool TRAIN>
+// shadeKernel (:3470-3696). A training path records every diffuse vertex
+// (features, sample and throughput on arrival) and is shaded as usual; on
+// other paths a diffuse hit at depth >= term_depth ends the path in a cache
+// query.
 __global__ __launch_bounds__(kBlock) void k_cc_shade(SceneView sv, CacheWs ws, int cur, Frame fr, vec3 bg,
                                                     float rr_cap) {
     const int slot = blockIdx.x * kBlock + threadIdx.x;
@@ -547,9 +543,10 @@ __global__ __launch_bounds__(kBlock) void k_cc_shade(SceneView sv, CacheWs ws, i
                 albedo = load3(m.albedo);
             }
         }
-        if (TRAIN && diffuse) {
-            const int k = ws.nv[path];
-            VertexRec& v = ws.vrec[(size_t)path * fr.max_depth + k];
+        const int ts = ws.tslot[path];
+        if (ts >= 0 && diffuse) {
+            const int k = ws.nv[ts];
+            VertexRec& v = ws.vrec[(size_t)ts * fr.max_depth + k];
             vertex_features(fr, h.p, rd, h.normal, albedo, v.f);
             v.s[0] = ps.smp.x;
             v.s[1] = ps.smp.y;
@@ -557,9 +554,9 @@ __global__ __launch_bounds__(kBlock) void k_cc_shade(SceneView sv, CacheWs ws, i
             v.t[0] = ps.thr.x;
             v.t[1] = ps.thr.y;
             v.t[2] = ps.thr.z;
-            ws.nv[path] = k + 1;
+            ws.nv[ts] = k + 1;
         }
-        if (!TRAIN && diffuse && fr.depth >= fr.term_depth) {
+        if (ts < 0 && diffuse && fr.depth >= fr.term_depth) {
             query = true;
             vertex_features(fr, h.p, rd, h.normal, albedo, qr.f);
             qr.thr[0] = ps.thr.x;
@@ -592,10 +589,8 @@ __global__ __launch_bounds__(kBlock) void k_cc_shade(SceneView sv, CacheWs ws, i
         si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, 0.0f);
         ws.sq[sslot] = si;
     }
-    if (!TRAIN) {
-        const int qslot = wave_push(&ws.counters[C_QCOUNT], query);
-        if (query) ws.queries[qslot] = qr;
-    }
+    const int qslot = wave_push(&ws.counters[C_QCOUNT], query);
+    if (query) ws.queries[qslot] = qr;
 }
 
 // shadowKernel (:3698-3728)
@@ -643,12 +638,13 @@ __global__ __launch_bounds__(1024) void k_cc_scan(const int* nv, int n, int* off
 // over the throughput that reached it) of every recorded vertex, in path
 // order
 __global__ __launch_bounds__(kBlock) void k_cc_targets(CacheWs ws, int n, int max_depth) {
-    const int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= n) return;
-    const float4 fin = ws.smp[p];
-    const int k = ws.nv[p], base = ws.offs[p];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int k = ws.nv[i], base = ws.offs[i];
+    if (k == 0) return;
+    const float4 fin = ws.smp[ws.tpix[i]];
     for (int v = 0; v < k; v++) {
-        const VertexRec r = ws.vrec[(size_t)p * max_depth + v];
+        const VertexRec r = ws.vrec[(size_t)i * max_depth + v];
         float* f = ws.tfeat + (size_t)(base + v) * MCPT_MLP_FEATURES;
         for (int i = 0; i < MCPT_MLP_FEATURES; i++) f[i] = r.f[i];
         const float fs[3] = {fin.x, fin.y, fin.z};
@@ -675,8 +671,8 @@ using namespace mcpt;
 
 struct mcpt_mlp_s {
     Ctx* ctx = nullptr;
-    DevBuf<float> master, m, v, grad, partial, partial_loss, loss;
-    DevBuf<__bf16> wimg;
+    DevBuf<float> master, m, v, grad, partial, partial_loss, loss, ema;
+    DevBuf<__bf16> wimg, wimg_ema;  // BF16 images of the weights and of their average
     DevBuf<float> io;  // staging for the host-array entry points
     int step = 0;
     // cached-render workspace, grown on demand
@@ -719,11 +715,12 @@ int mlp_gradient_dev(mcpt_mlp_s* mlp, const float* feat, const float* targ, int 
     return MCPT_OK;
 }
 
-int mlp_adam(mcpt_mlp_s* mlp, float lr, hipStream_t st) {
+int mlp_adam(mcpt_mlp_s* mlp, float lr, hipStream_t st, float ema_decay = 0.0f) {
     mlp->step++;
     const float bc1 = (float)(1.0 - std::pow(0.9, mlp->step)), bc2 = (float)(1.0 - std::pow(0.99, mlp->step));
     k_adam<<<blocks_for(kParams), kBlock, 0, st>>>(mlp->master.ptr, mlp->m.ptr, mlp->v.ptr, mlp->grad.ptr,
-                                                   mlp->wimg.ptr, lr, bc1, bc2);
+                                                   mlp->wimg.ptr, lr, bc1, bc2, mlp->ema.ptr, mlp->wimg_ema.ptr,
+                                                   ema_decay);
     MCPT_HIP(hipGetLastError());
     return MCPT_OK;
 }
@@ -737,11 +734,12 @@ int ensure_io(mcpt_mlp_s* mlp, size_t floats) {
 // training paths of up to `depth` bounces
 int ensure_cache_ws(mcpt_mlp_s* mlp, int64_t paths, int64_t train, int depth, CacheWs& ws) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const int64_t cap = std::max(paths, train);
+    const int64_t cap = paths;
     const int64_t nrec = train * depth;
     const size_t parts[] = {al(cap * sizeof(RayItem)),      al(cap * sizeof(RayItem)),   al(cap * sizeof(mcpt_packed_hit)),
                             al(cap * sizeof(ShadowItem)),   al(cap * sizeof(float4)),    al(cap * sizeof(float4)),
-                            al(cap * sizeof(uint64_t)),     al(cap * sizeof(int)),       al(paths * sizeof(Query)),
+                            al(cap * sizeof(uint64_t)),     al(cap * sizeof(int)),       al(cap * sizeof(int)),
+                            al(train * sizeof(int)),        al(paths * sizeof(Query)),
                             al(nrec * sizeof(VertexRec)),   al(train * sizeof(int)),     al((train + 1) * sizeof(int)),
                             al(nrec * MCPT_MLP_FEATURES * sizeof(float)), al(nrec * 3 * sizeof(float)),
                             al(C_NCOUNTERS * sizeof(int))};
@@ -766,7 +764,7 @@ int ensure_cache_ws(mcpt_mlp_s* mlp, int64_t paths, int64_t train, int depth, Ca
 void carve_cache_ws(mcpt_mlp_s* mlp, CacheWs& ws) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const int64_t paths = mlp->ws_paths, train = mlp->ws_train;
-    const int64_t cap = std::max(paths, train), nrec = train * mlp->ws_depth;
+    const int64_t cap = paths, nrec = train * mlp->ws_depth;
     char* b = (char*)mlp->ws_mem;
     ws.q[0] = (RayItem*)b, b += al(cap * sizeof(RayItem));
     ws.q[1] = (RayItem*)b, b += al(cap * sizeof(RayItem));
@@ -776,6 +774,8 @@ void carve_cache_ws(mcpt_mlp_s* mlp, CacheWs& ws) {
     ws.smp = (float4*)b, b += al(cap * sizeof(float4));
     ws.rng = (uint64_t*)b, b += al(cap * sizeof(uint64_t));
     ws.pix = (int*)b, b += al(cap * sizeof(int));
+    ws.tslot = (int*)b, b += al(cap * sizeof(int));
+    ws.tpix = (int*)b, b += al(train * sizeof(int));
     ws.queries = (Query*)b, b += al(paths * sizeof(Query));
     ws.vrec = (VertexRec*)b, b += al(nrec * sizeof(VertexRec));
     ws.nv = (int*)b, b += al(train * sizeof(int));
@@ -806,6 +806,8 @@ int mcpt_mlp_create(mcpt_ctx ctx, uint64_t seed, mcpt_mlp* out) {
     if (s >= 0) s = mlp->partial_loss.alloc(c, kMaxGradBlocks);
     if (s >= 0) s = mlp->loss.alloc(c, 4);
     if (s >= 0) s = mlp->wimg.alloc(c, kImg);
+    if (s >= 0) s = mlp->ema.alloc(c, kParams);
+    if (s >= 0) s = mlp->wimg_ema.alloc(c, kImg);
     if (s < 0) {
         mcpt_mlp_destroy(mlp);
         return s;
@@ -820,9 +822,10 @@ int mcpt_mlp_destroy(mcpt_mlp mlp) {
     if (!mlp) return MCPT_OK;
     Ctx* c = mlp->ctx;
     for (DevBuf<float>* b : {&mlp->master, &mlp->m, &mlp->v, &mlp->grad, &mlp->partial, &mlp->partial_loss,
-                             &mlp->loss, &mlp->io})
+                             &mlp->loss, &mlp->io, &mlp->ema})
         b->release(c);
     mlp->wimg.release(c);
+    mlp->wimg_ema.release(c);
     if (mlp->ws_mem) dev_free(c, mlp->ws_mem, mlp->ws_bytes);
     delete mlp;
     return MCPT_OK;
@@ -842,6 +845,8 @@ int mcpt_mlp_set_params(mcpt_mlp mlp, const float* params) {
     MCPT_HIP(hipMemsetAsync(mlp->m.ptr, 0, kParams * sizeof(float), st));
     MCPT_HIP(hipMemsetAsync(mlp->v.ptr, 0, kParams * sizeof(float), st));
     k_pack<<<blocks_for(kImg), kBlock, 0, st>>>(mlp->master.ptr, mlp->wimg.ptr);
+    MCPT_HIP(hipMemcpyAsync(mlp->ema.ptr, mlp->master.ptr, kParams * sizeof(float), hipMemcpyDeviceToDevice, st));
+    MCPT_HIP(hipMemcpyAsync(mlp->wimg_ema.ptr, mlp->wimg.ptr, kImg * sizeof(__bf16), hipMemcpyDeviceToDevice, st));
     MCPT_HIP(hipGetLastError());
     MCPT_HIP(hipStreamSynchronize(st));
     mlp->step = 0;
@@ -921,16 +926,17 @@ int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* cam, const mcpt_rend
     if (o->spp < 1 || o->max_depth < 1 || o->sample_begin < 0 || o->sample_begin + o->spp > 0x40000000)
         return set_error(MCPT_ERR_INVALID, "spp, max_depth or sample_begin out of range");
     if (scene->ctx != mlp->ctx) return set_error(MCPT_ERR_INVALID, "scene and MLP belong to different contexts");
-    const int train_paths = (co && co->train_paths > 0) ? co->train_paths : 65536;
+    const int npix = cam->width * cam->height, depth = o->max_depth;
+    const int train_paths = std::min(npix, (co && co->train_paths > 0) ? co->train_paths : 65536);
     const int batch = (co && co->train_batch > 0) ? co->train_batch : 16384;
     const int max_steps = (co && co->train_steps > 0) ? co->train_steps : 4;
     const int term_depth = (co && co->terminate_depth > 0) ? co->terminate_depth : 1;
     const float lr = (co && co->lr > 0.0f) ? co->lr : 0.005f;
-    if (train_paths > (1 << 24) || batch > (1 << 24))
-        return set_error(MCPT_ERR_INVALID, "train_paths and train_batch must be <= 16 Mi");
+    const float decay = (co && co->ema_decay > 0.0f) ? co->ema_decay : 0.0f;
+    if (!(decay < 1.0f)) return set_error(MCPT_ERR_INVALID, "ema_decay must be < 1");
+    if (batch > (1 << 24)) return set_error(MCPT_ERR_INVALID, "train_batch must be <= 16 Mi");
     Ctx* ctx = scene->ctx;
     hipStream_t st = ctx->stream;
-    const int npix = cam->width * cam->height, depth = o->max_depth;
     CacheWs ws;
     MCPT_TRY(ensure_cache_ws(mlp, npix, train_paths, depth, ws));
     carve_cache_ws(mlp, ws);
@@ -950,7 +956,6 @@ int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* cam, const mcpt_rend
         }
     }
     fr.max_depth = depth;
-    fr.term_depth = term_depth;
     const vec3 bg = load3(cam->background);
     mcpt_cache_stats acc;
     std::memset(&acc, 0, sizeof(acc));
@@ -961,62 +966,54 @@ int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* cam, const mcpt_rend
         (void)hipEventElapsedTime(&ms, a, b);
         return (double)ms;
     };
-    // one bounce loop over the current queue contents
-    auto trace = [&](bool train, int n) {
+    const int grid = blocks_for(npix);
+    for (int f = 0; f < o->spp; f++) {
+        const uint32_t sample = (uint32_t)(o->sample_begin + f);
+        // a network that has not been trained yet is not queried: the frame
+        // is traced without the cache
+        fr.term_depth = mlp->step > 0 ? term_depth : depth;
+        MCPT_HIP(hipEventRecord(ev[0], st));
+        k_cc_camera<<<grid, kBlock, 0, st>>>(*cam, sample, o->seed, ws);
+        k_cc_pick<<<blocks_for(train_paths), kBlock, 0, st>>>(sample, o->seed, train_paths, npix, ws);
         int cur = 0;
-        const int grid = blocks_for(n);
         for (int d = 0; d < depth; d++) {
             fr.depth = d;
             k_cc_bounce<<<1, 64, 0, st>>>(ws.counters, cur);
             k_cc_extend<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(sv, ws, cur, refill);
-            if (train)
-                k_cc_shade<true><<<grid, kBlock, 0, st>>>(sv, ws, cur, fr, bg, cam->rr_cap);
-            else
-                k_cc_shade<false><<<grid, kBlock, 0, st>>>(sv, ws, cur, fr, bg, cam->rr_cap);
+            k_cc_shade<<<grid, kBlock, 0, st>>>(sv, ws, cur, fr, bg, cam->rr_cap);
             k_cc_shadow<<<grid, kBlock, lds, st>>>(sv, ws, cur);
             cur = 1 - cur;
         }
-    };
-    for (int f = 0; f < o->spp; f++) {
-        const uint32_t sample = (uint32_t)(o->sample_begin + f);
-        // -- training paths, their samples, Adam steps
-        MCPT_HIP(hipEventRecord(ev[0], st));
-        k_cc_train_camera<<<blocks_for(train_paths), kBlock, 0, st>>>(*cam, sample, o->seed, train_paths, ws);
-        trace(true, train_paths);
+        MCPT_HIP(hipEventRecord(ev[1], st));
+        k_mlp_infer<<<std::max(1, std::min((npix + kTile - 1) / kTile, 2 * ctx->cu_count)), kBlock, 0, st>>>(
+            decay > 0.0f ? mlp->wimg_ema.ptr : mlp->wimg.ptr, ws.queries, ws.counters + C_QCOUNT, 0, nullptr,
+            ws.smp);
+        MCPT_HIP(hipEventRecord(ev[2], st));
+        k_cc_finalize<<<grid, kBlock, 0, st>>>(ws.smp, film->accum.ptr, npix);
+        // training samples of this frame's recorded paths, for the next frame
         k_cc_scan<<<1, 1024, 0, st>>>(ws.nv, train_paths, ws.offs);
         k_cc_targets<<<blocks_for(train_paths), kBlock, 0, st>>>(ws, train_paths, depth);
         MCPT_HIP(hipGetLastError());
-        int n_train = 0;
+        int n_train = 0, nq = 0;
         MCPT_HIP(hipMemcpyAsync(&n_train, ws.offs + train_paths, sizeof(int), hipMemcpyDeviceToHost, st));
-        MCPT_HIP(hipEventRecord(ev[1], st));
-        MCPT_HIP(hipEventSynchronize(ev[1]));
+        MCPT_HIP(hipMemcpyAsync(&nq, ws.counters + C_QCOUNT, sizeof(int), hipMemcpyDeviceToHost, st));
+        MCPT_HIP(hipEventRecord(ev[3], st));
+        MCPT_HIP(hipEventSynchronize(ev[3]));
         const int steps = std::min(max_steps, (n_train + batch - 1) / batch);
         for (int k = 0; k < steps; k++) {
             const int first = k * batch, cnt = std::min(batch, n_train - first);
             MCPT_TRY(mlp_gradient_dev(mlp, ws.tfeat, ws.ttarg, first, cnt, st));
-            MCPT_TRY(mlp_adam(mlp, lr, st));
+            MCPT_TRY(mlp_adam(mlp, lr, st, decay));
         }
-        MCPT_HIP(hipEventRecord(ev[2], st));
-        // -- image paths ending in the cache
-        k_cc_camera<<<blocks_for(npix), kBlock, 0, st>>>(*cam, sample, o->seed, ws);
-        trace(false, npix);
-        MCPT_HIP(hipEventRecord(ev[3], st));
-        k_mlp_infer<<<std::max(1, std::min((npix + kTile - 1) / kTile, 2 * ctx->cu_count)), kBlock, 0, st>>>(
-            mlp->wimg.ptr, ws.queries, ws.counters + C_QCOUNT, 0, nullptr, ws.smp);
         MCPT_HIP(hipEventRecord(ev[4], st));
-        k_cc_finalize<<<blocks_for(npix), kBlock, 0, st>>>(ws.smp, film->accum.ptr, npix);
-        MCPT_HIP(hipGetLastError());
-        MCPT_HIP(hipEventRecord(ev[5], st));
-        MCPT_HIP(hipEventSynchronize(ev[5]));
-        int nq = 0;
-        MCPT_HIP(hipMemcpy(&nq, ws.counters + C_QCOUNT, sizeof(int), hipMemcpyDeviceToHost));
+        MCPT_HIP(hipEventSynchronize(ev[4]));
         float l = 0.f;
         if (steps) MCPT_HIP(hipMemcpy(&l, mlp->loss.ptr, sizeof(float), hipMemcpyDeviceToHost));
-        acc.train_trace_ms += elapsed(ev[0], ev[1]);
-        acc.train_ms += elapsed(ev[1], ev[2]);
-        acc.render_ms += elapsed(ev[2], ev[3]) + elapsed(ev[4], ev[5]);
-        acc.infer_ms += elapsed(ev[3], ev[4]);
-        acc.total_ms += elapsed(ev[0], ev[5]);
+        acc.render_ms += elapsed(ev[0], ev[1]);
+        acc.train_trace_ms += elapsed(ev[2], ev[3]);
+        acc.infer_ms += elapsed(ev[1], ev[2]);
+        acc.train_ms += elapsed(ev[3], ev[4]);
+        acc.total_ms += elapsed(ev[0], ev[4]);
         acc.train_samples += (uint64_t)std::min(n_train, steps * batch);
         acc.queries += (uint64_t)nq;
         acc.paths += (uint64_t)npix;

@@ -117,11 +117,11 @@ def test_cached_render_gains_over_plain(mcpt, ctx):
     ref_img = ref.read() / 1024
     net = ctx.mlp(seed=1)
     warm = ctx.film(W, H)
-    st = scene.render_cached(cam, warm, net, spp=32, max_depth=depth, seed=5, train_paths=4096, train_batch=4096)
+    st = scene.render_cached(cam, warm, net, spp=32, max_depth=depth, seed=5, train_paths=1024, train_batch=4096)
     assert st["queries"] > 0 and st["train_samples"] > 0
     spp = 4
     cached = ctx.film(W, H)
-    scene.render_cached(cam, cached, net, spp=spp, sample_begin=32, max_depth=depth, seed=5, train_paths=4096,
+    scene.render_cached(cam, cached, net, spp=spp, sample_begin=32, max_depth=depth, seed=5, train_paths=1024,
                         train_batch=4096)
     plain = ctx.film(W, H)
     scene.render(cam, plain, spp=spp, max_depth=depth, seed=5)
