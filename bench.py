@@ -282,6 +282,75 @@ def serving_bench(mcpt, ctx, args):
     return out
 
 
+# ---------------------------------------------------------- batched scenes
+def batch_bench(mcpt, ctx, args):
+    """64 seeded scenes (displaced icospheres + random fill, triangle counts
+    log-uniform in 1k-500k), each 256x256, 8 spp, depth 8: one batched LBVH
+    build + one batched render (autotuned) against 64 sequential builds and
+    renders. Films are compared bit for bit."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(args.batch_seed)
+    tris = np.exp(rng.uniform(np.log(1e3), np.log(5e5), args.batch_scenes)).astype(np.int64)
+    t0 = time.time()
+    with ThreadPoolExecutor(8) as ex:
+        data = list(ex.map(lambda a: mcpt.gen_scene(mcpt.GEN_ICOSPHERE, seed=a[0], param=int(a[1])),
+                           enumerate(tris, start=200)))
+    S, spp, depth = args.batch_size, args.batch_spp, args.batch_depth
+    cams = [mcpt.camera_init(mcpt.gen_camera(mcpt.GEN_SOUP, S, S))] * len(data)
+    n_tris = sum(len(d[0]) for d in data)
+    log(f"batch: {len(data)} scenes, {n_tris} triangles generated in {time.time() - t0:.1f} s")
+    # sequential: build + render scene by scene (persistent, all samples in flight)
+    seq = [ctx.scene(*d) for d in data]
+    films = [ctx.film(S, S) for _ in data]
+    seq[0].build()
+    seq[0].render(cams[0], films[0], spp=1, sample_begin=1 << 20, max_depth=depth, seed=args.seed)  # warm-up
+    films[0].clear()
+    seq_build = seq_render = 0.0
+    t1 = time.perf_counter()
+    for sc, f, cam in zip(seq, films, cams):
+        seq_build += sc.build()["build_ms"]
+        seq_render += sc.render(cam, f, spp=spp, max_depth=depth, seed=args.seed)["total_ms"]
+    seq_wall = time.perf_counter() - t1
+    ref = [f.read() for f in films]
+    for sc in seq:
+        sc.close()
+    for f in films:
+        f.clear()
+    # batched
+    bat = [ctx.scene(*d) for d in data]
+    t2 = time.perf_counter()
+    b_build = mcpt.build_batch(bat)
+    tune, search_ms = mcpt.batch_autotune(bat, cams, spp=spp, max_depth=depth, seed=args.seed)
+    st = mcpt.render_batch(bat, cams, films, spp=spp, max_depth=depth, seed=args.seed, tune=tune)
+    b_wall = time.perf_counter() - t2
+    b_render = st["total_ms"]
+    equal = all(np.array_equal(f.read(), r) for f, r in zip(films, ref))
+    paths = S * S * spp * len(data)
+    seq_ms, b_ms = seq_build + seq_render, b_build + b_render
+    out = {
+        "workload": f"{len(data)} displaced-icosphere scenes ({int(tris.min())}-{int(tris.max())} requested tris, "
+                    f"{n_tris} total), {S}x{S}, {spp} spp, depth {depth}",
+        "films_identical": equal,
+        "sequential": {"build_ms": seq_build, "render_ms": seq_render, "device_ms": seq_ms, "wall_s": seq_wall,
+                       "scenes_per_s": len(data) / (seq_ms / 1e3), "paths_per_s": paths / (seq_ms / 1e3)},
+        "batched": {"build_ms": b_build, "render_ms": b_render, "device_ms": b_ms, "wall_s": b_wall,
+                    "autotune_ms": search_ms, "tune": tune.as_dict(),
+                    "scenes_per_s": len(data) / (b_ms / 1e3), "paths_per_s": paths / (b_ms / 1e3),
+                    "scenes_per_s_with_autotune": len(data) / ((b_ms + search_ms) / 1e3),
+                    "extend_rays": st["extend_rays"], "shadow_rays": st["shadow_rays"]},
+        "speedup": seq_ms / b_ms, "speedup_with_autotune": seq_ms / (b_ms + search_ms),
+        "build_speedup": seq_build / b_build,
+    }
+    log(f"batch: sequential {seq_ms:.1f} ms, batched {b_ms:.1f} ms (+ autotune {search_ms:.1f} ms), "
+        f"identical films {equal}")
+    for sc in bat:
+        sc.close()
+    for f in films:
+        f.close()
+    return out
+
+
 # ---------------------------------------------------------- radiance cache
 MLP_FLOPS_FWD = 2 * (4 * 64 * 64 + 3 * 64)  # multiply-adds x 2 per query
 MLP_FLOPS_TRAIN = 3 * MLP_FLOPS_FWD         # forward, input gradient, weight gradient
@@ -444,7 +513,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the traversal-counting pass")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", default="all", choices=["all", "extend", "serving", "cache"])
+    ap.add_argument("--workload", default="all", choices=["all", "extend", "serving", "cache", "batch"])
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--batch-seed", type=int, default=3)
+    ap.add_argument("--batch-size", type=int, default=256)
+    ap.add_argument("--batch-spp", type=int, default=8)
+    ap.add_argument("--batch-depth", type=int, default=8)
     ap.add_argument("--cache-width", type=int, default=1920)
     ap.add_argument("--cache-height", type=int, default=1080)
     ap.add_argument("--cache-depth", type=int, default=16)
@@ -475,6 +549,15 @@ def main():
         sys.exit(2)
 
     mcpt = importlib.import_module("monocuda-pt_amd")
+    if args.workload == "batch":
+        with mcpt.Context() as ctx:
+            bb = batch_bench(mcpt, ctx, args)
+        print(json.dumps({
+            "metric": "scenes/s and paths/s, batched build + render vs sequential", "value":
+            bb["batched"]["scenes_per_s"], "unit": "scenes/s", "n_gpus": 1, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scenes)",
+            "config": {"workload": bb["workload"]}, "batch": bb}), flush=True)
+        return
     if args.workload == "cache":
         with mcpt.Context() as ctx:
             cb = cache_bench(mcpt, ctx, args)
@@ -611,6 +694,8 @@ def main():
         del hot, cold, mats
         with mcpt.Context() as ctx:
             result["serving"] = serving_bench(mcpt, ctx, args)
+        with mcpt.Context() as ctx:
+            result["batch"] = batch_bench(mcpt, ctx, args)
     print(json.dumps(result), flush=True)
 
 

@@ -332,6 +332,43 @@ int mcpt_server_release(mcpt_server srv, uint64_t job_id);
 int mcpt_server_stats_get(mcpt_server srv, mcpt_server_stats* stats);
 int mcpt_server_destroy(mcpt_server srv);
 
+/* --------------------------------------------------------- batched scenes
+ * No counterpart in the reference, which builds and renders one scene per
+ * run. Every scene's BVH is bit-identical to its own mcpt_lbvh_build and
+ * every film to mcpt_render of its scene with the same options. */
+
+/* Builds the BVHs of n scenes together: each LBVH stage is one launch over
+ * the triangles of all of them (scenes of one triangle take the single
+ * path). build_ms (optional) receives the device time. */
+int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms);
+
+typedef struct mcpt_batch_tune {
+    int32_t samples_per_pass; /* samples of every scene in flight per pass
+                                 (0 = as many as fit a quarter of the budget) */
+    int32_t refill_lanes;     /* persistent extend refill threshold (0 = 32)   */
+    int32_t blocks_per_cu;    /* persistent extend blocks per CU (0 = the
+                                 occupancy limit)                              */
+    int32_t reserved;
+} mcpt_batch_tune;
+
+/* Renders scene i through cameras[i] into films[i]; the rays of all scenes
+ * share one wavefront (persistent extend). opts applies to every scene and
+ * its mode must be MCPT_MODE_PERSISTENT. tune may be NULL. */
+int mcpt_render_batch(const mcpt_scene* scenes, const mcpt_camera* cameras, int32_t n, const mcpt_render_opts* opts,
+                      mcpt_film* films, const mcpt_batch_tune* tune, mcpt_render_stats* stats);
+
+/* Chooses a tune for mcpt_render_batch. Queue capacity: every sample of
+ * every scene in one pass when a quarter of the HBM budget holds it (a pass
+ * ends with its slowest rays, so fewer, fuller passes are faster), else as
+ * many samples as fit. Launch shape: the refill threshold, then the
+ * persistent blocks per CU, each kept when it lowers the device time per
+ * path of a one-sample calibration pass (the samples after the requested
+ * range, into scratch films). A tune depends on the workload's shape and is
+ * meant to be reused across batches of that shape. search_ms (optional)
+ * receives the device time the search spent. */
+int mcpt_batch_autotune(const mcpt_scene* scenes, const mcpt_camera* cameras, int32_t n, const mcpt_render_opts* opts,
+                        mcpt_batch_tune* best, double* search_ms);
+
 /* --------------------------------------------------------- radiance cache
  * No counterpart in the reference. An MLP trained online on the tracer's
  * own paths predicts the radiance scattered at a path vertex (next-event

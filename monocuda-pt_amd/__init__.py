@@ -104,6 +104,14 @@ class GenSizes(C.Structure):
     _fields_ = [("n_tris", C.c_int32), ("n_materials", C.c_int32)]
 
 
+class BatchTune(C.Structure):
+    _fields_ = [("samples_per_pass", C.c_int32), ("refill_lanes", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("reserved", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class CacheOpts(C.Structure):
     _fields_ = [("train_paths", C.c_int32), ("train_batch", C.c_int32), ("train_steps", C.c_int32),
                 ("terminate_depth", C.c_int32), ("lr", C.c_float), ("ema_decay", C.c_float)]
@@ -179,6 +187,11 @@ _SIGS = {
     "mcpt_server_release": ([VP, C.c_uint64], C.c_int),
     "mcpt_server_stats_get": ([VP, C.POINTER(ServerStats)], C.c_int),
     "mcpt_server_destroy": ([VP], C.c_int),
+    "mcpt_lbvh_build_batch": ([VP, C.c_int32, C.POINTER(C.c_float)], C.c_int),
+    "mcpt_render_batch": ([VP, VP, C.c_int32, C.POINTER(RenderOpts), VP, C.POINTER(BatchTune),
+                           C.POINTER(RenderStats)], C.c_int),
+    "mcpt_batch_autotune": ([VP, VP, C.c_int32, C.POINTER(RenderOpts), C.POINTER(BatchTune),
+                             C.POINTER(C.c_double)], C.c_int),
     "mcpt_mlp_create": ([VP, C.c_uint64, C.POINTER(VP)], C.c_int),
     "mcpt_mlp_destroy": ([VP], C.c_int),
     "mcpt_mlp_get_params": ([VP, VP], C.c_int),
@@ -218,6 +231,42 @@ def _ptr(a):
 
 def version():
     return _lib.mcpt_version().decode()
+
+
+def _handles(objs):
+    return (VP * len(objs))(*[o.h for o in objs])
+
+
+def _cameras(cams):
+    return (Camera * len(cams))(*cams)
+
+
+def build_batch(scenes):
+    """mcpt_lbvh_build_batch: every LBVH stage as one launch over all scenes;
+    returns the device time in ms."""
+    ms = C.c_float()
+    _check(_lib.mcpt_lbvh_build_batch(_handles(scenes), len(scenes), C.byref(ms)), "mcpt_lbvh_build_batch")
+    return ms.value
+
+
+def render_batch(scenes, cameras, films, spp=1, sample_begin=0, max_depth=8, seed=0, tune=None):
+    """mcpt_render_batch: scene i through cameras[i] into films[i], one wavefront for all."""
+    o = RenderOpts(MODE_PERSISTENT, spp, sample_begin, max_depth, seed, 0, 0, 0, 0)
+    st = RenderStats()
+    t = C.byref(tune) if tune is not None else None
+    _check(_lib.mcpt_render_batch(_handles(scenes), _cameras(cameras), len(scenes), C.byref(o), _handles(films), t,
+                                  C.byref(st)), "mcpt_render_batch")
+    return st.as_dict()
+
+
+def batch_autotune(scenes, cameras, spp=1, sample_begin=0, max_depth=8, seed=0):
+    """mcpt_batch_autotune: returns (BatchTune, search device ms)."""
+    o = RenderOpts(MODE_PERSISTENT, spp, sample_begin, max_depth, seed, 0, 0, 0, 0)
+    best = BatchTune()
+    ms = C.c_double()
+    _check(_lib.mcpt_batch_autotune(_handles(scenes), _cameras(cameras), len(scenes), C.byref(o), C.byref(best),
+                                    C.byref(ms)), "mcpt_batch_autotune")
+    return best, ms.value
 
 
 # ------------------------------------------------------------ host helpers

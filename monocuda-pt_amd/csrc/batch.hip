@@ -1,0 +1,863 @@
+// batch.hip — many scenes at once: their LBVH builds as one launch per stage,
+// their renders as one wavefront whose queues hold the rays of all of them,
+// and an autotuner for that wavefront's shape. The reference builds and
+// renders one scene per process run; nothing here has a counterpart in it
+// except the per-scene stages, which are lbvh.hip's and render.hip's steps
+// applied per segment.
+//
+// Build: the triangles of scene s are the global indices [off_s, off_s+n_s).
+// Bounds are reduced per scene (one block each), Morton codes use their own
+// scene's bounds, and the stable LSD radix sort takes four 8-bit passes over
+// the codes and a fifth over the scene index: the result is ordered by
+// (scene, code, index), i.e. each scene's segment is exactly the (code,
+// index) order of its own build. Leaves, the Karras tree, the refit and the
+// depth then run over all segments in one launch each, writing every
+// scene's own node, leaf and box buffers, so each BVH is bit-identical to
+// mcpt_lbvh_build of that scene alone.
+//
+// Render: path = k * P + q, where q runs over the pixels of all scenes
+// (scene s owns q in [foff_s, foff_s + W_s H_s)) and k over the samples of
+// the pass. Rays carry their scene index; the persistent extend loads that
+// scene's BVH view per ray, shade and shadow its scene view. Finalize adds a
+// pixel's samples in sample order, so every film is bit-identical to
+// mcpt_render of its scene.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "lbvh_common.hpp"
+#include "scene.hpp"
+#include "shade.hpp"
+
+namespace mcpt {
+namespace {
+
+struct Seg {  // one scene of a batched build
+    int off, n, ioff, pad;
+    const float* hot;
+    BvhNode* nodes;
+    LeafTri* leaf_tri;
+    int* leaf_prim;
+    float* root_box;
+};
+
+// scene owning global index g (segments ordered by off)
+__device__ __forceinline__ int seg_of(const Seg* seg, int nseg, int g) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (seg[mid].off <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+// scene owning internal node g (offsets ioff)
+__device__ __forceinline__ int seg_of_node(const Seg* seg, int nseg, int g) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (seg[mid].ioff <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// scene bounds in two levels (min/max are exact in any order): chunks of
+// kChunk triangles of one scene per block, then one block per scene over its
+// chunks. chunk_off[s] is the first chunk of scene s.
+constexpr int kChunk = 8192;
+
+__device__ void reduce_box(float acc[6], float* out) {
+    __shared__ float sh[6][kBlock];
+    for (int k = 0; k < 6; k++) sh[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            for (int k = 0; k < 3; k++) sh[k][threadIdx.x] = fminf(sh[k][threadIdx.x], sh[k][threadIdx.x + s]);
+            for (int k = 3; k < 6; k++) sh[k][threadIdx.x] = fmaxf(sh[k][threadIdx.x], sh[k][threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_bounds_chunks(const Seg* seg, int nseg, const int* chunk_off,
+                                                           float* partial) {
+    const int c = blockIdx.x;
+    int lo = 0, hi = nseg - 1;  // scene of chunk c
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (chunk_off[mid] <= c) lo = mid;
+        else hi = mid - 1;
+    }
+    const Seg S = seg[lo];
+    const int b0 = (c - chunk_off[lo]) * kChunk, b1 = min(S.n, b0 + kChunk);
+    float acc[6] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), -__builtin_inff(), -__builtin_inff(),
+                    -__builtin_inff()};
+    for (int i = b0 + threadIdx.x; i < b1; i += kBlock) {
+        float t[9], b[6];
+        for (int k = 0; k < 9; k++) t[k] = S.hot[(size_t)i * 9 + k];
+        tri_box(t, b);
+        for (int k = 0; k < 3; k++) acc[k] = fminf(acc[k], b[k]);
+        for (int k = 3; k < 6; k++) acc[k] = fmaxf(acc[k], b[k]);
+    }
+    reduce_box(acc, partial + 6 * c);
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_bounds_final(const int* chunk_off, const float* partial, float* sb) {
+    const int s = blockIdx.x;
+    float acc[6] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), -__builtin_inff(), -__builtin_inff(),
+                    -__builtin_inff()};
+    for (int c = chunk_off[s] + threadIdx.x; c < chunk_off[s + 1]; c += kBlock) {
+        for (int k = 0; k < 3; k++) acc[k] = fminf(acc[k], partial[6 * c + k]);
+        for (int k = 3; k < 6; k++) acc[k] = fmaxf(acc[k], partial[6 * c + k]);
+    }
+    reduce_box(acc, sb + 6 * s);
+}
+
+// morton3D/quantize (:913-935) against the triangle's own scene bounds
+__global__ __launch_bounds__(kBlock) void k_b_morton(const Seg* seg, int nseg, int total, const float* sbs,
+                                                    uint32_t* morton, uint32_t* keys, int* vals) {
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= total) return;
+    const int s = seg_of(seg, nseg, g);
+    const float* t = seg[s].hot + (size_t)(g - seg[s].off) * 9;
+    const float* sb = sbs + 6 * s;
+    const vec3 c = (vec3(t[0], t[1], t[2]) + vec3(t[3], t[4], t[5]) + vec3(t[6], t[7], t[8])) / 3.0f;
+    const uint32_t qx = (uint32_t)((c.x - sb[0]) / (sb[3] - sb[0]) * 1023.0f);
+    const uint32_t qy = (uint32_t)((c.y - sb[1]) / (sb[4] - sb[1]) * 1023.0f);
+    const uint32_t qz = (uint32_t)((c.z - sb[2]) / (sb[5] - sb[2]) * 1023.0f);
+    const uint32_t m = expand_bits(qx) | expand_bits(qy) << 1 | expand_bits(qz) << 2;
+    morton[g] = m;
+    keys[g] = m;
+    vals[g] = g;
+}
+
+// scene index of each element in its current order, the key of the last pass
+__global__ __launch_bounds__(kBlock) void k_b_scene_keys(const Seg* seg, int nseg, int total, const int* vals,
+                                                        uint32_t* keys) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < total) keys[i] = (uint32_t)seg_of(seg, nseg, vals[i]);
+}
+
+// sorted codes by gathering through the sorted ids
+__global__ __launch_bounds__(kBlock) void k_b_gather_keys(const uint32_t* morton, const int* vals, int total,
+                                                         uint32_t* keys) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < total) keys[i] = morton[vals[i]];
+}
+
+// leaf records per scene, in its sorted order (k_leaves of lbvh.hip)
+__global__ __launch_bounds__(kBlock) void k_b_leaves(const Seg* seg, int nseg, int total, const int* ids) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= total) return;
+    const int s = seg_of(seg, nseg, j);
+    const Seg S = seg[s];
+    const int jl = j - S.off, p = ids[j] - S.off;
+    S.leaf_prim[jl] = p;
+    const float* t = S.hot + (size_t)p * 9;
+    const vec3 v0(t[0], t[1], t[2]), v1(t[3], t[4], t[5]), v2(t[6], t[7], t[8]);
+    const vec3 nn = triangle_normal(v0, v1, v2);
+    LeafTri L;
+    L.a = make_float4(v0.x, v0.y, v0.z, v1.x);
+    L.b = make_float4(v1.y, v1.z, v2.x, v2.y);
+    L.c = make_float4(v2.z, nn.x, nn.y, nn.z);
+    S.leaf_tri[jl] = L;
+}
+
+// Karras tree of each scene with >= 2 triangles (buildInternalNodesKernel,
+// :1417-1447, determineRange/findSplit :1349-1386); leaf_parent is indexed
+// by global leaf position
+__global__ __launch_bounds__(kBlock) void k_b_karras(const Seg* seg, int nseg, int total_internal,
+                                                    const uint32_t* skeys, int* leaf_parent) {
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= total_internal) return;
+    const Seg S = seg[seg_of_node(seg, nseg, g)];
+    const int idx = g - S.ioff, n = S.n;
+    const uint32_t* keys = skeys + S.off;
+    const int d = (delta(keys, n, idx, idx + 1) - delta(keys, n, idx, idx - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(keys, n, idx, idx - d);
+    int lmax = 2;
+    while (delta(keys, n, idx, idx + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (delta(keys, n, idx, idx + (l + t) * d) > dmin) l += t;
+    const int j = idx + l * d;
+    const int first = min(idx, j), last = max(idx, j);
+    const int cp = delta(keys, n, first, last);
+    int split = first, step = last - first;
+    do {
+        step = (step + 1) >> 1;
+        const int ns = split + step;
+        if (ns < last && delta(keys, n, first, ns) > cp) split = ns;
+    } while (step > 1);
+    int* rf = reinterpret_cast<int*>(S.nodes + idx) + 12;
+    const int kids[2] = {split, split + 1};
+    const bool leafs[2] = {split == first, split + 1 == last};
+    for (int side = 0; side < 2; side++) {
+        const int c = kids[side];
+        if (leafs[side]) {
+            rf[side] = ~c;
+            leaf_parent[S.off + c] = (idx << 1) | side;
+            float b[6];
+            const LeafTri L = S.leaf_tri[c];
+            const float tt[9] = {L.a.x, L.a.y, L.a.z, L.a.w, L.b.x, L.b.y, L.b.z, L.b.w, L.c.x};
+            tri_box(tt, b);
+            store_slot(S.nodes, idx, side, b);
+        } else {
+            rf[side] = c;
+            reinterpret_cast<int*>(S.nodes + c)[14] = (idx << 1) | side;
+        }
+    }
+    if (idx == 0) rf[2] = -1;
+    rf[3] = j;
+}
+
+// refitBoundsKernel (:1398-1415) per scene; counters indexed by global node
+__global__ __launch_bounds__(kBlock) void k_b_refit(const Seg* seg, int nseg, int total, const int* leaf_parent,
+                                                   int* counters) {
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= total) return;
+    const Seg S = seg[seg_of(seg, nseg, g)];
+    int p = leaf_parent[g] >> 1;
+    while (true) {
+        const int old = refit_arrive(&counters[S.ioff + p]);
+        if (old == 0) return;
+        const float* f = reinterpret_cast<const float*>(S.nodes + p);
+        float b[6];
+        for (int k = 0; k < 3; k++) b[k] = fminf(f[k], f[6 + k]);
+        for (int k = 3; k < 6; k++) b[k] = fmaxf(f[k], f[6 + k]);
+        const int pp = reinterpret_cast<const int*>(S.nodes + p)[14];
+        if (pp < 0) {
+            for (int k = 0; k < 6; k++) S.root_box[k] = b[k];
+            return;
+        }
+        store_slot(S.nodes, pp >> 1, pp & 1, b);
+        p = pp >> 1;
+    }
+}
+
+// deepest leaf of each scene: a wave reduces the depths of its lanes per
+// scene (nearly always one) before a single atomic
+__global__ __launch_bounds__(kBlock) void k_b_depth(const Seg* seg, int nseg, int total, const int* leaf_parent,
+                                                   int* depth) {
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = g < total;
+    int s = -1, d = 0;
+    if (valid) {
+        s = seg_of(seg, nseg, g);
+        const BvhNode* nodes = seg[s].nodes;
+        int p = leaf_parent[g] >> 1;
+        d = 1;
+        while (true) {
+            const int pp = reinterpret_cast<const int*>(nodes + p)[14];
+            if (pp < 0) break;
+            p = pp >> 1;
+            d++;
+        }
+    }
+    uint64_t todo = ballot64(valid);
+    while (todo) {
+        const int leader = __builtin_ctzll(todo);
+        const int s0 = __shfl(s, leader);
+        const bool mine = valid && s == s0;
+        int v = mine ? d : 0;
+        for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+        if (lane_id() == leader) atomicMax(&depth[s0], v);
+        todo &= ~ballot64(mine);
+    }
+}
+
+__global__ void k_b_fill(int* p, int n, int v) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int blocks_for(int64_t n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+// --------------------------------------------------------- batched render
+struct BScene {  // one scene of a batched render
+    SceneView sv;
+    mcpt_camera cam;
+    float* accum;
+    int foff;  // first pixel index of the scene across all scenes
+    int npix;
+};
+
+enum { C_NS0 = 0, C_NS1 = 32, C_WORK = 64, C_EXT = 96, C_SHD = 128, C_NCOUNTERS = 160 };
+
+__device__ __forceinline__ unsigned long long* ns_slot(int* c, int q) {
+    return reinterpret_cast<unsigned long long*>(c + (q ? C_NS1 : C_NS0));
+}
+__device__ __forceinline__ int queue_len(const int* c, int q) { return c[q ? C_NS1 : C_NS0]; }
+__device__ __forceinline__ int shadow_len(const int* c, int q) { return c[(q ? C_NS1 : C_NS0) + 1]; }
+
+__device__ __forceinline__ int scene_of_pixel(const BScene* tab, int nsc, int q) {
+    int lo = 0, hi = nsc - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].foff <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+struct BWs {
+    RayItem* q[2];
+    mcpt_packed_hit* hits;
+    ShadowItem* sq;
+    float4* thr;
+    float4* smp;
+    uint64_t* rng;
+    int* counters;
+};
+
+__global__ void k_bb_init(int* c, int count) {
+    if (threadIdx.x == 0) {
+        *ns_slot(c, 0) = (unsigned long long)(uint32_t)count;
+        *ns_slot(c, 1) = 0;
+        c[C_WORK] = 0;
+    }
+}
+__global__ void k_bb_bounce(int* c, int cur) {
+    if (threadIdx.x == 0) {
+        *ns_slot(c, 1 - cur) = 0;
+        c[C_WORK] = 0;
+        *reinterpret_cast<unsigned long long*>(c + C_EXT) += (unsigned long long)queue_len(c, cur);
+    }
+}
+__global__ void k_bb_shadow_total(int* c, int cur) {
+    if (threadIdx.x == 0) *reinterpret_cast<unsigned long long*>(c + C_SHD) += (unsigned long long)shadow_len(c, 1 - cur);
+}
+
+// generateCameraRaysKernel (:3342-3374) for every scene's pixels and the
+// pass's samples
+__global__ __launch_bounds__(kBlock) void k_bb_camera(const BScene* tab, int nsc, int P, uint32_t sample0, int nb,
+                                                     uint64_t seed, BWs ws) {
+    const int path = blockIdx.x * kBlock + threadIdx.x;
+    if (path >= P * nb) return;
+    const int k = path / P, q = path - k * P;
+    const int s = scene_of_pixel(tab, nsc, q);
+    const mcpt_camera& cam = tab[s].cam;
+    const int pixel = q - tab[s].foff;
+    Rng rng;
+    mcpt_pcg32_for_path(&rng.g, (uint32_t)pixel, sample0 + (uint32_t)k, seed);
+    vec3 o, d;
+    float time;
+    camera_ray(cam, pixel % cam.width, pixel / cam.width, rng, o, d, time);
+    (void)time;  // static scenes: no kernel reads it
+    RayItem it;
+    it.ot = make_float4(o.x, o.y, o.z, __int_as_float(s));
+    it.dp = make_float4(d.x, d.y, d.z, __int_as_float(path));
+    ws.q[0][path] = it;
+    ws.thr[path] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    ws.smp[path] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    ws.rng[path] = rng.g.state;
+}
+
+// persistent extend over the rays of every scene (extendKernelPersistent,
+// :3416-3468); a lane reloads the BVH view only when its scene changes
+__global__ __launch_bounds__(kBlock) void k_bb_extend(const BvhView* bvhs, BWs ws, int cur, int refill) {
+    extern __shared__ int lds_stack[];
+    int* stk = lds_stack + threadIdx.x;
+    const int total = queue_len(ws.counters, cur);
+    const int lane = lane_id();
+    Trav s;
+    BvhView bv;
+    int bv_scene = -1;
+    bool has = false;
+    int slot = -1;
+    bool exhausted = false;
+    uint32_t ns = 0, nl = 0;
+    while (true) {
+        if (!exhausted) {
+            const uint64_t idle = ballot64(!has);
+            const int nidle = __popcll(idle);
+            if (nidle >= refill) {
+                const int leader = __builtin_ctzll(idle);
+                int first = 0;
+                if (lane == leader) first = atomicAdd(&ws.counters[C_WORK], nidle);
+                first = __shfl(first, leader);
+                if (!has) {
+                    const int q = first + rank_below(idle);
+                    if (q < total) {
+                        const RayItem it = ws.q[cur][q];
+                        const int sc = __float_as_int(it.ot.w);
+                        if (sc != bv_scene) {
+                            bv = bvhs[sc];
+                            bv_scene = sc;
+                        }
+                        trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+                        slot = q;
+                        has = true;
+                    }
+                }
+                if (first + nidle >= total) exhausted = true;
+            }
+        }
+        if (exhausted && ballot64(has) == 0) break;
+        if (has && step_ordered<false>(bv, s, stk, kBlock, ns, nl)) {
+            mcpt_packed_hit h;
+            h.t = s.t;
+            h.prim_type = s.leaf >= 0 ? 1 : -1;
+            h.prim_index = s.leaf >= 0 ? bv.leaf_prim[s.leaf] : -1;
+            h.did_hit = s.leaf >= 0 ? 1 : 0;
+            ws.hits[slot] = h;
+            has = false;
+        }
+    }
+}
+
+// shadeKernel (:3470-3696) with the ray's own scene and camera
+__global__ __launch_bounds__(kBlock) void k_bb_shade(const BScene* tab, BWs ws, int cur, int depth, int P) {
+    const int slot = blockIdx.x * kBlock + threadIdx.x;
+    const bool active = slot < queue_len(ws.counters, cur);
+    Bounce out;
+    out.next = false;
+    out.shadow = false;
+    int path = 0, scene = 0;
+    if (active) {
+        const RayItem it = ws.q[cur][slot];
+        const mcpt_packed_hit hit = ws.hits[slot];
+        path = __float_as_int(it.dp.w);
+        scene = __float_as_int(it.ot.w);
+        const BScene& B = tab[scene];
+        const float4 thr = ws.thr[path], smp = ws.smp[path];
+        PathState ps;
+        ps.thr = vec3(thr.x, thr.y, thr.z);
+        ps.prev_pdf = thr.w;
+        ps.smp = vec3(smp.x, smp.y, smp.z);
+        ps.prev_spec = smp.w != 0.0f;
+        Rng rng;
+        rng.g.state = ws.rng[path];
+        const int pixel = path % P - B.foff;
+        rng.g.inc = ((uint64_t)(uint32_t)pixel << 1u) | 1u;
+        shade_bounce(B.sv, load3(B.cam.background), B.cam.rr_cap, depth, vec3(it.ot.x, it.ot.y, it.ot.z),
+                     vec3(it.dp.x, it.dp.y, it.dp.z), hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out);
+        ws.thr[path] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
+        ws.smp[path] = make_float4(ps.smp.x, ps.smp.y, ps.smp.z, ps.prev_spec ? 1.0f : 0.0f);
+        ws.rng[path] = rng.g.state;
+    }
+    int nslot, sslot;
+    wave_push2(ns_slot(ws.counters, 1 - cur), out.next, out.shadow, nslot, sslot);
+    if (out.next) {
+        RayItem it;
+        it.ot = make_float4(out.no.x, out.no.y, out.no.z, __int_as_float(scene));
+        it.dp = make_float4(out.nd.x, out.nd.y, out.nd.z, __int_as_float(path));
+        ws.q[1 - cur][nslot] = it;
+    }
+    if (out.shadow) {
+        ShadowItem si;
+        si.om = make_float4(out.so.x, out.so.y, out.so.z, out.smax);
+        si.dp = make_float4(out.sd.x, out.sd.y, out.sd.z, __int_as_float(path));
+        si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, __int_as_float(scene));
+        ws.sq[sslot] = si;
+    }
+}
+
+// shadowKernel (:3698-3728)
+__global__ __launch_bounds__(kBlock) void k_bb_shadow(const BScene* tab, BWs ws, int cur) {
+    extern __shared__ int lds_stack[];
+    const int slot = blockIdx.x * kBlock + threadIdx.x;
+    if (slot >= shadow_len(ws.counters, 1 - cur)) return;
+    const ShadowItem si = ws.sq[slot];
+    const int path = __float_as_int(si.dp.w);
+    const float tr = shadow_transmittance(tab[__float_as_int(si.c.w)].sv, vec3(si.om.x, si.om.y, si.om.z),
+                                          vec3(si.dp.x, si.dp.y, si.dp.z), si.om.w, lds_stack + threadIdx.x, kBlock);
+    if (tr > 0.0f) {
+        float4 v = ws.smp[path];
+        v.x = v.x + si.c.x * tr;
+        v.y = v.y + si.c.y * tr;
+        v.z = v.z + si.c.z * tr;
+        ws.smp[path] = v;
+    }
+}
+
+// finalizeSampleKernel (:3740-3755): a pixel's samples in sample order
+__global__ __launch_bounds__(kBlock) void k_bb_finalize(const BScene* tab, int nsc, int P, int nb, const float4* smp) {
+    const int q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= P) return;
+    const int s = scene_of_pixel(tab, nsc, q);
+    float* a = tab[s].accum + 3 * (q - tab[s].foff);
+    float r = a[0], g = a[1], b = a[2];
+    for (int k = 0; k < nb; k++) {
+        const float4 v = smp[(size_t)k * P + q];
+        const vec3 c = clamp_sample(vec3(v.x, v.y, v.z));
+        r = r + c.x;
+        g = g + c.y;
+        b = b + c.z;
+    }
+    a[0] = r;
+    a[1] = g;
+    a[2] = b;
+}
+
+constexpr size_t kPathBytes = 2 * sizeof(RayItem) + sizeof(mcpt_packed_hit) + sizeof(ShadowItem) + 2 * sizeof(float4) +
+                              sizeof(uint64_t);
+
+// One batched render. accum[i] overrides scene i's film (scratch films of
+// the autotuner); nullptr -> films.
+int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const mcpt_render_opts* o,
+                 float* const* accum, const mcpt_batch_tune* tune, mcpt_render_stats* stats) {
+    Ctx* ctx = scenes[0]->ctx;
+    hipStream_t st = ctx->stream;
+    std::vector<BScene> tab(n);
+    std::vector<BvhView> bvh(n);
+    int64_t P = 0;
+    int max_stack = 1;
+    for (int i = 0; i < n; i++) {
+        tab[i].sv = scenes[i]->view();
+        tab[i].cam = cams[i];
+        tab[i].accum = accum[i];
+        tab[i].foff = (int)P;
+        tab[i].npix = cams[i].width * cams[i].height;
+        bvh[i] = tab[i].sv.bvh;
+        P += tab[i].npix;
+        max_stack = std::max(max_stack, scenes[i]->max_depth);
+    }
+    if (P > (1 << 30)) return set_error(MCPT_ERR_INVALID, "too many pixels in one batch");
+    // samples of every scene per pass: the tuned value, or as many as fit a
+    // quarter of the budget (at most 64 Mi paths)
+    int nb = tune && tune->samples_per_pass > 0 ? tune->samples_per_pass : 0;
+    if (nb == 0) {
+        const int64_t cap = std::min<int64_t>(int64_t(64) << 20, (int64_t)(ctx->budget / 4) / (int64_t)kPathBytes);
+        nb = (int)std::max<int64_t>(1, cap / P);
+    }
+    nb = std::max(1, std::min(nb, o->spp));
+    if (P * nb > (int64_t(1) << 30)) nb = std::max(1, (int)((int64_t(1) << 30) / P));
+    const int refill = tune && tune->refill_lanes > 0 ? std::min(64, tune->refill_lanes) : 32;
+    const int64_t capacity = P * nb;
+    // device tables and workspace in one allocation
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t parts[] = {al(n * sizeof(BScene)),        al(n * sizeof(BvhView)),
+                            al(capacity * sizeof(RayItem)), al(capacity * sizeof(RayItem)),
+                            al(capacity * sizeof(mcpt_packed_hit)), al(capacity * sizeof(ShadowItem)),
+                            al(capacity * sizeof(float4)),  al(capacity * sizeof(float4)),
+                            al(capacity * sizeof(uint64_t)), al(C_NCOUNTERS * sizeof(int))};
+    size_t total = 0;
+    for (size_t x : parts) total += x;
+    void* mem = nullptr;
+    MCPT_TRY(dev_alloc(ctx, &mem, total));
+    char* b = (char*)mem;
+    int k = 0;
+    BScene* dtab = (BScene*)b;
+    b += parts[k++];
+    BvhView* dbvh = (BvhView*)b;
+    b += parts[k++];
+    BWs ws;
+    ws.q[0] = (RayItem*)b, b += parts[k++];
+    ws.q[1] = (RayItem*)b, b += parts[k++];
+    ws.hits = (mcpt_packed_hit*)b, b += parts[k++];
+    ws.sq = (ShadowItem*)b, b += parts[k++];
+    ws.thr = (float4*)b, b += parts[k++];
+    ws.smp = (float4*)b, b += parts[k++];
+    ws.rng = (uint64_t*)b, b += parts[k++];
+    ws.counters = (int*)b;
+    const size_t lds = (size_t)kBlock * max_stack * sizeof(int);
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bb_extend, kBlock, lds);
+    per_cu = std::max(1, per_cu);
+    if (tune && tune->blocks_per_cu > 0) per_cu = std::min(per_cu, tune->blocks_per_cu);
+    const int persistent_blocks = per_cu * ctx->cu_count;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemcpyAsync(dtab, tab.data(), n * sizeof(BScene), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dbvh, bvh.data(), n * sizeof(BvhView), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    uint64_t launches = 0;
+    if (e == hipSuccess) {
+        k_b_fill<<<1, C_NCOUNTERS, 0, st>>>(ws.counters, C_NCOUNTERS, 0);
+        for (int k0 = 0; k0 < o->spp; k0 += nb) {
+            const int cnt = std::min(nb, o->spp - k0);
+            const int64_t paths = P * cnt;
+            const int grid = blocks_for(paths);
+            int cur = 0;
+            k_bb_init<<<1, 64, 0, st>>>(ws.counters, (int)paths);
+            k_bb_camera<<<grid, kBlock, 0, st>>>(dtab, n, (int)P, (uint32_t)(o->sample_begin + k0), cnt, o->seed, ws);
+            for (int d = 0; d < o->max_depth; d++) {
+                k_bb_bounce<<<1, 64, 0, st>>>(ws.counters, cur);
+                k_bb_extend<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(dbvh, ws, cur, refill);
+                k_bb_shade<<<grid, kBlock, 0, st>>>(dtab, ws, cur, d, (int)P);
+                k_bb_shadow_total<<<1, 64, 0, st>>>(ws.counters, cur);
+                k_bb_shadow<<<grid, kBlock, lds, st>>>(dtab, ws, cur);
+                launches++;
+                cur = 1 - cur;
+            }
+            k_bb_finalize<<<blocks_for(P), kBlock, 0, st>>>(dtab, n, (int)P, cnt, ws.smp);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    uint64_t tot[2] = {0, 0};
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess) e = hipMemcpy(&tot[0], ws.counters + C_EXT, sizeof(uint64_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&tot[1], ws.counters + C_SHD, sizeof(uint64_t), hipMemcpyDeviceToHost);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dev_free(ctx, mem, total);
+    if (e != hipSuccess) return set_error(MCPT_ERR_HIP, "batched render: %s", hipGetErrorString(e));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->total_ms = ms;
+        stats->extend_launches = launches;
+        stats->extend_rays = tot[0];
+        stats->shadow_rays = tot[1];
+        stats->paths = (uint64_t)P * o->spp;
+    }
+    return MCPT_OK;
+}
+
+int check_batch(const mcpt_scene* scenes, int32_t n) {
+    if (!scenes || n < 1) return set_error(MCPT_ERR_INVALID, "no scenes");
+    for (int i = 0; i < n; i++)
+        if (!scenes[i] || scenes[i]->ctx != scenes[0]->ctx)
+            return set_error(MCPT_ERR_INVALID, "scene %d is missing or belongs to another context", i);
+    return MCPT_OK;
+}
+
+int check_render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int32_t n, const mcpt_render_opts* o) {
+    MCPT_TRY(check_batch(scenes, n));
+    if (!cams || !o) return set_error(MCPT_ERR_INVALID, "bad arguments");
+    if (o->mode != MCPT_MODE_PERSISTENT) return set_error(MCPT_ERR_INVALID, "batched renders use the persistent mode");
+    if (o->spp < 1 || o->max_depth < 1 || o->sample_begin < 0)
+        return set_error(MCPT_ERR_INVALID, "spp, max_depth must be >= 1 and sample_begin >= 0");
+    for (int i = 0; i < n; i++) {
+        if (!scenes[i]->built) return set_error(MCPT_ERR_INVALID, "scene %d has no BVH", i);
+        if (cams[i].width < 1 || cams[i].height < 1) return set_error(MCPT_ERR_INVALID, "camera %d is empty", i);
+    }
+    return MCPT_OK;
+}
+
+}  // namespace
+}  // namespace mcpt
+
+using namespace mcpt;
+
+extern "C" {
+
+int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) {
+    MCPT_TRY(check_batch(scenes, n));
+    Ctx* ctx = scenes[0]->ctx;
+    MCPT_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    // scenes with one triangle take the single-scene path (its special root)
+    std::vector<int> multi;
+    float single_ms = 0.f;
+    for (int i = 0; i < n; i++) {
+        if (scenes[i]->n < 1) return set_error(MCPT_ERR_INVALID, "scene %d is empty", i);
+        if (scenes[i]->n == 1) {
+            MCPT_TRY(lbvh_build(scenes[i], nullptr));
+            single_ms += scenes[i]->last_build_ms;
+        } else {
+            multi.push_back(i);
+        }
+    }
+    if (multi.empty()) {
+        if (build_ms) *build_ms = single_ms;
+        return MCPT_OK;
+    }
+    const int ns = (int)multi.size();
+    std::vector<Seg> seg(ns);
+    std::vector<int> chunk_off(ns + 1, 0);
+    int64_t total = 0, total_int = 0;
+    for (int k = 0; k < ns; k++) {
+        mcpt_scene_s* s = scenes[multi[k]];
+        MCPT_TRY(s->nodes.alloc(ctx, s->n - 1));
+        MCPT_TRY(s->leaf_tri.alloc(ctx, s->n));
+        MCPT_TRY(s->leaf_prim.alloc(ctx, s->n));
+        MCPT_TRY(s->root_box.alloc(ctx, 6));
+        seg[k] = Seg{(int)total, s->n, (int)total_int, 0, s->hot.ptr, s->nodes.ptr, s->leaf_tri.ptr, s->leaf_prim.ptr,
+                     s->root_box.ptr};
+        total += s->n;
+        total_int += s->n - 1;
+        chunk_off[k + 1] = chunk_off[k] + (s->n + kChunk - 1) / kChunk;
+    }
+    const int n_chunks = chunk_off[ns];
+    if (total > (int64_t(1) << 30)) return set_error(MCPT_ERR_INVALID, "too many triangles in one batch");
+    const int tiles = blocks_for(total, kTile);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t parts[] = {al(ns * sizeof(Seg)),         al(ns * 6 * sizeof(float)),  al(total * sizeof(uint32_t)),
+                            al(total * sizeof(uint32_t)), al(total * sizeof(uint32_t)), al(total * sizeof(int)),
+                            al(total * sizeof(int)),      al((size_t)256 * tiles * sizeof(int)),
+                            al(total * sizeof(int)),      al(total_int * sizeof(int)),  al(ns * sizeof(int)),
+                            al((ns + 1) * sizeof(int)),   al((size_t)n_chunks * 6 * sizeof(float))};
+    size_t bytes = 0;
+    for (size_t x : parts) bytes += x;
+    void* mem = nullptr;
+    MCPT_TRY(dev_alloc(ctx, &mem, bytes));
+    char* b = (char*)mem;
+    int k = 0;
+    Seg* dseg = (Seg*)b;
+    b += parts[k++];
+    float* sb = (float*)b;
+    b += parts[k++];
+    uint32_t* morton = (uint32_t*)b;
+    b += parts[k++];
+    uint32_t* keys_a = (uint32_t*)b;
+    b += parts[k++];
+    uint32_t* keys_b = (uint32_t*)b;
+    b += parts[k++];
+    int* vals_a = (int*)b;
+    b += parts[k++];
+    int* vals_b = (int*)b;
+    b += parts[k++];
+    int* counts = (int*)b;
+    b += parts[k++];
+    int* leaf_parent = (int*)b;
+    b += parts[k++];
+    int* counters = (int*)b;
+    b += parts[k++];
+    int* depth = (int*)b;
+    b += parts[k++];
+    int* dchunk = (int*)b;
+    b += parts[k++];
+    float* partial = (float*)b;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemcpyAsync(dseg, seg.data(), ns * sizeof(Seg), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dchunk, chunk_off.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    if (e == hipSuccess) {
+        const int n_all = (int)total, n_int = (int)total_int;
+        k_b_bounds_chunks<<<n_chunks, kBlock, 0, st>>>(dseg, ns, dchunk, partial);
+        k_b_bounds_final<<<ns, kBlock, 0, st>>>(dchunk, partial, sb);
+        k_b_morton<<<blocks_for(n_all), kBlock, 0, st>>>(dseg, ns, n_all, sb, morton, keys_a, vals_a);
+        uint32_t *kin = keys_a, *kout = keys_b;
+        int *vin = vals_a, *vout = vals_b;
+        for (int shift = 0; shift < 30; shift += 8) {
+            k_radix_hist<<<tiles, kBlock, 0, st>>>(kin, n_all, shift, tiles, counts);
+            k_scan_exclusive<<<1, 1024, 0, st>>>(counts, 256 * tiles);
+            k_radix_scatter<<<tiles, kBlock, 0, st>>>(kin, vin, kout, vout, n_all, shift, tiles, counts);
+            std::swap(kin, kout);
+            std::swap(vin, vout);
+        }
+        // then the scene index, 8 bits per pass (segments stay in scene order)
+        for (int shift = 0; (ns - 1) >> shift; shift += 8) {
+            k_b_scene_keys<<<blocks_for(n_all), kBlock, 0, st>>>(dseg, ns, n_all, vin, kin);
+            k_radix_hist<<<tiles, kBlock, 0, st>>>(kin, n_all, shift, tiles, counts);
+            k_scan_exclusive<<<1, 1024, 0, st>>>(counts, 256 * tiles);
+            k_radix_scatter<<<tiles, kBlock, 0, st>>>(kin, vin, kout, vout, n_all, shift, tiles, counts);
+            std::swap(kin, kout);
+            std::swap(vin, vout);
+        }
+        k_b_gather_keys<<<blocks_for(n_all), kBlock, 0, st>>>(morton, vin, n_all, kin);
+        k_b_leaves<<<blocks_for(n_all), kBlock, 0, st>>>(dseg, ns, n_all, vin);
+        k_b_fill<<<blocks_for(n_int), kBlock, 0, st>>>(counters, n_int, 0);
+        k_b_fill<<<blocks_for(ns), kBlock, 0, st>>>(depth, ns, 0);
+        k_b_karras<<<blocks_for(n_int), kBlock, 0, st>>>(dseg, ns, n_int, kin, leaf_parent);
+        k_b_refit<<<blocks_for(n_all), kBlock, 0, st>>>(dseg, ns, n_all, leaf_parent, counters);
+        k_b_depth<<<blocks_for(n_all), kBlock, 0, st>>>(dseg, ns, n_all, leaf_parent, depth);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    std::vector<int> depths(ns, 0);
+    if (e == hipSuccess) e = hipMemcpyAsync(depths.data(), depth, ns * sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dev_free(ctx, mem, bytes);
+    if (e != hipSuccess) return set_error(MCPT_ERR_HIP, "batched LBVH build: %s", hipGetErrorString(e));
+    for (int k2 = 0; k2 < ns; k2++) {
+        mcpt_scene_s* s = scenes[multi[k2]];
+        if (depths[k2] < 1 || depths[k2] > 64)
+            return set_error(MCPT_ERR_INVALID, "scene %d: BVH depth %d outside [1, 64]", multi[k2], depths[k2]);
+        s->max_depth = depths[k2];
+        s->built = true;
+        s->last_build_ms = ms;
+    }
+    if (build_ms) *build_ms = ms + single_ms;
+    return MCPT_OK;
+}
+
+int mcpt_render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int32_t n, const mcpt_render_opts* o,
+                      mcpt_film* films, const mcpt_batch_tune* tune, mcpt_render_stats* stats) {
+    MCPT_TRY(check_render_batch(scenes, cams, n, o));
+    if (!films) return set_error(MCPT_ERR_INVALID, "films is NULL");
+    std::vector<float*> acc(n);
+    for (int i = 0; i < n; i++) {
+        if (!films[i] || films[i]->w != cams[i].width || films[i]->h != cams[i].height)
+            return set_error(MCPT_ERR_INVALID, "film %d missing or not %dx%d", i, cams[i].width, cams[i].height);
+        acc[i] = films[i]->accum.ptr;
+    }
+    MCPT_HIP(hipSetDevice(scenes[0]->ctx->device));
+    return render_batch(scenes, cams, n, o, acc.data(), tune, stats);
+}
+
+int mcpt_batch_autotune(const mcpt_scene* scenes, const mcpt_camera* cams, int32_t n, const mcpt_render_opts* o,
+                        mcpt_batch_tune* best, double* search_ms) {
+    MCPT_TRY(check_render_batch(scenes, cams, n, o));
+    if (!best) return set_error(MCPT_ERR_INVALID, "best is NULL");
+    Ctx* ctx = scenes[0]->ctx;
+    MCPT_HIP(hipSetDevice(ctx->device));
+    // scratch films (the calibration passes use the scenes' samples after
+    // the requested range, so they draw the same kind of paths)
+    int64_t P = 0;
+    for (int i = 0; i < n; i++) P += (int64_t)cams[i].width * cams[i].height;
+    void* scratch = nullptr;
+    MCPT_TRY(dev_alloc(ctx, &scratch, (size_t)P * 3 * sizeof(float)));
+    std::vector<float*> acc(n);
+    {
+        int64_t off = 0;
+        for (int i = 0; i < n; i++) {
+            acc[i] = (float*)scratch + 3 * off;
+            off += (int64_t)cams[i].width * cams[i].height;
+        }
+    }
+    mcpt_render_opts cal = *o;
+    cal.sample_begin = o->sample_begin + o->spp;
+    cal.spp = 1;
+    double spent = 0.0;
+    auto trial = [&](const mcpt_batch_tune& t, double& per_path_ns) {
+        mcpt_render_stats rs;
+        const int s = render_batch(scenes, cams, n, &cal, acc.data(), &t, &rs);
+        if (s < 0) return s;
+        spent += rs.total_ms;
+        per_path_ns = rs.total_ms * 1e6 / (double)rs.paths;
+        return MCPT_OK;
+    };
+    // Queue capacity: every sample of every scene in one pass when the
+    // budget holds it (a pass ends with its slowest rays, so fewer, fuller
+    // passes are faster), otherwise as many samples as fit.
+    const int64_t cap = std::min<int64_t>(int64_t(64) << 20, (int64_t)(ctx->budget / 4) / (int64_t)kPathBytes);
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(o->spp, cap / std::max<int64_t>(1, P)));
+    // Launch shape, timed on one-sample passes: the refill threshold, then
+    // the persistent grid, each kept when it lowers the device time per path.
+    mcpt_batch_tune cur = {1, 32, 0, 0};
+    double cur_ns = 0.0;
+    int s = trial(cur, cur_ns);  // warm-up (allocation, caches)
+    if (s >= 0) s = trial(cur, cur_ns);
+    const int refill_c[] = {16, 48};
+    const int blocks_c[] = {4, 2};
+    for (int v : refill_c) {
+        if (s < 0) break;
+        mcpt_batch_tune t = cur;
+        t.refill_lanes = v;
+        double ns = 0.0;
+        s = trial(t, ns);
+        if (s >= 0 && ns < cur_ns) cur = t, cur_ns = ns;
+    }
+    for (int v : blocks_c) {
+        if (s < 0) break;
+        mcpt_batch_tune t = cur;
+        t.blocks_per_cu = v;
+        double ns = 0.0;
+        s = trial(t, ns);
+        if (s >= 0 && ns < cur_ns) cur = t, cur_ns = ns;
+    }
+    cur.samples_per_pass = nb;
+    dev_free(ctx, scratch, (size_t)P * 3 * sizeof(float));
+    if (s < 0) return s;
+    *best = cur;
+    if (search_ms) *search_ms = spent;
+    return MCPT_OK;
+}
+
+}  // extern "C"

@@ -36,6 +36,7 @@ def test_record_layouts(mcpt):
     assert C.sizeof(mcpt.ServerOpts) == 32 and C.sizeof(mcpt.Job) == 136
     assert C.sizeof(mcpt.JobResult) == 48 and C.sizeof(mcpt.ServerStats) == 88
     assert C.sizeof(mcpt.CacheOpts) == 24 and C.sizeof(mcpt.CacheStats) == 72
+    assert C.sizeof(mcpt.BatchTune) == 16
     assert mcpt.HOT_DTYPE.itemsize == 36 and mcpt.COLD_DTYPE.itemsize == 64
 
 
