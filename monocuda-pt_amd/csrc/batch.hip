@@ -6,12 +6,13 @@
 // applied per segment.
 //
 // Build: the triangles of scene s are the global indices [off_s, off_s+n_s).
-// Bounds are reduced per scene (one block each), Morton codes use their own
+// Bounds are reduced per scene (chunks, then one block per scene), Morton codes use their own
 // scene's bounds, and the stable LSD radix sort takes four 8-bit passes over
 // the codes and a fifth over the scene index: the result is ordered by
 // (scene, code, index), i.e. each scene's segment is exactly the (code,
-// index) order of its own build. Leaves, the Karras tree, the refit and the
-// depth then run over all segments in one launch each, writing every
+// index) order of its own build. Leaves, the Karras tree and the node levels
+// then run over all segments in one launch each, and the refit in one launch
+// per level over the nodes of that level in every scene, writing each
 // scene's own node, leaf and box buffers, so each BVH is bit-identical to
 // mcpt_lbvh_build of that scene alone.
 //
@@ -214,48 +215,20 @@ __global__ __launch_bounds__(kBlock) void k_b_karras(const Seg* seg, int nseg, i
     rf[3] = j;
 }
 
-// refitBoundsKernel (:1398-1415) per scene; counters indexed by global node
-__global__ __launch_bounds__(kBlock) void k_b_refit(const Seg* seg, int nseg, int total, const int* leaf_parent,
-                                                   int* counters) {
-    const int g = blockIdx.x * kBlock + threadIdx.x;
-    if (g >= total) return;
-    const Seg S = seg[seg_of(seg, nseg, g)];
-    int p = leaf_parent[g] >> 1;
-    while (true) {
-        const int old = refit_arrive(&counters[S.ioff + p]);
-        if (old == 0) return;
-        const float* f = reinterpret_cast<const float*>(S.nodes + p);
-        float b[6];
-        for (int k = 0; k < 3; k++) b[k] = fminf(f[k], f[6 + k]);
-        for (int k = 3; k < 6; k++) b[k] = fmaxf(f[k], f[6 + k]);
-        const int pp = reinterpret_cast<const int*>(S.nodes + p)[14];
-        if (pp < 0) {
-            for (int k = 0; k < 6; k++) S.root_box[k] = b[k];
-            return;
-        }
-        store_slot(S.nodes, pp >> 1, pp & 1, b);
-        p = pp >> 1;
-    }
-}
-
-// deepest leaf of each scene: a wave reduces the depths of its lanes per
-// scene (nearly always one) before a single atomic
-__global__ __launch_bounds__(kBlock) void k_b_depth(const Seg* seg, int nseg, int total, const int* leaf_parent,
-                                                   int* depth) {
+// Levels of every scene's internal nodes (keys and ids by global node index)
+// and each scene's deepest leaf: a wave reduces its lanes' depths per scene
+// (nearly always one) before a single atomic.
+__global__ __launch_bounds__(kBlock) void k_b_levels(const Seg* seg, int nseg, int total, const int* leaf_parent,
+                                                    uint32_t* level, int* ids, int* depth) {
     const int g = blockIdx.x * kBlock + threadIdx.x;
     const bool valid = g < total;
     int s = -1, d = 0;
     if (valid) {
         s = seg_of(seg, nseg, g);
-        const BvhNode* nodes = seg[s].nodes;
-        int p = leaf_parent[g] >> 1;
-        d = 1;
-        while (true) {
-            const int pp = reinterpret_cast<const int*>(nodes + p)[14];
-            if (pp < 0) break;
-            p = pp >> 1;
-            d++;
-        }
+        const Seg& S = seg[s];
+        const int j = g - S.off;
+        if (j < S.n - 1) ids[S.ioff + j] = S.ioff + j;
+        d = leaf_levels(S.nodes, leaf_parent[g], level + S.ioff);
     }
     uint64_t todo = ballot64(valid);
     while (todo) {
@@ -267,6 +240,15 @@ __global__ __launch_bounds__(kBlock) void k_b_depth(const Seg* seg, int nseg, in
         if (lane_id() == leader) atomicMax(&depth[s0], v);
         todo &= ~ballot64(mine);
     }
+}
+
+// refitBoundsKernel (:1398-1415) for the nodes of one level, all scenes
+__global__ __launch_bounds__(kBlock) void k_b_refit_level(const Seg* seg, int nseg, const int* ids, int cnt) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= cnt) return;
+    const int q = ids[i];
+    const Seg& S = seg[seg_of_node(seg, nseg, q)];
+    refit_node(S.nodes, q - S.ioff, S.root_box);
 }
 
 __global__ void k_b_fill(int* p, int n, int v) {
@@ -683,8 +665,9 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
     const size_t parts[] = {al(ns * sizeof(Seg)),         al(ns * 6 * sizeof(float)),  al(total * sizeof(uint32_t)),
                             al(total * sizeof(uint32_t)), al(total * sizeof(uint32_t)), al(total * sizeof(int)),
                             al(total * sizeof(int)),      al((size_t)256 * tiles * sizeof(int)),
-                            al(total * sizeof(int)),      al(total_int * sizeof(int)),  al(ns * sizeof(int)),
-                            al((ns + 1) * sizeof(int)),   al((size_t)n_chunks * 6 * sizeof(float))};
+                            al(total * sizeof(int)),      al(total_int * sizeof(int)),  al((ns + 65) * sizeof(int)),
+                            al((ns + 1) * sizeof(int)),   al((size_t)n_chunks * 6 * sizeof(float)),
+                            al(total_int * sizeof(uint32_t)), al(scan_chunks(256 * (int64_t)tiles) * sizeof(int))};
     size_t bytes = 0;
     for (size_t x : parts) bytes += x;
     void* mem = nullptr;
@@ -709,13 +692,17 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
     b += parts[k++];
     int* leaf_parent = (int*)b;
     b += parts[k++];
-    int* counters = (int*)b;
+    int* ids_sorted = (int*)b;
     b += parts[k++];
-    int* depth = (int*)b;
+    int* depth = (int*)b;  // per-scene depth, then the 65 level offsets
     b += parts[k++];
     int* dchunk = (int*)b;
     b += parts[k++];
     float* partial = (float*)b;
+    b += parts[k++];
+    uint32_t* lvl_sorted = (uint32_t*)b;
+    b += parts[k++];
+    int* ssum = (int*)b;  // chunk sums of the histogram scans
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
@@ -732,7 +719,7 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
         int *vin = vals_a, *vout = vals_b;
         for (int shift = 0; shift < 30; shift += 8) {
             k_radix_hist<<<tiles, kBlock, 0, st>>>(kin, n_all, shift, tiles, counts);
-            k_scan_exclusive<<<1, 1024, 0, st>>>(counts, 256 * tiles);
+            scan_exclusive(counts, 256 * tiles, ssum, st);
             k_radix_scatter<<<tiles, kBlock, 0, st>>>(kin, vin, kout, vout, n_all, shift, tiles, counts);
             std::swap(kin, kout);
             std::swap(vin, vout);
@@ -741,34 +728,54 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
         for (int shift = 0; (ns - 1) >> shift; shift += 8) {
             k_b_scene_keys<<<blocks_for(n_all), kBlock, 0, st>>>(dseg, ns, n_all, vin, kin);
             k_radix_hist<<<tiles, kBlock, 0, st>>>(kin, n_all, shift, tiles, counts);
-            k_scan_exclusive<<<1, 1024, 0, st>>>(counts, 256 * tiles);
+            scan_exclusive(counts, 256 * tiles, ssum, st);
             k_radix_scatter<<<tiles, kBlock, 0, st>>>(kin, vin, kout, vout, n_all, shift, tiles, counts);
             std::swap(kin, kout);
             std::swap(vin, vout);
         }
         k_b_gather_keys<<<blocks_for(n_all), kBlock, 0, st>>>(morton, vin, n_all, kin);
         k_b_leaves<<<blocks_for(n_all), kBlock, 0, st>>>(dseg, ns, n_all, vin);
-        k_b_fill<<<blocks_for(n_int), kBlock, 0, st>>>(counters, n_int, 0);
         k_b_fill<<<blocks_for(ns), kBlock, 0, st>>>(depth, ns, 0);
         k_b_karras<<<blocks_for(n_int), kBlock, 0, st>>>(dseg, ns, n_int, kin, leaf_parent);
-        k_b_refit<<<blocks_for(n_all), kBlock, 0, st>>>(dseg, ns, n_all, leaf_parent, counters);
-        k_b_depth<<<blocks_for(n_all), kBlock, 0, st>>>(dseg, ns, n_all, leaf_parent, depth);
+        // refit level by level over all scenes: levels into kout/vout (free
+        // after the sort), one radix pass by level, offsets to the host
+        const int itiles = blocks_for(n_int, kTile);
+        k_b_levels<<<blocks_for(n_all), kBlock, 0, st>>>(dseg, ns, n_all, leaf_parent, kout, vout, depth);
+        k_radix_hist<<<itiles, kBlock, 0, st>>>(kout, n_int, 0, itiles, counts);
+        scan_exclusive(counts, 256 * itiles, ssum, st);
+        k_radix_scatter<<<itiles, kBlock, 0, st>>>(kout, vout, lvl_sorted, ids_sorted, n_int, 0, itiles, counts);
+        k_level_offsets<<<1, 128, 0, st>>>(counts, itiles, n_int, depth + ns);
+        e = hipGetLastError();
+    }
+    std::vector<int> depths(ns + 65, 0);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(depths.data(), depth, (ns + 65) * sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    int bad = -1, max_d = 0;
+    for (int k2 = 0; k2 < ns && e == hipSuccess; k2++) {
+        if (depths[k2] < 1 || depths[k2] > 64) bad = k2;
+        max_d = std::max(max_d, depths[k2]);
+    }
+    if (e == hipSuccess && bad < 0) {
+        const int* off = depths.data() + ns;
+        for (int L = max_d - 1; L >= 0; L--)
+            if (off[L + 1] > off[L])
+                k_b_refit_level<<<blocks_for(off[L + 1] - off[L]), kBlock, 0, st>>>(dseg, ns, ids_sorted + off[L],
+                                                                                    off[L + 1] - off[L]);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(e1, st);
-    std::vector<int> depths(ns, 0);
-    if (e == hipSuccess) e = hipMemcpyAsync(depths.data(), depth, ns * sizeof(int), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     float ms = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess && bad < 0) e = hipEventElapsedTime(&ms, e0, e1);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     dev_free(ctx, mem, bytes);
     if (e != hipSuccess) return set_error(MCPT_ERR_HIP, "batched LBVH build: %s", hipGetErrorString(e));
+    if (bad >= 0)
+        return set_error(MCPT_ERR_INVALID, "scene %d: BVH depth %d outside [1, 64]", multi[bad], depths[bad]);
     for (int k2 = 0; k2 < ns; k2++) {
         mcpt_scene_s* s = scenes[multi[k2]];
-        if (depths[k2] < 1 || depths[k2] > 64)
-            return set_error(MCPT_ERR_INVALID, "scene %d: BVH depth %d outside [1, 64]", multi[k2], depths[k2]);
         s->max_depth = depths[k2];
         s->built = true;
         s->last_build_ms = ms;

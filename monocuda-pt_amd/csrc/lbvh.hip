@@ -13,10 +13,10 @@
 //   4. Karras radix tree (delta/determineRange/findSplit, :1337-1386,
 //      :1417-1447); each internal node records its children's references,
 //      and a leaf child's box is written straight into its parent's slot
-//   5. bottom-up refit: one thread per leaf climbs, an agent-scope acq_rel
-//      atomic per node lets the second arriver merge both child boxes
-//      (refitBoundsKernel, :1398-1415) and write the result into its own
-//      parent's slot.
+//   5. refit (refitBoundsKernel, :1398-1415) level by level: node levels
+//      from the leaves' climbs, one radix pass orders the nodes by level,
+//      then one launch per level, deepest first, merges each node's child
+//      boxes into its own slot in the parent.
 #include <algorithm>
 #include <cstring>
 
@@ -149,41 +149,24 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* keys, int n, 
     rf[3] = j;  // the node covers leaves [min(idx, j), max(idx, j)]
 }
 
-// refitBoundsKernel (:1398-1415) on the two-box node layout
-__global__ __launch_bounds__(kBlock) void k_refit(BvhNode* nodes, const int* leaf_parent, int n, int* counters,
-                                                  float* root_box) {
-    int j = blockIdx.x * kBlock + threadIdx.x;
-    if (j >= n) return;
-    int p = leaf_parent[j] >> 1;
-    while (true) {
-        const int old = refit_arrive(&counters[p]);
-        if (old == 0) return;
-        const float* f = reinterpret_cast<const float*>(nodes + p);
-        float b[6];
-        for (int k = 0; k < 3; k++) b[k] = fminf(f[k], f[6 + k]);
-        for (int k = 3; k < 6; k++) b[k] = fmaxf(f[k], f[6 + k]);
-        int pp = reinterpret_cast<const int*>(nodes + p)[14];
-        if (pp < 0) {
-            for (int k = 0; k < 6; k++) root_box[k] = b[k];
-            return;
-        }
-        store_slot(nodes, pp >> 1, pp & 1, b);
-        p = pp >> 1;
+// 5a. level of every internal node and the deepest leaf (one atomic per
+// wave); node ids beside the levels for the sort by level
+__global__ __launch_bounds__(kBlock) void k_levels(const BvhNode* nodes, const int* leaf_parent, int n,
+                                                   uint32_t* level, int* ids, int* max_depth) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    int d = 0;
+    if (j < n) {
+        if (j < n - 1) ids[j] = j;
+        d = leaf_levels(nodes, leaf_parent[j], level);
     }
+    for (int off = 32; off > 0; off >>= 1) d = max(d, __shfl_xor(d, off));
+    if (lane_id() == 0 && d > 0) atomicMax(max_depth, d);
 }
 
-__global__ __launch_bounds__(kBlock) void k_depth(const BvhNode* nodes, const int* leaf_parent, int n, int* max_depth) {
-    int j = blockIdx.x * kBlock + threadIdx.x;
-    if (j >= n) return;
-    int d = 1;
-    int p = leaf_parent[j] >> 1;
-    while (true) {
-        int pp = reinterpret_cast<const int*>(nodes + p)[14];
-        if (pp < 0) break;
-        p = pp >> 1;
-        d++;
-    }
-    atomicMax(max_depth, d);
+// 5b. refitBoundsKernel (:1398-1415) for the nodes of one level
+__global__ __launch_bounds__(kBlock) void k_refit_level(BvhNode* nodes, const int* ids, int cnt, float* root_box) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < cnt) refit_node(nodes, ids[i], root_box);
 }
 
 __global__ void k_fill_int(int* p, int n, int v) {
@@ -223,7 +206,8 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
     const int n_partial = std::min(1024, blocks_for(n));
     DevBuf<float> partial, sb;
     DevBuf<uint32_t> morton, keys_a, keys_b;
-    DevBuf<int> vals_a, vals_b, counts, leaf_parent, counters, depth;
+    DevBuf<uint32_t> lvl_sorted;
+    DevBuf<int> vals_a, vals_b, counts, ssum, leaf_parent, ids_sorted, depth;
     int status = MCPT_OK;
     auto fail = [&](int code) {
         partial.release(ctx);
@@ -234,8 +218,10 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
         vals_a.release(ctx);
         vals_b.release(ctx);
         counts.release(ctx);
+        ssum.release(ctx);
         leaf_parent.release(ctx);
-        counters.release(ctx);
+        lvl_sorted.release(ctx);
+        ids_sorted.release(ctx);
         depth.release(ctx);
         hipEventDestroy(e0);
         hipEventDestroy(e1);
@@ -260,9 +246,11 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
     STEP(vals_a.alloc(ctx, n));
     STEP(vals_b.alloc(ctx, n));
     STEP(counts.alloc(ctx, (size_t)256 * tiles));
+    STEP(ssum.alloc(ctx, scan_chunks(256 * (int64_t)tiles)));
     STEP(leaf_parent.alloc(ctx, n));
-    STEP(counters.alloc(ctx, n_int));
-    STEP(depth.alloc(ctx, 1));
+    STEP(lvl_sorted.alloc(ctx, n_int));
+    STEP(ids_sorted.alloc(ctx, n_int));
+    STEP(depth.alloc(ctx, 66));  // max depth, then the 65 level offsets
 
     // 1-2. bounds and Morton codes
     k_bounds_partial<<<n_partial, kBlock, 0, st>>>(s->hot.ptr, n, partial.ptr);
@@ -276,7 +264,7 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
     int* vout = vals_b.ptr;
     for (int shift = 0; shift < 30; shift += 8) {
         k_radix_hist<<<tiles, kBlock, 0, st>>>(kin, n, shift, tiles, counts.ptr);
-        k_scan_exclusive<<<1, 1024, 0, st>>>(counts.ptr, 256 * tiles);
+        scan_exclusive(counts.ptr, 256 * tiles, ssum.ptr, st);
         k_radix_scatter<<<tiles, kBlock, 0, st>>>(kin, vin, kout, vout, n, shift, tiles, counts.ptr);
         std::swap(kin, kout);
         std::swap(vin, vout);
@@ -284,7 +272,6 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
     HSTEP(hipGetLastError());
     // 4. leaves and the Karras tree
     k_leaves<<<blocks_for(n), kBlock, 0, st>>>(s->hot.ptr, vin, n, s->leaf_tri.ptr, s->leaf_prim.ptr);
-    k_fill_int<<<blocks_for(n_int), kBlock, 0, st>>>(counters.ptr, n_int, 0);
     k_fill_int<<<1, 64, 0, st>>>(depth.ptr, 1, 0);
     if (n == 1) {
         // One primitive: no internal node exists in the reference. The root
@@ -312,14 +299,30 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
         s->max_depth = 1;
     } else {
         k_karras<<<blocks_for(n - 1), kBlock, 0, st>>>(kin, n, s->leaf_tri.ptr, s->nodes.ptr, leaf_parent.ptr);
-        // 5. refit
-        k_refit<<<blocks_for(n), kBlock, 0, st>>>(s->nodes.ptr, leaf_parent.ptr, n, counters.ptr, s->root_box.ptr);
-        k_depth<<<blocks_for(n), kBlock, 0, st>>>(s->nodes.ptr, leaf_parent.ptr, n, depth.ptr);
+        // 5. refit: levels (keys into kout, ids into vout, both free after
+        // the sort), one radix pass by level, then one launch per level
+        const int ni = n - 1, itiles = blocks_for(ni, kTile);
+        k_levels<<<blocks_for(n), kBlock, 0, st>>>(s->nodes.ptr, leaf_parent.ptr, n, kout, vout, depth.ptr);
+        k_radix_hist<<<itiles, kBlock, 0, st>>>(kout, ni, 0, itiles, counts.ptr);
+        scan_exclusive(counts.ptr, 256 * itiles, ssum.ptr, st);
+        k_radix_scatter<<<itiles, kBlock, 0, st>>>(kout, vout, lvl_sorted.ptr, ids_sorted.ptr, ni, 0, itiles,
+                                                   counts.ptr);
+        k_level_offsets<<<1, 128, 0, st>>>(counts.ptr, itiles, ni, depth.ptr + 1);
         HSTEP(hipGetLastError());
-        int md = 0;
-        HSTEP(hipMemcpyAsync(&md, depth.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+        int hd[66];
+        HSTEP(hipMemcpyAsync(hd, depth.ptr, sizeof(hd), hipMemcpyDeviceToHost, st));
         HSTEP(hipStreamSynchronize(st));
-        s->max_depth = md;
+        s->max_depth = hd[0];
+        if (s->max_depth < 1 || s->max_depth > 64)
+            return fail(set_error(MCPT_ERR_INVALID, "BVH depth %d outside [1, 64]", s->max_depth));
+        const int* off = hd + 1;
+        for (int L = s->max_depth - 1; L >= 0; L--)
+            if (off[L + 1] > off[L])
+                k_refit_level<<<blocks_for(off[L + 1] - off[L]), kBlock, 0, st>>>(s->nodes.ptr,
+                                                                                  ids_sorted.ptr + off[L],
+                                                                                  off[L + 1] - off[L],
+                                                                                  s->root_box.ptr);
+        HSTEP(hipGetLastError());
     }
     HSTEP(hipEventRecord(e1, st));
     HSTEP(hipEventSynchronize(e1));

@@ -1,8 +1,9 @@
 // lbvh_common.hpp — pieces of the LBVH build shared by the single-scene
 // build (lbvh.hip) and the batched one (batch.hip): the padded triangle box,
-// the Morton bit spread, the Karras prefix length, the node-slot store and
-// the stable LSD radix sort (tile histogram, one-block exclusive scan,
-// LDS-ranked scatter whose ranks come from wave64 ballots).
+// the Morton bit spread, the Karras prefix length, the node-slot store, the
+// level-synchronous refit and the stable LSD radix sort (tile histogram,
+// one-block exclusive scan, LDS-ranked scatter whose ranks come from wave64
+// ballots).
 #pragma once
 
 #include "device.hpp"
@@ -43,16 +44,60 @@ __device__ __forceinline__ void store_slot(BvhNode* nodes, int node, int side, c
     for (int k = 0; k < 6; k++) f[o + k] = b[k];
 }
 
-// Refit arrival at node counter c: returns the count before this thread's
-// increment (0 = first child done, the caller stops; 1 = both children done,
-// the caller merges). The explicit s_waitcnt makes the box this thread stored
-// on its previous climb step complete before the release's L2 write-back:
-// inside the climb loop the compiler places the write-back straight after the
-// loop-carried stores without waiting for them, and a sibling on another XCD
-// could then read the slot before it reaches memory.
-__device__ __forceinline__ int refit_arrive(int* c) {
-    __builtin_amdgcn_s_waitcnt(0);
-    return __hip_atomic_fetch_add(c, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+// parent code of internal node p: (parent << 1) | side, -1 at the root
+__device__ __forceinline__ int parent_word(const BvhNode* nodes, int p) {
+    return reinterpret_cast<const int*>(nodes + p)[14];
+}
+
+// ---------------------------------------------------------------- refit
+// The refit (refitBoundsKernel, src/path_tracer.cu:1398-1415) runs level by
+// level instead of with per-node arrival counters: every internal node gets
+// its level (root 0), the nodes are sorted by level, and one launch per level,
+// deepest first, merges each node's two child boxes and stores the result in
+// its parent's slot. Kernel boundaries order the levels, so no atomics and no
+// agent-scope cache write-back/invalidate (which on a multi-XCD part costs a
+// whole-L2 operation per climbing step) are needed, and the result is the
+// same whatever order the threads run in.
+
+// Levels along the climb from one leaf, given its parent code. The leaf
+// first counts its internal ancestors (d), then climbs again while it is a
+// left child, writing the level of each node whose leaf range it starts:
+// every internal node has exactly one such leaf, so it is written once.
+// Returns d (the leaf's depth).
+__device__ __forceinline__ int leaf_levels(const BvhNode* nodes, int code, uint32_t* level) {
+    int d = 1;
+    for (int p = code >> 1, pp; (pp = parent_word(nodes, p)) >= 0; p = pp >> 1) d++;
+    for (int c = code, lvl = d - 1; c >= 0 && !(c & 1); lvl--) {
+        const int q = c >> 1;
+        level[q] = (uint32_t)lvl;
+        c = parent_word(nodes, q);
+    }
+    return d;
+}
+
+// merge the two child boxes held in node p into the node's own box: its slot
+// in the parent, or the root box
+__device__ __forceinline__ void refit_node(BvhNode* nodes, int p, float* root_box) {
+    const float* f = reinterpret_cast<const float*>(nodes + p);
+    float b[6];
+    for (int k = 0; k < 3; k++) b[k] = fminf(f[k], f[6 + k]);
+    for (int k = 3; k < 6; k++) b[k] = fmaxf(f[k], f[6 + k]);
+    const int pp = parent_word(nodes, p);
+    if (pp < 0) {
+        for (int k = 0; k < 6; k++) root_box[k] = b[k];
+        return;
+    }
+    store_slot(nodes, pp >> 1, pp & 1, b);
+}
+
+// First sorted position of each level 0..63 after one radix pass over the
+// level keys (the exclusive scan of the digit-major tile histograms), and n
+// at index 64. Levels stay below 64: a Karras tree over 30-bit codes with
+// the index tie-break is at most 62 deep, and the host checks the depth.
+__global__ void k_level_offsets(const int* counts, int tiles, int n, int* off) {
+    const int L = threadIdx.x;
+    if (L < 64) off[L] = counts[L * tiles];
+    else if (L == 64) off[L] = n;
 }
 
 // ------------------------------------------------------------ radix sort
@@ -92,6 +137,77 @@ __global__ __launch_bounds__(1024) void k_scan_exclusive(int* data, int total) {
         data[i] = run;
         run += v;
     }
+}
+
+// Multi-block exclusive scan of the tile histograms: each block of kBlock
+// threads owns kTile consecutive ints. Pass 1 sums every chunk (coalesced
+// loads), k_scan_exclusive scans the chunk sums in one block, pass 2 stages
+// its chunk in LDS (padded one int per 16 so the per-thread runs of 16 are
+// conflict-free), scans it and adds the chunk's offset.
+__device__ __forceinline__ int scan_pad(int i) { return i + (i >> 4); }
+
+__global__ __launch_bounds__(kBlock) void k_scan_chunk_sums(const int* data, int total, int* sums) {
+    __shared__ int wsum[kBlock / 64];
+    const int base = blockIdx.x * kTile;
+    int s = 0;
+    for (int r = 0; r < kItems; r++) {
+        const int i = base + r * kBlock + (int)threadIdx.x;
+        if (i < total) s += data[i];
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane_id() == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kBlock / 64; w++) t += wsum[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_chunk_apply(int* data, int total, const int* sums) {
+    __shared__ int buf[kTile + kTile / 16];
+    __shared__ int wsum[kBlock / 64];
+    const int base = blockIdx.x * kTile, tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    for (int r = 0; r < kItems; r++) {
+        const int i = r * kBlock + tid;
+        buf[scan_pad(i)] = base + i < total ? data[base + i] : 0;
+    }
+    __syncthreads();
+    int v[kItems], s = 0;
+    for (int k = 0; k < kItems; k++) {
+        v[k] = buf[scan_pad(tid * kItems + k)];
+        s += v[k];
+    }
+    // exclusive scan of the per-thread sums: within the wave, then across waves
+    int inc = s;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(inc, off);
+        if (lane >= off) inc += u;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    int run = sums[blockIdx.x] + inc - s;
+    for (int w = 0; w < wave; w++) run += wsum[w];
+    for (int k = 0; k < kItems; k++) {
+        buf[scan_pad(tid * kItems + k)] = run;
+        run += v[k];
+    }
+    __syncthreads();
+    for (int r = 0; r < kItems; r++) {
+        const int i = r * kBlock + tid;
+        if (base + i < total) data[base + i] = buf[scan_pad(i)];
+    }
+}
+
+// chunk-sum slots scan_exclusive needs for `total` ints
+inline int scan_chunks(int64_t total) { return (int)((total + kTile - 1) / kTile); }
+
+// exclusive scan of data[0, total) in place; sums holds scan_chunks(total) ints
+inline void scan_exclusive(int* data, int total, int* sums, hipStream_t st) {
+    const int nb = scan_chunks(total);
+    k_scan_chunk_sums<<<nb, kBlock, 0, st>>>(data, total, sums);
+    k_scan_exclusive<<<1, 1024, 0, st>>>(sums, nb);
+    k_scan_chunk_apply<<<nb, kBlock, 0, st>>>(data, total, sums);
 }
 
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* kin, const int* vin, uint32_t* kout,
