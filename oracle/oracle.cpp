@@ -350,8 +350,10 @@ float schlick(float c, float eta) {
     r0 = r0 * r0;
     return r0 + (1.f - r0) * mf_pow((1 - c), 5.f);
 }
-// dielectric_scatter_direction (:524-540)
-V3 dielectric_dir(const V3& in, const V3& n, float eta, Rng& r) {
+// dielectric_scatter_direction (:524-540); R is any source of uniforms
+// (the PCG32 stream, or fixed values in the closed-form tests)
+template <class R>
+V3 dielectric_dir(const V3& in, const V3& n, float eta, R& r) {
     V3 ud = in.normalize();
     float c = fminf((-ud).dot(n), 1.0f);
     float sn = sqrtf(1.0f - c * c);
@@ -361,7 +363,8 @@ V3 dielectric_dir(const V3& in, const V3& n, float eta, Rng& r) {
     return ud.refract(n, eta);
 }
 // isotropic_scatter_direction (:542-551)
-V3 isotropic_dir(Rng& r) {
+template <class R>
+V3 isotropic_dir(R& r) {
     float u1 = r.u();
     float u2 = r.u();
     float z = 1.0f - 2.0f * u1;
@@ -370,7 +373,8 @@ V3 isotropic_dir(Rng& r) {
     return V3(rr * mf_cos(phi), rr * mf_sin(phi), z);
 }
 // henyeyGreenstein_scatter_direction (:556-578)
-V3 hg_dir(const V3& in, float g, Rng& r) {
+template <class R>
+V3 hg_dir(const V3& in, float g, R& r) {
     if (fabsf(g) < 1e-3f) return isotropic_dir(r);
     float u1 = r.u();
     float u2 = r.u();
@@ -478,7 +482,8 @@ float env_dir_pdf(const Env& e, const V3& d) {  // :452-458
     int row = std::min((int)(v * e.h), e.h - 1);
     return env_texel_pdf(e, row, col, sqrtf(fmaxf(1.0f - ud.y * ud.y, 0.0f)));
 }
-V3 env_sample(const Env& e, Rng& r, float& pdf) {  // :460-467
+template <class R>
+V3 env_sample(const Env& e, R& r, float& pdf) {  // :460-467
     int row = find_cdf_interval(e.row_cdf.data(), e.h, r.u() * e.total);
     const float* cdf = e.col_cdf.data() + (size_t)row * (e.w + 1);
     int col = find_cdf_interval(cdf, e.w, r.u() * cdf[e.w]);
@@ -741,6 +746,78 @@ void oracle_env_tables(const float* rgba, int w, int h, float* row_cdf, float* c
         row_cdf[row + 1] = row_cdf[row] + cdf[w];
     }
     *total_weight = row_cdf[h];
+}
+
+namespace {
+struct FixedUniforms {  // replays given uniforms in draw order
+    const float* v;
+    int i = 0;
+    float u() { return v[i++]; }
+};
+}  // namespace
+
+int oracle_sampling(int fn, const float* in, int n, float* out) {
+    static const int ins[] = {7, 2, 8, 6, 2}, outs[] = {3, 1, 3, 3, 1};
+    if (fn < 0 || fn > 4) return -1;
+    for (int i = 0; i < n; i++) {
+        const float* a = in + (size_t)i * ins[fn];
+        float* o = out + (size_t)i * outs[fn];
+        V3 d;
+        switch (fn) {
+            case 0: d = v3(a).refract(v3(a + 3), a[6]); break;
+            case 1: o[0] = schlick(a[0], a[1]); continue;
+            case 2: {
+                FixedUniforms r{a + 7};
+                d = dielectric_dir(v3(a), v3(a + 3), a[6], r);
+                break;
+            }
+            case 3: {
+                FixedUniforms r{a + 4};
+                d = hg_dir(v3(a), a[3], r);
+                break;
+            }
+            default: o[0] = hg_pdf(a[0], a[1]); continue;
+        }
+        o[0] = d.x;
+        o[1] = d.y;
+        o[2] = d.z;
+    }
+    return 0;
+}
+
+int oracle_env_probe(const float* rgba, int w, int h, float rot, int fn, const float* in, int n, float* out) {
+    if (fn < 0 || fn > 3 || w < 1 || h < 1) return -1;
+    Env e;
+    e.enabled = true;
+    e.w = w;
+    e.h = h;
+    e.rot = rot;
+    e.rgba.assign(rgba, rgba + (size_t)w * h * 4);
+    e.row_cdf.resize(h + 1);
+    e.col_cdf.resize((size_t)h * (w + 1));
+    oracle_env_tables(e.rgba.data(), w, h, e.row_cdf.data(), e.col_cdf.data(), &e.total);
+    static const int ins[] = {4, 3, 3, 2}, outs[] = {4, 1, 2, 3};
+    for (int i = 0; i < n; i++) {
+        const float* a = in + (size_t)i * ins[fn];
+        float* o = out + (size_t)i * outs[fn];
+        if (fn == 0) {
+            FixedUniforms r{a};
+            const V3 d = env_sample(e, r, o[3]);
+            o[0] = d.x;
+            o[1] = d.y;
+            o[2] = d.z;
+        } else if (fn == 1) {
+            o[0] = env_dir_pdf(e, v3(a));
+        } else if (fn == 2) {
+            env_uv(e, v3(a).normalize(), o[0], o[1]);
+        } else {
+            const V3 d = env_direction(e, a[0], a[1]);
+            o[0] = d.x;
+            o[1] = d.y;
+            o[2] = d.z;
+        }
+    }
+    return 0;
 }
 
 uint32_t oracle_expand_bits(uint32_t v) { return expand_bits(v); }

@@ -81,6 +81,17 @@ void oracle_fmath(int fn, const float* x, const float* y, float* out, int n);
 /* first n outputs of the PCG32 stream of (pixel, sample, seed) */
 void oracle_pcg32(uint32_t pixel, uint32_t sample, uint64_t seed, int n, uint32_t* out);
 
+/* Sampling functions on given uniforms, one record per call (closed-form
+ * tests). fn: 0 refract(d[3], n[3], eta) -> dir[3]; 1 schlick(cos, eta) -> r;
+ * 2 dielectric_scatter_direction(d[3], n[3], eta, u) -> dir[3];
+ * 3 henyeyGreenstein_scatter_direction(in[3], g, u1, u2) -> dir[3];
+ * 4 HG pdf(cos, g) -> pdf. Returns -1 for an unknown fn. */
+int oracle_sampling(int fn, const float* in, int n, float* out);
+/* Environment sampling on an RGBA map (w x h, rotation in radians):
+ * fn 0 sample(u[4]) -> dir[3], pdf; 1 pdf(dir[3]) -> pdf;
+ * 2 uv(dir[3]) -> u, v; 3 direction(u, v) -> dir[3]. */
+int oracle_env_probe(const float* rgba, int w, int h, float rot, int fn, const float* in, int n, float* out);
+
 /* resolve_kernel (src/path_tracer.cu:3062-3100) */
 void oracle_resolve(const float* accum, int w, int h, int spp, int tonemap, float exposure, float* out);
 

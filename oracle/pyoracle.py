@@ -62,6 +62,8 @@ def lib():
             "oracle_resolve": ([VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, VP], None),
             "oracle_fmath": ([C.c_int, VP, VP, VP, C.c_int], None),
             "oracle_pcg32": ([C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, VP], None),
+            "oracle_sampling": ([C.c_int, VP, C.c_int, VP], C.c_int),
+            "oracle_env_probe": ([VP, C.c_int, C.c_int, C.c_float, C.c_int, VP, C.c_int, VP], C.c_int),
         }
         for name, (args, res) in sigs.items():
             f = getattr(_lib, name)
@@ -238,6 +240,37 @@ def fmath(name, x, y=None):
     out = np.zeros_like(x)
     lib().oracle_fmath(FMATH[name], _p(x), _p(y), _p(out), len(x))
     return out
+
+
+SAMPLING = {"refract": (0, 7, 3), "schlick": (1, 2, 1), "dielectric": (2, 8, 3), "hg": (3, 6, 3), "hg_pdf": (4, 2, 1)}
+
+
+def sampling(name, args):
+    """Evaluate one of the sampling functions on rows of arguments (float32):
+    refract(d3, n3, eta), schlick(cos, eta), dielectric(d3, n3, eta, u),
+    hg(in3, g, u1, u2), hg_pdf(cos, g)."""
+    fn, nin, nout = SAMPLING[name]
+    a = np.ascontiguousarray(np.asarray(args, np.float32).reshape(-1, nin))
+    out = np.zeros((len(a), nout), np.float32)
+    if lib().oracle_sampling(fn, _p(a), len(a), _p(out)) != 0:
+        raise ValueError(name)
+    return out if nout > 1 else out[:, 0]
+
+
+ENV = {"sample": (0, 4, 4), "pdf": (1, 3, 1), "uv": (2, 3, 2), "direction": (3, 2, 3)}
+
+
+def env_probe(rgba, rot, name, args):
+    """Environment sampling on an (h, w, 4) map: sample(u4) -> (dir3, pdf),
+    pdf(dir3), uv(dir3) -> (u, v), direction(u, v) -> dir3."""
+    rgba = np.ascontiguousarray(rgba, np.float32)
+    h, w = rgba.shape[:2]
+    fn, nin, nout = ENV[name]
+    a = np.ascontiguousarray(np.asarray(args, np.float32).reshape(-1, nin))
+    out = np.zeros((len(a), nout), np.float32)
+    if lib().oracle_env_probe(_p(rgba), w, h, rot, fn, _p(a), len(a), _p(out)) != 0:
+        raise ValueError(name)
+    return out if nout > 1 else out[:, 0]
 
 
 def pcg32(pixel, sample, seed, n):
