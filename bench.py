@@ -11,12 +11,14 @@ HBM before the timed region starts. Prints one JSON line.
   value     Mrays/s = (closest-hit rays + shadow rays) / render seconds
   paths/s   W*H*spp / render seconds (the reference's Mpaths/s definition)
   roofline  extend kernel: algorithmic bytes per ray
-            B = 32 + 16 + 80*S + 44*L (28 B ray + 4 B queue slot, 16 B hit
-            record, 16 B child refs + 2x32 B child boxes per internal-node
-            step, 4 B index + 4 B type + 36 B triangle per leaf test), with
-            S and L counted by the kernel itself in a separate counting pass
-            over the same samples, divided by the extend time measured with
-            HIP events on the library's stream.
+            B = 32 + 16 + 64*S + 48*L (28 B ray + 4 B queue slot, 16 B hit
+            record, one 64 B four-wide node per step, one 48 B leaf record
+            per triangle test), with S and L counted by the kernel itself in
+            a separate counting pass over the same samples, divided by the
+            extend time measured with HIP events on the library's stream.
+            The reference's binary walk of the same rays is reported beside
+            it as 32 + 16 + 80*S + 44*L (16 B child refs + 2x32 B child
+            boxes per step, 4 B index + 4 B type + 36 B triangle per leaf).
   cpu_baseline  the CPU checker (oracle/, a host C++ restatement of the
             reference's device functions) on a band of image rows at 1 spp,
             all host threads.
@@ -651,7 +653,7 @@ def main():
     }
     if counted is not None and st["extend_ms"] > 0:
         n = st["extend_rays"]
-        alg_bytes = 48 * n + 80 * counted["extend_steps"] + 44 * counted["extend_leaves"]
+        alg_bytes = 48 * n + 64 * counted["extend_steps"] + 48 * counted["extend_leaves"]
         launches = st["extend_launches"]
         avg_ms = st["extend_ms"] / launches
         achieved = alg_bytes / (st["extend_ms"] / 1e3)

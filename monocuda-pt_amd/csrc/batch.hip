@@ -35,12 +35,14 @@ namespace mcpt {
 namespace {
 
 struct Seg {  // one scene of a batched build
-    int off, n, ioff, pad;
+    int off, n, ioff, wbase;  // first triangle, count, first internal node, first wide node (global)
     const float* hot;
     BvhNode* nodes;
     LeafTri* leaf_tri;
     int* leaf_prim;
     float* root_box;
+    int* tie;
+    WideNode* wide;
 };
 
 // scene owning global index g (segments ordered by off)
@@ -215,31 +217,50 @@ __global__ __launch_bounds__(kBlock) void k_b_karras(const Seg* seg, int nseg, i
     rf[3] = j;
 }
 
-// Levels of every scene's internal nodes (keys and ids by global node index)
-// and each scene's deepest leaf: a wave reduces its lanes' depths per scene
-// (nearly always one) before a single atomic.
+// Levels of every scene's internal nodes (keys and ids by global node index),
+// each leaf's rank in the reference's leaf-test order, each scene's deepest
+// leaf and its number of even-level (wide) nodes: a wave reduces its lanes'
+// values per scene (nearly always one) before a single atomic each.
 __global__ __launch_bounds__(kBlock) void k_b_levels(const Seg* seg, int nseg, int total, const int* leaf_parent,
-                                                    uint32_t* level, int* ids, int* depth) {
+                                                    uint32_t* level, int* ids, int* depth, int* wcount) {
     const int g = blockIdx.x * kBlock + threadIdx.x;
     const bool valid = g < total;
-    int s = -1, d = 0;
+    int s = -1, d = 0, ne = 0;
     if (valid) {
         s = seg_of(seg, nseg, g);
         const Seg& S = seg[s];
         const int j = g - S.off;
         if (j < S.n - 1) ids[S.ioff + j] = S.ioff + j;
-        d = leaf_levels(S.nodes, leaf_parent[g], level + S.ioff);
+        const LeafClimb c = leaf_levels(S.nodes, leaf_parent[g], level + S.ioff);
+        S.tie[j] = c.rank;
+        d = c.depth;
+        ne = c.n_even;
     }
     uint64_t todo = ballot64(valid);
     while (todo) {
         const int leader = __builtin_ctzll(todo);
         const int s0 = __shfl(s, leader);
         const bool mine = valid && s == s0;
-        int v = mine ? d : 0;
-        for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
-        if (lane_id() == leader) atomicMax(&depth[s0], v);
+        int v = mine ? d : 0, w = mine ? ne : 0;
+        for (int off = 32; off > 0; off >>= 1) {
+            v = max(v, __shfl_xor(v, off));
+            w += __shfl_xor(w, off);
+        }
+        if (lane_id() == leader) {
+            atomicMax(&depth[s0], v);
+            if (w) atomicAdd(&wcount[s0], w);
+        }
         todo &= ~ballot64(mine);
     }
+}
+
+// one wide node per even-level binary node of every scene (global index q)
+__global__ __launch_bounds__(kBlock) void k_b_collapse(const Seg* seg, int nseg, const uint32_t* level,
+                                                      const int* widx, int n) {
+    const int q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= n || (level[q] & 1u)) return;
+    const Seg& S = seg[seg_of_node(seg, nseg, q)];
+    collapse_node(S.nodes, q - S.ioff, widx, S.ioff, S.wbase, S.wide);
 }
 
 // refitBoundsKernel (:1398-1415) for the nodes of one level, all scenes
@@ -379,7 +400,7 @@ __global__ __launch_bounds__(kBlock) void k_bb_extend(const BvhView* bvhs, BWs w
             }
         }
         if (exhausted && ballot64(has) == 0) break;
-        if (has && step_ordered<false>(bv, s, stk, kBlock, ns, nl)) {
+        if (has && step_wide<false>(bv, s, stk, kBlock, ns, nl)) {
             mcpt_packed_hit h;
             h.t = s.t;
             h.prim_type = s.leaf >= 0 ? 1 : -1;
@@ -536,7 +557,7 @@ int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const
     ws.smp = (float4*)b, b += parts[k++];
     ws.rng = (uint64_t*)b, b += parts[k++];
     ws.counters = (int*)b;
-    const size_t lds = (size_t)kBlock * max_stack * sizeof(int);
+    const size_t lds = (size_t)kBlock * stack_entries(max_stack) * sizeof(int);
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bb_extend, kBlock, lds);
     per_cu = std::max(1, per_cu);
@@ -652,8 +673,9 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
         MCPT_TRY(s->leaf_tri.alloc(ctx, s->n));
         MCPT_TRY(s->leaf_prim.alloc(ctx, s->n));
         MCPT_TRY(s->root_box.alloc(ctx, 6));
+        MCPT_TRY(s->tie.alloc(ctx, s->n));
         seg[k] = Seg{(int)total, s->n, (int)total_int, 0, s->hot.ptr, s->nodes.ptr, s->leaf_tri.ptr, s->leaf_prim.ptr,
-                     s->root_box.ptr};
+                     s->root_box.ptr, s->tie.ptr, nullptr};
         total += s->n;
         total_int += s->n - 1;
         chunk_off[k + 1] = chunk_off[k] + (s->n + kChunk - 1) / kChunk;
@@ -665,7 +687,7 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
     const size_t parts[] = {al(ns * sizeof(Seg)),         al(ns * 6 * sizeof(float)),  al(total * sizeof(uint32_t)),
                             al(total * sizeof(uint32_t)), al(total * sizeof(uint32_t)), al(total * sizeof(int)),
                             al(total * sizeof(int)),      al((size_t)256 * tiles * sizeof(int)),
-                            al(total * sizeof(int)),      al(total_int * sizeof(int)),  al((ns + 65) * sizeof(int)),
+                            al(total * sizeof(int)),      al(total_int * sizeof(int)),  al((2 * ns + 65) * sizeof(int)),
                             al((ns + 1) * sizeof(int)),   al((size_t)n_chunks * 6 * sizeof(float)),
                             al(total_int * sizeof(uint32_t)), al(scan_chunks(256 * (int64_t)tiles) * sizeof(int))};
     size_t bytes = 0;
@@ -694,7 +716,7 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
     b += parts[k++];
     int* ids_sorted = (int*)b;
     b += parts[k++];
-    int* depth = (int*)b;  // per-scene depth, then the 65 level offsets
+    int* depth = (int*)b;  // per-scene depth, the 65 level offsets, per-scene wide node counts
     b += parts[k++];
     int* dchunk = (int*)b;
     b += parts[k++];
@@ -710,6 +732,8 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
     if (e == hipSuccess)
         e = hipMemcpyAsync(dchunk, chunk_off.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipEventRecord(e0, st);
+    uint32_t* lvl_keys = nullptr;  // level of each global internal node
+    int* widx = nullptr;           // its wide index (after the level sort)
     if (e == hipSuccess) {
         const int n_all = (int)total, n_int = (int)total_int;
         k_b_bounds_chunks<<<n_chunks, kBlock, 0, st>>>(dseg, ns, dchunk, partial);
@@ -735,34 +759,57 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
         }
         k_b_gather_keys<<<blocks_for(n_all), kBlock, 0, st>>>(morton, vin, n_all, kin);
         k_b_leaves<<<blocks_for(n_all), kBlock, 0, st>>>(dseg, ns, n_all, vin);
-        k_b_fill<<<blocks_for(ns), kBlock, 0, st>>>(depth, ns, 0);
+        k_b_fill<<<blocks_for(2 * ns + 65), kBlock, 0, st>>>(depth, 2 * ns + 65, 0);
         k_b_karras<<<blocks_for(n_int), kBlock, 0, st>>>(dseg, ns, n_int, kin, leaf_parent);
         // refit level by level over all scenes: levels into kout/vout (free
         // after the sort), one radix pass by level, offsets to the host
         const int itiles = blocks_for(n_int, kTile);
-        k_b_levels<<<blocks_for(n_all), kBlock, 0, st>>>(dseg, ns, n_all, leaf_parent, kout, vout, depth);
+        k_b_levels<<<blocks_for(n_all), kBlock, 0, st>>>(dseg, ns, n_all, leaf_parent, kout, vout, depth,
+                                                         depth + ns + 65);
         k_radix_hist<<<itiles, kBlock, 0, st>>>(kout, n_int, 0, itiles, counts);
         scan_exclusive(counts, 256 * itiles, ssum, st);
         k_radix_scatter<<<itiles, kBlock, 0, st>>>(kout, vout, lvl_sorted, ids_sorted, n_int, 0, itiles, counts);
         k_level_offsets<<<1, 128, 0, st>>>(counts, itiles, n_int, depth + ns);
+        lvl_keys = kout;
+        widx = vout;
         e = hipGetLastError();
     }
-    std::vector<int> depths(ns + 65, 0);
+    std::vector<int> depths(2 * ns + 65, 0);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(depths.data(), depth, (ns + 65) * sizeof(int), hipMemcpyDeviceToHost, st);
+        e = hipMemcpyAsync(depths.data(), depth, (2 * ns + 65) * sizeof(int), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     int bad = -1, max_d = 0;
     for (int k2 = 0; k2 < ns && e == hipSuccess; k2++) {
         if (depths[k2] < 1 || depths[k2] > 64) bad = k2;
         max_d = std::max(max_d, depths[k2]);
     }
-    if (e == hipSuccess && bad < 0) {
+    // each scene's four-wide tree, in its own buffer
+    int status = MCPT_OK;
+    for (int k2 = 0, wb = 0; k2 < ns && e == hipSuccess && bad < 0 && status == MCPT_OK; k2++) {
+        mcpt_scene_s* s = scenes[multi[k2]];
+        const int nw = depths[ns + 65 + k2];
+        status = s->wide.alloc(ctx, nw);
+        s->n_wide = nw;
+        seg[k2].wide = s->wide.ptr;
+        seg[k2].wbase = wb;
+        wb += nw;
+    }
+    if (e == hipSuccess && bad < 0 && status == MCPT_OK) {
+        const int n_int = (int)total_int;
+        e = hipMemcpyAsync(dseg, seg.data(), ns * sizeof(Seg), hipMemcpyHostToDevice, st);
         const int* off = depths.data() + ns;
-        for (int L = max_d - 1; L >= 0; L--)
+        for (int L = max_d - 1; L >= 0 && e == hipSuccess; L--)
             if (off[L + 1] > off[L])
                 k_b_refit_level<<<blocks_for(off[L + 1] - off[L]), kBlock, 0, st>>>(dseg, ns, ids_sorted + off[L],
                                                                                     off[L + 1] - off[L]);
-        e = hipGetLastError();
+        // wide indices of the even-level nodes (in the level sort's value
+        // buffer, free once the sort is done), then the collapse
+        if (e == hipSuccess) {
+            k_wide_flags<<<blocks_for(n_int), kBlock, 0, st>>>(lvl_keys, n_int, widx);
+            scan_exclusive(widx, n_int, ssum, st);
+            k_b_collapse<<<blocks_for(n_int), kBlock, 0, st>>>(dseg, ns, lvl_keys, widx, n_int);
+            e = hipGetLastError();
+        }
     }
     if (e == hipSuccess) e = hipEventRecord(e1, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -772,6 +819,7 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
     if (e1) (void)hipEventDestroy(e1);
     dev_free(ctx, mem, bytes);
     if (e != hipSuccess) return set_error(MCPT_ERR_HIP, "batched LBVH build: %s", hipGetErrorString(e));
+    if (status < 0) return status;
     if (bad >= 0)
         return set_error(MCPT_ERR_INVALID, "scene %d: BVH depth %d outside [1, 64]", multi[bad], depths[bad]);
     for (int k2 = 0; k2 < ns; k2++) {

@@ -493,7 +493,7 @@ __global__ __launch_bounds__(kBlock) void k_cc_extend(SceneView sv, CacheWs ws, 
             }
         }
         if (exhausted && ballot64(has) == 0) break;
-        if (has && step_ordered<false>(sv.bvh, s, stk, kBlock, ns, nl)) {
+        if (has && step_wide<false>(sv.bvh, s, stk, kBlock, ns, nl)) {
             mcpt_packed_hit h;
             h.t = s.t;
             h.prim_type = s.leaf >= 0 ? 1 : -1;
@@ -941,7 +941,7 @@ int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* cam, const mcpt_rend
     MCPT_TRY(ensure_cache_ws(mlp, npix, train_paths, depth, ws));
     carve_cache_ws(mlp, ws);
     const SceneView sv = scene->view();
-    const size_t lds = (size_t)kBlock * std::max(1, scene->max_depth) * sizeof(int);
+    const size_t lds = (size_t)kBlock * stack_entries(scene->max_depth) * sizeof(int);
     const int refill = o->refill_lanes > 0 ? std::min(64, o->refill_lanes) : 32;
     int per_cu = 0;
     MCPT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cc_extend, kBlock, lds));

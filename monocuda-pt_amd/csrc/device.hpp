@@ -74,6 +74,22 @@ struct alignas(16) BvhNode {
 };
 static_assert(sizeof(BvhNode) == 64, "BVH node must be one 64-byte segment");
 
+// Four-wide quantized node, what the fast traversal orders walk: the binary
+// node at an even level with its two to four children (its grandchildren,
+// or its children where those are leaves), in left-to-right leaf order.
+// Child boxes are 8-bit offsets from the node's origin in units of a power
+// of two per axis, rounded outward when built: origin + q * 2^e is one
+// monotone rounding of an exact product, so a decoded box never shrinks.
+// Half the bytes of two binary levels, in one 64-byte segment.
+struct alignas(16) WideNode {
+    float4 o;     // origin xyz; w holds the bits of ex | ey << 8 | ez << 16 |
+                  // count << 24 (exponents biased by 127 like a float's)
+    uint4 q0;     // byte k of each word is child k's: lo.x, lo.y, lo.z, hi.x
+    uint4 q1;     // hi.y, hi.z, unused, unused
+    int4 child;   // >= 0 wide node, < 0 leaf ~r; slots >= count unused
+};
+static_assert(sizeof(WideNode) == 64, "wide node must be one 64-byte segment");
+
 struct Box {
     vec3 mn, mx;
 };
@@ -189,12 +205,14 @@ struct alignas(16) LeafTri {
 };
 static_assert(sizeof(LeafTri) == 48, "leaf record is 48 bytes");
 
-// What traversal needs of a scene (24 bytes, loaded per ray when the rays of
+// What traversal needs of a scene (40 bytes, loaded per ray when the rays of
 // several scenes share one queue).
 struct BvhView {
-    const BvhNode* nodes;
+    const BvhNode* nodes;     // binary tree (the reference's visit order)
+    const WideNode* wide;     // four-wide quantized tree (fast orders)
     const LeafTri* leaf_tri;  // leaf (Morton) order
     const int* leaf_prim;     // leaf -> original triangle index
+    const int* tie;           // leaf -> rank in the reference's leaf-test order
 };
 
 struct SceneView {

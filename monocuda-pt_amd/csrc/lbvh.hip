@@ -16,7 +16,11 @@
 //   5. refit (refitBoundsKernel, :1398-1415) level by level: node levels
 //      from the leaves' climbs, one radix pass orders the nodes by level,
 //      then one launch per level, deepest first, merges each node's child
-//      boxes into its own slot in the parent.
+//      boxes into its own slot in the parent. The climbs also rank every
+//      leaf in the reference's leaf-test order (the tie rule of the fast
+//      traversal).
+//   6. the four-wide quantized tree the fast traversal walks: one wide node
+//      per even-level binary node, children quantized outward to 8 bits.
 #include <algorithm>
 #include <cstring>
 
@@ -149,15 +153,18 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* keys, int n, 
     rf[3] = j;  // the node covers leaves [min(idx, j), max(idx, j)]
 }
 
-// 5a. level of every internal node and the deepest leaf (one atomic per
-// wave); node ids beside the levels for the sort by level
+// 5a. level of every internal node, each leaf's rank in the reference's
+// leaf-test order and the deepest leaf (one atomic per wave); node ids
+// beside the levels for the sort by level
 __global__ __launch_bounds__(kBlock) void k_levels(const BvhNode* nodes, const int* leaf_parent, int n,
-                                                   uint32_t* level, int* ids, int* max_depth) {
+                                                   uint32_t* level, int* ids, int* tie, int* max_depth) {
     const int j = blockIdx.x * kBlock + threadIdx.x;
     int d = 0;
     if (j < n) {
         if (j < n - 1) ids[j] = j;
-        d = leaf_levels(nodes, leaf_parent[j], level);
+        const LeafClimb c = leaf_levels(nodes, leaf_parent[j], level);
+        d = c.depth;
+        tie[j] = c.rank;
     }
     for (int off = 32; off > 0; off >>= 1) d = max(d, __shfl_xor(d, off));
     if (lane_id() == 0 && d > 0) atomicMax(max_depth, d);
@@ -167,6 +174,13 @@ __global__ __launch_bounds__(kBlock) void k_levels(const BvhNode* nodes, const i
 __global__ __launch_bounds__(kBlock) void k_refit_level(BvhNode* nodes, const int* ids, int cnt, float* root_box) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < cnt) refit_node(nodes, ids[i], root_box);
+}
+
+// 6. the four-wide tree: one wide node per even-level binary node
+__global__ __launch_bounds__(kBlock) void k_collapse(const BvhNode* nodes, const uint32_t* level, const int* widx,
+                                                     int n, WideNode* wide) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n && !(level[i] & 1u)) collapse_node(nodes, i, widx, 0, 0, wide);
 }
 
 __global__ void k_fill_int(int* p, int n, int v) {
@@ -201,6 +215,7 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
     MCPT_TRY(s->leaf_tri.alloc(ctx, n));
     MCPT_TRY(s->leaf_prim.alloc(ctx, n));
     MCPT_TRY(s->root_box.alloc(ctx, 6));
+    MCPT_TRY(s->tie.alloc(ctx, n));
 
     const int tiles = blocks_for(n, kTile);
     const int n_partial = std::min(1024, blocks_for(n));
@@ -296,13 +311,23 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
         HSTEP(hipMemcpy(s->nodes.ptr, &root, sizeof(root), hipMemcpyHostToDevice));
         HSTEP(hipMemcpy(s->root_box.ptr, b, sizeof(b), hipMemcpyHostToDevice));
         HSTEP(hipMemcpy(leaf_parent.ptr, &lp, sizeof(int), hipMemcpyHostToDevice));
+        HSTEP(hipMemcpy(s->tie.ptr, &lp, sizeof(int), hipMemcpyHostToDevice));
+        // the wide root likewise holds leaf 0 twice
+        const float cb[2][6] = {{b[0], b[1], b[2], b[3], b[4], b[5]}, {b[0], b[1], b[2], b[3], b[4], b[5]}};
+        const int refs[2] = {~0, ~0};
+        WideNode wroot;
+        wide_encode(cb, refs, 2, &wroot);
+        STEP(s->wide.alloc(ctx, 1));
+        HSTEP(hipMemcpy(s->wide.ptr, &wroot, sizeof(wroot), hipMemcpyHostToDevice));
+        s->n_wide = 1;
         s->max_depth = 1;
     } else {
         k_karras<<<blocks_for(n - 1), kBlock, 0, st>>>(kin, n, s->leaf_tri.ptr, s->nodes.ptr, leaf_parent.ptr);
         // 5. refit: levels (keys into kout, ids into vout, both free after
         // the sort), one radix pass by level, then one launch per level
         const int ni = n - 1, itiles = blocks_for(ni, kTile);
-        k_levels<<<blocks_for(n), kBlock, 0, st>>>(s->nodes.ptr, leaf_parent.ptr, n, kout, vout, depth.ptr);
+        k_levels<<<blocks_for(n), kBlock, 0, st>>>(s->nodes.ptr, leaf_parent.ptr, n, kout, vout, s->tie.ptr,
+                                                   depth.ptr);
         k_radix_hist<<<itiles, kBlock, 0, st>>>(kout, ni, 0, itiles, counts.ptr);
         scan_exclusive(counts.ptr, 256 * itiles, ssum.ptr, st);
         k_radix_scatter<<<itiles, kBlock, 0, st>>>(kout, vout, lvl_sorted.ptr, ids_sorted.ptr, ni, 0, itiles,
@@ -322,6 +347,15 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
                                                                                   ids_sorted.ptr + off[L],
                                                                                   off[L + 1] - off[L],
                                                                                   s->root_box.ptr);
+        // 6. four-wide tree: wide indices of the even-level nodes (into vout,
+        // free after the level sort), then one wide node per such node
+        int n_wide = 0;
+        for (int L = 0; L < 64; L += 2) n_wide += off[L + 1] - off[L];
+        STEP(s->wide.alloc(ctx, n_wide));
+        s->n_wide = n_wide;
+        k_wide_flags<<<blocks_for(ni), kBlock, 0, st>>>(kout, ni, vout);
+        scan_exclusive(vout, ni, ssum.ptr, st);
+        k_collapse<<<blocks_for(ni), kBlock, 0, st>>>(s->nodes.ptr, kout, vout, ni, s->wide.ptr);
         HSTEP(hipGetLastError());
     }
     HSTEP(hipEventRecord(e1, st));

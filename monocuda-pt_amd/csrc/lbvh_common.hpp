@@ -63,16 +63,39 @@ __device__ __forceinline__ int parent_word(const BvhNode* nodes, int p) {
 // first counts its internal ancestors (d), then climbs again while it is a
 // left child, writing the level of each node whose leaf range it starts:
 // every internal node has exactly one such leaf, so it is written once.
-// Returns d (the leaf's depth).
-__device__ __forceinline__ int leaf_levels(const BvhNode* nodes, int code, uint32_t* level) {
-    int d = 1;
-    for (int p = code >> 1, pp; (pp = parent_word(nodes, p)) >= 0; p = pp >> 1) d++;
+//
+// The same climb gives the leaf's rank in the reference's leaf-test order
+// (leaves are tested when their parent is visited, parents in left-first
+// pre-order, left side first): rank = 2 * preorder(P) + side, where the
+// pre-order number of internal node P is depth(P) + first(P) - m(P), with
+// first(P) the first leaf of P's range and m(P) the number of right-child
+// edges on the path from the root to P (each such edge skips a left subtree
+// of k leaves and k - 1 internal nodes).
+struct LeafClimb {
+    int depth;   // internal ancestors of the leaf
+    int rank;    // position in the reference's leaf-test order
+    int n_even;  // even-level nodes this leaf wrote the level of
+};
+
+__device__ __forceinline__ LeafClimb leaf_levels(const BvhNode* nodes, int code, uint32_t* level) {
+    LeafClimb r;
+    int d = 1, m = 0;
+    for (int p = code >> 1, pp; (pp = parent_word(nodes, p)) >= 0; p = pp >> 1) {
+        d++;
+        m += pp & 1;
+    }
+    const int P = code >> 1;
+    const int first = min(P, reinterpret_cast<const int*>(nodes + P)[15]);
+    r.depth = d;
+    r.rank = 2 * (d - 1 + first - m) + (code & 1);
+    r.n_even = 0;
     for (int c = code, lvl = d - 1; c >= 0 && !(c & 1); lvl--) {
         const int q = c >> 1;
         level[q] = (uint32_t)lvl;
+        r.n_even += !(lvl & 1);
         c = parent_word(nodes, q);
     }
-    return d;
+    return r;
 }
 
 // merge the two child boxes held in node p into the node's own box: its slot
@@ -88,6 +111,115 @@ __device__ __forceinline__ void refit_node(BvhNode* nodes, int p, float* root_bo
         return;
     }
     store_slot(nodes, pp >> 1, pp & 1, b);
+}
+
+// ------------------------------------------------------- four-wide tree
+// Every binary node at an even level becomes a wide node whose children are
+// its children that are leaves and the children of its inner children, in
+// left-to-right order; wide index = exclusive prefix count of even-level
+// nodes in node order, so the root stays 0.
+
+MCPT_HDI float exp2i(int e) {  // 2^e for e in [-126, 127]
+    union {
+        uint32_t u;
+        float f;
+    } v;
+    v.u = (uint32_t)(e + 127) << 23;
+    return v.f;
+}
+
+// Quantizes up to four child boxes (min xyz, max xyz) outward against
+// their union: per axis the scale is the smallest power of two (>= 2^-126)
+// that spans the union in 255 steps, lo = floor and hi = ceil, each then
+// checked by the decode the traversal bounds (origin + q * 2^e, one
+// rounding of an exact product) and stepped outward until it contains the
+// exact bound.
+MCPT_HDI void wide_encode(const float (*b)[6], const int* ref, int cnt, WideNode* out) {
+    float org[3];
+    uint32_t ebits = 0;
+    uint32_t qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0};
+    for (int a = 0; a < 3; a++) {
+        float lo = b[0][a], hi = b[0][3 + a];
+        for (int k = 1; k < cnt; k++) {
+            lo = fminf(lo, b[k][a]);
+            hi = fmaxf(hi, b[k][3 + a]);
+        }
+        org[a] = lo;
+        const double ext = (double)hi - (double)lo;
+        int e = -126;
+        if (ext > 0.0) {
+            int ex = 0;
+            frexp(ext / 255.0, &ex);
+            e = ex < -126 ? -126 : (ex > 127 ? 127 : ex);
+        }
+        while (true) {
+            const float sc = exp2i(e);
+            bool ok = true;
+            uint32_t ql = 0, qh = 0;
+            for (int k = 0; k < cnt && ok; k++) {
+                double fl = floor(((double)b[k][a] - (double)lo) / (double)sc);
+                double fh = ceil(((double)b[k][3 + a] - (double)lo) / (double)sc);
+                int l = fl < 0.0 ? 0 : (fl > 255.0 ? 255 : (int)fl);
+                int h = fh < 0.0 ? 0 : (fh > 255.0 ? 255 : (int)fh);
+                while (l > 0 && lo + (float)l * sc > b[k][a]) l--;
+                while (h < 255 && lo + (float)h * sc < b[k][3 + a]) h++;
+                if (lo + (float)h * sc < b[k][3 + a]) ok = false;
+                ql |= (uint32_t)l << (8 * k);
+                qh |= (uint32_t)h << (8 * k);
+            }
+            if (ok || e >= 127) {
+                qlo[a] = ql;
+                qhi[a] = qh;
+                break;
+            }
+            e++;
+        }
+        ebits |= (uint32_t)(e + 127) << (8 * a);
+    }
+    ebits |= (uint32_t)cnt << 24;
+    union {
+        uint32_t u;
+        float f;
+    } w;
+    w.u = ebits;
+    out->o = make_float4(org[0], org[1], org[2], w.f);
+    out->q0 = make_uint4(qlo[0], qlo[1], qlo[2], qhi[0]);
+    out->q1 = make_uint4(qhi[1], qhi[2], 0u, 0u);
+    out->child = make_int4(ref[0], cnt > 1 ? ref[1] : 0, cnt > 2 ? ref[2] : 0, cnt > 3 ? ref[3] : 0);
+}
+
+// wide flags: 1 for even-level nodes (scanned in place into wide indices)
+__global__ __launch_bounds__(kBlock) void k_wide_flags(const uint32_t* level, int n, int* flag) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) flag[i] = (level[i] & 1u) ? 0 : 1;
+}
+
+// The wide node of binary node p (at an even level): child boxes come from
+// p's slots and its inner children's slots; wide references are
+// widx[global node] - wbase for inner grandchildren, ~leaf for leaves.
+__device__ __forceinline__ void collapse_node(const BvhNode* nodes, int p, const int* widx, int gbase, int wbase,
+                                              WideNode* wide) {
+    float b[4][6];
+    int ref[4];
+    int cnt = 0;
+    const float* fp = reinterpret_cast<const float*>(nodes + p);
+    const int* rp = reinterpret_cast<const int*>(nodes + p) + 12;
+    for (int side = 0; side < 2; side++) {
+        const int c = rp[side];
+        if (c < 0) {
+            for (int k = 0; k < 6; k++) b[cnt][k] = fp[6 * side + k];
+            ref[cnt++] = c;
+        } else {
+            const float* fc = reinterpret_cast<const float*>(nodes + c);
+            const int* rc = reinterpret_cast<const int*>(nodes + c) + 12;
+            for (int s2 = 0; s2 < 2; s2++) {
+                const int g = rc[s2];
+                for (int k = 0; k < 6; k++) b[cnt][k] = fc[6 * s2 + k];
+                ref[cnt++] = g < 0 ? g : widx[gbase + g] - wbase;
+            }
+        }
+    }
+    wide_encode(b, ref, cnt, wide + (widx[gbase + p] - wbase));
 }
 
 // First sorted position of each level 0..63 after one radix pass over the

@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Work
         }
         if (exhausted && ballot64(has) == 0) break;
         if (has) {
-            if (REF ? step_closest<COUNT>(sv.bvh, s, stk, kBlock, ns, nl) : step_ordered<COUNT>(sv.bvh, s, stk, kBlock, ns, nl)) {
+            if (REF ? step_closest<COUNT>(sv.bvh, s, stk, kBlock, ns, nl) : step_wide<COUNT>(sv.bvh, s, stk, kBlock, ns, nl)) {
                 ws.hits[slot] = pack_hit(sv.bvh, s);
                 has = false;
             }
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow_persistent(SceneView sv, Work
         }
         if (exhausted && ballot64(has) == 0) break;
         if (has) {
-            const int st = step_any(sv.bvh, s, stk, kBlock);
+            const int st = step_wide_any(sv.bvh, s, stk, kBlock);
             if (st != 0) {
                 if (st == 1) {
                     const ShadowItem si = ws.sq[slot];
@@ -458,7 +458,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(SceneView sv, const mcpt_r
 }
 
 int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
-size_t stack_bytes(const mcpt_scene_s* s) { return (size_t)kBlock * std::max(1, s->max_depth) * sizeof(int); }
+size_t stack_bytes(const mcpt_scene_s* s) { return (size_t)kBlock * stack_entries(s->max_depth) * sizeof(int); }
 
 // The wavefront workspace lives in the context and grows to the largest
 // path capacity requested.

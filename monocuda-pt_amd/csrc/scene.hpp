@@ -34,6 +34,9 @@ struct mcpt_scene_s {
     mcpt::DevBuf<mcpt::LeafTri> leaf_tri;
     mcpt::DevBuf<int> leaf_prim;
     mcpt::DevBuf<float> root_box;  // 6 floats: min xyz, max xyz
+    mcpt::DevBuf<mcpt::WideNode> wide;  // four-wide quantized tree, root 0
+    mcpt::DevBuf<int> tie;              // leaf -> reference leaf-test rank
+    int n_wide = 0;
     float last_build_ms = 0.f;
 
     mcpt::SceneView view() const {
@@ -41,6 +44,8 @@ struct mcpt_scene_s {
         v.bvh.nodes = nodes.ptr;
         v.bvh.leaf_tri = leaf_tri.ptr;
         v.bvh.leaf_prim = leaf_prim.ptr;
+        v.bvh.wide = wide.ptr;
+        v.bvh.tie = tie.ptr;
         v.hot = hot.ptr;
         v.cold = cold.ptr;
         v.mats = mats.ptr;
@@ -72,8 +77,21 @@ struct mcpt_scene_s {
         leaf_tri.release(ctx);
         leaf_prim.release(ctx);
         root_box.release(ctx);
+        wide.release(ctx);
+        tie.release(ctx);
     }
 };
+
+namespace mcpt {
+// Traversal stack ints per lane for a tree whose deepest leaf has max_depth
+// internal ancestors: the binary walk keeps at most one pending node per
+// level, the four-wide walk one two-int group per wide level (two binary
+// levels).
+inline int stack_entries(int max_depth) {
+    const int d = max_depth < 1 ? 1 : max_depth;
+    return 2 * ((d + 1) / 2);
+}
+}  // namespace mcpt
 
 struct mcpt_film_s {
     mcpt::Ctx* ctx = nullptr;

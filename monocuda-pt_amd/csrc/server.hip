@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void k_srv_extend(Pool p, int cur, int refi
             }
         }
         if (exhausted && ballot64(has) == 0) break;
-        if (has && step_ordered<false>(bv, s, stk, kBlock, ns, nl)) {
+        if (has && step_wide<false>(bv, s, stk, kBlock, ns, nl)) {
             mcpt_packed_hit h;
             h.t = s.t;
             h.prim_type = s.leaf >= 0 ? 1 : -1;
@@ -634,7 +634,7 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
     s->n_scenes = n_scenes;
     s->scenes.assign(scenes, scenes + n_scenes);
     for (int i = 0; i < n_scenes; i++) s->max_depth_scene = std::max(s->max_depth_scene, scenes[i]->max_depth);
-    s->lds = (size_t)kBlock * s->max_depth_scene * sizeof(int);
+    s->lds = (size_t)kBlock * stack_entries(s->max_depth_scene) * sizeof(int);
     // device memory: pool + tables
     const size_t C = s->capacity;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };

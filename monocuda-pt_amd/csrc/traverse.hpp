@@ -1,22 +1,24 @@
-// traverse.hpp — stack traversal of the two-box BVH layout.
+// traverse.hpp — stack traversal of the scene's two trees.
 //
-// Two visit orders return the same closest hit (t and primitive, bit for
-// bit):
+// Two walks return the same closest hit (t and primitive, bit for bit):
 //   step_closest()  one iteration of BVH::traverseRay's loop
 //                   (src/path_tracer.cu:1102-1147, split as stepTraversal,
-//                   :1276-1321): both child boxes against the running
-//                   closest distance, leaf children tested at once (left,
-//                   then right), left child first, the right one pushed when
-//                   both are internal and hit. Its step and leaf-test counts
-//                   are the reference's.
-//   step_ordered()  the nearer child first, with the reference's tie rule
-//                   (ref_earlier) so exact-t ties resolve as the reference's
-//                   order would resolve them. The default.
+//                   :1276-1321) on the binary tree: both child boxes against
+//                   the running closest distance, leaf children tested at
+//                   once (left, then right), left child first, the right one
+//                   pushed when both are internal and hit. Its step and
+//                   leaf-test counts are the reference's.
+//   step_wide()     the four-wide quantized tree, nearest inner child first,
+//                   with the reference's tie rule (a per-leaf rank in its
+//                   leaf-test order) so exact-t ties resolve as the
+//                   reference's order would resolve them. The default.
+// step_wide_any() is the any-hit walk (shadow rays) on the four-wide tree.
 //
 // The traversal stack lives in LDS, one column per lane
 // (stack[level * stride + lane_in_block]), so lanes at any depth hit
-// distinct banks. Its height is the tree's measured depth (<= 64: Karras
-// prefixes grow strictly along a root-to-leaf path and stay below 64).
+// distinct banks. Its height (stack_entries) is the binary tree's measured
+// depth (<= 64: Karras prefixes grow strictly along a root-to-leaf path and
+// stay below 64), rounded up to even for the wide walk's two-int groups.
 #pragma once
 
 #include "device.hpp"
@@ -28,7 +30,6 @@ struct Trav {
     vec3 oi;   // o * inv, for the fused box test of the fast orders
     float t;   // running closest distance (or the any-hit limit)
     int leaf;  // closest leaf so far, -1 = none
-    int best;  // near-first order: the closest leaf's parent << 1 | side
     int node;
     int sp;
 };
@@ -51,7 +52,6 @@ MCPT_DEV void trav_begin(Trav& s, const vec3& o, const vec3& d, float tmax) {
     s.oi = vec3(o.x * s.inv.x, o.y * s.inv.y, o.z * s.inv.z);
     s.t = tmax;
     s.leaf = -1;
-    s.best = -1;
     s.node = 0;
     s.sp = 0;
 }
@@ -132,143 +132,206 @@ MCPT_DEV bool step_closest(const BvhView& bv, Trav& s, int* stk, int stride, uin
     return false;
 }
 
-// ------------------------------------------------------ near-first order
-// The reference's closest hit does not depend on the visit order except on
-// exact ties of t between different triangles: every triangle lies at least
-// 1e-3 inside its box (the padding of calculate_bounding_box, :651-655), so a
-// box culled against the running distance never holds a triangle at or
-// before it, and the minimum t is found in any order. On a tie the
-// reference keeps the triangle it tests first. It tests the leaf children of
-// an internal node when it visits that node (left, then right) and visits
-// internal nodes in left-first pre-order, so of two candidate leaves under
-// parents P and Q the reference tests P's first when P == Q and P's side is
-// left, when P is an ancestor of Q (P's range contains Q's), or when P's
-// range lies left of Q's. ref_earlier() applies that rule, which lets the
-// traversal below descend into the nearer child first and still return the
-// reference's (t, primitive) bit for bit.
-MCPT_DEV bool ref_earlier(const BvhView& bv, int p, int p_other_end, int p_side, int q_code) {
-    const int q = q_code >> 1, q_side = q_code & 1;
-    if (p == q) return p_side < q_side;
-    const int fp = min(p, p_other_end), lp = max(p, p_other_end);
-    const int q_other_end = bv.nodes[q].r.w;
-    const int fq = min(q, q_other_end), lq = max(q, q_other_end);
-    if (fp <= fq && lq <= lp) return true;
-    if (fq <= fp && lp <= lq) return false;
-    return lp < fq;
+// ---------------------------------------------------- four-wide walk
+// The fast orders walk the four-wide quantized tree (WideNode). Child box k
+// of node w spans w.o + q * 2^e per axis; its slab distances are formed as
+// fma(q, 2^e / d, (w.o - o) / d), i.e. (b - o) / d with a few roundings of
+// coordinate-sized terms. As with slab_fma, that can move a slab end by a
+// few ulps of the coordinates, far less than the 1e-3 every triangle sits
+// inside its boxes, and the quantized boxes contain the exact ones, so no
+// box that holds a reachable hit is culled and the exact triangle test alone
+// decides t and the primitive.
+//
+// Ties: the reference tests a leaf when it visits the leaf's parent, parents
+// in left-first pre-order, left side first, and keeps the first of equal t.
+// Every leaf that reaches the minimum t is tested in any order (its boxes are
+// entered before that t), so keeping the leaf of smallest rank in that order
+// (tie[], computed by the build) returns the reference's (t, primitive).
+
+// Bit k set when child k's box is entered before s.t; tn[k] its entry.
+MCPT_DEV uint32_t wide_boxes(const float4& o, const uint4& q0, const uint4& q1, const Trav& s, float tn[4]) {
+    const uint32_t meta = __float_as_uint(o.w);
+    const int cnt = (int)(meta >> 24);
+    const float ax = __uint_as_float((meta & 255u) << 23) * s.inv.x;
+    const float ay = __uint_as_float(((meta >> 8) & 255u) << 23) * s.inv.y;
+    const float az = __uint_as_float(((meta >> 16) & 255u) << 23) * s.inv.z;
+    const float bx = __builtin_fmaf(o.x, s.inv.x, -s.oi.x);
+    const float by = __builtin_fmaf(o.y, s.inv.y, -s.oi.y);
+    const float bz = __builtin_fmaf(o.z, s.inv.z, -s.oi.z);
+    uint32_t hit = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int sh = 8 * k;
+        const float lx = __builtin_fmaf((float)((q0.x >> sh) & 255u), ax, bx);
+        const float hx = __builtin_fmaf((float)((q0.w >> sh) & 255u), ax, bx);
+        const float ly = __builtin_fmaf((float)((q0.y >> sh) & 255u), ay, by);
+        const float hy = __builtin_fmaf((float)((q1.x >> sh) & 255u), ay, by);
+        const float lz = __builtin_fmaf((float)((q0.z >> sh) & 255u), az, bz);
+        const float hz = __builtin_fmaf((float)((q1.y >> sh) & 255u), az, bz);
+        const float tnear = fmaxf(fmaxf(0.0001f, fminf(lx, hx)), fmaxf(fminf(ly, hy), fminf(lz, hz)));
+        const float tfar = fminf(fminf(s.t, fmaxf(lx, hx)), fminf(fmaxf(ly, hy), fmaxf(lz, hz)));
+        tn[k] = tnear;
+        if (k < cnt && !(tfar <= tnear)) hit |= 1u << k;
+    }
+    return hit;
 }
 
-MCPT_DEV void test_leaf_ordered(const BvhView& bv, Trav& s, int leaf, int parent, int parent_other_end, int side) {
+MCPT_DEV void test_leaf_ranked(const BvhView& bv, Trav& s, int leaf) {
     vec3 v0, v1, v2, n;
     load_leaf(bv.leaf_tri, leaf, v0, v1, v2, n);
     float t;
     if (hit_triangle_n(v0, v1, v2, n, s.o, s.d, t) && t > 0.0001f) {
-        if (t < s.t || (t == s.t && s.leaf >= 0 && ref_earlier(bv, parent, parent_other_end, side, s.best))) {
+        if (t < s.t || (t == s.t && s.leaf >= 0 && bv.tie[leaf] < bv.tie[s.leaf])) {
             s.t = t;
             s.leaf = leaf;
-            s.best = (parent << 1) | side;
         }
     }
 }
 
-// One iteration: both boxes, leaf children, then the nearer internal child
-// (the left one on equal entry distances) with the other one pushed.
+
+// The wide walk's stack holds one group per wide level: two ints, the wide
+// node and the slots of its children still to visit, in visit order (bits
+// 0-5, two bits each) with their count (bits 8-9). Popping re-reads only
+// the node's 16-byte child references. Height: two ints per wide level,
+// i.e. the binary depth rounded up to even.
+MCPT_DEV void push_group(int* stk, int stride, Trav& s, int node, uint32_t seq) {
+    stk[(2 * s.sp) * stride] = node;
+    stk[(2 * s.sp + 1) * stride] = (int)seq;
+    s.sp++;
+}
+
+// Moves to the next child of the top group; false when the stack is empty.
+MCPT_DEV bool pop_group(const BvhView& bv, int* stk, int stride, Trav& s) {
+    if (s.sp == 0) return false;
+    const int top = s.sp - 1;
+    const int w = stk[(2 * top) * stride];
+    const uint32_t seq = (uint32_t)stk[(2 * top + 1) * stride];
+    const uint32_t k = seq & 3u, cnt = seq >> 8;
+    const int4 ch = bv.wide[w].child;
+    s.node = k == 0 ? ch.x : k == 1 ? ch.y : k == 2 ? ch.z : ch.w;
+    if (cnt > 1) stk[(2 * top + 1) * stride] = (int)(((seq & 0x3Fu) >> 2) | ((cnt - 1) << 8));
+    else s.sp = top;
+    return true;
+}
+
+MCPT_DEV void cswap(float& da, int& ka, float& db, int& kb) {
+    if (db < da) {
+        const float d = da;
+        da = db;
+        db = d;
+        const int k = ka;
+        ka = kb;
+        kb = k;
+    }
+}
+
+// One closest-hit step on a wide node: the hit leaf children are tested,
+// then the nearest hit inner child is visited next and the others wait on
+// the stack as one group, nearest first.
 template <bool COUNT>
-MCPT_DEV bool step_ordered(const BvhView& bv, Trav& s, int* stk, int stride, uint32_t& ns, uint32_t& nl) {
-    const BvhNode* np = bv.nodes + s.node;
-    const float4 a = np->a, b = np->b, c = np->c;
-    const int4 r = np->r;
-    float nL, nR;
-    const bool oL = slab_fma(a.x, a.y, a.z, a.w, b.x, b.y, s.oi, s.inv, s.t, nL);
-    const bool oR = slab_fma(b.z, b.w, c.x, c.y, c.z, c.w, s.oi, s.inv, s.t, nR);
-    const bool lL = r.x < 0, rL = r.y < 0;
+MCPT_DEV bool step_wide(const BvhView& bv, Trav& s, int* stk, int stride, uint32_t& ns, uint32_t& nl) {
+    const WideNode* wp = bv.wide + s.node;
+    const float4 o = wp->o;
+    const uint4 q0 = wp->q0, q1 = wp->q1;
+    const int4 ch = wp->child;
     if (COUNT) ns++;
-    int first = -1, second = -1;
-    if (oL && lL) first = 0;
-    if (oR && rL) {
-        if (first < 0) first = 1;
-        else second = 1;
+    float tn[4];
+    const uint32_t hit = wide_boxes(o, q0, q1, s, tn);
+    const int c[4] = {ch.x, ch.y, ch.z, ch.w};
+    uint32_t leaves = 0, inner = 0;
+    float d[4];
+    int k[4] = {0, 1, 2, 3};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const bool h = (hit >> j) & 1u;
+        if (h) {
+            if (c[j] < 0) leaves |= 1u << j;
+            else inner |= 1u << j;
+        }
+        d[j] = (h && c[j] >= 0) ? tn[j] : __builtin_inff();
     }
-    while (first >= 0) {
+    while (leaves) {
+        const int j = __builtin_ctz(leaves);
+        leaves &= leaves - 1u;
         if (COUNT) nl++;
-        test_leaf_ordered(bv, s, first ? ~r.y : ~r.x, s.node, r.w, first);
-        first = second;
-        second = -1;
+        test_leaf_ranked(bv, s, ~(j == 0 ? c[0] : j == 1 ? c[1] : j == 2 ? c[2] : c[3]));
     }
-    const bool tL = oL && !lL, tR = oR && !rL;
-    if (!tL && !tR) {
-        if (s.sp == 0) return true;
-        s.sp--;
-        s.node = stk[s.sp * stride];
-    } else if (tL && tR) {
-        const bool left_first = !(nR < nL);
-        stk[s.sp * stride] = left_first ? r.y : r.x;
-        s.sp++;
-        s.node = left_first ? r.x : r.y;
-    } else {
-        s.node = tL ? r.x : r.y;
-    }
+    if (!inner) return !pop_group(bv, stk, stride, s);
+    // order the inner children by entry distance (non-inner slots sort last)
+    cswap(d[0], k[0], d[1], k[1]);
+    cswap(d[2], k[2], d[3], k[3]);
+    cswap(d[0], k[0], d[2], k[2]);
+    cswap(d[1], k[1], d[3], k[3]);
+    cswap(d[1], k[1], d[2], k[2]);
+    const uint32_t n = __builtin_popcount(inner);
+    if (n > 1) push_group(stk, stride, s, s.node, (uint32_t)k[1] | (uint32_t)k[2] << 2 | (uint32_t)k[3] << 4 | (n - 1) << 8);
+    s.node = k[0] == 0 ? c[0] : k[0] == 1 ? c[1] : k[0] == 2 ? c[2] : c[3];
     return false;
 }
 
-// Full closest-hit traversal. REF selects the reference's visit order (for
-// step and leaf-test counts that match the CPU checker); otherwise near
-// child first. Both return the same t and leaf.
+// Any-hit step on a wide node, children in left-to-right order; returns 0
+// while walking, 1 when the ray is unblocked, 2 when blocked.
+MCPT_DEV int step_wide_any(const BvhView& bv, Trav& s, int* stk, int stride) {
+    const WideNode* wp = bv.wide + s.node;
+    const float4 o = wp->o;
+    const uint4 q0 = wp->q0, q1 = wp->q1;
+    const int4 ch = wp->child;
+    float tn[4];
+    const uint32_t hit = wide_boxes(o, q0, q1, s, tn);
+    const int c[4] = {ch.x, ch.y, ch.z, ch.w};
+    uint32_t leaves = 0, inner = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if ((hit >> j) & 1u) {
+            if (c[j] < 0) leaves |= 1u << j;
+            else inner |= 1u << j;
+        }
+    }
+    while (leaves) {
+        const int j = __builtin_ctz(leaves);
+        leaves &= leaves - 1u;
+        if (test_leaf_any(bv, s, ~(j == 0 ? c[0] : j == 1 ? c[1] : j == 2 ? c[2] : c[3]))) return 2;
+    }
+    if (!inner) return pop_group(bv, stk, stride, s) ? 0 : 1;
+    // the leftmost inner child next, the others as one group, left to right
+    const int first = __builtin_ctz(inner);
+    uint32_t rest = inner & (inner - 1u);
+    const uint32_t n = __builtin_popcount(rest);
+    if (n) {
+        uint32_t seq = 0;
+        for (int i = 0; i < 3 && rest; i++) {
+            seq |= (uint32_t)__builtin_ctz(rest) << (2 * i);
+            rest &= rest - 1u;
+        }
+        push_group(stk, stride, s, s.node, seq | n << 8);
+    }
+    s.node = first == 0 ? c[0] : first == 1 ? c[1] : first == 2 ? c[2] : c[3];
+    return 0;
+}
+
+// Full closest-hit traversal. REF selects the reference's visit order on the
+// binary tree (for step and leaf-test counts that match the CPU checker);
+// otherwise the four-wide tree, near child first. Both return the same t and
+// leaf.
 template <bool COUNT, bool REF = false>
 MCPT_DEV void trace_closest(const BvhView& bv, Trav& s, int* stk, int stride, uint32_t& ns, uint32_t& nl) {
     if (REF) {
         while (!step_closest<COUNT>(bv, s, stk, stride, ns, nl)) {
         }
     } else {
-        while (!step_ordered<COUNT>(bv, s, stk, stride, ns, nl)) {
+        while (!step_wide<COUNT>(bv, s, stk, stride, ns, nl)) {
         }
     }
 }
 
-// BVH::traverseAnyHit (src/path_tracer.cu:1152-1207): boxes are culled with
-// the fixed limit and the walk stops at the first blocker, in the
-// reference's left-first order (measured faster than near-first for shadow
-// rays, which mostly start on a surface inside overlapping boxes). The
-// answer does not depend on the order. One iteration per call: returns 0
-// while walking, 1 when the ray is unblocked, 2 when blocked.
-MCPT_DEV int step_any(const BvhView& bv, Trav& s, int* stk, int stride) {
-    const BvhNode* np = bv.nodes + s.node;
-    const float4 a = np->a, b = np->b, c = np->c;
-    const int4 r = np->r;
-    float nL, nR;
-    const bool oL = slab_fma(a.x, a.y, a.z, a.w, b.x, b.y, s.oi, s.inv, s.t, nL);
-    const bool oR = slab_fma(b.z, b.w, c.x, c.y, c.z, c.w, s.oi, s.inv, s.t, nR);
-    const bool lL = r.x < 0, rL = r.y < 0;
-    int first = -1, second = -1;
-    if (oL && lL) first = ~r.x;
-    if (oR && rL) {
-        if (first < 0) first = ~r.y;
-        else second = ~r.y;
-    }
-    bool blocked = false;
-    while (first >= 0 && !blocked) {
-        blocked = test_leaf_any(bv, s, first);
-        first = second;
-        second = -1;
-    }
-    if (blocked) return 2;
-    const bool tL = oL && !lL, tR = oR && !rL;
-    if (!tL && !tR) {
-        if (s.sp == 0) return 1;
-        s.sp--;
-        s.node = stk[s.sp * stride];
-    } else {
-        if (tL && tR) {
-            stk[s.sp * stride] = r.y;
-            s.sp++;
-        }
-        s.node = tL ? r.x : r.y;
-    }
-    return 0;
-}
-
+// BVH::traverseAnyHit (src/path_tracer.cu:1152-1207) on the four-wide tree:
+// boxes are culled with the fixed limit and the walk stops at the first
+// blocker, children in left-to-right order (measured faster than near-first
+// for shadow rays, which mostly start on a surface inside overlapping
+// boxes). The answer does not depend on the order.
 MCPT_DEV bool trace_any(const BvhView& bv, Trav& s, int* stk, int stride) {
     int st;
-    while ((st = step_any(bv, s, stk, stride)) == 0) {
+    while ((st = step_wide_any(bv, s, stk, stride)) == 0) {
     }
     return st == 2;
 }
