@@ -509,7 +509,7 @@ def main():
     ap.add_argument("--tris", type=int, default=1048576)
     ap.add_argument("--mode", default="persistent", choices=["megakernel", "wavefront", "persistent"])
     ap.add_argument("--refill", type=int, default=0)
-    ap.add_argument("--persistent-shadow", action="store_true", help="shadow rays on the persistent scheme too")
+    ap.add_argument("--shadow-per-ray", action="store_true", help="shadow rays one lane per ray (default: persistent)")
     ap.add_argument("--batch", type=int, default=0, help="samples in flight per wavefront pass (0 = auto)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -581,7 +581,7 @@ def main():
     mode = {"megakernel": mcpt.MODE_MEGAKERNEL, "wavefront": mcpt.MODE_WAVEFRONT,
             "persistent": mcpt.MODE_PERSISTENT}[args.mode]
     W, H, K = args.width, args.height, args.steps
-    xf = mcpt.RENDER_PERSISTENT_SHADOW if args.persistent_shadow else 0
+    xf = mcpt.RENDER_SHADOW_PER_RAY if args.shadow_per_ray else 0
 
     t0 = time.time()
     hot, cold, mats = mcpt.gen_scene(mcpt.GEN_SOUP, seed=args.seed, param=args.tris)

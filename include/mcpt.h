@@ -217,9 +217,12 @@ int mcpt_trace_any(mcpt_scene scene, const mcpt_ray* rays, int32_t n, uint8_t* o
                                           order (left child first) instead of
                                           near child first; same hits, the
                                           reference's step/leaf counts       */
-#define MCPT_RENDER_PERSISTENT_SHADOW 8u /* persistent mode: shadow rays on the
-                                            persistent scheme too (measured
-                                            slower than one lane per ray)    */
+#define MCPT_RENDER_PERSISTENT_SHADOW 8u /* accepted for compatibility: the
+                                            persistent mode now traces shadow
+                                            rays on the persistent scheme by
+                                            default (scenes without media)   */
+#define MCPT_RENDER_SHADOW_PER_RAY 16u   /* persistent mode: shadow rays one
+                                            lane per ray instead            */
 
 typedef struct mcpt_render_opts {
     int32_t mode;

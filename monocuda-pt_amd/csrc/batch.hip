@@ -35,7 +35,7 @@ namespace mcpt {
 namespace {
 
 struct Seg {  // one scene of a batched build
-    int off, n, ioff, wbase;  // first triangle, count, first internal node, first wide node (global)
+    int off, n, ioff, pad;  // first triangle, count, first internal node
     const float* hot;
     BvhNode* nodes;
     LeafTri* leaf_tri;
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_b_collapse(const Seg* seg, int nseg,
     const int q = blockIdx.x * kBlock + threadIdx.x;
     if (q >= n || (level[q] & 1u)) return;
     const Seg& S = seg[seg_of_node(seg, nseg, q)];
-    collapse_node(S.nodes, q - S.ioff, widx, S.ioff, S.wbase, S.wide);
+    collapse_node(S.nodes, q - S.ioff, widx, S.ioff, S.wide);
 }
 
 // refitBoundsKernel (:1398-1415) for the nodes of one level, all scenes
@@ -288,7 +288,7 @@ struct BScene {  // one scene of a batched render
     int npix;
 };
 
-enum { C_NS0 = 0, C_NS1 = 32, C_WORK = 64, C_EXT = 96, C_SHD = 128, C_NCOUNTERS = 160 };
+enum { C_NS0 = 0, C_NS1 = 32, C_WORK = 64, C_EXT = 96, C_SHD = 128, C_WORK_SHADOW = 160, C_NCOUNTERS = 192 };
 
 __device__ __forceinline__ unsigned long long* ns_slot(int* c, int q) {
     return reinterpret_cast<unsigned long long*>(c + (q ? C_NS1 : C_NS0));
@@ -331,7 +331,10 @@ __global__ void k_bb_bounce(int* c, int cur) {
     }
 }
 __global__ void k_bb_shadow_total(int* c, int cur) {
-    if (threadIdx.x == 0) *reinterpret_cast<unsigned long long*>(c + C_SHD) += (unsigned long long)shadow_len(c, 1 - cur);
+    if (threadIdx.x == 0) {
+        *reinterpret_cast<unsigned long long*>(c + C_SHD) += (unsigned long long)shadow_len(c, 1 - cur);
+        c[C_WORK_SHADOW] = 0;
+    }
 }
 
 // generateCameraRaysKernel (:3342-3374) for every scene's pixels and the
@@ -365,6 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_bb_extend(const BvhView* bvhs, BWs w
     extern __shared__ int lds_stack[];
     int* stk = lds_stack + threadIdx.x;
     const int total = queue_len(ws.counters, cur);
+    if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
     const int lane = lane_id();
     Trav s;
     BvhView bv;
@@ -459,21 +463,72 @@ __global__ __launch_bounds__(kBlock) void k_bb_shade(const BScene* tab, BWs ws, 
     }
 }
 
-// shadowKernel (:3698-3728)
-__global__ __launch_bounds__(kBlock) void k_bb_shadow(const BScene* tab, BWs ws, int cur) {
-    extern __shared__ int lds_stack[];
-    const int slot = blockIdx.x * kBlock + threadIdx.x;
-    if (slot >= shadow_len(ws.counters, 1 - cur)) return;
-    const ShadowItem si = ws.sq[slot];
+// the light a shadow ray carries, into its path's sample
+__device__ __forceinline__ void add_shadow(float4* smp, const ShadowItem& si, float tr) {
     const int path = __float_as_int(si.dp.w);
-    const float tr = shadow_transmittance(tab[__float_as_int(si.c.w)].sv, vec3(si.om.x, si.om.y, si.om.z),
-                                          vec3(si.dp.x, si.dp.y, si.dp.z), si.om.w, lds_stack + threadIdx.x, kBlock);
-    if (tr > 0.0f) {
-        float4 v = ws.smp[path];
-        v.x = v.x + si.c.x * tr;
-        v.y = v.y + si.c.y * tr;
-        v.z = v.z + si.c.z * tr;
-        ws.smp[path] = v;
+    float4 v = smp[path];
+    v.x = v.x + si.c.x * tr;
+    v.y = v.y + si.c.y * tr;
+    v.z = v.z + si.c.z * tr;
+    smp[path] = v;
+}
+
+// shadowKernel (:3698-3728) for the shadow rays of every scene, on the
+// persistent scheme of k_bb_extend: a lane walks the any-hit steps of its
+// ray and refills when done. A ray of a scene with a medium (closest hits
+// through the medium) is handled whole in one iteration.
+__global__ __launch_bounds__(kBlock) void k_bb_shadow_persistent(const BScene* tab, BWs ws, int cur, int refill) {
+    extern __shared__ int lds_stack[];
+    int* stk = lds_stack + threadIdx.x;
+    const int total = shadow_len(ws.counters, 1 - cur);
+    if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
+    const int lane = lane_id();
+    Trav s;
+    BvhView bv;
+    int bv_scene = -1;
+    bool has = false;
+    int slot = -1;
+    bool exhausted = false;
+    while (true) {
+        if (!exhausted) {
+            const uint64_t idle = ballot64(!has);
+            const int nidle = __popcll(idle);
+            if (nidle >= refill) {
+                const int leader = __builtin_ctzll(idle);
+                int first = 0;
+                if (lane == leader) first = atomicAdd(&ws.counters[C_WORK_SHADOW], nidle);
+                first = __shfl(first, leader);
+                if (!has) {
+                    const int q = first + rank_below(idle);
+                    if (q < total) {
+                        const ShadowItem si = ws.sq[q];
+                        const int sc = __float_as_int(si.c.w);
+                        const vec3 o(si.om.x, si.om.y, si.om.z), d(si.dp.x, si.dp.y, si.dp.z);
+                        if (tab[sc].sv.has_medium) {
+                            const float tr = shadow_transmittance(tab[sc].sv, o, d, si.om.w, stk, kBlock);
+                            if (tr > 0.0f) add_shadow(ws.smp, si, tr);
+                        } else {
+                            if (sc != bv_scene) {
+                                bv = tab[sc].sv.bvh;
+                                bv_scene = sc;
+                            }
+                            trav_begin(s, o, d, si.om.w);
+                            slot = q;
+                            has = true;
+                        }
+                    }
+                }
+                if (first + nidle >= total) exhausted = true;
+            }
+        }
+        if (exhausted && ballot64(has) == 0) break;
+        if (has) {
+            const int st = step_wide_any(bv, s, stk, kBlock);
+            if (st != 0) {
+                if (st == 1) add_shadow(ws.smp, ws.sq[slot], 1.0f);
+                has = false;
+            }
+        }
     }
 }
 
@@ -584,7 +639,8 @@ int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const
                 k_bb_extend<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(dbvh, ws, cur, refill);
                 k_bb_shade<<<grid, kBlock, 0, st>>>(dtab, ws, cur, d, (int)P);
                 k_bb_shadow_total<<<1, 64, 0, st>>>(ws.counters, cur);
-                k_bb_shadow<<<grid, kBlock, lds, st>>>(dtab, ws, cur);
+                k_bb_shadow_persistent<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(dtab, ws, cur,
+                                                                                                refill);
                 launches++;
                 cur = 1 - cur;
             }
@@ -674,8 +730,9 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
         MCPT_TRY(s->leaf_prim.alloc(ctx, s->n));
         MCPT_TRY(s->root_box.alloc(ctx, 6));
         MCPT_TRY(s->tie.alloc(ctx, s->n));
+        MCPT_TRY(s->wide.alloc(ctx, wide_capacity(s->n)));
         seg[k] = Seg{(int)total, s->n, (int)total_int, 0, s->hot.ptr, s->nodes.ptr, s->leaf_tri.ptr, s->leaf_prim.ptr,
-                     s->root_box.ptr, s->tie.ptr, nullptr};
+                     s->root_box.ptr, s->tie.ptr, s->wide.ptr};
         total += s->n;
         total_int += s->n - 1;
         chunk_off[k + 1] = chunk_off[k] + (s->n + kChunk - 1) / kChunk;
@@ -783,20 +840,8 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
         if (depths[k2] < 1 || depths[k2] > 64) bad = k2;
         max_d = std::max(max_d, depths[k2]);
     }
-    // each scene's four-wide tree, in its own buffer
-    int status = MCPT_OK;
-    for (int k2 = 0, wb = 0; k2 < ns && e == hipSuccess && bad < 0 && status == MCPT_OK; k2++) {
-        mcpt_scene_s* s = scenes[multi[k2]];
-        const int nw = depths[ns + 65 + k2];
-        status = s->wide.alloc(ctx, nw);
-        s->n_wide = nw;
-        seg[k2].wide = s->wide.ptr;
-        seg[k2].wbase = wb;
-        wb += nw;
-    }
-    if (e == hipSuccess && bad < 0 && status == MCPT_OK) {
+    if (e == hipSuccess && bad < 0) {
         const int n_int = (int)total_int;
-        e = hipMemcpyAsync(dseg, seg.data(), ns * sizeof(Seg), hipMemcpyHostToDevice, st);
         const int* off = depths.data() + ns;
         for (int L = max_d - 1; L >= 0 && e == hipSuccess; L--)
             if (off[L + 1] > off[L])
@@ -819,12 +864,12 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
     if (e1) (void)hipEventDestroy(e1);
     dev_free(ctx, mem, bytes);
     if (e != hipSuccess) return set_error(MCPT_ERR_HIP, "batched LBVH build: %s", hipGetErrorString(e));
-    if (status < 0) return status;
     if (bad >= 0)
         return set_error(MCPT_ERR_INVALID, "scene %d: BVH depth %d outside [1, 64]", multi[bad], depths[bad]);
     for (int k2 = 0; k2 < ns; k2++) {
         mcpt_scene_s* s = scenes[multi[k2]];
         s->max_depth = depths[k2];
+        s->n_wide = depths[ns + 65 + k2];
         s->built = true;
         s->last_build_ms = ms;
     }

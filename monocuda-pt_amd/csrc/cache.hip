@@ -465,6 +465,7 @@ __global__ __launch_bounds__(kBlock) void k_cc_extend(SceneView sv, CacheWs ws, 
     extern __shared__ int lds_stack[];
     int* stk = lds_stack + threadIdx.x;
     const int total = queue_len(ws.counters, cur);
+    if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
     const int lane = lane_id();
     Trav s;
     bool has = false;

@@ -180,7 +180,7 @@ __global__ __launch_bounds__(kBlock) void k_refit_level(BvhNode* nodes, const in
 __global__ __launch_bounds__(kBlock) void k_collapse(const BvhNode* nodes, const uint32_t* level, const int* widx,
                                                      int n, WideNode* wide) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n && !(level[i] & 1u)) collapse_node(nodes, i, widx, 0, 0, wide);
+    if (i < n && !(level[i] & 1u)) collapse_node(nodes, i, widx, 0, wide);
 }
 
 __global__ void k_fill_int(int* p, int n, int v) {
@@ -216,6 +216,7 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
     MCPT_TRY(s->leaf_prim.alloc(ctx, n));
     MCPT_TRY(s->root_box.alloc(ctx, 6));
     MCPT_TRY(s->tie.alloc(ctx, n));
+    MCPT_TRY(s->wide.alloc(ctx, wide_capacity(n)));
 
     const int tiles = blocks_for(n, kTile);
     const int n_partial = std::min(1024, blocks_for(n));
@@ -317,7 +318,6 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
         const int refs[2] = {~0, ~0};
         WideNode wroot;
         wide_encode(cb, refs, 2, &wroot);
-        STEP(s->wide.alloc(ctx, 1));
         HSTEP(hipMemcpy(s->wide.ptr, &wroot, sizeof(wroot), hipMemcpyHostToDevice));
         s->n_wide = 1;
         s->max_depth = 1;
@@ -351,7 +351,6 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
         // free after the level sort), then one wide node per such node
         int n_wide = 0;
         for (int L = 0; L < 64; L += 2) n_wide += off[L + 1] - off[L];
-        STEP(s->wide.alloc(ctx, n_wide));
         s->n_wide = n_wide;
         k_wide_flags<<<blocks_for(ni), kBlock, 0, st>>>(kout, ni, vout);
         scan_exclusive(vout, ni, ssum.ptr, st);

