@@ -187,9 +187,12 @@ typedef struct mcpt_lbvh_dump {
     float build_ms;         /* out: device time of the whole build          */
 } mcpt_lbvh_dump;
 
-/* Morton codes, stable radix sort, Karras radix tree and atomic refit, all
- * on the GPU (BVH::generateHierarchy + the host part of finalizeScene and
- * the host std::sort, src/path_tracer.cu:1930-1972, :4224-4245, :1063-1090). */
+/* Morton codes, stable radix sort, Karras radix tree and refit, all on the
+ * GPU (BVH::generateHierarchy + the host part of finalizeScene and the host
+ * std::sort, src/path_tracer.cu:1930-1972, :4224-4245, :1063-1090); the
+ * refit runs level by level, and the build also collapses the binary tree
+ * into the four-wide quantized tree the fast traversal walks. The dump holds
+ * the binary tree in the reference's node convention. */
 int mcpt_lbvh_build(mcpt_scene scene, mcpt_lbvh_dump* dump /* may be NULL */);
 
 /* ------------------------------------------------------------ trace hooks */
