@@ -195,6 +195,18 @@ typedef struct mcpt_lbvh_dump {
  * the binary tree in the reference's node convention. */
 int mcpt_lbvh_build(mcpt_scene scene, mcpt_lbvh_dump* dump /* may be NULL */);
 
+/* Parity hook for the four-wide tree the fast traversal walks (no
+ * counterpart in the reference, which walks its binary tree). nodes
+ * receives up to cap 64-byte records: float origin[3]; uint32 meta (bits
+ * 0-7, 8-15, 16-23: x, y, z scale exponents biased by 127; bits 24-31: child
+ * count); uint8 q[6][4] (lo.x, lo.y, lo.z, hi.x, hi.y, hi.z of children 0-3:
+ * child k's box is origin + q * 2^(e - 127) per axis); 8 unused bytes;
+ * int32 child[4] (>= 0 wide node, < 0 leaf ~r). Wide node i is the i-th
+ * binary internal node at an even level (root 0). n_wide receives the
+ * count; tie (scene size ints, may be NULL) each leaf's rank in the
+ * reference's leaf-test order. The scene must be built. */
+int mcpt_lbvh_dump_wide(mcpt_scene scene, void* nodes, int32_t cap, int32_t* n_wide, int32_t* tie);
+
 /* ------------------------------------------------------------ trace hooks */
 /* Closest hit for n rays (BVH::traverseRay / stepTraversal,
  * src/path_tracer.cu:1092-1150, :1262-1321). `steps` and `leaves`, when not

@@ -54,6 +54,10 @@ LIGHT_DTYPE = np.dtype([("tri", HOT_DTYPE), ("emission", "<f4", 3), ("area", "<f
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("time", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_type", "<i4"), ("prim_index", "<i4"), ("did_hit", "<i4")])
 AABB_DTYPE = np.dtype([("min", "<f4", 3), ("pad0", "<f4"), ("max", "<f4", 3), ("pad1", "<f4")])
+# four-wide node of mcpt_lbvh_dump_wide (64 bytes)
+WIDE_DTYPE = np.dtype([("origin", "<f4", 3), ("meta", "<u4"), ("q", "u1", (6, 4)), ("pad", "u1", 8),
+                       ("child", "<i4", 4)])
+assert WIDE_DTYPE.itemsize == 64
 assert HOT_DTYPE.itemsize == 36 and COLD_DTYPE.itemsize == 64 and MAT_DTYPE.itemsize == 48
 assert LIGHT_DTYPE.itemsize == 64 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16
 assert AABB_DTYPE.itemsize == 32
@@ -171,6 +175,7 @@ _SIGS = {
     "mcpt_scene_lights": ([VP, VP], C.c_int),
     "mcpt_scene_env_tables": ([VP, VP, VP, C.POINTER(C.c_float)], C.c_int),
     "mcpt_lbvh_build": ([VP, C.POINTER(LbvhDump)], C.c_int),
+    "mcpt_lbvh_dump_wide": ([VP, VP, C.c_int32, C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_trace_closest": ([VP, VP, C.c_int32, VP, VP, VP], C.c_int),
     "mcpt_trace_any": ([VP, VP, C.c_int32, VP], C.c_int),
     "mcpt_camera_init": ([C.POINTER(CameraDesc), C.POINTER(Camera)], C.c_int),
@@ -407,6 +412,16 @@ class Scene:
         out["max_depth"] = d.max_depth
         out["build_ms"] = d.build_ms
         return out
+
+    def dump_wide(self):
+        """The four-wide tree (WIDE_DTYPE records, root first) and each leaf's
+        rank in the reference's leaf-test order."""
+        n = C.c_int32()
+        _check(_lib.mcpt_lbvh_dump_wide(self.h, None, 0, C.byref(n), None), "mcpt_lbvh_dump_wide")
+        nodes = np.zeros(n.value, WIDE_DTYPE)
+        tie = np.zeros(self.n, np.int32)
+        _check(_lib.mcpt_lbvh_dump_wide(self.h, _ptr(nodes), n.value, C.byref(n), _ptr(tie)), "mcpt_lbvh_dump_wide")
+        return nodes, tie
 
     def lights(self):
         n = C.c_int32()

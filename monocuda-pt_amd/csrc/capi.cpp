@@ -1,5 +1,6 @@
 // capi.cpp — the extern "C" boundary declared in include/mcpt.h: contexts,
 // scenes, films and argument checking around the kernel launchers.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -307,6 +308,19 @@ int mcpt_trace_any(mcpt_scene scene, const mcpt_ray* rays, int32_t n, uint8_t* o
     CHECK_ARG(scene, "scene is NULL");
     MCPT_HIP(hipSetDevice(scene->ctx->device));
     return trace_any_host(scene, rays, n, occluded);
+}
+
+int mcpt_lbvh_dump_wide(mcpt_scene scene, void* nodes, int32_t cap, int32_t* n_wide, int32_t* tie) {
+    CHECK_ARG(scene && n_wide && cap >= 0, "bad arguments");
+    if (!scene->built) return set_error(MCPT_ERR_INVALID, "scene is not built");
+    MCPT_HIP(hipSetDevice(scene->ctx->device));
+    MCPT_HIP(hipStreamSynchronize(scene->ctx->stream));
+    *n_wide = scene->n_wide;
+    if (nodes && cap > 0)
+        MCPT_HIP(hipMemcpy(nodes, scene->wide.ptr, (size_t)std::min(cap, scene->n_wide) * sizeof(WideNode),
+                           hipMemcpyDeviceToHost));
+    if (tie) MCPT_HIP(hipMemcpy(tie, scene->tie.ptr, (size_t)scene->n * sizeof(int), hipMemcpyDeviceToHost));
+    return MCPT_OK;
 }
 
 int mcpt_camera_init(const mcpt_camera_desc* desc, mcpt_camera* out) {

@@ -65,6 +65,125 @@ def test_lbvh_full_soup(mcpt, ctx, oracle):
     _check_lbvh(mcpt, ctx, oracle, hot, cold, mats).close()
 
 
+def _reference_leaf_order(d):
+    """Each leaf's position in the reference's leaf-test order, from a
+    left-first pre-order walk of the checker's tree in which a node's leaf
+    children are tested when it is visited, left then right
+    (BVH::traverseRay, src/path_tracer.cu:1102-1147)."""
+    L, R, Ll, Rl = d["left"], d["right"], d["left_is_leaf"], d["right_is_leaf"]
+    rank = np.zeros(len(d["leaf_parent"]), np.int64)
+    pos, stack = 0, [0] if len(L) else []
+    while stack:
+        p = stack.pop()
+        for c, is_leaf in ((L[p], Ll[p]), (R[p], Rl[p])):
+            if is_leaf:
+                rank[c] = pos
+                pos += 1
+        if not Rl[p]:
+            stack.append(R[p])
+        if not Ll[p]:
+            stack.append(L[p])
+    return rank
+
+
+def _box(b):
+    return np.concatenate([b["min"], b["max"]]).astype(np.float32)
+
+
+def _check_wide(mcpt, scene, ref):
+    """The four-wide tree against the checker's binary tree: wide node i is
+    the i-th even-level binary node, its children are the binary node's leaf
+    children and its inner children's children in left-to-right order, every
+    decoded box contains the exact one and exceeds it by less than one
+    quantization step per side, the scale per axis is the smallest power of
+    two (>= 2^-126) that spans the node in 255 steps, and the leaf ranks
+    follow the reference's leaf-test order."""
+    nodes, tie = scene.dump_wide()
+    L, R, Ll, Rl = ref["left"], ref["right"], ref["left_is_leaf"], ref["right_is_leaf"]
+    ni, n = len(L), len(ref["leaf_parent"])
+    order = np.argsort(_reference_leaf_order(ref), kind="stable")
+    assert len(np.unique(tie)) == n and np.all(np.diff(tie[order]) > 0)
+    if ni == 0:
+        assert len(nodes) == 1 and list(nodes[0]["child"][:2]) == [-1, -1]
+        return
+    level = np.full(ni, -1)
+    level[0] = 0
+    stack = [0]
+    while stack:
+        p = stack.pop()
+        for c, is_leaf in ((L[p], Ll[p]), (R[p], Rl[p])):
+            if not is_leaf:
+                level[c] = level[p] + 1
+                stack.append(c)
+    even = np.flatnonzero(level % 2 == 0)
+    assert len(nodes) == len(even)
+    widx = np.full(ni, -1)
+    widx[even] = np.arange(len(even))
+    exp_ref = np.zeros((len(even), 4), np.int64)
+    exp_box = np.zeros((len(even), 4, 6), np.float32)
+    cnt = np.zeros(len(even), np.int64)
+    for i, p in enumerate(even):
+        for c, is_leaf in ((L[p], Ll[p]), (R[p], Rl[p])):
+            kids = [(c, True)] if is_leaf else [(L[c], Ll[c]), (R[c], Rl[c])]
+            for g, g_leaf in kids:
+                exp_ref[i, cnt[i]] = ~g if g_leaf else widx[g]
+                exp_box[i, cnt[i]] = _box(ref["leaf_bounds"][g] if g_leaf else ref["internal_bounds"][g])
+                cnt[i] += 1
+    assert np.array_equal(nodes["meta"] >> 24, cnt)
+    valid = np.arange(4)[None, :] < cnt[:, None]
+    assert np.array_equal(np.where(valid, nodes["child"], 0), np.where(valid, exp_ref, 0))
+    # every leaf once, every wide node but the root once
+    refs = nodes["child"][valid]
+    assert np.array_equal(np.sort(~refs[refs < 0]), np.arange(n))
+    assert np.array_equal(np.sort(refs[refs >= 0]), np.arange(1, len(even)))
+    # decode in float32 exactly as the traversal bounds it
+    e = np.stack([(nodes["meta"] >> (8 * a)) & 255 for a in range(3)], 1).astype(np.int64)
+    scale = np.ldexp(np.float32(1.0), (e - 127).astype(np.int32)).astype(np.float32)  # (W, 3)
+    q = nodes["q"].astype(np.float32)  # (W, 6, 4): lo xyz, hi xyz per child
+    org = nodes["origin"]
+    lo = org[:, :, None] + q[:, 0:3, :] * scale[:, :, None]
+    hi = org[:, :, None] + q[:, 3:6, :] * scale[:, :, None]
+    lo, hi = np.transpose(lo, (0, 2, 1)), np.transpose(hi, (0, 2, 1))  # (W, 4, 3)
+    emin, emax = exp_box[:, :, 0:3], exp_box[:, :, 3:6]
+    v3 = valid[:, :, None]
+    assert np.all(~v3 | (lo <= emin)) and np.all(~v3 | (hi >= emax))
+    s3 = scale[:, None, :]
+    assert np.all(~v3 | (emin - lo < s3)) and np.all(~v3 | (hi - emax < s3))
+    assert np.array_equal(org, np.where(v3, emin, np.inf).min(axis=1))
+    ext = np.where(v3, emax.astype(np.float64), -np.inf).max(axis=1) - org.astype(np.float64)
+    assert np.all(255.0 * scale >= ext)
+    assert np.all((e == 1) | (255.0 * scale / 2 <= ext))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 4097, 65536])
+def test_wide_tree(mcpt, ctx, oracle, n):
+    hot, cold, mats = mcpt.gen_scene(mcpt.GEN_SOUP, seed=n, param=n)
+    if n <= 16:
+        hot, cold = hot[:n], cold[:n]
+    print(f"wide tree n={len(hot)}", flush=True)
+    ref = oracle.Scene(hot, cold, mats).dump()
+    scene = ctx.scene(hot, cold, mats)
+    scene.build()
+    _check_wide(mcpt, scene, ref)
+    scene.close()
+
+
+def test_wide_tree_cornell_and_duplicates(mcpt, ctx, oracle):
+    hot, cold, mats = mcpt.gen_scene(mcpt.GEN_CORNELL)
+    scenes = [(hot, cold, mats)]
+    hot, cold, mats = mcpt.gen_scene(mcpt.GEN_SOUP, seed=3, param=1000)
+    c = (hot["v0"] + hot["v1"] + hot["v2"]) / 3
+    for k in ("v0", "v1", "v2"):
+        hot[k] = hot[k] - c + np.float32(0.5)
+    scenes.append((hot, cold, mats))
+    for hot, cold, mats in scenes:
+        ref = oracle.Scene(hot, cold, mats).dump()
+        scene = ctx.scene(hot, cold, mats)
+        scene.build()
+        _check_wide(mcpt, scene, ref)
+        scene.close()
+
+
 def _rays(n, seed, origin_box=(0.0, 1.0), tmax=1e30):
     rng = np.random.default_rng(seed)
     r = np.zeros(n, mcpt_ray_dtype())
