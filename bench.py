@@ -387,6 +387,8 @@ def cache_bench(mcpt, ctx, args):
         ref += film.read()
         ref_ms += st["total_ms"]
         done += chunk
+        if done % 512 == 0:
+            log(f"cache: reference {done} / {args.cache_ref_spp} spp")
     ref /= done
     log(f"cache: reference {done} spp in {ref_ms / 1e3:.1f} s")
 

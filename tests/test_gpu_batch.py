@@ -44,6 +44,10 @@ def test_batched_build_matches_single_builds(mcpt, ctx):
         hb, sb, lb = b.trace_closest(rays, counts=True)
         assert np.array_equal(ha, hb) and np.array_equal(sa, sb) and np.array_equal(la, lb), n
         assert np.array_equal(a.trace_closest(rays), b.trace_closest(rays)), n
+        # the four-wide tree and leaf ranks are the single build's, bit for bit
+        wa, ta = a.dump_wide()
+        wb, tb = b.dump_wide()
+        assert wa.tobytes() == wb.tobytes() and np.array_equal(ta, tb), n
     for sc in single + batched:
         sc.close()
 
