@@ -229,7 +229,8 @@ def serving_bench(mcpt, ctx, args):
                     f"128^2-1024^2, spp uniform {{1,4,16,64}}, depth {depth}; Poisson arrivals",
         "jobs": len(jobs), "mean_paths_per_job": float(np.mean(paths)),
         "pool_paths": args.serve_pool or 8 << 20,
-        "policy": "shortest-first" if args.serve_policy == mcpt.SCHED_SHORTEST_FIRST else "fifo",
+        "policy": {mcpt.SCHED_SHORTEST_FIRST: "shortest-first", mcpt.SCHED_FIFO: "fifo",
+                   mcpt.SCHED_SHORTEST_FIRST_AGED: "shortest-first-aged"}[args.serve_policy],
         "capacity_jobs_per_s": {"server": srv_cap, "plain": plain_cap, "plain_one_sample_per_pass": plain1_cap},
         "capacity_ratio": {"vs_plain": srv_cap / plain_cap, "vs_plain_one_sample_per_pass": srv_cap / plain1_cap},
         "back_to_back": {"server": srv_dev, "plain": plain_dev},
@@ -543,7 +544,8 @@ def main():
     ap.add_argument("--serve-seed", type=int, default=7)
     ap.add_argument("--serve-depth", type=int, default=16)
     ap.add_argument("--serve-pool", type=int, default=0, help="server path pool (0 = library default)")
-    ap.add_argument("--serve-policy", type=int, default=0, help="0 shortest-first, 1 FIFO")
+    ap.add_argument("--serve-policy", type=int, default=1,
+                    help="0 shortest-first, 1 FIFO (default: best p99), 2 shortest-first aged")
     ap.add_argument("--serve-loads", type=float, nargs="*", default=[0.5, 0.8])
     args = ap.parse_args()
 

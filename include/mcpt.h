@@ -312,6 +312,10 @@ typedef struct mcpt_server_opts {
 
 #define MCPT_SCHED_SHORTEST_FIRST 0 /* least remaining paths to start first */
 #define MCPT_SCHED_FIFO 1           /* admission order                      */
+#define MCPT_SCHED_SHORTEST_FIRST_AGED 2 /* shortest first, but the oldest
+                                        running job may take up to half of
+                                        each step's free slots before the
+                                        others (bounds a large job's wait) */
 
 typedef struct mcpt_job {
     int32_t scene;     /* index into the server's scene list */

@@ -42,7 +42,7 @@ RENDER_SHADOW_PER_RAY = 16
 GEN_CORNELL, GEN_SOUP, GEN_ICOSPHERE, GEN_MEDIUM = 1, 2, 3, 5
 MLP_FEATURES, MLP_INPUTS, MLP_WIDTH, MLP_HIDDEN, MLP_OUTPUTS = 11, 64, 64, 4, 3
 MLP_PARAMS = 4 * 64 * 64 + 3 * 64
-SCHED_SHORTEST_FIRST, SCHED_FIFO = 0, 1
+SCHED_SHORTEST_FIRST, SCHED_FIFO, SCHED_SHORTEST_FIRST_AGED = 0, 1, 2
 
 # numpy layouts of the C records (byte-identical)
 HOT_DTYPE = np.dtype([("v0", "<f4", 3), ("v1", "<f4", 3), ("v2", "<f4", 3)])

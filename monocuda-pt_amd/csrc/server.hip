@@ -545,20 +545,28 @@ void srv_step(mcpt_server_s* s) {
         std::vector<JobHost*>& order = s->order;
         order.clear();
         for (uint64_t id : s->running) order.push_back(&s->jobs[id]);
-        if (s->policy == MCPT_SCHED_SHORTEST_FIRST)  // least paths left to start first, ties by admission
+        // the oldest running job (running holds admission order) and its
+        // reserve: half of this step's free slots under the aged policy
+        JobHost* oldest = order.empty() ? nullptr : order.front();
+        int reserve = s->policy == MCPT_SCHED_SHORTEST_FIRST_AGED ? free / 2 : 0;
+        if (s->policy != MCPT_SCHED_FIFO)  // least paths left to start first, ties by admission
             std::stable_sort(order.begin(), order.end(), [](const JobHost* a, const JobHost* b) {
                 return a->unstarted() < b->unstarted();
             });
-        for (JobHost* jp : order) {
-            JobHost& j = *jp;
-            // open frames
-            while (j.opened < j.job.spp && j.opened - j.committed < j.nframes) j.opened++;
-            while (free > 0 && nq < kMaxQuotas && j.gen_sample < j.opened) {
-                const int n = std::min(j.npix - j.gen_pixel, free);
+        if (reserve > 0 && oldest) {
+            order.erase(std::find(order.begin(), order.end(), oldest));
+            order.insert(order.begin(), oldest);
+        }
+        // new paths of job j into at most `cap` free slots, in its sample order
+        auto take = [&](JobHost& j, int cap) {
+            while (j.opened < j.job.spp && j.opened - j.committed < j.nframes) j.opened++;  // open frames
+            while (free > 0 && cap > 0 && nq < kMaxQuotas && j.gen_sample < j.opened) {
+                const int n = std::min(j.npix - j.gen_pixel, std::min(free, cap));
                 s->h_quotas[nq] = Quota{j.slot, j.gen_sample, j.gen_pixel, n, total};
                 nq++;
                 total += n;
                 free -= n;
+                cap -= n;
                 j.paths += n;
                 j.gen_pixel += n;
                 if (j.gen_pixel == j.npix) {
@@ -566,7 +574,11 @@ void srv_step(mcpt_server_s* s) {
                     j.gen_pixel = 0;
                 }
             }
-        }
+        };
+        // the oldest job first takes at most its reserve; the other slots go
+        // to the jobs in policy order, the oldest included again
+        for (JobHost* jp : order) take(*jp, (reserve > 0 && jp == oldest) ? reserve : free);
+        if (reserve > 0 && free > 0) take(*oldest, free);
     }
     if (s->cur_count == 0 && total == 0) return;
     const int cur = s->cur;
@@ -673,7 +685,8 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
     s->frames_opt = opts ? opts->frames_per_job : 0;
     s->refill = (opts && opts->refill_lanes > 0) ? std::min(64, opts->refill_lanes) : 32;
     s->policy = opts ? opts->policy : MCPT_SCHED_SHORTEST_FIRST;
-    if (s->policy != MCPT_SCHED_SHORTEST_FIRST && s->policy != MCPT_SCHED_FIFO) {
+    if (s->policy != MCPT_SCHED_SHORTEST_FIRST && s->policy != MCPT_SCHED_FIFO &&
+        s->policy != MCPT_SCHED_SHORTEST_FIRST_AGED) {
         delete s;
         return set_error(MCPT_ERR_INVALID, "unknown scheduling policy %d", opts->policy);
     }

@@ -44,10 +44,10 @@ JOBS = [  # scene, width, height, spp, depth, seed
 ]
 
 
-@pytest.mark.parametrize("capacity,frames", [(0, 0), (4096, 2), (3100, 1)])
-def test_server_films_match_render(mcpt, ctx, scenes, capacity, frames):
-    print(f"server mix capacity={capacity} frames={frames}", flush=True)
-    srv = ctx.server([sc for sc, _ in scenes], path_capacity=capacity, frames_per_job=frames)
+@pytest.mark.parametrize("capacity,frames,policy", [(0, 0, 0), (4096, 2, 0), (3100, 1, 1), (4096, 3, 2)])
+def test_server_films_match_render(mcpt, ctx, scenes, capacity, frames, policy):
+    print(f"server mix capacity={capacity} frames={frames} policy={policy}", flush=True)
+    srv = ctx.server([sc for sc, _ in scenes], path_capacity=capacity, frames_per_job=frames, policy=policy)
     ids = []
     for (si, w, h, spp, depth, seed) in JOBS:
         cam = mcpt.camera_init(mcpt.gen_camera(scenes[si][1], w, h))
