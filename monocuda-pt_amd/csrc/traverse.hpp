@@ -149,6 +149,11 @@ MCPT_DEV bool step_closest(const BvhView& bv, Trav& s, int* stk, int stride, uin
 // (tie[], computed by the build) returns the reference's (t, primitive).
 
 // Bit k set when child k's box is entered before s.t; tn[k] its entry.
+// The slab that a ray enters first on an axis is the lower bound when the
+// direction component is positive and the upper one when it is negative
+// (fma(q, a, b) is monotone in q with the sign of a = 2^e / d), so each
+// axis's near and far byte words are chosen once per node instead of a
+// min and a max per child and axis; the distances are the same values.
 MCPT_DEV uint32_t wide_boxes(const float4& o, const uint4& q0, const uint4& q1, const Trav& s, float tn[4]) {
     const uint32_t meta = __float_as_uint(o.w);
     const int cnt = (int)(meta >> 24);
@@ -158,18 +163,22 @@ MCPT_DEV uint32_t wide_boxes(const float4& o, const uint4& q0, const uint4& q1, 
     const float bx = __builtin_fmaf(o.x, s.inv.x, -s.oi.x);
     const float by = __builtin_fmaf(o.y, s.inv.y, -s.oi.y);
     const float bz = __builtin_fmaf(o.z, s.inv.z, -s.oi.z);
+    const bool rx = s.inv.x < 0.0f, ry = s.inv.y < 0.0f, rz = s.inv.z < 0.0f;
+    const uint32_t nx = rx ? q0.w : q0.x, fx = rx ? q0.x : q0.w;
+    const uint32_t ny = ry ? q1.x : q0.y, fy = ry ? q0.y : q1.x;
+    const uint32_t nz = rz ? q1.y : q0.z, fz = rz ? q0.z : q1.y;
     uint32_t hit = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const int sh = 8 * k;
-        const float lx = __builtin_fmaf((float)((q0.x >> sh) & 255u), ax, bx);
-        const float hx = __builtin_fmaf((float)((q0.w >> sh) & 255u), ax, bx);
-        const float ly = __builtin_fmaf((float)((q0.y >> sh) & 255u), ay, by);
-        const float hy = __builtin_fmaf((float)((q1.x >> sh) & 255u), ay, by);
-        const float lz = __builtin_fmaf((float)((q0.z >> sh) & 255u), az, bz);
-        const float hz = __builtin_fmaf((float)((q1.y >> sh) & 255u), az, bz);
-        const float tnear = fmaxf(fmaxf(0.0001f, fminf(lx, hx)), fmaxf(fminf(ly, hy), fminf(lz, hz)));
-        const float tfar = fminf(fminf(s.t, fmaxf(lx, hx)), fminf(fmaxf(ly, hy), fmaxf(lz, hz)));
+        const float tnx = __builtin_fmaf((float)((nx >> sh) & 255u), ax, bx);
+        const float tfx = __builtin_fmaf((float)((fx >> sh) & 255u), ax, bx);
+        const float tny = __builtin_fmaf((float)((ny >> sh) & 255u), ay, by);
+        const float tfy = __builtin_fmaf((float)((fy >> sh) & 255u), ay, by);
+        const float tnz = __builtin_fmaf((float)((nz >> sh) & 255u), az, bz);
+        const float tfz = __builtin_fmaf((float)((fz >> sh) & 255u), az, bz);
+        const float tnear = fmaxf(fmaxf(0.0001f, tnx), fmaxf(tny, tnz));
+        const float tfar = fminf(fminf(s.t, tfx), fminf(tfy, tfz));
         tn[k] = tnear;
         if (k < cnt && !(tfar <= tnear)) hit |= 1u << k;
     }
