@@ -161,7 +161,10 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneView sv, Workspace ws, i
 
 // extendKernelPersistent (:3416-3468): lanes keep their traversal across
 // iterations; a wave with at least `refill` idle lanes takes that many queue
-// slots with one atomic and hands them out by ballot rank.
+// slots with one atomic and hands them out by ballot rank. In the fast order
+// leaf tests are deferred and run for all lanes that have some pending once
+// kLeafBatch lanes do (or no lane has a node left to visit).
+constexpr int kLeafBatch = 16;
 template <bool COUNT, bool REF>
 __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Workspace ws, int cur, int refill) {
     extern __shared__ int lds_stack[];
@@ -170,7 +173,8 @@ __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Work
     if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
     const int lane = lane_id();
     Trav s;
-    bool has = false;
+    s.pend = 0;
+    bool has = false, walked = false;  // walked: no node left, only pending leaves
     int slot = -1;
     bool exhausted = false;  // uniform across the wave
     uint32_t ns = 0, nl = 0;
@@ -196,11 +200,25 @@ __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Work
             }
         }
         if (exhausted && ballot64(has) == 0) break;
-        if (has) {
-            if (REF ? step_closest<COUNT>(sv.bvh, s, stk, kBlock, ns, nl) : step_wide<COUNT>(sv.bvh, s, stk, kBlock, ns, nl)) {
+        if (REF) {
+            if (has && step_closest<COUNT>(sv.bvh, s, stk, kBlock, ns, nl)) {
                 ws.hits[slot] = pack_hit(sv.bvh, s);
                 has = false;
             }
+            continue;
+        }
+        // node steps for lanes with nothing pending; leaf tests once enough
+        // lanes have leaves pending, or when no lane has a node to visit
+        if (has && !s.pend && !walked) walked = step_wide_defer<COUNT>(sv.bvh, s, stk, kBlock, ns);
+        const bool want = has && s.pend != 0;
+        const uint64_t wm = ballot64(want);
+        if (wm && (__popcll(wm) >= kLeafBatch || ballot64(has && !s.pend && !walked) == 0)) {
+            if (want) test_pending<COUNT>(sv.bvh, s, nl);
+        }
+        if (has && walked && !s.pend) {
+            ws.hits[slot] = pack_hit(sv.bvh, s);
+            has = false;
+            walked = false;
         }
     }
     count_flush<COUNT>(ws.counters, ns, nl);

@@ -32,6 +32,8 @@ struct Trav {
     int leaf;  // closest leaf so far, -1 = none
     int node;
     int sp;
+    uint32_t pend;  // deferred walk: slots of pend_ch still to test as leaves
+    int4 pend_ch;   // the child references of the node they belong to
 };
 
 // EXACT keeps the IEEE reciprocals that the reference's slab test uses
@@ -54,6 +56,7 @@ MCPT_DEV void trav_begin(Trav& s, const vec3& o, const vec3& d, float tmax) {
     s.leaf = -1;
     s.node = 0;
     s.sp = 0;
+    s.pend = 0;
 }
 
 // original-order hot triangle (9 floats), used when shading
@@ -276,6 +279,60 @@ MCPT_DEV bool step_wide(const BvhView& bv, Trav& s, int* stk, int stride, uint32
     if (n > 1) push_group(stk, stride, s, s.node, (uint32_t)k[1] | (uint32_t)k[2] << 2 | (uint32_t)k[3] << 4 | (n - 1) << 8);
     s.node = k[0] == 0 ? c[0] : k[0] == 1 ? c[1] : k[0] == 2 ? c[2] : c[3];
     return false;
+}
+
+// The same step with the hit leaf children deferred: they are recorded in
+// s.pend / s.pend_ch for test_pending, which the caller runs for many lanes
+// at once (a wave otherwise runs the triangle test whenever any one lane
+// reaches a leaf). The walk continues meanwhile with the running distance it
+// has; the closest hit does not depend on the order of the tests. Returns
+// true when no node is left to visit (pending leaves may remain).
+template <bool COUNT>
+MCPT_DEV bool step_wide_defer(const BvhView& bv, Trav& s, int* stk, int stride, uint32_t& ns) {
+    const WideNode* wp = bv.wide + s.node;
+    const float4 o = wp->o;
+    const uint4 q0 = wp->q0, q1 = wp->q1;
+    const int4 ch = wp->child;
+    if (COUNT) ns++;
+    float tn[4];
+    const uint32_t hit = wide_boxes(o, q0, q1, s, tn);
+    const int c[4] = {ch.x, ch.y, ch.z, ch.w};
+    uint32_t leaves = 0, inner = 0;
+    float d[4];
+    int k[4] = {0, 1, 2, 3};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const bool h = (hit >> j) & 1u;
+        if (h) {
+            if (c[j] < 0) leaves |= 1u << j;
+            else inner |= 1u << j;
+        }
+        d[j] = (h && c[j] >= 0) ? tn[j] : __builtin_inff();
+    }
+    s.pend = leaves;
+    s.pend_ch = ch;
+    if (!inner) return !pop_group(bv, stk, stride, s);
+    cswap(d[0], k[0], d[1], k[1]);
+    cswap(d[2], k[2], d[3], k[3]);
+    cswap(d[0], k[0], d[2], k[2]);
+    cswap(d[1], k[1], d[3], k[3]);
+    cswap(d[1], k[1], d[2], k[2]);
+    const uint32_t n = __builtin_popcount(inner);
+    if (n > 1) push_group(stk, stride, s, s.node, (uint32_t)k[1] | (uint32_t)k[2] << 2 | (uint32_t)k[3] << 4 | (n - 1) << 8);
+    s.node = k[0] == 0 ? c[0] : k[0] == 1 ? c[1] : k[0] == 2 ? c[2] : c[3];
+    return false;
+}
+
+// the leaves step_wide_defer left pending
+template <bool COUNT>
+MCPT_DEV void test_pending(const BvhView& bv, Trav& s, uint32_t& nl) {
+    const int c[4] = {s.pend_ch.x, s.pend_ch.y, s.pend_ch.z, s.pend_ch.w};
+    while (s.pend) {
+        const int j = __builtin_ctz(s.pend);
+        s.pend &= s.pend - 1u;
+        if (COUNT) nl++;
+        test_leaf_ranked(bv, s, ~(j == 0 ? c[0] : j == 1 ? c[1] : j == 2 ? c[2] : c[3]));
+    }
 }
 
 // Any-hit step on a wide node, children in left-to-right order; returns 0
