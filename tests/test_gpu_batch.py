@@ -55,9 +55,12 @@ def test_batched_build_matches_single_builds(mcpt, ctx):
 def test_batched_render_matches_single_renders(mcpt, ctx):
     sizes = [3000, 12000, 500, 40000]
     data = [_scene_data(mcpt, n, 60 + i) for i, n in enumerate(sizes)]
+    # and a scene with a homogeneous medium (its shadow rays take the
+    # transmittance walk inside the persistent shadow kernel)
+    data.append(mcpt.gen_scene(mcpt.GEN_MEDIUM, seed=65, param=20000))
     scenes = [ctx.scene(*d) for d in data]
     mcpt.build_batch(scenes)
-    dims = [(40, 30), (64, 64), (17, 9), (48, 48)]
+    dims = [(40, 30), (64, 64), (17, 9), (48, 48), (32, 24)]
     cams = [mcpt.camera_init(mcpt.gen_camera(mcpt.GEN_SOUP, w, h)) for w, h in dims]
     spp, depth, seed = 5, 8, 11
     refs = []
