@@ -483,6 +483,36 @@ typedef struct mcpt_gen_sizes {
  * param = soup/sphere triangle count (0 = the workload default). */
 int mcpt_gen_scene(int32_t kind, uint64_t seed, int64_t param, mcpt_tri_hot* hot, mcpt_tri_cold* cold,
                    mcpt_material* materials, mcpt_gen_sizes* sizes);
+/* ------------------------------------------------------------ OBJ import
+ * Wavefront OBJ + MTL into triangle records (loadObject / loadMtlFile,
+ * src/path_tracer.cu:2527-2859): fan triangulation, 1-based and negative
+ * indices, the model normalised (bottom centre at the origin, largest side
+ * 1) then scaled, rotated about +y and translated; area-weighted vertex
+ * normals when the file has none; degenerate and out-of-range faces
+ * skipped. Materials: index 0 is `fallback` (faces without a known
+ * material), 1 + i the i-th MTL material (Ke > 0: emissive; otherwise
+ * Lambertian with Kd converted from sRGB). Image textures are not loaded
+ * (map_Kd materials keep their Kd colour). Two-call pattern: NULL arrays
+ * to get the counts in `stats`, then arrays of that size. */
+typedef struct mcpt_obj_opts {
+    float translation[3];    /* where the model's bottom centre ends up  */
+    float uniform_scale;     /* its largest side after normalisation (> 0) */
+    float rotation_y;        /* radians about +y                          */
+    mcpt_material fallback;
+} mcpt_obj_opts;
+
+typedef struct mcpt_obj_stats {
+    int32_t n_tris;
+    int32_t n_materials;       /* fallback + MTL materials               */
+    int32_t skipped_faces;     /* out-of-range indices or zero area      */
+    int32_t texture_requests;  /* map_Kd materials (rendered with Kd)    */
+    int32_t missing_libraries; /* mtllib files that could not be opened  */
+    int32_t reserved;
+} mcpt_obj_stats;
+
+int mcpt_load_obj(const char* path, const mcpt_obj_opts* opts, mcpt_tri_hot* hot, mcpt_tri_cold* cold,
+                  mcpt_material* materials, mcpt_obj_stats* stats);
+
 /* Procedural equirect sky: gradient + sun disk + glow (closed form of
  * scripts/make_sky.py:5-27), width*height*4 floats, top row first. */
 int mcpt_gen_sky(int32_t width, int32_t height, float* rgba);

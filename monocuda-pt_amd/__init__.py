@@ -105,6 +105,19 @@ class RenderStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ObjOpts(C.Structure):
+    _fields_ = [("translation", C.c_float * 3), ("uniform_scale", C.c_float), ("rotation_y", C.c_float),
+                ("fallback", C.c_uint8 * MAT_DTYPE.itemsize)]
+
+
+class ObjStats(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("n_tris", "n_materials", "skipped_faces", "texture_requests",
+                                          "missing_libraries", "reserved")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 class GenSizes(C.Structure):
     _fields_ = [("n_tris", C.c_int32), ("n_materials", C.c_int32)]
 
@@ -208,6 +221,7 @@ _SIGS = {
     "mcpt_render_cached": ([VP, C.POINTER(Camera), C.POINTER(RenderOpts), VP, VP, C.POINTER(CacheOpts),
                             C.POINTER(CacheStats)], C.c_int),
     "mcpt_gen_scene": ([C.c_int32, C.c_uint64, C.c_int64, VP, VP, VP, C.POINTER(GenSizes)], C.c_int),
+    "mcpt_load_obj": ([C.c_char_p, C.POINTER(ObjOpts), VP, VP, VP, C.POINTER(ObjStats)], C.c_int),
     "mcpt_gen_sky": ([C.c_int32, C.c_int32, VP], C.c_int),
     "mcpt_gen_camera": ([C.c_int32, C.c_int32, C.c_int32, C.POINTER(CameraDesc)], C.c_int),
 }
@@ -285,6 +299,28 @@ def gen_scene(kind, seed=1, param=0):
     mats = np.zeros(sz.n_materials, MAT_DTYPE)
     _check(_lib.mcpt_gen_scene(kind, seed, param, _ptr(hot), _ptr(cold), _ptr(mats), None), "mcpt_gen_scene")
     return hot, cold, mats
+
+
+def load_obj(path, translation=(0.0, 0.0, 0.0), scale=1.0, rotation_y=0.0, fallback=None):
+    """mcpt_load_obj: an OBJ (+ MTL) file -> (hot, cold, materials, stats).
+    Material 0 is `fallback` (a MAT_DTYPE record; default Lambertian grey
+    0.73 linear), 1 + i the file's i-th MTL material."""
+    if fallback is None:
+        fallback = np.zeros(1, MAT_DTYPE)[0]
+        fallback["albedo"] = 0.73
+    o = ObjOpts()
+    o.translation[:] = [float(v) for v in translation]
+    o.uniform_scale = scale
+    o.rotation_y = rotation_y
+    C.memmove(o.fallback, np.asarray(fallback, MAT_DTYPE).tobytes(), MAT_DTYPE.itemsize)
+    st = ObjStats()
+    p = os.fsencode(path)
+    _check(_lib.mcpt_load_obj(p, C.byref(o), None, None, None, C.byref(st)), "mcpt_load_obj")
+    hot = np.zeros(st.n_tris, HOT_DTYPE)
+    cold = np.zeros(st.n_tris, COLD_DTYPE)
+    mats = np.zeros(st.n_materials, MAT_DTYPE)
+    _check(_lib.mcpt_load_obj(p, C.byref(o), _ptr(hot), _ptr(cold), _ptr(mats), C.byref(st)), "mcpt_load_obj")
+    return hot, cold, mats, st.as_dict()
 
 
 def gen_sky(width=512, height=256):

@@ -407,6 +407,11 @@ int mcpt_gen_scene(int32_t kind, uint64_t seed, int64_t param, mcpt_tri_hot* hot
 
 int mcpt_gen_sky(int32_t width, int32_t height, float* rgba) { return gen_sky(width, height, rgba); }
 
+int mcpt_load_obj(const char* path, const mcpt_obj_opts* opts, mcpt_tri_hot* hot, mcpt_tri_cold* cold,
+                  mcpt_material* materials, mcpt_obj_stats* stats) {
+    return load_obj(path, opts, hot, cold, materials, stats);
+}
+
 int mcpt_gen_camera(int32_t kind, int32_t width, int32_t height, mcpt_camera_desc* out) {
     return gen_camera(kind, width, height, out);
 }

@@ -71,7 +71,12 @@ struct DevBuf {
     size_t bytes() const { return count * sizeof(T); }
 };
 
-// scene generators (scenegen.cpp)
+// OBJ + MTL import (objload.cpp)
+int load_obj(const char* path, const mcpt_obj_opts* opts, mcpt_tri_hot* hot, mcpt_tri_cold* cold,
+             mcpt_material* materials, mcpt_obj_stats* stats);
+// scene generators (scenegen.cpp); srgb_to_linear is srgbToLinear
+// (src/path_tracer.cu:200-202), the albedo conversion of make_solid_texture
+float srgb_to_linear(float e);
 int gen_scene(int kind, uint64_t seed, int64_t param, mcpt_tri_hot* hot, mcpt_tri_cold* cold,
               mcpt_material* materials, mcpt_gen_sizes* sizes);
 int gen_sky(int w, int h, float* rgba);

@@ -22,9 +22,6 @@ struct HostScene {  // hostScene (src/path_tracer.cu:1667-1675), triangles only
     std::vector<mcpt_material> mats;
 };
 
-float srgb_to_linear(float e) {  // src/path_tracer.cu:200-202
-    return e <= 0.04045f ? e / 12.92f : mf_pow((e + 0.055f) / 1.055f, 2.4f);
-}
 
 int add_material(HostScene& s, int type, const float albedo_srgb[3], const float emission[3], float fuzz, float ior,
                  float density, float g) {
@@ -273,6 +270,10 @@ void build_medium(HostScene& s, uint64_t seed, int64_t n) {
 }
 
 }  // namespace
+
+float srgb_to_linear(float e) {  // src/path_tracer.cu:200-202
+    return e <= 0.04045f ? e / 12.92f : mf_pow((e + 0.055f) / 1.055f, 2.4f);
+}
 
 int gen_scene(int kind, uint64_t seed, int64_t param, mcpt_tri_hot* hot, mcpt_tri_cold* cold,
               mcpt_material* materials, mcpt_gen_sizes* sizes) {

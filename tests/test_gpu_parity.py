@@ -184,6 +184,81 @@ def test_wide_tree_cornell_and_duplicates(mcpt, ctx, oracle):
         scene.close()
 
 
+ROOM_MTL = """newmtl white
+Kd 0.73 0.73 0.73
+newmtl red
+Kd 0.65 0.05 0.05
+newmtl lamp
+Ke 15 15 15
+"""
+
+ROOM_OBJ = """mtllib room.mtl
+v -1 0 -1
+v 1 0 -1
+v 1 0 1
+v -1 0 1
+v -1 2 -1
+v 1 2 -1
+v 1 2 1
+v -1 2 1
+v -0.3 1.98 -0.3
+v 0.3 1.98 -0.3
+v 0.3 1.98 0.3
+v -0.3 1.98 0.3
+v -0.5 0 -0.2
+v 0.1 0 -0.4
+v 0.3 0 0.2
+v -0.3 0 0.4
+v -0.5 0.8 -0.2
+v 0.1 0.8 -0.4
+v 0.3 0.8 0.2
+v -0.3 0.8 0.4
+usemtl white
+f 1 2 3 4
+f 5 8 7 6
+f 1 5 6 2
+usemtl red
+f 1 4 8 5
+usemtl white
+f 2 6 7 3
+usemtl lamp
+f 9 10 11 12
+usemtl white
+f 13 14 15 16
+f 17 20 19 18
+f 13 17 18 14
+f 14 18 19 15
+f 15 19 20 16
+f 16 20 17 13
+"""
+
+
+def test_render_obj_scene(mcpt, ctx, oracle, tmp_path):
+    """An OBJ + MTL room loaded by mcpt_load_obj renders like the checker's
+    render of the same records, in every mode."""
+    (tmp_path / "room.mtl").write_text(ROOM_MTL)
+    (tmp_path / "room.obj").write_text(ROOM_OBJ)
+    hot, cold, mats, st = mcpt.load_obj(str(tmp_path / "room.obj"), translation=(0, 0, 0), scale=2.0)
+    assert st["n_tris"] == 24 and st["n_materials"] == 4
+    W = H = 64
+    spp, depth, seed = 4, 8, 3
+    desc = mcpt.gen_camera(mcpt.GEN_CORNELL, W, H)
+    desc.look_from[:] = (0.0, 1.0, 3.5)
+    desc.look_at[:] = (0.0, 1.0, 0.0)
+    cam = mcpt.camera_init(desc)
+    scene = ctx.scene(hot, cold, mats)
+    scene.build()
+    ref = oracle.Scene(hot, cold, mats)
+    ref_img, _, _ = ref.render(oracle.camera_init(desc), spp, depth, seed=seed)
+    assert ref_img.mean() > 0.01  # the lamp lights the room
+    out = _render_modes(mcpt, ctx, scene, cam, spp, depth, seed)
+    base = out[mcpt.MODE_MEGAKERNEL][0]
+    for mode, (img, _) in out.items():
+        assert np.array_equal(img, base), mode
+        assert rel_mse(img, ref_img) < 1e-4, mode
+    scene.close()
+
+
 def _rays(n, seed, origin_box=(0.0, 1.0), tmax=1e30):
     rng = np.random.default_rng(seed)
     r = np.zeros(n, mcpt_ray_dtype())
