@@ -284,6 +284,22 @@ int mcpt_render(mcpt_scene scene, const mcpt_camera* camera, const mcpt_render_o
  * tone map, sRGB OETF. out_rgb is host memory, W*H*3. */
 int mcpt_resolve(mcpt_film film, int32_t samples_so_far, int32_t tonemap, float exposure, float* out_rgb);
 
+/* resolve_kernel followed by the output conversion of main
+ * (src/path_tracer.cu:4366-4380) on the device: NaN shield, clamp to
+ * [0, 1], (unsigned char)(255.99 x). out_rgb8 is host memory, W*H*3 bytes,
+ * top row first. */
+int mcpt_resolve_rgb8(mcpt_film film, int32_t samples_so_far, int32_t tonemap, float exposure, uint8_t* out_rgb8);
+
+/* computeAutoExposure (src/path_tracer.cu:3101-3124): target_gray (0.18 in
+ * the reference) / exp(mean over pixels of ln(1e-4 + luminance)), where a
+ * non-finite luminance counts as 0. The sum is FP64 in a fixed order (the
+ * reference adds FP32 warp sums atomically, in no fixed order). */
+int mcpt_auto_exposure(mcpt_film film, int32_t samples_so_far, float target_gray, float* exposure);
+
+/* PNG file of W*H RGB8 pixels, top row first (saveImageRgb8Png,
+ * src/image_io.cpp:62-65); rows stored uncompressed in the zlib stream. */
+int mcpt_write_png(const char* path, int32_t width, int32_t height, const uint8_t* rgb8);
+
 /* ------------------------------------------------------------ render server
  * Continuous batching. The reference renders one image per process run;
  * these entry points have no counterpart in it. A server owns a pool of

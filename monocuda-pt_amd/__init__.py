@@ -199,6 +199,9 @@ _SIGS = {
     "mcpt_film_write": ([VP, VP], C.c_int),
     "mcpt_render": ([VP, C.POINTER(Camera), C.POINTER(RenderOpts), VP, C.POINTER(RenderStats)], C.c_int),
     "mcpt_resolve": ([VP, C.c_int32, C.c_int32, C.c_float, VP], C.c_int),
+    "mcpt_resolve_rgb8": ([VP, C.c_int32, C.c_int32, C.c_float, VP], C.c_int),
+    "mcpt_auto_exposure": ([VP, C.c_int32, C.c_float, C.POINTER(C.c_float)], C.c_int),
+    "mcpt_write_png": ([C.c_char_p, C.c_int32, C.c_int32, VP], C.c_int),
     "mcpt_server_create": ([VP, VP, C.c_int32, C.POINTER(ServerOpts), C.POINTER(VP)], C.c_int),
     "mcpt_server_submit": ([VP, C.POINTER(Job), C.POINTER(C.c_uint64)], C.c_int),
     "mcpt_server_poll": ([VP, C.c_uint64, C.POINTER(JobResult)], C.c_int),
@@ -321,6 +324,13 @@ def load_obj(path, translation=(0.0, 0.0, 0.0), scale=1.0, rotation_y=0.0, fallb
     mats = np.zeros(st.n_materials, MAT_DTYPE)
     _check(_lib.mcpt_load_obj(p, C.byref(o), _ptr(hot), _ptr(cold), _ptr(mats), C.byref(st)), "mcpt_load_obj")
     return hot, cold, mats, st.as_dict()
+
+
+def write_png(path, rgb8):
+    """mcpt_write_png: an (H, W, 3) uint8 image to a PNG file."""
+    rgb8 = np.ascontiguousarray(rgb8, np.uint8)
+    h, w = rgb8.shape[:2]
+    _check(_lib.mcpt_write_png(os.fsencode(path), w, h, _ptr(rgb8)), "mcpt_write_png")
 
 
 def gen_sky(width=512, height=256):
@@ -541,6 +551,18 @@ class Film:
         out = np.zeros((self.hgt, self.w, 3), np.float32)
         _check(_lib.mcpt_resolve(self.h, spp, tonemap, exposure, _ptr(out)), "mcpt_resolve")
         return out
+
+    def resolve_rgb8(self, spp, tonemap=TONEMAP_NONE, exposure=1.0):
+        """mcpt_resolve_rgb8: the resolved image as 8-bit RGB (H, W, 3)."""
+        out = np.zeros((self.hgt, self.w, 3), np.uint8)
+        _check(_lib.mcpt_resolve_rgb8(self.h, spp, tonemap, exposure, _ptr(out)), "mcpt_resolve_rgb8")
+        return out
+
+    def auto_exposure(self, spp, target_gray=0.18):
+        """mcpt_auto_exposure: target_gray / log-average luminance."""
+        e = C.c_float()
+        _check(_lib.mcpt_auto_exposure(self.h, spp, target_gray, C.byref(e)), "mcpt_auto_exposure")
+        return e.value
 
 
 class Server:

@@ -400,6 +400,23 @@ int mcpt_resolve(mcpt_film film, int32_t samples_so_far, int32_t tonemap, float 
     return resolve(film, samples_so_far, tonemap, exposure, out_rgb);
 }
 
+int mcpt_resolve_rgb8(mcpt_film film, int32_t samples_so_far, int32_t tonemap, float exposure, uint8_t* out_rgb8) {
+    CHECK_ARG(film && out_rgb8 && samples_so_far > 0, "bad arguments");
+    CHECK_ARG(tonemap >= MCPT_TONEMAP_NONE && tonemap <= MCPT_TONEMAP_ACES, "unknown tone map");
+    MCPT_HIP(hipSetDevice(film->ctx->device));
+    return resolve_rgb8(film, samples_so_far, tonemap, exposure, out_rgb8);
+}
+
+int mcpt_auto_exposure(mcpt_film film, int32_t samples_so_far, float target_gray, float* exposure) {
+    CHECK_ARG(film && exposure && samples_so_far > 0, "bad arguments");
+    MCPT_HIP(hipSetDevice(film->ctx->device));
+    return auto_exposure(film, samples_so_far, target_gray, exposure);
+}
+
+int mcpt_write_png(const char* path, int32_t width, int32_t height, const uint8_t* rgb8) {
+    return write_png(path, width, height, rgb8);
+}
+
 int mcpt_gen_scene(int32_t kind, uint64_t seed, int64_t param, mcpt_tri_hot* hot, mcpt_tri_cold* cold,
                    mcpt_material* materials, mcpt_gen_sizes* sizes) {
     return gen_scene(kind, seed, param, hot, cold, materials, sizes);

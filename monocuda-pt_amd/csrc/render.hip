@@ -17,6 +17,7 @@
 //                idle (extendKernelPersistent, :3416-3468, on 64-lane waves).
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "scene.hpp"
 #include "shade.hpp"
@@ -450,6 +451,41 @@ __global__ __launch_bounds__(kBlock) void k_resolve(const float* accum, float* o
     out[i] = linear_to_srgb(tonemap_channel(a, tm));
 }
 
+// resolve_kernel plus the output conversion of main (:4366-4380): the NaN
+// shield, clamp to [0, 1] and (unsigned char)(255.99 x), three bytes per
+// pixel written on the device
+__global__ __launch_bounds__(kBlock) void k_resolve_rgb8(const float* accum, uint8_t* out, int n, int spp, int tm,
+                                                         float exposure) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= 3 * n) return;
+    const float a = accum[i] / float(spp) * exposure;
+    const float c = linear_to_srgb(tonemap_channel(a, tm));
+    const float v = __builtin_isfinite(c) ? fminf(fmaxf(c, 0.0f), 1.0f) : 0.0f;
+    out[i] = (uint8_t)(255.99f * v);
+}
+
+// logLuminanceSumKernel (:3101-3111): ln(1e-4 + luminance) per pixel (a
+// non-finite luminance counts as 0), summed per block in FP64 in a fixed
+// order; the host adds the block sums in block order (the reference adds
+// warp sums with float atomics, in no fixed order)
+__global__ __launch_bounds__(kBlock) void k_log_luminance(const float* accum, int n, int spp, double* partial) {
+    __shared__ double sh[kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        const float r = accum[3 * i] / float(spp), g = accum[3 * i + 1] / float(spp), b = accum[3 * i + 2] / float(spp);
+        const float lum = 0.2126f * r + 0.7152f * g + 0.0722f * b;
+        v = (double)mf_log(1e-4f + (__builtin_isfinite(lum) ? fmaxf(lum, 0.0f) : 0.0f));
+    }
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
 // ------------------------------------------------------------ trace hooks
 template <bool COUNT, bool REF>
 __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const mcpt_ray* rays, int n,
@@ -688,6 +724,40 @@ int resolve(mcpt_film_s* film, int spp, int tonemap, float exposure, float* out)
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     o.release(ctx);
     if (e != hipSuccess) return set_error(MCPT_ERR_HIP, "resolve: %s", hipGetErrorString(e));
+    return MCPT_OK;
+}
+
+int resolve_rgb8(mcpt_film_s* film, int spp, int tonemap, float exposure, uint8_t* out) {
+    Ctx* ctx = film->ctx;
+    const int n = film->w * film->h;
+    DevBuf<uint8_t> o;
+    MCPT_TRY(o.alloc(ctx, (size_t)n * 3));
+    k_resolve_rgb8<<<blocks_for(3 * n), kBlock, 0, ctx->stream>>>(film->accum.ptr, o.ptr, n, spp, tonemap, exposure);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, o.ptr, (size_t)n * 3, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    o.release(ctx);
+    if (e != hipSuccess) return set_error(MCPT_ERR_HIP, "resolve_rgb8: %s", hipGetErrorString(e));
+    return MCPT_OK;
+}
+
+// computeAutoExposure (:3113-3124): target / exp(mean log luminance)
+int auto_exposure(mcpt_film_s* film, int spp, float target, float* exposure) {
+    Ctx* ctx = film->ctx;
+    const int n = film->w * film->h;
+    const int nb = blocks_for(n);
+    DevBuf<double> part;
+    MCPT_TRY(part.alloc(ctx, (size_t)nb));
+    std::vector<double> h(nb);
+    k_log_luminance<<<nb, kBlock, 0, ctx->stream>>>(film->accum.ptr, n, spp, part.ptr);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part.ptr, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    part.release(ctx);
+    if (e != hipSuccess) return set_error(MCPT_ERR_HIP, "auto_exposure: %s", hipGetErrorString(e));
+    double sum = 0.0;
+    for (double v : h) sum += v;
+    *exposure = target / mf_exp((float)(sum / (double)n));
     return MCPT_OK;
 }
 

@@ -113,5 +113,8 @@ int trace_any_host(mcpt_scene_s* s, const mcpt_ray* rays, int n, uint8_t* occlud
 int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, mcpt_film_s* film,
            mcpt_render_stats* stats);
 int resolve(mcpt_film_s* film, int spp, int tonemap, float exposure, float* out);
+int resolve_rgb8(mcpt_film_s* film, int spp, int tonemap, float exposure, uint8_t* out);
+int auto_exposure(mcpt_film_s* film, int spp, float target, float* exposure);
+int write_png(const char* path, int w, int h, const uint8_t* rgb8);
 
 }  // namespace mcpt

@@ -1187,6 +1187,30 @@ void oracle_resolve(const float* accum, int w, int h, int spp, int tonemap, floa
     }
 }
 
+// main's output conversion after resolve_kernel (src/path_tracer.cu:4366-4380)
+void oracle_resolve_rgb8(const float* accum, int w, int h, int spp, int tonemap, float exposure, uint8_t* out) {
+    std::vector<float> f((size_t)w * h * 3);
+    oracle_resolve(accum, w, h, spp, tonemap, exposure, f.data());
+    for (size_t i = 0; i < f.size(); i++) {
+        const float c = f[i];
+        const float v = std::isfinite(c) ? fminf(fmaxf(c, 0.0f), 1.0f) : 0.0f;
+        out[i] = (uint8_t)(255.99f * v);
+    }
+}
+
+// logLuminanceSumKernel + computeAutoExposure (src/path_tracer.cu:3101-3124),
+// the per-pixel terms in FP32, their sum in FP64 in pixel order
+float oracle_auto_exposure(const float* accum, int w, int h, int spp, float target) {
+    double sum = 0.0;
+    const int n = w * h;
+    for (int p = 0; p < n; p++) {
+        const float r = accum[3 * p] / float(spp), g = accum[3 * p + 1] / float(spp), b = accum[3 * p + 2] / float(spp);
+        const float lum = 0.2126f * r + 0.7152f * g + 0.0722f * b;
+        sum += (double)mf_log(1e-4f + (std::isfinite(lum) ? fmaxf(lum, 0.0f) : 0.0f));
+    }
+    return target / mf_exp((float)(sum / (double)n));
+}
+
 void oracle_fmath(int fn, const float* x, const float* y, float* out, int n) {
     for (int i = 0; i < n; i++) {
         switch (fn) {

@@ -94,6 +94,10 @@ int oracle_env_probe(const float* rgba, int w, int h, float rot, int fn, const f
 
 /* resolve_kernel (src/path_tracer.cu:3062-3100) */
 void oracle_resolve(const float* accum, int w, int h, int spp, int tonemap, float exposure, float* out);
+/* resolve, then main's NaN shield and 8-bit conversion (:4366-4380) */
+void oracle_resolve_rgb8(const float* accum, int w, int h, int spp, int tonemap, float exposure, uint8_t* out);
+/* computeAutoExposure (:3101-3124), FP64 sum in pixel order */
+float oracle_auto_exposure(const float* accum, int w, int h, int spp, float target);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,8 @@ def lib():
             "oracle_render": ([VP, VP, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int, VP, VP, C.c_int,
                                VP], C.c_int),
             "oracle_resolve": ([VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, VP], None),
+            "oracle_resolve_rgb8": ([VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, VP], None),
+            "oracle_auto_exposure": ([VP, C.c_int, C.c_int, C.c_int, C.c_float], C.c_float),
             "oracle_fmath": ([C.c_int, VP, VP, VP, C.c_int], None),
             "oracle_pcg32": ([C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, VP], None),
             "oracle_sampling": ([C.c_int, VP, C.c_int, VP], C.c_int),
@@ -285,6 +287,20 @@ def brute_closest(hot, rays):
     hits = np.zeros(len(rays), HIT_DTYPE)
     lib().oracle_brute_closest(_p(hot), len(hot), _p(rays), len(rays), _p(hits))
     return hits
+
+
+def resolve_rgb8(accum, spp, tonemap=0, exposure=1.0):
+    accum = np.ascontiguousarray(accum, np.float32)
+    h, w = accum.shape[:2]
+    out = np.zeros((h, w, 3), np.uint8)
+    lib().oracle_resolve_rgb8(_p(accum), w, h, spp, tonemap, exposure, _p(out))
+    return out
+
+
+def auto_exposure(accum, spp, target=0.18):
+    accum = np.ascontiguousarray(accum, np.float32)
+    h, w = accum.shape[:2]
+    return lib().oracle_auto_exposure(_p(accum), w, h, spp, target)
 
 
 def resolve(accum, spp, tonemap=0, exposure=1.0):
