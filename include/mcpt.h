@@ -74,7 +74,8 @@ typedef struct mcpt_material {
     float ior;
     float density;
     float phase_g;
-    int32_t pad;
+    int32_t texture; /* 0: the solid albedo above; k > 0: the scene's texture
+                        k - 1 (mcpt_scene_add_texture) gives the albedo     */
 } mcpt_material;
 
 /* Light (src/path_tracer.cu:942-947): an emissive triangle. 64 bytes. */
@@ -499,6 +500,34 @@ typedef struct mcpt_gen_sizes {
  * param = soup/sphere triangle count (0 = the workload default). */
 int mcpt_gen_scene(int32_t kind, uint64_t seed, int64_t param, mcpt_tri_hot* hot, mcpt_tri_cold* cold,
                    mcpt_material* materials, mcpt_gen_sizes* sizes);
+/* ------------------------------------------------------------- textures
+ * sampleTexture (src/path_tracer.cu:216-234): a checker of two sRGB colours
+ * (converted to linear like make_checker_texture, :211-213) or an RGBA8
+ * image (sRGB encoded, top row first) read bilinearly with wrap in u and v
+ * at (u, 1 - v), each texel decoded from sRGB before filtering. The
+ * reference filters in the texture unit; here the filter is FP32 in
+ * software, so image-texture results can differ from it in the last bits. */
+#define MCPT_TEX_CHECKER 1
+#define MCPT_TEX_IMAGE 2
+
+typedef struct mcpt_texture_desc {
+    int32_t type;         /* MCPT_TEX_CHECKER or MCPT_TEX_IMAGE            */
+    float color_a[3];     /* checker: sRGB colour of even cells            */
+    float color_b[3];     /* checker: sRGB colour of odd cells             */
+    float scale;          /* checker: cells per unit of u and of v         */
+    int32_t width, height;
+    const uint8_t* rgba8; /* image: width*height*4 bytes, top row first    */
+} mcpt_texture_desc;
+
+/* Adds a texture to a scene; *index receives k - 1 for the material field
+ * `texture = k`. A render refuses materials that name a texture not added. */
+int mcpt_scene_add_texture(mcpt_scene scene, const mcpt_texture_desc* desc, int32_t* index);
+
+/* loadPpmAsRgba8 (src/path_tracer.cu:236-294): a binary PPM (P6, maxval
+ * 255, '#' comments in the header) as RGBA8 with alpha 255. Two-call
+ * pattern: rgba8 NULL to get the size. */
+int mcpt_load_ppm(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8);
+
 /* ------------------------------------------------------------ OBJ import
  * Wavefront OBJ + MTL into triangle records (loadObject / loadMtlFile,
  * src/path_tracer.cu:2527-2859): fan triangulation, 1-based and negative

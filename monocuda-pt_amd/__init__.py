@@ -39,6 +39,7 @@ MODE_MEGAKERNEL, MODE_WAVEFRONT, MODE_PERSISTENT = 0, 1, 2
 TONEMAP_NONE, TONEMAP_REINHARD, TONEMAP_ACES = 0, 1, 2
 RENDER_COUNT_TRAVERSAL, RENDER_TIME_KERNELS, RENDER_REFERENCE_ORDER, RENDER_PERSISTENT_SHADOW = 1, 2, 4, 8
 RENDER_SHADOW_PER_RAY = 16
+TEX_CHECKER, TEX_IMAGE = 1, 2
 GEN_CORNELL, GEN_SOUP, GEN_ICOSPHERE, GEN_MEDIUM = 1, 2, 3, 5
 MLP_FEATURES, MLP_INPUTS, MLP_WIDTH, MLP_HIDDEN, MLP_OUTPUTS = 11, 64, 64, 4, 3
 MLP_PARAMS = 4 * 64 * 64 + 3 * 64
@@ -49,7 +50,7 @@ HOT_DTYPE = np.dtype([("v0", "<f4", 3), ("v1", "<f4", 3), ("v2", "<f4", 3)])
 COLD_DTYPE = np.dtype([("uv0", "<f4", 2), ("uv1", "<f4", 2), ("uv2", "<f4", 2), ("n0", "<f4", 3),
                        ("n1", "<f4", 3), ("n2", "<f4", 3), ("material", "<i4")])
 MAT_DTYPE = np.dtype([("type", "<i4"), ("albedo", "<f4", 3), ("emission", "<f4", 3), ("fuzz", "<f4"),
-                      ("ior", "<f4"), ("density", "<f4"), ("phase_g", "<f4"), ("pad", "<i4")])
+                      ("ior", "<f4"), ("density", "<f4"), ("phase_g", "<f4"), ("texture", "<i4")])
 LIGHT_DTYPE = np.dtype([("tri", HOT_DTYPE), ("emission", "<f4", 3), ("area", "<f4"), ("normal", "<f4", 3)])
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("time", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_type", "<i4"), ("prim_index", "<i4"), ("did_hit", "<i4")])
@@ -116,6 +117,26 @@ class ObjStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("color_a", C.c_float * 3), ("color_b", C.c_float * 3), ("scale", C.c_float),
+                ("width", C.c_int32), ("height", C.c_int32), ("rgba8", C.c_void_p)]
+
+
+def _texture_desc(kind, color_a=(0, 0, 0), color_b=(1, 1, 1), scale=1.0, image=None):
+    d = TextureDesc()
+    d.type = kind
+    d.color_a[:] = [float(x) for x in color_a]
+    d.color_b[:] = [float(x) for x in color_b]
+    d.scale = scale
+    keep = None
+    if image is not None:
+        keep = np.ascontiguousarray(image, np.uint8)
+        assert keep.ndim == 3 and keep.shape[2] == 4, "image must be (H, W, 4) RGBA8"
+        d.height, d.width = keep.shape[:2]
+        d.rgba8 = keep.ctypes.data
+    return d, keep
 
 
 class GenSizes(C.Structure):
@@ -225,6 +246,8 @@ _SIGS = {
                             C.POINTER(CacheStats)], C.c_int),
     "mcpt_gen_scene": ([C.c_int32, C.c_uint64, C.c_int64, VP, VP, VP, C.POINTER(GenSizes)], C.c_int),
     "mcpt_load_obj": ([C.c_char_p, C.POINTER(ObjOpts), VP, VP, VP, C.POINTER(ObjStats)], C.c_int),
+    "mcpt_scene_add_texture": ([VP, C.POINTER(TextureDesc), C.POINTER(C.c_int32)], C.c_int),
+    "mcpt_load_ppm": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_gen_sky": ([C.c_int32, C.c_int32, VP], C.c_int),
     "mcpt_gen_camera": ([C.c_int32, C.c_int32, C.c_int32, C.POINTER(CameraDesc)], C.c_int),
 }
@@ -324,6 +347,16 @@ def load_obj(path, translation=(0.0, 0.0, 0.0), scale=1.0, rotation_y=0.0, fallb
     mats = np.zeros(st.n_materials, MAT_DTYPE)
     _check(_lib.mcpt_load_obj(p, C.byref(o), _ptr(hot), _ptr(cold), _ptr(mats), C.byref(st)), "mcpt_load_obj")
     return hot, cold, mats, st.as_dict()
+
+
+def load_ppm(path):
+    """mcpt_load_ppm: a binary PPM as an (H, W, 4) RGBA8 array."""
+    w, h = C.c_int32(), C.c_int32()
+    p = os.fsencode(path)
+    _check(_lib.mcpt_load_ppm(p, C.byref(w), C.byref(h), None), "mcpt_load_ppm")
+    out = np.zeros((h.value, w.value, 4), np.uint8)
+    _check(_lib.mcpt_load_ppm(p, C.byref(w), C.byref(h), _ptr(out)), "mcpt_load_ppm")
+    return out
 
 
 def write_png(path, rgb8):
@@ -468,6 +501,13 @@ class Scene:
         tie = np.zeros(self.n, np.int32)
         _check(_lib.mcpt_lbvh_dump_wide(self.h, _ptr(nodes), n.value, C.byref(n), _ptr(tie)), "mcpt_lbvh_dump_wide")
         return nodes, tie
+
+    def add_texture(self, kind, color_a=(0, 0, 0), color_b=(1, 1, 1), scale=1.0, image=None):
+        """mcpt_scene_add_texture; returns the index k (materials use texture = k + 1)."""
+        d, _keep = _texture_desc(kind, color_a, color_b, scale, image)
+        k = C.c_int32()
+        _check(_lib.mcpt_scene_add_texture(self.h, C.byref(d), C.byref(k)), "mcpt_scene_add_texture")
+        return k.value
 
     def lights(self):
         n = C.c_int32()

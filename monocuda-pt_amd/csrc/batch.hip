@@ -880,6 +880,7 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
 int mcpt_render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int32_t n, const mcpt_render_opts* o,
                       mcpt_film* films, const mcpt_batch_tune* tune, mcpt_render_stats* stats) {
     MCPT_TRY(check_render_batch(scenes, cams, n, o));
+    for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_textures());
     if (!films) return set_error(MCPT_ERR_INVALID, "films is NULL");
     std::vector<float*> acc(n);
     for (int i = 0; i < n; i++) {
@@ -894,6 +895,7 @@ int mcpt_render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int32_t
 int mcpt_batch_autotune(const mcpt_scene* scenes, const mcpt_camera* cams, int32_t n, const mcpt_render_opts* o,
                         mcpt_batch_tune* best, double* search_ms) {
     MCPT_TRY(check_render_batch(scenes, cams, n, o));
+    for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_textures());
     if (!best) return set_error(MCPT_ERR_INVALID, "best is NULL");
     Ctx* ctx = scenes[0]->ctx;
     MCPT_HIP(hipSetDevice(ctx->device));

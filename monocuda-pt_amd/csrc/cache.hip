@@ -541,7 +541,7 @@ __global__ __launch_bounds__(kBlock) void k_cc_shade(SceneView sv, CacheWs ws, i
             diffuse = m.type == MCPT_MAT_LAMBERTIAN;
             if (diffuse) {
                 rebuild_hit(sv, hit.prim_index, ro, rd, hit.t, h);
-                albedo = load3(m.albedo);
+                albedo = sample_albedo(sv, m, h.u, h.v);
             }
         }
         const int ts = ws.tslot[path];
@@ -920,6 +920,7 @@ int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* cam, const mcpt_rend
                        mcpt_mlp mlp, const mcpt_cache_opts* co, mcpt_cache_stats* stats) {
     if (!scene || !cam || !o || !film || !mlp) return set_error(MCPT_ERR_INVALID, "bad arguments");
     if (!scene->built) return set_error(MCPT_ERR_INVALID, "scene has no BVH (call mcpt_lbvh_build)");
+    MCPT_TRY(scene->check_textures());
     if (film->w != cam->width || film->h != cam->height)
         return set_error(MCPT_ERR_INVALID, "film %dx%d does not match camera %dx%d", film->w, film->h, cam->width,
                          cam->height);

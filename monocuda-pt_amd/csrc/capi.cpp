@@ -1,6 +1,7 @@
 // capi.cpp — the extern "C" boundary declared in include/mcpt.h: contexts,
 // scenes, films and argument checking around the kernel launchers.
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -205,12 +206,18 @@ int mcpt_scene_create(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_cold
         if (cold[i].material < 0 || cold[i].material >= n_materials)
             return set_error(MCPT_ERR_INVALID, "triangle %d has material %d outside [0, %d)", i, cold[i].material,
                              n_materials);
+    int max_tex = 0;
+    for (int m = 0; m < n_materials; m++) {
+        if (materials[m].texture < 0) return set_error(MCPT_ERR_INVALID, "material %d: negative texture", m);
+        max_tex = std::max(max_tex, (int)materials[m].texture);
+    }
     Ctx* c = &ctx->c;
     MCPT_HIP(hipSetDevice(c->device));
     auto* s = new mcpt_scene_s();
     s->ctx = c;
     s->n = n_tris;
     s->n_mats = n_materials;
+    s->max_material_texture = max_tex;
     for (int m = 0; m < n_materials; m++)
         if (materials[m].type == MCPT_MAT_ISOTROPIC)
             for (int i = 0; i < n_tris; i++)
@@ -261,6 +268,113 @@ int mcpt_scene_create(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_cold
     return MCPT_OK;
 }
 
+int mcpt_scene_s::check_textures() const {
+    if (max_material_texture > (int)tex_host.size())
+        return set_error(MCPT_ERR_INVALID, "a material names texture %d but the scene has %d", max_material_texture,
+                         (int)tex_host.size());
+    return MCPT_OK;
+}
+
+int mcpt_scene_add_texture(mcpt_scene scene, const mcpt_texture_desc* d, int32_t* index) {
+    CHECK_ARG(scene && d, "bad arguments");
+    if (scene->servers.load() > 0) return set_error(MCPT_ERR_INVALID, "the scene is in use by a server");
+    Ctx* c = scene->ctx;
+    MCPT_HIP(hipSetDevice(c->device));
+    TexView v;
+    std::memset(&v, 0, sizeof(v));
+    v.type = d->type;
+    DevBuf<uint32_t> px;
+    if (d->type == MCPT_TEX_CHECKER) {
+        v.scale = d->scale;
+        for (int k = 0; k < 3; k++) {  // make_checker_texture converts (:211-213)
+            v.ca[k] = srgb_to_linear(d->color_a[k]);
+            v.cb[k] = srgb_to_linear(d->color_b[k]);
+        }
+    } else if (d->type == MCPT_TEX_IMAGE) {
+        CHECK_ARG(d->width >= 1 && d->height >= 1 && d->rgba8, "an image texture needs pixels");
+        v.w = d->width;
+        v.h = d->height;
+        MCPT_TRY(upload(c, px, reinterpret_cast<const uint32_t*>(d->rgba8), (size_t)d->width * d->height));
+        v.px = px.ptr;
+    } else {
+        return set_error(MCPT_ERR_INVALID, "unknown texture type %d", d->type);
+    }
+    if (!scene->srgb8.ptr) {  // the sRGB decode of a texel byte, once per scene
+        float lut[256];
+        for (int i = 0; i < 256; i++) lut[i] = srgb_to_linear((float)i / 255.0f);
+        int st = upload(c, scene->srgb8, lut, 256);
+        if (st < 0) {
+            px.release(c);
+            return st;
+        }
+    }
+    scene->tex_host.push_back(v);
+    int st = upload(c, scene->tex, scene->tex_host.data(), scene->tex_host.size());
+    if (st >= 0 && hipStreamSynchronize(c->stream) != hipSuccess) st = set_error(MCPT_ERR_HIP, "texture upload failed");
+    if (st < 0) {
+        scene->tex_host.pop_back();
+        px.release(c);
+        return st;
+    }
+    if (px.ptr) scene->tex_px.push_back(px);  // the scene owns the pixels now
+    if (index) *index = (int32_t)scene->tex_host.size() - 1;
+    return MCPT_OK;
+}
+
+// readNextPpmInteger (:236-256): skips whitespace and '#' comment lines,
+// reads decimal digits and consumes the one separator after them
+static bool ppm_int(std::FILE* f, int& value) {
+    if (std::feof(f) || std::ferror(f)) return false;
+    int ch = std::fgetc(f);
+    while (ch != EOF) {
+        if (ch == '#') {
+            while (ch != '\n' && ch != EOF) ch = std::fgetc(f);
+        } else if (std::isspace(ch)) {
+            ch = std::fgetc(f);
+        } else {
+            break;
+        }
+    }
+    if (ch == EOF || !std::isdigit(ch)) return false;
+    value = 0;
+    while (ch != EOF && std::isdigit(ch)) {
+        value = value * 10 + (ch - '0');
+        ch = std::fgetc(f);
+    }
+    return true;
+}
+
+int mcpt_load_ppm(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8) {
+    CHECK_ARG(path && width && height, "bad arguments");
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) return set_error(MCPT_ERR_INVALID, "cannot open '%s'", path);
+    int w = 0, h = 0, maxv = 0;
+    const bool ok = std::fgetc(f) == 'P' && std::fgetc(f) == '6' && ppm_int(f, w) && ppm_int(f, h) &&
+                    ppm_int(f, maxv) && maxv == 255 && w > 0 && h > 0 && !std::feof(f) && !std::ferror(f);
+    if (!ok) {
+        std::fclose(f);
+        return set_error(MCPT_ERR_INVALID, "'%s' is not a binary PPM with maxval 255", path);
+    }
+    *width = w;
+    *height = h;
+    if (rgba8) {
+        std::vector<uint8_t> rgb((size_t)w * h * 3);
+        const size_t got = std::fread(rgb.data(), 1, rgb.size(), f);
+        if (got != rgb.size()) {
+            std::fclose(f);
+            return set_error(MCPT_ERR_INVALID, "'%s': %zu of %zu pixel bytes", path, got, rgb.size());
+        }
+        for (size_t i = 0; i < (size_t)w * h; i++) {
+            rgba8[4 * i] = rgb[3 * i];
+            rgba8[4 * i + 1] = rgb[3 * i + 1];
+            rgba8[4 * i + 2] = rgb[3 * i + 2];
+            rgba8[4 * i + 3] = 255;
+        }
+    }
+    std::fclose(f);
+    return MCPT_OK;
+}
+
 int mcpt_scene_destroy(mcpt_scene scene) {
     if (!scene) return MCPT_OK;
     hipSetDevice(scene->ctx->device);
@@ -293,6 +407,7 @@ int mcpt_scene_env_tables(mcpt_scene scene, float* row_cdf, float* col_cdf, floa
 
 int mcpt_lbvh_build(mcpt_scene scene, mcpt_lbvh_dump* dump) {
     CHECK_ARG(scene, "scene is NULL");
+    if (scene->servers.load() > 0) return set_error(MCPT_ERR_INVALID, "the scene is in use by a server");
     MCPT_HIP(hipSetDevice(scene->ctx->device));
     return lbvh_build(scene, dump);
 }
@@ -388,6 +503,7 @@ int mcpt_render(mcpt_scene scene, const mcpt_camera* camera, const mcpt_render_o
     CHECK_ARG(scene && camera && opts && film, "bad arguments");
     CHECK_ARG(film->ctx == scene->ctx, "film and scene belong to different contexts");
     CHECK_ARG(opts->mode >= MCPT_MODE_MEGAKERNEL && opts->mode <= MCPT_MODE_PERSISTENT, "unknown render mode");
+    MCPT_TRY(scene->check_textures());
     MCPT_HIP(hipSetDevice(scene->ctx->device));
     return render(scene, camera, opts, film, stats);
 }

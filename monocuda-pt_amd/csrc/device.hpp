@@ -215,6 +215,16 @@ struct BvhView {
     const int* tie;           // leaf -> rank in the reference's leaf-test order
 };
 
+// A texture a material can take its albedo from (sampleTexture,
+// src/path_tracer.cu:216-234; the solid albedo is the material's own).
+struct TexView {
+    int type;            // MCPT_TEX_CHECKER / MCPT_TEX_IMAGE
+    int w, h;            // image size
+    float scale;         // checker cells per unit of u and v
+    float ca[3], cb[3];  // checker colours, linear
+    const uint32_t* px;  // image RGBA8, top row first
+};
+
 struct SceneView {
     BvhView bvh;
     const float* hot;           // 9 floats per triangle, original order
@@ -226,7 +236,41 @@ struct SceneView {
     int has_medium;
     int stack_depth;  // deepest leaf in internal levels (<= 64)
     EnvView env;
+    const TexView* tex;   // the scene's textures
+    const float* srgb8;   // sRGB byte -> linear, 256 entries
 };
+
+// sampleTexture (src/path_tracer.cu:216-234). The image filter restates the
+// texture unit's normalised-coordinate bilinear read with wrap addressing in
+// FP32: x = u w - 1/2, y = (1 - v) h - 1/2, the four texels around (x, y)
+// decoded from sRGB and blended by the fractional parts.
+MCPT_DEV vec3 sample_albedo(const SceneView& sv, const mcpt_material& m, float u, float v) {
+    if (m.texture <= 0) return load3(m.albedo);
+    const TexView& t = sv.tex[m.texture - 1];
+    if (t.type == MCPT_TEX_CHECKER) {
+        const int cu = (int)floorf(u * t.scale), cv = (int)floorf(v * t.scale);
+        return ((cu + cv) & 1) == 0 ? vec3(t.ca[0], t.ca[1], t.ca[2]) : vec3(t.cb[0], t.cb[1], t.cb[2]);
+    }
+    // far-off or non-finite coordinates are brought into range first
+    const float x = fminf(fmaxf(u * (float)t.w - 0.5f, -1e9f), 1e9f);
+    const float y = fminf(fmaxf((1.0f - v) * (float)t.h - 0.5f, -1e9f), 1e9f);
+    const float x0 = floorf(x), y0 = floorf(y);
+    const float fx = x - x0, fy = y - y0;
+    const int ix0 = (((int)((long long)x0 % t.w)) + t.w) % t.w, iy0 = (((int)((long long)y0 % t.h)) + t.h) % t.h;
+    const int ix1 = (ix0 + 1) % t.w, iy1 = (iy0 + 1) % t.h;
+    const float* lut = sv.srgb8;
+    const uint32_t p00 = t.px[(size_t)iy0 * t.w + ix0], p10 = t.px[(size_t)iy0 * t.w + ix1];
+    const uint32_t p01 = t.px[(size_t)iy1 * t.w + ix0], p11 = t.px[(size_t)iy1 * t.w + ix1];
+    const float gx = 1.0f - fx, gy = 1.0f - fy;
+    float c[3];
+    for (int k = 0; k < 3; k++) {
+        const int sh = 8 * k;
+        const float a00 = lut[(p00 >> sh) & 255u], a10 = lut[(p10 >> sh) & 255u];
+        const float a01 = lut[(p01 >> sh) & 255u], a11 = lut[(p11 >> sh) & 255u];
+        c[k] = (a00 * gx + a10 * fx) * gy + (a01 * gx + a11 * fx) * fy;
+    }
+    return vec3(c[0], c[1], c[2]);
+}
 
 // ------------------------------------------------------- queue records
 struct alignas(16) RayItem {  // 32 bytes: two 16-byte loads per ray

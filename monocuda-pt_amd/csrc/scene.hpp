@@ -2,6 +2,8 @@
 // file provides.
 #pragma once
 
+#include <atomic>
+
 #include "device.hpp"
 #include "host_common.hpp"
 
@@ -34,6 +36,17 @@ struct mcpt_scene_s {
     mcpt::DevBuf<mcpt::LeafTri> leaf_tri;
     mcpt::DevBuf<int> leaf_prim;
     mcpt::DevBuf<float> root_box;  // 6 floats: min xyz, max xyz
+    // textures (mcpt_scene_add_texture): the table the kernels read, each
+    // image's pixels, the sRGB byte table, and the highest texture index a
+    // material names (checked before a render)
+    std::vector<mcpt::TexView> tex_host;
+    mcpt::DevBuf<mcpt::TexView> tex;
+    std::vector<mcpt::DevBuf<uint32_t>> tex_px;
+    mcpt::DevBuf<float> srgb8;
+    int max_material_texture = 0;
+    // servers holding views of this scene: it may not be rebuilt or get
+    // textures (the views would point at freed buffers) until they end
+    std::atomic<int> servers{0};
     mcpt::DevBuf<mcpt::WideNode> wide;  // four-wide quantized tree, root 0
     mcpt::DevBuf<int> tie;              // leaf -> reference leaf-test rank
     int n_wide = 0;
@@ -63,6 +76,8 @@ struct mcpt_scene_s {
         v.env.rot = env_rot;
         v.env.total = env_total;
         v.env.enabled = env_enabled ? 1 : 0;
+        v.tex = tex.ptr;
+        v.srgb8 = srgb8.ptr;
         return v;
     }
     void release() {
@@ -79,7 +94,14 @@ struct mcpt_scene_s {
         root_box.release(ctx);
         wide.release(ctx);
         tie.release(ctx);
+        tex.release(ctx);
+        for (auto& b : tex_px) b.release(ctx);
+        tex_px.clear();
+        tex_host.clear();
+        srgb8.release(ctx);
     }
+    // every texture a material names has been added
+    int check_textures() const;
 };
 
 namespace mcpt {

@@ -673,6 +673,7 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
     for (int i = 0; i < n_scenes; i++) {
         if (!scenes[i] || scenes[i]->ctx != &ctx->c) return set_error(MCPT_ERR_INVALID, "scene %d is invalid", i);
         if (!scenes[i]->built) return set_error(MCPT_ERR_INVALID, "scene %d has no BVH", i);
+        MCPT_TRY(scenes[i]->check_textures());
     }
     Ctx* c = &ctx->c;
     MCPT_HIP(hipSetDevice(c->device));
@@ -784,6 +785,7 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
     }
     s->st.film_budget = s->film_budget;
     s->worker = std::thread(srv_loop, s);
+    for (mcpt_scene sc : s->scenes) sc->servers++;  // held until mcpt_server_destroy
     *out = s;
     return MCPT_OK;
 }
@@ -913,6 +915,7 @@ int mcpt_server_destroy(mcpt_server s) {
     (void)hipEventDestroy(s->ev[1]);
     (void)hipStreamDestroy(s->stream);
     dev_free(s->ctx, s->pool_mem, s->pool_bytes);
+    for (mcpt_scene sc : s->scenes) sc->servers--;
     delete s;
     return MCPT_OK;
 }

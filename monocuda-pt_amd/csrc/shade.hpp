@@ -303,7 +303,7 @@ MCPT_DEV void shade_bounce(const SceneView& sv, const vec3& bg, float rr_cap, in
         }
         so = ro + rd * sd;
     }
-    ps.thr = ps.thr * load3(m.albedo);
+    ps.thr = ps.thr * sample_albedo(sv, m, h.u, h.v);
 
     const float pe = env_select_prob(sv.env, nl);
     if ((m.type == MCPT_MAT_LAMBERTIAN || m.type == MCPT_MAT_ISOTROPIC) && (nl > 0 || pe > 0.0f)) {

@@ -151,12 +151,23 @@ struct Env {
 
 }  // namespace
 
+namespace {
+struct Tex {  // a checker (linear colours) or an RGBA8 image
+    int type = 0, w = 0, h = 0;
+    float scale = 1.f;
+    float ca[3] = {0, 0, 0}, cb[3] = {0, 0, 0};
+    std::vector<uint32_t> px;
+};
+}  // namespace
+
 struct oracle_scene_s {
     std::vector<mcpt_tri_hot> hot;
     std::vector<mcpt_tri_cold> cold;
     std::vector<mcpt_material> mats;
     std::vector<mcpt_light> lights;
     Env env;
+    std::vector<Tex> tex;
+    float srgb8[256];  // sRGB byte -> linear
     // BVH
     int n = 0;
     std::vector<uint32_t> morton, keys;
@@ -498,6 +509,39 @@ struct PathCounts {
     uint64_t ext = 0, shd = 0, steps = 0, leaves = 0;
 };
 
+// srgbToLinear (src/path_tracer.cu:200-202)
+float srgb_to_linear(float e) { return e <= 0.04045f ? e / 12.92f : mf_pow((e + 0.055f) / 1.055f, 2.4f); }
+
+// sampleTexture (src/path_tracer.cu:216-234); images are read with the
+// texture unit's normalised-coordinate bilinear filter and wrap addressing,
+// restated in FP32: x = u w - 1/2, y = (1 - v) h - 1/2, the four texels
+// decoded from sRGB, then blended by the fractional parts.
+V3 albedo_at(const oracle_scene_s& s, const mcpt_material& m, float u, float v) {
+    if (m.texture <= 0) return v3(m.albedo);
+    const Tex& t = s.tex[m.texture - 1];
+    if (t.type == MCPT_TEX_CHECKER) {
+        const int cu = (int)floorf(u * t.scale), cv = (int)floorf(v * t.scale);
+        return ((cu + cv) & 1) == 0 ? V3(t.ca[0], t.ca[1], t.ca[2]) : V3(t.cb[0], t.cb[1], t.cb[2]);
+    }
+    const float x = fminf(fmaxf(u * (float)t.w - 0.5f, -1e9f), 1e9f);
+    const float y = fminf(fmaxf((1.0f - v) * (float)t.h - 0.5f, -1e9f), 1e9f);
+    const float x0 = floorf(x), y0 = floorf(y);
+    const float fx = x - x0, fy = y - y0;
+    const int ix0 = (((int)((long long)x0 % t.w)) + t.w) % t.w, iy0 = (((int)((long long)y0 % t.h)) + t.h) % t.h;
+    const int ix1 = (ix0 + 1) % t.w, iy1 = (iy0 + 1) % t.h;
+    const uint32_t p00 = t.px[(size_t)iy0 * t.w + ix0], p10 = t.px[(size_t)iy0 * t.w + ix1];
+    const uint32_t p01 = t.px[(size_t)iy1 * t.w + ix0], p11 = t.px[(size_t)iy1 * t.w + ix1];
+    const float gx = 1.0f - fx, gy = 1.0f - fy;
+    float c[3];
+    for (int k = 0; k < 3; k++) {
+        const int sh = 8 * k;
+        const float a00 = s.srgb8[(p00 >> sh) & 255u], a10 = s.srgb8[(p10 >> sh) & 255u];
+        const float a01 = s.srgb8[(p01 >> sh) & 255u], a11 = s.srgb8[(p11 >> sh) & 255u];
+        c[k] = (a00 * gx + a10 * fx) * gy + (a01 * gx + a11 * fx) * fy;
+    }
+    return V3(c[0], c[1], c[2]);
+}
+
 // mediumTransmittance (src/path_tracer.cu:2874-2898), homogeneous media
 float medium_transmittance(const oracle_scene_s& s, V3 o, const V3& d, float maxd, PathCounts& pc) {
     float tr = 1.0f;
@@ -579,7 +623,7 @@ V3 trace_path(const oracle_scene_s& s, const mcpt_camera& cam, int px, int py, R
             }
             so = ro + rd * sd;
         }
-        att = att * v3(m.albedo);
+        att = att * albedo_at(s, m, hit.u, hit.v);
 
         // next-event estimation (src/path_tracer.cu:2962-3007)
         float pe = env_select_prob(s.env, nl);
@@ -1036,6 +1080,37 @@ int oracle_scene_create(const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int 
 }
 
 void oracle_scene_destroy(oracle_scene s) { delete s; }
+
+void oracle_sample_albedo(oracle_scene s, const mcpt_material* m, const float* uv, int n, float* out) {
+    for (int i = 0; i < n; i++) {
+        const V3 c = albedo_at(*s, *m, uv[2 * i], uv[2 * i + 1]);
+        out[3 * i] = c.x;
+        out[3 * i + 1] = c.y;
+        out[3 * i + 2] = c.z;
+    }
+}
+
+int oracle_scene_add_texture(oracle_scene s, const mcpt_texture_desc* d) {
+    Tex t;
+    t.type = d->type;
+    if (d->type == MCPT_TEX_CHECKER) {
+        t.scale = d->scale;
+        for (int k = 0; k < 3; k++) {  // make_checker_texture (:211-213)
+            t.ca[k] = srgb_to_linear(d->color_a[k]);
+            t.cb[k] = srgb_to_linear(d->color_b[k]);
+        }
+    } else if (d->type == MCPT_TEX_IMAGE && d->width > 0 && d->height > 0 && d->rgba8) {
+        t.w = d->width;
+        t.h = d->height;
+        t.px.resize((size_t)t.w * t.h);
+        std::memcpy(t.px.data(), d->rgba8, t.px.size() * 4);
+    } else {
+        return -1;
+    }
+    for (int i = 0; i < 256; i++) s->srgb8[i] = srgb_to_linear((float)i / 255.0f);
+    s->tex.push_back(std::move(t));
+    return (int)s->tex.size() - 1;
+}
 
 int oracle_scene_lights(oracle_scene s, mcpt_light* out, int cap) {
     int nl = (int)s->lights.size();

@@ -51,6 +51,11 @@ void oracle_karras(const uint32_t* keys, int n, int32_t* left, int32_t* right, u
 int oracle_scene_create(const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int n, const mcpt_material* mats,
                         int n_mats, const mcpt_env_desc* env, oracle_scene* out);
 void oracle_scene_destroy(oracle_scene s);
+/* sampleTexture's checker and image textures (src/path_tracer.cu:216-234);
+ * returns the texture's index (materials name it as texture = index + 1). */
+int oracle_scene_add_texture(oracle_scene s, const mcpt_texture_desc* desc);
+/* the albedo material m gives at n (u, v) points (closed-form tests) */
+void oracle_sample_albedo(oracle_scene s, const mcpt_material* m, const float* uv, int n, float* out);
 int oracle_scene_dump(oracle_scene s, mcpt_lbvh_dump* dump);
 int oracle_scene_lights(oracle_scene s, mcpt_light* out, int cap);
 

@@ -18,7 +18,7 @@ HOT_DTYPE = np.dtype([("v0", "<f4", 3), ("v1", "<f4", 3), ("v2", "<f4", 3)])
 COLD_DTYPE = np.dtype([("uv0", "<f4", 2), ("uv1", "<f4", 2), ("uv2", "<f4", 2), ("n0", "<f4", 3),
                        ("n1", "<f4", 3), ("n2", "<f4", 3), ("material", "<i4")])
 MAT_DTYPE = np.dtype([("type", "<i4"), ("albedo", "<f4", 3), ("emission", "<f4", 3), ("fuzz", "<f4"),
-                      ("ior", "<f4"), ("density", "<f4"), ("phase_g", "<f4"), ("pad", "<i4")])
+                      ("ior", "<f4"), ("density", "<f4"), ("phase_g", "<f4"), ("texture", "<i4")])
 LIGHT_DTYPE = np.dtype([("tri", HOT_DTYPE), ("emission", "<f4", 3), ("area", "<f4"), ("normal", "<f4", 3)])
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("time", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_type", "<i4"), ("prim_index", "<i4"), ("did_hit", "<i4")])
@@ -52,6 +52,8 @@ def lib():
             "oracle_karras": ([VP, C.c_int, VP, VP, VP, VP, VP, VP], None),
             "oracle_scene_create": ([VP, VP, C.c_int, VP, C.c_int, VP, C.POINTER(VP)], C.c_int),
             "oracle_scene_destroy": ([VP], None),
+            "oracle_scene_add_texture": ([VP, VP], C.c_int),
+            "oracle_sample_albedo": ([VP, VP, VP, C.c_int, VP], None),
             "oracle_scene_dump": ([VP, VP], C.c_int),
             "oracle_scene_lights": ([VP, VP, C.c_int], C.c_int),
             "oracle_trace_closest": ([VP, VP, C.c_int, VP, VP, VP, C.c_int], None),
@@ -175,6 +177,35 @@ class Scene:
         if getattr(self, "h", None):
             lib().oracle_scene_destroy(self.h)
             self.h = None
+
+    def add_texture(self, kind, color_a=(0, 0, 0), color_b=(1, 1, 1), scale=1.0, image=None):
+        """the checker's copy of a scene texture; kind 1 checker, 2 image
+        ((H, W, 4) RGBA8); returns the index (materials use index + 1)"""
+        class Desc(C.Structure):
+            _fields_ = [("type", C.c_int32), ("color_a", C.c_float * 3), ("color_b", C.c_float * 3),
+                        ("scale", C.c_float), ("width", C.c_int32), ("height", C.c_int32), ("rgba8", C.c_void_p)]
+        d = Desc()
+        d.type = kind
+        d.color_a[:] = [float(x) for x in color_a]
+        d.color_b[:] = [float(x) for x in color_b]
+        d.scale = scale
+        keep = None
+        if image is not None:
+            keep = np.ascontiguousarray(image, np.uint8)
+            d.height, d.width = keep.shape[:2]
+            d.rgba8 = keep.ctypes.data
+        k = lib().oracle_scene_add_texture(self.h, C.byref(d))
+        if k < 0:
+            raise ValueError("bad texture")
+        return k
+
+    def sample_albedo(self, mat, uv):
+        """albedo of material record `mat` at (u, v) rows"""
+        m = np.ascontiguousarray(np.asarray(mat, MAT_DTYPE).reshape(1))
+        uv = np.ascontiguousarray(np.asarray(uv, np.float32).reshape(-1, 2))
+        out = np.zeros((len(uv), 3), np.float32)
+        lib().oracle_sample_albedo(self.h, _p(m), _p(uv), len(uv), _p(out))
+        return out
 
     def dump(self):
         n, ni = self.n, max(self.n - 1, 0)

@@ -259,6 +259,54 @@ def test_render_obj_scene(mcpt, ctx, oracle, tmp_path):
     scene.close()
 
 
+def test_render_textures(mcpt, ctx, oracle, tmp_path):
+    """A room with a checker floor and an image on the back wall renders the
+    same on the GPU (every mode) and in the checker (sampleTexture,
+    src/path_tracer.cu:216-234, the same FP32 filter on both sides)."""
+    (tmp_path / "room.mtl").write_text(ROOM_MTL)
+    text = ROOM_OBJ.replace("usemtl white\nf 1 2 3 4\nf 5 8 7 6\nf 1 5 6 2\n",
+                            "vt 0 0\nvt 1 0\nvt 1 1\nvt 0 1\n"
+                            "usemtl white\nf 1/1 2/2 3/3 4/4\nf 5 8 7 6\nf 1/1 5/4 6/3 2/2\n")
+    (tmp_path / "room.obj").write_text(text)
+    hot, cold, mats, st = mcpt.load_obj(str(tmp_path / "room.obj"), scale=2.0)
+    rng = np.random.default_rng(5)
+    img = rng.integers(0, 256, (37, 53, 4), dtype=np.uint8)
+    # materials: the floor and back wall (white, used by faces 0-1 and 4-5)
+    # get their own copies: floor -> checker, back wall -> image
+    mats = np.concatenate([mats, mats[1:2], mats[1:2]])
+    mats[-2]["texture"], mats[-1]["texture"] = 1, 2
+    cold["material"][0:2] = len(mats) - 2
+    cold["material"][4:6] = len(mats) - 1
+    W = H = 64
+    spp, depth, seed = 4, 6, 9
+    desc = mcpt.gen_camera(mcpt.GEN_CORNELL, W, H)
+    desc.look_from[:] = (0.0, 1.0, 3.5)
+    desc.look_at[:] = (0.0, 1.0, 0.0)
+    cam = mcpt.camera_init(desc)
+    scene = ctx.scene(hot, cold, mats)
+    ref = oracle.Scene(hot, cold, mats)
+    for target in (scene, ref):
+        assert target.add_texture(mcpt.TEX_CHECKER, color_a=(0.9, 0.9, 0.9), color_b=(0.1, 0.3, 0.8), scale=6.0) == 0
+        assert target.add_texture(mcpt.TEX_IMAGE, image=img) == 1
+    scene.build()
+    ref_img, _, _ = ref.render(oracle.camera_init(desc), spp, depth, seed=seed)
+    out = _render_modes(mcpt, ctx, scene, cam, spp, depth, seed)
+    base = out[mcpt.MODE_MEGAKERNEL][0]
+    for mode, (im, _) in out.items():
+        assert np.array_equal(im, base), mode
+        assert rel_mse(im, ref_img) < 1e-4, mode
+        print(f"textures mode {mode}: identical to the checker {np.mean(im == ref_img):.5f}", flush=True)
+    # a material naming a texture the scene lacks is refused
+    lone = ctx.scene(hot, cold, mats)
+    lone.build()
+    film = ctx.film(W, H)
+    with pytest.raises(mcpt.McptError):
+        lone.render(cam, film, spp=1, max_depth=2)
+    film.close()
+    lone.close()
+    scene.close()
+
+
 def _rays(n, seed, origin_box=(0.0, 1.0), tmax=1e30):
     rng = np.random.default_rng(seed)
     r = np.zeros(n, mcpt_ray_dtype())

@@ -88,3 +88,22 @@ def test_png_round_trip(mcpt, tmp_path, w, h):
     p = str(tmp_path / "out.png")
     mcpt.write_png(p, img)
     assert np.array_equal(_read_png(p), img)  # 300x250 spans several stored blocks
+
+
+def test_load_ppm(mcpt, tmp_path):
+    """loadPpmAsRgba8 (src/path_tracer.cu:236-294): P6, '#' comment lines in
+    the header, maxval 255, alpha 255 appended."""
+    px = bytes([10, 20, 30, 40, 50, 60, 70, 80, 90, 200, 210, 220])
+    p = tmp_path / "a.ppm"
+    p.write_bytes(b"P6\n# made for a test\n2 2\n# another\n255\n" + px)
+    img = mcpt.load_ppm(str(p))
+    assert img.shape == (2, 2, 4)
+    assert np.array_equal(img[..., :3].reshape(-1), np.frombuffer(px, np.uint8)) and np.all(img[..., 3] == 255)
+    bad = tmp_path / "b.ppm"
+    bad.write_bytes(b"P6\n2 2\n65535\n" + px)
+    with pytest.raises(mcpt.McptError):
+        mcpt.load_ppm(str(bad))
+    short = tmp_path / "c.ppm"
+    short.write_bytes(b"P6 2 2 255\n" + px[:7])
+    with pytest.raises(mcpt.McptError):
+        mcpt.load_ppm(str(short))
