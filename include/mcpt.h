@@ -528,6 +528,18 @@ int mcpt_scene_add_texture(mcpt_scene scene, const mcpt_texture_desc* desc, int3
  * pattern: rgba8 NULL to get the size. */
 int mcpt_load_ppm(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8);
 
+/* Environment map files for mcpt_env_desc (RGBA FP32, top row first, alpha
+ * 1). Two-call pattern: rgba NULL to get the size.
+ * mcpt_load_pfm: loadPfmAsRgba32f (src/path_tracer.cu:335-368), "PF"
+ * little-endian files only (a positive scale is refused), bottom row first
+ * in the file.
+ * mcpt_load_hdr: Radiance RGBE, the file type `--env` reads with
+ * loadImageRgbaFloat (src/image_io.cpp:41-60) and scripts/hdr2pfm.py:14-61
+ * converts: "-Y h +X w" only, flat or new-style run-length rows, each texel
+ * byte * 2^(e - 136), 0 when e = 0. OpenEXR is not read. */
+int mcpt_load_pfm(const char* path, int32_t* width, int32_t* height, float* rgba);
+int mcpt_load_hdr(const char* path, int32_t* width, int32_t* height, float* rgba);
+
 /* ------------------------------------------------------------ OBJ import
  * Wavefront OBJ + MTL into triangle records (loadObject / loadMtlFile,
  * src/path_tracer.cu:2527-2859): fan triangulation, 1-based and negative
@@ -536,27 +548,43 @@ int mcpt_load_ppm(const char* path, int32_t* width, int32_t* height, uint8_t* rg
  * normals when the file has none; degenerate and out-of-range faces
  * skipped. Materials: index 0 is `fallback` (faces without a known
  * material), 1 + i the i-th MTL material (Ke > 0: emissive; otherwise
- * Lambertian with Kd converted from sRGB). Image textures are not loaded
- * (map_Kd materials keep their Kd colour). Two-call pattern: NULL arrays
- * to get the counts in `stats`, then arrays of that size. */
+ * Lambertian with Kd converted from sRGB). With load_textures, a map_Kd
+ * image is read as the PPM beside it (the path's extension replaced by
+ * .ppm, convertTexturePathToPpm :2580-2591; one read per path,
+ * getOrLoadObjTexture :2594-2611) and its material gets texture = 1 + the
+ * image's position in the list mcpt_obj_texture hands out; a material
+ * whose image cannot be read keeps its Kd colour, as in the reference.
+ * Two-call pattern: NULL arrays to get the counts in `stats`, then arrays
+ * of that size. */
 typedef struct mcpt_obj_opts {
     float translation[3];    /* where the model's bottom centre ends up  */
     float uniform_scale;     /* its largest side after normalisation (> 0) */
     float rotation_y;        /* radians about +y                          */
     mcpt_material fallback;
+    int32_t load_textures;   /* 1: read map_Kd images (PPM)               */
 } mcpt_obj_opts;
 
 typedef struct mcpt_obj_stats {
     int32_t n_tris;
     int32_t n_materials;       /* fallback + MTL materials               */
     int32_t skipped_faces;     /* out-of-range indices or zero area      */
-    int32_t texture_requests;  /* map_Kd materials (rendered with Kd)    */
+    int32_t texture_requests;  /* map_Kd materials that are not emissive */
     int32_t missing_libraries; /* mtllib files that could not be opened  */
+    int32_t textures_loaded;   /* distinct images read (load_textures)   */
+    int32_t missing_textures;  /* map_Kd materials left with Kd (load_textures) */
     int32_t reserved;
 } mcpt_obj_stats;
 
 int mcpt_load_obj(const char* path, const mcpt_obj_opts* opts, mcpt_tri_hot* hot, mcpt_tri_cold* cold,
                   mcpt_material* materials, mcpt_obj_stats* stats);
+
+/* The index-th image of this thread's last successful mcpt_load_obj call
+ * (RGBA8, top row first), kept until that thread's next load. Add the
+ * images to the scene in order with mcpt_scene_add_texture so that the
+ * materials' texture fields name them. Two-call pattern: rgba8 NULL to get
+ * the size. No reference counterpart: the reference uploads each image to
+ * a texture object inside its loader. */
+int mcpt_obj_texture(int32_t index, int32_t* width, int32_t* height, uint8_t* rgba8);
 
 /* Procedural equirect sky: gradient + sun disk + glow (closed form of
  * scripts/make_sky.py:5-27), width*height*4 floats, top row first. */

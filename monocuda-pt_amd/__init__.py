@@ -108,12 +108,13 @@ class RenderStats(C.Structure):
 
 class ObjOpts(C.Structure):
     _fields_ = [("translation", C.c_float * 3), ("uniform_scale", C.c_float), ("rotation_y", C.c_float),
-                ("fallback", C.c_uint8 * MAT_DTYPE.itemsize)]
+                ("fallback", C.c_uint8 * MAT_DTYPE.itemsize), ("load_textures", C.c_int32)]
 
 
 class ObjStats(C.Structure):
     _fields_ = [(k, C.c_int32) for k in ("n_tris", "n_materials", "skipped_faces", "texture_requests",
-                                          "missing_libraries", "reserved")]
+                                          "missing_libraries", "textures_loaded", "missing_textures",
+                                          "reserved")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -248,6 +249,9 @@ _SIGS = {
     "mcpt_load_obj": ([C.c_char_p, C.POINTER(ObjOpts), VP, VP, VP, C.POINTER(ObjStats)], C.c_int),
     "mcpt_scene_add_texture": ([VP, C.POINTER(TextureDesc), C.POINTER(C.c_int32)], C.c_int),
     "mcpt_load_ppm": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
+    "mcpt_load_pfm": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
+    "mcpt_load_hdr": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
+    "mcpt_obj_texture": ([C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_gen_sky": ([C.c_int32, C.c_int32, VP], C.c_int),
     "mcpt_gen_camera": ([C.c_int32, C.c_int32, C.c_int32, C.POINTER(CameraDesc)], C.c_int),
 }
@@ -327,10 +331,12 @@ def gen_scene(kind, seed=1, param=0):
     return hot, cold, mats
 
 
-def load_obj(path, translation=(0.0, 0.0, 0.0), scale=1.0, rotation_y=0.0, fallback=None):
-    """mcpt_load_obj: an OBJ (+ MTL) file -> (hot, cold, materials, stats).
-    Material 0 is `fallback` (a MAT_DTYPE record; default Lambertian grey
-    0.73 linear), 1 + i the file's i-th MTL material."""
+def load_obj(path, translation=(0.0, 0.0, 0.0), scale=1.0, rotation_y=0.0, fallback=None, load_textures=False):
+    """mcpt_load_obj: an OBJ (+ MTL) file -> (hot, cold, materials, stats),
+    plus the list of map_Kd images (RGBA8 arrays; material texture k names
+    the k-th, counting from 1) when load_textures is set. Material 0 is
+    `fallback` (a MAT_DTYPE record; default Lambertian grey 0.73 linear),
+    1 + i the file's i-th MTL material."""
     if fallback is None:
         fallback = np.zeros(1, MAT_DTYPE)[0]
         fallback["albedo"] = 0.73
@@ -339,6 +345,7 @@ def load_obj(path, translation=(0.0, 0.0, 0.0), scale=1.0, rotation_y=0.0, fallb
     o.uniform_scale = scale
     o.rotation_y = rotation_y
     C.memmove(o.fallback, np.asarray(fallback, MAT_DTYPE).tobytes(), MAT_DTYPE.itemsize)
+    o.load_textures = 1 if load_textures else 0
     st = ObjStats()
     p = os.fsencode(path)
     _check(_lib.mcpt_load_obj(p, C.byref(o), None, None, None, C.byref(st)), "mcpt_load_obj")
@@ -346,7 +353,16 @@ def load_obj(path, translation=(0.0, 0.0, 0.0), scale=1.0, rotation_y=0.0, fallb
     cold = np.zeros(st.n_tris, COLD_DTYPE)
     mats = np.zeros(st.n_materials, MAT_DTYPE)
     _check(_lib.mcpt_load_obj(p, C.byref(o), _ptr(hot), _ptr(cold), _ptr(mats), C.byref(st)), "mcpt_load_obj")
-    return hot, cold, mats, st.as_dict()
+    if not load_textures:
+        return hot, cold, mats, st.as_dict()
+    images = []
+    for k in range(st.textures_loaded):
+        w, h = C.c_int32(), C.c_int32()
+        _check(_lib.mcpt_obj_texture(k, C.byref(w), C.byref(h), None), "mcpt_obj_texture")
+        img = np.zeros((h.value, w.value, 4), np.uint8)
+        _check(_lib.mcpt_obj_texture(k, C.byref(w), C.byref(h), _ptr(img)), "mcpt_obj_texture")
+        images.append(img)
+    return hot, cold, mats, st.as_dict(), images
 
 
 def load_ppm(path):
@@ -357,6 +373,37 @@ def load_ppm(path):
     out = np.zeros((h.value, w.value, 4), np.uint8)
     _check(_lib.mcpt_load_ppm(p, C.byref(w), C.byref(h), _ptr(out)), "mcpt_load_ppm")
     return out
+
+
+def _load_float_image(fn, name, path):
+    w, h = C.c_int32(), C.c_int32()
+    p = os.fsencode(path)
+    _check(fn(p, C.byref(w), C.byref(h), None), name)
+    out = np.zeros((h.value, w.value, 4), np.float32)
+    _check(fn(p, C.byref(w), C.byref(h), _ptr(out)), name)
+    return out
+
+
+def load_pfm(path):
+    """mcpt_load_pfm: a little-endian PFM as an (H, W, 4) RGBA FP32 array,
+    top row first."""
+    return _load_float_image(_lib.mcpt_load_pfm, "mcpt_load_pfm", path)
+
+
+def load_hdr(path):
+    """mcpt_load_hdr: a Radiance RGBE file as an (H, W, 4) RGBA FP32 array."""
+    return _load_float_image(_lib.mcpt_load_hdr, "mcpt_load_hdr", path)
+
+
+def load_env(path):
+    """An environment map file by its extension, as the reference's --env
+    option chooses (src/path_tracer.cu:4197-4199): .pfm or .hdr."""
+    low = os.fspath(path).lower()
+    if low.endswith(".pfm"):
+        return load_pfm(path)
+    if low.endswith(".hdr"):
+        return load_hdr(path)
+    raise ValueError(f"{path}: environment maps are read from .pfm or .hdr files (OpenEXR is not read)")
 
 
 def write_png(path, rgb8):

@@ -321,58 +321,36 @@ int mcpt_scene_add_texture(mcpt_scene scene, const mcpt_texture_desc* d, int32_t
     return MCPT_OK;
 }
 
-// readNextPpmInteger (:236-256): skips whitespace and '#' comment lines,
-// reads decimal digits and consumes the one separator after them
-static bool ppm_int(std::FILE* f, int& value) {
-    if (std::feof(f) || std::ferror(f)) return false;
-    int ch = std::fgetc(f);
-    while (ch != EOF) {
-        if (ch == '#') {
-            while (ch != '\n' && ch != EOF) ch = std::fgetc(f);
-        } else if (std::isspace(ch)) {
-            ch = std::fgetc(f);
-        } else {
-            break;
-        }
-    }
-    if (ch == EOF || !std::isdigit(ch)) return false;
-    value = 0;
-    while (ch != EOF && std::isdigit(ch)) {
-        value = value * 10 + (ch - '0');
-        ch = std::fgetc(f);
-    }
-    return true;
-}
-
 int mcpt_load_ppm(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8) {
     CHECK_ARG(path && width && height, "bad arguments");
-    std::FILE* f = std::fopen(path, "rb");
-    if (!f) return set_error(MCPT_ERR_INVALID, "cannot open '%s'", path);
-    int w = 0, h = 0, maxv = 0;
-    const bool ok = std::fgetc(f) == 'P' && std::fgetc(f) == '6' && ppm_int(f, w) && ppm_int(f, h) &&
-                    ppm_int(f, maxv) && maxv == 255 && w > 0 && h > 0 && !std::feof(f) && !std::ferror(f);
-    if (!ok) {
-        std::fclose(f);
-        return set_error(MCPT_ERR_INVALID, "'%s' is not a binary PPM with maxval 255", path);
-    }
+    int w = 0, h = 0;
+    std::vector<uint8_t> px;
+    MCPT_TRY(read_ppm(path, w, h, rgba8 ? &px : nullptr));
     *width = w;
     *height = h;
-    if (rgba8) {
-        std::vector<uint8_t> rgb((size_t)w * h * 3);
-        const size_t got = std::fread(rgb.data(), 1, rgb.size(), f);
-        if (got != rgb.size()) {
-            std::fclose(f);
-            return set_error(MCPT_ERR_INVALID, "'%s': %zu of %zu pixel bytes", path, got, rgb.size());
-        }
-        for (size_t i = 0; i < (size_t)w * h; i++) {
-            rgba8[4 * i] = rgb[3 * i];
-            rgba8[4 * i + 1] = rgb[3 * i + 1];
-            rgba8[4 * i + 2] = rgb[3 * i + 2];
-            rgba8[4 * i + 3] = 255;
-        }
-    }
-    std::fclose(f);
+    if (rgba8) std::memcpy(rgba8, px.data(), px.size());
     return MCPT_OK;
+}
+
+// the environment readers share the two-call shape
+static int load_env_file(int (*reader)(const char*, int&, int&, std::vector<float>*), const char* path,
+                         int32_t* width, int32_t* height, float* rgba) {
+    CHECK_ARG(path && width && height, "bad arguments");
+    int w = 0, h = 0;
+    std::vector<float> px;
+    MCPT_TRY(reader(path, w, h, rgba ? &px : nullptr));
+    *width = w;
+    *height = h;
+    if (rgba) std::memcpy(rgba, px.data(), px.size() * sizeof(float));
+    return MCPT_OK;
+}
+
+int mcpt_load_pfm(const char* path, int32_t* width, int32_t* height, float* rgba) {
+    return load_env_file(read_pfm, path, width, height, rgba);
+}
+
+int mcpt_load_hdr(const char* path, int32_t* width, int32_t* height, float* rgba) {
+    return load_env_file(read_hdr, path, width, height, rgba);
 }
 
 int mcpt_scene_destroy(mcpt_scene scene) {
@@ -543,6 +521,10 @@ int mcpt_gen_sky(int32_t width, int32_t height, float* rgba) { return gen_sky(wi
 int mcpt_load_obj(const char* path, const mcpt_obj_opts* opts, mcpt_tri_hot* hot, mcpt_tri_cold* cold,
                   mcpt_material* materials, mcpt_obj_stats* stats) {
     return load_obj(path, opts, hot, cold, materials, stats);
+}
+
+int mcpt_obj_texture(int32_t index, int32_t* width, int32_t* height, uint8_t* rgba8) {
+    return obj_texture(index, width, height, rgba8);
 }
 
 int mcpt_gen_camera(int32_t kind, int32_t width, int32_t height, mcpt_camera_desc* out) {

@@ -71,7 +71,12 @@ struct DevBuf {
     size_t bytes() const { return count * sizeof(T); }
 };
 
+// image files (imageio.cpp); a NULL vector reads the header only
+int read_ppm(const char* path, int& w, int& h, std::vector<uint8_t>* rgba);
+int read_pfm(const char* path, int& w, int& h, std::vector<float>* rgba);
+int read_hdr(const char* path, int& w, int& h, std::vector<float>* rgba);
 // OBJ + MTL import (objload.cpp)
+int obj_texture(int index, int32_t* width, int32_t* height, uint8_t* rgba8);
 int load_obj(const char* path, const mcpt_obj_opts* opts, mcpt_tri_hot* hot, mcpt_tri_cold* cold,
              mcpt_material* materials, mcpt_obj_stats* stats);
 // scene generators (scenegen.cpp); srgb_to_linear is srgbToLinear

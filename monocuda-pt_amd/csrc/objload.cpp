@@ -22,8 +22,15 @@
 //     from sRGB to linear (make_solid_texture, :207-209). usemtl of an
 //     unknown name uses the library's only material when it has exactly one,
 //     otherwise the fallback material.
-// Image textures are not loaded: a map_Kd material keeps its Kd colour,
-// which is what the reference renders when its texture cannot be loaded.
+//   * map_Kd (with opts.load_textures): the path with '\\' turned into '/'
+//     and its extension replaced by ".ppm", relative to the OBJ file's
+//     folder (convertTexturePathToPpm, :2580-2591), is read as a binary PPM
+//     once per path (getOrLoadObjTexture, :2594-2611); a material whose
+//     image loads is Lambertian with that image texture (material.texture =
+//     1 + its position in the list mcpt_obj_texture hands out), otherwise
+//     it keeps its Kd colour and counts as missing, as in the reference.
+//     Other image formats, which the reference decodes with stb_image, are
+//     not read: those materials keep Kd.
 #include <sys/stat.h>
 
 #include <cmath>
@@ -32,6 +39,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -81,11 +89,48 @@ std::string directory_of(const std::string& path) {
     return s == std::string::npos ? std::string(".") : path.substr(0, s);
 }
 
+struct Image {
+    int w = 0, h = 0;
+    std::vector<uint8_t> rgba;
+};
+
 struct Library {
     std::vector<mcpt_material> mats;
     std::map<std::string, int> by_name;
-    int texture_requests = 0;  // map_Kd materials (rendered with their Kd)
+    int texture_requests = 0;  // map_Kd materials
+    // texture loading (opts.load_textures)
+    bool load_textures = false;
+    std::string obj_dir;
+    std::map<std::string, int> tex_by_path;  // -1: the file could not be read
+    std::shared_ptr<std::vector<Image>> images = std::make_shared<std::vector<Image>>();
+    int missing_textures = 0;
 };
+
+// convertTexturePathToPpm (:2580-2591)
+std::string ppm_path_of(std::string p) {
+    for (char& ch : p)
+        if (ch == '\\') ch = '/';
+    const size_t dot = p.find_last_of('.'), slash = p.find_last_of('/');
+    if (dot != std::string::npos && (slash == std::string::npos || dot > slash)) p = p.substr(0, dot);
+    return p + ".ppm";
+}
+
+// getOrLoadObjTexture (:2594-2611): one read per path, failures remembered
+int texture_for(Library& lib, const std::string& map_kd) {
+    const std::string path = lib.obj_dir + "/" + ppm_path_of(map_kd);
+    auto it = lib.tex_by_path.find(path);
+    if (it != lib.tex_by_path.end()) return it->second;
+    Image im;
+    int idx = -1;
+    if (read_ppm(path.c_str(), im.w, im.h, &im.rgba) == MCPT_OK) {
+        idx = (int)lib.images->size();
+        lib.images->push_back(std::move(im));
+    } else {
+        clear_error();  // a missing texture is not an error of the load
+    }
+    lib.tex_by_path[path] = idx;
+    return idx;
+}
 
 mcpt_material lambertian_srgb(const vec3& srgb) {
     mcpt_material m;
@@ -115,6 +160,13 @@ bool load_mtl(const std::string& path, Library& lib) {
             m.emission[2] = ke.z;
         } else if (!texture.empty()) {
             lib.texture_requests++;
+            if (lib.load_textures) {
+                const int idx = texture_for(lib, texture);
+                if (idx >= 0)
+                    m.texture = idx + 1;
+                else
+                    lib.missing_textures++;
+            }
         }
         lib.by_name[name] = (int)lib.mats.size();
         lib.mats.push_back(m);
@@ -153,6 +205,7 @@ struct Loaded {
     std::vector<mcpt_tri_hot> hot;
     std::vector<mcpt_tri_cold> cold;
     std::vector<mcpt_material> mats;
+    std::shared_ptr<std::vector<Image>> images;
     mcpt_obj_stats stats;
 };
 
@@ -202,6 +255,8 @@ int load(const char* path, const mcpt_obj_opts* o, Loaded& out) {
     std::vector<Vec2> uvs;
     std::vector<Face> faces;
     Library lib;
+    lib.load_textures = o->load_textures != 0;
+    lib.obj_dir = dir;
     int current = -1, missing_libs = 0;
     std::string key, tok;
     std::vector<Corner> cs;
@@ -318,6 +373,9 @@ int load(const char* path, const mcpt_obj_opts* o, Loaded& out) {
     out.stats.n_materials = (int32_t)out.mats.size();
     out.stats.texture_requests = lib.texture_requests;
     out.stats.missing_libraries = missing_libs;
+    out.stats.textures_loaded = (int32_t)lib.images->size();
+    out.stats.missing_textures = lib.missing_textures;
+    out.images = lib.images;
     if (out.hot.empty()) return set_error(MCPT_ERR_INVALID, "OBJ file '%s': every face was skipped", path);
     return MCPT_OK;
 }
@@ -336,6 +394,8 @@ struct Cached {
     Loaded data;
 };
 thread_local Cached g_cache;
+// the images of this thread's last successful load, for obj_texture
+thread_local std::shared_ptr<std::vector<Image>> g_images;
 
 bool file_id(const char* path, long long& size, long long& mtime) {
     struct stat sb;
@@ -365,6 +425,7 @@ int load_obj(const char* path, const mcpt_obj_opts* opts, mcpt_tri_hot* hot, mcp
         c.valid = have_id;
     }
     const Loaded& l = c.data;
+    g_images = l.images;
     if (stats) *stats = l.stats;
     if (hot) std::memcpy(hot, l.hot.data(), l.hot.size() * sizeof(mcpt_tri_hot));
     if (cold) std::memcpy(cold, l.cold.data(), l.cold.size() * sizeof(mcpt_tri_cold));
@@ -372,6 +433,18 @@ int load_obj(const char* path, const mcpt_obj_opts* opts, mcpt_tri_hot* hot, mcp
     if (hot || cold || materials) {  // the arrays are filled: free the copy
         c = Cached();
     }
+    return MCPT_OK;
+}
+
+int obj_texture(int index, int32_t* width, int32_t* height, uint8_t* rgba8) {
+    if (!width || !height) return set_error(MCPT_ERR_INVALID, "bad arguments");
+    const int n = g_images ? (int)g_images->size() : 0;
+    if (index < 0 || index >= n)
+        return set_error(MCPT_ERR_INVALID, "texture %d: the last OBJ load on this thread holds %d", index, n);
+    const Image& im = (*g_images)[index];
+    *width = im.w;
+    *height = im.h;
+    if (rgba8) std::memcpy(rgba8, im.rgba.data(), im.rgba.size());
     return MCPT_OK;
 }
 

@@ -307,6 +307,38 @@ def test_render_textures(mcpt, ctx, oracle, tmp_path):
     scene.close()
 
 
+def test_render_obj_map_kd(mcpt, ctx, oracle, tmp_path):
+    """A map_Kd image read by the OBJ loader (load_textures) and handed to
+    the scene renders the same on the GPU and in the checker."""
+    rng = np.random.default_rng(8)
+    img = rng.integers(0, 256, (16, 24, 3), dtype=np.uint8)
+    (tmp_path / "poster.ppm").write_bytes(b"P6\n24 16\n255\n" + img.tobytes())
+    (tmp_path / "room.mtl").write_text(ROOM_MTL + "newmtl poster\nKd 0.5 0.5 0.5\nmap_Kd poster.png\n")
+    text = ROOM_OBJ.replace("usemtl white\nf 1 2 3 4\nf 5 8 7 6\nf 1 5 6 2\n",
+                            "vt 0 0\nvt 1 0\nvt 1 1\nvt 0 1\n"
+                            "usemtl white\nf 1 2 3 4\nf 5 8 7 6\nusemtl poster\nf 1/1 5/4 6/3 2/2\n")
+    (tmp_path / "room.obj").write_text(text)
+    hot, cold, mats, st, images = mcpt.load_obj(str(tmp_path / "room.obj"), scale=2.0, load_textures=True)
+    assert st["textures_loaded"] == 1 and len(images) == 1 and np.count_nonzero(mats["texture"]) == 1
+    W = H = 48
+    spp, depth, seed = 4, 5, 21
+    desc = mcpt.gen_camera(mcpt.GEN_CORNELL, W, H)
+    desc.look_from[:] = (0.0, 1.0, 3.5)
+    desc.look_at[:] = (0.0, 1.0, 0.0)
+    cam = mcpt.camera_init(desc)
+    scene = ctx.scene(hot, cold, mats)
+    ref = oracle.Scene(hot, cold, mats)
+    for target in (scene, ref):
+        assert target.add_texture(mcpt.TEX_IMAGE, image=images[0]) == 0
+    scene.build()
+    ref_img, _, _ = ref.render(oracle.camera_init(desc), spp, depth, seed=seed)
+    out = _render_modes(mcpt, ctx, scene, cam, spp, depth, seed)
+    for mode, (im, _) in out.items():
+        assert rel_mse(im, ref_img) < 1e-4, mode
+        print(f"map_Kd mode {mode}: identical to the checker {np.mean(im == ref_img):.5f}", flush=True)
+    scene.close()
+
+
 def _rays(n, seed, origin_box=(0.0, 1.0), tmax=1e30):
     rng = np.random.default_rng(seed)
     r = np.zeros(n, mcpt_ray_dtype())

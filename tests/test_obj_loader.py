@@ -4,6 +4,7 @@ src/path_tracer.cu:2527-2859): fan triangulation, 1-based and negative
 indices, the normalisation transform, area-weighted vertex normals, rotated
 file normals, sRGB albedo, emissive Ke, the usemtl fallbacks and skipped
 faces. Runs on the CPU (no GPU call)."""
+import ctypes as C
 import math
 
 import numpy as np
@@ -170,3 +171,45 @@ def test_round_trip_generated_mesh(mcpt, tmp_path):
     scale = f32(1.0) / (hi - lo).max()
     want = ((v - bottom) * scale).reshape(-1, 3, 3)
     assert np.array_equal(np.stack([got["v0"], got["v1"], got["v2"]], 1), want)
+
+
+def _ppm(path, img):
+    h, w = img.shape[:2]
+    path.write_bytes(b"P6\n%d %d\n255\n" % (w, h) + np.ascontiguousarray(img[..., :3]).tobytes())
+
+
+def test_map_kd_textures(mcpt, tmp_path):
+    """map_Kd with load_textures (convertTexturePathToPpm and
+    getOrLoadObjTexture, src/path_tracer.cu:2580-2611, 2636-2646): the path's
+    backslashes become '/', its extension becomes .ppm, it is read relative
+    to the OBJ's folder once per path, a material whose image loads gets
+    that image, one whose image is missing keeps Kd."""
+    (tmp_path / "tex").mkdir()
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, 256, (5, 7, 3), dtype=np.uint8)
+    b = rng.integers(0, 256, (4, 2, 3), dtype=np.uint8)
+    _ppm(tmp_path / "tex" / "wall.ppm", a)
+    _ppm(tmp_path / "tex" / "b.v2.ppm", b)
+    _write(tmp_path, "m.mtl", "\n".join([
+        "newmtl w1", "Kd 0.5 0.5 0.5", "map_Kd tex\\wall.png",
+        "newmtl w2", "Kd 0.1 0.1 0.1", "map_Kd tex/wall.jpg",      # same .ppm: shared
+        "newmtl gone", "Kd 0.2 0.3 0.4", "map_Kd tex/none.png",    # missing: keeps Kd
+        "newmtl lamp", "Ke 1 1 1", "map_Kd tex/wall.png",          # emissive wins
+        "newmtl dotted", "Kd 1 1 1", "map_Kd tex/b.v2.tga",
+    ]) + "\n")
+    obj = "mtllib m.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\n" + "".join(
+        f"usemtl {m}\nf 1 2 3\n" for m in ("w1", "w2", "gone", "lamp", "dotted"))
+    path = _write(tmp_path, "t.obj", obj)
+    hot, cold, mats, st, images = mcpt.load_obj(path, load_textures=True)
+    assert st["texture_requests"] == 4 and st["textures_loaded"] == 2 and st["missing_textures"] == 1
+    assert len(images) == 2
+    for img, want in zip(images, (a, b)):
+        assert np.array_equal(img[..., :3], want) and np.all(img[..., 3] == 255)
+    assert list(mats["texture"][1:]) == [1, 1, 0, 0, 2]
+    assert np.allclose(mats[3]["albedo"], [0.0331048, 0.0732390, 0.1328683], rtol=1e-5)  # sRGB 0.2, 0.3, 0.4
+    # without load_textures nothing is read and every material keeps Kd
+    _, _, mats2, st2 = mcpt.load_obj(path)
+    assert np.all(mats2["texture"] == 0) and st2["textures_loaded"] == 0 and st2["texture_requests"] == 4
+    # the images handed out are those of the thread's last load, which has none
+    w, h = C.c_int32(), C.c_int32()
+    assert mcpt._lib.mcpt_obj_texture(0, C.byref(w), C.byref(h), None) != 0
