@@ -78,6 +78,19 @@ typedef struct mcpt_material {
                         k - 1 (mcpt_scene_add_texture) gives the albedo     */
 } mcpt_material;
 
+/* Sphere (src/path_tracer.cu:845-875), 48 bytes. A moving sphere's centre
+ * goes from `center` at time 0 to `center1` at time 1 (motion blur: every
+ * camera ray draws a time in [0, 1) and its path keeps it). Spheres are
+ * never on the light list; an emissive sphere is seen by rays that hit it. */
+typedef struct mcpt_sphere {
+    float center[3];
+    float radius;
+    float center1[3];
+    int32_t moving;   /* 0: static (center1 unused) */
+    int32_t material;
+    int32_t pad[3];
+} mcpt_sphere;
+
 /* Light (src/path_tracer.cu:942-947): an emissive triangle. 64 bytes. */
 typedef struct mcpt_light {
     mcpt_tri_hot tri;
@@ -106,8 +119,8 @@ typedef struct mcpt_ray {
 /* PackedHit (src/path_tracer.cu:991-996), the deferred 16-byte hit record. */
 typedef struct mcpt_packed_hit {
     float t;
-    int32_t prim_type;  /* 1 = triangle, -1 = miss */
-    int32_t prim_index; /* original triangle index, -1 = miss */
+    int32_t prim_type;  /* 0 = sphere, 1 = triangle, -1 = miss */
+    int32_t prim_index; /* original sphere or triangle index, -1 = miss */
     int32_t did_hit;
 } mcpt_packed_hit;
 
@@ -162,6 +175,15 @@ int mcpt_ctx_describe(mcpt_ctx ctx, char* buf, size_t buflen);
 int mcpt_scene_create(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int32_t n_tris,
                       const mcpt_material* materials, int32_t n_materials, const mcpt_env_desc* env,
                       mcpt_scene* out);
+/* The same with spheres. The BVH is built over the primitive list the
+ * reference's finalizeScene forms (:1955-1972): the spheres first, then the
+ * triangles, so sorted ids and leaf order count spheres from 0 and
+ * triangles from n_spheres. n_tris may be 0 when there are spheres. The
+ * batched build, batched render, server and radiance cache take
+ * triangle-only scenes. */
+int mcpt_scene_create_ex(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int32_t n_tris,
+                         const mcpt_sphere* spheres, int32_t n_spheres, const mcpt_material* materials,
+                         int32_t n_materials, const mcpt_env_desc* env, mcpt_scene* out);
 int mcpt_scene_destroy(mcpt_scene scene);
 int mcpt_scene_light_count(mcpt_scene scene, int32_t* n_lights);
 int mcpt_scene_lights(mcpt_scene scene, mcpt_light* out /* n_lights */);

@@ -51,6 +51,8 @@ COLD_DTYPE = np.dtype([("uv0", "<f4", 2), ("uv1", "<f4", 2), ("uv2", "<f4", 2), 
                        ("n1", "<f4", 3), ("n2", "<f4", 3), ("material", "<i4")])
 MAT_DTYPE = np.dtype([("type", "<i4"), ("albedo", "<f4", 3), ("emission", "<f4", 3), ("fuzz", "<f4"),
                       ("ior", "<f4"), ("density", "<f4"), ("phase_g", "<f4"), ("texture", "<i4")])
+SPHERE_DTYPE = np.dtype([("center", "<f4", 3), ("radius", "<f4"), ("center1", "<f4", 3), ("moving", "<i4"),
+                         ("material", "<i4"), ("pad", "<i4", 3)])
 LIGHT_DTYPE = np.dtype([("tri", HOT_DTYPE), ("emission", "<f4", 3), ("area", "<f4"), ("normal", "<f4", 3)])
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("time", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_type", "<i4"), ("prim_index", "<i4"), ("did_hit", "<i4")])
@@ -205,6 +207,8 @@ _SIGS = {
     "mcpt_ctx_memory": ([VP, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
     "mcpt_ctx_describe": ([VP, C.c_char_p, C.c_size_t], C.c_int),
     "mcpt_scene_create": ([VP, VP, VP, C.c_int32, VP, C.c_int32, C.POINTER(EnvDesc), C.POINTER(VP)], C.c_int),
+    "mcpt_scene_create_ex": ([VP, VP, VP, C.c_int32, VP, C.c_int32, VP, C.c_int32, C.POINTER(EnvDesc),
+                              C.POINTER(VP)], C.c_int),
     "mcpt_scene_destroy": ([VP], C.c_int),
     "mcpt_scene_light_count": ([VP, C.POINTER(C.c_int32)], C.c_int),
     "mcpt_scene_lights": ([VP, VP], C.c_int),
@@ -478,8 +482,8 @@ class Context:
         _check(_lib.mcpt_ctx_describe(self.h, buf, 1024), "mcpt_ctx_describe")
         return buf.value.decode()
 
-    def scene(self, hot, cold, mats, env=None):
-        return Scene(self, hot, cold, mats, env)
+    def scene(self, hot, cold, mats, env=None, spheres=None):
+        return Scene(self, hot, cold, mats, env, spheres)
 
     def film(self, width, height):
         return Film(self, width, height)
@@ -492,17 +496,25 @@ class Context:
 
 
 class Scene:
-    def __init__(self, ctx, hot, cold, mats, env=None):
+    def __init__(self, ctx, hot, cold, mats, env=None, spheres=None):
         self.ctx = ctx
         self.hot = np.ascontiguousarray(hot, HOT_DTYPE)
         self.cold = np.ascontiguousarray(cold, COLD_DTYPE)
         self.mats = np.ascontiguousarray(mats, MAT_DTYPE)
-        self.n = len(self.hot)
+        self.spheres = np.ascontiguousarray(spheres if spheres is not None else np.zeros(0, SPHERE_DTYPE),
+                                            SPHERE_DTYPE)
+        self.n = len(self.hot) + len(self.spheres)  # BVH primitives: spheres, then triangles
         self.h = VP()
         ctx._children.add(self)
         envp = C.byref(env) if env is not None else None
-        _check(_lib.mcpt_scene_create(ctx.h, _ptr(self.hot), _ptr(self.cold), self.n, _ptr(self.mats),
-                                      len(self.mats), envp, C.byref(self.h)), "mcpt_scene_create")
+        if len(self.spheres):
+            _check(_lib.mcpt_scene_create_ex(ctx.h, _ptr(self.hot) if len(self.hot) else None,
+                                             _ptr(self.cold) if len(self.cold) else None, len(self.hot),
+                                             _ptr(self.spheres), len(self.spheres), _ptr(self.mats), len(self.mats),
+                                             envp, C.byref(self.h)), "mcpt_scene_create_ex")
+        else:
+            _check(_lib.mcpt_scene_create(ctx.h, _ptr(self.hot), _ptr(self.cold), len(self.hot), _ptr(self.mats),
+                                          len(self.mats), envp, C.byref(self.h)), "mcpt_scene_create")
 
     def close(self):
         if self.h:

@@ -706,6 +706,7 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
     // scenes with one triangle take the single-scene path (its special root)
     std::vector<int> multi;
     float single_ms = 0.f;
+    for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_no_spheres("the batched build"));
     for (int i = 0; i < n; i++) {
         if (scenes[i]->n < 1) return set_error(MCPT_ERR_INVALID, "scene %d is empty", i);
         if (scenes[i]->n == 1) {
@@ -881,6 +882,7 @@ int mcpt_render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int32_t
                       mcpt_film* films, const mcpt_batch_tune* tune, mcpt_render_stats* stats) {
     MCPT_TRY(check_render_batch(scenes, cams, n, o));
     for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_textures());
+    for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_no_spheres("the batched render"));
     if (!films) return set_error(MCPT_ERR_INVALID, "films is NULL");
     std::vector<float*> acc(n);
     for (int i = 0; i < n; i++) {
@@ -896,6 +898,7 @@ int mcpt_batch_autotune(const mcpt_scene* scenes, const mcpt_camera* cams, int32
                         mcpt_batch_tune* best, double* search_ms) {
     MCPT_TRY(check_render_batch(scenes, cams, n, o));
     for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_textures());
+    for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_no_spheres("the batched render"));
     if (!best) return set_error(MCPT_ERR_INVALID, "best is NULL");
     Ctx* ctx = scenes[0]->ctx;
     MCPT_HIP(hipSetDevice(ctx->device));

@@ -540,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_cc_shade(SceneView sv, CacheWs ws, i
             const mcpt_material& m = sv.mats[sv.cold[hit.prim_index].material];
             diffuse = m.type == MCPT_MAT_LAMBERTIAN;
             if (diffuse) {
-                rebuild_hit(sv, hit.prim_index, ro, rd, hit.t, h);
+                rebuild_hit(sv, hit.prim_index, ro, rd, hit.t, 0.0f, h);
                 albedo = sample_albedo(sv, m, h.u, h.v);
             }
         }
@@ -921,6 +921,7 @@ int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* cam, const mcpt_rend
     if (!scene || !cam || !o || !film || !mlp) return set_error(MCPT_ERR_INVALID, "bad arguments");
     if (!scene->built) return set_error(MCPT_ERR_INVALID, "scene has no BVH (call mcpt_lbvh_build)");
     MCPT_TRY(scene->check_textures());
+    MCPT_TRY(scene->check_no_spheres("the radiance cache"));
     if (film->w != cam->width || film->h != cam->height)
         return set_error(MCPT_ERR_INVALID, "film %dx%d does not match camera %dx%d", film->w, film->h, cam->width,
                          cam->height);

@@ -198,14 +198,32 @@ int mcpt_ctx_describe(mcpt_ctx ctx, char* buf, size_t buflen) {
 int mcpt_scene_create(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int32_t n_tris,
                       const mcpt_material* materials, int32_t n_materials, const mcpt_env_desc* env,
                       mcpt_scene* out) {
+    CHECK_ARG(n_tris >= 1, "a scene needs at least one triangle");
+    return mcpt_scene_create_ex(ctx, hot, cold, n_tris, nullptr, 0, materials, n_materials, env, out);
+}
+
+int mcpt_scene_create_ex(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int32_t n_tris,
+                         const mcpt_sphere* spheres, int32_t n_spheres, const mcpt_material* materials,
+                         int32_t n_materials, const mcpt_env_desc* env, mcpt_scene* out) {
     CHECK_ARG(ctx && out, "ctx/out is NULL");
     *out = nullptr;
-    CHECK_ARG(n_tris >= 1 && hot && cold, "a scene needs at least one triangle");
+    CHECK_ARG(n_tris >= 0 && n_spheres >= 0 && n_tris + n_spheres >= 1, "a scene needs at least one primitive");
+    CHECK_ARG((int64_t)n_tris + n_spheres < (1LL << 30), "too many primitives");
+    CHECK_ARG(n_tris == 0 || (hot && cold), "triangle arrays are NULL");
+    CHECK_ARG(n_spheres == 0 || spheres, "sphere array is NULL");
     CHECK_ARG(n_materials >= 1 && materials, "a scene needs at least one material");
     for (int i = 0; i < n_tris; i++)
         if (cold[i].material < 0 || cold[i].material >= n_materials)
             return set_error(MCPT_ERR_INVALID, "triangle %d has material %d outside [0, %d)", i, cold[i].material,
                              n_materials);
+    for (int i = 0; i < n_spheres; i++) {
+        const mcpt_sphere& sp = spheres[i];
+        if (sp.material < 0 || sp.material >= n_materials)
+            return set_error(MCPT_ERR_INVALID, "sphere %d has material %d outside [0, %d)", i, sp.material,
+                             n_materials);
+        if (!(sp.radius > 0.0f) || !std::isfinite(sp.radius))
+            return set_error(MCPT_ERR_INVALID, "sphere %d: radius must be positive and finite", i);
+    }
     int max_tex = 0;
     for (int m = 0; m < n_materials; m++) {
         if (materials[m].texture < 0) return set_error(MCPT_ERR_INVALID, "material %d: negative texture", m);
@@ -216,12 +234,15 @@ int mcpt_scene_create(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_cold
     auto* s = new mcpt_scene_s();
     s->ctx = c;
     s->n = n_tris;
+    s->n_spheres = n_spheres;
     s->n_mats = n_materials;
     s->max_material_texture = max_tex;
     for (int m = 0; m < n_materials; m++)
         if (materials[m].type == MCPT_MAT_ISOTROPIC)
             for (int i = 0; i < n_tris; i++)
                 if (cold[i].material == m) s->has_medium = true;
+    for (int i = 0; i < n_spheres; i++)
+        if (materials[spheres[i].material].type == MCPT_MAT_ISOTROPIC) s->has_medium = true;
     // light list (finalizeScene, src/path_tracer.cu:1973-1985)
     for (int i = 0; i < n_tris; i++) {
         const mcpt_material& m = materials[cold[i].material];
@@ -242,6 +263,7 @@ int mcpt_scene_create(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_cold
     int st = upload(c, s->hot, reinterpret_cast<const float*>(hot), (size_t)n_tris * 9);
     if (st >= 0) st = upload(c, s->cold, cold, (size_t)n_tris);
     if (st >= 0) st = upload(c, s->mats, materials, (size_t)n_materials);
+    if (st >= 0 && n_spheres) st = upload(c, s->spheres, spheres, (size_t)n_spheres);
     if (st >= 0 && !s->host_lights.empty()) st = upload(c, s->lights, s->host_lights.data(), s->host_lights.size());
     if (st >= 0 && env && env->rgba) {
         if (env->width < 1 || env->height < 1) st = set_error(MCPT_ERR_INVALID, "bad environment size");
@@ -265,6 +287,12 @@ int mcpt_scene_create(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_cold
         return st;
     }
     *out = s;
+    return MCPT_OK;
+}
+
+int mcpt_scene_s::check_no_spheres(const char* what) const {
+    if (n_spheres > 0) return set_error(MCPT_ERR_INVALID, "%s takes triangle-only scenes (this one has %d spheres)",
+                                        what, n_spheres);
     return MCPT_OK;
 }
 
@@ -412,7 +440,7 @@ int mcpt_lbvh_dump_wide(mcpt_scene scene, void* nodes, int32_t cap, int32_t* n_w
     if (nodes && cap > 0)
         MCPT_HIP(hipMemcpy(nodes, scene->wide.ptr, (size_t)std::min(cap, scene->n_wide) * sizeof(WideNode),
                            hipMemcpyDeviceToHost));
-    if (tie) MCPT_HIP(hipMemcpy(tie, scene->tie.ptr, (size_t)scene->n * sizeof(int), hipMemcpyDeviceToHost));
+    if (tie) MCPT_HIP(hipMemcpy(tie, scene->tie.ptr, (size_t)scene->n_prims() * sizeof(int), hipMemcpyDeviceToHost));
     return MCPT_OK;
 }
 

@@ -183,6 +183,76 @@ MCPT_DEV bool hit_triangle_n(const vec3& v0, const vec3& v1, const vec3& v2, con
     return true;
 }
 
+// ---------------------------------------------------------------- spheres
+MCPT_HDI uint32_t f2u(float f) {
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    return u;
+}
+MCPT_HDI float u2f(uint32_t u) {
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+// A leaf record (below) holds a triangle or a sphere. A sphere's record is
+// (centre, radius), (centre at time 1, 1 if moving else 0), (0, 0, 0, tag)
+// with the tag a signalling-NaN bit pattern in the slot of a triangle's
+// normal z. Arithmetic never yields a signalling NaN (the normal of a
+// degenerate triangle is a quiet NaN), so the slot tells the two apart.
+constexpr uint32_t kSphereTag = 0x7f80abcdu;
+
+// sphere::center_at (src/path_tracer.cu:870-872)
+MCPT_HDI vec3 sphere_center(const vec3& c0, const vec3& c1, bool moving, float time) {
+    return moving ? c0 + (c1 - c0) * time : c0;
+}
+
+// hit_sphere (src/path_tracer.cu:886-911): the chord around the ray's
+// closest approach to the centre; the near root, or the far one when the
+// near one is behind (the ray starts inside)
+MCPT_HDI bool hit_sphere(const vec3& c, float r, const vec3& o, const vec3& d, float& tout) {
+    const vec3 oc = c - o;
+    const float tc = oc.dot(d) / d.dot(d);
+    const vec3 cp = o + d * tc;
+    const float d2 = (c - cp).length_squared();
+    const float disc = r * r - d2;
+    if (disc < 0.0f) return false;
+    const float h = __builtin_sqrtf(disc);
+    float t = tc - h;
+    if (t < 0.0001f) t = tc + h;
+    if (t < 0.0001f) return false;
+    tout = t;
+    return true;
+}
+
+// computeSphereUvFromOutwardNormal (src/path_tracer.cu:877-884)
+MCPT_HDI void sphere_uv(const vec3& n, float& u, float& v) {
+    const float cy = fminf(1.0f, fmaxf(-1.0f, -n.y));
+    const float theta = mf_acos(cy);
+    const float phi = mf_atan2(-n.z, n.x) + MCPT_PI_F;
+    u = phi / (2.0f * MCPT_PI_F);
+    v = theta / MCPT_PI_F;
+}
+
+// sphere::calculate_bounding_box (:862-869): centre +- radius, the union
+// over both ends of the motion for a moving sphere (no padding)
+MCPT_HDI void sphere_box(const vec3& c0, const vec3& c1, bool moving, float r, float b[6]) {
+    b[0] = c0.x - r;
+    b[1] = c0.y - r;
+    b[2] = c0.z - r;
+    b[3] = c0.x + r;
+    b[4] = c0.y + r;
+    b[5] = c0.z + r;
+    if (moving) {
+        b[0] = fminf(b[0], c1.x - r);
+        b[1] = fminf(b[1], c1.y - r);
+        b[2] = fminf(b[2], c1.z - r);
+        b[3] = fmaxf(b[3], c1.x + r);
+        b[4] = fmaxf(b[4], c1.y + r);
+        b[5] = fmaxf(b[5], c1.z + r);
+    }
+}
+
 // ------------------------------------------------------------ scene view
 struct EnvView {
     const float4* texels = nullptr;
@@ -205,13 +275,39 @@ struct alignas(16) LeafTri {
 };
 static_assert(sizeof(LeafTri) == 48, "leaf record is 48 bytes");
 
+MCPT_HDI bool leaf_is_sphere(const float4& c) { return f2u(c.w) == kSphereTag; }
+
+MCPT_HDI LeafTri sphere_leaf(const mcpt_sphere& sp) {
+    LeafTri L;
+    L.a = make_float4(sp.center[0], sp.center[1], sp.center[2], sp.radius);
+    L.b = make_float4(sp.center1[0], sp.center1[1], sp.center1[2], sp.moving ? 1.0f : 0.0f);
+    L.c = make_float4(0.0f, 0.0f, 0.0f, u2f(kSphereTag));
+    return L;
+}
+
+// the primitive's box from its leaf record: padded triangle box
+// (triangle::calculate_bounding_box, :651-655) or sphere box
+MCPT_HDI void leaf_box(const LeafTri& L, float b[6]) {
+    if (leaf_is_sphere(L.c)) {
+        sphere_box(vec3(L.a.x, L.a.y, L.a.z), vec3(L.b.x, L.b.y, L.b.z), L.b.w != 0.0f, L.a.w, b);
+        return;
+    }
+    const float eps = 1e-3f;
+    b[0] = fminf(L.a.x, fminf(L.a.w, L.b.z)) - eps;
+    b[1] = fminf(L.a.y, fminf(L.b.x, L.b.w)) - eps;
+    b[2] = fminf(L.a.z, fminf(L.b.y, L.c.x)) - eps;
+    b[3] = fmaxf(L.a.x, fmaxf(L.a.w, L.b.z)) + eps;
+    b[4] = fmaxf(L.a.y, fmaxf(L.b.x, L.b.w)) + eps;
+    b[5] = fmaxf(L.a.z, fmaxf(L.b.y, L.c.x)) + eps;
+}
+
 // What traversal needs of a scene (40 bytes, loaded per ray when the rays of
 // several scenes share one queue).
 struct BvhView {
     const BvhNode* nodes;     // binary tree (the reference's visit order)
     const WideNode* wide;     // four-wide quantized tree (fast orders)
     const LeafTri* leaf_tri;  // leaf (Morton) order
-    const int* leaf_prim;     // leaf -> original triangle index
+    const int* leaf_prim;     // leaf -> original triangle index, ~sphere index
     const int* tie;           // leaf -> rank in the reference's leaf-test order
 };
 
@@ -236,6 +332,8 @@ struct SceneView {
     int has_medium;
     int stack_depth;  // deepest leaf in internal levels (<= 64)
     EnvView env;
+    const mcpt_sphere* spheres;  // original order
+    int n_spheres;
     const TexView* tex;   // the scene's textures
     const float* srgb8;   // sRGB byte -> linear, 256 entries
 };
@@ -280,7 +378,7 @@ struct alignas(16) RayItem {  // 32 bytes: two 16-byte loads per ray
 struct alignas(16) ShadowItem {
     float4 om;  // origin xyz, max distance
     float4 dp;  // direction xyz, path index (int bits)
-    float4 c;   // contribution rgb, unused
+    float4 c;   // contribution rgb, the path's time
 };
 
 // ------------------------------------------------------ wave64 utilities

@@ -52,14 +52,26 @@ __device__ void block_reduce_box(float b[6], float* out) {
         for (int k = 0; k < 6; k++) out[k] = sh[k][0];
 }
 
-__global__ __launch_bounds__(kBlock) void k_bounds_partial(const float* hot, int n, float* partial) {
+// Primitive i of the build: the spheres first, then the triangles (the
+// order of finalizeScene's primitive list, :1955-1972).
+__device__ __forceinline__ void prim_box(const float* hot, const mcpt_sphere* sph, int S, int i, float b[6]) {
+    if (i < S) {
+        const mcpt_sphere& sp = sph[i];
+        sphere_box(load3(sp.center), load3(sp.center1), sp.moving != 0, sp.radius, b);
+        return;
+    }
+    float t[9];
+    for (int k = 0; k < 9; k++) t[k] = hot[(size_t)(i - S) * 9 + k];
+    tri_box(t, b);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bounds_partial(const float* hot, const mcpt_sphere* sph, int S, int n,
+                                                           float* partial) {
     float acc[6] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), -__builtin_inff(), -__builtin_inff(),
                     -__builtin_inff()};
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        float t[9];
-        for (int k = 0; k < 9; k++) t[k] = hot[(size_t)i * 9 + k];
         float b[6];
-        tri_box(t, b);
+        prim_box(hot, sph, S, i, b);
         merge6(acc, b);
     }
     block_reduce_box(acc, partial + 6 * blockIdx.x);
@@ -72,12 +84,18 @@ __global__ __launch_bounds__(kBlock) void k_bounds_final(const float* partial, i
     block_reduce_box(acc, out);
 }
 
-__global__ __launch_bounds__(kBlock) void k_morton(const float* hot, int n, const float* sb, uint32_t* morton,
-                                                   uint32_t* keys, int* vals) {
+__global__ __launch_bounds__(kBlock) void k_morton(const float* hot, const mcpt_sphere* sph, int S, int n,
+                                                   const float* sb, uint32_t* morton, uint32_t* keys, int* vals) {
     int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const float* t = hot + (size_t)i * 9;
-    vec3 c = (vec3(t[0], t[1], t[2]) + vec3(t[3], t[4], t[5]) + vec3(t[6], t[7], t[8])) / 3.0f;
+    vec3 c;
+    if (i < S) {  // sphere::calculateCentroid (:858-860)
+        const mcpt_sphere& sp = sph[i];
+        c = sp.moving ? (load3(sp.center) + load3(sp.center1)) * 0.5f : load3(sp.center);
+    } else {
+        const float* t = hot + (size_t)(i - S) * 9;
+        c = (vec3(t[0], t[1], t[2]) + vec3(t[3], t[4], t[5]) + vec3(t[6], t[7], t[8])) / 3.0f;
+    }
     // quantize (:922-925): (v - min) / (max - min) * 1023, truncated
     uint32_t qx = (uint32_t)((c.x - sb[0]) / (sb[3] - sb[0]) * 1023.0f);
     uint32_t qy = (uint32_t)((c.y - sb[1]) / (sb[4] - sb[1]) * 1023.0f);
@@ -90,12 +108,19 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float* hot, int n, cons
 
 // ------------------------------------------------------------- leaves
 // leaf records in Morton order: vertices plus the unit normal hit_triangle
-// forms from them (same operations, so the same bits)
-__global__ __launch_bounds__(kBlock) void k_leaves(const float* hot, const int* ids, int n, LeafTri* leaf_tri,
-                                                   int* leaf_prim) {
+// forms from them (same operations, so the same bits), or the sphere
+// record; leaf_prim holds the triangle index, or ~ the sphere index
+__global__ __launch_bounds__(kBlock) void k_leaves(const float* hot, const mcpt_sphere* sph, int S, const int* ids,
+                                                   int n, LeafTri* leaf_tri, int* leaf_prim) {
     int j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= n) return;
     int p = ids[j];
+    if (p < S) {
+        leaf_prim[j] = ~p;
+        leaf_tri[j] = sphere_leaf(sph[p]);
+        return;
+    }
+    p -= S;
     leaf_prim[j] = p;
     const float* t = hot + (size_t)p * 9;
     const vec3 v0(t[0], t[1], t[2]), v1(t[3], t[4], t[5]), v2(t[6], t[7], t[8]);
@@ -140,9 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* keys, int n, 
             rf[side] = ~c;
             leaf_parent[c] = (idx << 1) | side;
             float b[6];
-            const LeafTri L = leaf_tri[c];
-            const float tt[9] = {L.a.x, L.a.y, L.a.z, L.a.w, L.b.x, L.b.y, L.b.z, L.b.w, L.c.x};
-            tri_box(tt, b);
+            leaf_box(leaf_tri[c], b);
             store_slot(nodes, idx, side, b);
         } else {
             rf[side] = c;
@@ -203,7 +226,8 @@ void to_aabb(const float* b, mcpt_aabb* o) {
 int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
     Ctx* ctx = s->ctx;
     hipStream_t st = ctx->stream;
-    const int n = s->n;
+    const int S = s->n_spheres;
+    const int n = s->n + S;  // primitives: spheres, then triangles
     if (n < 1) return set_error(MCPT_ERR_INVALID, "empty scene");
     hipEvent_t e0, e1;
     MCPT_HIP(hipEventCreate(&e0));
@@ -269,9 +293,10 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
     STEP(depth.alloc(ctx, 66));  // max depth, then the 65 level offsets
 
     // 1-2. bounds and Morton codes
-    k_bounds_partial<<<n_partial, kBlock, 0, st>>>(s->hot.ptr, n, partial.ptr);
+    k_bounds_partial<<<n_partial, kBlock, 0, st>>>(s->hot.ptr, s->spheres.ptr, S, n, partial.ptr);
     k_bounds_final<<<1, kBlock, 0, st>>>(partial.ptr, n_partial, sb.ptr);
-    k_morton<<<blocks_for(n), kBlock, 0, st>>>(s->hot.ptr, n, sb.ptr, morton.ptr, keys_a.ptr, vals_a.ptr);
+    k_morton<<<blocks_for(n), kBlock, 0, st>>>(s->hot.ptr, s->spheres.ptr, S, n, sb.ptr, morton.ptr, keys_a.ptr,
+                                               vals_a.ptr);
     HSTEP(hipGetLastError());
     // 3. stable radix sort, 8 bits per pass over the 30 code bits
     uint32_t* kin = keys_a.ptr;
@@ -287,22 +312,18 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
     }
     HSTEP(hipGetLastError());
     // 4. leaves and the Karras tree
-    k_leaves<<<blocks_for(n), kBlock, 0, st>>>(s->hot.ptr, vin, n, s->leaf_tri.ptr, s->leaf_prim.ptr);
+    k_leaves<<<blocks_for(n), kBlock, 0, st>>>(s->hot.ptr, s->spheres.ptr, S, vin, n, s->leaf_tri.ptr,
+                                               s->leaf_prim.ptr);
     k_fill_int<<<1, 64, 0, st>>>(depth.ptr, 1, 0);
     if (n == 1) {
         // One primitive: no internal node exists in the reference. The root
         // here holds leaf 0 on both sides, so traversal tests it once per
         // side and keeps the first (strict '<').
         HSTEP(hipStreamSynchronize(st));
-        float t[9], b[6];
-        HSTEP(hipMemcpy(t, s->hot.ptr, sizeof(t), hipMemcpyDeviceToHost));
-        const float eps = 1e-3f;
-        b[0] = fminf(t[0], fminf(t[3], t[6])) - eps;
-        b[1] = fminf(t[1], fminf(t[4], t[7])) - eps;
-        b[2] = fminf(t[2], fminf(t[5], t[8])) - eps;
-        b[3] = fmaxf(t[0], fmaxf(t[3], t[6])) + eps;
-        b[4] = fmaxf(t[1], fmaxf(t[4], t[7])) + eps;
-        b[5] = fmaxf(t[2], fmaxf(t[5], t[8])) + eps;
+        LeafTri L;
+        float b[6];
+        HSTEP(hipMemcpy(&L, s->leaf_tri.ptr, sizeof(L), hipMemcpyDeviceToHost));
+        leaf_box(L, b);
         BvhNode root;
         std::memset(&root, 0, sizeof(root));
         float* f = reinterpret_cast<float*>(&root);

@@ -132,8 +132,9 @@ __device__ __forceinline__ mcpt_packed_hit pack_hit(const BvhView& bv, const Tra
     mcpt_packed_hit h;
     h.t = s.t;
     if (s.leaf >= 0) {
-        h.prim_type = 1;
-        h.prim_index = bv.leaf_prim[s.leaf];
+        const int p = bv.leaf_prim[s.leaf];  // triangle, or ~sphere
+        h.prim_type = p >= 0 ? 1 : 0;
+        h.prim_index = p >= 0 ? p : ~p;
         h.did_hit = 1;
     } else {
         h.prim_type = -1;
@@ -154,6 +155,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneView sv, Workspace ws, i
         const RayItem it = ws.q[cur][slot];
         Trav s;
         trav_begin<REF>(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+        s.time = it.ot.w;
         trace_closest<COUNT, REF>(sv.bvh, s, lds_stack + threadIdx.x, kBlock, ns, nl);
         ws.hits[slot] = pack_hit(sv.bvh, s);
     }
@@ -193,6 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Work
                     if (q < total) {
                         const RayItem it = ws.q[cur][q];
                         trav_begin<REF>(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+                        s.time = it.ot.w;
                         slot = q;
                         has = true;
                     }
@@ -252,7 +255,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, Workspace ws, in
         Rng rng;
         rng.g.state = ws.rng[path];
         rng.g.inc = ((uint64_t)(uint32_t)(path % npix) << 1u) | 1u;  // the stream of the path's pixel
-        shade_bounce(sv, bg, rr_cap, depth, ro, rd, hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out);
+        const int prim = hit.prim_type == 0 ? ~hit.prim_index : hit.prim_index;
+        shade_bounce(sv, bg, rr_cap, depth, ro, rd, hit.did_hit != 0, hit.t, prim, ps, rng, out, time);
         ws.thr[path] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
         ws.smp[path] = make_float4(ps.smp.x, ps.smp.y, ps.smp.z, ps.prev_spec ? 1.0f : 0.0f);
         ws.rng[path] = rng.g.state;
@@ -271,7 +275,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, Workspace ws, in
         ShadowItem si;
         si.om = make_float4(out.so.x, out.so.y, out.so.z, out.smax);
         si.dp = make_float4(out.sd.x, out.sd.y, out.sd.z, __int_as_float(path));
-        si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, 0.0f);
+        si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, time);
         ws.sq[sslot] = si;
     }
 }
@@ -285,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneView sv, Workspace ws, i
     const ShadowItem si = ws.sq[slot];
     const int path = __float_as_int(si.dp.w);
     const float tr = shadow_transmittance(sv, vec3(si.om.x, si.om.y, si.om.z), vec3(si.dp.x, si.dp.y, si.dp.z),
-                                          si.om.w, lds_stack + threadIdx.x, kBlock);
+                                          si.om.w, lds_stack + threadIdx.x, kBlock, si.c.w);
     if (tr > 0.0f) {
         float4 s = ws.smp[path];
         s.x = s.x + si.c.x * tr;
@@ -324,6 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow_persistent(SceneView sv, Work
                     if (q < total) {
                         const ShadowItem si = ws.sq[q];
                         trav_begin(s, vec3(si.om.x, si.om.y, si.om.z), vec3(si.dp.x, si.dp.y, si.dp.z), si.om.w);
+                        s.time = si.c.w;
                         slot = q;
                         has = true;
                     }
@@ -396,15 +401,16 @@ __global__ __launch_bounds__(kBlock) void k_megakernel(SceneView sv, mcpt_camera
         for (int depth = 0; depth < max_depth; depth++) {
             Trav s;
             trav_begin(s, ro, rd, 1e30f);
+            s.time = time;
             uint32_t ns = 0, nl = 0;
             trace_closest<false>(sv.bvh, s, stk, kBlock, ns, nl);
             n_ext++;
             const int prim = s.leaf >= 0 ? sv.bvh.leaf_prim[s.leaf] : -1;
             Bounce out;
-            shade_bounce(sv, bg, cam.rr_cap, depth, ro, rd, s.leaf >= 0, s.t, prim, ps, rng, out);
+            shade_bounce(sv, bg, cam.rr_cap, depth, ro, rd, s.leaf >= 0, s.t, prim, ps, rng, out, time);
             if (out.shadow) {
                 n_shd++;
-                const float tr = shadow_transmittance(sv, out.so, out.sd, out.smax, stk, kBlock);
+                const float tr = shadow_transmittance(sv, out.so, out.sd, out.smax, stk, kBlock, time);
                 if (tr > 0.0f) ps.smp = ps.smp + out.contrib * tr;
             }
             if (!out.next) break;
@@ -496,6 +502,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const mc
     const mcpt_ray r = rays[i];
     Trav s;
     trav_begin<REF>(s, load3(r.origin), load3(r.dir), r.tmax);
+    s.time = r.time;
     uint32_t ns = 0, nl = 0;
     trace_closest<COUNT, REF>(sv.bvh, s, lds_stack + threadIdx.x, kBlock, ns, nl);
     hits[i] = pack_hit(sv.bvh, s);
@@ -511,6 +518,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(SceneView sv, const mcpt_r
     const mcpt_ray r = rays[i];
     Trav s;
     trav_begin(s, load3(r.origin), load3(r.dir), r.tmax);
+    s.time = r.time;
     occ[i] = trace_any(sv.bvh, s, lds_stack + threadIdx.x, kBlock) ? 1 : 0;
 }
 

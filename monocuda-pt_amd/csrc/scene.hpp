@@ -13,13 +13,15 @@ struct mcpt_ctx_s {
 
 struct mcpt_scene_s {
     mcpt::Ctx* ctx = nullptr;
-    int n = 0;
+    int n = 0;          // triangles
+    int n_spheres = 0;  // the BVH's primitives are the spheres, then the triangles
     int n_mats = 0;
     bool has_medium = false;
     // inputs (original triangle order)
     mcpt::DevBuf<float> hot;  // 9 floats per triangle
     mcpt::DevBuf<mcpt_tri_cold> cold;
     mcpt::DevBuf<mcpt_material> mats;
+    mcpt::DevBuf<mcpt_sphere> spheres;
     mcpt::DevBuf<mcpt_light> lights;
     std::vector<mcpt_light> host_lights;
     // environment
@@ -76,6 +78,8 @@ struct mcpt_scene_s {
         v.env.rot = env_rot;
         v.env.total = env_total;
         v.env.enabled = env_enabled ? 1 : 0;
+        v.spheres = spheres.ptr;
+        v.n_spheres = n_spheres;
         v.tex = tex.ptr;
         v.srgb8 = srgb8.ptr;
         return v;
@@ -84,6 +88,7 @@ struct mcpt_scene_s {
         hot.release(ctx);
         cold.release(ctx);
         mats.release(ctx);
+        spheres.release(ctx);
         lights.release(ctx);
         env_texels.release(ctx);
         env_row_cdf.release(ctx);
@@ -100,8 +105,11 @@ struct mcpt_scene_s {
         tex_host.clear();
         srgb8.release(ctx);
     }
+    int n_prims() const { return n + n_spheres; }
     // every texture a material names has been added
     int check_textures() const;
+    // the paths that take triangle-only scenes refuse this one
+    int check_no_spheres(const char* what) const;
 };
 
 namespace mcpt {

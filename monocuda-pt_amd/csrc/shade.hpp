@@ -168,9 +168,21 @@ struct HitInfo {
     bool front;
     float u, v;
 };
-// the fields testPrimitiveHit fills (:1511-1533), rebuilt from the 16-byte
-// hit record as shadeKernel does (:3508-3535)
-MCPT_DEV void rebuild_hit(const SceneView& sv, int prim, const vec3& o, const vec3& d, float t, HitInfo& h) {
+// the fields testPrimitiveHit fills (:1490-1533), rebuilt from the 16-byte
+// hit record as shadeKernel does (:3508-3535). prim >= 0 is a triangle,
+// prim < 0 the sphere ~prim, whose centre is taken at the ray's time.
+MCPT_DEV void rebuild_hit(const SceneView& sv, int prim, const vec3& o, const vec3& d, float t, float time,
+                          HitInfo& h) {
+    if (prim < 0) {
+        const mcpt_sphere& sp = sv.spheres[~prim];
+        const vec3 c = sphere_center(load3(sp.center), load3(sp.center1), sp.moving != 0, time);
+        h.p = o + d * t;
+        const vec3 on = (h.p - c).normalize();
+        h.front = d.dot(on) < 0.0f;
+        h.normal = h.front ? on : -on;
+        sphere_uv(on, h.u, h.v);
+        return;
+    }
     vec3 v0, v1, v2;
     load_tri(sv.hot, prim, v0, v1, v2);
     const mcpt_tri_cold& c = sv.cold[prim];
@@ -201,6 +213,11 @@ MCPT_DEV void rebuild_hit(const SceneView& sv, int prim, const vec3& o, const ve
     h.v = ((c.uv0[1] * w0) + (c.uv1[1] * w1)) + (c.uv2[1] * w2);
 }
 
+// the material of a primitive in the hit encoding above
+MCPT_DEV int prim_material(const SceneView& sv, int prim) {
+    return prim >= 0 ? sv.cold[prim].material : sv.spheres[~prim].material;
+}
+
 MCPT_DEV float tri_area(const float* hot, int prim) {  // :837-839
     vec3 v0, v1, v2;
     load_tri(hot, prim, v0, v1, v2);
@@ -209,7 +226,8 @@ MCPT_DEV float tri_area(const float* hot, int prim) {  // :837-839
 
 // mediumTransmittance (:2874-2898), homogeneous media: closest hits along
 // the segment, exp(-sigma d) for each stretch that ends on a back face.
-MCPT_DEV float medium_transmittance(const SceneView& sv, vec3 o, const vec3& d, float maxd, int* stk, int stride) {
+MCPT_DEV float medium_transmittance(const SceneView& sv, vec3 o, const vec3& d, float maxd, int* stk, int stride,
+                                    float time) {
     float tr = 1.0f;
     float travelled = 0.0f;
     for (int k = 0; k < 4; k++) {
@@ -217,16 +235,24 @@ MCPT_DEV float medium_transmittance(const SceneView& sv, vec3 o, const vec3& d, 
         if (remaining <= 0.0f) break;
         Trav s;
         trav_begin(s, o, d, remaining);
+        s.time = time;
         uint32_t ns = 0, nl = 0;
         trace_closest<false>(sv.bvh, s, stk, stride, ns, nl);
         if (s.leaf < 0) break;
         int prim = sv.bvh.leaf_prim[s.leaf];
-        const mcpt_material& m = sv.mats[sv.cold[prim].material];
+        const mcpt_material& m = sv.mats[prim_material(sv, prim)];
         if (m.type != MCPT_MAT_ISOTROPIC) return 0.0f;
-        vec3 v0, v1, v2;
-        load_tri(sv.hot, prim, v0, v1, v2);
-        vec3 n = (v1 - v0).cross(v2 - v0).normalize();
-        bool front = d.dot(n) < 0.0f;
+        bool front;
+        if (prim < 0) {  // the sphere's outward normal at the hit
+            const mcpt_sphere& sp = sv.spheres[~prim];
+            const vec3 c = sphere_center(load3(sp.center), load3(sp.center1), sp.moving != 0, time);
+            front = d.dot(((o + d * s.t) - c).normalize()) < 0.0f;
+        } else {
+            vec3 v0, v1, v2;
+            load_tri(sv.hot, prim, v0, v1, v2);
+            vec3 n = (v1 - v0).cross(v2 - v0).normalize();
+            front = d.dot(n) < 0.0f;
+        }
         if (!front) tr = tr * mf_exp(-m.density * s.t);
         travelled = travelled + s.t;
         o = o + d * s.t;
@@ -237,10 +263,11 @@ MCPT_DEV float medium_transmittance(const SceneView& sv, vec3 o, const vec3& d, 
 // any-hit visibility, or transmittance when the scene holds a medium
 // (shadowKernel, :3698-3728)
 MCPT_DEV float shadow_transmittance(const SceneView& sv, const vec3& o, const vec3& d, float maxd, int* stk,
-                                    int stride) {
-    if (sv.has_medium) return medium_transmittance(sv, o, d, maxd, stk, stride);
+                                    int stride, float time = 0.0f) {
+    if (sv.has_medium) return medium_transmittance(sv, o, d, maxd, stk, stride, time);
     Trav s;
     trav_begin(s, o, d, maxd);
+    s.time = time;
     return trace_any(sv.bvh, s, stk, stride) ? 0.0f : 1.0f;
 }
 
@@ -265,7 +292,8 @@ struct Bounce {
 // emission terms, free flight in a medium, albedo, next-event estimation,
 // Russian roulette and the scatter (:2917-3048, :3486-3693).
 MCPT_DEV void shade_bounce(const SceneView& sv, const vec3& bg, float rr_cap, int depth, const vec3& ro,
-                           const vec3& rd, bool did_hit, float t, int prim, PathState& ps, Rng& rng, Bounce& out) {
+                           const vec3& rd, bool did_hit, float t, int prim, PathState& ps, Rng& rng, Bounce& out,
+                           float time = 0.0f) {
     out.next = false;
     out.shadow = false;
     const int nl = sv.n_lights;
@@ -276,12 +304,12 @@ MCPT_DEV void shade_bounce(const SceneView& sv, const vec3& bg, float rr_cap, in
         ps.smp = ps.smp + ps.thr * env_radiance(sv.env, rd, bg) * w;
         return;
     }
-    const mcpt_material m = sv.mats[sv.cold[prim].material];
+    const mcpt_material m = sv.mats[prim_material(sv, prim)];
     HitInfo h;
-    rebuild_hit(sv, prim, ro, rd, t, h);
+    rebuild_hit(sv, prim, ro, rd, t, time, h);
     if (m.type == MCPT_MAT_EMISSIVE) {
         float ew = 1.0f;
-        if (!(depth == 0 || ps.prev_spec)) {
+        if (!(depth == 0 || ps.prev_spec) && prim >= 0) {  // spheres are not on the light list
             float cl = fmaxf(fabsf(rd.dot(h.normal)), 1e-4f);
             float area = tri_area(sv.hot, prim);
             float lp = ((1.0f - env_select_prob(sv.env, nl)) / float(nl)) * (t * t) / (cl * area);

@@ -29,6 +29,7 @@ struct Trav {
     vec3 o, d, inv;
     vec3 oi;   // o * inv, for the fused box test of the fast orders
     float t;   // running closest distance (or the any-hit limit)
+    float time;  // the ray's time (where moving spheres are)
     int leaf;  // closest leaf so far, -1 = none
     int node;
     int sp;
@@ -53,6 +54,7 @@ MCPT_DEV void trav_begin(Trav& s, const vec3& o, const vec3& d, float tmax) {
                      fminf(fmaxf(s.inv.z, -1e30f), 1e30f));
     s.oi = vec3(o.x * s.inv.x, o.y * s.inv.y, o.z * s.inv.z);
     s.t = tmax;
+    s.time = 0.0f;
     s.leaf = -1;
     s.node = 0;
     s.sp = 0;
@@ -67,21 +69,24 @@ MCPT_DEV void load_tri(const float* base, int i, vec3& v0, vec3& v1, vec3& v2) {
     v2 = vec3(t[6], t[7], t[8]);
 }
 
-MCPT_DEV void load_leaf(const LeafTri* base, int i, vec3& v0, vec3& v1, vec3& v2, vec3& n) {
+// The ray against leaf i's primitive: hit_triangle with the stored normal,
+// or hit_sphere at the ray's time (the primitive-type switch of
+// testPrimitiveClosest / doesPrimitiveBlockRay, :1537-1563).
+MCPT_DEV bool leaf_hit(const LeafTri* base, int i, const Trav& s, float& t) {
     const LeafTri* p = base + i;
     const float4 a = p->a, b = p->b, c = p->c;
-    v0 = vec3(a.x, a.y, a.z);
-    v1 = vec3(a.w, b.x, b.y);
-    v2 = vec3(b.z, b.w, c.x);
-    n = vec3(c.y, c.z, c.w);
+    if (leaf_is_sphere(c)) {
+        const vec3 c0(a.x, a.y, a.z);
+        return hit_sphere(sphere_center(c0, vec3(b.x, b.y, b.z), b.w != 0.0f, s.time), a.w, s.o, s.d, t);
+    }
+    return hit_triangle_n(vec3(a.x, a.y, a.z), vec3(a.w, b.x, b.y), vec3(b.z, b.w, c.x), vec3(c.y, c.z, c.w), s.o,
+                          s.d, t);
 }
 
 // testPrimitiveClosest (src/path_tracer.cu:1537-1551)
 MCPT_DEV void test_leaf_closest(const BvhView& bv, Trav& s, int leaf) {
-    vec3 v0, v1, v2, n;
-    load_leaf(bv.leaf_tri, leaf, v0, v1, v2, n);
     float t;
-    if (hit_triangle_n(v0, v1, v2, n, s.o, s.d, t) && t < s.t && t > 0.0001f) {
+    if (leaf_hit(bv.leaf_tri, leaf, s, t) && t < s.t && t > 0.0001f) {
         s.t = t;
         s.leaf = leaf;
     }
@@ -89,10 +94,8 @@ MCPT_DEV void test_leaf_closest(const BvhView& bv, Trav& s, int leaf) {
 
 // doesPrimitiveBlockRay (src/path_tracer.cu:1553-1563)
 MCPT_DEV bool test_leaf_any(const BvhView& bv, const Trav& s, int leaf) {
-    vec3 v0, v1, v2, n;
-    load_leaf(bv.leaf_tri, leaf, v0, v1, v2, n);
     float t;
-    return hit_triangle_n(v0, v1, v2, n, s.o, s.d, t) && t > 0.0001f && t < s.t;
+    return leaf_hit(bv.leaf_tri, leaf, s, t) && t > 0.0001f && t < s.t;
 }
 
 // One loop iteration; returns true when the traversal is finished. The up
@@ -189,10 +192,8 @@ MCPT_DEV uint32_t wide_boxes(const float4& o, const uint4& q0, const uint4& q1, 
 }
 
 MCPT_DEV void test_leaf_ranked(const BvhView& bv, Trav& s, int leaf) {
-    vec3 v0, v1, v2, n;
-    load_leaf(bv.leaf_tri, leaf, v0, v1, v2, n);
     float t;
-    if (hit_triangle_n(v0, v1, v2, n, s.o, s.d, t) && t > 0.0001f) {
+    if (leaf_hit(bv.leaf_tri, leaf, s, t) && t > 0.0001f) {
         if (t < s.t || (t == s.t && s.leaf >= 0 && bv.tie[leaf] < bv.tie[s.leaf])) {
             s.t = t;
             s.leaf = leaf;

@@ -117,6 +117,53 @@ bool hit_triangle(const mcpt_tri_hot& tri, const V3& o, const V3& d, float& tout
     return true;
 }
 
+// sphere::calculate_bounding_box (src/path_tracer.cu:862-869): centre +-
+// radius, merged over both ends of the motion for a moving sphere
+AABB sphere_box(const mcpt_sphere& sp) {
+    const float r = sp.radius;
+    AABB b;
+    b.mn = V3(sp.center[0] - r, sp.center[1] - r, sp.center[2] - r);
+    b.mx = V3(sp.center[0] + r, sp.center[1] + r, sp.center[2] + r);
+    if (sp.moving) {
+        AABB b1;
+        b1.mn = V3(sp.center1[0] - r, sp.center1[1] - r, sp.center1[2] - r);
+        b1.mx = V3(sp.center1[0] + r, sp.center1[1] + r, sp.center1[2] + r);
+        b = merge(b, b1);
+    }
+    return b;
+}
+
+// sphere::center_at (src/path_tracer.cu:870-872)
+V3 sphere_center_at(const mcpt_sphere& sp, float time) {
+    return sp.moving ? v3(sp.center) + (v3(sp.center1) - v3(sp.center)) * time : v3(sp.center);
+}
+
+// hit_sphere (src/path_tracer.cu:886-911)
+bool hit_sphere(const mcpt_sphere& sp, const V3& o, const V3& d, float time, float& tout) {
+    V3 c = sphere_center_at(sp, time);
+    V3 to_center = c - o;
+    float along = to_center.dot(d) / d.dot(d);
+    V3 closest = o + d * along;
+    float dist2 = (c - closest).length_squared();
+    float disc = sp.radius * sp.radius - dist2;
+    if (disc < 0.0f) return false;
+    float half = sqrtf(disc);
+    float t = along - half;
+    if (t < 0.0001f) t = along + half;
+    if (t < 0.0001f) return false;
+    tout = t;
+    return true;
+}
+
+// computeSphereUvFromOutwardNormal (src/path_tracer.cu:877-884)
+void sphere_uv(const V3& n, float& u, float& v) {
+    float ny = fminf(1.0f, fmaxf(-1.0f, -n.y));
+    float theta = mf_acos(ny);
+    float phi = mf_atan2(-n.z, n.x) + MCPT_PI_F;
+    u = phi / (2.0f * MCPT_PI_F);
+    v = theta / MCPT_PI_F;
+}
+
 // Morton (src/path_tracer.cu:913-935)
 uint32_t expand_bits(uint32_t p) {
     uint32_t e = 0;
@@ -161,6 +208,7 @@ struct Tex {  // a checker (linear colours) or an RGBA8 image
 }  // namespace
 
 struct oracle_scene_s {
+    std::vector<mcpt_sphere> spheres;
     std::vector<mcpt_tri_hot> hot;
     std::vector<mcpt_tri_cold> cold;
     std::vector<mcpt_material> mats;
@@ -168,7 +216,9 @@ struct oracle_scene_s {
     Env env;
     std::vector<Tex> tex;
     float srgb8[256];  // sRGB byte -> linear
-    // BVH
+    // BVH over the primitive list of finalizeScene (:1955-1972): spheres
+    // 0..S-1, then triangles; leaf_prim holds a triangle index, or ~ a
+    // sphere index
     int n = 0;
     std::vector<uint32_t> morton, keys;
     std::vector<int32_t> ids;
@@ -186,15 +236,22 @@ namespace {
 // ---------------------------------------------------------------- traversal
 struct Closest {
     float t;
-    int prim;
+    int prim;  // triangle index, or ~ sphere index
+    bool hit;
 };
+
+// the primitive-type switch of testPrimitiveClosest (src/path_tracer.cu:1537-1551)
+bool prim_hit(const oracle_scene_s& s, int p, const V3& o, const V3& d, float time, float& t) {
+    if (p < 0) return hit_sphere(s.spheres[~p], o, d, time, t);
+    return hit_triangle(s.hot[p], o, d, t);
+}
 
 // BVH::traverseRay / stepTraversal / testPrimitiveClosest
 // (src/path_tracer.cu:1092-1150, :1262-1321, :1537-1551). Returns the closest
 // candidate; counts internal-node iterations and leaf tests.
-Closest traverse_closest(const oracle_scene_s& s, const V3& o, const V3& d, float tmax, uint32_t* steps,
-                         uint32_t* leaves) {
-    Closest c{tmax, -1};
+Closest traverse_closest(const oracle_scene_s& s, const V3& o, const V3& d, float tmax, float time,
+                         uint32_t* steps, uint32_t* leaves) {
+    Closest c{tmax, 0, false};
     int stack[64];
     int sp = 0;
     int node = 0;
@@ -212,13 +269,13 @@ Closest traverse_closest(const oracle_scene_s& s, const V3& o, const V3& d, floa
             nl++;
             int p = s.leaf_prim[nd.left];
             float t;
-            if (hit_triangle(s.hot[p], o, d, t) && t < c.t && t > 0.0001f) c = Closest{t, p};
+            if (prim_hit(s, p, o, d, time, t) && t < c.t && t > 0.0001f) c = Closest{t, p, true};
         }
         if (oR && nd.right_leaf) {
             nl++;
             int p = s.leaf_prim[nd.right];
             float t;
-            if (hit_triangle(s.hot[p], o, d, t) && t < c.t && t > 0.0001f) c = Closest{t, p};
+            if (prim_hit(s, p, o, d, time, t) && t < c.t && t > 0.0001f) c = Closest{t, p, true};
         }
         bool tL = oL && !nd.left_leaf;
         bool tR = oR && !nd.right_leaf;
@@ -236,7 +293,7 @@ Closest traverse_closest(const oracle_scene_s& s, const V3& o, const V3& d, floa
 }
 
 // BVH::traverseAnyHit + doesPrimitiveBlockRay (src/path_tracer.cu:1152-1207, :1553-1563)
-bool traverse_any(const oracle_scene_s& s, const V3& o, const V3& d, float tmax) {
+bool traverse_any(const oracle_scene_s& s, const V3& o, const V3& d, float tmax, float time) {
     int stack[64];
     int sp = 0;
     int node = 0;
@@ -249,9 +306,9 @@ bool traverse_any(const oracle_scene_s& s, const V3& o, const V3& d, float tmax)
         float nR = 0.0001f, fR = tmax;
         bool oR = hit_aabb(br, o, d, nR, fR);
         float t;
-        if (oL && nd.left_leaf && hit_triangle(s.hot[s.leaf_prim[nd.left]], o, d, t) && t > 0.0001f && t < tmax)
+        if (oL && nd.left_leaf && prim_hit(s, s.leaf_prim[nd.left], o, d, time, t) && t > 0.0001f && t < tmax)
             return true;
-        if (oR && nd.right_leaf && hit_triangle(s.hot[s.leaf_prim[nd.right]], o, d, t) && t > 0.0001f && t < tmax)
+        if (oR && nd.right_leaf && prim_hit(s, s.leaf_prim[nd.right], o, d, time, t) && t > 0.0001f && t < tmax)
             return true;
         bool tL = oL && !nd.left_leaf;
         bool tR = oR && !nd.right_leaf;
@@ -272,7 +329,7 @@ struct Hit {
     float t = 1e30f;
     V3 p, normal;
     bool front = true;
-    int prim = -1;
+    int prim = 0;  // triangle index, or ~ sphere index
     float u = 0, v = 0;
     const mcpt_material* mat = nullptr;
 };
@@ -306,11 +363,24 @@ V3 tri_shading_normal(const mcpt_tri_hot& h, const mcpt_tri_cold& c, const V3& p
     return sn;
 }
 
-// the hitRecord fields testPrimitiveHit fills (src/path_tracer.cu:1511-1533)
-Hit make_hit(const oracle_scene_s& s, const V3& o, const V3& d, const Closest& c) {
+// the hitRecord fields testPrimitiveHit fills (src/path_tracer.cu:1490-1533)
+Hit make_hit(const oracle_scene_s& s, const V3& o, const V3& d, float time, const Closest& c) {
     Hit h;
-    if (c.prim < 0) {
+    if (!c.hit) {
         h.t = c.t;
+        return h;
+    }
+    if (c.prim < 0) {
+        const mcpt_sphere& sp = s.spheres[~c.prim];
+        h.did = true;
+        h.t = c.t;
+        h.prim = c.prim;
+        h.p = o + d * c.t;
+        h.mat = &s.mats[sp.material];
+        V3 outward = (h.p - sphere_center_at(sp, time)).normalize();
+        h.front = d.dot(outward) < 0.0f;
+        h.normal = h.front ? outward : -outward;
+        sphere_uv(outward, h.u, h.v);
         return h;
     }
     const mcpt_tri_hot& th = s.hot[c.prim];
@@ -328,9 +398,10 @@ Hit make_hit(const oracle_scene_s& s, const V3& o, const V3& d, const Closest& c
     return h;
 }
 
-Hit trace_ray(const oracle_scene_s& s, const V3& o, const V3& d, float tmax, uint32_t* steps, uint32_t* leaves) {
-    Closest c = traverse_closest(s, o, d, tmax, steps, leaves);
-    return make_hit(s, o, d, c);
+Hit trace_ray(const oracle_scene_s& s, const V3& o, const V3& d, float tmax, float time, uint32_t* steps,
+              uint32_t* leaves) {
+    Closest c = traverse_closest(s, o, d, tmax, time, steps, leaves);
+    return make_hit(s, o, d, time, c);
 }
 
 // ---------------------------------------------------------------- sampling
@@ -543,14 +614,14 @@ V3 albedo_at(const oracle_scene_s& s, const mcpt_material& m, float u, float v) 
 }
 
 // mediumTransmittance (src/path_tracer.cu:2874-2898), homogeneous media
-float medium_transmittance(const oracle_scene_s& s, V3 o, const V3& d, float maxd, PathCounts& pc) {
+float medium_transmittance(const oracle_scene_s& s, V3 o, const V3& d, float maxd, float time, PathCounts& pc) {
     float tr = 1.0f;
     float travelled = 0.0f;
     for (int k = 0; k < 4; k++) {
         float remaining = maxd - travelled;
         if (remaining <= 0.0f) break;
         pc.shd++;
-        Hit b = trace_ray(s, o, d, remaining, nullptr, nullptr);
+        Hit b = trace_ray(s, o, d, remaining, time, nullptr, nullptr);
         if (!b.did) break;
         if (b.mat->type != MCPT_MAT_ISOTROPIC) return 0.0f;
         if (!b.front) tr = tr * mf_exp(-b.mat->density * b.t);
@@ -578,7 +649,7 @@ V3 trace_path(const oracle_scene_s& s, const mcpt_camera& cam, int px, int py, R
     lens = lens * cam.lens_radius;
     V3 off = v3(cam.u) * lens.x + v3(cam.v) * lens.y;
     V3 ro = v3(cam.center) + off;
-    (void)rng.u();  // ray time
+    const float time = rng.u();  // the path's time (moving spheres)
     V3 rd = (pc_ - ro).normalize();
 
     const int nl = (int)s.lights.size();
@@ -590,7 +661,7 @@ V3 trace_path(const oracle_scene_s& s, const mcpt_camera& cam, int px, int py, R
     for (int depth = 0; depth < max_depth; depth++) {
         pc.ext++;
         uint32_t st = 0, lf = 0;
-        Hit hit = trace_ray(s, ro, rd, 1e30f, &st, &lf);
+        Hit hit = trace_ray(s, ro, rd, 1e30f, time, &st, &lf);
         pc.steps += st;
         pc.leaves += lf;
         if (!hit.did) {
@@ -603,7 +674,7 @@ V3 trace_path(const oracle_scene_s& s, const mcpt_camera& cam, int px, int py, R
         const mcpt_material& m = *hit.mat;
         float ew = 1.0f;
         bool from_bsdf = !(depth == 0 || prev_spec);
-        if (from_bsdf && m.type == MCPT_MAT_EMISSIVE) {
+        if (from_bsdf && m.type == MCPT_MAT_EMISSIVE && hit.prim >= 0) {  // triangles only (:2930)
             float cl = fmaxf(fabsf(rd.dot(hit.normal)), 1e-4f);
             float area = tri_area(s.hot[hit.prim]);
             float lp = ((1.0f - env_select_prob(s.env, nl)) / float(nl)) * (hit.t * hit.t) / (cl * area);
@@ -651,7 +722,7 @@ V3 trace_path(const oracle_scene_s& s, const mcpt_camera& cam, int px, int py, R
             bool iso = (m.type == MCPT_MAT_ISOTROPIC);
             float cs = iso ? 1.0f : hit.normal.dot(dl);
             if (cs > 0.0f && sapdf > 0.0f) {
-                float tr = medium_transmittance(s, so, dl, dist * 0.999f, pc);
+                float tr = medium_transmittance(s, so, dl, dist * 0.999f, time, pc);
                 if (tr > 0.0f) {
                     float php = hg_pdf(rd.dot(dl), m.phase_g);
                     float spdf = iso ? php : lambert_pdf(cs);
@@ -867,17 +938,23 @@ int oracle_env_probe(const float* rgba, int w, int h, float rot, int fn, const f
 uint32_t oracle_expand_bits(uint32_t v) { return expand_bits(v); }
 uint32_t oracle_quantize(float v, float lo, float hi) { return quantize(v, lo, hi); }
 
-void oracle_finalize(const mcpt_tri_hot* hot, int n, mcpt_aabb* tri_bounds, mcpt_aabb* scene_bounds, float* centroids,
-                     uint32_t* morton) {
-    // finalizeScene (src/path_tracer.cu:1930-1972), triangles only
+// finalizeScene (src/path_tracer.cu:1930-1972) over the primitive list:
+// spheres 0..S-1, then the triangles
+static void finalize_prims(const mcpt_tri_hot* hot, int T, const mcpt_sphere* sph, int S, mcpt_aabb* tri_bounds,
+                           mcpt_aabb* scene_bounds, float* centroids, uint32_t* morton) {
+    const int n = S + T;
     std::vector<AABB> boxes(n);
     AABB sb;
     for (int i = 0; i < n; i++) {
-        boxes[i] = tri_box(hot[i]);
+        boxes[i] = i < S ? sphere_box(sph[i]) : tri_box(hot[i - S]);
         sb = (i == 0) ? boxes[i] : merge(sb, boxes[i]);
     }
     for (int i = 0; i < n; i++) {
-        V3 c = (v3(hot[i].v0) + v3(hot[i].v1) + v3(hot[i].v2)) / 3.;
+        V3 c;
+        if (i < S)  // sphere::calculateCentroid (:858-860)
+            c = sph[i].moving ? (v3(sph[i].center) + v3(sph[i].center1)) * 0.5f : v3(sph[i].center);
+        else
+            c = (v3(hot[i - S].v0) + v3(hot[i - S].v1) + v3(hot[i - S].v2)) / 3.;
         uint32_t qx = quantize(c.x, sb.mn.x, sb.mx.x);
         uint32_t qy = quantize(c.y, sb.mn.y, sb.mx.y);
         uint32_t qz = quantize(c.z, sb.mn.z, sb.mx.z);
@@ -907,6 +984,11 @@ void oracle_finalize(const mcpt_tri_hot* hot, int n, mcpt_aabb* tri_bounds, mcpt
         scene_bounds->max[2] = sb.mx.z;
         scene_bounds->pad0 = scene_bounds->pad1 = 0.f;
     }
+}
+
+void oracle_finalize(const mcpt_tri_hot* hot, int n, mcpt_aabb* tri_bounds, mcpt_aabb* scene_bounds, float* centroids,
+                     uint32_t* morton) {
+    finalize_prims(hot, n, nullptr, 0, tri_bounds, scene_bounds, centroids, morton);
 }
 
 void oracle_sort_stable(const uint32_t* morton, int n, uint32_t* keys, int32_t* ids) {
@@ -969,16 +1051,27 @@ void oracle_karras(const uint32_t* k, int n, int32_t* left, int32_t* right, uint
 
 int oracle_scene_create(const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int n, const mcpt_material* mats,
                         int n_mats, const mcpt_env_desc* env, oracle_scene* out) {
-    if (n < 1 || !hot || !cold || !mats || n_mats < 1) return -1;
-    for (int i = 0; i < n; i++)
+    if (n < 1) return -1;
+    return oracle_scene_create_ex(hot, cold, n, nullptr, 0, mats, n_mats, env, out);
+}
+
+int oracle_scene_create_ex(const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int n_tris, const mcpt_sphere* spheres,
+                           int n_spheres, const mcpt_material* mats, int n_mats, const mcpt_env_desc* env,
+                           oracle_scene* out) {
+    const int T = n_tris, S = n_spheres, n = S + T;
+    if (T < 0 || S < 0 || n < 1 || (T && (!hot || !cold)) || (S && !spheres) || !mats || n_mats < 1) return -1;
+    for (int i = 0; i < T; i++)
         if (cold[i].material < 0 || cold[i].material >= n_mats) return -1;
+    for (int i = 0; i < S; i++)
+        if (spheres[i].material < 0 || spheres[i].material >= n_mats || !(spheres[i].radius > 0.0f)) return -1;
     oracle_scene_s* s = new oracle_scene_s();
-    s->hot.assign(hot, hot + n);
-    s->cold.assign(cold, cold + n);
+    s->hot.assign(hot, hot + T);
+    s->cold.assign(cold, cold + T);
+    s->spheres.assign(spheres, spheres + S);
     s->mats.assign(mats, mats + n_mats);
     s->n = n;
-    // lights (finalizeScene, src/path_tracer.cu:1973-1985)
-    for (int i = 0; i < n; i++) {
+    // lights (finalizeScene, src/path_tracer.cu:1973-1985): emissive triangles
+    for (int i = 0; i < T; i++) {
         if (mats[cold[i].material].type != MCPT_MAT_EMISSIVE) continue;
         V3 e1 = v3(hot[i].v1) - v3(hot[i].v0);
         V3 e2 = v3(hot[i].v2) - v3(hot[i].v0);
@@ -1009,7 +1102,7 @@ int oracle_scene_create(const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int 
     s->morton.resize(n);
     std::vector<mcpt_aabb> tb(n);
     mcpt_aabb sb;
-    oracle_finalize(hot, n, tb.data(), &sb, nullptr, s->morton.data());
+    finalize_prims(hot, T, spheres, S, tb.data(), &sb, nullptr, s->morton.data());
     s->tri_boxes.resize(n);
     for (int i = 0; i < n; i++) {
         s->tri_boxes[i].mn = v3(tb[i].min);
@@ -1020,7 +1113,8 @@ int oracle_scene_create(const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int 
     s->keys.resize(n);
     s->ids.resize(n);
     oracle_sort_stable(s->morton.data(), n, s->keys.data(), s->ids.data());
-    s->leaf_prim.assign(s->ids.begin(), s->ids.end());
+    s->leaf_prim.resize(n);
+    for (int i = 0; i < n; i++) s->leaf_prim[i] = s->ids[i] < S ? ~s->ids[i] : s->ids[i] - S;
     s->leaf_box.resize(n);
     for (int i = 0; i < n; i++) s->leaf_box[i] = s->tri_boxes[s->ids[i]];
     s->leaf_parent.assign(n, -1);
@@ -1080,6 +1174,10 @@ int oracle_scene_create(const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int 
 }
 
 void oracle_scene_destroy(oracle_scene s) { delete s; }
+
+void oracle_sphere_uv(const float* normals, int n, float* uv) {
+    for (int i = 0; i < n; i++) sphere_uv(v3(normals + 3 * i), uv[2 * i], uv[2 * i + 1]);
+}
 
 void oracle_sample_albedo(oracle_scene s, const mcpt_material* m, const float* uv, int n, float* out) {
     for (int i = 0; i < n; i++) {
@@ -1156,19 +1254,19 @@ void oracle_trace_closest(oracle_scene s, const mcpt_ray* rays, int n, mcpt_pack
                           uint32_t* leaves, int threads) {
     parallel_for(n, 256, threads, [&](int i) {
         const mcpt_ray& r = rays[i];
-        Closest c = traverse_closest(*s, v3(r.origin), v3(r.dir), r.tmax, steps ? &steps[i] : nullptr,
+        Closest c = traverse_closest(*s, v3(r.origin), v3(r.dir), r.tmax, r.time, steps ? &steps[i] : nullptr,
                                      leaves ? &leaves[i] : nullptr);
         hits[i].t = c.t;
-        hits[i].prim_index = c.prim;
-        hits[i].prim_type = c.prim >= 0 ? 1 : -1;
-        hits[i].did_hit = c.prim >= 0 ? 1 : 0;
+        hits[i].prim_index = !c.hit ? -1 : c.prim >= 0 ? c.prim : ~c.prim;
+        hits[i].prim_type = !c.hit ? -1 : c.prim >= 0 ? 1 : 0;
+        hits[i].did_hit = c.hit ? 1 : 0;
     });
 }
 
 void oracle_trace_any(oracle_scene s, const mcpt_ray* rays, int n, uint8_t* occluded, int threads) {
     parallel_for(n, 256, threads, [&](int i) {
         const mcpt_ray& r = rays[i];
-        occluded[i] = traverse_any(*s, v3(r.origin), v3(r.dir), r.tmax) ? 1 : 0;
+        occluded[i] = traverse_any(*s, v3(r.origin), v3(r.dir), r.tmax, r.time) ? 1 : 0;
     });
 }
 
@@ -1176,15 +1274,15 @@ void oracle_brute_closest(const mcpt_tri_hot* hot, int n_tris, const mcpt_ray* r
     parallel_for(n, 64, 0, [&](int i) {
         const mcpt_ray& r = rays[i];
         V3 o = v3(r.origin), d = v3(r.dir);
-        Closest c{r.tmax, -1};
+        Closest c{r.tmax, -1, false};
         for (int k = 0; k < n_tris; k++) {
             float t;
-            if (hit_triangle(hot[k], o, d, t) && t < c.t && t > 0.0001f) c = Closest{t, k};
+            if (hit_triangle(hot[k], o, d, t) && t < c.t && t > 0.0001f) c = Closest{t, k, true};
         }
         hits[i].t = c.t;
-        hits[i].prim_index = c.prim;
-        hits[i].prim_type = c.prim >= 0 ? 1 : -1;
-        hits[i].did_hit = c.prim >= 0 ? 1 : 0;
+        hits[i].prim_index = c.hit ? c.prim : -1;
+        hits[i].prim_type = c.hit ? 1 : -1;
+        hits[i].did_hit = c.hit ? 1 : 0;
     });
 }
 

@@ -50,6 +50,13 @@ void oracle_karras(const uint32_t* keys, int n, int32_t* left, int32_t* right, u
  * environment (env may be NULL). */
 int oracle_scene_create(const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int n, const mcpt_material* mats,
                         int n_mats, const mcpt_env_desc* env, oracle_scene* out);
+/* computeSphereUvFromOutwardNormal (src/path_tracer.cu:877-884) for n unit normals */
+void oracle_sphere_uv(const float* normals, int n, float* uv);
+/* the same with spheres: the BVH's primitives are the spheres, then the
+ * triangles (finalizeScene, src/path_tracer.cu:1955-1972) */
+int oracle_scene_create_ex(const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int n_tris, const mcpt_sphere* spheres,
+                           int n_spheres, const mcpt_material* mats, int n_mats, const mcpt_env_desc* env,
+                           oracle_scene* out);
 void oracle_scene_destroy(oracle_scene s);
 /* sampleTexture's checker and image textures (src/path_tracer.cu:216-234);
  * returns the texture's index (materials name it as texture = index + 1). */

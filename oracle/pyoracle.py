@@ -19,6 +19,8 @@ COLD_DTYPE = np.dtype([("uv0", "<f4", 2), ("uv1", "<f4", 2), ("uv2", "<f4", 2), 
                        ("n1", "<f4", 3), ("n2", "<f4", 3), ("material", "<i4")])
 MAT_DTYPE = np.dtype([("type", "<i4"), ("albedo", "<f4", 3), ("emission", "<f4", 3), ("fuzz", "<f4"),
                       ("ior", "<f4"), ("density", "<f4"), ("phase_g", "<f4"), ("texture", "<i4")])
+SPHERE_DTYPE = np.dtype([("center", "<f4", 3), ("radius", "<f4"), ("center1", "<f4", 3), ("moving", "<i4"),
+                         ("material", "<i4"), ("pad", "<i4", 3)])
 LIGHT_DTYPE = np.dtype([("tri", HOT_DTYPE), ("emission", "<f4", 3), ("area", "<f4"), ("normal", "<f4", 3)])
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("time", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_type", "<i4"), ("prim_index", "<i4"), ("did_hit", "<i4")])
@@ -51,7 +53,9 @@ def lib():
             "oracle_sort_reference": ([VP, C.c_int, VP, VP], None),
             "oracle_karras": ([VP, C.c_int, VP, VP, VP, VP, VP, VP], None),
             "oracle_scene_create": ([VP, VP, C.c_int, VP, C.c_int, VP, C.POINTER(VP)], C.c_int),
+            "oracle_scene_create_ex": ([VP, VP, C.c_int, VP, C.c_int, VP, C.c_int, VP, C.POINTER(VP)], C.c_int),
             "oracle_scene_destroy": ([VP], None),
+            "oracle_sphere_uv": ([VP, C.c_int, VP], None),
             "oracle_scene_add_texture": ([VP, VP], C.c_int),
             "oracle_sample_albedo": ([VP, VP, VP, C.c_int, VP], None),
             "oracle_scene_dump": ([VP, VP], C.c_int),
@@ -157,19 +161,32 @@ def karras(keys):
     return {"left": L, "right": R, "left_is_leaf": LL, "right_is_leaf": RL, "parent": P, "leaf_parent": LP}
 
 
+def sphere_uv(normals):
+    """computeSphereUvFromOutwardNormal for (n, 3) unit normals -> (n, 2)."""
+    nrm = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    out = np.zeros((len(nrm), 2), np.float32)
+    lib().oracle_sphere_uv(_p(nrm), len(nrm), _p(out))
+    return out
+
+
 class Scene:
-    def __init__(self, hot, cold, mats, sky=None, intensity=1.0, rotation=0.0):
+    def __init__(self, hot, cold, mats, sky=None, intensity=1.0, rotation=0.0, spheres=None):
         self.hot = np.ascontiguousarray(hot, HOT_DTYPE)
         self.cold = np.ascontiguousarray(cold, COLD_DTYPE)
         self.mats = np.ascontiguousarray(mats, MAT_DTYPE)
-        self.n = len(self.hot)
+        self.spheres = np.ascontiguousarray(spheres if spheres is not None else np.zeros(0, SPHERE_DTYPE),
+                                            SPHERE_DTYPE)
+        self.n = len(self.hot) + len(self.spheres)  # BVH primitives
         env = None
         if sky is not None:
             self._sky = np.ascontiguousarray(sky, np.float32)
             env = EnvDesc(self._sky.shape[1], self._sky.shape[0], intensity, rotation, self._sky.ctypes.data)
         self.h = C.c_void_p()
-        rc = lib().oracle_scene_create(_p(self.hot), _p(self.cold), self.n, _p(self.mats), len(self.mats),
-                                       C.byref(env) if env is not None else None, C.byref(self.h))
+        rc = lib().oracle_scene_create_ex(_p(self.hot) if len(self.hot) else None,
+                                          _p(self.cold) if len(self.cold) else None, len(self.hot),
+                                          _p(self.spheres) if len(self.spheres) else None, len(self.spheres),
+                                          _p(self.mats), len(self.mats),
+                                          C.byref(env) if env is not None else None, C.byref(self.h))
         if rc != 0:
             raise ValueError("oracle_scene_create failed")
 
