@@ -393,15 +393,20 @@ def cache_bench(mcpt, ctx, args):
     ref /= done
     log(f"cache: reference {done} spp in {ref_ms / 1e3:.1f} s")
 
-    checkpoints = [1 << k for k in range(0, 30) if (1 << k) <= args.cache_max_spp]
+    # both tracers render frames of S samples per pixel traced together
+    S = args.cache_frame_spp
+    grid = [1, 2, 4, 8, 16, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024]
+    checkpoints = [c for c in grid if c % S == 0 and c <= args.cache_max_spp]
 
-    def converge(render_one):
+    def converge(render_frame, step=S):
         film.clear()
         curve, t_ms, spp = [], 0.0, 0
         for cp in checkpoints:
+            if cp % step:
+                continue
             while spp < cp:
-                t_ms += render_one(spp)
-                spp += 1
+                t_ms += render_frame(spp)
+                spp += step
             e = relmse(film.read() / spp, ref)
             curve.append({"spp": spp, "ms": t_ms, "relmse": e})
             log(f"cache:   spp {spp:5d}  {t_ms:9.1f} ms  relMSE {e:.5f}")
@@ -423,15 +428,21 @@ def cache_bench(mcpt, ctx, args):
 
     log("cache: plain")
     plain = [] if args.cache_only else converge(
-        lambda k: scene.render(cam, film, spp=1, sample_begin=k, max_depth=depth, seed=args.seed)["total_ms"])
+        lambda k: scene.render(cam, film, spp=S, sample_begin=k, max_depth=depth, seed=args.seed)["total_ms"])
+    # the plain tracer at its own best throughput: 16 samples per pixel in
+    # flight per pass
+    log("cache: plain, 16 samples per pass")
+    plain16 = [] if args.cache_only else converge(
+        lambda k: scene.render(cam, film, spp=16, sample_begin=k, max_depth=depth, seed=args.seed)["total_ms"], 16)
     net = ctx.mlp(seed=1)
     agg = {"train_ms": 0.0, "infer_ms": 0.0, "train_samples": 0, "queries": 0, "train_trace_ms": 0.0,
            "render_ms": 0.0}
 
     def cached_one(k):
-        st = scene.render_cached(cam, film, net, spp=1, sample_begin=k, max_depth=depth, seed=args.seed,
+        st = scene.render_cached(cam, film, net, spp=S, sample_begin=k, max_depth=depth, seed=args.seed,
                                  train_paths=args.cache_train_paths, terminate_depth=args.cache_terminate,
-                                 lr=args.cache_lr, train_steps=args.cache_steps, ema_decay=args.cache_ema)
+                                 lr=args.cache_lr, train_steps=args.cache_steps, ema_decay=args.cache_ema,
+                                 frame_spp=S)
         for key in agg:
             agg[key] += st[key]
         agg["loss"] = st["loss"]
@@ -440,30 +451,38 @@ def cache_bench(mcpt, ctx, args):
     log("cache: cached")
     warm_ms = 0.0
     if args.cache_warmup:
-        scratch = ctx.film(W, H)
-        st = scene.render_cached(cam, scratch, net, spp=args.cache_warmup, sample_begin=1 << 20, max_depth=depth,
-                                 seed=args.seed, train_paths=args.cache_train_paths,
+        # training frames of one sample per pixel through a camera with the
+        # same view at 1/div of the resolution: the training paths see the
+        # same scene, each frame costs a fraction of a full one
+        wdiv = args.cache_warmup_div
+        wcam = mcpt.camera_init(mcpt.gen_camera(mcpt.GEN_SOUP, max(1, W // wdiv), max(1, H // wdiv)))
+        scratch = ctx.film(wcam.width, wcam.height)
+        st = scene.render_cached(wcam, scratch, net, spp=args.cache_warmup, sample_begin=1 << 20, max_depth=depth,
+                                 seed=args.seed, train_paths=args.cache_warmup_paths,
                                  terminate_depth=args.cache_terminate, lr=args.cache_lr,
-                                 train_steps=args.cache_steps, ema_decay=args.cache_ema)
+                                 train_steps=args.cache_warmup_steps, ema_decay=args.cache_ema)
         warm_ms = st["total_ms"]
         scratch.close()
-        log(f"cache: warm-up {args.cache_warmup} frames {warm_ms:.1f} ms")
+        log(f"cache: warm-up {args.cache_warmup} frames at {wcam.width}x{wcam.height} {warm_ms:.1f} ms")
     cached = converge(cached_one)
     for c in cached:
         c["ms"] += warm_ms
     target = 0.01
     tp, tc = time_to(plain, target), time_to(cached, target)
+    tp16 = time_to(plain16, target)
     infer_tflops = agg["queries"] * MLP_FLOPS_FWD / (agg["infer_ms"] / 1e3) / 1e12 if agg["infer_ms"] else None
     train_tflops = agg["train_samples"] * MLP_FLOPS_TRAIN / (agg["train_ms"] / 1e3) / 1e12 if agg["train_ms"] else None
     out = {
         "workload": f"{len(hot)} tris incoherent mesh + medium box (sigma_t 0.5, albedo 0.8, g 0.3) + 512x256 sky, "
                     f"{W}x{H}, depth {depth}; reference {done} spp FP64-accumulated",
         "relmse_eps": 0.01,
-        "time_to_relmse_ms": {"plain": tp, "cached": tc, "target": target},
+        "time_to_relmse_ms": {"plain": tp, "cached": tc, "plain_16_per_pass": tp16, "target": target},
         "speedup": (tp / tc) if (tp and tc) else None,
+        "speedup_vs_plain_16_per_pass": (tp16 / tc) if (tp16 and tc) else None,
+        "frame_spp": S,
         "relmse_at_16spp": {"plain": next((c["relmse"] for c in plain if c["spp"] == 16), None),
                             "cached": next((c["relmse"] for c in cached if c["spp"] == 16), None)},
-        "curves": {"plain": plain, "cached": cached},
+        "curves": {"plain": plain, "cached": cached, "plain_16_per_pass": plain16},
         "mlp": {"train_samples": agg["train_samples"], "queries": agg["queries"], "train_ms": agg["train_ms"],
                 "infer_ms": agg["infer_ms"], "train_trace_ms": agg["train_trace_ms"], "render_ms": agg["render_ms"],
                 "last_loss": agg.get("loss"),
@@ -472,7 +491,9 @@ def cache_bench(mcpt, ctx, args):
                          "infer_frac": infer_tflops / (MFMA_BF16_PEAK / 1e12) if infer_tflops else None,
                          "flops_per_query": MLP_FLOPS_FWD, "flops_per_train_sample": MLP_FLOPS_TRAIN}},
         "cache_opts": {"train_paths": args.cache_train_paths, "terminate_depth": args.cache_terminate,
-                       "warmup_frames": args.cache_warmup, "warmup_ms": warm_ms,
+                       "warmup_frames": args.cache_warmup, "warmup_div": args.cache_warmup_div,
+                       "warmup_train_paths": args.cache_warmup_paths, "warmup_steps": args.cache_warmup_steps,
+                       "warmup_ms": warm_ms, "frame_spp": S,
                        "lr": args.cache_lr, "train_steps": args.cache_steps, "ema_decay": args.cache_ema},
     }
     film.close()
@@ -531,14 +552,20 @@ def main():
     ap.add_argument("--cache-ref-spp", type=int, default=4096)
     ap.add_argument("--cache-max-spp", type=int, default=256)
     ap.add_argument("--cache-train-paths", type=int, default=65536)
-    ap.add_argument("--cache-terminate", type=int, default=1)
+    ap.add_argument("--cache-terminate", type=int, default=2)
     ap.add_argument("--cache-lr", type=float, default=0.005)
     ap.add_argument("--cache-steps", type=int, default=4)
     ap.add_argument("--cache-ema", type=float, default=0.0)
     ap.add_argument("--cache-only", action="store_true", help="skip the plain curve (tuning runs)")
-    ap.add_argument("--cache-warmup", type=int, default=0,
+    ap.add_argument("--cache-warmup", type=int, default=4,
                     help="cached frames traced first to train the network; their time counts, their samples "
                          "are not kept")
+    ap.add_argument("--cache-warmup-div", type=int, default=2,
+                    help="the warm-up frames' camera has the same view at 1/div of the width and height")
+    ap.add_argument("--cache-warmup-paths", type=int, default=262144, help="training paths per warm-up frame")
+    ap.add_argument("--cache-warmup-steps", type=int, default=32, help="Adam steps per warm-up frame at most")
+    ap.add_argument("--cache-frame-spp", type=int, default=1,
+                    help="samples per pixel per frame, for the plain and the cached tracer alike")
     ap.add_argument("--serve-jobs", type=int, default=256)
     ap.add_argument("--serve-scenes", type=int, default=32)
     ap.add_argument("--serve-seed", type=int, default=7)

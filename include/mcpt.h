@@ -480,6 +480,8 @@ typedef struct mcpt_cache_opts {
     float ema_decay;         /* queries use an exponential average of the
                                 weights, this decay per Adam step (0 = the
                                 latest weights)                              */
+    int32_t frame_spp;       /* samples per pixel traced together in one
+                                frame, between two training rounds (0 = 1)  */
 } mcpt_cache_opts;
 
 typedef struct mcpt_cache_stats {

@@ -156,7 +156,8 @@ class BatchTune(C.Structure):
 
 class CacheOpts(C.Structure):
     _fields_ = [("train_paths", C.c_int32), ("train_batch", C.c_int32), ("train_steps", C.c_int32),
-                ("terminate_depth", C.c_int32), ("lr", C.c_float), ("ema_decay", C.c_float)]
+                ("terminate_depth", C.c_int32), ("lr", C.c_float), ("ema_decay", C.c_float),
+                ("frame_spp", C.c_int32)]
 
 
 class CacheStats(C.Structure):
@@ -606,9 +607,10 @@ class Scene:
         return st.as_dict()
 
     def render_cached(self, camera, film, mlp, spp=1, sample_begin=0, max_depth=8, seed=0, refill_lanes=0,
-                      train_paths=0, train_batch=0, train_steps=0, terminate_depth=0, lr=0.0, ema_decay=0.0):
+                      train_paths=0, train_batch=0, train_steps=0, terminate_depth=0, lr=0.0, ema_decay=0.0,
+                      frame_spp=0):
         o = RenderOpts(MODE_PERSISTENT, spp, sample_begin, max_depth, seed, refill_lanes, 0, 0, 0)
-        co = CacheOpts(train_paths, train_batch, train_steps, terminate_depth, lr, ema_decay)
+        co = CacheOpts(train_paths, train_batch, train_steps, terminate_depth, lr, ema_decay, frame_spp)
         st = CacheStats()
         _check(_lib.mcpt_render_cached(self.h, C.byref(camera), C.byref(o), film.h, mlp.h, C.byref(co),
                                        C.byref(st)), "mcpt_render_cached")

@@ -415,20 +415,23 @@ MCPT_DEV void vertex_features(const Frame& fr, const vec3& p, const vec3& rd, co
     f[10] = albedo.z;
 }
 
-// image paths: one per pixel (generateCameraRaysKernel, :3342-3374)
-__global__ __launch_bounds__(kBlock) void k_cc_camera(mcpt_camera cam, uint32_t sample, uint64_t seed, CacheWs ws) {
+// image paths: spp per pixel, path p = pixel + npix * k for sample
+// sample0 + k (generateCameraRaysKernel, :3342-3374)
+__global__ __launch_bounds__(kBlock) void k_cc_camera(mcpt_camera cam, uint32_t sample0, int spp, uint64_t seed,
+                                                     CacheWs ws) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
-    const int W = cam.width, npix = cam.width * cam.height;
+    const int W = cam.width, npix = cam.width * cam.height, n = npix * spp;
     if (p == 0) {
-        *ns_slot(ws.counters, 0) = (unsigned long long)(uint32_t)npix;
+        *ns_slot(ws.counters, 0) = (unsigned long long)(uint32_t)n;
         ws.counters[C_QCOUNT] = 0;
     }
-    if (p >= npix) return;
+    if (p >= n) return;
+    const int pixel = p % npix;
     Rng rng;
-    mcpt_pcg32_for_path(&rng.g, (uint32_t)p, sample, seed);
+    mcpt_pcg32_for_path(&rng.g, (uint32_t)pixel, sample0 + (uint32_t)(p / npix), seed);
     vec3 o, d;
     float time;
-    camera_ray(cam, p % W, p / W, rng, o, d, time);
+    camera_ray(cam, pixel % W, pixel / W, rng, o, d, time);
     RayItem it;
     it.ot = make_float4(o.x, o.y, o.z, time);
     it.dp = make_float4(d.x, d.y, d.z, __int_as_float(p));
@@ -436,18 +439,18 @@ __global__ __launch_bounds__(kBlock) void k_cc_camera(mcpt_camera cam, uint32_t 
     ws.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
     ws.smp[p] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
     ws.rng[p] = rng.g.state;
-    ws.pix[p] = p;
+    ws.pix[p] = pixel;
     ws.tslot[p] = -1;
 }
 
-// training slots: pixels drawn at random whose paths are traced without the
-// cache and recorded; a pixel drawn twice keeps its larger slot
-__global__ __launch_bounds__(kBlock) void k_cc_pick(uint32_t frame, uint64_t seed, int n, int npix, CacheWs ws) {
+// training slots: image paths drawn at random that are traced without the
+// cache and recorded; a path drawn twice keeps its larger slot
+__global__ __launch_bounds__(kBlock) void k_cc_pick(uint32_t frame, uint64_t seed, int n, int npaths, CacheWs ws) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     mcpt_pcg32 pick;
     mcpt_pcg32_for_path(&pick, (uint32_t)i, frame, seed ^ 0x5851F42D4C957F2DULL);
-    const int p = (int)(mcpt_pcg32_next(&pick) % (uint32_t)npix);
+    const int p = (int)(mcpt_pcg32_next(&pick) % (uint32_t)npaths);
     ws.tpix[i] = p;
     ws.nv[i] = 0;
     atomicMax(&ws.tslot[p], i);
@@ -654,15 +657,22 @@ __global__ __launch_bounds__(kBlock) void k_cc_targets(CacheWs ws, int n, int ma
     }
 }
 
-// finalizeSampleKernel (:3740-3755): one sample per pixel this frame
-__global__ __launch_bounds__(kBlock) void k_cc_finalize(const float4* smp, float* accum, int npix) {
+// finalizeSampleKernel (:3740-3755): the frame's samples of a pixel added
+// in sample order, as the plain render adds them
+__global__ __launch_bounds__(kBlock) void k_cc_finalize(const float4* smp, float* accum, int npix, int spp) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= npix) return;
-    const float4 s = smp[p];
-    const vec3 c = clamp_sample(vec3(s.x, s.y, s.z));
-    accum[3 * p] = accum[3 * p] + c.x;
-    accum[3 * p + 1] = accum[3 * p + 1] + c.y;
-    accum[3 * p + 2] = accum[3 * p + 2] + c.z;
+    float a0 = accum[3 * p], a1 = accum[3 * p + 1], a2 = accum[3 * p + 2];
+    for (int k = 0; k < spp; k++) {
+        const float4 s = smp[(size_t)k * npix + p];
+        const vec3 c = clamp_sample(vec3(s.x, s.y, s.z));
+        a0 = a0 + c.x;
+        a1 = a1 + c.y;
+        a2 = a2 + c.z;
+    }
+    accum[3 * p] = a0;
+    accum[3 * p + 1] = a1;
+    accum[3 * p + 2] = a2;
 }
 
 }  // namespace
@@ -930,7 +940,9 @@ int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* cam, const mcpt_rend
         return set_error(MCPT_ERR_INVALID, "spp, max_depth or sample_begin out of range");
     if (scene->ctx != mlp->ctx) return set_error(MCPT_ERR_INVALID, "scene and MLP belong to different contexts");
     const int npix = cam->width * cam->height, depth = o->max_depth;
-    const int train_paths = std::min(npix, (co && co->train_paths > 0) ? co->train_paths : 65536);
+    const int fspp = std::min(o->spp, (co && co->frame_spp > 0) ? co->frame_spp : 1);
+    if ((int64_t)npix * fspp > (1 << 28)) return set_error(MCPT_ERR_INVALID, "frame_spp x pixels exceeds 256 Mi paths");
+    const int train_paths = std::min(npix * fspp, (co && co->train_paths > 0) ? co->train_paths : 65536);
     const int batch = (co && co->train_batch > 0) ? co->train_batch : 16384;
     const int max_steps = (co && co->train_steps > 0) ? co->train_steps : 4;
     const int term_depth = (co && co->terminate_depth > 0) ? co->terminate_depth : 1;
@@ -941,7 +953,7 @@ int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* cam, const mcpt_rend
     Ctx* ctx = scene->ctx;
     hipStream_t st = ctx->stream;
     CacheWs ws;
-    MCPT_TRY(ensure_cache_ws(mlp, npix, train_paths, depth, ws));
+    MCPT_TRY(ensure_cache_ws(mlp, (int64_t)npix * fspp, train_paths, depth, ws));
     carve_cache_ws(mlp, ws);
     const SceneView sv = scene->view();
     const size_t lds = (size_t)kBlock * stack_entries(scene->max_depth) * sizeof(int);
@@ -969,15 +981,17 @@ int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* cam, const mcpt_rend
         (void)hipEventElapsedTime(&ms, a, b);
         return (double)ms;
     };
-    const int grid = blocks_for(npix);
-    for (int f = 0; f < o->spp; f++) {
-        const uint32_t sample = (uint32_t)(o->sample_begin + f);
+    for (int f0 = 0; f0 < o->spp; f0 += fspp) {
+        const int fs = std::min(fspp, o->spp - f0);  // samples in this frame
+        const int npaths = npix * fs;
+        const int grid = blocks_for(npaths);
+        const uint32_t sample = (uint32_t)(o->sample_begin + f0);
         // a network that has not been trained yet is not queried: the frame
         // is traced without the cache
         fr.term_depth = mlp->step > 0 ? term_depth : depth;
         MCPT_HIP(hipEventRecord(ev[0], st));
-        k_cc_camera<<<grid, kBlock, 0, st>>>(*cam, sample, o->seed, ws);
-        k_cc_pick<<<blocks_for(train_paths), kBlock, 0, st>>>(sample, o->seed, train_paths, npix, ws);
+        k_cc_camera<<<grid, kBlock, 0, st>>>(*cam, sample, fs, o->seed, ws);
+        k_cc_pick<<<blocks_for(train_paths), kBlock, 0, st>>>(sample, o->seed, train_paths, npaths, ws);
         int cur = 0;
         for (int d = 0; d < depth; d++) {
             fr.depth = d;
@@ -988,11 +1002,11 @@ int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* cam, const mcpt_rend
             cur = 1 - cur;
         }
         MCPT_HIP(hipEventRecord(ev[1], st));
-        k_mlp_infer<<<std::max(1, std::min((npix + kTile - 1) / kTile, 2 * ctx->cu_count)), kBlock, 0, st>>>(
+        k_mlp_infer<<<std::max(1, std::min((npaths + kTile - 1) / kTile, 2 * ctx->cu_count)), kBlock, 0, st>>>(
             decay > 0.0f ? mlp->wimg_ema.ptr : mlp->wimg.ptr, ws.queries, ws.counters + C_QCOUNT, 0, nullptr,
             ws.smp);
         MCPT_HIP(hipEventRecord(ev[2], st));
-        k_cc_finalize<<<grid, kBlock, 0, st>>>(ws.smp, film->accum.ptr, npix);
+        k_cc_finalize<<<blocks_for(npix), kBlock, 0, st>>>(ws.smp, film->accum.ptr, npix, fs);
         // training samples of this frame's recorded paths, for the next frame
         k_cc_scan<<<1, 1024, 0, st>>>(ws.nv, train_paths, ws.offs);
         k_cc_targets<<<blocks_for(train_paths), kBlock, 0, st>>>(ws, train_paths, depth);
@@ -1019,7 +1033,7 @@ int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* cam, const mcpt_rend
         acc.total_ms += elapsed(ev[0], ev[4]);
         acc.train_samples += (uint64_t)std::min(n_train, steps * batch);
         acc.queries += (uint64_t)nq;
-        acc.paths += (uint64_t)npix;
+        acc.paths += (uint64_t)npaths;
         if (steps) acc.loss = l;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);

@@ -186,9 +186,15 @@ MCPT_DEV uint32_t wide_boxes(const float4& o, const uint4& q0, const uint4& q1, 
         const float tnear = fmaxf(fmaxf(0.0001f, tnx), fmaxf(tny, tnz));
         const float tfar = fminf(fminf(s.t, tfx), fminf(tfy, tfz));
         tn[k] = tnear;
-        if (k < cnt && !(tfar <= tnear)) hit |= 1u << k;
+        hit |= (uint32_t)(k < cnt && !(tfar <= tnear)) << k;
     }
     return hit;
+}
+
+// bit j set when child slot j holds a leaf (the sign bits of the references)
+MCPT_DEV uint32_t leaf_slots(const int4& ch) {
+    return ((uint32_t)ch.x >> 31) | (((uint32_t)ch.y >> 31) << 1) | (((uint32_t)ch.z >> 31) << 2) |
+           (((uint32_t)ch.w >> 31) << 3);
 }
 
 MCPT_DEV void test_leaf_ranked(const BvhView& bv, Trav& s, int leaf) {
@@ -251,18 +257,13 @@ MCPT_DEV bool step_wide(const BvhView& bv, Trav& s, int* stk, int stride, uint32
     float tn[4];
     const uint32_t hit = wide_boxes(o, q0, q1, s, tn);
     const int c[4] = {ch.x, ch.y, ch.z, ch.w};
-    uint32_t leaves = 0, inner = 0;
+    const uint32_t lm = leaf_slots(ch);
+    uint32_t leaves = hit & lm;
+    const uint32_t inner = hit & ~lm;
     float d[4];
     int k[4] = {0, 1, 2, 3};
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const bool h = (hit >> j) & 1u;
-        if (h) {
-            if (c[j] < 0) leaves |= 1u << j;
-            else inner |= 1u << j;
-        }
-        d[j] = (h && c[j] >= 0) ? tn[j] : __builtin_inff();
-    }
+    for (int j = 0; j < 4; j++) d[j] = ((inner >> j) & 1u) ? tn[j] : __builtin_inff();
     while (leaves) {
         const int j = __builtin_ctz(leaves);
         leaves &= leaves - 1u;
@@ -298,18 +299,13 @@ MCPT_DEV bool step_wide_defer(const BvhView& bv, Trav& s, int* stk, int stride, 
     float tn[4];
     const uint32_t hit = wide_boxes(o, q0, q1, s, tn);
     const int c[4] = {ch.x, ch.y, ch.z, ch.w};
-    uint32_t leaves = 0, inner = 0;
+    const uint32_t lm = leaf_slots(ch);
+    uint32_t leaves = hit & lm;
+    const uint32_t inner = hit & ~lm;
     float d[4];
     int k[4] = {0, 1, 2, 3};
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const bool h = (hit >> j) & 1u;
-        if (h) {
-            if (c[j] < 0) leaves |= 1u << j;
-            else inner |= 1u << j;
-        }
-        d[j] = (h && c[j] >= 0) ? tn[j] : __builtin_inff();
-    }
+    for (int j = 0; j < 4; j++) d[j] = ((inner >> j) & 1u) ? tn[j] : __builtin_inff();
     s.pend = leaves;
     s.pend_ch = ch;
     if (!inner) return !pop_group(bv, stk, stride, s);
@@ -346,14 +342,9 @@ MCPT_DEV int step_wide_any(const BvhView& bv, Trav& s, int* stk, int stride) {
     float tn[4];
     const uint32_t hit = wide_boxes(o, q0, q1, s, tn);
     const int c[4] = {ch.x, ch.y, ch.z, ch.w};
-    uint32_t leaves = 0, inner = 0;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        if ((hit >> j) & 1u) {
-            if (c[j] < 0) leaves |= 1u << j;
-            else inner |= 1u << j;
-        }
-    }
+    const uint32_t lm = leaf_slots(ch);
+    uint32_t leaves = hit & lm;
+    const uint32_t inner = hit & ~lm;
     while (leaves) {
         const int j = __builtin_ctz(leaves);
         leaves &= leaves - 1u;

@@ -35,7 +35,7 @@ def test_record_layouts(mcpt):
     assert C.sizeof(mcpt.RenderStats) == 96 and C.sizeof(mcpt.EnvDesc) == 24
     assert C.sizeof(mcpt.ServerOpts) == 32 and C.sizeof(mcpt.Job) == 136
     assert C.sizeof(mcpt.JobResult) == 48 and C.sizeof(mcpt.ServerStats) == 88
-    assert C.sizeof(mcpt.CacheOpts) == 24 and C.sizeof(mcpt.CacheStats) == 72
+    assert C.sizeof(mcpt.CacheOpts) == 28 and C.sizeof(mcpt.CacheStats) == 72
     assert C.sizeof(mcpt.BatchTune) == 16
     assert C.sizeof(mcpt.ObjOpts) == 72 and C.sizeof(mcpt.ObjStats) == 32
     assert C.sizeof(mcpt.TextureDesc) == 48 and mcpt.MAT_DTYPE.itemsize == 48

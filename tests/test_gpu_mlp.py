@@ -102,6 +102,34 @@ def test_training_fits_a_smooth_target(mcpt, ctx):
     net.close()
 
 
+@pytest.mark.parametrize("frame_spp", [1, 4])
+def test_untrained_frame_is_the_plain_render(mcpt, ctx, frame_spp):
+    """A network that has not been trained is not queried, so the first
+    frame of a cached render (frame_spp samples of every pixel traced
+    together) is the plain render of those samples, bit for bit; the
+    remaining samples of the call, after training, are not."""
+    W, H, depth = 48, 40, 8
+    hot, cold, mats = mcpt.gen_scene(mcpt.GEN_CORNELL)
+    cam = mcpt.camera_init(mcpt.gen_camera(mcpt.GEN_CORNELL, W, H))
+    scene = ctx.scene(hot, cold, mats)
+    scene.build()
+    net = ctx.mlp(seed=3)
+    cached, plain = ctx.film(W, H), ctx.film(W, H)
+    st = scene.render_cached(cam, cached, net, spp=frame_spp, sample_begin=7, max_depth=depth, seed=2,
+                             train_paths=512, train_batch=1024, frame_spp=frame_spp)
+    assert st["queries"] == 0 and st["paths"] == W * H * frame_spp and st["train_samples"] > 0
+    scene.render(cam, plain, mode=mcpt.MODE_WAVEFRONT, spp=frame_spp, sample_begin=7, max_depth=depth, seed=2)
+    assert np.array_equal(cached.read(), plain.read())
+    # a second call now queries the trained network
+    st = scene.render_cached(cam, cached, net, spp=2 * frame_spp, sample_begin=7 + frame_spp, max_depth=depth,
+                             seed=2, train_paths=512, train_batch=1024, frame_spp=frame_spp)
+    assert st["queries"] > 0 and st["paths"] == 2 * W * H * frame_spp
+    for fl in (cached, plain):
+        fl.close()
+    net.close()
+    scene.close()
+
+
 def test_cached_render_gains_over_plain(mcpt, ctx):
     """Cornell box, 64x64: after the cache has trained for a few frames, a
     few cached samples per pixel are closer to a 1024-sample plain render
