@@ -191,6 +191,13 @@ MCPT_DEV uint32_t wide_boxes(const float4& o, const uint4& q0, const uint4& q1, 
     return hit;
 }
 
+// the reference in child slot j (0..3), as selects rather than branches
+MCPT_DEV int pick4(const int c[4], int j) {
+    const int lo = (j & 1) ? c[1] : c[0];
+    const int hi = (j & 1) ? c[3] : c[2];
+    return (j & 2) ? hi : lo;
+}
+
 // bit j set when child slot j holds a leaf (the sign bits of the references)
 MCPT_DEV uint32_t leaf_slots(const int4& ch) {
     return ((uint32_t)ch.x >> 31) | (((uint32_t)ch.y >> 31) << 1) | (((uint32_t)ch.z >> 31) << 2) |
@@ -227,7 +234,8 @@ MCPT_DEV bool pop_group(const BvhView& bv, int* stk, int stride, Trav& s) {
     const uint32_t seq = (uint32_t)stk[(2 * top + 1) * stride];
     const uint32_t k = seq & 3u, cnt = seq >> 8;
     const int4 ch = bv.wide[w].child;
-    s.node = k == 0 ? ch.x : k == 1 ? ch.y : k == 2 ? ch.z : ch.w;
+    const int c[4] = {ch.x, ch.y, ch.z, ch.w};
+    s.node = pick4(c, (int)k);
     if (cnt > 1) stk[(2 * top + 1) * stride] = (int)(((seq & 0x3Fu) >> 2) | ((cnt - 1) << 8));
     else s.sp = top;
     return true;
@@ -268,7 +276,7 @@ MCPT_DEV bool step_wide(const BvhView& bv, Trav& s, int* stk, int stride, uint32
         const int j = __builtin_ctz(leaves);
         leaves &= leaves - 1u;
         if (COUNT) nl++;
-        test_leaf_ranked(bv, s, ~(j == 0 ? c[0] : j == 1 ? c[1] : j == 2 ? c[2] : c[3]));
+        test_leaf_ranked(bv, s, ~pick4(c, j));
     }
     if (!inner) return !pop_group(bv, stk, stride, s);
     // order the inner children by entry distance (non-inner slots sort last)
@@ -279,7 +287,7 @@ MCPT_DEV bool step_wide(const BvhView& bv, Trav& s, int* stk, int stride, uint32
     cswap(d[1], k[1], d[2], k[2]);
     const uint32_t n = __builtin_popcount(inner);
     if (n > 1) push_group(stk, stride, s, s.node, (uint32_t)k[1] | (uint32_t)k[2] << 2 | (uint32_t)k[3] << 4 | (n - 1) << 8);
-    s.node = k[0] == 0 ? c[0] : k[0] == 1 ? c[1] : k[0] == 2 ? c[2] : c[3];
+    s.node = pick4(c, k[0]);
     return false;
 }
 
@@ -316,7 +324,7 @@ MCPT_DEV bool step_wide_defer(const BvhView& bv, Trav& s, int* stk, int stride, 
     cswap(d[1], k[1], d[2], k[2]);
     const uint32_t n = __builtin_popcount(inner);
     if (n > 1) push_group(stk, stride, s, s.node, (uint32_t)k[1] | (uint32_t)k[2] << 2 | (uint32_t)k[3] << 4 | (n - 1) << 8);
-    s.node = k[0] == 0 ? c[0] : k[0] == 1 ? c[1] : k[0] == 2 ? c[2] : c[3];
+    s.node = pick4(c, k[0]);
     return false;
 }
 
@@ -328,7 +336,7 @@ MCPT_DEV void test_pending(const BvhView& bv, Trav& s, uint32_t& nl) {
         const int j = __builtin_ctz(s.pend);
         s.pend &= s.pend - 1u;
         if (COUNT) nl++;
-        test_leaf_ranked(bv, s, ~(j == 0 ? c[0] : j == 1 ? c[1] : j == 2 ? c[2] : c[3]));
+        test_leaf_ranked(bv, s, ~pick4(c, j));
     }
 }
 
@@ -348,7 +356,7 @@ MCPT_DEV int step_wide_any(const BvhView& bv, Trav& s, int* stk, int stride) {
     while (leaves) {
         const int j = __builtin_ctz(leaves);
         leaves &= leaves - 1u;
-        if (test_leaf_any(bv, s, ~(j == 0 ? c[0] : j == 1 ? c[1] : j == 2 ? c[2] : c[3]))) return 2;
+        if (test_leaf_any(bv, s, ~pick4(c, j))) return 2;
     }
     if (!inner) return pop_group(bv, stk, stride, s) ? 0 : 1;
     // the leftmost inner child next, the others as one group, left to right
@@ -363,7 +371,7 @@ MCPT_DEV int step_wide_any(const BvhView& bv, Trav& s, int* stk, int stride) {
         }
         push_group(stk, stride, s, s.node, seq | n << 8);
     }
-    s.node = first == 0 ? c[0] : first == 1 ? c[1] : first == 2 ? c[2] : c[3];
+    s.node = pick4(c, first);
     return 0;
 }
 
