@@ -703,13 +703,13 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
     Ctx* ctx = scenes[0]->ctx;
     MCPT_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
-    // scenes with one triangle take the single-scene path (its special root)
+    // scenes with one primitive (their special root) or with spheres take
+    // the single-scene path
     std::vector<int> multi;
     float single_ms = 0.f;
-    for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_no_spheres("the batched build"));
     for (int i = 0; i < n; i++) {
-        if (scenes[i]->n < 1) return set_error(MCPT_ERR_INVALID, "scene %d is empty", i);
-        if (scenes[i]->n == 1) {
+        if (scenes[i]->n_prims() < 1) return set_error(MCPT_ERR_INVALID, "scene %d is empty", i);
+        if (scenes[i]->n_prims() == 1 || scenes[i]->n_spheres > 0) {
             MCPT_TRY(lbvh_build(scenes[i], nullptr));
             single_ms += scenes[i]->last_build_ms;
         } else {
@@ -882,7 +882,7 @@ int mcpt_render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int32_t
                       mcpt_film* films, const mcpt_batch_tune* tune, mcpt_render_stats* stats) {
     MCPT_TRY(check_render_batch(scenes, cams, n, o));
     for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_textures());
-    for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_no_spheres("the batched render"));
+    for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_no_moving_spheres("the batched render"));
     if (!films) return set_error(MCPT_ERR_INVALID, "films is NULL");
     std::vector<float*> acc(n);
     for (int i = 0; i < n; i++) {
@@ -898,7 +898,7 @@ int mcpt_batch_autotune(const mcpt_scene* scenes, const mcpt_camera* cams, int32
                         mcpt_batch_tune* best, double* search_ms) {
     MCPT_TRY(check_render_batch(scenes, cams, n, o));
     for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_textures());
-    for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_no_spheres("the batched render"));
+    for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_no_moving_spheres("the batched render"));
     if (!best) return set_error(MCPT_ERR_INVALID, "best is NULL");
     Ctx* ctx = scenes[0]->ctx;
     MCPT_HIP(hipSetDevice(ctx->device));

@@ -489,6 +489,7 @@ __global__ __launch_bounds__(kBlock) void k_cc_extend(SceneView sv, CacheWs ws, 
                     if (q < total) {
                         const RayItem it = ws.q[cur][q];
                         trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+                        s.time = it.ot.w;
                         slot = q;
                         has = true;
                     }
@@ -540,10 +541,10 @@ __global__ __launch_bounds__(kBlock) void k_cc_shade(SceneView sv, CacheWs ws, i
         HitInfo h;
         vec3 albedo;
         if (hit.did_hit) {
-            const mcpt_material& m = sv.mats[sv.cold[hit.prim_index].material];
+            const mcpt_material& m = sv.mats[prim_material(sv, hit.prim_index)];
             diffuse = m.type == MCPT_MAT_LAMBERTIAN;
             if (diffuse) {
-                rebuild_hit(sv, hit.prim_index, ro, rd, hit.t, 0.0f, h);
+                rebuild_hit(sv, hit.prim_index, ro, rd, hit.t, time, h);
                 albedo = sample_albedo(sv, m, h.u, h.v);
             }
         }
@@ -572,7 +573,8 @@ __global__ __launch_bounds__(kBlock) void k_cc_shade(SceneView sv, CacheWs ws, i
             Rng rng;
             rng.g.state = ws.rng[path];
             rng.g.inc = ((uint64_t)(uint32_t)ws.pix[path] << 1u) | 1u;
-            shade_bounce(sv, bg, rr_cap, fr.depth, ro, rd, hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out);
+            shade_bounce(sv, bg, rr_cap, fr.depth, ro, rd, hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out,
+                         time);
             ws.thr[path] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
             ws.smp[path] = make_float4(ps.smp.x, ps.smp.y, ps.smp.z, ps.prev_spec ? 1.0f : 0.0f);
             ws.rng[path] = rng.g.state;
@@ -590,7 +592,7 @@ __global__ __launch_bounds__(kBlock) void k_cc_shade(SceneView sv, CacheWs ws, i
         ShadowItem si;
         si.om = make_float4(out.so.x, out.so.y, out.so.z, out.smax);
         si.dp = make_float4(out.sd.x, out.sd.y, out.sd.z, __int_as_float(path));
-        si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, 0.0f);
+        si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, time);
         ws.sq[sslot] = si;
     }
     const int qslot = wave_push(&ws.counters[C_QCOUNT], query);
@@ -605,7 +607,7 @@ __global__ __launch_bounds__(kBlock) void k_cc_shadow(SceneView sv, CacheWs ws, 
     const ShadowItem si = ws.sq[slot];
     const int path = __float_as_int(si.dp.w);
     const float tr = shadow_transmittance(sv, vec3(si.om.x, si.om.y, si.om.z), vec3(si.dp.x, si.dp.y, si.dp.z),
-                                          si.om.w, lds_stack + threadIdx.x, kBlock);
+                                          si.om.w, lds_stack + threadIdx.x, kBlock, si.c.w);
     if (tr > 0.0f) {
         float4 v = ws.smp[path];
         v.x = v.x + si.c.x * tr;
@@ -931,7 +933,6 @@ int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* cam, const mcpt_rend
     if (!scene || !cam || !o || !film || !mlp) return set_error(MCPT_ERR_INVALID, "bad arguments");
     if (!scene->built) return set_error(MCPT_ERR_INVALID, "scene has no BVH (call mcpt_lbvh_build)");
     MCPT_TRY(scene->check_textures());
-    MCPT_TRY(scene->check_no_spheres("the radiance cache"));
     if (film->w != cam->width || film->h != cam->height)
         return set_error(MCPT_ERR_INVALID, "film %dx%d does not match camera %dx%d", film->w, film->h, cam->width,
                          cam->height);

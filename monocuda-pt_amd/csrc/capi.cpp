@@ -241,8 +241,10 @@ int mcpt_scene_create_ex(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_c
         if (materials[m].type == MCPT_MAT_ISOTROPIC)
             for (int i = 0; i < n_tris; i++)
                 if (cold[i].material == m) s->has_medium = true;
-    for (int i = 0; i < n_spheres; i++)
+    for (int i = 0; i < n_spheres; i++) {
         if (materials[spheres[i].material].type == MCPT_MAT_ISOTROPIC) s->has_medium = true;
+        if (spheres[i].moving) s->moving_spheres = true;
+    }
     // light list (finalizeScene, src/path_tracer.cu:1973-1985)
     for (int i = 0; i < n_tris; i++) {
         const mcpt_material& m = materials[cold[i].material];
@@ -290,9 +292,9 @@ int mcpt_scene_create_ex(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_c
     return MCPT_OK;
 }
 
-int mcpt_scene_s::check_no_spheres(const char* what) const {
-    if (n_spheres > 0) return set_error(MCPT_ERR_INVALID, "%s takes triangle-only scenes (this one has %d spheres)",
-                                        what, n_spheres);
+int mcpt_scene_s::check_no_moving_spheres(const char* what) const {
+    if (moving_spheres)
+        return set_error(MCPT_ERR_INVALID, "%s takes scenes without moving spheres (its rays carry no time)", what);
     return MCPT_OK;
 }
 

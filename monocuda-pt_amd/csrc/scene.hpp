@@ -15,6 +15,7 @@ struct mcpt_scene_s {
     mcpt::Ctx* ctx = nullptr;
     int n = 0;          // triangles
     int n_spheres = 0;  // the BVH's primitives are the spheres, then the triangles
+    bool moving_spheres = false;
     int n_mats = 0;
     bool has_medium = false;
     // inputs (original triangle order)
@@ -108,8 +109,8 @@ struct mcpt_scene_s {
     int n_prims() const { return n + n_spheres; }
     // every texture a material names has been added
     int check_textures() const;
-    // the paths that take triangle-only scenes refuse this one
-    int check_no_spheres(const char* what) const;
+    // the paths whose rays carry no time refuse moving spheres
+    int check_no_moving_spheres(const char* what) const;
 };
 
 namespace mcpt {

@@ -674,7 +674,7 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
         if (!scenes[i] || scenes[i]->ctx != &ctx->c) return set_error(MCPT_ERR_INVALID, "scene %d is invalid", i);
         if (!scenes[i]->built) return set_error(MCPT_ERR_INVALID, "scene %d has no BVH", i);
         MCPT_TRY(scenes[i]->check_textures());
-        MCPT_TRY(scenes[i]->check_no_spheres("the server"));
+        MCPT_TRY(scenes[i]->check_no_moving_spheres("the server"));
     }
     Ctx* c = &ctx->c;
     MCPT_HIP(hipSetDevice(c->device));

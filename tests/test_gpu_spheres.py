@@ -134,21 +134,93 @@ def test_render_sphere_room(mcpt, ctx, oracle, aperture):
     scene.close()
 
 
-def test_triangle_only_paths_refuse_spheres(mcpt, ctx):
+def _static_room(mcpt, rng):
+    hot, cold, mats, sp, img = _sphere_room(mcpt, rng)
+    sp["moving"] = 0
+    mats["texture"] = 0  # the shared-queue paths take scenes without textures here
+    return hot, cold, mats, sp
+
+
+def _alone(mcpt, ctx, scene, cam, spp, depth, seed):
+    film = ctx.film(cam.width, cam.height)
+    scene.render(cam, film, spp=spp, max_depth=depth, seed=seed)
+    img = film.read()
+    film.close()
+    return img
+
+
+def test_batch_and_server_with_static_spheres(mcpt, ctx):
+    """The batched build (sphere scenes take the single-scene build), the
+    batched render and the server give each sphere scene the film
+    mcpt_render gives it alone, bit for bit."""
+    rng = np.random.default_rng(12)
+    hot, cold, mats, sp = _static_room(mcpt, rng)
+    h2, c2, m2, sp2 = _random_scene(mcpt, rng, 200, 4096)
+    sp2["moving"] = 0
+    soup = mcpt.gen_scene(mcpt.GEN_SOUP, seed=3, param=3000)
+    scenes = [ctx.scene(hot, cold, mats, spheres=sp), ctx.scene(h2, c2, m2, spheres=sp2), ctx.scene(*soup)]
+    mcpt.build_batch(scenes)
+    cams = [mcpt.camera_init(mcpt.gen_camera(k, w, h)) for k, w, h in
+            ((mcpt.GEN_CORNELL, 48, 40), (mcpt.GEN_SOUP, 32, 32), (mcpt.GEN_SOUP, 24, 16))]
+    spp, depth, seed = 3, 8, 4
+    refs = [_alone(mcpt, ctx, sc, cam, spp, depth, seed) for sc, cam in zip(scenes, cams)]
+    films = [ctx.film(c.width, c.height) for c in cams]
+    mcpt.render_batch(scenes, cams, films, spp=spp, max_depth=depth, seed=seed)
+    for i, (f, ref) in enumerate(zip(films, refs)):
+        assert np.array_equal(f.read(), ref), i
+        f.close()
+    print("batched render with spheres: films identical", flush=True)
+    srv = ctx.server(scenes)
+    ids = [srv.submit(i, cam, spp, max_depth=depth, seed=seed)[0] for i, cam in enumerate(cams)]
+    for i, jid in enumerate(ids):
+        res, img = srv.wait(jid)
+        assert res["status"] == mcpt.OK and np.array_equal(img, refs[i]), i
+        srv.release(jid)
+    srv.close()
+    print("server with spheres: films identical", flush=True)
+    for sc in scenes:
+        sc.close()
+
+
+def test_moving_spheres_need_ray_time(mcpt, ctx):
+    """Rays of the shared queues carry their scene, not their time: the
+    batched render and the server refuse moving spheres."""
     rng = np.random.default_rng(4)
     hot, cold, mats, sp = _random_scene(mcpt, rng, 10, 256)
     scene = ctx.scene(hot, cold, mats, spheres=sp)
+    mcpt.build_batch([scene])
+    cam = mcpt.camera_init(mcpt.gen_camera(mcpt.GEN_SOUP, 16, 16))
+    film = ctx.film(16, 16)
     with pytest.raises(mcpt.McptError):
-        mcpt.build_batch([scene])
-    scene.build()
+        mcpt.render_batch([scene], [cam], [film], spp=1)
     with pytest.raises(mcpt.McptError):
         ctx.server([scene])
-    desc = mcpt.gen_camera(mcpt.GEN_CORNELL, 16, 16)
-    cam = mcpt.camera_init(desc)
-    film = ctx.film(16, 16)
-    mlp = ctx.mlp()
-    with pytest.raises(mcpt.McptError):
-        scene.render_cached(cam, film, mlp, spp=1)
-    mlp.close()
     film.close()
+    scene.close()
+
+
+@pytest.mark.parametrize("frame_spp", [1, 3])
+def test_cache_with_spheres(mcpt, ctx, frame_spp):
+    """The radiance cache on a room with spheres (moving ones included):
+    its first, untrained frame is the plain render bit for bit, and after
+    training it queries the network."""
+    rng = np.random.default_rng(31)
+    hot, cold, mats, sp, img = _sphere_room(mcpt, rng)
+    scene = ctx.scene(hot, cold, mats, spheres=sp)
+    assert scene.add_texture(mcpt.TEX_IMAGE, image=img) == 0
+    scene.build()
+    cam = mcpt.camera_init(mcpt.gen_camera(mcpt.GEN_CORNELL, 40, 32))
+    net = ctx.mlp(seed=2)
+    cached, plain = ctx.film(40, 32), ctx.film(40, 32)
+    st = scene.render_cached(cam, cached, net, spp=frame_spp, sample_begin=5, max_depth=8, seed=7, train_paths=256,
+                             train_batch=512, frame_spp=frame_spp)
+    assert st["queries"] == 0 and st["train_samples"] > 0
+    scene.render(cam, plain, mode=mcpt.MODE_WAVEFRONT, spp=frame_spp, sample_begin=5, max_depth=8, seed=7)
+    assert np.array_equal(cached.read(), plain.read())
+    st = scene.render_cached(cam, cached, net, spp=2, sample_begin=20, max_depth=8, seed=7, train_paths=256,
+                             train_batch=512)
+    assert st["queries"] > 0 and np.all(np.isfinite(cached.read()))
+    for f in (cached, plain):
+        f.close()
+    net.close()
     scene.close()
