@@ -179,8 +179,8 @@ int mcpt_scene_create(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_cold
  * reference's finalizeScene forms (:1955-1972): the spheres first, then the
  * triangles, so sorted ids and leaf order count spheres from 0 and
  * triangles from n_spheres. n_tris may be 0 when there are spheres. The
- * batched render and the server refuse moving spheres (their rays carry
- * no time). */
+ * batched render and the server take at most 256 scenes when one of them
+ * has moving spheres (their rays then carry scene and time in one word). */
 int mcpt_scene_create_ex(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int32_t n_tris,
                          const mcpt_sphere* spheres, int32_t n_spheres, const mcpt_material* materials,
                          int32_t n_materials, const mcpt_env_desc* env, mcpt_scene* out);

@@ -314,6 +314,7 @@ struct BWs {
     float4* smp;
     uint64_t* rng;
     int* counters;
+    int tb;  // 24: ray records carry scene and time (see pack_scene_time)
 };
 
 __global__ void k_bb_init(int* c, int count) {
@@ -352,9 +353,8 @@ __global__ __launch_bounds__(kBlock) void k_bb_camera(const BScene* tab, int nsc
     vec3 o, d;
     float time;
     camera_ray(cam, pixel % cam.width, pixel / cam.width, rng, o, d, time);
-    (void)time;  // static scenes: no kernel reads it
     RayItem it;
-    it.ot = make_float4(o.x, o.y, o.z, __int_as_float(s));
+    it.ot = make_float4(o.x, o.y, o.z, pack_scene_time(s, time, ws.tb));
     it.dp = make_float4(d.x, d.y, d.z, __int_as_float(path));
     ws.q[0][path] = it;
     ws.thr[path] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
@@ -390,12 +390,13 @@ __global__ __launch_bounds__(kBlock) void k_bb_extend(const BvhView* bvhs, BWs w
                     const int q = first + rank_below(idle);
                     if (q < total) {
                         const RayItem it = ws.q[cur][q];
-                        const int sc = __float_as_int(it.ot.w);
+                        const int sc = unpack_scene(it.ot.w, ws.tb);
                         if (sc != bv_scene) {
                             bv = bvhs[sc];
                             bv_scene = sc;
                         }
                         trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+                        s.time = unpack_time(it.ot.w, ws.tb);
                         slot = q;
                         has = true;
                     }
@@ -424,11 +425,13 @@ __global__ __launch_bounds__(kBlock) void k_bb_shade(const BScene* tab, BWs ws, 
     out.next = false;
     out.shadow = false;
     int path = 0, scene = 0;
+    float time = 0.0f;
     if (active) {
         const RayItem it = ws.q[cur][slot];
         const mcpt_packed_hit hit = ws.hits[slot];
         path = __float_as_int(it.dp.w);
-        scene = __float_as_int(it.ot.w);
+        scene = unpack_scene(it.ot.w, ws.tb);
+        time = unpack_time(it.ot.w, ws.tb);
         const BScene& B = tab[scene];
         const float4 thr = ws.thr[path], smp = ws.smp[path];
         PathState ps;
@@ -441,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_bb_shade(const BScene* tab, BWs ws, 
         const int pixel = path % P - B.foff;
         rng.g.inc = ((uint64_t)(uint32_t)pixel << 1u) | 1u;
         shade_bounce(B.sv, load3(B.cam.background), B.cam.rr_cap, depth, vec3(it.ot.x, it.ot.y, it.ot.z),
-                     vec3(it.dp.x, it.dp.y, it.dp.z), hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out);
+                     vec3(it.dp.x, it.dp.y, it.dp.z), hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out, time);
         ws.thr[path] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
         ws.smp[path] = make_float4(ps.smp.x, ps.smp.y, ps.smp.z, ps.prev_spec ? 1.0f : 0.0f);
         ws.rng[path] = rng.g.state;
@@ -450,7 +453,7 @@ __global__ __launch_bounds__(kBlock) void k_bb_shade(const BScene* tab, BWs ws, 
     wave_push2(ns_slot(ws.counters, 1 - cur), out.next, out.shadow, nslot, sslot);
     if (out.next) {
         RayItem it;
-        it.ot = make_float4(out.no.x, out.no.y, out.no.z, __int_as_float(scene));
+        it.ot = make_float4(out.no.x, out.no.y, out.no.z, pack_scene_time(scene, time, ws.tb));
         it.dp = make_float4(out.nd.x, out.nd.y, out.nd.z, __int_as_float(path));
         ws.q[1 - cur][nslot] = it;
     }
@@ -458,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_bb_shade(const BScene* tab, BWs ws, 
         ShadowItem si;
         si.om = make_float4(out.so.x, out.so.y, out.so.z, out.smax);
         si.dp = make_float4(out.sd.x, out.sd.y, out.sd.z, __int_as_float(path));
-        si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, __int_as_float(scene));
+        si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, pack_scene_time(scene, time, ws.tb));
         ws.sq[sslot] = si;
     }
 }
@@ -502,10 +505,11 @@ __global__ __launch_bounds__(kBlock) void k_bb_shadow_persistent(const BScene* t
                     const int q = first + rank_below(idle);
                     if (q < total) {
                         const ShadowItem si = ws.sq[q];
-                        const int sc = __float_as_int(si.c.w);
+                        const int sc = unpack_scene(si.c.w, ws.tb);
+                        const float tm = unpack_time(si.c.w, ws.tb);
                         const vec3 o(si.om.x, si.om.y, si.om.z), d(si.dp.x, si.dp.y, si.dp.z);
                         if (tab[sc].sv.has_medium) {
-                            const float tr = shadow_transmittance(tab[sc].sv, o, d, si.om.w, stk, kBlock);
+                            const float tr = shadow_transmittance(tab[sc].sv, o, d, si.om.w, stk, kBlock, tm);
                             if (tr > 0.0f) add_shadow(ws.smp, si, tr);
                         } else {
                             if (sc != bv_scene) {
@@ -513,6 +517,7 @@ __global__ __launch_bounds__(kBlock) void k_bb_shadow_persistent(const BScene* t
                                 bv_scene = sc;
                             }
                             trav_begin(s, o, d, si.om.w);
+                            s.time = tm;
                             slot = q;
                             has = true;
                         }
@@ -560,6 +565,8 @@ int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const
                  float* const* accum, const mcpt_batch_tune* tune, mcpt_render_stats* stats) {
     Ctx* ctx = scenes[0]->ctx;
     hipStream_t st = ctx->stream;
+    bool moving = false;  // ray records then carry their time too
+    for (int i = 0; i < n; i++) moving = moving || scenes[i]->moving_spheres;
     std::vector<BScene> tab(n);
     std::vector<BvhView> bvh(n);
     int64_t P = 0;
@@ -612,6 +619,7 @@ int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const
     ws.smp = (float4*)b, b += parts[k++];
     ws.rng = (uint64_t*)b, b += parts[k++];
     ws.counters = (int*)b;
+    ws.tb = moving ? 24 : 0;
     const size_t lds = (size_t)kBlock * stack_entries(max_stack) * sizeof(int);
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bb_extend, kBlock, lds);
@@ -675,6 +683,14 @@ int check_batch(const mcpt_scene* scenes, int32_t n) {
     for (int i = 0; i < n; i++)
         if (!scenes[i] || scenes[i]->ctx != scenes[0]->ctx)
             return set_error(MCPT_ERR_INVALID, "scene %d is missing or belongs to another context", i);
+    return MCPT_OK;
+}
+
+// ray records carry scene and time in one slot for at most 256 scenes
+int check_moving_spheres(const mcpt_scene* scenes, int32_t n) {
+    bool moving = false;
+    for (int i = 0; i < n; i++) moving = moving || scenes[i]->moving_spheres;
+    if (moving && n > 256) return set_error(MCPT_ERR_INVALID, "a batch with moving spheres takes at most 256 scenes");
     return MCPT_OK;
 }
 
@@ -882,7 +898,7 @@ int mcpt_render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int32_t
                       mcpt_film* films, const mcpt_batch_tune* tune, mcpt_render_stats* stats) {
     MCPT_TRY(check_render_batch(scenes, cams, n, o));
     for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_textures());
-    for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_no_moving_spheres("the batched render"));
+    MCPT_TRY(check_moving_spheres(scenes, n));
     if (!films) return set_error(MCPT_ERR_INVALID, "films is NULL");
     std::vector<float*> acc(n);
     for (int i = 0; i < n; i++) {
@@ -898,7 +914,7 @@ int mcpt_batch_autotune(const mcpt_scene* scenes, const mcpt_camera* cams, int32
                         mcpt_batch_tune* best, double* search_ms) {
     MCPT_TRY(check_render_batch(scenes, cams, n, o));
     for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_textures());
-    for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_no_moving_spheres("the batched render"));
+    MCPT_TRY(check_moving_spheres(scenes, n));
     if (!best) return set_error(MCPT_ERR_INVALID, "best is NULL");
     Ctx* ctx = scenes[0]->ctx;
     MCPT_HIP(hipSetDevice(ctx->device));

@@ -292,12 +292,6 @@ int mcpt_scene_create_ex(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_c
     return MCPT_OK;
 }
 
-int mcpt_scene_s::check_no_moving_spheres(const char* what) const {
-    if (moving_spheres)
-        return set_error(MCPT_ERR_INVALID, "%s takes scenes without moving spheres (its rays carry no time)", what);
-    return MCPT_OK;
-}
-
 int mcpt_scene_s::check_textures() const {
     if (max_material_texture > (int)tex_host.size())
         return set_error(MCPT_ERR_INVALID, "a material names texture %d but the scene has %d", max_material_texture,

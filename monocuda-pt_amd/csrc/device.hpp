@@ -253,6 +253,23 @@ MCPT_HDI void sphere_box(const vec3& c0, const vec3& c1, bool moving, float r, f
     }
 }
 
+// Rays of a queue shared by several scenes (server, batched render) carry
+// their scene in the ray record's time slot. With tb = 24 the slot also
+// holds the time, as scene << 24 | k: a camera time is k * 2^-24 with
+// k < 2^24 (the uniform draw, mcpt_pcg32.h), so it is kept exactly; this
+// form serves up to 256 scenes. With tb = 0 the slot is the scene alone and
+// the time 0 (scenes without moving spheres do not read it).
+MCPT_DEV float pack_scene_time(int scene, float time, int tb) {
+    return __int_as_float(tb ? (int)(((uint32_t)scene << 24) | (uint32_t)(time * 16777216.0f)) : scene);
+}
+MCPT_DEV int unpack_scene(float w, int tb) {
+    const uint32_t b = (uint32_t)__float_as_int(w);
+    return tb ? (int)(b >> 24) : (int)b;
+}
+MCPT_DEV float unpack_time(float w, int tb) {
+    return tb ? (float)((uint32_t)__float_as_int(w) & 0xFFFFFFu) * (1.0f / 16777216.0f) : 0.0f;
+}
+
 // ------------------------------------------------------------ scene view
 struct EnvView {
     const float4* texels = nullptr;

@@ -109,8 +109,7 @@ struct mcpt_scene_s {
     int n_prims() const { return n + n_spheres; }
     // every texture a material names has been added
     int check_textures() const;
-    // the paths whose rays carry no time refuse moving spheres
-    int check_no_moving_spheres(const char* what) const;
+
 };
 
 namespace mcpt {

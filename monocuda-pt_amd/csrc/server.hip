@@ -83,6 +83,7 @@ struct Pool {
     const SceneView* scenes;
     const BvhView* bvhs;
     int max_frames;
+    int tb;  // 24: ray records carry scene and time (see pack_scene_time)
 };
 
 // ------------------------------------------------------------------ kernels
@@ -115,10 +116,9 @@ __global__ __launch_bounds__(kBlock) void k_srv_generate(Pool p, int cur, int to
     vec3 o, d;
     float time;
     camera_ray(J.cam, pixel % J.cam.width, pixel / J.cam.width, rng, o, d, time);
-    (void)time;  // camera time: static scenes, no kernel reads it
     const int slot = base + g;
     RayItem it;
-    it.ot = make_float4(o.x, o.y, o.z, __int_as_float(J.scene));
+    it.ot = make_float4(o.x, o.y, o.z, pack_scene_time(J.scene, time, p.tb));
     it.dp = make_float4(d.x, d.y, d.z, 0.0f);
     p.q[cur][slot] = it;
     p.thr[cur][slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
@@ -155,12 +155,13 @@ __global__ __launch_bounds__(kBlock) void k_srv_extend(Pool p, int cur, int refi
                     const int q = first + rank_below(idle);
                     if (q < total) {
                         const RayItem it = p.q[cur][q];
-                        const int sc = __float_as_int(it.ot.w);
+                        const int sc = unpack_scene(it.ot.w, p.tb);
                         if (sc != bv_scene) {
                             bv = p.bvhs[sc];
                             bv_scene = sc;
                         }
                         trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+                        s.time = unpack_time(it.ot.w, p.tb);
                         slot = q;
                         has = true;
                     }
@@ -192,13 +193,15 @@ __global__ __launch_bounds__(kBlock) void k_srv_shade(Pool p, int cur) {
     out.shadow = false;
     bool finished = false;
     int scene = 0;
+    float time = 0.0f;
     PathState ps;
     Rng rng;
     int4 m = make_int4(0, 0, 0, 0);
     if (active) {
         const RayItem it = p.q[cur][slot];
         const mcpt_packed_hit hit = p.hits[slot];
-        scene = __float_as_int(it.ot.w);
+        scene = unpack_scene(it.ot.w, p.tb);
+        time = unpack_time(it.ot.w, p.tb);
         m = p.meta[cur][slot];
         const JobDev& J = p.jobs[m.x];
         const SceneView& sv = p.scenes[scene];
@@ -210,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_srv_shade(Pool p, int cur) {
         rng.g.state = p.rng[cur][slot];
         rng.g.inc = ((uint64_t)(uint32_t)m.y << 1u) | 1u;
         shade_bounce(sv, load3(J.cam.background), J.cam.rr_cap, m.w, vec3(it.ot.x, it.ot.y, it.ot.z),
-                     vec3(it.dp.x, it.dp.y, it.dp.z), hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out);
+                     vec3(it.dp.x, it.dp.y, it.dp.z), hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out, time);
         // the reference's depth loop ends after shading bounce max_depth-1
         if (out.next && m.w + 1 >= J.max_depth) out.next = false;
         finished = !out.next;
@@ -223,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_srv_shade(Pool p, int cur) {
     if (out.next) {
         const int n = 1 - cur;
         RayItem it;
-        it.ot = make_float4(out.no.x, out.no.y, out.no.z, __int_as_float(scene));
+        it.ot = make_float4(out.no.x, out.no.y, out.no.z, pack_scene_time(scene, time, p.tb));
         it.dp = make_float4(out.nd.x, out.nd.y, out.nd.z, 0.0f);
         p.q[n][nslot] = it;
         p.thr[n][nslot] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
@@ -239,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_srv_shade(Pool p, int cur) {
         ShadowItem si;
         si.om = make_float4(out.so.x, out.so.y, out.so.z, out.smax);
         si.dp = make_float4(out.sd.x, out.sd.y, out.sd.z, __int_as_float(out.next ? nslot : ~dslot));
-        si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, __int_as_float(scene));
+        si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, pack_scene_time(scene, time, p.tb));
         p.sq[sslot] = si;
     }
 }
@@ -285,10 +288,11 @@ __global__ __launch_bounds__(kBlock) void k_srv_shadow_persistent(Pool p, int cu
                     const int q = first + rank_below(idle);
                     if (q < total) {
                         const ShadowItem si = p.sq[q];
-                        const int sc = __float_as_int(si.c.w);
+                        const int sc = unpack_scene(si.c.w, p.tb);
+                        const float tm = unpack_time(si.c.w, p.tb);
                         const vec3 o(si.om.x, si.om.y, si.om.z), d(si.dp.x, si.dp.y, si.dp.z);
                         if (p.scenes[sc].has_medium) {
-                            const float tr = shadow_transmittance(p.scenes[sc], o, d, si.om.w, stk, kBlock);
+                            const float tr = shadow_transmittance(p.scenes[sc], o, d, si.om.w, stk, kBlock, tm);
                             if (tr > 0.0f) srv_add_shadow(p, cur, si, tr);
                         } else {
                             if (sc != bv_scene) {
@@ -296,6 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_srv_shadow_persistent(Pool p, int cu
                                 bv_scene = sc;
                             }
                             trav_begin(s, o, d, si.om.w);
+                            s.time = tm;
                             slot = q;
                             has = true;
                         }
@@ -674,8 +679,11 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
         if (!scenes[i] || scenes[i]->ctx != &ctx->c) return set_error(MCPT_ERR_INVALID, "scene %d is invalid", i);
         if (!scenes[i]->built) return set_error(MCPT_ERR_INVALID, "scene %d has no BVH", i);
         MCPT_TRY(scenes[i]->check_textures());
-        MCPT_TRY(scenes[i]->check_no_moving_spheres("the server"));
     }
+    bool moving = false;
+    for (int i = 0; i < n_scenes; i++) moving = moving || scenes[i]->moving_spheres;
+    if (moving && n_scenes > 256)
+        return set_error(MCPT_ERR_INVALID, "a server with moving spheres takes at most 256 scenes");
     Ctx* c = &ctx->c;
     MCPT_HIP(hipSetDevice(c->device));
     auto* s = new mcpt_server_s();
@@ -742,6 +750,7 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
     p.scenes = s->scene_tab;
     p.bvhs = s->bvh_tab;
     p.max_frames = s->max_frames;
+    p.tb = moving ? 24 : 0;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&s->ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&s->ev[1]);

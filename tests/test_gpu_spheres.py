@@ -134,13 +134,6 @@ def test_render_sphere_room(mcpt, ctx, oracle, aperture):
     scene.close()
 
 
-def _static_room(mcpt, rng):
-    hot, cold, mats, sp, img = _sphere_room(mcpt, rng)
-    sp["moving"] = 0
-    mats["texture"] = 0  # the shared-queue paths take scenes without textures here
-    return hot, cold, mats, sp
-
-
 def _alone(mcpt, ctx, scene, cam, spp, depth, seed):
     film = ctx.film(cam.width, cam.height)
     scene.render(cam, film, spp=spp, max_depth=depth, seed=seed)
@@ -149,19 +142,23 @@ def _alone(mcpt, ctx, scene, cam, spp, depth, seed):
     return img
 
 
-def test_batch_and_server_with_static_spheres(mcpt, ctx):
+def test_batch_and_server_with_spheres(mcpt, ctx):
     """The batched build (sphere scenes take the single-scene build), the
-    batched render and the server give each sphere scene the film
-    mcpt_render gives it alone, bit for bit."""
+    batched render and the server give each sphere scene, moving spheres
+    included, the film mcpt_render gives it alone, bit for bit: their shared
+    queues carry each ray's scene and time together."""
     rng = np.random.default_rng(12)
-    hot, cold, mats, sp = _static_room(mcpt, rng)
+    hot, cold, mats, sp, _ = _sphere_room(mcpt, rng)
+    mats["texture"] = 0
     h2, c2, m2, sp2 = _random_scene(mcpt, rng, 200, 4096)
-    sp2["moving"] = 0
+    h3, c3, m3, sp3 = _random_scene(mcpt, rng, 50, 1024)
+    sp3["moving"] = 0
     soup = mcpt.gen_scene(mcpt.GEN_SOUP, seed=3, param=3000)
-    scenes = [ctx.scene(hot, cold, mats, spheres=sp), ctx.scene(h2, c2, m2, spheres=sp2), ctx.scene(*soup)]
+    scenes = [ctx.scene(hot, cold, mats, spheres=sp), ctx.scene(h2, c2, m2, spheres=sp2),
+              ctx.scene(h3, c3, m3, spheres=sp3), ctx.scene(*soup)]
     mcpt.build_batch(scenes)
     cams = [mcpt.camera_init(mcpt.gen_camera(k, w, h)) for k, w, h in
-            ((mcpt.GEN_CORNELL, 48, 40), (mcpt.GEN_SOUP, 32, 32), (mcpt.GEN_SOUP, 24, 16))]
+            ((mcpt.GEN_CORNELL, 48, 40), (mcpt.GEN_SOUP, 32, 32), (mcpt.GEN_SOUP, 20, 12), (mcpt.GEN_SOUP, 24, 16))]
     spp, depth, seed = 3, 8, 4
     refs = [_alone(mcpt, ctx, sc, cam, spp, depth, seed) for sc, cam in zip(scenes, cams)]
     films = [ctx.film(c.width, c.height) for c in cams]
@@ -170,6 +167,12 @@ def test_batch_and_server_with_static_spheres(mcpt, ctx):
         assert np.array_equal(f.read(), ref), i
         f.close()
     print("batched render with spheres: films identical", flush=True)
+    # the static-sphere and triangle scenes alone use the scene-only records
+    films = [ctx.film(c.width, c.height) for c in cams[2:]]
+    mcpt.render_batch(scenes[2:], cams[2:], films, spp=spp, max_depth=depth, seed=seed)
+    for i, (f, ref) in enumerate(zip(films, refs[2:])):
+        assert np.array_equal(f.read(), ref), i
+        f.close()
     srv = ctx.server(scenes)
     ids = [srv.submit(i, cam, spp, max_depth=depth, seed=seed)[0] for i, cam in enumerate(cams)]
     for i, jid in enumerate(ids):
@@ -180,23 +183,6 @@ def test_batch_and_server_with_static_spheres(mcpt, ctx):
     print("server with spheres: films identical", flush=True)
     for sc in scenes:
         sc.close()
-
-
-def test_moving_spheres_need_ray_time(mcpt, ctx):
-    """Rays of the shared queues carry their scene, not their time: the
-    batched render and the server refuse moving spheres."""
-    rng = np.random.default_rng(4)
-    hot, cold, mats, sp = _random_scene(mcpt, rng, 10, 256)
-    scene = ctx.scene(hot, cold, mats, spheres=sp)
-    mcpt.build_batch([scene])
-    cam = mcpt.camera_init(mcpt.gen_camera(mcpt.GEN_SOUP, 16, 16))
-    film = ctx.film(16, 16)
-    with pytest.raises(mcpt.McptError):
-        mcpt.render_batch([scene], [cam], [film], spp=1)
-    with pytest.raises(mcpt.McptError):
-        ctx.server([scene])
-    film.close()
-    scene.close()
 
 
 @pytest.mark.parametrize("frame_spp", [1, 3])
