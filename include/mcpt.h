@@ -610,6 +610,37 @@ int mcpt_load_obj(const char* path, const mcpt_obj_opts* opts, mcpt_tri_hot* hot
  * a texture object inside its loader. */
 int mcpt_obj_texture(int32_t index, int32_t* width, int32_t* height, uint8_t* rgba8);
 
+/* ------------------------------------------------------------ pbrt import
+ * A pbrt-v3 scene file into triangle, sphere and material records
+ * (loadPbrtScene, src/path_tracer.cu:1990-2483): trianglemesh, plymesh and
+ * sphere shapes with their transforms and instances; materials, area
+ * lights, media, colour textures (image maps read as PPM), lights and the
+ * camera mapped as the reference maps them. Two-call pattern: NULL arrays
+ * to get the counts in `info`, then arrays of that size. max_prims bounds
+ * triangles + spheres (0: 2^30). */
+typedef struct mcpt_pbrt_info {
+    int32_t n_tris, n_spheres, n_materials;
+    int32_t textures_loaded;    /* images for mcpt_pbrt_texture              */
+    int32_t has_camera;         /* look_from .. vfov_degrees are valid       */
+    int32_t width, height;      /* Film resolution (0 without a Film)        */
+    float look_from[3], look_at[3], up[3];
+    float vfov_degrees;
+    float background[3];        /* an infinite light without a map           */
+    int32_t has_env_map;        /* env_path names the infinite light's map   */
+    float env_intensity, env_rotation_degrees;
+    char env_path[1024];
+    int32_t shapes, instances, skipped_shapes, approximated_lights;
+    int32_t unsupported_textures, approximated_materials;
+} mcpt_pbrt_info;
+
+int mcpt_load_pbrt(const char* path, int64_t max_prims, mcpt_tri_hot* hot, mcpt_tri_cold* cold,
+                   mcpt_sphere* spheres, mcpt_material* materials, mcpt_pbrt_info* info);
+/* The index-th image texture of this thread's last successful
+ * mcpt_load_pbrt call (material.texture = index + 1), RGBA8, top row first;
+ * two-call pattern. No reference counterpart: the reference uploads each
+ * image to a texture object inside its loader. */
+int mcpt_pbrt_texture(int32_t index, int32_t* width, int32_t* height, uint8_t* rgba8);
+
 /* Procedural equirect sky: gradient + sun disk + glow (closed form of
  * scripts/make_sky.py:5-27), width*height*4 floats, top row first. */
 int mcpt_gen_sky(int32_t width, int32_t height, float* rgba);

@@ -122,6 +122,24 @@ class ObjStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class PbrtInfo(C.Structure):
+    _fields_ = [("n_tris", C.c_int32), ("n_spheres", C.c_int32), ("n_materials", C.c_int32),
+                ("textures_loaded", C.c_int32), ("has_camera", C.c_int32), ("width", C.c_int32),
+                ("height", C.c_int32), ("look_from", C.c_float * 3), ("look_at", C.c_float * 3), ("up", C.c_float * 3),
+                ("vfov_degrees", C.c_float), ("background", C.c_float * 3), ("has_env_map", C.c_int32),
+                ("env_intensity", C.c_float), ("env_rotation_degrees", C.c_float), ("env_path", C.c_char * 1024),
+                ("shapes", C.c_int32), ("instances", C.c_int32), ("skipped_shapes", C.c_int32),
+                ("approximated_lights", C.c_int32), ("unsupported_textures", C.c_int32),
+                ("approximated_materials", C.c_int32)]
+
+    def as_dict(self):
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            out[k] = v.decode(errors="replace") if k == "env_path" else (list(v) if hasattr(v, "__len__") else v)
+        return out
+
+
 class TextureDesc(C.Structure):
     _fields_ = [("type", C.c_int32), ("color_a", C.c_float * 3), ("color_b", C.c_float * 3), ("scale", C.c_float),
                 ("width", C.c_int32), ("height", C.c_int32), ("rgba8", C.c_void_p)]
@@ -257,6 +275,8 @@ _SIGS = {
     "mcpt_load_pfm": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_hdr": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_obj_texture": ([C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
+    "mcpt_load_pbrt": ([C.c_char_p, C.c_int64, VP, VP, VP, VP, C.POINTER(PbrtInfo)], C.c_int),
+    "mcpt_pbrt_texture": ([C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_gen_sky": ([C.c_int32, C.c_int32, VP], C.c_int),
     "mcpt_gen_camera": ([C.c_int32, C.c_int32, C.c_int32, C.POINTER(CameraDesc)], C.c_int),
 }
@@ -368,6 +388,28 @@ def load_obj(path, translation=(0.0, 0.0, 0.0), scale=1.0, rotation_y=0.0, fallb
         _check(_lib.mcpt_obj_texture(k, C.byref(w), C.byref(h), _ptr(img)), "mcpt_obj_texture")
         images.append(img)
     return hot, cold, mats, st.as_dict(), images
+
+
+def load_pbrt(path, max_prims=0):
+    """mcpt_load_pbrt: a pbrt-v3 scene -> (hot, cold, spheres, materials,
+    info, images); material texture k names images[k - 1]."""
+    p = os.fsencode(path)
+    info = PbrtInfo()
+    _check(_lib.mcpt_load_pbrt(p, max_prims, None, None, None, None, C.byref(info)), "mcpt_load_pbrt")
+    hot = np.zeros(info.n_tris, HOT_DTYPE)
+    cold = np.zeros(info.n_tris, COLD_DTYPE)
+    sph = np.zeros(info.n_spheres, SPHERE_DTYPE)
+    mats = np.zeros(info.n_materials, MAT_DTYPE)
+    _check(_lib.mcpt_load_pbrt(p, max_prims, _ptr(hot), _ptr(cold), _ptr(sph), _ptr(mats), C.byref(info)),
+           "mcpt_load_pbrt")
+    images = []
+    for k in range(info.textures_loaded):
+        w, h = C.c_int32(), C.c_int32()
+        _check(_lib.mcpt_pbrt_texture(k, C.byref(w), C.byref(h), None), "mcpt_pbrt_texture")
+        img = np.zeros((h.value, w.value, 4), np.uint8)
+        _check(_lib.mcpt_pbrt_texture(k, C.byref(w), C.byref(h), _ptr(img)), "mcpt_pbrt_texture")
+        images.append(img)
+    return hot, cold, sph, mats, info.as_dict(), images
 
 
 def load_ppm(path):

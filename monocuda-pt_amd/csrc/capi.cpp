@@ -551,6 +551,15 @@ int mcpt_obj_texture(int32_t index, int32_t* width, int32_t* height, uint8_t* rg
     return obj_texture(index, width, height, rgba8);
 }
 
+int mcpt_load_pbrt(const char* path, int64_t max_prims, mcpt_tri_hot* hot, mcpt_tri_cold* cold, mcpt_sphere* spheres,
+                   mcpt_material* materials, mcpt_pbrt_info* info) {
+    return load_pbrt(path, max_prims, hot, cold, spheres, materials, info);
+}
+
+int mcpt_pbrt_texture(int32_t index, int32_t* width, int32_t* height, uint8_t* rgba8) {
+    return pbrt_texture(index, width, height, rgba8);
+}
+
 int mcpt_gen_camera(int32_t kind, int32_t width, int32_t height, mcpt_camera_desc* out) {
     return gen_camera(kind, width, height, out);
 }
