@@ -560,9 +560,14 @@ int mcpt_load_ppm(const char* path, int32_t* width, int32_t* height, uint8_t* rg
  * mcpt_load_hdr: Radiance RGBE, the file type `--env` reads with
  * loadImageRgbaFloat (src/image_io.cpp:41-60) and scripts/hdr2pfm.py:14-61
  * converts: "-Y h +X w" only, flat or new-style run-length rows, each texel
- * byte * 2^(e - 136), 0 when e = 0. OpenEXR is not read. */
+ * byte * 2^(e - 136), 0 when e = 0.
+ * mcpt_load_exr: OpenEXR, the other file type loadImageRgbaFloat reads
+ * (src/image_io.cpp:41-52, tinyexr's LoadEXR): single-part scanline files,
+ * UINT / HALF / FLOAT channels, NONE / RLE / ZIPS / ZIP compression; R, G,
+ * B, A by name (A = 1 when absent, one channel repeated in R, G, B). */
 int mcpt_load_pfm(const char* path, int32_t* width, int32_t* height, float* rgba);
 int mcpt_load_hdr(const char* path, int32_t* width, int32_t* height, float* rgba);
+int mcpt_load_exr(const char* path, int32_t* width, int32_t* height, float* rgba);
 
 /* ------------------------------------------------------------ OBJ import
  * Wavefront OBJ + MTL into triangle records (loadObject / loadMtlFile,

@@ -274,6 +274,7 @@ _SIGS = {
     "mcpt_load_ppm": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_pfm": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_hdr": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
+    "mcpt_load_exr": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_obj_texture": ([C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_pbrt": ([C.c_char_p, C.c_int64, VP, VP, VP, VP, C.POINTER(PbrtInfo)], C.c_int),
     "mcpt_pbrt_texture": ([C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
@@ -442,15 +443,23 @@ def load_hdr(path):
     return _load_float_image(_lib.mcpt_load_hdr, "mcpt_load_hdr", path)
 
 
+def load_exr(path):
+    """mcpt_load_exr: a scanline OpenEXR file as an (H, W, 4) RGBA FP32 array."""
+    return _load_float_image(_lib.mcpt_load_exr, "mcpt_load_exr", path)
+
+
 def load_env(path):
     """An environment map file by its extension, as the reference's --env
-    option chooses (src/path_tracer.cu:4197-4199): .pfm or .hdr."""
+    option chooses (src/path_tracer.cu:4197-4199, src/image_io.cpp:41-60):
+    .pfm, .exr or .hdr."""
     low = os.fspath(path).lower()
     if low.endswith(".pfm"):
         return load_pfm(path)
+    if low.endswith(".exr"):
+        return load_exr(path)
     if low.endswith(".hdr"):
         return load_hdr(path)
-    raise ValueError(f"{path}: environment maps are read from .pfm or .hdr files (OpenEXR is not read)")
+    raise ValueError(f"{path}: environment maps are read from .pfm, .exr or .hdr files")
 
 
 def write_png(path, rgb8):

@@ -377,6 +377,10 @@ int mcpt_load_hdr(const char* path, int32_t* width, int32_t* height, float* rgba
     return load_env_file(read_hdr, path, width, height, rgba);
 }
 
+int mcpt_load_exr(const char* path, int32_t* width, int32_t* height, float* rgba) {
+    return load_env_file(read_exr, path, width, height, rgba);
+}
+
 int mcpt_scene_destroy(mcpt_scene scene) {
     if (!scene) return MCPT_OK;
     hipSetDevice(scene->ctx->device);

@@ -19,14 +19,27 @@
 //     count - 128 times, otherwise count literal bytes follow) when
 //     8 <= width < 32768, or width flat RGBE quadruples; each texel is
 //     byte * 2^(e - 136) for e > 0 and 0 for e = 0, alpha 1.
+//   * OpenEXR environment maps, which the reference decodes with tinyexr
+//     (github.com/syoyo/tinyexr at e48a52f, LoadEXR) into RGBA FP32. Read
+//     here from the published OpenEXR file layout: single-part scanline
+//     files; channels of type UINT, HALF or FLOAT; compression NONE, RLE,
+//     ZIPS or ZIP (zlib, then the byte predictor and the two-half
+//     interleave undone); pixels placed by their data-window row, so either
+//     line order reads the same. R, G, B and A are taken by name (A = 1 when
+//     absent); a file with one channel repeats it in R, G and B. Tiled,
+//     deep and multi-part files and the PIZ, PXR24, B44 and DWA
+//     compressions are refused.
 // Every size is checked against the bytes the file holds before anything
 // of that size is allocated.
+#include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
+
+#include <zlib.h>
 
 #include "host_common.hpp"
 
@@ -216,6 +229,193 @@ int read_hdr(const char* path, int& w, int& h, std::vector<float>* rgba) {
             dst[4 * x + 1] = (float)q[1] * s;
             dst[4 * x + 2] = (float)q[2] * s;
             dst[4 * x + 3] = 1.0f;
+        }
+    }
+    return MCPT_OK;
+}
+
+namespace {
+
+float half_to_float(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h >> 15) << 31, e = (h >> 10) & 31u, m = h & 1023u;
+    uint32_t bits;
+    if (e == 0) {
+        if (m == 0) {
+            bits = sign;
+        } else {  // subnormal: m * 2^-24
+            const float v = (float)m * (1.0f / 16777216.0f);
+            uint32_t b;
+            std::memcpy(&b, &v, 4);
+            bits = sign | b;
+        }
+    } else if (e == 31) {
+        bits = sign | 0x7f800000u | (m << 13);
+    } else {
+        bits = sign | ((e + 112u) << 23) | (m << 13);
+    }
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+
+// undo the ZIP/RLE byte predictor, then interleave the two halves
+void exr_unpredict(std::vector<uint8_t>& t) {
+    for (size_t i = 1; i < t.size(); i++) t[i] = (uint8_t)(t[i - 1] + t[i] - 128);
+    std::vector<uint8_t> out(t.size());
+    const size_t half = (t.size() + 1) / 2;
+    for (size_t i = 0; i < t.size(); i++) out[i] = (i & 1) ? t[half + i / 2] : t[i / 2];
+    t.swap(out);
+}
+
+}  // namespace
+
+int read_exr(const char* path, int& w, int& h, std::vector<float>* rgba) {
+    File file(path);
+    if (!file.f) return set_error(MCPT_ERR_INVALID, "cannot open '%s'", path);
+    const long long size = file.remaining();
+    if (size < 8) return set_error(MCPT_ERR_INVALID, "'%s' is not an OpenEXR file", path);
+    std::vector<uint8_t> d((size_t)size);
+    if (std::fread(d.data(), 1, d.size(), file.f) != d.size()) return set_error(MCPT_ERR_INVALID, "'%s': short read", path);
+    auto u32 = [&](size_t at) {
+        uint32_t v;
+        std::memcpy(&v, &d[at], 4);
+        return v;
+    };
+    if (u32(0) != 20000630u) return set_error(MCPT_ERR_INVALID, "'%s' is not an OpenEXR file", path);
+    const uint32_t ver = u32(4);
+    if ((ver & 0xff) != 2 || (ver & (0x200u | 0x800u | 0x1000u)))
+        return set_error(MCPT_ERR_INVALID, "'%s': tiled, deep or multi-part OpenEXR files are not read", path);
+    struct Chan {
+        std::string name;
+        int type, xs, ys;
+    };
+    std::vector<Chan> ch;
+    int comp = -1, xmin = 0, ymin = 0, xmax = -1, ymax = -1;
+    size_t at = 8;
+    auto cstr = [&](std::string& out) {
+        const void* z = at < d.size() ? std::memchr(&d[at], 0, d.size() - at) : nullptr;
+        if (!z) return false;
+        const size_t e = (size_t)((const uint8_t*)z - d.data());
+        out.assign((const char*)&d[at], e - at);
+        at = e + 1;
+        return true;
+    };
+    for (;;) {
+        std::string name, type;
+        if (!cstr(name)) return set_error(MCPT_ERR_INVALID, "'%s': the header does not end", path);
+        if (name.empty()) break;
+        if (!cstr(type) || at + 4 > d.size()) return set_error(MCPT_ERR_INVALID, "'%s': bad header", path);
+        const uint32_t len = u32(at);
+        at += 4;
+        if (at + len > d.size()) return set_error(MCPT_ERR_INVALID, "'%s': bad header", path);
+        const size_t end = at + len;
+        if (name == "channels" && type == "chlist") {
+            while (at < end && d[at] != 0) {
+                Chan c;
+                if (!cstr(c.name) || at + 16 > end) return set_error(MCPT_ERR_INVALID, "'%s': bad channel list", path);
+                c.type = (int)u32(at);
+                c.xs = (int)u32(at + 8);
+                c.ys = (int)u32(at + 12);
+                at += 16;
+                if (c.type < 0 || c.type > 2 || c.xs != 1 || c.ys != 1)
+                    return set_error(MCPT_ERR_INVALID, "'%s': channel %s is subsampled or of unknown type", path,
+                                     c.name.c_str());
+                ch.push_back(c);
+            }
+        } else if (name == "compression" && len >= 1) {
+            comp = d[at];
+        } else if (name == "dataWindow" && len >= 16) {
+            xmin = (int)u32(at), ymin = (int)u32(at + 4), xmax = (int)u32(at + 8), ymax = (int)u32(at + 12);
+        }
+        at = end;
+    }
+    const long long W = (long long)xmax - xmin + 1, H = (long long)ymax - ymin + 1;
+    if (ch.empty() || W < 1 || H < 1 || W * H > kMaxPixels)
+        return set_error(MCPT_ERR_INVALID, "'%s': no channels or a bad data window", path);
+    const int lines = comp == 0 || comp == 1 || comp == 2 ? 1 : comp == 3 ? 16 : 0;
+    if (lines == 0) return set_error(MCPT_ERR_INVALID, "'%s': compression %d is not read (NONE, RLE, ZIPS, ZIP are)", path, comp);
+    w = (int)W;
+    h = (int)H;
+    if (!rgba) return MCPT_OK;
+    // channels are stored in name order; each line holds every channel in turn
+    std::vector<int> order(ch.size());
+    for (size_t i = 0; i < ch.size(); i++) order[i] = (int)i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return ch[a].name < ch[b].name; });
+    int slot[4] = {-1, -1, -1, -1};  // channel feeding R, G, B, A
+    const char* want[4] = {"R", "G", "B", "A"};
+    for (int k = 0; k < 4; k++)
+        for (size_t i = 0; i < ch.size(); i++)
+            if (ch[i].name == want[k]) slot[k] = (int)i;
+    if (ch.size() == 1 && slot[0] < 0) slot[0] = slot[1] = slot[2] = 0;
+    size_t line_bytes = 0;
+    for (const Chan& c : ch) line_bytes += (size_t)W * (c.type == 1 ? 2 : 4);
+    const long long chunks = (H + lines - 1) / lines;
+    if (at + (size_t)chunks * 8 > d.size()) return set_error(MCPT_ERR_INVALID, "'%s': truncated offset table", path);
+    rgba->assign((size_t)W * H * 4, 0.0f);
+    for (long long p = 0; p < W * H; p++) (*rgba)[p * 4 + 3] = 1.0f;
+    std::vector<uint8_t> buf;
+    for (long long k = 0; k < chunks; k++) {
+        uint64_t off;
+        std::memcpy(&off, &d[at + k * 8], 8);
+        if (off + 8 > d.size()) return set_error(MCPT_ERR_INVALID, "'%s': bad chunk offset", path);
+        const int y0 = (int)u32(off);
+        const uint32_t n = u32(off + 4);
+        if (off + 8 + n > d.size()) return set_error(MCPT_ERR_INVALID, "'%s': truncated chunk", path);
+        const long long first = (long long)y0 - ymin;
+        if (first < 0 || first >= H) return set_error(MCPT_ERR_INVALID, "'%s': chunk outside the data window", path);
+        const long long nl = std::min<long long>(lines, H - first);
+        const size_t raw = (size_t)nl * line_bytes;
+        const uint8_t* src = &d[off + 8];
+        if (n == raw) {  // stored as is (also the fallback of a block that would not shrink)
+            buf.assign(src, src + raw);
+        } else if (comp == 2 || comp == 3) {
+            buf.resize(raw);
+            uLongf got = (uLongf)raw;
+            if (uncompress(buf.data(), &got, src, n) != Z_OK || got != raw)
+                return set_error(MCPT_ERR_INVALID, "'%s': bad ZIP block", path);
+            exr_unpredict(buf);
+        } else if (comp == 1) {
+            buf.clear();
+            for (uint32_t i = 0; i < n;) {
+                const int c = (int8_t)src[i++];
+                if (c < 0) {
+                    if (i + (uint32_t)(-c) > n) return set_error(MCPT_ERR_INVALID, "'%s': bad RLE block", path);
+                    buf.insert(buf.end(), src + i, src + i - c);
+                    i += (uint32_t)(-c);
+                } else {
+                    if (i >= n) return set_error(MCPT_ERR_INVALID, "'%s': bad RLE block", path);
+                    buf.insert(buf.end(), (size_t)c + 1, src[i++]);
+                }
+                if (buf.size() > raw) return set_error(MCPT_ERR_INVALID, "'%s': bad RLE block", path);
+            }
+            if (buf.size() != raw) return set_error(MCPT_ERR_INVALID, "'%s': bad RLE block", path);
+            exr_unpredict(buf);
+        } else {
+            return set_error(MCPT_ERR_INVALID, "'%s': chunk size %u, expected %zu", path, n, raw);
+        }
+        size_t q = 0;
+        for (long long l = 0; l < nl; l++) {
+            const long long row = first + l;
+            for (int oi : order) {
+                const Chan& c = ch[oi];
+                const int sz = c.type == 1 ? 2 : 4;
+                for (long long x = 0; x < W; x++, q += sz) {
+                    float v;
+                    if (c.type == 1) {
+                        uint16_t hv;
+                        std::memcpy(&hv, &buf[q], 2);
+                        v = half_to_float(hv);
+                    } else if (c.type == 2) {
+                        std::memcpy(&v, &buf[q], 4);
+                    } else {
+                        uint32_t uv;
+                        std::memcpy(&uv, &buf[q], 4);
+                        v = (float)uv;
+                    }
+                    for (int k2 = 0; k2 < 4; k2++)
+                        if (slot[k2] == oi) (*rgba)[((size_t)row * W + x) * 4 + k2] = v;
+                }
+            }
         }
     }
     return MCPT_OK;

@@ -9,6 +9,8 @@ RGBE texel decodes to mantissa * 2^(e - 136) exactly (0 for e = 0), for
 run-length and flat rows alike. The reference's own texture asset
 (models/earth.ppm) is compared with an independent parse when present."""
 import os
+import struct
+import zlib
 
 import numpy as np
 import pytest
@@ -131,7 +133,123 @@ def test_hdr_refused(mcpt, tmp_path, case):
 
 def test_env_extension(mcpt, tmp_path):
     with pytest.raises(ValueError):
-        mcpt.load_env(str(tmp_path / "sky.exr"))
+        mcpt.load_env(str(tmp_path / "sky.tga"))
+
+
+# ------------------------------------------------------------- OpenEXR
+# A writer of the published OpenEXR scanline layout, for the reader's known
+# answers: magic, version 2, the header attributes, the chunk offset table,
+# and per chunk its first row, its size and the rows (channels in name
+# order); ZIP / ZIPS blocks are the rows with their bytes split into even
+# and odd halves, delta-coded (+128) and deflated, RLE blocks the same
+# delta-coded bytes in signed-count runs.
+EXR_TYPES = {0: np.uint32, 1: np.float16, 2: np.float32}
+
+
+def _exr_predict(raw):
+    b = np.frombuffer(raw, np.uint8)
+    t = np.concatenate([b[0::2], b[1::2]]).astype(np.int32)
+    d = t.copy()
+    d[1:] = (t[1:] - t[:-1] + 128) & 255
+    return d.astype(np.uint8).tobytes()
+
+
+def _exr_rle(b):
+    out, i = bytearray(), 0
+    while i < len(b):
+        j = i
+        while j < len(b) and j - i < 128 and b[j] == b[i]:
+            j += 1
+        if j - i >= 3:
+            out += struct.pack("<bB", j - i - 1, b[i])
+            i = j
+            continue
+        j = i + 1
+        while j < len(b) and j - i < 127 and not (j + 2 < len(b) and b[j] == b[j + 1] == b[j + 2]):
+            j += 1
+        out += struct.pack("<b", -(j - i)) + b[i:j]
+        i = j
+    return bytes(out)
+
+
+def _exr(path, planes, comp, ymin=0, decreasing=False):
+    """planes: {name: (type, (H, W) array)}"""
+    names = sorted(planes)
+    H, W = planes[names[0]][1].shape
+
+    def attr(n, t, v):
+        return n.encode() + b"\0" + t.encode() + b"\0" + struct.pack("<i", len(v)) + v
+
+    chl = b"".join(n.encode() + b"\0" + struct.pack("<iB3xii", planes[n][0], 0, 1, 1) for n in names) + b"\0"
+    box = struct.pack("<iiii", 0, ymin, W - 1, ymin + H - 1)
+    hdr = (attr("channels", "chlist", chl) + attr("compression", "compression", bytes([comp])) +
+           attr("dataWindow", "box2i", box) + attr("displayWindow", "box2i", box) +
+           attr("lineOrder", "lineOrder", bytes([1 if decreasing else 0])) +
+           attr("pixelAspectRatio", "float", struct.pack("<f", 1)) +
+           attr("screenWindowCenter", "v2f", struct.pack("<ff", 0, 0)) +
+           attr("screenWindowWidth", "float", struct.pack("<f", 1)) + b"\0")
+    lines = 16 if comp == 3 else 1
+    chunks = []
+    for y0 in range(0, H, lines):
+        rows = range(y0, min(H, y0 + lines))
+        raw = b"".join(np.ascontiguousarray(planes[n][1][y], EXR_TYPES[planes[n][0]]).tobytes()
+                       for y in rows for n in names)
+        if comp in (2, 3):
+            pay = zlib.compress(_exr_predict(raw))
+        elif comp == 1:
+            pay = _exr_rle(_exr_predict(raw))
+        else:
+            pay = raw
+        if len(pay) >= len(raw):
+            pay = raw
+        chunks.append((y0, struct.pack("<ii", ymin + y0, len(pay)) + pay))
+    order = chunks[::-1] if decreasing else chunks
+    base = 8 + len(hdr) + 8 * len(chunks)
+    offs, body, at = {}, b"", base
+    for y0, c in order:
+        offs[y0] = at
+        body += c
+        at += len(c)
+    table = b"".join(struct.pack("<Q", offs[y0]) for y0, _ in chunks)
+    path.write_bytes(struct.pack("<ii", 20000630, 2) + hdr + table + body)
+    return str(path)
+
+
+@pytest.mark.parametrize("comp", [0, 1, 2, 3])
+def test_exr_decode(mcpt, tmp_path, comp):
+    rng = np.random.default_rng(comp)
+    H, W = 37, 23
+    r = rng.uniform(0, 50, (H, W)).astype(np.float16)
+    r[0, :4] = [np.float16(6e-8), np.float16(np.inf), np.float16(65504), np.float16(-2.5)]  # subnormal, inf, max
+    g = rng.uniform(0, 5, (H, W)).astype(np.float32)
+    b = np.repeat(rng.uniform(0, 1, (H, 1)), W, 1).astype(np.float16)  # runs for RLE
+    a = rng.integers(0, 1000, (H, W)).astype(np.uint32)
+    planes = {"R": (1, r), "G": (2, g), "B": (1, b), "A": (0, a)}
+    got = mcpt.load_exr(_exr(tmp_path / "a.exr", planes, comp, ymin=-5, decreasing=comp == 2))
+    assert got.shape == (H, W, 4)
+    assert np.array_equal(got[..., 0], r.astype(np.float32))
+    assert np.array_equal(got[..., 1], g) and np.array_equal(got[..., 2], b.astype(np.float32))
+    assert np.array_equal(got[..., 3], a.astype(np.float32))
+    assert np.array_equal(mcpt.load_env(str(tmp_path / "a.exr")), got)
+
+
+def test_exr_channels_and_refusals(mcpt, tmp_path):
+    y = np.arange(12, dtype=np.float16).reshape(3, 4)
+    got = mcpt.load_exr(_exr(tmp_path / "y.exr", {"Y": (1, y)}, 3))
+    assert np.array_equal(got[..., 0], y.astype(np.float32)) and np.array_equal(got[..., 2], got[..., 0])
+    assert np.all(got[..., 3] == 1)
+    rgb = {"R": (2, np.ones((2, 2), np.float32)), "G": (2, np.zeros((2, 2), np.float32)),
+           "B": (2, np.ones((2, 2), np.float32))}
+    assert np.all(mcpt.load_exr(_exr(tmp_path / "rgb.exr", rgb, 0))[..., 3] == 1)  # no A: 1
+    p = _exr(tmp_path / "piz.exr", rgb, 0)
+    data = bytearray(open(p, "rb").read())
+    at = data.index(b"compression\0compression\0") + len(b"compression\0compression\0") + 4
+    data[at] = 4  # PIZ
+    open(p, "wb").write(bytes(data))
+    with pytest.raises(mcpt.McptError):
+        mcpt.load_exr(p)
+    with pytest.raises(mcpt.McptError):
+        mcpt.load_exr(_pfm(tmp_path / "not.exr", np.ones((2, 2, 4), np.float32)))
 
 
 EARTH = "/root/reference/models/earth.ppm"
