@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <string>
 
 #include "scene.hpp"
@@ -132,6 +133,19 @@ using namespace mcpt;
     do {                                                              \
         if (!(cond)) return set_error(MCPT_ERR_INVALID, "%s", msg);   \
     } while (0)
+
+// The file readers run C++ library code that can throw (out of memory on
+// a huge but consistent file); nothing may cross the C boundary.
+template <class F>
+static int guarded(const char* what, F f) {
+    try {
+        return f();
+    } catch (const std::exception& e) {
+        return set_error(MCPT_ERR_INVALID, "%s: %s", what, e.what());
+    } catch (...) {
+        return set_error(MCPT_ERR_INVALID, "%s failed", what);
+    }
+}
 
 extern "C" {
 
@@ -345,7 +359,13 @@ int mcpt_scene_add_texture(mcpt_scene scene, const mcpt_texture_desc* d, int32_t
     return MCPT_OK;
 }
 
+static int load_ppm_file(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8);
+
 int mcpt_load_ppm(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8) {
+    return guarded("mcpt_load_ppm", [&] { return load_ppm_file(path, width, height, rgba8); });
+}
+
+static int load_ppm_file(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8) {
     CHECK_ARG(path && width && height, "bad arguments");
     int w = 0, h = 0;
     std::vector<uint8_t> px;
@@ -370,15 +390,15 @@ static int load_env_file(int (*reader)(const char*, int&, int&, std::vector<floa
 }
 
 int mcpt_load_pfm(const char* path, int32_t* width, int32_t* height, float* rgba) {
-    return load_env_file(read_pfm, path, width, height, rgba);
+    return guarded("mcpt_load_pfm", [&] { return load_env_file(read_pfm, path, width, height, rgba); });
 }
 
 int mcpt_load_hdr(const char* path, int32_t* width, int32_t* height, float* rgba) {
-    return load_env_file(read_hdr, path, width, height, rgba);
+    return guarded("mcpt_load_hdr", [&] { return load_env_file(read_hdr, path, width, height, rgba); });
 }
 
 int mcpt_load_exr(const char* path, int32_t* width, int32_t* height, float* rgba) {
-    return load_env_file(read_exr, path, width, height, rgba);
+    return guarded("mcpt_load_exr", [&] { return load_env_file(read_exr, path, width, height, rgba); });
 }
 
 int mcpt_scene_destroy(mcpt_scene scene) {
@@ -548,7 +568,7 @@ int mcpt_gen_sky(int32_t width, int32_t height, float* rgba) { return gen_sky(wi
 
 int mcpt_load_obj(const char* path, const mcpt_obj_opts* opts, mcpt_tri_hot* hot, mcpt_tri_cold* cold,
                   mcpt_material* materials, mcpt_obj_stats* stats) {
-    return load_obj(path, opts, hot, cold, materials, stats);
+    return guarded("mcpt_load_obj", [&] { return load_obj(path, opts, hot, cold, materials, stats); });
 }
 
 int mcpt_obj_texture(int32_t index, int32_t* width, int32_t* height, uint8_t* rgba8) {
@@ -557,7 +577,7 @@ int mcpt_obj_texture(int32_t index, int32_t* width, int32_t* height, uint8_t* rg
 
 int mcpt_load_pbrt(const char* path, int64_t max_prims, mcpt_tri_hot* hot, mcpt_tri_cold* cold, mcpt_sphere* spheres,
                    mcpt_material* materials, mcpt_pbrt_info* info) {
-    return load_pbrt(path, max_prims, hot, cold, spheres, materials, info);
+    return guarded("mcpt_load_pbrt", [&] { return load_pbrt(path, max_prims, hot, cold, spheres, materials, info); });
 }
 
 int mcpt_pbrt_texture(int32_t index, int32_t* width, int32_t* height, uint8_t* rgba8) {

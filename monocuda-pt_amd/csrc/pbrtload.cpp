@@ -353,10 +353,11 @@ struct Parser {
         while (i < t.size() && t[i].kind == Tok::Str) {
             const std::string decl = t[i].text;
             const size_t sp = decl.find_first_of(" \t");
-            if (sp == std::string::npos) return true;  // a plain string: the next directive's argument
+            const size_t nm = sp == std::string::npos ? sp : decl.find_first_not_of(" \t", sp);
+            if (nm == std::string::npos) return true;  // a plain string: the next directive's argument
             Param q;
             q.type = decl.substr(0, sp);
-            q.name = decl.substr(decl.find_first_not_of(" \t", sp));
+            q.name = decl.substr(nm);
             i++;
             auto take = [&](const Tok& v) {
                 if (v.kind == Tok::Num) q.nums.push_back(v.num);
