@@ -547,6 +547,29 @@ typedef struct mcpt_texture_desc {
  * `texture = k`. A render refuses materials that name a texture not added. */
 int mcpt_scene_add_texture(mcpt_scene scene, const mcpt_texture_desc* desc, int32_t* index);
 
+/* Heterogeneous medium: a dense density grid, 48 bytes. Voxel (i, j, k)
+ * holds density[(k * dims[1] + j) * dims[0] + i] at world position
+ * origin + (i, j, k) * voxel_size; between voxels the density is trilinear
+ * and outside the grid it is 0. It stands in for the reference's NanoVDB
+ * float grid (VolumeGrid, src/volume_io.h:7-12; sampleVolumeDensity,
+ * src/path_tracer.cu:784-789), stored dense rather than as a sparse tree. */
+typedef struct mcpt_density_grid {
+    int32_t dims[3];      /* voxels along x, y, z (>= 1)                   */
+    float origin[3];      /* world position of voxel (0, 0, 0)             */
+    float voxel_size[3];  /* world size of a voxel along x, y, z (> 0)     */
+    int32_t reserved;
+    const float* density; /* host values, finite and >= 0                  */
+} mcpt_density_grid;
+
+/* Gives every isotropic material of the scene the density grid, as the
+ * reference does for a loaded volume (src/path_tracer.cu:2509-2519): such a
+ * medium's extinction at p is material.density * grid(p); free flight is
+ * delta tracking (:796-812) and shadow rays ratio tracking (:817-835)
+ * against the majorant material.density * max(grid). *max_density (may be
+ * NULL) receives max(grid). Replaces an earlier grid; not allowed while a
+ * server holds the scene. */
+int mcpt_scene_set_density_grid(mcpt_scene scene, const mcpt_density_grid* grid, float* max_density);
+
 /* loadPpmAsRgba8 (src/path_tracer.cu:236-294): a binary PPM (P6, maxval
  * 255, '#' comments in the header) as RGBA8 with alpha 255. Two-call
  * pattern: rgba8 NULL to get the size. */

@@ -51,6 +51,24 @@ MCPT_PCG_HD void mcpt_pcg32_for_path(mcpt_pcg32* rng, uint32_t pixel, uint32_t s
     mcpt_pcg32_seed(rng, ((uint64_t)sample << 32) ^ seed, (uint64_t)pixel);
 }
 
+/* SplitMix64's output function (Steele, Lea and Flood 2014), used to hash
+ * words into a seed. */
+MCPT_PCG_HD uint64_t mcpt_mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+/* The stream a shadow ray's ratio tracking draws from, keyed by the bits
+ * of the ray itself (origin, direction, length): a path's shadow ray is
+ * the same bits whether it is traced inline or from a queue, so every
+ * execution mode draws the same numbers without storing a key. */
+MCPT_PCG_HD void mcpt_pcg32_for_shadow(mcpt_pcg32* rng, const uint32_t bits[7]) {
+    uint64_t h = 0x6a09e667f3bcc909ULL;
+    for (int k = 0; k < 7; k++) h = mcpt_mix64(h + 0x9e3779b97f4a7c15ULL + bits[k]);
+    mcpt_pcg32_seed(rng, h, mcpt_mix64(h ^ 0x3c6ef372fe94f82bULL));
+}
+
 MCPT_PCG_HD float mcpt_pcg32_uniform(mcpt_pcg32* rng) {
     return (float)(mcpt_pcg32_next(rng) >> 8) * 5.9604644775390625e-08f;
 }

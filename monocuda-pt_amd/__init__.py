@@ -140,6 +140,24 @@ class PbrtInfo(C.Structure):
         return out
 
 
+class DensityGrid(C.Structure):
+    """mcpt_density_grid (48 bytes): dense voxels, x fastest."""
+    _fields_ = [("dims", C.c_int32 * 3), ("origin", C.c_float * 3), ("voxel_size", C.c_float * 3),
+                ("reserved", C.c_int32), ("density", C.c_void_p)]
+
+
+def _density_grid_desc(density, origin, voxel_size):
+    keep = np.ascontiguousarray(density, np.float32)
+    if keep.ndim != 3:
+        raise ValueError("density must be a (nz, ny, nx) array")
+    g = DensityGrid()
+    g.dims[:] = [keep.shape[2], keep.shape[1], keep.shape[0]]
+    g.origin[:] = [float(x) for x in origin]
+    g.voxel_size[:] = [float(x) for x in (voxel_size if np.ndim(voxel_size) else (voxel_size,) * 3)]
+    g.density = keep.ctypes.data
+    return g, keep
+
+
 class TextureDesc(C.Structure):
     _fields_ = [("type", C.c_int32), ("color_a", C.c_float * 3), ("color_b", C.c_float * 3), ("scale", C.c_float),
                 ("width", C.c_int32), ("height", C.c_int32), ("rgba8", C.c_void_p)]
@@ -271,6 +289,7 @@ _SIGS = {
     "mcpt_gen_scene": ([C.c_int32, C.c_uint64, C.c_int64, VP, VP, VP, C.POINTER(GenSizes)], C.c_int),
     "mcpt_load_obj": ([C.c_char_p, C.POINTER(ObjOpts), VP, VP, VP, C.POINTER(ObjStats)], C.c_int),
     "mcpt_scene_add_texture": ([VP, C.POINTER(TextureDesc), C.POINTER(C.c_int32)], C.c_int),
+    "mcpt_scene_set_density_grid": ([VP, C.POINTER(DensityGrid), C.POINTER(C.c_float)], C.c_int),
     "mcpt_load_ppm": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_pfm": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_hdr": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
@@ -619,6 +638,16 @@ class Scene:
         k = C.c_int32()
         _check(_lib.mcpt_scene_add_texture(self.h, C.byref(d), C.byref(k)), "mcpt_scene_add_texture")
         return k.value
+
+    def set_density_grid(self, density, origin, voxel_size):
+        """mcpt_scene_set_density_grid: density is a (nz, ny, nx) float32
+        array whose voxel [k, j, i] sits at origin + (i, j, k) * voxel_size;
+        every isotropic material becomes a heterogeneous medium. Returns
+        the grid's largest value."""
+        g, _keep = _density_grid_desc(density, origin, voxel_size)
+        mx = C.c_float()
+        _check(_lib.mcpt_scene_set_density_grid(self.h, C.byref(g), C.byref(mx)), "mcpt_scene_set_density_grid")
+        return mx.value
 
     def lights(self):
         n = C.c_int32()

@@ -359,6 +359,52 @@ int mcpt_scene_add_texture(mcpt_scene scene, const mcpt_texture_desc* d, int32_t
     return MCPT_OK;
 }
 
+int mcpt_scene_set_density_grid(mcpt_scene scene, const mcpt_density_grid* g, float* max_density) {
+    CHECK_ARG(scene && g, "bad arguments");
+    if (scene->servers.load() > 0) return set_error(MCPT_ERR_INVALID, "the scene is in use by a server");
+    const int nx = g->dims[0], ny = g->dims[1], nz = g->dims[2];
+    CHECK_ARG(nx >= 1 && ny >= 1 && nz >= 1 && g->density, "a density grid needs voxels");
+    const size_t n = (size_t)nx * (size_t)ny * (size_t)nz;
+    CHECK_ARG(n <= ((size_t)1 << 34), "density grid too large");
+    float inv[3];
+    for (int k = 0; k < 3; k++) {
+        CHECK_ARG(std::isfinite(g->origin[k]) && std::isfinite(g->voxel_size[k]) && g->voxel_size[k] > 0.0f,
+                  "bad density grid origin or voxel size");
+        inv[k] = 1.0f / g->voxel_size[k];
+        CHECK_ARG(std::isfinite(inv[k]) && inv[k] > 0.0f, "bad density grid voxel size");
+    }
+    float mx = 0.0f;  // the grid's largest value: the majorant's grid part
+    for (size_t i = 0; i < n; i++) {
+        const float v = g->density[i];
+        if (!(std::isfinite(v) && v >= 0.0f))
+            return set_error(MCPT_ERR_INVALID, "density grid value %zu is negative or not finite", i);
+        mx = std::max(mx, v);
+    }
+    Ctx* c = scene->ctx;
+    MCPT_HIP(hipSetDevice(c->device));
+    DevBuf<float> buf;
+    MCPT_TRY(upload(c, buf, g->density, n));
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        buf.release(c);
+        return set_error(MCPT_ERR_HIP, "density grid upload failed");
+    }
+    scene->grid.release(c);
+    scene->grid = buf;
+    GridView& v = scene->grid_view;
+    v.nx = nx;
+    v.ny = ny;
+    v.nz = nz;
+    v.ox = g->origin[0];
+    v.oy = g->origin[1];
+    v.oz = g->origin[2];
+    v.ix = inv[0];
+    v.iy = inv[1];
+    v.iz = inv[2];
+    v.max_density = mx;
+    if (max_density) *max_density = mx;
+    return MCPT_OK;
+}
+
 static int load_ppm_file(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8);
 
 int mcpt_load_ppm(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8) {

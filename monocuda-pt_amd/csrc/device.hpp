@@ -338,6 +338,18 @@ struct TexView {
     const uint32_t* px;  // image RGBA8, top row first
 };
 
+// Heterogeneous medium: a dense density grid, x fastest, whose voxel
+// (i, j, k) sits at world position origin + (i, j, k) * voxel size; the
+// reference samples a NanoVDB float grid the same way (sampleVolumeDensity,
+// src/path_tracer.cu:784-789). data == nullptr: every medium is homogeneous.
+struct GridView {
+    const float* data;
+    int nx, ny, nz;
+    float ox, oy, oz;  // world position of voxel (0, 0, 0)
+    float ix, iy, iz;  // 1 / voxel size
+    float max_density;
+};
+
 struct SceneView {
     BvhView bvh;
     const float* hot;           // 9 floats per triangle, original order
@@ -353,7 +365,34 @@ struct SceneView {
     int n_spheres;
     const TexView* tex;   // the scene's textures
     const float* srgb8;   // sRGB byte -> linear, 256 entries
+    GridView grid;        // density grid of every isotropic material
 };
+
+// Trilinear density at world point p (NanoVDB's TrilinearSampler over
+// worldToIndexF): index coordinates q = (p - origin) / voxel, the eight
+// voxels around floor(q), voxels outside the grid read 0 (the background of
+// a fog volume), interpolated along z, then y, then x as a + w (b - a).
+MCPT_DEV float grid_voxel(const GridView& g, int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return 0.0f;
+    return g.data[((size_t)k * g.ny + j) * g.nx + i];
+}
+MCPT_DEV float grid_density(const GridView& g, const vec3& p) {
+    const float x = (p.x - g.ox) * g.ix, y = (p.y - g.oy) * g.iy, z = (p.z - g.oz) * g.iz;
+    // all eight voxels outside (or a non-finite point): background
+    if (!(x > -1.0f && y > -1.0f && z > -1.0f && x < (float)g.nx && y < (float)g.ny && z < (float)g.nz)) return 0.0f;
+    const float x0 = floorf(x), y0 = floorf(y), z0 = floorf(z);
+    const float u = x - x0, v = y - y0, w = z - z0;
+    const int i = (int)x0, j = (int)y0, k = (int)z0;
+    float a[2][2];
+    for (int di = 0; di < 2; di++)
+        for (int dj = 0; dj < 2; dj++) {
+            const float c0 = grid_voxel(g, i + di, j + dj, k), c1 = grid_voxel(g, i + di, j + dj, k + 1);
+            a[di][dj] = c0 + w * (c1 - c0);
+        }
+    const float b0 = a[0][0] + v * (a[0][1] - a[0][0]);
+    const float b1 = a[1][0] + v * (a[1][1] - a[1][0]);
+    return b0 + u * (b1 - b0);
+}
 
 // sampleTexture (src/path_tracer.cu:216-234). The image filter restates the
 // texture unit's normalised-coordinate bilinear read with wrap addressing in

@@ -47,6 +47,9 @@ struct mcpt_scene_s {
     std::vector<mcpt::DevBuf<uint32_t>> tex_px;
     mcpt::DevBuf<float> srgb8;
     int max_material_texture = 0;
+    // density grid of every isotropic material (mcpt_scene_set_density_grid)
+    mcpt::DevBuf<float> grid;
+    mcpt::GridView grid_view{};
     // servers holding views of this scene: it may not be rebuilt or get
     // textures (the views would point at freed buffers) until they end
     std::atomic<int> servers{0};
@@ -83,6 +86,8 @@ struct mcpt_scene_s {
         v.n_spheres = n_spheres;
         v.tex = tex.ptr;
         v.srgb8 = srgb8.ptr;
+        v.grid = grid_view;
+        v.grid.data = grid.ptr;
         return v;
     }
     void release() {
@@ -105,6 +110,7 @@ struct mcpt_scene_s {
         tex_px.clear();
         tex_host.clear();
         srgb8.release(ctx);
+        grid.release(ctx);
     }
     int n_prims() const { return n + n_spheres; }
     // every texture a material names has been added

@@ -224,12 +224,53 @@ MCPT_DEV float tri_area(const float* hot, int prim) {  // :837-839
     return 0.5f * (v1 - v0).cross(v2 - v0).length();
 }
 
-// mediumTransmittance (:2874-2898), homogeneous media: closest hits along
-// the segment, exp(-sigma d) for each stretch that ends on a back face.
+// heterogeneousFreeFlightDistance (:796-812): delta tracking against the
+// majorant sigma * max_density; a tentative collision is real with
+// probability density / max_density. Returns maxd when the ray leaves
+// without a real collision (or after 8192 steps).
+MCPT_DEV float grid_free_flight(const GridView& g, float sigma, const vec3& o, const vec3& d, float maxd, Rng& rng) {
+    const float maj = sigma * g.max_density;
+    if (!(maj > 0.0f)) return maxd;
+    float t = 0.0f;
+    for (int step = 0; step < 8192; step++) {
+        t = t - mf_log(1.0f - rng.u()) / maj;
+        if (t >= maxd) return maxd;
+        const float rho = grid_density(g, o + d * t);
+        if (rng.u() < rho / g.max_density) return t;
+    }
+    return maxd;
+}
+
+// heterogeneousTransmittance (:817-835): ratio tracking, each tentative
+// collision scales the estimate by 1 - density / max_density; below 1e-4
+// the segment counts as opaque.
+MCPT_DEV float grid_transmittance(const GridView& g, float sigma, const vec3& o, const vec3& d, float len, Rng& rng) {
+    const float maj = sigma * g.max_density;
+    if (!(maj > 0.0f)) return 1.0f;
+    float tr = 1.0f, t = 0.0f;
+    for (int step = 0; step < 8192; step++) {
+        t = t - mf_log(1.0f - rng.u()) / maj;
+        if (t >= len) break;
+        const float rho = grid_density(g, o + d * t);
+        tr = tr * (1.0f - rho / g.max_density);
+        if (tr < 1e-4f) return 0.0f;
+    }
+    return tr;
+}
+
+// mediumTransmittance (:2874-2898): closest hits along the segment; each
+// stretch that ends on a back face is inside a medium and contributes
+// exp(-sigma d), or with a density grid a ratio-tracking estimate whose
+// numbers come from the shadow ray's own stream (mcpt_pcg32_for_shadow).
 MCPT_DEV float medium_transmittance(const SceneView& sv, vec3 o, const vec3& d, float maxd, int* stk, int stride,
                                     float time) {
     float tr = 1.0f;
     float travelled = 0.0f;
+    Rng srng;
+    if (sv.grid.data) {
+        const uint32_t bits[7] = {f2u(o.x), f2u(o.y), f2u(o.z), f2u(d.x), f2u(d.y), f2u(d.z), f2u(maxd)};
+        mcpt_pcg32_for_shadow(&srng.g, bits);
+    }
     for (int k = 0; k < 4; k++) {
         float remaining = maxd - travelled;
         if (remaining <= 0.0f) break;
@@ -253,7 +294,8 @@ MCPT_DEV float medium_transmittance(const SceneView& sv, vec3 o, const vec3& d, 
             vec3 n = (v1 - v0).cross(v2 - v0).normalize();
             front = d.dot(n) < 0.0f;
         }
-        if (!front) tr = tr * mf_exp(-m.density * s.t);
+        if (!front)
+            tr = tr * (sv.grid.data ? grid_transmittance(sv.grid, m.density, o, d, s.t, srng) : mf_exp(-m.density * s.t));
         travelled = travelled + s.t;
         o = o + d * s.t;
     }
@@ -321,7 +363,8 @@ MCPT_DEV void shade_bounce(const SceneView& sv, const vec3& bg, float rr_cap, in
     vec3 so = h.p;
     if (m.type == MCPT_MAT_ISOTROPIC) {
         float sd;
-        if (h.front) sd = t;
+        if (h.front) sd = t;  // this hit only enters the medium
+        else if (sv.grid.data) sd = grid_free_flight(sv.grid, m.density, ro, rd, t, rng);
         else sd = -mf_log(1.0f - rng.u()) / m.density;
         if (sd >= t) {  // pass through the boundary, same direction
             out.next = true;

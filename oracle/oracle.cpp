@@ -229,6 +229,11 @@ struct oracle_scene_s {
     std::vector<int> leaf_prim;    // n
     std::vector<AABB> leaf_box;    // n
     int max_depth = 0;
+    // density grid of every isotropic material (empty: homogeneous media)
+    std::vector<float> grid;
+    int gdim[3] = {0, 0, 0};
+    float gorg[3] = {0, 0, 0}, ginv[3] = {0, 0, 0};
+    float gmax = 0.0f;
 };
 
 namespace {
@@ -613,10 +618,80 @@ V3 albedo_at(const oracle_scene_s& s, const mcpt_material& m, float u, float v) 
     return V3(c[0], c[1], c[2]);
 }
 
-// mediumTransmittance (src/path_tracer.cu:2874-2898), homogeneous media
+// sampleVolumeDensity (src/path_tracer.cu:784-789) over a dense grid:
+// index point q = (p - origin) * (1 / voxel), trilinear between the eight
+// voxels around floor(q) (NanoVDB's TrilinearSampler: lerp along z, then
+// y, then x), voxels outside the grid read 0.
+float grid_at(const oracle_scene_s& s, long long i, long long j, long long k) {
+    if (i < 0 || j < 0 || k < 0 || i >= s.gdim[0] || j >= s.gdim[1] || k >= s.gdim[2]) return 0.0f;
+    return s.grid[((size_t)k * s.gdim[1] + (size_t)j) * s.gdim[0] + (size_t)i];
+}
+float grid_density(const oracle_scene_s& s, const V3& p) {
+    const float q[3] = {(p.x - s.gorg[0]) * s.ginv[0], (p.y - s.gorg[1]) * s.ginv[1], (p.z - s.gorg[2]) * s.ginv[2]};
+    long long c[3];
+    float f[3];
+    for (int a = 0; a < 3; a++) {
+        if (!(q[a] > -1.0f && q[a] < (float)s.gdim[a])) return 0.0f;  // every neighbour outside, or NaN
+        const float fl = floorf(q[a]);
+        c[a] = (long long)fl;
+        f[a] = q[a] - fl;
+    }
+    auto lerp = [](float a, float b, float w) { return a + w * (b - a); };
+    float xs[2];
+    for (int dx = 0; dx < 2; dx++) {
+        float ys[2];
+        for (int dy = 0; dy < 2; dy++)
+            ys[dy] = lerp(grid_at(s, c[0] + dx, c[1] + dy, c[2]), grid_at(s, c[0] + dx, c[1] + dy, c[2] + 1), f[2]);
+        xs[dx] = lerp(ys[0], ys[1], f[1]);
+    }
+    return lerp(xs[0], xs[1], f[0]);
+}
+
+// heterogeneousFreeFlightDistance (src/path_tracer.cu:796-812)
+float grid_free_flight(const oracle_scene_s& s, float sigma, const V3& o, const V3& d, float maxd, Rng& rng) {
+    const float majorant = sigma * s.gmax;
+    if (!(majorant > 0.0f)) return maxd;
+    float t = 0.0f;
+    for (int step = 0; step < 8192; step++) {
+        t = t - mf_log(1.0f - rng.u()) / majorant;
+        if (t >= maxd) return maxd;
+        if (rng.u() < grid_density(s, o + d * t) / s.gmax) return t;
+    }
+    return maxd;
+}
+
+// heterogeneousTransmittance (src/path_tracer.cu:817-835)
+float grid_transmittance(const oracle_scene_s& s, float sigma, const V3& o, const V3& d, float len, Rng& rng) {
+    const float majorant = sigma * s.gmax;
+    if (!(majorant > 0.0f)) return 1.0f;
+    float tr = 1.0f, t = 0.0f;
+    for (int step = 0; step < 8192; step++) {
+        t = t - mf_log(1.0f - rng.u()) / majorant;
+        if (t >= len) break;
+        tr = tr * (1.0f - grid_density(s, o + d * t) / s.gmax);
+        if (tr < 1e-4f) return 0.0f;
+    }
+    return tr;
+}
+
+uint32_t bits_of(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
+// mediumTransmittance (src/path_tracer.cu:2874-2898). With a density grid,
+// ratio tracking draws from the shadow ray's own stream, keyed by the ray's
+// bits (mcpt_pcg32_for_shadow) rather than by the path's state.
 float medium_transmittance(const oracle_scene_s& s, V3 o, const V3& d, float maxd, float time, PathCounts& pc) {
     float tr = 1.0f;
     float travelled = 0.0f;
+    Rng srng;
+    if (!s.grid.empty()) {
+        const uint32_t key[7] = {bits_of(o.x), bits_of(o.y), bits_of(o.z), bits_of(d.x),
+                                 bits_of(d.y), bits_of(d.z), bits_of(maxd)};
+        mcpt_pcg32_for_shadow(&srng.g, key);
+    }
     for (int k = 0; k < 4; k++) {
         float remaining = maxd - travelled;
         if (remaining <= 0.0f) break;
@@ -624,7 +699,9 @@ float medium_transmittance(const oracle_scene_s& s, V3 o, const V3& d, float max
         Hit b = trace_ray(s, o, d, remaining, time, nullptr, nullptr);
         if (!b.did) break;
         if (b.mat->type != MCPT_MAT_ISOTROPIC) return 0.0f;
-        if (!b.front) tr = tr * mf_exp(-b.mat->density * b.t);
+        if (!b.front)
+            tr = tr * (s.grid.empty() ? mf_exp(-b.mat->density * b.t)
+                                      : grid_transmittance(s, b.mat->density, o, d, b.t, srng));
         travelled = travelled + b.t;
         o = b.p;
     }
@@ -687,6 +764,7 @@ V3 trace_path(const oracle_scene_s& s, const mcpt_camera& cam, int px, int py, R
         if (m.type == MCPT_MAT_ISOTROPIC) {
             float sd;
             if (hit.front) sd = hit.t;
+            else if (!s.grid.empty()) sd = grid_free_flight(s, m.density, ro, rd, hit.t, rng);
             else sd = -mf_log(1.0f - rng.u()) / m.density;
             if (sd >= hit.t) {
                 ro = hit.p;
@@ -1185,6 +1263,43 @@ void oracle_sample_albedo(oracle_scene s, const mcpt_material* m, const float* u
         out[3 * i] = c.x;
         out[3 * i + 1] = c.y;
         out[3 * i + 2] = c.z;
+    }
+}
+
+// A density grid for every isotropic material (the reference attaches a
+// loaded volume to them all, src/path_tracer.cu:2509-2519); max_density is
+// the grid's largest value. Returns -1 on a malformed grid.
+int oracle_scene_set_density_grid(oracle_scene s, const mcpt_density_grid* g, float* max_density) {
+    for (int a = 0; a < 3; a++)
+        if (g->dims[a] < 1 || !(g->voxel_size[a] > 0.0f)) return -1;
+    const size_t n = (size_t)g->dims[0] * g->dims[1] * g->dims[2];
+    s->grid.assign(g->density, g->density + n);
+    float mx = 0.0f;
+    for (float v : s->grid) mx = std::max(mx, v);
+    for (int a = 0; a < 3; a++) {
+        s->gdim[a] = g->dims[a];
+        s->gorg[a] = g->origin[a];
+        s->ginv[a] = 1.0f / g->voxel_size[a];
+    }
+    s->gmax = mx;
+    if (max_density) *max_density = mx;
+    return 0;
+}
+
+void oracle_grid_density(oracle_scene s, const float* pts, int n, float* out) {
+    for (int i = 0; i < n; i++) out[i] = grid_density(*s, V3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+}
+
+// fn 0: delta-tracking free flight, 1: ratio-tracking transmittance, for n
+// rays of 7 floats (origin, direction, length), ray i drawing from the path
+// stream (pixel i, sample 0, seed); sigma is the extinction scale
+void oracle_grid_track(oracle_scene s, int fn, float sigma, const float* rays, int n, uint64_t seed, float* out) {
+    for (int i = 0; i < n; i++) {
+        const float* r = rays + 7 * i;
+        Rng rng;
+        mcpt_pcg32_for_path(&rng.g, (uint32_t)i, 0u, seed);
+        const V3 o(r[0], r[1], r[2]), d(r[3], r[4], r[5]);
+        out[i] = fn == 0 ? grid_free_flight(*s, sigma, o, d, r[6], rng) : grid_transmittance(*s, sigma, o, d, r[6], rng);
     }
 }
 

@@ -61,6 +61,11 @@ void oracle_scene_destroy(oracle_scene s);
 /* sampleTexture's checker and image textures (src/path_tracer.cu:216-234);
  * returns the texture's index (materials name it as texture = index + 1). */
 int oracle_scene_add_texture(oracle_scene s, const mcpt_texture_desc* desc);
+/* heterogeneous media: the grid, its trilinear density at n points, and
+ * delta tracking (fn 0) / ratio tracking (fn 1) along n rays of 7 floats */
+int oracle_scene_set_density_grid(oracle_scene s, const mcpt_density_grid* g, float* max_density);
+void oracle_grid_density(oracle_scene s, const float* pts, int n, float* out);
+void oracle_grid_track(oracle_scene s, int fn, float sigma, const float* rays, int n, uint64_t seed, float* out);
 /* the albedo material m gives at n (u, v) points (closed-form tests) */
 void oracle_sample_albedo(oracle_scene s, const mcpt_material* m, const float* uv, int n, float* out);
 int oracle_scene_dump(oracle_scene s, mcpt_lbvh_dump* dump);

@@ -57,6 +57,9 @@ def lib():
             "oracle_scene_destroy": ([VP], None),
             "oracle_sphere_uv": ([VP, C.c_int, VP], None),
             "oracle_scene_add_texture": ([VP, VP], C.c_int),
+            "oracle_scene_set_density_grid": ([VP, VP, C.POINTER(C.c_float)], C.c_int),
+            "oracle_grid_density": ([VP, VP, C.c_int, VP], None),
+            "oracle_grid_track": ([VP, C.c_int, C.c_float, VP, C.c_int, C.c_uint64, VP], None),
             "oracle_sample_albedo": ([VP, VP, VP, C.c_int, VP], None),
             "oracle_scene_dump": ([VP, VP], C.c_int),
             "oracle_scene_lights": ([VP, VP, C.c_int], C.c_int),
@@ -82,6 +85,25 @@ def lib():
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class DensityGrid(C.Structure):
+    """mcpt_density_grid: dense voxels, x fastest"""
+    _fields_ = [("dims", C.c_int32 * 3), ("origin", C.c_float * 3), ("voxel_size", C.c_float * 3),
+                ("reserved", C.c_int32), ("density", C.c_void_p)]
+
+
+def density_grid_desc(density, origin, voxel_size):
+    """(desc, keep): density is a (nz, ny, nx) float32 array"""
+    keep = np.ascontiguousarray(density, np.float32)
+    if keep.ndim != 3:
+        raise ValueError("density must be (nz, ny, nx)")
+    g = DensityGrid()
+    g.dims[:] = [keep.shape[2], keep.shape[1], keep.shape[0]]
+    g.origin[:] = [float(x) for x in origin]
+    g.voxel_size[:] = [float(x) for x in (voxel_size if np.ndim(voxel_size) else (voxel_size,) * 3)]
+    g.density = keep.ctypes.data
+    return g, keep
 
 
 class EnvDesc(C.Structure):
@@ -215,6 +237,31 @@ class Scene:
         if k < 0:
             raise ValueError("bad texture")
         return k
+
+    def set_density_grid(self, density, origin, voxel_size):
+        """the checker's copy of a density grid ((nz, ny, nx) float32);
+        returns the largest voxel value"""
+        g, keep = density_grid_desc(density, origin, voxel_size)
+        mx = C.c_float(0.0)
+        if lib().oracle_scene_set_density_grid(self.h, C.byref(g), C.byref(mx)) < 0:
+            raise ValueError("bad density grid")
+        self._grid = keep
+        return mx.value
+
+    def grid_density(self, pts):
+        pts = np.ascontiguousarray(np.asarray(pts, np.float32).reshape(-1, 3))
+        out = np.zeros(len(pts), np.float32)
+        lib().oracle_grid_density(self.h, _p(pts), len(pts), _p(out))
+        return out
+
+    def grid_track(self, fn, sigma, rays, seed=0):
+        """fn "free_flight" (delta tracking) or "transmittance" (ratio
+        tracking) along rays of 7 floats: origin, direction, length"""
+        rays = np.ascontiguousarray(np.asarray(rays, np.float32).reshape(-1, 7))
+        out = np.zeros(len(rays), np.float32)
+        lib().oracle_grid_track(self.h, {"free_flight": 0, "transmittance": 1}[fn], sigma, _p(rays), len(rays),
+                                seed, _p(out))
+        return out
 
     def sample_albedo(self, mat, uv):
         """albedo of material record `mat` at (u, v) rows"""

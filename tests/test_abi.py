@@ -39,6 +39,7 @@ def test_record_layouts(mcpt):
     assert C.sizeof(mcpt.BatchTune) == 16
     assert C.sizeof(mcpt.ObjOpts) == 72 and C.sizeof(mcpt.ObjStats) == 32
     assert C.sizeof(mcpt.TextureDesc) == 48 and mcpt.MAT_DTYPE.itemsize == 48
+    assert C.sizeof(mcpt.DensityGrid) == 48
     assert mcpt.HOT_DTYPE.itemsize == 36 and mcpt.COLD_DTYPE.itemsize == 64
 
 
