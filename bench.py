@@ -623,7 +623,9 @@ def main():
 
     with mcpt.Context() as ctx:
         log(f"bench: device {ctx.describe()}")
-        scene = ctx.scene(hot, cold, mats, env=mcpt.make_env(sky))
+        t_up = time.perf_counter()
+        scene = ctx.scene(hot, cold, mats, env=mcpt.make_env(sky))  # host buffers cross PCIe here
+        upload_s = time.perf_counter() - t_up
         build = scene.build()
         log(f"bench: LBVH build {build['build_ms']:.2f} ms, depth {build['max_depth']}")
         film = ctx.film(W, H)
@@ -651,7 +653,9 @@ def main():
                                        flags=mcpt.RENDER_COUNT_TRAVERSAL | mcpt.RENDER_REFERENCE_ORDER | xf)
             assert ref_counted["extend_rays"] == st["extend_rays"]
             log("bench: counting passes done")
+        t_rd = time.perf_counter()
         img = film.read()
+        read_s = time.perf_counter() - t_rd
         film.close()
         scratch.close()
         scene.close()
@@ -680,6 +684,10 @@ def main():
         "kernel_ms": {k: round(st[k], 3) for k in ("camera_ms", "extend_ms", "shade_ms", "shadow_ms",
                                                      "finalize_ms")},
         "lbvh_build_ms": build["build_ms"],
+        # the same job with the scene handed over in host memory and the film
+        # read back: scene creation (upload) + build + render + read
+        "host_io": {"scene_upload_s": upload_s, "film_read_s": read_s,
+                    "Mrays_per_s_incl_pcie": rays / (upload_s + build["build_ms"] / 1e3 + secs + read_s) / 1e6},
         "image_mean": [float(x) for x in img.reshape(-1, 3).mean(0) / K],
     }
     if counted is not None and st["extend_ms"] > 0:
