@@ -42,12 +42,14 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(hot, cold, mats, sky, desc, depth, seed, budget_s, threads):
+def cpu_baseline(hot, cold, mats, sky, desc, depth, seed, budget_s, threads, spheres=None, grid=None):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
     t0 = time.time()
-    ref = pyoracle.Scene(hot, cold, mats, sky=sky)
+    ref = pyoracle.Scene(hot, cold, mats, sky=sky, spheres=spheres)
+    if grid is not None:
+        ref.set_density_grid(*grid)
     build_s = time.time() - t0
     cam = pyoracle.camera_init(desc)
     H = cam.height
@@ -502,6 +504,72 @@ def cache_bench(mcpt, ctx, args):
     return out
 
 
+# ----------------------------------------------------------- cloud medium
+def cloud_scene(mcpt, seed, dims=(128, 96, 128), voxel=0.016):
+    """The Cornell room around a heterogeneous cloud: an isotropic sphere
+    enclosing a density grid of soft blobs (the reference's cloud scene
+    wraps its NanoVDB volume in a sphere the same way)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    hot, cold, mats = mcpt.gen_scene(mcpt.GEN_CORNELL)
+    cloud = np.zeros(1, mcpt.MAT_DTYPE)
+    cloud["type"], cloud["albedo"], cloud["density"], cloud["phase_g"] = mcpt.MAT_ISOTROPIC, 0.95, 6.0, 0.6
+    mats = np.concatenate([mats, cloud])
+    nx, ny, nz = dims
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    z = (np.arange(nz) + 0.5) / nz
+    g = np.zeros((nz, ny, nx), np.float32)
+    for _ in range(24):
+        c = rng.uniform(0.2, 0.8, 3)
+        r = rng.uniform(0.08, 0.25)
+        d2 = ((x[None, None, :] - c[0]) ** 2 + (y[None, :, None] - c[1]) ** 2 + (z[:, None, None] - c[2]) ** 2) / r ** 2
+        g += (np.maximum(0.0, 1.0 - d2) * rng.uniform(0.5, 1.5)).astype(np.float32)
+    extent = np.array(dims, np.float32) * np.float32(voxel)
+    center = np.array([0.0, 1.6, -2.2], np.float32)
+    origin = center - extent / 2
+    sp = np.zeros(1, mcpt.SPHERE_DTYPE)
+    sp["center"], sp["radius"], sp["material"] = center, 0.5 * float(np.linalg.norm(extent)), len(mats) - 1
+    return hot, cold, mats, sp, (g, origin, voxel)
+
+
+def cloud_bench(mcpt, args):
+    """One frame of the cloud room (delta tracking inside, ratio tracking
+    for shadow rays) with the scene resident, timed by HIP events."""
+    W = H = args.cloud_size
+    K, depth = args.cloud_spp, args.depth
+    hot, cold, mats, sp, grid = cloud_scene(mcpt, args.seed)
+    desc = mcpt.gen_camera(mcpt.GEN_CORNELL, W, H)
+    cam = mcpt.camera_init(desc)
+    mode = {"megakernel": mcpt.MODE_MEGAKERNEL, "wavefront": mcpt.MODE_WAVEFRONT,
+            "persistent": mcpt.MODE_PERSISTENT}[args.mode]
+    with mcpt.Context() as ctx:
+        scene = ctx.scene(hot, cold, mats, spheres=sp)
+        gmax = scene.set_density_grid(*grid)
+        scene.build()
+        film = ctx.film(W, H)
+        scene.render(cam, film, mode=mode, spp=max(args.warmup, 1), sample_begin=K, max_depth=depth, seed=args.seed)
+        film.clear()
+        st = scene.render(cam, film, mode=mode, spp=K, max_depth=depth, seed=args.seed,
+                          flags=mcpt.RENDER_TIME_KERNELS)
+        img = film.read()
+        film.close()
+        scene.close()
+    secs = st["total_ms"] / 1e3
+    rays = st["extend_rays"] + st["shadow_rays"]
+    out = {"workload": f"Cornell room + {grid[0].shape[2]}x{grid[0].shape[1]}x{grid[0].shape[0]} density grid "
+                       f"cloud (max {gmax:.2f}, sigma 6), {W}x{H}, {K} spp, depth {depth}, {args.mode}",
+           "Mrays_per_s": rays / secs / 1e6, "paths_per_s": st["paths"] / secs, "ms_per_spp": st["total_ms"] / K,
+           "kernel_ms": {k: round(st[k], 3) for k in ("camera_ms", "extend_ms", "shade_ms", "shadow_ms",
+                                                        "finalize_ms")},
+           "extend_rays": st["extend_rays"], "shadow_rays": st["shadow_rays"],
+           "grid_MB": grid[0].nbytes / 1e6, "image_mean": float(img.mean() / K)}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(hot, cold, mats, None, desc, depth, args.seed, args.cpu_seconds,
+                                           min(os.cpu_count() or 1, 16), spheres=sp, grid=grid)
+    return out
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device copy of `nbytes`, for scaling rocprofv3's
     FETCH_SIZE / WRITE_SIZE counters: a copy kernel of known size reads and
@@ -539,7 +607,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the traversal-counting pass")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", default="all", choices=["all", "extend", "serving", "cache", "batch"])
+    ap.add_argument("--workload", default="all", choices=["all", "extend", "serving", "cache", "batch", "cloud"])
+    ap.add_argument("--cloud-size", type=int, default=512)
+    ap.add_argument("--cloud-spp", type=int, default=64)
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-seed", type=int, default=3)
     ap.add_argument("--batch-size", type=int, default=256)
@@ -582,6 +652,15 @@ def main():
         sys.exit(2)
 
     mcpt = importlib.import_module("monocuda-pt_amd")
+    if args.workload == "cloud":
+        cl = cloud_bench(mcpt, args)
+        print(json.dumps({
+            "metric": "Mrays/s through a heterogeneous medium", "value": cl["Mrays_per_s"], "unit": "Mrays/s",
+            "n_gpus": 1, "steps": args.cloud_spp, "warmup": args.warmup, "ms_per_step": cl["ms_per_spp"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded procedural density grid)", "config": {"workload": cl["workload"]},
+            "cpu_baseline": cl.get("cpu_baseline"), "cloud": cl}), flush=True)
+        return
     if args.workload == "batch":
         with mcpt.Context() as ctx:
             bb = batch_bench(mcpt, ctx, args)
