@@ -575,6 +575,14 @@ int mcpt_scene_set_density_grid(mcpt_scene scene, const mcpt_density_grid* grid,
  * pattern: rgba8 NULL to get the size. */
 int mcpt_load_ppm(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8);
 
+/* A PNG texture as RGBA8, top row first: the files the reference reads with
+ * stb_image (loadImageRgba8, src/image_io.cpp:30-39; the texture fallback
+ * at src/path_tracer.cu:2602). Every colour type and bit depth, Adam7,
+ * tRNS; 16-bit samples keep their high byte, 1/2/4-bit grey is scaled to
+ * 0..255. Two-call pattern: rgba8 NULL to get the size. mcpt_load_obj
+ * (load_textures) and mcpt_load_pbrt read PNG textures the same way. */
+int mcpt_load_png(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8);
+
 /* Environment map files for mcpt_env_desc (RGBA FP32, top row first, alpha
  * 1). Two-call pattern: rgba NULL to get the size.
  * mcpt_load_pfm: loadPfmAsRgba32f (src/path_tracer.cu:335-368), "PF"

@@ -291,6 +291,7 @@ _SIGS = {
     "mcpt_scene_add_texture": ([VP, C.POINTER(TextureDesc), C.POINTER(C.c_int32)], C.c_int),
     "mcpt_scene_set_density_grid": ([VP, C.POINTER(DensityGrid), C.POINTER(C.c_float)], C.c_int),
     "mcpt_load_ppm": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
+    "mcpt_load_png": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_pfm": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_hdr": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_exr": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
@@ -439,6 +440,16 @@ def load_ppm(path):
     _check(_lib.mcpt_load_ppm(p, C.byref(w), C.byref(h), None), "mcpt_load_ppm")
     out = np.zeros((h.value, w.value, 4), np.uint8)
     _check(_lib.mcpt_load_ppm(p, C.byref(w), C.byref(h), _ptr(out)), "mcpt_load_ppm")
+    return out
+
+
+def load_png(path):
+    """mcpt_load_png: a PNG file as an (H, W, 4) RGBA8 array."""
+    w, h = C.c_int32(), C.c_int32()
+    p = os.fsencode(path)
+    _check(_lib.mcpt_load_png(p, C.byref(w), C.byref(h), None), "mcpt_load_png")
+    out = np.zeros((h.value, w.value, 4), np.uint8)
+    _check(_lib.mcpt_load_png(p, C.byref(w), C.byref(h), _ptr(out)), "mcpt_load_png")
     return out
 
 

@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <cctype>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -421,4 +422,226 @@ int read_exr(const char* path, int& w, int& h, std::vector<float>* rgba) {
     return MCPT_OK;
 }
 
+
+// ---------------------------------------------------------------- PNG
+// PNG textures, which the reference reads with stb_image
+// (loadImageRgba8, src/image_io.cpp:30-39, the fallback of
+// getOrLoadObjTexture at src/path_tracer.cu:2602). Read here from the PNG
+// specification (ISO/IEC 15948): chunks with their CRC checked, IHDR first,
+// IDAT concatenated and inflated, unknown critical chunks refused; every
+// colour type and bit depth the format allows, filters None/Sub/Up/
+// Average/Paeth, Adam7 interlacing; 16-bit samples keep their high byte
+// and 1/2/4-bit grey is scaled to 0..255, as stb_image returns them; a tRNS
+// chunk gives palette alpha, or alpha 0 to the one grey or RGB value it
+// names; returned as RGBA8, top row first.
+namespace {
+
+const uint8_t kPngSig[8] = {137, 80, 78, 71, 13, 10, 26, 10};
+
+uint32_t be32(const uint8_t* p) {
+    return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | (uint32_t)p[3];
+}
+
+int paeth(int a, int b, int c) {
+    const int p = a + b - c, pa = std::abs(p - a), pb = std::abs(p - b), pc = std::abs(p - c);
+    if (pa <= pb && pa <= pc) return a;
+    return pb <= pc ? b : c;
+}
+
+// reverses one row's filter in place; prev is the previous row of the same
+// pass after its own reversal (zeros above a pass's first row)
+bool png_unfilter(int type, uint8_t* row, const uint8_t* prev, size_t n, size_t bpp) {
+    switch (type) {
+        case 0:
+            return true;
+        case 1:
+            for (size_t i = bpp; i < n; i++) row[i] = (uint8_t)(row[i] + row[i - bpp]);
+            return true;
+        case 2:
+            for (size_t i = 0; i < n; i++) row[i] = (uint8_t)(row[i] + prev[i]);
+            return true;
+        case 3:
+            for (size_t i = 0; i < n; i++) row[i] = (uint8_t)(row[i] + (((i >= bpp ? row[i - bpp] : 0) + prev[i]) >> 1));
+            return true;
+        case 4:
+            for (size_t i = 0; i < n; i++) {
+                const int a = i >= bpp ? row[i - bpp] : 0, c = i >= bpp ? prev[i - bpp] : 0;
+                row[i] = (uint8_t)(row[i] + paeth(a, prev[i], c));
+            }
+            return true;
+    }
+    return false;
+}
+
+}  // namespace
+
+int read_png(const char* path, int& w, int& h, std::vector<uint8_t>* rgba) {
+    File file(path);
+    if (!file.f) return set_error(MCPT_ERR_INVALID, "cannot open '%s'", path);
+    const long long size = file.remaining();
+    if (size < 8 + 25) return set_error(MCPT_ERR_INVALID, "'%s' is not a PNG file", path);
+    std::vector<uint8_t> d((size_t)size);
+    if (std::fread(d.data(), 1, d.size(), file.f) != d.size()) return set_error(MCPT_ERR_INVALID, "'%s': short read", path);
+    if (std::memcmp(d.data(), kPngSig, 8) != 0) return set_error(MCPT_ERR_INVALID, "'%s' is not a PNG file", path);
+    int depth = 0, ctype = -1, interlace = 0;
+    std::vector<uint8_t> plte, trns, idat;
+    bool ihdr = false, iend = false;
+    size_t at = 8;
+    while (!iend) {
+        if (d.size() - at < 12) return set_error(MCPT_ERR_INVALID, "'%s': truncated PNG chunk", path);
+        const uint32_t len = be32(&d[at]);
+        if (len > 0x7fffffffu || (size_t)len > d.size() - at - 12)
+            return set_error(MCPT_ERR_INVALID, "'%s': PNG chunk runs past the end of the file", path);
+        const uint8_t* type = &d[at + 4];
+        const uint8_t* data = &d[at + 8];
+        const uint32_t crc = (uint32_t)crc32(crc32(0L, Z_NULL, 0), type, len + 4);
+        if (crc != be32(data + len)) return set_error(MCPT_ERR_INVALID, "'%s': PNG chunk CRC mismatch", path);
+        const std::string t((const char*)type, 4);
+        if (!ihdr && t != "IHDR") return set_error(MCPT_ERR_INVALID, "'%s': PNG does not start with IHDR", path);
+        if (t == "IHDR") {
+            if (ihdr || len != 13) return set_error(MCPT_ERR_INVALID, "'%s': bad PNG IHDR", path);
+            ihdr = true;
+            const uint32_t pw = be32(data), ph = be32(data + 4);
+            depth = data[8];
+            ctype = data[9];
+            interlace = data[12];
+            const bool ok_depth = (ctype == 0 && (depth == 1 || depth == 2 || depth == 4 || depth == 8 || depth == 16)) ||
+                                  (ctype == 3 && (depth == 1 || depth == 2 || depth == 4 || depth == 8)) ||
+                                  ((ctype == 2 || ctype == 4 || ctype == 6) && (depth == 8 || depth == 16));
+            if (pw == 0 || ph == 0 || pw > 0x7fffffffu || ph > 0x7fffffffu || !ok_depth || data[10] != 0 ||
+                data[11] != 0 || interlace > 1)
+                return set_error(MCPT_ERR_INVALID, "'%s': unsupported or invalid PNG header", path);
+            if ((long long)pw * (long long)ph > kMaxPixels)
+                return set_error(MCPT_ERR_INVALID, "'%s': PNG image too large", path);
+            w = (int)pw;
+            h = (int)ph;
+            if (!rgba) return MCPT_OK;
+        } else if (t == "PLTE") {
+            if (len % 3 != 0 || len == 0 || len > 768) return set_error(MCPT_ERR_INVALID, "'%s': bad PNG palette", path);
+            plte.assign(data, data + len);
+        } else if (t == "tRNS") {
+            trns.assign(data, data + len);
+        } else if (t == "IDAT") {
+            idat.insert(idat.end(), data, data + len);
+        } else if (t == "IEND") {
+            iend = true;
+        } else if (!(type[0] & 0x20)) {
+            return set_error(MCPT_ERR_INVALID, "'%s': unknown critical PNG chunk %s", path, t.c_str());
+        }
+        at += 12 + (size_t)len;
+    }
+    if (ctype == 3 && plte.empty()) return set_error(MCPT_ERR_INVALID, "'%s': palette PNG without PLTE", path);
+    const int chans = ctype == 0 || ctype == 3 ? 1 : ctype == 2 ? 3 : ctype == 4 ? 2 : 4;
+    const size_t bits = (size_t)chans * depth;           // per pixel
+    const size_t bpp = std::max<size_t>(1, bits / 8);    // filter distance in bytes
+    // the Adam7 passes (x0, y0, dx, dy), or the whole image
+    static const int kAdam7[7][4] = {{0, 0, 8, 8}, {4, 0, 8, 8}, {0, 4, 4, 8}, {2, 0, 4, 4},
+                                     {0, 2, 2, 4}, {1, 0, 2, 2}, {0, 1, 1, 2}};
+    static const int kWhole[1][4] = {{0, 0, 1, 1}};
+    const int (*passes)[4] = interlace ? kAdam7 : kWhole;
+    const int npass = interlace ? 7 : 1;
+    size_t raw_bytes = 0;
+    for (int p = 0; p < npass; p++) {
+        const size_t pw = (size_t)(w > passes[p][0] ? (w - passes[p][0] + passes[p][2] - 1) / passes[p][2] : 0);
+        const size_t ph = (size_t)(h > passes[p][1] ? (h - passes[p][1] + passes[p][3] - 1) / passes[p][3] : 0);
+        if (pw && ph) raw_bytes += ph * (1 + (pw * bits + 7) / 8);
+    }
+    // deflate cannot expand its input more than 1032-fold: a longer image
+    // cannot be in this file, so nothing of its size is allocated
+    if (raw_bytes / 1032 > idat.size() + 64) return set_error(MCPT_ERR_INVALID, "'%s': PNG image data too short", path);
+    std::vector<uint8_t> raw(raw_bytes);
+    z_stream zs;
+    std::memset(&zs, 0, sizeof(zs));
+    if (inflateInit(&zs) != Z_OK) return set_error(MCPT_ERR_INVALID, "'%s': zlib init failed", path);
+    zs.next_in = idat.data();
+    zs.avail_in = (uInt)idat.size();
+    zs.next_out = raw.data();
+    zs.avail_out = (uInt)raw.size();
+    const int zr = inflate(&zs, Z_FINISH);
+    const size_t got = raw.size() - zs.avail_out;
+    inflateEnd(&zs);
+    if (zr != Z_STREAM_END || got != raw.size())
+        return set_error(MCPT_ERR_INVALID, "'%s': PNG image data does not inflate to the image size", path);
+    std::vector<uint8_t>& out = *rgba;
+    out.assign((size_t)w * h * 4, 0);
+    const int maxv = (1 << depth) - 1;
+    size_t off = 0;
+    for (int p = 0; p < npass; p++) {
+        const int x0 = passes[p][0], y0 = passes[p][1], dx = passes[p][2], dy = passes[p][3];
+        const size_t pw = (size_t)(w > x0 ? (w - x0 + dx - 1) / dx : 0);
+        const size_t ph = (size_t)(h > y0 ? (h - y0 + dy - 1) / dy : 0);
+        if (!pw || !ph) continue;
+        const size_t rb = (pw * bits + 7) / 8;
+        std::vector<uint8_t> zero(rb, 0);
+        const uint8_t* prev = zero.data();
+        for (size_t r = 0; r < ph; r++) {
+            uint8_t* row = &raw[off + 1];
+            if (!png_unfilter(raw[off], row, prev, rb, bpp))
+                return set_error(MCPT_ERR_INVALID, "'%s': bad PNG filter type %d", path, raw[off]);
+            for (size_t x = 0; x < pw; x++) {
+                // the pixel's samples at full depth
+                uint32_t v[4] = {0, 0, 0, 0};
+                for (int c = 0; c < chans; c++) {
+                    if (depth < 8) {
+                        const size_t bit = x * depth;
+                        v[c] = (uint32_t)(row[bit / 8] >> (8 - depth - (int)(bit % 8))) & (uint32_t)maxv;
+                    } else if (depth == 8) {
+                        v[c] = row[x * chans + c];
+                    } else {
+                        v[c] = (uint32_t)row[(x * chans + c) * 2] << 8 | row[(x * chans + c) * 2 + 1];
+                    }
+                }
+                auto to8 = [&](uint32_t s) -> uint8_t {
+                    return depth == 16 ? (uint8_t)(s >> 8) : depth == 8 ? (uint8_t)s : (uint8_t)(s * (255 / maxv));
+                };
+                uint8_t px[4];
+                switch (ctype) {
+                    case 0: {
+                        px[0] = px[1] = px[2] = to8(v[0]);
+                        const bool key = trns.size() >= 2 && v[0] == (uint32_t)(trns[0] << 8 | trns[1]);
+                        px[3] = key ? 0 : 255;
+                        break;
+                    }
+                    case 2: {
+                        for (int c = 0; c < 3; c++) px[c] = to8(v[c]);
+                        const bool key = trns.size() >= 6 && v[0] == (uint32_t)(trns[0] << 8 | trns[1]) &&
+                                         v[1] == (uint32_t)(trns[2] << 8 | trns[3]) &&
+                                         v[2] == (uint32_t)(trns[4] << 8 | trns[5]);
+                        px[3] = key ? 0 : 255;
+                        break;
+                    }
+                    case 3: {
+                        if ((size_t)v[0] * 3 + 2 >= plte.size())
+                            return set_error(MCPT_ERR_INVALID, "'%s': PNG palette index %u out of range", path, v[0]);
+                        for (int c = 0; c < 3; c++) px[c] = plte[v[0] * 3 + c];
+                        px[3] = v[0] < trns.size() ? trns[v[0]] : 255;
+                        break;
+                    }
+                    case 4:
+                        px[0] = px[1] = px[2] = to8(v[0]);
+                        px[3] = to8(v[1]);
+                        break;
+                    default:
+                        for (int c = 0; c < 4; c++) px[c] = to8(v[c]);
+                        break;
+                }
+                const size_t X = (size_t)x0 + x * dx, Y = (size_t)y0 + r * dy;
+                std::memcpy(&out[(Y * w + X) * 4], px, 4);
+            }
+            prev = row;
+            off += 1 + rb;
+        }
+    }
+    return MCPT_OK;
+}
+
+int read_image8(const char* path, int& w, int& h, std::vector<uint8_t>* rgba) {
+    uint8_t sig[8] = {0};
+    {
+        File file(path);
+        if (!file.f) return set_error(MCPT_ERR_INVALID, "cannot open '%s'", path);
+        if (std::fread(sig, 1, 8, file.f) != 8) return read_ppm(path, w, h, rgba);
+    }
+    return std::memcmp(sig, kPngSig, 8) == 0 ? read_png(path, w, h, rgba) : read_ppm(path, w, h, rgba);
+}
 }  // namespace mcpt

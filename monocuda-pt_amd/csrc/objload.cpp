@@ -115,20 +115,26 @@ std::string ppm_path_of(std::string p) {
     return p + ".ppm";
 }
 
-// getOrLoadObjTexture (:2594-2611): one read per path, failures remembered
+// getOrLoadObjTexture (:2594-2611): one read per .ppm path, failures
+// remembered; the .ppm first, then the named file itself (PNG or PPM here,
+// where the reference's fallback is stb_image, :2602)
 int texture_for(Library& lib, const std::string& map_kd) {
-    const std::string path = lib.obj_dir + "/" + ppm_path_of(map_kd);
-    auto it = lib.tex_by_path.find(path);
+    const std::string ppm = lib.obj_dir + "/" + ppm_path_of(map_kd);
+    auto it = lib.tex_by_path.find(ppm);
     if (it != lib.tex_by_path.end()) return it->second;
+    std::string named = map_kd;
+    for (char& ch : named)
+        if (ch == '\\') ch = '/';
+    named = lib.obj_dir + "/" + named;
     Image im;
     int idx = -1;
-    if (read_ppm(path.c_str(), im.w, im.h, &im.rgba) == MCPT_OK) {
+    if (read_ppm(ppm.c_str(), im.w, im.h, &im.rgba) == MCPT_OK ||
+        (named != ppm && read_image8(named.c_str(), im.w, im.h, &im.rgba) == MCPT_OK)) {
         idx = (int)lib.images->size();
         lib.images->push_back(std::move(im));
-    } else {
-        clear_error();  // a missing texture is not an error of the load
     }
-    lib.tex_by_path[path] = idx;
+    clear_error();  // a missing texture is not an error of the load
+    lib.tex_by_path[ppm] = idx;
     return idx;
 }
 

@@ -783,7 +783,7 @@ struct Builder {
                 } else {
                     ImageTex im;
                     idx = -1;
-                    if (read_ppm(path.c_str(), im.w, im.h, &im.rgba) == MCPT_OK) {
+                    if (read_image8(path.c_str(), im.w, im.h, &im.rgba) == MCPT_OK) {
                         idx = (int)L.images->size();
                         L.images->push_back(std::move(im));
                     } else {
