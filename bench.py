@@ -816,6 +816,11 @@ def main():
             result["serving"] = serving_bench(mcpt, ctx, args)
         with mcpt.Context() as ctx:
             result["batch"] = batch_bench(mcpt, ctx, args)
+        # the heterogeneous medium at its default size; its checker sample is
+        # kept short so the default run stays within minutes
+        cloud_args = argparse.Namespace(**vars(args))
+        cloud_args.cpu_seconds = min(args.cpu_seconds, 8.0)
+        result["cloud"] = cloud_bench(mcpt, cloud_args)
     print(json.dumps(result), flush=True)
 
 
