@@ -583,6 +583,13 @@ int mcpt_load_ppm(const char* path, int32_t* width, int32_t* height, uint8_t* rg
  * (load_textures) and mcpt_load_pbrt read PNG textures the same way. */
 int mcpt_load_png(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8);
 
+/* A Truevision TGA texture as RGBA8, top row first (another stb_image
+ * format the reference's texture fallback reads): colour-mapped (8-bit
+ * index), true-colour (15/16/24/32-bit) and grey (8-bit) images, raw or
+ * run-length encoded, either row origin. Two-call pattern: rgba8 NULL to
+ * get the size. The texture loaders read .tga files the same way. */
+int mcpt_load_tga(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8);
+
 /* Environment map files for mcpt_env_desc (RGBA FP32, top row first, alpha
  * 1). Two-call pattern: rgba NULL to get the size.
  * mcpt_load_pfm: loadPfmAsRgba32f (src/path_tracer.cu:335-368), "PF"

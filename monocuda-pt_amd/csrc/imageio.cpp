@@ -635,7 +635,110 @@ int read_png(const char* path, int& w, int& h, std::vector<uint8_t>* rgba) {
     return MCPT_OK;
 }
 
+// ---------------------------------------------------------------- TGA
+// Truevision TGA textures (another of stb_image's formats, the reference's
+// texture fallback): image types 1/9 (colour-mapped, 8-bit index), 2/10
+// (true colour, 15/16/24/32 bits) and 3/11 (grey, 8 bits), raw or
+// run-length packets (a header byte c: (c & 127) + 1 pixels, one pixel
+// repeated when c & 128, else that many literal pixels); BGR(A) order;
+// 15/16-bit colour expanded as v * 255 / 31 with alpha 255; rows bottom
+// first unless descriptor bit 5 is set; returned as RGBA8, top row first.
+int read_tga(const char* path, int& w, int& h, std::vector<uint8_t>* rgba) {
+    File file(path);
+    if (!file.f) return set_error(MCPT_ERR_INVALID, "cannot open '%s'", path);
+    const long long size = file.remaining();
+    if (size < 18) return set_error(MCPT_ERR_INVALID, "'%s' is not a TGA file", path);
+    std::vector<uint8_t> d((size_t)size);
+    if (std::fread(d.data(), 1, d.size(), file.f) != d.size()) return set_error(MCPT_ERR_INVALID, "'%s': short read", path);
+    auto le16 = [&](size_t at) { return (int)d[at] | (int)d[at + 1] << 8; };
+    const int idlen = d[0], cmtype = d[1], itype = d[2], cmfirst = le16(3), cmlen = le16(5), cmbits = d[7];
+    const int tw = le16(12), th = le16(14), bits = d[16], desc = d[17];
+    const int base = itype & 7;
+    const bool rle = (itype & 8) != 0;
+    const bool ok = (itype & ~11) == 0 && base >= 1 && base <= 3 && tw > 0 && th > 0 &&
+                    ((base == 1 && cmtype == 1 && bits == 8 && cmlen > 0 &&
+                      (cmbits == 15 || cmbits == 16 || cmbits == 24 || cmbits == 32)) ||
+                     (base == 2 && (bits == 15 || bits == 16 || bits == 24 || bits == 32)) || (base == 3 && bits == 8));
+    if (!ok) return set_error(MCPT_ERR_INVALID, "'%s': unsupported or invalid TGA header", path);
+    w = tw;
+    h = th;
+    if (!rgba) return MCPT_OK;
+    size_t at = 18 + (size_t)idlen;
+    const size_t cbp = (size_t)(cmbits + 7) / 8, bp = (size_t)(bits + 7) / 8;
+    const size_t cm_bytes = cmtype == 1 ? (size_t)cmlen * cbp : 0;
+    if (at + cm_bytes > d.size()) return set_error(MCPT_ERR_INVALID, "'%s': TGA file too short", path);
+    const uint8_t* cmap = &d[at];
+    at += cm_bytes;
+    const size_t npx = (size_t)tw * th, left = d.size() - at;
+    // every pixel needs its bytes, or a run packet's share of them
+    if (rle ? npx > 128 * (left / (1 + bp) + 1) : npx * bp > left)
+        return set_error(MCPT_ERR_INVALID, "'%s': TGA file too short", path);
+    auto colour = [](const uint8_t* p, int nbits, uint8_t* o) {
+        if (nbits == 15 || nbits == 16) {
+            const int v = (int)p[0] | (int)p[1] << 8;
+            o[0] = (uint8_t)(((v >> 10) & 31) * 255 / 31);
+            o[1] = (uint8_t)(((v >> 5) & 31) * 255 / 31);
+            o[2] = (uint8_t)((v & 31) * 255 / 31);
+            o[3] = 255;
+        } else {
+            o[0] = p[2];
+            o[1] = p[1];
+            o[2] = p[0];
+            o[3] = nbits == 32 ? p[3] : 255;
+        }
+    };
+    std::vector<uint8_t>& out = *rgba;
+    out.assign(npx * 4, 0);
+    uint8_t px[4];
+    auto decode = [&](const uint8_t* p) -> bool {
+        if (base == 3) {
+            px[0] = px[1] = px[2] = p[0];
+            px[3] = 255;
+        } else if (base == 1) {
+            const int k = (int)p[0] - cmfirst;
+            if (k < 0 || k >= cmlen) return false;
+            colour(cmap + (size_t)k * cbp, cmbits, px);
+        } else {
+            colour(p, bits, px);
+        }
+        return true;
+    };
+    const bool top_first = (desc & 0x20) != 0;
+    auto put = [&](size_t i) {
+        const size_t r = i / tw, c = i % tw;
+        const size_t row = top_first ? r : (size_t)th - 1 - r;
+        std::memcpy(&out[(row * tw + c) * 4], px, 4);
+    };
+    size_t i = 0;
+    while (i < npx) {
+        size_t count = 1;
+        bool repeat = false;
+        if (rle) {
+            if (at >= d.size()) return set_error(MCPT_ERR_INVALID, "'%s': TGA run data ends early", path);
+            const uint8_t c = d[at++];
+            count = (size_t)(c & 127) + 1;
+            repeat = (c & 128) != 0;
+            if (i + count > npx) return set_error(MCPT_ERR_INVALID, "'%s': TGA run passes the last pixel", path);
+        }
+        for (size_t k = 0; k < count; k++) {
+            if (k == 0 || !repeat) {
+                if (at + bp > d.size()) return set_error(MCPT_ERR_INVALID, "'%s': TGA pixel data ends early", path);
+                if (!decode(&d[at])) return set_error(MCPT_ERR_INVALID, "'%s': TGA colour index out of range", path);
+                at += bp;
+            }
+            put(i++);
+        }
+    }
+    return MCPT_OK;
+}
+
 int read_image8(const char* path, int& w, int& h, std::vector<uint8_t>* rgba) {
+    const size_t n = std::strlen(path);
+    if (n >= 4) {
+        std::string ext(path + n - 4);
+        for (char& ch : ext) ch = (char)std::tolower((unsigned char)ch);
+        if (ext == ".tga") return read_tga(path, w, h, rgba);
+    }
     uint8_t sig[8] = {0};
     {
         File file(path);
