@@ -19,7 +19,7 @@ HBM before the timed region starts. Prints one JSON line.
             The reference's binary walk of the same rays is reported beside
             it as 32 + 16 + 80*S + 44*L (16 B child refs + 2x32 B child
             boxes per step, 4 B index + 4 B type + 36 B triangle per leaf).
-  cpu_baseline  the CPU checker (oracle/, a host C++ restatement of the
+  cpu_baseline  the CPU oracle (oracle/, a host C++ restatement of the
             reference's device functions) on a band of image rows at 1 spp,
             all host threads.
 """
@@ -816,7 +816,7 @@ def main():
             result["serving"] = serving_bench(mcpt, ctx, args)
         with mcpt.Context() as ctx:
             result["batch"] = batch_bench(mcpt, ctx, args)
-        # the heterogeneous medium at its default size; its checker sample is
+        # the heterogeneous medium at its default size; its oracle sample is
         # kept short so the default run stays within minutes
         cloud_args = argparse.Namespace(**vars(args))
         cloud_args.cpu_seconds = min(args.cpu_seconds, 8.0)

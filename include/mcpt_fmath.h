@@ -6,7 +6,7 @@
  * without floating-point contraction (-ffp-contract=off) and without
  * fast-math, any IEEE-conforming compiler produces identical bits for the
  * same input: g++ on x86-64 (SSE) and hipcc for gfx950 both do. This is what
- * lets the GPU kernels and the CPU checker agree bit for bit on paths that
+ * lets the GPU kernels and the CPU oracle agree bit for bit on paths that
  * refract, scatter in a medium or look up the environment map.
  *
  * Methods (all published, coefficients fitted here):

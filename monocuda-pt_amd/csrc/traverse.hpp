@@ -376,7 +376,7 @@ MCPT_DEV int step_wide_any(const BvhView& bv, Trav& s, int* stk, int stride) {
 }
 
 // Full closest-hit traversal. REF selects the reference's visit order on the
-// binary tree (for step and leaf-test counts that match the CPU checker);
+// binary tree (for step and leaf-test counts that match the CPU oracle);
 // otherwise the four-wide tree, near child first. Both return the same t and
 // leaf.
 template <bool COUNT, bool REF = false>

@@ -1,4 +1,4 @@
-"""CPU checker for the radiance-cache MLP (test infrastructure only).
+"""CPU oracle for the radiance-cache MLP (test infrastructure only).
 
 Only tests/, __graft_entry__.smoke() and bench.py's CPU baseline may import
 this module; the library never calls it.

@@ -1,4 +1,4 @@
-// oracle.cpp — CPU checker for the path-tracing core. TEST INFRASTRUCTURE:
+// oracle.cpp — CPU oracle for the path-tracing core. TEST INFRASTRUCTURE:
 // only the tests, the smoke check and the benchmark's CPU-baseline leg use
 // it. It restates MonoCUDA-PT's device functions (src/path_tracer.cu, cited
 // per function) with the same operation order, in host C++17 built with

@@ -1,5 +1,5 @@
 /*
- * oracle.h — CPU checker for the path-tracing core. TEST INFRASTRUCTURE ONLY:
+ * oracle.h — CPU oracle for the path-tracing core. TEST INFRASTRUCTURE ONLY:
  * the test suite, the smoke check and the CPU-baseline leg of the benchmark
  * call it as the thing results are compared against. The product library
  * never links it.

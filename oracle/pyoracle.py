@@ -1,4 +1,4 @@
-"""ctypes binding of liboracle.so, the CPU checker (test infrastructure only).
+"""ctypes binding of liboracle.so, the CPU oracle (test infrastructure only).
 
 Only the test suite, the smoke check and the benchmark's CPU-baseline leg
 import this module; the product library never does. Record layouts are the
@@ -218,7 +218,7 @@ class Scene:
             self.h = None
 
     def add_texture(self, kind, color_a=(0, 0, 0), color_b=(1, 1, 1), scale=1.0, image=None):
-        """the checker's copy of a scene texture; kind 1 checker, 2 image
+        """the oracle's copy of a scene texture; kind 1 checker, 2 image
         ((H, W, 4) RGBA8); returns the index (materials use index + 1)"""
         class Desc(C.Structure):
             _fields_ = [("type", C.c_int32), ("color_a", C.c_float * 3), ("color_b", C.c_float * 3),
@@ -239,7 +239,7 @@ class Scene:
         return k
 
     def set_density_grid(self, density, origin, voxel_size):
-        """the checker's copy of a density grid ((nz, ny, nx) float32);
+        """the oracle's copy of a density grid ((nz, ny, nx) float32);
         returns the largest voxel value"""
         g, keep = density_grid_desc(density, origin, voxel_size)
         mx = C.c_float(0.0)

@@ -7,15 +7,15 @@
                     Parallelism in the Construction of BVHs, Octrees, and k-d
                     Trees", Fig. 3), children derived by hand
   cornell_lbvh.npz  Morton codes, sorted ids and tree of the 36-triangle
-                    Cornell box, from the CPU checker
+                    Cornell box, from the CPU oracle
   trace_soup.npz    2048 closest hits on a seeded 4096-triangle soup, from
-                    the CPU checker
+                    the CPU oracle
   cornell_64.npy    64x64, 4 spp, depth 8 Cornell accumulation (FP32), from
-                    the CPU checker
+                    the CPU oracle
 
 The reference ships no tests or vectors (it has no test directory), so only
 the first two are independent of this build; the rest freeze the CPU
-checker's output so that a later change to it shows up as a diff.
+oracle's output so that a later change to it shows up as a diff.
 
 Run from the repository root: python tests/golden/make_golden.py
 """

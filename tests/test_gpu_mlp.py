@@ -1,8 +1,8 @@
 """GPU tests of the radiance cache: the BF16 MFMA forward and backward
-kernels against the CPU checker (oracle/mlp.py, same bfloat16 rounding
+kernels against the CPU oracle (oracle/mlp.py, same bfloat16 rounding
 points, float64 accumulation), Adam, and the cached render.
 
-Tolerances: FP32 accumulation on the GPU against float64 in the checker,
+Tolerances: FP32 accumulation on the GPU against float64 in the oracle,
 with a bfloat16 rounding after every layer, so an occasional activation
 lands on the other side of a bfloat16 rounding boundary. Outputs: max
 abs error <= 2e-2 * max |y| + 1e-3; gradients: relative L2 error <= 2e-2
@@ -26,14 +26,14 @@ def _batch(n, seed, mcpt):
     return f, t
 
 
-def test_init_matches_checker(mcpt, ctx):
+def test_init_matches_oracle(mcpt, ctx):
     net = ctx.mlp(seed=3)
     assert np.array_equal(net.params(), M.init_params(3))
     net.close()
 
 
 @pytest.mark.parametrize("n", [1, 63, 64, 1000, 4099])
-def test_forward_matches_checker(mcpt, ctx, n):
+def test_forward_matches_oracle(mcpt, ctx, n):
     print(f"mlp forward n={n}", flush=True)
     net = ctx.mlp(seed=5)
     p = net.params()
@@ -46,7 +46,7 @@ def test_forward_matches_checker(mcpt, ctx, n):
 
 
 @pytest.mark.parametrize("n", [64, 777, 16384])
-def test_gradient_matches_checker(mcpt, ctx, n):
+def test_gradient_matches_oracle(mcpt, ctx, n):
     print(f"mlp gradient n={n}", flush=True)
     net = ctx.mlp(seed=7)
     p = net.params()
@@ -73,7 +73,7 @@ def test_gradient_is_deterministic(mcpt, ctx):
     net.close()
 
 
-def test_adam_steps_match_checker(mcpt, ctx):
+def test_adam_steps_match_oracle(mcpt, ctx):
     net = ctx.mlp(seed=11)
     p0 = net.params()
     f, t = _batch(4096, 8, mcpt)

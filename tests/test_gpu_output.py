@@ -1,4 +1,4 @@
-"""GPU output stage against the checker: mcpt_resolve_rgb8 bit for bit (every
+"""GPU output stage against the oracle: mcpt_resolve_rgb8 bit for bit (every
 tone map, with planted NaN / inf / negative values) and mcpt_auto_exposure
 within FP32 rounding (both sum the same FP32 terms in FP64, in different
 orders)."""
@@ -19,7 +19,7 @@ def _film_values(shape, seed):
 
 
 @pytest.mark.parametrize("tonemap", [0, 1, 2])
-def test_resolve_rgb8_matches_checker(mcpt, ctx, oracle, tonemap):
+def test_resolve_rgb8_matches_oracle(mcpt, ctx, oracle, tonemap):
     W, H, spp = 123, 45, 7
     vals = _film_values((H, W, 3), tonemap)
     film = ctx.film(W, H)
@@ -31,7 +31,7 @@ def test_resolve_rgb8_matches_checker(mcpt, ctx, oracle, tonemap):
     film.close()
 
 
-def test_auto_exposure_matches_checker(mcpt, ctx, oracle):
+def test_auto_exposure_matches_oracle(mcpt, ctx, oracle):
     W, H, spp = 640, 360, 5
     vals = _film_values((H, W, 3), 11)
     film = ctx.film(W, H)

@@ -1,4 +1,4 @@
-"""GPU parity: the gfx950 kernels against the CPU checker (oracle/).
+"""GPU parity: the gfx950 kernels against the CPU oracle (oracle/).
 
 Bar: LBVH stages (Morton codes, stable sort, Karras topology, refit bounds)
 and closest-hit (t, primitive id, step and leaf-test counts) bit-exact;
@@ -67,7 +67,7 @@ def test_lbvh_full_soup(mcpt, ctx, oracle):
 
 def _reference_leaf_order(d):
     """Each leaf's position in the reference's leaf-test order, from a
-    left-first pre-order walk of the checker's tree in which a node's leaf
+    left-first pre-order walk of the oracle's tree in which a node's leaf
     children are tested when it is visited, left then right
     (BVH::traverseRay, src/path_tracer.cu:1102-1147)."""
     L, R, Ll, Rl = d["left"], d["right"], d["left_is_leaf"], d["right_is_leaf"]
@@ -91,7 +91,7 @@ def _box(b):
 
 
 def _check_wide(mcpt, scene, ref):
-    """The four-wide tree against the checker's binary tree: wide node i is
+    """The four-wide tree against the oracle's binary tree: wide node i is
     the i-th even-level binary node, its children are the binary node's leaf
     children and its inner children's children in left-to-right order, every
     decoded box contains the exact one and exceeds it by less than one
@@ -234,7 +234,7 @@ f 16 20 17 13
 
 
 def test_render_obj_scene(mcpt, ctx, oracle, tmp_path):
-    """An OBJ + MTL room loaded by mcpt_load_obj renders like the checker's
+    """An OBJ + MTL room loaded by mcpt_load_obj renders like the oracle's
     render of the same records, in every mode."""
     (tmp_path / "room.mtl").write_text(ROOM_MTL)
     (tmp_path / "room.obj").write_text(ROOM_OBJ)
@@ -261,7 +261,7 @@ def test_render_obj_scene(mcpt, ctx, oracle, tmp_path):
 
 def test_render_textures(mcpt, ctx, oracle, tmp_path):
     """A room with a checker floor and an image on the back wall renders the
-    same on the GPU (every mode) and in the checker (sampleTexture,
+    same on the GPU (every mode) and in the oracle (sampleTexture,
     src/path_tracer.cu:216-234, the same FP32 filter on both sides)."""
     (tmp_path / "room.mtl").write_text(ROOM_MTL)
     text = ROOM_OBJ.replace("usemtl white\nf 1 2 3 4\nf 5 8 7 6\nf 1 5 6 2\n",
@@ -295,7 +295,7 @@ def test_render_textures(mcpt, ctx, oracle, tmp_path):
     for mode, (im, _) in out.items():
         assert np.array_equal(im, base), mode
         assert rel_mse(im, ref_img) < 1e-4, mode
-        print(f"textures mode {mode}: identical to the checker {np.mean(im == ref_img):.5f}", flush=True)
+        print(f"textures mode {mode}: identical to the oracle {np.mean(im == ref_img):.5f}", flush=True)
     # a material naming a texture the scene lacks is refused
     lone = ctx.scene(hot, cold, mats)
     lone.build()
@@ -309,7 +309,7 @@ def test_render_textures(mcpt, ctx, oracle, tmp_path):
 
 def test_render_obj_map_kd(mcpt, ctx, oracle, tmp_path):
     """A map_Kd image read by the OBJ loader (load_textures) and handed to
-    the scene renders the same on the GPU and in the checker."""
+    the scene renders the same on the GPU and in the oracle."""
     rng = np.random.default_rng(8)
     img = rng.integers(0, 256, (16, 24, 3), dtype=np.uint8)
     (tmp_path / "poster.ppm").write_bytes(b"P6\n24 16\n255\n" + img.tobytes())
@@ -335,7 +335,7 @@ def test_render_obj_map_kd(mcpt, ctx, oracle, tmp_path):
     out = _render_modes(mcpt, ctx, scene, cam, spp, depth, seed)
     for mode, (im, _) in out.items():
         assert rel_mse(im, ref_img) < 1e-4, mode
-        print(f"map_Kd mode {mode}: identical to the checker {np.mean(im == ref_img):.5f}", flush=True)
+        print(f"map_Kd mode {mode}: identical to the oracle {np.mean(im == ref_img):.5f}", flush=True)
     scene.close()
 
 
@@ -492,7 +492,7 @@ def test_render_cornell(mcpt, ctx, oracle):
         assert np.array_equal(film.read(), base), batch
         film.close()
     # reference visit order: same image, and the extend kernels' step and
-    # leaf-test totals equal the checker's
+    # leaf-test totals equal the oracle's
     for mode in (mcpt.MODE_WAVEFRONT, mcpt.MODE_PERSISTENT):
         film = ctx.film(W, H)
         st = scene.render(cam, film, mode=mode, spp=spp, max_depth=depth, seed=seed,

@@ -1,4 +1,4 @@
-"""Spheres, motion blur and the thin lens on the GPU against the CPU checker.
+"""Spheres, motion blur and the thin lens on the GPU against the CPU oracle.
 
 The build over finalizeScene's primitive list (spheres first, then the
 triangles, src/path_tracer.cu:1955-1972) is bit-exact stage by stage; closest
@@ -6,7 +6,7 @@ and any hits (hit_sphere at the ray's time, :886-911) are bit-exact in t,
 primitive type and index in both traversal orders; renders of a room holding
 Lambertian (image-textured), metal, glass, emissive, smoke (constant medium)
 and moving spheres are bit-identical across the render modes and within
-relMSE 1e-4 of the checker, and so is a render through a thin lens
+relMSE 1e-4 of the oracle, and so is a render through a thin lens
 (Camera::getRay's lens disk, :1648-1658).
 """
 import numpy as np
@@ -130,7 +130,7 @@ def test_render_sphere_room(mcpt, ctx, oracle, aperture):
     for mode, (im, _) in out.items():
         assert np.array_equal(im, base), mode
         assert rel_mse(im, ref_img) < 1e-4, mode
-        print(f"sphere room mode {mode}: identical to the checker {np.mean(im == ref_img):.5f}", flush=True)
+        print(f"sphere room mode {mode}: identical to the oracle {np.mean(im == ref_img):.5f}", flush=True)
     scene.close()
 
 
@@ -245,7 +245,7 @@ WorldEnd
 def test_render_pbrt_scene(mcpt, ctx, oracle, tmp_path):
     """A pbrt-v3 room read by mcpt_load_pbrt (triangles, an area light,
     glass and metal spheres, the file's camera and film) renders the same
-    on the GPU in every mode and in the checker."""
+    on the GPU in every mode and in the oracle."""
     p = tmp_path / "room.pbrt"
     p.write_text(PBRT_ROOM)
     hot, cold, sp, mats, info, _ = mcpt.load_pbrt(str(p))
@@ -264,5 +264,5 @@ def test_render_pbrt_scene(mcpt, ctx, oracle, tmp_path):
     for mode, (im, _) in out.items():
         assert np.array_equal(im, base), mode
         assert rel_mse(im, ref_img) < 1e-4, mode
-        print(f"pbrt room mode {mode}: identical to the checker {np.mean(im == ref_img):.5f}", flush=True)
+        print(f"pbrt room mode {mode}: identical to the oracle {np.mean(im == ref_img):.5f}", flush=True)
     scene.close()

@@ -1,11 +1,11 @@
-"""Heterogeneous media on the GPU against the CPU checker.
+"""Heterogeneous media on the GPU against the CPU oracle.
 
 A cloud in the Cornell room: an isotropic sphere that encloses a density
 grid, as the reference's cloud scene wraps its NanoVDB volume
 (buildCloudScene, src/path_tracer.cu:1845-1862; the grid attached to every
 isotropic material, :2509-2519). Free flight inside is delta tracking and
 shadow rays ratio tracking (:796-835). Renders are bit-identical across the
-render modes and within relMSE 1e-4 of the checker; the batched render, the
+render modes and within relMSE 1e-4 of the oracle; the batched render, the
 server and the radiance cache's first frame give the same films as the plain
 render.
 """
@@ -64,7 +64,7 @@ def test_render_cloud(mcpt, ctx, oracle):
     for mode, (im, _) in out.items():
         assert np.array_equal(im, base), mode
         assert rel_mse(im, ref_img) < 1e-4, mode
-        print(f"cloud mode {mode}: identical to the checker {np.mean(im == ref_img):.5f}", flush=True)
+        print(f"cloud mode {mode}: identical to the oracle {np.mean(im == ref_img):.5f}", flush=True)
     # the grid matters: the same room with a homogeneous medium renders otherwise
     plain = ctx.scene(hot, cold, mats, spheres=sp)
     plain.build()

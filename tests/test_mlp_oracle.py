@@ -1,6 +1,6 @@
-"""CPU checks of the radiance-cache checker (oracle/mlp.py).
+"""CPU checks of the radiance-cache oracle (oracle/mlp.py).
 
-The network has no reference counterpart, so the checker is pinned from
+The network has no reference counterpart, so the oracle is pinned from
 outside: its backward pass against torch autograd and against central
 finite differences, its encoding against closed forms, its bfloat16
 rounding against hand-worked cases, and its Adam against torch.optim.Adam.

@@ -1,4 +1,4 @@
-"""Pins the CPU checker's LBVH stages (Morton codes, sort, Karras tree,
+"""Pins the CPU oracle's LBVH stages (Morton codes, sort, Karras tree,
 refit) with closed forms, an independent bit-interleave, the Karras 2012
 8-key example and brute-force invariants."""
 import json
@@ -12,7 +12,7 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 def magic_expand(v):
     """10-bit spread by the usual magic-number sequence (independent of the
-    loop form used by the reference and the checker)."""
+    loop form used by the reference and the oracle)."""
     v = np.asarray(v, np.uint64) & np.uint64(0x3FF)
     v = (v * np.uint64(0x00010001)) & np.uint64(0xFF0000FF)
     v = (v * np.uint64(0x00000101)) & np.uint64(0x0F00F00F)

@@ -1,4 +1,4 @@
-"""Pins include/mcpt_fmath.h (shared by the kernels and the CPU checker)
+"""Pins include/mcpt_fmath.h (shared by the kernels and the CPU oracle)
 against libm through numpy's float64 functions, and PCG32 against the
 published pcg32 demo sequence and an independent restatement."""
 import numpy as np

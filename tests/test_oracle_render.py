@@ -1,4 +1,4 @@
-"""Pins the CPU checker's integrator with closed forms: emitters seen
+"""Pins the CPU oracle's integrator with closed forms: emitters seen
 directly, a convex diffuse object under a uniform sky (L = albedo * sky),
 a diffuse plane under a triangle light (Lambert's polygon formula), the
 resume contract of the (pixel, sample) RNG keying, camera set-up and the

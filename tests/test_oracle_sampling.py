@@ -1,8 +1,8 @@
-"""Closed-form pins for the checker's sampling functions: the dielectric
+"""Closed-form pins for the oracle's sampling functions: the dielectric
 branch (refract, Schlick, the reflect/refract choice), the Henyey-Greenstein
 sampler and pdf, and the environment-map importance sampling. Each check comes
 from the mathematics (Snell's law in vector form, the HG inverse CDF, solid-
-angle integrals), not from the checker's own formula, so a dropped term, a
+angle integrals), not from the oracle's own formula, so a dropped term, a
 swapped operand or a wrong constant fails one of them.
 
 Reference functions: refract (src/path_tracer.cu:146-163), schlick_approx

@@ -1,4 +1,4 @@
-"""The checker's spheres (sphere / hit_sphere / computeSphereUvFromOutwardNormal
+"""The oracle's spheres (sphere / hit_sphere / computeSphereUvFromOutwardNormal
 and finalizeScene's primitive list, src/path_tracer.cu:845-911, :1930-1972)
 pinned by the mathematics, on the CPU:
 

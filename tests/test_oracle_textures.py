@@ -1,4 +1,4 @@
-"""Closed-form pins for the checker's texture sampling (sampleTexture,
+"""Closed-form pins for the oracle's texture sampling (sampleTexture,
 src/path_tracer.cu:216-234, with make_checker_texture's sRGB conversion,
 :211-213): checker cells, and the image read's texel centres, half-way
 blends, wrap in u and v, the v flip (row 0 is the top of the image) and the

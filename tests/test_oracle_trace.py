@@ -1,4 +1,4 @@
-"""Pins the CPU checker's ray-triangle test and BVH traversal: known
+"""Pins the CPU oracle's ray-triangle test and BVH traversal: known
 answers on one triangle, brute force over every triangle, and the
 committed closest-hit fixture."""
 import os

@@ -1,4 +1,4 @@
-"""The checker's heterogeneous medium (sampleVolumeDensity,
+"""The oracle's heterogeneous medium (sampleVolumeDensity,
 heterogeneousFreeFlightDistance and heterogeneousTransmittance,
 src/path_tracer.cu:784-835) pinned by the mathematics, on the CPU:
 

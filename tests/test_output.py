@@ -1,7 +1,7 @@
-"""The output stage after the accumulation buffer: the checker's 8-bit
+"""The output stage after the accumulation buffer: the oracle's 8-bit
 conversion and auto exposure against closed forms, and the library's PNG
 writer read back by an independent decoder (zlib + the PNG chunk rules).
-CPU only; the GPU versions are compared with the checker in
+CPU only; the GPU versions are compared with the oracle in
 test_gpu_output.py.
 
 Reference: computeAutoExposure / logLuminanceSumKernel
