@@ -162,6 +162,9 @@ const char* mcpt_version(void);
 /* ----------------------------------------------------------------- context */
 /* hbm_budget_bytes = 0 means "all free memory at creation". */
 int mcpt_ctx_create(int device, uint64_t hbm_budget_bytes, mcpt_ctx* out);
+/* Returns MCPT_ERR_INVALID, and destroys nothing, while scenes, films,
+ * servers or networks made from the context are still alive: they use its
+ * stream and allocator. */
 int mcpt_ctx_destroy(mcpt_ctx ctx);
 /* bytes the context has allocated / may allocate */
 int mcpt_ctx_memory(mcpt_ctx ctx, uint64_t* used, uint64_t* budget);
@@ -184,6 +187,8 @@ int mcpt_scene_create(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_cold
 int mcpt_scene_create_ex(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, int32_t n_tris,
                          const mcpt_sphere* spheres, int32_t n_spheres, const mcpt_material* materials,
                          int32_t n_materials, const mcpt_env_desc* env, mcpt_scene* out);
+/* Returns MCPT_ERR_INVALID, and destroys nothing, while a server renders
+ * the scene. */
 int mcpt_scene_destroy(mcpt_scene scene);
 int mcpt_scene_light_count(mcpt_scene scene, int32_t* n_lights);
 int mcpt_scene_lights(mcpt_scene scene, mcpt_light* out /* n_lights */);

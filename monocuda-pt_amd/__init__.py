@@ -550,7 +550,7 @@ class Context:
             # servers first: they hold the scenes they render
             for child in sorted(self._children, key=lambda c: not isinstance(c, Server)):
                 child.close()
-            _lib.mcpt_ctx_destroy(self.h)
+            _check(_lib.mcpt_ctx_destroy(self.h), "mcpt_ctx_destroy")
             self.h = VP()
 
     def __enter__(self):
@@ -612,7 +612,7 @@ class Scene:
     def close(self):
         if self.h:
             if self.ctx.h:
-                _lib.mcpt_scene_destroy(self.h)
+                _check(_lib.mcpt_scene_destroy(self.h), "mcpt_scene_destroy")
             self.h = VP()
 
     def __del__(self):

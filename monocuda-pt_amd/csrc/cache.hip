@@ -810,6 +810,7 @@ int mcpt_mlp_create(mcpt_ctx ctx, uint64_t seed, mcpt_mlp* out) {
     MCPT_HIP(hipSetDevice(c->device));
     auto* mlp = new mcpt_mlp_s();
     mlp->ctx = c;
+    c->children++;  // mcpt_mlp_destroy gives it back, also on the error path below
     int s = MCPT_OK;
     if (s >= 0) s = mlp->master.alloc(c, kParams);
     if (s >= 0) s = mlp->m.alloc(c, kParams);
@@ -841,6 +842,7 @@ int mcpt_mlp_destroy(mcpt_mlp mlp) {
     mlp->wimg_ema.release(c);
     if (mlp->ws_mem) dev_free(c, mlp->ws_mem, mlp->ws_bytes);
     delete mlp;
+    c->children--;
     return MCPT_OK;
 }
 

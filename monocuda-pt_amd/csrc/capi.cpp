@@ -186,6 +186,10 @@ int mcpt_ctx_create(int device, uint64_t hbm_budget_bytes, mcpt_ctx* out) {
 
 int mcpt_ctx_destroy(mcpt_ctx ctx) {
     if (!ctx) return MCPT_OK;
+    const int live = ctx->c.children.load();
+    if (live > 0)
+        return set_error(MCPT_ERR_INVALID,
+                         "the context still has %d live scene/film/server/network handle(s); destroy them first", live);
     hipSetDevice(ctx->c.device);
     hipStreamSynchronize(ctx->c.stream);
     if (ctx->c.ws) dev_free(&ctx->c, ctx->c.ws, ctx->c.ws_bytes);
@@ -302,6 +306,7 @@ int mcpt_scene_create_ex(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_c
         delete s;
         return st;
     }
+    c->children++;
     *out = s;
     return MCPT_OK;
 }
@@ -475,10 +480,15 @@ int mcpt_load_exr(const char* path, int32_t* width, int32_t* height, float* rgba
 
 int mcpt_scene_destroy(mcpt_scene scene) {
     if (!scene) return MCPT_OK;
-    hipSetDevice(scene->ctx->device);
-    hipStreamSynchronize(scene->ctx->stream);
+    const int srv = scene->servers.load();
+    if (srv > 0)
+        return set_error(MCPT_ERR_INVALID, "the scene is in use by %d server(s); destroy them first", srv);
+    Ctx* c = scene->ctx;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
     scene->release();
     delete scene;
+    c->children--;
     return MCPT_OK;
 }
 
@@ -559,16 +569,19 @@ int mcpt_film_create(mcpt_ctx ctx, int32_t width, int32_t height, mcpt_film* out
         delete f;
         return st;
     }
+    ctx->c.children++;
     *out = f;
     return MCPT_OK;
 }
 
 int mcpt_film_destroy(mcpt_film film) {
     if (!film) return MCPT_OK;
-    hipSetDevice(film->ctx->device);
-    hipStreamSynchronize(film->ctx->stream);
-    film->accum.release(film->ctx);
+    Ctx* c = film->ctx;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    film->accum.release(c);
     delete film;
+    c->children--;
     return MCPT_OK;
 }
 

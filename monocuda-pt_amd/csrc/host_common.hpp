@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <mutex>
 #include <vector>
@@ -38,6 +39,10 @@ struct Ctx {
     int cu_count = 0;
     int clock_khz = 0;
     char name[256] = {0};
+    // live scenes, films, servers and networks made from this context; the
+    // context may not be destroyed while any remains (they hold its stream
+    // and allocator)
+    std::atomic<int> children{0};
     // reusable wavefront workspace (see render.hip)
     void* ws = nullptr;
     size_t ws_bytes = 0;

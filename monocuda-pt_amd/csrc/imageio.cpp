@@ -358,10 +358,12 @@ int read_exr(const char* path, int& w, int& h, std::vector<float>* rgba) {
     for (long long k = 0; k < chunks; k++) {
         uint64_t off;
         std::memcpy(&off, &d[at + k * 8], 8);
-        if (off + 8 > d.size()) return set_error(MCPT_ERR_INVALID, "'%s': bad chunk offset", path);
+        // compare without adding to the file's value, which may be near 2^64
+        if (off > d.size() || d.size() - off < 8)
+            return set_error(MCPT_ERR_INVALID, "'%s': bad chunk offset", path);
         const int y0 = (int)u32(off);
         const uint32_t n = u32(off + 4);
-        if (off + 8 + n > d.size()) return set_error(MCPT_ERR_INVALID, "'%s': truncated chunk", path);
+        if (n > d.size() - off - 8) return set_error(MCPT_ERR_INVALID, "'%s': truncated chunk", path);
         const long long first = (long long)y0 - ymin;
         if (first < 0 || first >= H) return set_error(MCPT_ERR_INVALID, "'%s': chunk outside the data window", path);
         const long long nl = std::min<long long>(lines, H - first);

@@ -796,6 +796,7 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
     s->st.film_budget = s->film_budget;
     s->worker = std::thread(srv_loop, s);
     for (mcpt_scene sc : s->scenes) sc->servers++;  // held until mcpt_server_destroy
+    c->children++;
     *out = s;
     return MCPT_OK;
 }
@@ -926,6 +927,7 @@ int mcpt_server_destroy(mcpt_server s) {
     (void)hipStreamDestroy(s->stream);
     dev_free(s->ctx, s->pool_mem, s->pool_bytes);
     for (mcpt_scene sc : s->scenes) sc->servers--;
+    s->ctx->children--;
     delete s;
     return MCPT_OK;
 }

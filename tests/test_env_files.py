@@ -252,6 +252,27 @@ def test_exr_channels_and_refusals(mcpt, tmp_path):
         mcpt.load_exr(_pfm(tmp_path / "not.exr", np.ones((2, 2, 4), np.float32)))
 
 
+@pytest.mark.parametrize("off,size", [(2**64 - 8, None), (2**64 - 1, None), (2**63, None), (None, 2**32 - 1),
+                                      (None, 2**32 - 16)])
+def test_exr_crafted_chunk_table(mcpt, tmp_path, off, size):
+    """Chunk offsets near 2^64 and chunk sizes near 2^32 are refused as such,
+    without wrapping past the bounds test."""
+    rgb = {"R": (2, np.ones((2, 2), np.float32)), "G": (2, np.zeros((2, 2), np.float32)),
+           "B": (2, np.ones((2, 2), np.float32))}
+    p = _exr(tmp_path / "crafted.exr", rgb, 0)
+    data = bytearray(open(p, "rb").read())
+    table = data.index(b"\0", data.index(b"screenWindowWidth\0float\0") + 30) + 1  # after the header's last byte
+    first = struct.unpack_from("<Q", data, table)[0]
+    if off is not None:
+        struct.pack_into("<Q", data, table, off)
+    else:
+        struct.pack_into("<I", data, first + 4, size)
+    open(p, "wb").write(bytes(data))
+    with pytest.raises(mcpt.McptError) as e:
+        mcpt.load_exr(p)
+    assert ("chunk offset" if off is not None else "truncated chunk") in str(e.value)
+
+
 EARTH = "/root/reference/models/earth.ppm"
 
 
