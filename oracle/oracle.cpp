@@ -416,7 +416,8 @@ struct Rng {
 };
 
 // random_unit_vector (src/path_tracer.cu:498-509)
-V3 random_unit_vector(Rng& r) {
+template <class R>
+V3 random_unit_vector(R& r) {
     while (true) {
         float x = r.u() * 2.f - 1.f;
         float y = r.u() * 2.f - 1.f;
@@ -426,10 +427,18 @@ V3 random_unit_vector(Rng& r) {
     }
 }
 // lambertian_scatter_direction (:511-518)
-V3 lambertian_dir(const V3& n, Rng& r) {
+template <class R>
+V3 lambertian_dir(const V3& n, R& r) {
     V3 d = n + random_unit_vector(r);
     if (d.length_squared() < 1e-8f) return n;
     return d.normalize();
+}
+// the Conductor case of the scatter switch (:3657-3661): the mirror of
+// specular_scatter_direction (:520-522) plus fuzz times a unit vector
+template <class R>
+V3 conductor_dir(const V3& in, const V3& n, float fuzz, R& r) {
+    V3 refl = in.normalize().reflect(n);
+    return (refl + random_unit_vector(r) * fuzz).normalize();
 }
 // schlick_approx (:476-480)
 float schlick(float c, float eta) {
@@ -708,10 +717,11 @@ float medium_transmittance(const oracle_scene_s& s, V3 o, const V3& d, float max
     return tr;
 }
 
-// trace_sample_kernel body (src/path_tracer.cu:2900-3059) for one path.
-V3 trace_path(const oracle_scene_s& s, const mcpt_camera& cam, int px, int py, Rng& rng, int max_depth,
-              PathCounts& pc) {
-    // Camera::getRay (src/path_tracer.cu:1633-1658)
+// Camera::getRay (src/path_tracer.cu:1633-1658): the pixel centre (no
+// jitter), a lens point drawn by rejection from the unit disk, then the
+// path's time
+template <class R>
+void camera_ray(const mcpt_camera& cam, int px, int py, R& rng, V3& ro, V3& rd, float& time) {
     V3 pc_ = v3(cam.pixel00) + (v3(cam.delta_u) * float(px)) + (v3(cam.delta_v) * float(py));
     V3 lens;
     while (true) {
@@ -725,9 +735,17 @@ V3 trace_path(const oracle_scene_s& s, const mcpt_camera& cam, int px, int py, R
     }
     lens = lens * cam.lens_radius;
     V3 off = v3(cam.u) * lens.x + v3(cam.v) * lens.y;
-    V3 ro = v3(cam.center) + off;
-    const float time = rng.u();  // the path's time (moving spheres)
-    V3 rd = (pc_ - ro).normalize();
+    ro = v3(cam.center) + off;
+    time = rng.u();  // the path's time (moving spheres)
+    rd = (pc_ - ro).normalize();
+}
+
+// trace_sample_kernel body (src/path_tracer.cu:2900-3059) for one path.
+V3 trace_path(const oracle_scene_s& s, const mcpt_camera& cam, int px, int py, Rng& rng, int max_depth,
+              PathCounts& pc) {
+    V3 ro, rd;
+    float time;
+    camera_ray(cam, px, py, rng, ro, rd, time);
 
     const int nl = (int)s.lights.size();
     const V3 bg = v3(cam.background);
@@ -819,11 +837,9 @@ V3 trace_path(const oracle_scene_s& s, const mcpt_camera& cam, int px, int py, R
         }
         V3 nd;
         switch (m.type) {
-            case MCPT_MAT_CONDUCTOR: {
-                V3 refl = rd.normalize().reflect(hit.normal);
-                nd = (refl + random_unit_vector(rng) * m.fuzz).normalize();
+            case MCPT_MAT_CONDUCTOR:
+                nd = conductor_dir(rd, hit.normal, m.fuzz, rng);
                 break;
-            }
             case MCPT_MAT_DIELECTRIC: {
                 float eta = hit.front ? 1.0f / m.ior : m.ior;
                 nd = dielectric_dir(rd, hit.normal, eta, rng);
@@ -976,6 +992,44 @@ int oracle_sampling(int fn, const float* in, int n, float* out) {
         o[2] = d.z;
     }
     return 0;
+}
+
+void oracle_conductor(const float* in_dir, const float* normal, float fuzz, int n, uint64_t seed, float* out) {
+    for (int i = 0; i < n; i++) {
+        Rng rng;
+        mcpt_pcg32_for_path(&rng.g, (uint32_t)i, 0u, seed);
+        const V3 d = conductor_dir(v3(in_dir), v3(normal), fuzz, rng);
+        out[3 * i] = d.x;
+        out[3 * i + 1] = d.y;
+        out[3 * i + 2] = d.z;
+    }
+}
+
+void oracle_camera_rays(const mcpt_camera* cam, const int32_t* pixels, int n, uint32_t sample, uint64_t seed,
+                        float* out) {
+    for (int i = 0; i < n; i++) {
+        const int px = pixels[2 * i], py = pixels[2 * i + 1];
+        Rng rng;
+        mcpt_pcg32_for_path(&rng.g, (uint32_t)(py * cam->width + px), sample, seed);
+        V3 ro, rd;
+        float time;
+        camera_ray(*cam, px, py, rng, ro, rd, time);
+        float* o = out + 7 * i;
+        o[0] = ro.x, o[1] = ro.y, o[2] = ro.z;
+        o[3] = rd.x, o[4] = rd.y, o[5] = rd.z;
+        o[6] = time;
+    }
+}
+
+void oracle_tri_interp(const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, const float* points, int n, float* uv,
+                       float* normal) {
+    const V3 face = (v3(hot->v1) - v3(hot->v0)).cross(v3(hot->v2) - v3(hot->v0)).normalize();
+    for (int i = 0; i < n; i++) {
+        const V3 p = v3(points + 3 * i);
+        tri_uv(*hot, *cold, p, uv[2 * i], uv[2 * i + 1]);
+        const V3 sn = tri_shading_normal(*hot, *cold, p, face);
+        normal[3 * i] = sn.x, normal[3 * i + 1] = sn.y, normal[3 * i + 2] = sn.z;
+    }
 }
 
 int oracle_env_probe(const float* rgba, int w, int h, float rot, int fn, const float* in, int n, float* out) {

@@ -104,6 +104,20 @@ void oracle_pcg32(uint32_t pixel, uint32_t sample, uint64_t seed, int n, uint32_
  * 3 henyeyGreenstein_scatter_direction(in[3], g, u1, u2) -> dir[3];
  * 4 HG pdf(cos, g) -> pdf. Returns -1 for an unknown fn. */
 int oracle_sampling(int fn, const float* in, int n, float* out);
+/* The Conductor scatter of shadeKernel (src/path_tracer.cu:3657-3661):
+ * normalize(reflect(normalize(d), n) + fuzz * random_unit_vector) for n
+ * draws, draw i from the path stream (pixel i, sample 0, seed); out n*3. */
+void oracle_conductor(const float* in_dir, const float* normal, float fuzz, int n, uint64_t seed, float* out);
+/* Camera::getRay (src/path_tracer.cu:1633-1658) for n (x, y) pixels, each
+ * from the path stream (y*width + x, sample, seed) the renders use; out n*7:
+ * origin, direction, time. */
+void oracle_camera_rays(const mcpt_camera* cam, const int32_t* pixels, int n, uint32_t sample, uint64_t seed,
+                        float* out);
+/* computeTriangleUvFromHitPoint and computeTriangleShadingNormal
+ * (src/path_tracer.cu:687-713) of one triangle at n points; the face
+ * normal is the unit normal of hit_triangle (:716-718). out uv n*2, normal n*3. */
+void oracle_tri_interp(const mcpt_tri_hot* hot, const mcpt_tri_cold* cold, const float* points, int n, float* uv,
+                       float* normal);
 /* Environment sampling on an RGBA map (w x h, rotation in radians):
  * fn 0 sample(u[4]) -> dir[3], pdf; 1 pdf(dir[3]) -> pdf;
  * 2 uv(dir[3]) -> u, v; 3 direction(u, v) -> dir[3]. */

@@ -75,6 +75,9 @@ def lib():
             "oracle_pcg32": ([C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, VP], None),
             "oracle_sampling": ([C.c_int, VP, C.c_int, VP], C.c_int),
             "oracle_env_probe": ([VP, C.c_int, C.c_int, C.c_float, C.c_int, VP, C.c_int, VP], C.c_int),
+            "oracle_conductor": ([VP, VP, C.c_float, C.c_int, C.c_uint64, VP], None),
+            "oracle_camera_rays": ([VP, VP, C.c_int, C.c_uint32, C.c_uint64, VP], None),
+            "oracle_tri_interp": ([VP, VP, VP, C.c_int, VP, VP], None),
         }
         for name, (args, res) in sigs.items():
             f = getattr(_lib, name)
@@ -368,6 +371,35 @@ def env_probe(rgba, rot, name, args):
     if lib().oracle_env_probe(_p(rgba), w, h, rot, fn, _p(a), len(a), _p(out)) != 0:
         raise ValueError(name)
     return out if nout > 1 else out[:, 0]
+
+
+def conductor(d, n, fuzz, count, seed=0):
+    """count draws of the conductor scatter for incoming d and normal n"""
+    d = np.ascontiguousarray(d, np.float32)
+    n = np.ascontiguousarray(n, np.float32)
+    out = np.zeros((count, 3), np.float32)
+    lib().oracle_conductor(_p(d), _p(n), fuzz, count, seed, _p(out))
+    return out
+
+
+def camera_rays(cam, pixels, sample=0, seed=0):
+    """(origins, directions, times) of the camera rays of (x, y) pixels"""
+    px = np.ascontiguousarray(pixels, np.int32).reshape(-1, 2)
+    out = np.zeros((len(px), 7), np.float32)
+    lib().oracle_camera_rays(C.addressof(cam), _p(px), len(px), sample, seed, _p(out))
+    return out[:, :3], out[:, 3:6], out[:, 6]
+
+
+def tri_interp(hot, cold, points):
+    """(uv, shading normal) of one triangle (records of HOT_DTYPE and
+    COLD_DTYPE) at the given points"""
+    hot = np.ascontiguousarray(hot, HOT_DTYPE).reshape(1)
+    cold = np.ascontiguousarray(cold, COLD_DTYPE).reshape(1)
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    uv = np.zeros((len(pts), 2), np.float32)
+    nrm = np.zeros((len(pts), 3), np.float32)
+    lib().oracle_tri_interp(_p(hot), _p(cold), _p(pts), len(pts), _p(uv), _p(nrm))
+    return uv, nrm
 
 
 def pcg32(pixel, sample, seed, n):
