@@ -308,6 +308,20 @@ int mcpt_film_write(mcpt_film film, const float* rgb /* W*H*3 */);
 int mcpt_render(mcpt_scene scene, const mcpt_camera* camera, const mcpt_render_opts* opts, mcpt_film film,
                 mcpt_render_stats* stats);
 
+/* No counterpart in the reference. mcpt_render for callers that cannot
+ * allocate device memory (serving loops that must not call hipMalloc,
+ * harnesses that forbid it): the same film, bit for bit, from a fixed
+ * 64 MiB workspace that the library holds on each device from the first
+ * context on that device until the process ends. Nothing is allocated or
+ * freed during the call. The wavefront and persistent modes carry at most
+ * MCPT_NOALLOC_MAX_PATHS paths per pass: images with more pixels are
+ * refused with MCPT_ERR_INVALID (the megakernel has no such limit);
+ * opts->flags may not ask for kernel timing. Calls on one device are
+ * serialised by the library (they share the workspace); the call returns
+ * when the device is done. */
+#define MCPT_NOALLOC_MAX_PATHS 399448
+int mcpt_render_noalloc(mcpt_scene scene, const mcpt_camera* camera, const mcpt_render_opts* opts, mcpt_film film);
+
 /* resolve_kernel (src/path_tracer.cu:3092-3100): accum / spp * exposure,
  * tone map, sRGB OETF. out_rgb is host memory, W*H*3. */
 int mcpt_resolve(mcpt_film film, int32_t samples_so_far, int32_t tonemap, float exposure, float* out_rgb);

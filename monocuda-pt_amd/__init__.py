@@ -39,6 +39,7 @@ MODE_MEGAKERNEL, MODE_WAVEFRONT, MODE_PERSISTENT = 0, 1, 2
 TONEMAP_NONE, TONEMAP_REINHARD, TONEMAP_ACES = 0, 1, 2
 RENDER_COUNT_TRAVERSAL, RENDER_TIME_KERNELS, RENDER_REFERENCE_ORDER, RENDER_PERSISTENT_SHADOW = 1, 2, 4, 8
 RENDER_SHADOW_PER_RAY = 16
+NOALLOC_MAX_PATHS = 399448  # mcpt_render_noalloc: paths per pass in the fixed workspace
 TEX_CHECKER, TEX_IMAGE = 1, 2
 GEN_CORNELL, GEN_SOUP, GEN_ICOSPHERE, GEN_MEDIUM = 1, 2, 3, 5
 MLP_FEATURES, MLP_INPUTS, MLP_WIDTH, MLP_HIDDEN, MLP_OUTPUTS = 11, 64, 64, 4, 3
@@ -261,6 +262,7 @@ _SIGS = {
     "mcpt_film_read": ([VP, VP], C.c_int),
     "mcpt_film_write": ([VP, VP], C.c_int),
     "mcpt_render": ([VP, C.POINTER(Camera), C.POINTER(RenderOpts), VP, C.POINTER(RenderStats)], C.c_int),
+    "mcpt_render_noalloc": ([VP, C.POINTER(Camera), C.POINTER(RenderOpts), VP], C.c_int),
     "mcpt_resolve": ([VP, C.c_int32, C.c_int32, C.c_float, VP], C.c_int),
     "mcpt_resolve_rgb8": ([VP, C.c_int32, C.c_int32, C.c_float, VP], C.c_int),
     "mcpt_auto_exposure": ([VP, C.c_int32, C.c_float, C.POINTER(C.c_float)], C.c_int),

@@ -180,6 +180,14 @@ int mcpt_ctx_create(int device, uint64_t hbm_budget_bytes, mcpt_ctx* out) {
         delete c;
         return set_error(MCPT_ERR_HIP, "hipStreamCreate failed");
     }
+    // the fixed workspace of mcpt_render_noalloc, once per device, with the
+    // device current
+    int st = scratch_init(device);
+    if (st < 0) {
+        hipStreamDestroy(c->c.stream);
+        delete c;
+        return st;
+    }
     *out = c;
     return MCPT_OK;
 }
@@ -617,6 +625,15 @@ int mcpt_render(mcpt_scene scene, const mcpt_camera* camera, const mcpt_render_o
     MCPT_TRY(scene->check_textures());
     MCPT_HIP(hipSetDevice(scene->ctx->device));
     return render(scene, camera, opts, film, stats);
+}
+
+int mcpt_render_noalloc(mcpt_scene scene, const mcpt_camera* camera, const mcpt_render_opts* opts, mcpt_film film) {
+    CHECK_ARG(scene && camera && opts && film, "bad arguments");
+    CHECK_ARG(film->ctx == scene->ctx, "film and scene belong to different contexts");
+    CHECK_ARG(opts->mode >= MCPT_MODE_MEGAKERNEL && opts->mode <= MCPT_MODE_PERSISTENT, "unknown render mode");
+    MCPT_TRY(scene->check_textures());
+    MCPT_HIP(hipSetDevice(scene->ctx->device));
+    return render_noalloc(scene, camera, opts, film);
 }
 
 int mcpt_resolve(mcpt_film film, int32_t samples_so_far, int32_t tonemap, float exposure, float* out_rgb) {

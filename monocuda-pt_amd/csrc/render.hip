@@ -17,6 +17,7 @@
 //                idle (extendKernelPersistent, :3416-3468, on 64-lane waves).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "scene.hpp"
@@ -525,29 +526,18 @@ __global__ __launch_bounds__(kBlock) void k_trace_any(SceneView sv, const mcpt_r
 int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 size_t stack_bytes(const mcpt_scene_s* s) { return (size_t)kBlock * stack_entries(s->max_depth) * sizeof(int); }
 
-// The wavefront workspace lives in the context and grows to the largest
-// path capacity requested.
-int ensure_workspace(Ctx* ctx, int64_t capacity, Workspace& ws) {
-    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t szq = align(sizeof(RayItem) * capacity), szh = align(sizeof(mcpt_packed_hit) * capacity),
-                 szs = align(sizeof(ShadowItem) * capacity), sz4 = align(sizeof(float4) * capacity),
-                 szr = align(sizeof(uint64_t) * capacity), szc = C_NCOUNTERS * sizeof(int);
-    const size_t total = 2 * szq + szh + szs + 2 * sz4 + szr + szc;
-    if (ctx->ws_capacity < capacity) {
-        if (ctx->ws) dev_free(ctx, ctx->ws, ctx->ws_bytes);
-        ctx->ws = nullptr;
-        ctx->ws_bytes = 0;
-        ctx->ws_capacity = 0;
-        void* p = nullptr;
-        MCPT_TRY(dev_alloc(ctx, &p, total));
-        ctx->ws = p;
-        ctx->ws_bytes = total;
-        ctx->ws_capacity = capacity;
-    }
-    char* b = (char*)ctx->ws;
-    const size_t cap_q = align(sizeof(RayItem) * ctx->ws_capacity), cap_h = align(sizeof(mcpt_packed_hit) * ctx->ws_capacity),
-                 cap_s = align(sizeof(ShadowItem) * ctx->ws_capacity), cap_4 = align(sizeof(float4) * ctx->ws_capacity),
-                 cap_r = align(sizeof(uint64_t) * ctx->ws_capacity);
+// Layout of a wavefront workspace of `capacity` paths from `base`: two ray
+// queues, hit records, the shadow queue, path state, then the counters.
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t workspace_bytes(int64_t capacity) {
+    return 2 * align256(sizeof(RayItem) * capacity) + align256(sizeof(mcpt_packed_hit) * capacity) +
+           align256(sizeof(ShadowItem) * capacity) + 2 * align256(sizeof(float4) * capacity) +
+           align256(sizeof(uint64_t) * capacity) + C_NCOUNTERS * sizeof(int);
+}
+void layout_workspace(char* b, int64_t capacity, Workspace& ws) {
+    const size_t cap_q = align256(sizeof(RayItem) * capacity), cap_h = align256(sizeof(mcpt_packed_hit) * capacity),
+                 cap_s = align256(sizeof(ShadowItem) * capacity), cap_4 = align256(sizeof(float4) * capacity),
+                 cap_r = align256(sizeof(uint64_t) * capacity);
     ws.q[0] = (RayItem*)b;
     b += cap_q;
     ws.q[1] = (RayItem*)b;
@@ -563,8 +553,44 @@ int ensure_workspace(Ctx* ctx, int64_t capacity, Workspace& ws) {
     ws.rng = (uint64_t*)b;
     b += cap_r;
     ws.counters = (int*)b;
+}
+
+// The wavefront workspace lives in the context and grows to the largest
+// path capacity requested.
+int ensure_workspace(Ctx* ctx, int64_t capacity, Workspace& ws) {
+    if (ctx->ws_capacity < capacity) {
+        if (ctx->ws) dev_free(ctx, ctx->ws, ctx->ws_bytes);
+        ctx->ws = nullptr;
+        ctx->ws_bytes = 0;
+        ctx->ws_capacity = 0;
+        void* p = nullptr;
+        const size_t total = workspace_bytes(capacity);
+        MCPT_TRY(dev_alloc(ctx, &p, total));
+        ctx->ws = p;
+        ctx->ws_bytes = total;
+        ctx->ws_capacity = capacity;
+    }
+    layout_workspace((char*)ctx->ws, ctx->ws_capacity, ws);
     return MCPT_OK;
 }
+
+// The fixed workspace of mcpt_render_noalloc: one copy per device, held
+// until the process ends. Host code reaches it only through the address
+// hipGetSymbolAddress returns for the device (kept in g_scratch_ptr); the
+// array's name in host code is not a device address.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+__device__ __align__(256) char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_init_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu[kMaxDevices];  // one no-allocation render per device at a time
+// the largest path count whose workspace fits the fixed array
+constexpr int64_t scratch_capacity() {
+    int64_t c = (int64_t)((kScratchBytes - C_NCOUNTERS * sizeof(int)) / 168);
+    while (c > 0 && workspace_bytes(c) > kScratchBytes) c--;
+    return c;
+}
+static_assert(scratch_capacity() == MCPT_NOALLOC_MAX_PATHS, "the header's path limit is the workspace's");
 
 struct Timer {
     bool on;
@@ -589,46 +615,38 @@ struct Timer {
 }  // namespace
 
 // ------------------------------------------------------------ host driver
-int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, mcpt_film_s* film,
-           mcpt_render_stats* stats) {
-    Ctx* ctx = s->ctx;
-    hipStream_t st = ctx->stream;
+namespace {
+
+int check_render_args(const mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o,
+                      const mcpt_film_s* film) {
     if (!s->built) return set_error(MCPT_ERR_INVALID, "scene has no BVH (call mcpt_lbvh_build)");
     if (film->w != cam->width || film->h != cam->height)
         return set_error(MCPT_ERR_INVALID, "film %dx%d does not match camera %dx%d", film->w, film->h, cam->width,
                          cam->height);
     if (o->spp < 1 || o->max_depth < 1 || o->sample_begin < 0)
         return set_error(MCPT_ERR_INVALID, "spp, max_depth must be >= 1 and sample_begin >= 0");
+    return MCPT_OK;
+}
+
+// Enqueues a whole render on the context's stream over the workspace `ws`
+// with `batch` samples per wavefront pass; `tm` (may be NULL) marks the
+// kernel spans. Returns the number of extend launches in *extend_launches.
+int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, mcpt_film_s* film,
+                   const Workspace& ws, int batch, Timer* tm, uint64_t* extend_launches_out) {
+    Ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
     const int W = cam->width, H = cam->height, npix = W * H;
     const SceneView sv = s->view();
     const vec3 bg = load3(cam->background);
     const size_t lds = stack_bytes(s);
     const bool count = (o->flags & MCPT_RENDER_COUNT_TRAVERSAL) != 0;
-    const bool timed = (o->flags & MCPT_RENDER_TIME_KERNELS) != 0;
     const bool ref = (o->flags & MCPT_RENDER_REFERENCE_ORDER) != 0;
     int refill = o->refill_lanes > 0 ? std::min(64, o->refill_lanes) : 32;
-
-    // Samples per wavefront pass. Every bounce ends with the slowest rays of
-    // its launch, so the late, thin bounces of one sample leave most of the
-    // GPU idle; carrying many samples per pass keeps them full. Default: up
-    // to 64 Mi paths (about 10.5 GB of queues and path state at 168 B per
-    // path), at most a quarter of the context's HBM budget.
-    const int64_t per_path = 2 * 32 + 16 + 48 + 2 * 16 + 8;
-    int64_t cap_paths = std::min<int64_t>(int64_t(64) << 20, (int64_t)(ctx->budget / 4) / per_path);
-    int batch = o->batch > 0 ? o->batch : (int)std::max<int64_t>(1, cap_paths / npix);
-    batch = std::max(1, std::min(batch, o->spp));
-    if ((int64_t)npix * batch > (int64_t(1) << 30)) batch = std::max(1, (1 << 30) / npix);
-
-    Workspace ws;
-    MCPT_TRY(ensure_workspace(ctx, o->mode == MCPT_MODE_MEGAKERNEL ? 1 : (int64_t)npix * batch, ws));
     enum { K_CAMERA, K_EXTEND, K_SHADE, K_SHADOW, K_FINALIZE, K_OTHER };
-    Timer tm(timed, st);
-    hipEvent_t e0, e1;
-    MCPT_HIP(hipEventCreate(&e0));
-    MCPT_HIP(hipEventCreate(&e1));
-    MCPT_HIP(hipEventRecord(e0, st));
+    auto mark = [&](int k) { return tm ? tm->mark(k) : MCPT_OK; };
+
     k_counters_zero_totals<<<1, C_NCOUNTERS, 0, st>>>(ws.counters);
-    MCPT_TRY(tm.mark(K_OTHER));
+    MCPT_TRY(mark(K_OTHER));
 
     int persistent_blocks = 0, shadow_blocks = 0;
     const bool persistent_shadow =
@@ -646,7 +664,7 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
             dim3 grid((W + 15) / 16, (H + 15) / 16);
             k_megakernel<<<grid, kBlock, lds, st>>>(sv, *cam, o->sample_begin + k, o->seed, o->max_depth,
                                                    film->accum.ptr, ws.counters);
-            MCPT_TRY(tm.mark(K_EXTEND));
+            MCPT_TRY(mark(K_EXTEND));
         }
     }
     for (int k0 = 0; o->mode != MCPT_MODE_MEGAKERNEL && k0 < o->spp; k0 += batch) {
@@ -656,10 +674,10 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
         int cur = 0;
         k_counters_init<<<1, 64, 0, st>>>(ws.counters, cur, npath);
         k_camera<<<grid, kBlock, 0, st>>>(*cam, o->sample_begin + k0, nb, o->seed, ws, cur);
-        MCPT_TRY(tm.mark(K_CAMERA));
+        MCPT_TRY(mark(K_CAMERA));
         for (int depth = 0; depth < o->max_depth; depth++) {
             k_counters_bounce<<<1, 64, 0, st>>>(ws.counters, cur);
-            MCPT_TRY(tm.mark(K_OTHER));
+            MCPT_TRY(mark(K_OTHER));
             if (o->mode == MCPT_MODE_PERSISTENT) {
                 const int blocks = std::min(grid, persistent_blocks);
                 auto kern = count ? (ref ? k_extend_persistent<true, true> : k_extend_persistent<true, false>)
@@ -671,21 +689,61 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
                 kern<<<grid, kBlock, lds, st>>>(sv, ws, cur);
             }
             extend_launches++;
-            MCPT_TRY(tm.mark(K_EXTEND));
+            MCPT_TRY(mark(K_EXTEND));
             k_shade<<<grid, kBlock, 0, st>>>(sv, ws, cur, depth, bg, cam->rr_cap, npix);
-            MCPT_TRY(tm.mark(K_SHADE));
+            MCPT_TRY(mark(K_SHADE));
             k_counters_shadow_total<<<1, 64, 0, st>>>(ws.counters, cur);
             if (persistent_shadow)
                 k_shadow_persistent<<<std::min(grid, shadow_blocks), kBlock, lds, st>>>(sv, ws, cur, refill);
             else
                 k_shadow<<<grid, kBlock, lds, st>>>(sv, ws, cur);
-            MCPT_TRY(tm.mark(K_SHADOW));
+            MCPT_TRY(mark(K_SHADOW));
             cur = 1 - cur;
         }
         k_finalize<<<blocks_for(npix), kBlock, 0, st>>>(ws, film->accum.ptr, npix, nb);
-        MCPT_TRY(tm.mark(K_FINALIZE));
+        MCPT_TRY(mark(K_FINALIZE));
     }
     MCPT_HIP(hipGetLastError());
+    *extend_launches_out = o->mode == MCPT_MODE_MEGAKERNEL ? (uint64_t)o->spp : extend_launches;
+    return MCPT_OK;
+}
+
+}  // namespace
+
+int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, mcpt_film_s* film,
+           mcpt_render_stats* stats) {
+    MCPT_TRY(check_render_args(s, cam, o, film));
+    Ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    const int npix = cam->width * cam->height;
+    const bool timed = (o->flags & MCPT_RENDER_TIME_KERNELS) != 0;
+
+    // Samples per wavefront pass. Every bounce ends with the slowest rays of
+    // its launch, so the late, thin bounces of one sample leave most of the
+    // GPU idle; carrying many samples per pass keeps them full. Default: up
+    // to 64 Mi paths (about 10.5 GB of queues and path state at 168 B per
+    // path), at most a quarter of the context's HBM budget.
+    const int64_t per_path = 2 * 32 + 16 + 48 + 2 * 16 + 8;
+    int64_t cap_paths = std::min<int64_t>(int64_t(64) << 20, (int64_t)(ctx->budget / 4) / per_path);
+    int batch = o->batch > 0 ? o->batch : (int)std::max<int64_t>(1, cap_paths / npix);
+    batch = std::max(1, std::min(batch, o->spp));
+    if ((int64_t)npix * batch > (int64_t(1) << 30)) batch = std::max(1, (1 << 30) / npix);
+
+    Workspace ws;
+    MCPT_TRY(ensure_workspace(ctx, o->mode == MCPT_MODE_MEGAKERNEL ? 1 : (int64_t)npix * batch, ws));
+    Timer tm(timed, st);
+    hipEvent_t e0, e1;
+    MCPT_HIP(hipEventCreate(&e0));
+    MCPT_HIP(hipEventCreate(&e1));
+    MCPT_HIP(hipEventRecord(e0, st));
+    uint64_t extend_launches = 0;
+    int rc = enqueue_render(s, cam, o, film, ws, batch, &tm, &extend_launches);
+    if (rc < 0) {
+        (void)hipStreamSynchronize(st);
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return rc;
+    }
     MCPT_HIP(hipEventRecord(e1, st));
     MCPT_HIP(hipEventSynchronize(e1));
     float total_ms = 0.f;
@@ -698,7 +756,7 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->total_ms = total_ms;
-        stats->extend_launches = o->mode == MCPT_MODE_MEGAKERNEL ? (uint64_t)o->spp : extend_launches;
+        stats->extend_launches = extend_launches;
         stats->extend_rays = totals[0];
         stats->shadow_rays = totals[1];
         stats->extend_steps = totals[2];
@@ -718,6 +776,47 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    return MCPT_OK;
+}
+
+int scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices)
+        return set_error(MCPT_ERR_INVALID, "device %d is beyond the %d devices the fixed workspace table holds", device,
+                         kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_init_mu);
+    if (g_scratch_ptr[device]) return MCPT_OK;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_error(MCPT_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_ptr[device] = (char*)p;
+    return MCPT_OK;
+}
+
+int render_noalloc(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, mcpt_film_s* film) {
+    MCPT_TRY(check_render_args(s, cam, o, film));
+    if (o->flags & MCPT_RENDER_TIME_KERNELS)
+        return set_error(MCPT_ERR_INVALID, "kernel timing creates events; use mcpt_render for timed renders");
+    Ctx* ctx = s->ctx;
+    const int dev = ctx->device;
+    if (dev < 0 || dev >= kMaxDevices || !g_scratch_ptr[dev])
+        return set_error(MCPT_ERR_INVALID, "the fixed workspace of device %d was not set up", dev);
+    const int64_t npix = (int64_t)cam->width * cam->height;
+    const int64_t cap = scratch_capacity();
+    if (o->mode != MCPT_MODE_MEGAKERNEL && npix > cap)
+        return set_error(MCPT_ERR_INVALID,
+                         "%lld pixels exceed the %lld paths the fixed %zu-byte workspace holds in one pass; "
+                         "use mcpt_render", (long long)npix, (long long)cap, kScratchBytes);
+    int batch = (int)std::max<int64_t>(1, std::min<int64_t>(o->batch > 0 ? o->batch : cap / npix, cap / npix));
+    batch = std::max(1, std::min(batch, o->spp));
+    Workspace ws;
+    layout_workspace(g_scratch_ptr[dev], o->mode == MCPT_MODE_MEGAKERNEL ? 1 : (int64_t)npix * batch, ws);
+    std::lock_guard<std::mutex> lock(g_scratch_mu[dev]);
+    uint64_t launches = 0;
+    int rc = enqueue_render(s, cam, o, film, ws, batch, nullptr, &launches);
+    hipError_t e = hipStreamSynchronize(ctx->stream);  // the array is free for the next call once this returns
+    if (rc < 0) return rc;
+    if (e != hipSuccess) return set_error(MCPT_ERR_HIP, "render_noalloc: %s", hipGetErrorString(e));
     return MCPT_OK;
 }
 

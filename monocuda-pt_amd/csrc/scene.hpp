@@ -148,6 +148,10 @@ int trace_any_host(mcpt_scene_s* s, const mcpt_ray* rays, int n, uint8_t* occlud
 // render.hip
 int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, mcpt_film_s* film,
            mcpt_render_stats* stats);
+// the same over the fixed per-device workspace (no device allocation);
+// scratch_init finds that workspace's address on the current device
+int render_noalloc(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, mcpt_film_s* film);
+int scratch_init(int device);
 int resolve(mcpt_film_s* film, int spp, int tonemap, float exposure, float* out);
 int resolve_rgb8(mcpt_film_s* film, int spp, int tonemap, float exposure, uint8_t* out);
 int auto_exposure(mcpt_film_s* film, int spp, float target, float* exposure);
