@@ -42,7 +42,29 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(hot, cold, mats, sky, desc, depth, seed, budget_s, threads, spheres=None, grid=None):
+def host_threads():
+    """Threads this process may run on (its CPU affinity set), which on a
+    shared box can be fewer than os.cpu_count()."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(hot, cold, mats, sky, desc, depth, seed, budget_s, threads, spheres=None, grid=None, runs=3):
+    """The CPU oracle on a band of rows sized to about budget_s / runs per
+    run; the median of `runs` timed runs of the same band."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
@@ -53,24 +75,30 @@ def cpu_baseline(hot, cold, mats, sky, desc, depth, seed, budget_s, threads, sph
     build_s = time.time() - t0
     cam = pyoracle.camera_init(desc)
     H = cam.height
-    # calibrate on a thin band, then size the band to ~budget_s
+    # calibrate on a thin band, then size the band to ~budget_s / runs
     mid = H // 2
     rows = 8
     _, _, st = ref.render(cam, 1, depth, seed=seed, rows=(mid, mid + rows), threads=threads)
     per_row = max(st["seconds"] / rows, 1e-4)
-    rows = int(max(8, min(H, budget_s / per_row)))
-    spp = 1 if rows < H else int(max(1, min(64, budget_s / (per_row * H))))
+    per_run = budget_s / runs
+    rows = int(max(8, min(H, per_run / per_row)))
+    spp = 1 if rows < H else int(max(1, min(64, per_run / (per_row * H))))
     r0 = max(0, mid - rows // 2)
-    _, _, st = ref.render(cam, spp, depth, seed=seed, rows=(r0, r0 + rows), threads=threads)
-    secs = st["seconds"]
+    times = []
+    for _ in range(runs):
+        _, _, st = ref.render(cam, spp, depth, seed=seed, rows=(r0, r0 + rows), threads=threads)
+        times.append(st["seconds"])
+    secs = sorted(times)[len(times) // 2]
     return {
         "value": (st["extend_rays"] + st["shadow_rays"]) / secs / 1e6,
         "unit": "Mrays/s",
         "paths_per_s": st["paths"] / secs,
         "cores": st["threads"],
         "kind": "port",
-        "sample": f"rows {r0}-{r0 + rows - 1} of {desc.width}x{H}, {spp} spp, depth {depth}, {secs:.1f} s "
-                  f"(BVH build {build_s:.1f} s not timed)",
+        "cpu": cpu_model(),
+        "runs_s": [round(t, 3) for t in times],
+        "sample": f"rows {r0}-{r0 + rows - 1} of {desc.width}x{H}, {spp} spp, depth {depth}, median of {runs} "
+                  f"runs of {secs:.1f} s on {st['threads']} threads (BVH build {build_s:.1f} s not timed)",
     }
 
 
@@ -566,7 +594,7 @@ def cloud_bench(mcpt, args):
            "grid_MB": grid[0].nbytes / 1e6, "image_mean": float(img.mean() / K)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(hot, cold, mats, None, desc, depth, args.seed, args.cpu_seconds,
-                                           min(os.cpu_count() or 1, 16), spheres=sp, grid=grid)
+                                           host_threads(), spheres=sp, grid=grid)
     return out
 
 
@@ -807,7 +835,7 @@ def main():
             "note": "the reference's left-first visit order does this many bytes of work for the same hits",
         }
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         result["cpu_baseline"] = cpu_baseline(hot, cold, mats, sky, desc, args.depth, args.seed, args.cpu_seconds,
                                               threads)
     if args.workload == "all":

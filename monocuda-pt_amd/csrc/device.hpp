@@ -130,36 +130,6 @@ MCPT_DEV bool slab(float mnx, float mny, float mnz, float mxx, float mxy, float 
     return !(tfar <= tnear);
 }
 
-// Box test of the fast traversal orders: (b - o) / d as fma(b, 1/d, -o/d).
-// It may round the entry and exit distances differently from the reference
-// by an ulp, which only matters for boxes the ray barely touches; every
-// triangle sits at least 1e-3 inside its boxes, so such a box cannot hold a
-// hit, and the exact triangle test alone decides t and the primitive. The
-// reciprocals are finite (trav_begin clamps them), so no axis turns NaN.
-MCPT_DEV bool slab_fma(float mnx, float mny, float mnz, float mxx, float mxy, float mxz, const vec3& oi,
-                       const vec3& inv, float tfar, float& tnear_out) {
-    float tnear = 0.0001f;
-    float t0 = __builtin_fmaf(mnx, inv.x, -oi.x), t1 = __builtin_fmaf(mxx, inv.x, -oi.x);
-    float lo = inv.x < 0.0f ? t1 : t0, hi = inv.x < 0.0f ? t0 : t1;
-    tnear = fmaxf(tnear, lo);
-    tfar = fminf(tfar, hi);
-    t0 = __builtin_fmaf(mny, inv.y, -oi.y);
-    t1 = __builtin_fmaf(mxy, inv.y, -oi.y);
-    lo = inv.y < 0.0f ? t1 : t0;
-    hi = inv.y < 0.0f ? t0 : t1;
-    tnear = fmaxf(tnear, lo);
-    tfar = fminf(tfar, hi);
-    t0 = __builtin_fmaf(mnz, inv.z, -oi.z);
-    t1 = __builtin_fmaf(mxz, inv.z, -oi.z);
-    lo = inv.z < 0.0f ? t1 : t0;
-    hi = inv.z < 0.0f ? t0 : t1;
-    tnear = fmaxf(tnear, lo);
-    tfar = fminf(tfar, hi);
-    tnear_out = tnear;
-    return !(tfar <= tnear);
-}
-
-
 // unit normal exactly as hit_triangle forms it (src/path_tracer.cu:716-718)
 MCPT_HDI vec3 triangle_normal(const vec3& v0, const vec3& v1, const vec3& v2) {
     vec3 e1 = v1 - v0;

@@ -27,7 +27,6 @@ namespace mcpt {
 
 struct Trav {
     vec3 o, d, inv;
-    vec3 oi;   // o * inv, for the fused box test of the fast orders
     float t;   // running closest distance (or the any-hit limit)
     float time;  // the ray's time (where moving spheres are)
     int leaf;  // closest leaf so far, -1 = none
@@ -52,7 +51,6 @@ MCPT_DEV void trav_begin(Trav& s, const vec3& o, const vec3& d, float tmax) {
     if (!EXACT)
         s.inv = vec3(fminf(fmaxf(s.inv.x, -1e30f), 1e30f), fminf(fmaxf(s.inv.y, -1e30f), 1e30f),
                      fminf(fmaxf(s.inv.z, -1e30f), 1e30f));
-    s.oi = vec3(o.x * s.inv.x, o.y * s.inv.y, o.z * s.inv.z);
     s.t = tmax;
     s.time = 0.0f;
     s.leaf = -1;
@@ -141,12 +139,16 @@ MCPT_DEV bool step_closest(const BvhView& bv, Trav& s, int* stk, int stride, uin
 // ---------------------------------------------------- four-wide walk
 // The fast orders walk the four-wide quantized tree (WideNode). Child box k
 // of node w spans w.o + q * 2^e per axis; its slab distances are formed as
-// fma(q, 2^e / d, (w.o - o) / d), i.e. (b - o) / d with a few roundings of
-// coordinate-sized terms. As with slab_fma, that can move a slab end by a
-// few ulps of the coordinates, far less than the 1e-3 every triangle sits
-// inside its boxes, and the quantized boxes contain the exact ones, so no
-// box that holds a reachable hit is culled and the exact triangle test alone
-// decides t and the primitive.
+// fma(q, 2^e / d, (w.o - o) / d), i.e. (b - o) / d with four roundings, each
+// relative to |w.o - o| or to the node's extent, as in the reference's
+// (b - o) * (1/d). A slab end therefore moves by at most about
+// 4 * 2^-24 * (|w.o - o| + extent) along its axis, below the 1e-3 every
+// triangle sits inside its boxes while the node lies within ~4000 units of
+// the ray's origin (the same bound under which the reference's own slab test
+// cannot cull a reachable hit), whatever the absolute coordinates; and the
+// quantized boxes contain the exact ones. So no box that holds a reachable
+// hit is culled and the exact triangle test alone decides t and the
+// primitive.
 //
 // Ties: the reference tests a leaf when it visits the leaf's parent, parents
 // in left-first pre-order, left side first, and keeps the first of equal t.
@@ -166,9 +168,9 @@ MCPT_DEV uint32_t wide_boxes(const float4& o, const uint4& q0, const uint4& q1, 
     const float ax = __uint_as_float((meta & 255u) << 23) * s.inv.x;
     const float ay = __uint_as_float(((meta >> 8) & 255u) << 23) * s.inv.y;
     const float az = __uint_as_float(((meta >> 16) & 255u) << 23) * s.inv.z;
-    const float bx = __builtin_fmaf(o.x, s.inv.x, -s.oi.x);
-    const float by = __builtin_fmaf(o.y, s.inv.y, -s.oi.y);
-    const float bz = __builtin_fmaf(o.z, s.inv.z, -s.oi.z);
+    const float bx = (o.x - s.o.x) * s.inv.x;
+    const float by = (o.y - s.o.y) * s.inv.y;
+    const float bz = (o.z - s.o.z) * s.inv.z;
     const bool rx = s.inv.x < 0.0f, ry = s.inv.y < 0.0f, rz = s.inv.z < 0.0f;
     const uint32_t nx = rx ? q0.w : q0.x, fx = rx ? q0.x : q0.w;
     const uint32_t ny = ry ? q1.x : q0.y, fy = ry ? q0.y : q1.x;

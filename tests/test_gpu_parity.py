@@ -522,3 +522,43 @@ def test_render_soup_sky(mcpt, ctx, oracle):
         assert np.array_equal(img, base), f"mode {mode} differs from the megakernel"
         assert rel_mse(img, ref_img) < 1e-4
     scene.close()
+
+
+@pytest.mark.parametrize("scale,offset", [(1000.0, (4096.0, -8192.0, 12288.0)),
+                                          (2000.0, (-20000.0, 9000.0, 25000.0))])
+def test_trace_large_coordinates(mcpt, ctx, oracle, scale, offset):
+    """The 64k-triangle soup scaled up and moved far from the origin, so that
+    coordinates are 1e4-3e4 while the 1e-3 box padding stays the same: both
+    visit orders return the reference's hits, for rays inside the mesh and
+    primary rays from a camera 3000 units away."""
+    hot, cold, mats = mcpt.gen_scene(mcpt.GEN_SOUP, seed=12, param=65536)
+    off = np.array(offset)
+    for k in ("v0", "v1", "v2"):
+        hot[k] = (hot[k].astype(np.float64) * scale + off).astype(np.float32)
+    n = 1 << 16
+    rng = np.random.default_rng(8)
+    rays = np.zeros(n, mcpt_ray_dtype())
+    rays["origin"] = (rng.uniform(-0.1, 1.1, (n, 3)) * scale + off).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    rays["dir"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    far = np.arange(n) % 4 == 0  # from a point 1.5 scene sizes in front, aimed into the mesh
+    eye = off + scale * np.array([0.5, 0.5, -1.5])
+    tgt = rng.uniform(0.0, 1.0, (far.sum(), 3)) * scale + off
+    dd = tgt - eye
+    rays["origin"][far] = eye.astype(np.float32)
+    rays["dir"][far] = (dd / np.linalg.norm(dd, axis=1, keepdims=True)).astype(np.float32)
+    rays["tmax"] = np.float32(1e30)
+    print(f"trace large coordinates scale {scale} offset {offset}", flush=True)
+    scene = ctx.scene(hot, cold, mats)
+    scene.build()
+    ref = oracle.Scene(hot, cold, mats)
+    rh, rs, rl = ref.trace_closest(rays)
+    assert rh["did_hit"].mean() > 0.1
+    hits, steps, leaves = scene.trace_closest(rays, counts=True)
+    assert np.array_equal(hits, rh) and np.array_equal(steps, rs) and np.array_equal(leaves, rl)
+    fast = scene.trace_closest(rays)
+    bad = np.flatnonzero(fast != rh)
+    assert bad.size == 0, f"four-wide walk differs on {bad.size} rays, e.g. {bad[:5]}: {fast[bad[:3]]} vs {rh[bad[:3]]}"
+    rays["tmax"] = np.float32(0.05 * scale)
+    assert np.array_equal(scene.trace_any(rays), ref.trace_any(rays))
+    scene.close()
