@@ -231,53 +231,65 @@ __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Work
 
 // ----------------------------------------------------------------- shade
 // shadeKernel (:3470-3696)
-__global__ __launch_bounds__(kBlock) void k_shade(SceneView sv, Workspace ws, int cur, int depth, vec3 bg, float rr_cap,
-                                                  int npix) {
-    const int slot = blockIdx.x * kBlock + threadIdx.x;
+// Grid-stride over chunks of kShadeBlock queue slots (a grid sized to the
+// occupancy limit, so the thin late bounces launch no empty waves), each
+// chunk ordered by shading bin before it is shaded.
+constexpr int kShadeBlock = 1024;
+__global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView sv, Workspace ws, int cur, int depth, vec3 bg,
+                                                       float rr_cap, int npix) {
+    __shared__ int lds[bin_sort_lds_ints<kShadeBlock>()];
     const int count = queue_len(ws.counters, cur);
-    const bool active = slot < count;
-    Bounce out;
-    out.next = false;
-    out.shadow = false;
-    int path = 0;
-    float time = 0.f;
-    if (active) {
-        const RayItem it = ws.q[cur][slot];
-        const mcpt_packed_hit hit = ws.hits[slot];
-        path = __float_as_int(it.dp.w);
-        time = it.ot.w;
-        const vec3 ro(it.ot.x, it.ot.y, it.ot.z), rd(it.dp.x, it.dp.y, it.dp.z);
-        const float4 thr = ws.thr[path], smp = ws.smp[path];
-        PathState ps;
-        ps.thr = vec3(thr.x, thr.y, thr.z);
-        ps.prev_pdf = thr.w;
-        ps.smp = vec3(smp.x, smp.y, smp.z);
-        ps.prev_spec = smp.w != 0.0f;
-        Rng rng;
-        rng.g.state = ws.rng[path];
-        rng.g.inc = ((uint64_t)(uint32_t)(path % npix) << 1u) | 1u;  // the stream of the path's pixel
-        const int prim = hit.prim_type == 0 ? ~hit.prim_index : hit.prim_index;
-        shade_bounce(sv, bg, rr_cap, depth, ro, rd, hit.did_hit != 0, hit.t, prim, ps, rng, out, time);
-        ws.thr[path] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
-        ws.smp[path] = make_float4(ps.smp.x, ps.smp.y, ps.smp.z, ps.prev_spec ? 1.0f : 0.0f);
-        ws.rng[path] = rng.g.state;
-    }
-    // compacted pushes: every lane of the wave takes part
-    int nslot, sslot;
-    wave_push2(ns_slot(ws.counters, 1 - cur), out.next, out.shadow, nslot, sslot);
-    if (out.next) {
-        RayItem it;
-        it.ot = make_float4(out.no.x, out.no.y, out.no.z, time);
-        it.dp = make_float4(out.nd.x, out.nd.y, out.nd.z, __int_as_float(path));
-        ws.q[1 - cur][nslot] = it;
-    }
-
-    if (out.shadow) {
-        ShadowItem si;
-        si.om = make_float4(out.so.x, out.so.y, out.so.z, out.smax);
-        si.dp = make_float4(out.sd.x, out.sd.y, out.sd.z, __int_as_float(path));
-        si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, time);
-        ws.sq[sslot] = si;
+    for (int base = blockIdx.x * kShadeBlock; base < count; base += gridDim.x * kShadeBlock) {
+        int slot = base + (int)threadIdx.x, mat = -1, bin = kShadeBins;
+        mcpt_packed_hit hit;
+        if (slot < count) {
+            hit = ws.hits[slot];
+            bin = shade_bin(sv, hit.did_hit != 0, hit.prim_type == 0 ? ~hit.prim_index : hit.prim_index, mat);
+        }
+        block_bin_sort<kShadeBlock>(bin, slot, mat, lds);
+        const bool active = slot >= 0;
+        Bounce out;
+        out.next = false;
+        out.shadow = false;
+        int path = 0;
+        float time = 0.f;
+        if (active) {
+            const RayItem it = ws.q[cur][slot];
+            hit = ws.hits[slot];
+            path = __float_as_int(it.dp.w);
+            time = it.ot.w;
+            const vec3 ro(it.ot.x, it.ot.y, it.ot.z), rd(it.dp.x, it.dp.y, it.dp.z);
+            const float4 thr = ws.thr[path], smp = ws.smp[path];
+            PathState ps;
+            ps.thr = vec3(thr.x, thr.y, thr.z);
+            ps.prev_pdf = thr.w;
+            ps.smp = vec3(smp.x, smp.y, smp.z);
+            ps.prev_spec = smp.w != 0.0f;
+            Rng rng;
+            rng.g.state = ws.rng[path];
+            rng.g.inc = ((uint64_t)(uint32_t)(path % npix) << 1u) | 1u;  // the stream of the path's pixel
+            const int prim = hit.prim_type == 0 ? ~hit.prim_index : hit.prim_index;
+            shade_bounce(sv, bg, rr_cap, depth, ro, rd, hit.did_hit != 0, hit.t, prim, ps, rng, out, time, mat);
+            ws.thr[path] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
+            ws.smp[path] = make_float4(ps.smp.x, ps.smp.y, ps.smp.z, ps.prev_spec ? 1.0f : 0.0f);
+            ws.rng[path] = rng.g.state;
+        }
+        // compacted pushes: every lane of the wave takes part
+        int nslot, sslot;
+        wave_push2(ns_slot(ws.counters, 1 - cur), out.next, out.shadow, nslot, sslot);
+        if (out.next) {
+            RayItem it;
+            it.ot = make_float4(out.no.x, out.no.y, out.no.z, time);
+            it.dp = make_float4(out.nd.x, out.nd.y, out.nd.z, __int_as_float(path));
+            ws.q[1 - cur][nslot] = it;
+        }
+        if (out.shadow) {
+            ShadowItem si;
+            si.om = make_float4(out.so.x, out.so.y, out.so.z, out.smax);
+            si.dp = make_float4(out.sd.x, out.sd.y, out.sd.z, __int_as_float(path));
+            si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, time);
+            ws.sq[sslot] = si;
+        }
     }
 }
 
@@ -658,6 +670,12 @@ int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
         MCPT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_shadow_persistent, kBlock, lds));
         shadow_blocks = std::max(1, per_cu) * ctx->cu_count;
     }
+    int shade_blocks = 0;
+    {
+        int per_cu = 0;
+        MCPT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_shade, kShadeBlock, 0));
+        shade_blocks = std::max(1, per_cu) * ctx->cu_count;
+    }
     uint64_t extend_launches = 0;
     if (o->mode == MCPT_MODE_MEGAKERNEL) {
         for (int k = 0; k < o->spp; k++) {
@@ -690,7 +708,8 @@ int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
             }
             extend_launches++;
             MCPT_TRY(mark(K_EXTEND));
-            k_shade<<<grid, kBlock, 0, st>>>(sv, ws, cur, depth, bg, cam->rr_cap, npix);
+            k_shade<<<std::min((npath + kShadeBlock - 1) / kShadeBlock, shade_blocks), kShadeBlock, 0, st>>>(
+                sv, ws, cur, depth, bg, cam->rr_cap, npix);
             MCPT_TRY(mark(K_SHADE));
             k_counters_shadow_total<<<1, 64, 0, st>>>(ws.counters, cur);
             if (persistent_shadow)

@@ -330,12 +330,79 @@ struct Bounce {
     vec3 contrib;
 };
 
+// ------------------------------------------------------- shading bins
+// Lanes of one wave run every branch any of them takes, and the branches of
+// a bounce differ by material: only diffuse surfaces and media sample a
+// light, mirrors and glass each scatter their own way. Shade kernels
+// therefore order each block's queue slots by bin before shading them, so
+// most waves take one branch. The order of shading does not change any
+// path's result (every path draws from its own stream and owns its state).
+constexpr int kShadeBins = 5;  // miss, emitter, light-sampling surface or medium, conductor, dielectric
+MCPT_DEV int shade_bin(const SceneView& sv, bool did_hit, int prim, int& mat) {
+    if (!did_hit) {
+        mat = -1;
+        return 0;
+    }
+    mat = prim_material(sv, prim);
+    const int type = sv.mats[mat].type;
+    return type == MCPT_MAT_EMISSIVE ? 1 : type == MCPT_MAT_CONDUCTOR ? 3 : type == MCPT_MAT_DIELECTRIC ? 4 : 2;
+}
+
+// LDS ints block_bin_sort<B> needs
+template <int B>
+constexpr int bin_sort_lds_ints() {
+    return 2 * kShadeBins * (B / 64) + 2 * B;
+}
+
+// Stable block-wide counting sort of (item, aux) pairs by bin (bin =
+// kShadeBins marks a thread without an item). Afterwards thread i holds the
+// i-th pair in (bin, thread) order; threads past the number of items get
+// item = -1. Every thread of the block must call it.
+template <int B>
+MCPT_DEV void block_bin_sort(int bin, int& item, int& aux, int* lds) {
+    constexpr int W = B / 64;
+    int* cnt = lds;                      // [kShadeBins][W]
+    int* off = lds + kShadeBins * W;     // [kShadeBins][W]
+    int* s_item = lds + 2 * kShadeBins * W;
+    int* s_aux = s_item + B;
+    const int wave = (int)threadIdx.x >> 6;
+    int rank = 0;
+#pragma unroll
+    for (int b = 0; b < kShadeBins; b++) {
+        const uint64_t m = ballot64(bin == b);
+        if (bin == b) rank = rank_below(m);
+        if (lane_id() == 0) cnt[b * W + wave] = __popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int i = 0; i < kShadeBins * W; i++) {
+            off[i] = acc;
+            acc += cnt[i];
+        }
+        cnt[0] = acc;  // the number of items, read below
+    }
+    __syncthreads();
+    if (bin < kShadeBins) {
+        const int at = off[bin * W + wave] + rank;
+        s_item[at] = item;
+        s_aux[at] = aux;
+    }
+    __syncthreads();
+    const int n = cnt[0];
+    item = (int)threadIdx.x < n ? s_item[threadIdx.x] : -1;
+    aux = (int)threadIdx.x < n ? s_aux[threadIdx.x] : -1;
+    __syncthreads();  // the arrays are free for the next call
+}
+
 // Everything one bounce does after the closest hit is known: the miss and
 // emission terms, free flight in a medium, albedo, next-event estimation,
 // Russian roulette and the scatter (:2917-3048, :3486-3693).
+// mat is the primitive's material index when the caller has it (-1: look
+// it up).
 MCPT_DEV void shade_bounce(const SceneView& sv, const vec3& bg, float rr_cap, int depth, const vec3& ro,
                            const vec3& rd, bool did_hit, float t, int prim, PathState& ps, Rng& rng, Bounce& out,
-                           float time = 0.0f) {
+                           float time = 0.0f, int mat = -1) {
     out.next = false;
     out.shadow = false;
     const int nl = sv.n_lights;
@@ -346,7 +413,7 @@ MCPT_DEV void shade_bounce(const SceneView& sv, const vec3& bg, float rr_cap, in
         ps.smp = ps.smp + ps.thr * env_radiance(sv.env, rd, bg) * w;
         return;
     }
-    const mcpt_material m = sv.mats[prim_material(sv, prim)];
+    const mcpt_material m = sv.mats[mat >= 0 ? mat : prim_material(sv, prim)];
     HitInfo h;
     rebuild_hit(sv, prim, ro, rd, t, time, h);
     if (m.type == MCPT_MAT_EMISSIVE) {
