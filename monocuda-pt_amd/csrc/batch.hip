@@ -418,9 +418,20 @@ __global__ __launch_bounds__(kBlock) void k_bb_extend(const BvhView* bvhs, BWs w
 }
 
 // shadeKernel (:3470-3696) with the ray's own scene and camera
-__global__ __launch_bounds__(kBlock) void k_bb_shade(const BScene* tab, BWs ws, int cur, int depth, int P) {
-    const int slot = blockIdx.x * kBlock + threadIdx.x;
-    const bool active = slot < queue_len(ws.counters, cur);
+// Each block's slots are shaded in shading-bin order (block_bin_sort).
+constexpr int kShadeBlock = 1024;
+__global__ __launch_bounds__(kShadeBlock) void k_bb_shade(const BScene* tab, BWs ws, int cur, int depth, int P) {
+    __shared__ int lds[bin_sort_lds_ints<kShadeBlock>()];
+    const int count = queue_len(ws.counters, cur);
+    if ((int)blockIdx.x * kShadeBlock >= count) return;  // the whole block: past the queue
+    int slot = blockIdx.x * kShadeBlock + threadIdx.x, mat = -1, bin = kShadeBins;
+    if (slot < count) {
+        const mcpt_packed_hit hit = ws.hits[slot];
+        const BScene& B = tab[unpack_scene(ws.q[cur][slot].ot.w, ws.tb)];
+        bin = shade_bin(B.sv, hit.did_hit != 0, hit.prim_index, mat);
+    }
+    block_bin_sort<kShadeBlock>(bin, slot, mat, lds);
+    const bool active = slot >= 0;
     Bounce out;
     out.next = false;
     out.shadow = false;
@@ -444,7 +455,8 @@ __global__ __launch_bounds__(kBlock) void k_bb_shade(const BScene* tab, BWs ws, 
         const int pixel = path % P - B.foff;
         rng.g.inc = ((uint64_t)(uint32_t)pixel << 1u) | 1u;
         shade_bounce(B.sv, load3(B.cam.background), B.cam.rr_cap, depth, vec3(it.ot.x, it.ot.y, it.ot.z),
-                     vec3(it.dp.x, it.dp.y, it.dp.z), hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out, time);
+                     vec3(it.dp.x, it.dp.y, it.dp.z), hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out, time,
+                     mat);
         ws.thr[path] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
         ws.smp[path] = make_float4(ps.smp.x, ps.smp.y, ps.smp.z, ps.prev_spec ? 1.0f : 0.0f);
         ws.rng[path] = rng.g.state;
@@ -645,7 +657,8 @@ int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const
             for (int d = 0; d < o->max_depth; d++) {
                 k_bb_bounce<<<1, 64, 0, st>>>(ws.counters, cur);
                 k_bb_extend<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(dbvh, ws, cur, refill);
-                k_bb_shade<<<grid, kBlock, 0, st>>>(dtab, ws, cur, d, (int)P);
+                k_bb_shade<<<(int)((paths + kShadeBlock - 1) / kShadeBlock), kShadeBlock, 0, st>>>(dtab, ws, cur, d,
+                                                                                                  (int)P);
                 k_bb_shadow_total<<<1, 64, 0, st>>>(ws.counters, cur);
                 k_bb_shadow_persistent<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(dtab, ws, cur,
                                                                                                 refill);

@@ -185,9 +185,19 @@ __global__ __launch_bounds__(kBlock) void k_srv_extend(Pool p, int cur, int refi
 // shadeKernel (src/path_tracer.cu:3470-3696) per queue slot; survivors
 // move to the other queue with their state, terminated paths to the done
 // list
-__global__ __launch_bounds__(kBlock) void k_srv_shade(Pool p, int cur) {
-    const int slot = blockIdx.x * kBlock + threadIdx.x;
-    const bool active = slot < p.counters[S_QCOUNT];
+// Each block's slots are shaded in shading-bin order (block_bin_sort).
+constexpr int kShadeBlock = 1024;
+__global__ __launch_bounds__(kShadeBlock) void k_srv_shade(Pool p, int cur) {
+    __shared__ int lds[bin_sort_lds_ints<kShadeBlock>()];
+    const int count = p.counters[S_QCOUNT];
+    int slot = blockIdx.x * kShadeBlock + threadIdx.x, mat = -1, bin = kShadeBins;
+    if (slot < count) {
+        const mcpt_packed_hit hit = p.hits[slot];
+        const SceneView& sv = p.scenes[unpack_scene(p.q[cur][slot].ot.w, p.tb)];
+        bin = shade_bin(sv, hit.did_hit != 0, hit.prim_index, mat);
+    }
+    block_bin_sort<kShadeBlock>(bin, slot, mat, lds);
+    const bool active = slot >= 0;
     Bounce out;
     out.next = false;
     out.shadow = false;
@@ -213,7 +223,8 @@ __global__ __launch_bounds__(kBlock) void k_srv_shade(Pool p, int cur) {
         rng.g.state = p.rng[cur][slot];
         rng.g.inc = ((uint64_t)(uint32_t)m.y << 1u) | 1u;
         shade_bounce(sv, load3(J.cam.background), J.cam.rr_cap, m.w, vec3(it.ot.x, it.ot.y, it.ot.z),
-                     vec3(it.dp.x, it.dp.y, it.dp.z), hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out, time);
+                     vec3(it.dp.x, it.dp.y, it.dp.z), hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out, time,
+                     mat);
         // the reference's depth loop ends after shading bounce max_depth-1
         if (out.next && m.w + 1 >= J.max_depth) out.next = false;
         finished = !out.next;
@@ -595,7 +606,7 @@ void srv_step(mcpt_server_s* s) {
     const int inflight = s->cur_count + total;
     const int grid = blocks_for(inflight);
     k_srv_extend<<<std::min(grid, s->extend_blocks), kBlock, s->lds, st>>>(p, cur, s->refill);
-    k_srv_shade<<<grid, kBlock, 0, st>>>(p, cur);
+    k_srv_shade<<<(inflight + kShadeBlock - 1) / kShadeBlock, kShadeBlock, 0, st>>>(p, cur);
     k_srv_shadow_persistent<<<std::min(grid, s->extend_blocks), kBlock, s->lds, st>>>(p, cur, s->refill);
     k_srv_finish<<<(inflight + kFinishBlock - 1) / kFinishBlock, kFinishBlock, 0, st>>>(p);
     SRV_HIP(hipGetLastError());
