@@ -138,41 +138,126 @@ MCPT_DEV f32x4 output_forward(const __bf16* W4, const __bf16* xin) {
 }
 
 // ------------------------------------------------------------- kernels
+// Inference keeps each sample's activations in registers from the encoding
+// to the output. A wave owns 16 samples and computes all 64 rows of every
+// layer for them (4 row blocks x 2 K chunks = 8 MFMAs per layer). Its D
+// registers (lane r + 16g: rows 16i + 4g + e of sample r) become the next
+// layer's B operand (lane r + 16g: inputs 8g..8g+7 of each 32-input chunk)
+// without moving: B element e of chunk kc is taken to be row
+// 16(2kc + e/4) + 4g + e%4, and the next layer's weight columns are stored
+// in that order (infer_inh). Layer 0's inputs are computed in registers the
+// same way, each lane encoding 16 of its sample's 64 inputs (infer_in0).
+// So waves exchange nothing, need no barrier after the weights are staged,
+// and read only weights (16 bytes per MFMA) from LDS.
+constexpr int kInferBlock = 512;
+
+// the input of layer 0 that logical input k stands for: lane group g of
+// chunk kc supplies inputs j = 8kc + e of its group (position axis g's
+// octaves, then direction / normal component g's; group 3: the normal's
+// second component, then the raw features)
+MCPT_HDI int infer_in0(int k) {
+    const int kc = k >> 5, g = (k >> 3) & 3, j = 8 * kc + (k & 7);
+    if (g < 3) return j < 12 ? 12 * g + j : 36 + 4 * g + (j - 12);
+    return j < 4 ? 48 + j : 52 + (j - 4);
+}
+// the row of the previous layer that logical input k of layers 1..4 stands for
+MCPT_HDI int infer_inh(int k) {
+    const int kc = k >> 5, g = (k >> 3) & 3, e = k & 7;
+    return 16 * (2 * kc + (e >> 2)) + 4 * g + (e & 3);
+}
+
+// the weight image with each layer's columns in logical input order
+MCPT_DEV void load_weights_infer(const __bf16* wimg, __bf16* w) {
+    for (int i = threadIdx.x; i < kImgRows * kIn; i += blockDim.x) {
+        const int row = i / kIn, k = i % kIn;
+        w[row * kRow + k] = wimg[row * kRow + (row < kW ? infer_in0(k) : infer_inh(k))];
+    }
+}
+
+// sin and cos of 2^k pi x for k < n: the hardware sine and cosine of pi x
+// (their argument is in turns), then the double-angle identities
+MCPT_DEV void octaves(float x, int n, float* out) {
+    float s = __builtin_amdgcn_sinf(0.5f * x), c = __builtin_amdgcn_cosf(0.5f * x);
+    for (int k = 0; k < n; k++) {
+        out[2 * k] = s;
+        out[2 * k + 1] = c;
+        const float s2 = 2.0f * s * c, c2 = (c - s) * (c + s);
+        s = s2;
+        c = c2;
+    }
+}
+
+// lane group g's 16 inputs of a sample (features f) as two B chunks
+MCPT_DEV void encode_lane(const float* f, int g, bf16x8& b0, bf16x8& b1) {
+    float v[16];
+    if (g < 3) {
+        octaves(f[g], 6, v);
+        octaves(f[g < 2 ? 3 + g : 5], 2, v + 12);
+    } else {
+        octaves(f[6], 2, v);
+        v[4] = f[7];
+        v[5] = f[8];
+        v[6] = f[9];
+        v[7] = f[10];
+        v[8] = 1.0f;
+        for (int i = 0; i < 7; i++) v[9 + i] = f[i];
+    }
+    for (int e = 0; e < 8; e++) {
+        b0[e] = (__bf16)v[e];
+        b1[e] = (__bf16)v[8 + e];
+    }
+}
+
 // Inference over queries. yout != nullptr: write y (n*3); otherwise add
 // thr * max(y, 0) to the path's sample (the cached render).
-__global__ __launch_bounds__(kBlock) void k_mlp_infer(const __bf16* wimg, const Query* q, const int* count_ptr,
-                                                     int count, float* yout, float4* smp) {
+__global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const __bf16* wimg, const Query* q, const int* count_ptr,
+                                                          int count, float* yout, float4* smp) {
     __shared__ __attribute__((aligned(16))) __bf16 w[kImg];
-    __shared__ __attribute__((aligned(16))) __bf16 x[2][kTile * kRow];
     const int n = count_ptr ? *count_ptr : count;
-    load_weights(wimg, w);
-    const int ntiles = (n + kTile - 1) / kTile;
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int s = tile * kTile + (threadIdx.x >> 2);
-        __syncthreads();
-        encode_tile(q[min(s, n - 1)].f, s < n, x[0]);
-        __syncthreads();
-        int c = 0;
+    load_weights_infer(wimg, w);
+    __syncthreads();
+    const int lane = lane_id(), r = lane & 15, g = lane >> 4;
+    const int waves = (int)(gridDim.x * blockDim.x) >> 6;
+    const int chunks = (n + 15) >> 4;
+    for (int chunk = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); chunk < chunks; chunk += waves) {
+        const int s = 16 * chunk + r;
+        float f[MCPT_MLP_FEATURES];
+        const Query& qs = q[min(s, n - 1)];
+        for (int i = 0; i < MCPT_MLP_FEATURES; i++) f[i] = s < n ? qs.f[i] : 0.0f;
+        bf16x8 b0, b1;
+        encode_lane(f, g, b0, b1);
         for (int l = 0; l < MCPT_MLP_HIDDEN; l++) {
-            hidden_forward(w + img_row(l, 0) * kRow, x[c], x[1 - c]);
-            __syncthreads();
-            c = 1 - c;
+            const __bf16* Wl = w + img_row(l, 0) * kRow;
+            f32x4 d[4];
+            for (int i = 0; i < 4; i++) {
+                const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(Wl + (16 * i + r) * kRow + 8 * g);
+                const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(Wl + (16 * i + r) * kRow + 32 + 8 * g);
+                d[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                d[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, d[i], 0, 0, 0);
+            }
+            for (int e = 0; e < 4; e++) {
+                b0[e] = (__bf16)fmaxf(d[0][e], 0.0f);
+                b0[4 + e] = (__bf16)fmaxf(d[1][e], 0.0f);
+                b1[e] = (__bf16)fmaxf(d[2][e], 0.0f);
+                b1[4 + e] = (__bf16)fmaxf(d[3][e], 0.0f);
+            }
         }
-        const f32x4 y = output_forward(w + img_row(MCPT_MLP_HIDDEN, 0) * kRow, x[c]);
-        const int lane = lane_id();
-        const int so = tile * kTile + 16 * (threadIdx.x >> 6) + (lane & 15);
-        if ((lane >> 4) == 0 && so < n) {
+        const __bf16* W4 = w + img_row(MCPT_MLP_HIDDEN, 0) * kRow;
+        const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(W4 + r * kRow + 8 * g);
+        const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(W4 + r * kRow + 32 + 8 * g);
+        f32x4 y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, y, 0, 0, 0);
+        if (g == 0 && s < n) {  // lanes of group 0 hold outputs 0..3 of sample r
             if (yout) {
-                yout[3 * so] = y[0];
-                yout[3 * so + 1] = y[1];
-                yout[3 * so + 2] = y[2];
+                yout[3 * s] = y[0];
+                yout[3 * s + 1] = y[1];
+                yout[3 * s + 2] = y[2];
             } else {
-                const Query& qq = q[so];
-                float4 v = smp[qq.path];
-                v.x = v.x + qq.thr[0] * fmaxf(y[0], 0.0f);
-                v.y = v.y + qq.thr[1] * fmaxf(y[1], 0.0f);
-                v.z = v.z + qq.thr[2] * fmaxf(y[2], 0.0f);
-                smp[qq.path] = v;
+                float4 v = smp[qs.path];
+                v.x = v.x + qs.thr[0] * fmaxf(y[0], 0.0f);
+                v.y = v.y + qs.thr[1] * fmaxf(y[1], 0.0f);
+                v.z = v.z + qs.thr[2] * fmaxf(y[2], 0.0f);
+                smp[qs.path] = v;
             }
         }
     }
@@ -305,19 +390,32 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(const __bf16* wimg, const f
     if (threadIdx.x == 0) partial_loss[blockIdx.x] = (lsum[0] + lsum[1]) + (lsum[2] + lsum[3]);
 }
 
-// grad[p] = sum over blocks in block order; loss likewise
+// grad[p] = sum over the gradient blocks: 8 threads per parameter each add
+// a contiguous eighth of the blocks in block order, then the eight sums are
+// added in order, so the result does not depend on scheduling; loss likewise
+// (block 0, one thread)
+constexpr int kReduceSplit = 8;
+constexpr int kReduceParams = kBlock / kReduceSplit;  // parameters per block
 __global__ __launch_bounds__(kBlock) void k_mlp_reduce(const float* partial, const float* partial_loss, int nblk,
                                                       float* grad, float* loss_out, float loss_scale) {
-    const int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p < kParams) {
-        float s = 0.0f;
-        for (int b = 0; b < nblk; b++) s += partial[(size_t)b * kParams + p];
-        grad[p] = s;
+    __shared__ float part[kReduceSplit][kReduceParams];
+    const int j = threadIdx.x % kReduceParams, q = threadIdx.x / kReduceParams;
+    const int p = blockIdx.x * kReduceParams + j;
+    const int b0 = (int)((int64_t)nblk * q / kReduceSplit), b1 = (int)((int64_t)nblk * (q + 1) / kReduceSplit);
+    float s = 0.0f;
+    if (p < kParams)
+        for (int b = b0; b < b1; b++) s += partial[(size_t)b * kParams + p];
+    part[q][j] = s;
+    __syncthreads();
+    if (q == 0 && p < kParams) {
+        float t = part[0][j];
+        for (int k = 1; k < kReduceSplit; k++) t += part[k][j];
+        grad[p] = t;
     }
-    if (p == 0) {
-        float s = 0.0f;
-        for (int b = 0; b < nblk; b++) s += partial_loss[b];
-        loss_out[0] = s * loss_scale;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        float l = 0.0f;
+        for (int b = 0; b < nblk; b++) l += partial_loss[b];
+        loss_out[0] = l * loss_scale;
     }
 }
 
@@ -356,6 +454,15 @@ __global__ __launch_bounds__(kBlock) void k_pack(const float* master, __bf16* wi
 }
 
 int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
+
+// inference grid: every block stages the weights once, so no more blocks
+// than fit on the device at once (occupancy), nor than 16-sample chunks
+int infer_blocks(const Ctx* ctx, int64_t n) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mlp_infer, kInferBlock, 0) != hipSuccess) per_cu = 1;
+    const int64_t need = (n + 16 * (kInferBlock / 64) - 1) / (16 * (kInferBlock / 64));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)std::max(1, per_cu) * ctx->cu_count));
+}
 
 // ----------------------------------------------------- cached wavefront
 // Counters, one 128-byte line each (see render.hip): C_NS0/C_NS1 are u64
@@ -722,7 +829,7 @@ int mlp_gradient_dev(mcpt_mlp_s* mlp, const float* feat, const float* targ, int 
     const int nblk = std::max(1, std::min(ntiles, kMaxGradBlocks));
     k_mlp_grad<<<nblk, kBlock, 0, st>>>(mlp->wimg.ptr, feat, targ, first, n, n, mlp->partial.ptr,
                                         mlp->partial_loss.ptr);
-    k_mlp_reduce<<<blocks_for(kParams), kBlock, 0, st>>>(mlp->partial.ptr, mlp->partial_loss.ptr, nblk,
+    k_mlp_reduce<<<(kParams + kReduceParams - 1) / kReduceParams, kBlock, 0, st>>>(mlp->partial.ptr, mlp->partial_loss.ptr, nblk,
                                                          mlp->grad.ptr, mlp->loss.ptr, 1.0f / float(3 * n));
     MCPT_HIP(hipGetLastError());
     return MCPT_OK;
@@ -882,8 +989,7 @@ int mcpt_mlp_forward(mcpt_mlp mlp, const float* features, int32_t n, float* rgb)
     Query* dq = reinterpret_cast<Query*>(mlp->io.ptr);
     float* dy = mlp->io.ptr + (size_t)n * 16;
     MCPT_HIP(hipMemcpyAsync(dq, q.data(), (size_t)n * sizeof(Query), hipMemcpyHostToDevice, st));
-    const int ntiles = (n + kTile - 1) / kTile;
-    k_mlp_infer<<<std::min(ntiles, 4096), kBlock, 0, st>>>(mlp->wimg.ptr, dq, nullptr, n, dy, nullptr);
+    k_mlp_infer<<<infer_blocks(mlp->ctx, n), kInferBlock, 0, st>>>(mlp->wimg.ptr, dq, nullptr, n, dy, nullptr);
     MCPT_HIP(hipGetLastError());
     MCPT_HIP(hipMemcpyAsync(rgb, dy, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost, st));
     MCPT_HIP(hipStreamSynchronize(st));
@@ -1005,7 +1111,7 @@ int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* cam, const mcpt_rend
             cur = 1 - cur;
         }
         MCPT_HIP(hipEventRecord(ev[1], st));
-        k_mlp_infer<<<std::max(1, std::min((npaths + kTile - 1) / kTile, 2 * ctx->cu_count)), kBlock, 0, st>>>(
+        k_mlp_infer<<<infer_blocks(ctx, npaths), kInferBlock, 0, st>>>(
             decay > 0.0f ? mlp->wimg_ema.ptr : mlp->wimg.ptr, ws.queries, ws.counters + C_QCOUNT, 0, nullptr,
             ws.smp);
         MCPT_HIP(hipEventRecord(ev[2], st));
