@@ -598,6 +598,34 @@ def cloud_bench(mcpt, args):
     return out
 
 
+def mlp_bench(mcpt, ctx, args):
+    """The radiance-cache network alone: inference over 2,097,152 queries
+    (the paths of a 1920x1080 frame) and training steps of 65,536 samples
+    (the workload's training paths per frame), on random features. Wall
+    times include the host copies of mcpt_mlp_forward / mcpt_mlp_train;
+    kernel durations come from rocprofv3 (profiles/mlp_*.md)."""
+    import numpy as np
+    rng = np.random.default_rng(args.seed)
+    n, nt = 2097152, 65536
+    f = rng.uniform(0.0, 1.0, (n, 11)).astype(np.float32)
+    t = rng.uniform(0.0, 1.0, (nt, 3)).astype(np.float32)
+    net = ctx.mlp(seed=3)
+    fwd, trn = [], []
+    for _ in range(args.mlp_reps):
+        t0 = time.perf_counter()
+        net.forward(f)
+        fwd.append(time.perf_counter() - t0)
+    for _ in range(args.mlp_reps):
+        t0 = time.perf_counter()
+        loss = net.train(f[:nt], t, batch=nt, lr=1e-3)
+        trn.append(time.perf_counter() - t0)
+    net.close()
+    return {"workload": f"MLP 64-64x4-3 BF16, inference over {n} queries, training steps of {nt} samples",
+            "forward_wall_ms_median": 1e3 * sorted(fwd)[len(fwd) // 2],
+            "train_step_wall_ms_median": 1e3 * sorted(trn)[len(trn) // 2], "reps": args.mlp_reps, "loss": loss,
+            "flop_per_query": 33152, "flop_per_training_sample": 99456}
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device copy of `nbytes`, for scaling rocprofv3's
     FETCH_SIZE / WRITE_SIZE counters: a copy kernel of known size reads and
@@ -635,7 +663,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the traversal-counting pass")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", default="all", choices=["all", "extend", "serving", "cache", "batch", "cloud"])
+    ap.add_argument("--workload", default="all", choices=["all", "extend", "serving", "cache", "batch", "cloud", "mlp"])
+    ap.add_argument("--mlp-reps", type=int, default=5)
     ap.add_argument("--cloud-size", type=int, default=512)
     ap.add_argument("--cloud-spp", type=int, default=64)
     ap.add_argument("--batch-scenes", type=int, default=64)
@@ -697,6 +726,14 @@ def main():
             bb["batched"]["scenes_per_s"], "unit": "scenes/s", "n_gpus": 1, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scenes)",
             "config": {"workload": bb["workload"]}, "batch": bb}), flush=True)
+        return
+    if args.workload == "mlp":
+        with mcpt.Context() as ctx:
+            mb = mlp_bench(mcpt, ctx, args)
+        print(json.dumps({
+            "metric": "MLP forward and training-step wall time", "value": mb["forward_wall_ms_median"], "unit": "ms",
+            "n_gpus": 1, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16+f32",
+            "data": "synthetic (random features)", "config": {"workload": mb["workload"]}, "mlp": mb}), flush=True)
         return
     if args.workload == "cache":
         with mcpt.Context() as ctx:
@@ -810,24 +847,30 @@ def main():
             "launches": launches, "avg_launch_ms": avg_ms, "alg_bytes_per_launch": alg_bytes / launches,
             "steps_per_ray": counted["extend_steps"] / n, "leaves_per_ray": counted["extend_leaves"] / n,
         }
-        # HBM-side traffic per launch, measured by rocprofv3 --pmc FETCH_SIZE /
-        # WRITE_SIZE (copy-kernel scaled) for exactly this configuration
-        prof = os.path.join(ROOT, "profiles", "extend_soup1M_64spp.json")
-        if os.path.exists(prof):
+        # Measured traffic per launch: rocprofv3 --pmc read and write request
+        # counts at the L2's fabric interface (TCC_EA0_RDREQ_{32B,64B,128B},
+        # TCC_EA0_WRREQ{,_64B}) per closest-hit ray, profiled on this scene,
+        # resolution, depth and mode, times this run's rays per launch. They
+        # count every L2 miss, whether the Infinity Cache or HBM serves it.
+        prof = os.path.join(ROOT, "profiles", "traffic_soup1M_1024x1024_d16.json")
+        if os.path.exists(prof) and mode == mcpt.MODE_PERSISTENT:
             with open(prof) as f:
                 p = json.load(f)
             wl = p["workload"]
-            if (wl["width"], wl["height"], wl["spp"], wl["max_depth"], wl["triangles"], wl["mode"], wl["launches"]) == \
-                    (W, H, K, args.depth, len(hot), args.mode, launches) and mode == mcpt.MODE_PERSISTENT:
-                tb = p["k_extend_persistent"]["traffic_bytes_per_launch"]
+            if (wl["width"], wl["height"], wl["max_depth"], wl["triangles"], wl["mode"]) == \
+                    (W, H, args.depth, len(hot), args.mode):
+                k = p["kernels"]["k_extend_persistent"]
+                tb = (k["read_bytes_per_ray"] + k["write_bytes_per_ray"]) * n / launches
                 result["roofline"]["traffic"] = tb
-                result["roofline"]["traffic_source"] = "profiles/extend_soup1M_64spp.json (rocprofv3 --pmc)"
+                result["roofline"]["traffic_source"] = (
+                    f"profiles/traffic_soup1M_1024x1024_d16.json: {k['read_bytes_per_ray']:.1f} B read + "
+                    f"{k['write_bytes_per_ray']:.1f} B written per ray at the L2-fabric interface "
+                    f"(rocprofv3 --pmc at {wl['profiled_spp']} spp) x this run's rays per launch")
                 result["roofline"]["traffic_GBps"] = tb / (avg_ms / 1e3) / 1e9
                 result["roofline"]["traffic_frac"] = tb / (avg_ms / 1e3) / HBM_PEAK
                 result["roofline"]["note"] = (
-                    "achieved counts every node and leaf a ray reads as HBM bytes (the algorithmic unit); "
-                    "BVH reuse is served by L2 and the Infinity Cache, so it can exceed the HBM peak. "
-                    "traffic_GBps is the measured L2-to-fabric traffic of the same launch")
+                    "achieved counts every node and leaf a ray reads (the algorithmic unit); BVH reuse is served by "
+                    "L2 and the Infinity Cache, so the measured L2-to-fabric traffic is lower")
         ref_bytes = 48 * n + 80 * ref_counted["extend_steps"] + 44 * ref_counted["extend_leaves"]
         result["roofline"]["reference_order"] = {
             "steps_per_ray": ref_counted["extend_steps"] / n, "leaves_per_ray": ref_counted["extend_leaves"] / n,

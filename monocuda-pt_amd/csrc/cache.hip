@@ -651,7 +651,7 @@ __global__ __launch_bounds__(kBlock) void k_cc_shade(SceneView sv, CacheWs ws, i
             const mcpt_material& m = sv.mats[prim_material(sv, hit.prim_index)];
             diffuse = m.type == MCPT_MAT_LAMBERTIAN;
             if (diffuse) {
-                rebuild_hit(sv, hit.prim_index, ro, rd, hit.t, time, h);
+                rebuild_hit(sv, hit.prim_index, ro, rd, hit.t, time, h, m.texture > 0);
                 albedo = sample_albedo(sv, m, h.u, h.v);
             }
         }

@@ -238,6 +238,7 @@ int mcpt_scene_create_ex(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_c
     CHECK_ARG(n_tris == 0 || (hot && cold), "triangle arrays are NULL");
     CHECK_ARG(n_spheres == 0 || spheres, "sphere array is NULL");
     CHECK_ARG(n_materials >= 1 && materials, "a scene needs at least one material");
+    CHECK_ARG(n_materials <= (int32_t)TRI_MATERIAL_MASK, "too many materials");
     for (int i = 0; i < n_tris; i++)
         if (cold[i].material < 0 || cold[i].material >= n_materials)
             return set_error(MCPT_ERR_INVALID, "triangle %d has material %d outside [0, %d)", i, cold[i].material,
@@ -290,6 +291,17 @@ int mcpt_scene_create_ex(mcpt_ctx ctx, const mcpt_tri_hot* hot, const mcpt_tri_c
     }
     int st = upload(c, s->hot, reinterpret_cast<const float*>(hot), (size_t)n_tris * 9);
     if (st >= 0) st = upload(c, s->cold, cold, (size_t)n_tris);
+    if (st >= 0 && n_tris) {
+        std::vector<uint32_t> info(n_tris);
+        for (int i = 0; i < n_tris; i++) {
+            const bool vn = !(cold[i].n0[0] * cold[i].n0[0] + cold[i].n0[1] * cold[i].n0[1] +
+                                  cold[i].n0[2] * cold[i].n0[2] == 0.0f);
+            info[i] = (uint32_t)cold[i].material | (vn ? TRI_VERTEX_NORMALS : 0u);
+        }
+        st = upload(c, s->tri_info, info.data(), info.size());
+        if (st >= 0 && hipStreamSynchronize(c->stream) != hipSuccess)  // `info` leaves scope
+            st = set_error(MCPT_ERR_HIP, "scene upload failed");
+    }
     if (st >= 0) st = upload(c, s->mats, materials, (size_t)n_materials);
     if (st >= 0 && n_spheres) st = upload(c, s->spheres, spheres, (size_t)n_spheres);
     if (st >= 0 && !s->host_lights.empty()) st = upload(c, s->lights, s->host_lights.data(), s->host_lights.size());

@@ -320,10 +320,17 @@ struct GridView {
     float max_density;
 };
 
+constexpr uint32_t TRI_VERTEX_NORMALS = 1u << 31;
+constexpr uint32_t TRI_MATERIAL_MASK = (1u << 30) - 1;
+
 struct SceneView {
     BvhView bvh;
     const float* hot;           // 9 floats per triangle, original order
     const mcpt_tri_cold* cold;  // original order
+    // per triangle, original order: the material index (bits 0-29) and
+    // TRI_VERTEX_NORMALS when the record has vertex normals (n0 != 0); 4 bytes
+    // where the 64-byte cold record would be gathered for one field
+    const uint32_t* tri_info;
     const mcpt_material* mats;
     const mcpt_light* lights;
     int n_lights;

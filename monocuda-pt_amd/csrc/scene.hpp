@@ -21,6 +21,7 @@ struct mcpt_scene_s {
     // inputs (original triangle order)
     mcpt::DevBuf<float> hot;  // 9 floats per triangle
     mcpt::DevBuf<mcpt_tri_cold> cold;
+    mcpt::DevBuf<uint32_t> tri_info;  // material index | TRI_VERTEX_NORMALS
     mcpt::DevBuf<mcpt_material> mats;
     mcpt::DevBuf<mcpt_sphere> spheres;
     mcpt::DevBuf<mcpt_light> lights;
@@ -67,6 +68,7 @@ struct mcpt_scene_s {
         v.bvh.tie = tie.ptr;
         v.hot = hot.ptr;
         v.cold = cold.ptr;
+        v.tri_info = tri_info.ptr;
         v.mats = mats.ptr;
         v.lights = lights.ptr;
         v.n_lights = (int)host_lights.size();
@@ -93,6 +95,7 @@ struct mcpt_scene_s {
     void release() {
         hot.release(ctx);
         cold.release(ctx);
+        tri_info.release(ctx);
         mats.release(ctx);
         spheres.release(ctx);
         lights.release(ctx);

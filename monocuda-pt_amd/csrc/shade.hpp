@@ -171,8 +171,12 @@ struct HitInfo {
 // the fields testPrimitiveHit fills (:1490-1533), rebuilt from the 16-byte
 // hit record as shadeKernel does (:3508-3535). prim >= 0 is a triangle,
 // prim < 0 the sphere ~prim, whose centre is taken at the ray's time.
+// need_uv: the caller reads h.u, h.v (a textured material). A triangle
+// without vertex normals whose uv is not needed is shaded from its vertices
+// alone: its shading normal is the face normal, and the cold record and the
+// barycentric weights are skipped (h.u = h.v = 0).
 MCPT_DEV void rebuild_hit(const SceneView& sv, int prim, const vec3& o, const vec3& d, float t, float time,
-                          HitInfo& h) {
+                          HitInfo& h, bool need_uv = true) {
     if (prim < 0) {
         const mcpt_sphere& sp = sv.spheres[~prim];
         const vec3 c = sphere_center(load3(sp.center), load3(sp.center1), sp.moving != 0, time);
@@ -185,10 +189,15 @@ MCPT_DEV void rebuild_hit(const SceneView& sv, int prim, const vec3& o, const ve
     }
     vec3 v0, v1, v2;
     load_tri(sv.hot, prim, v0, v1, v2);
-    const mcpt_tri_cold& c = sv.cold[prim];
     h.p = o + d * t;
     vec3 n = (v1 - v0).cross(v2 - v0).normalize();
     h.front = d.dot(n) < 0.0f;
+    if (!need_uv && !(sv.tri_info[prim] & TRI_VERTEX_NORMALS)) {
+        h.normal = h.front ? n : -n;
+        h.u = h.v = 0.0f;
+        return;
+    }
+    const mcpt_tri_cold& c = sv.cold[prim];
     // computeTriangleShadingNormal (:698-713)
     vec3 sn = n;
     float a2 = (v1 - v0).cross(v2 - v0).length();
@@ -215,7 +224,7 @@ MCPT_DEV void rebuild_hit(const SceneView& sv, int prim, const vec3& o, const ve
 
 // the material of a primitive in the hit encoding above
 MCPT_DEV int prim_material(const SceneView& sv, int prim) {
-    return prim >= 0 ? sv.cold[prim].material : sv.spheres[~prim].material;
+    return prim >= 0 ? (int)(sv.tri_info[prim] & TRI_MATERIAL_MASK) : sv.spheres[~prim].material;
 }
 
 MCPT_DEV float tri_area(const float* hot, int prim) {  // :837-839
@@ -415,7 +424,7 @@ MCPT_DEV void shade_bounce(const SceneView& sv, const vec3& bg, float rr_cap, in
     }
     const mcpt_material m = sv.mats[mat >= 0 ? mat : prim_material(sv, prim)];
     HitInfo h;
-    rebuild_hit(sv, prim, ro, rd, t, time, h);
+    rebuild_hit(sv, prim, ro, rd, t, time, h, m.texture > 0);
     if (m.type == MCPT_MAT_EMISSIVE) {
         float ew = 1.0f;
         if (!(depth == 0 || ps.prev_spec) && prim >= 0) {  // spheres are not on the light list
