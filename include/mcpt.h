@@ -609,6 +609,15 @@ int mcpt_load_png(const char* path, int32_t* width, int32_t* height, uint8_t* rg
  * get the size. The texture loaders read .tga files the same way. */
 int mcpt_load_tga(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8);
 
+/* A baseline JPEG texture as RGBA8, top row first (the reference reads
+ * JPEG textures through stb_image, src/image_io.cpp:1-40): sequential
+ * Huffman-coded 8-bit files, grey or YCbCr with any sampling factors,
+ * restart intervals; progressive and arithmetic-coded files are refused.
+ * Chroma at half resolution is upsampled with the triangle filter.
+ * Two-call pattern: rgba8 NULL to get the size. The texture loaders read
+ * JPEG files (by their signature) the same way. */
+int mcpt_load_jpeg(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8);
+
 /* Environment map files for mcpt_env_desc (RGBA FP32, top row first, alpha
  * 1). Two-call pattern: rgba NULL to get the size.
  * mcpt_load_pfm: loadPfmAsRgba32f (src/path_tracer.cu:335-368), "PF"

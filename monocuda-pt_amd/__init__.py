@@ -295,6 +295,7 @@ _SIGS = {
     "mcpt_load_ppm": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_png": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_tga": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
+    "mcpt_load_jpeg": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_pfm": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_hdr": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_exr": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
@@ -463,6 +464,16 @@ def load_tga(path):
     _check(_lib.mcpt_load_tga(p, C.byref(w), C.byref(h), None), "mcpt_load_tga")
     out = np.zeros((h.value, w.value, 4), np.uint8)
     _check(_lib.mcpt_load_tga(p, C.byref(w), C.byref(h), _ptr(out)), "mcpt_load_tga")
+    return out
+
+
+def load_jpeg(path):
+    """mcpt_load_jpeg: a baseline JPEG file as an (H, W, 4) RGBA8 array."""
+    w, h = C.c_int32(), C.c_int32()
+    p = os.fsencode(path)
+    _check(_lib.mcpt_load_jpeg(p, C.byref(w), C.byref(h), None), "mcpt_load_jpeg")
+    out = np.zeros((h.value, w.value, 4), np.uint8)
+    _check(_lib.mcpt_load_jpeg(p, C.byref(w), C.byref(h), _ptr(out)), "mcpt_load_jpeg")
     return out
 
 

@@ -462,6 +462,19 @@ int mcpt_load_tga(const char* path, int32_t* width, int32_t* height, uint8_t* rg
     });
 }
 
+int mcpt_load_jpeg(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8) {
+    return guarded("mcpt_load_jpeg", [&]() -> int {
+        CHECK_ARG(path && width && height, "bad arguments");
+        int w = 0, h = 0;
+        std::vector<uint8_t> px;
+        MCPT_TRY(read_jpeg(path, w, h, rgba8 ? &px : nullptr));
+        *width = w;
+        *height = h;
+        if (rgba8) std::memcpy(rgba8, px.data(), px.size());
+        return MCPT_OK;
+    });
+}
+
 static int load_ppm_file(const char* path, int32_t* width, int32_t* height, uint8_t* rgba8) {
     CHECK_ARG(path && width && height, "bad arguments");
     int w = 0, h = 0;

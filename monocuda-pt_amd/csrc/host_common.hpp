@@ -83,7 +83,8 @@ int read_hdr(const char* path, int& w, int& h, std::vector<float>* rgba);
 int read_exr(const char* path, int& w, int& h, std::vector<float>* rgba);
 int read_png(const char* path, int& w, int& h, std::vector<uint8_t>* rgba);
 int read_tga(const char* path, int& w, int& h, std::vector<uint8_t>* rgba);
-// an 8-bit texture: TGA by its extension, PNG by its signature, else binary PPM
+int read_jpeg(const char* path, int& w, int& h, std::vector<uint8_t>* rgba);
+// an 8-bit texture: TGA by its extension, PNG or JPEG by its signature, else binary PPM
 int read_image8(const char* path, int& w, int& h, std::vector<uint8_t>* rgba);
 // pbrt-v3 import (pbrtload.cpp)
 int load_pbrt(const char* path, int64_t max_prims, mcpt_tri_hot* hot, mcpt_tri_cold* cold, mcpt_sphere* spheres,

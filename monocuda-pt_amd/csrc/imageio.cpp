@@ -747,6 +747,7 @@ int read_image8(const char* path, int& w, int& h, std::vector<uint8_t>* rgba) {
         if (!file.f) return set_error(MCPT_ERR_INVALID, "cannot open '%s'", path);
         if (std::fread(sig, 1, 8, file.f) != 8) return read_ppm(path, w, h, rgba);
     }
+    if (sig[0] == 0xFF && sig[1] == 0xD8 && sig[2] == 0xFF) return read_jpeg(path, w, h, rgba);
     return std::memcmp(sig, kPngSig, 8) == 0 ? read_png(path, w, h, rgba) : read_ppm(path, w, h, rgba);
 }
 }  // namespace mcpt
