@@ -319,7 +319,7 @@ int mcpt_render(mcpt_scene scene, const mcpt_camera* camera, const mcpt_render_o
  * opts->flags may not ask for kernel timing. Calls on one device are
  * serialised by the library (they share the workspace); the call returns
  * when the device is done. */
-#define MCPT_NOALLOC_MAX_PATHS 399448
+#define MCPT_NOALLOC_MAX_PATHS 299584
 int mcpt_render_noalloc(mcpt_scene scene, const mcpt_camera* camera, const mcpt_render_opts* opts, mcpt_film film);
 
 /* resolve_kernel (src/path_tracer.cu:3092-3100): accum / spp * exposure,

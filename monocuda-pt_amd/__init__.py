@@ -39,7 +39,7 @@ MODE_MEGAKERNEL, MODE_WAVEFRONT, MODE_PERSISTENT = 0, 1, 2
 TONEMAP_NONE, TONEMAP_REINHARD, TONEMAP_ACES = 0, 1, 2
 RENDER_COUNT_TRAVERSAL, RENDER_TIME_KERNELS, RENDER_REFERENCE_ORDER, RENDER_PERSISTENT_SHADOW = 1, 2, 4, 8
 RENDER_SHADOW_PER_RAY = 16
-NOALLOC_MAX_PATHS = 399448  # mcpt_render_noalloc: paths per pass in the fixed workspace
+NOALLOC_MAX_PATHS = 299584  # mcpt_render_noalloc: paths per pass in the fixed workspace
 TEX_CHECKER, TEX_IMAGE = 1, 2
 GEN_CORNELL, GEN_SOUP, GEN_ICOSPHERE, GEN_MEDIUM = 1, 2, 3, 5
 MLP_FEATURES, MLP_INPUTS, MLP_WIDTH, MLP_HIDDEN, MLP_OUTPUTS = 11, 64, 64, 4, 3

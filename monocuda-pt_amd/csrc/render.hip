@@ -47,15 +47,28 @@ enum {
     C_NCOUNTERS = 256
 };
 
+// Path state travels with its ray: thr/smp/rng[q] hold the state of the
+// path in slot i of ray queue q, and shade writes a surviving path's state
+// beside its next ray, so every state access of a wave stays inside the
+// slots the wave (or its block) works on. A path that ends leaves its sample
+// in done[path]; a shadow ray adds its light to the sample where the path
+// is now (next-queue slot, or done[path]; ShadowItem.dp.w).
 struct Workspace {
     RayItem* q[2];
     mcpt_packed_hit* hits;
     ShadowItem* sq;
-    float4* thr;  // throughput rgb, previous bsdf pdf
-    float4* smp;  // sample rgb, previous-bounce-specular flag
-    uint64_t* rng;
-    int* counters;  // int slots; the *_TOTAL and count slots at 4.. are u64
+    float4* thr[2];  // throughput rgb, previous bsdf pdf
+    float4* smp[2];  // sample rgb, previous-bounce-specular flag
+    uint64_t* rng[2];
+    float4* done;    // by path: the finished sample
+    int* counters;   // int slots; the *_TOTAL and count slots at 4.. are u64
 };
+
+// where a shadow ray's light goes: the path's next-queue slot (>= 0) or,
+// for a path that ended, ~path into done[]
+__device__ __forceinline__ float4* shadow_target(const Workspace& ws, int cur, int target) {
+    return target >= 0 ? ws.smp[1 - cur] + target : ws.done + ~target;
+}
 
 __device__ __forceinline__ uint64_t* u64slot(int* c, int i) { return reinterpret_cast<uint64_t*>(c + i); }
 __device__ __forceinline__ unsigned long long* ns_slot(int* c, int q) {
@@ -108,9 +121,9 @@ __global__ __launch_bounds__(kBlock) void k_camera(mcpt_camera cam, int sample0,
     it.ot = make_float4(o.x, o.y, o.z, time);
     it.dp = make_float4(d.x, d.y, d.z, __int_as_float(p));
     ws.q[cur][p] = it;
-    ws.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    ws.smp[p] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-    ws.rng[p] = rng.g.state;
+    ws.thr[cur][p] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    ws.smp[cur][p] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    ws.rng[cur][p] = rng.g.state;
 }
 
 // ---------------------------------------------------------------- extend
@@ -235,8 +248,8 @@ __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Work
 // occupancy limit, so the thin late bounces launch no empty waves), each
 // chunk ordered by shading bin before it is shaded.
 constexpr int kShadeBlock = 1024;
-__global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView sv, Workspace ws, int cur, int depth, vec3 bg,
-                                                       float rr_cap, int npix) {
+__global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView sv, Workspace ws, int cur, int depth, int max_depth,
+                                                       vec3 bg, float rr_cap, int npix) {
     __shared__ int lds[bin_sort_lds_ints<kShadeBlock>()];
     const int count = queue_len(ws.counters, cur);
     for (int base = blockIdx.x * kShadeBlock; base < count; base += gridDim.x * kShadeBlock) {
@@ -253,40 +266,47 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView sv, Workspace w
         out.shadow = false;
         int path = 0;
         float time = 0.f;
+        PathState ps;
+        Rng rng;
         if (active) {
             const RayItem it = ws.q[cur][slot];
             hit = ws.hits[slot];
             path = __float_as_int(it.dp.w);
             time = it.ot.w;
             const vec3 ro(it.ot.x, it.ot.y, it.ot.z), rd(it.dp.x, it.dp.y, it.dp.z);
-            const float4 thr = ws.thr[path], smp = ws.smp[path];
-            PathState ps;
+            const float4 thr = ws.thr[cur][slot], smp = ws.smp[cur][slot];
             ps.thr = vec3(thr.x, thr.y, thr.z);
             ps.prev_pdf = thr.w;
             ps.smp = vec3(smp.x, smp.y, smp.z);
             ps.prev_spec = smp.w != 0.0f;
-            Rng rng;
-            rng.g.state = ws.rng[path];
+            rng.g.state = ws.rng[cur][slot];
             rng.g.inc = ((uint64_t)(uint32_t)(path % npix) << 1u) | 1u;  // the stream of the path's pixel
             const int prim = hit.prim_type == 0 ? ~hit.prim_index : hit.prim_index;
             shade_bounce(sv, bg, rr_cap, depth, ro, rd, hit.did_hit != 0, hit.t, prim, ps, rng, out, time, mat);
-            ws.thr[path] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
-            ws.smp[path] = make_float4(ps.smp.x, ps.smp.y, ps.smp.z, ps.prev_spec ? 1.0f : 0.0f);
-            ws.rng[path] = rng.g.state;
+            // the pass ends after bounce max_depth - 1: nothing continues past it
+            if (depth + 1 >= max_depth) out.next = false;
         }
         // compacted pushes: every lane of the wave takes part
         int nslot, sslot;
         wave_push2(ns_slot(ws.counters, 1 - cur), out.next, out.shadow, nslot, sslot);
-        if (out.next) {
-            RayItem it;
-            it.ot = make_float4(out.no.x, out.no.y, out.no.z, time);
-            it.dp = make_float4(out.nd.x, out.nd.y, out.nd.z, __int_as_float(path));
-            ws.q[1 - cur][nslot] = it;
+        if (active) {
+            const float4 smp_out = make_float4(ps.smp.x, ps.smp.y, ps.smp.z, ps.prev_spec ? 1.0f : 0.0f);
+            if (out.next) {  // the state goes beside the path's next ray
+                RayItem it;
+                it.ot = make_float4(out.no.x, out.no.y, out.no.z, time);
+                it.dp = make_float4(out.nd.x, out.nd.y, out.nd.z, __int_as_float(path));
+                ws.q[1 - cur][nslot] = it;
+                ws.thr[1 - cur][nslot] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
+                ws.smp[1 - cur][nslot] = smp_out;
+                ws.rng[1 - cur][nslot] = rng.g.state;
+            } else {
+                ws.done[path] = smp_out;
+            }
         }
         if (out.shadow) {
             ShadowItem si;
             si.om = make_float4(out.so.x, out.so.y, out.so.z, out.smax);
-            si.dp = make_float4(out.sd.x, out.sd.y, out.sd.z, __int_as_float(path));
+            si.dp = make_float4(out.sd.x, out.sd.y, out.sd.z, __int_as_float(out.next ? nslot : ~path));
             si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, time);
             ws.sq[sslot] = si;
         }
@@ -300,15 +320,15 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneView sv, Workspace ws, i
     const int slot = blockIdx.x * kBlock + threadIdx.x;
     if (slot >= shadow_len(ws.counters, 1 - cur)) return;
     const ShadowItem si = ws.sq[slot];
-    const int path = __float_as_int(si.dp.w);
     const float tr = shadow_transmittance(sv, vec3(si.om.x, si.om.y, si.om.z), vec3(si.dp.x, si.dp.y, si.dp.z),
                                           si.om.w, lds_stack + threadIdx.x, kBlock, si.c.w);
     if (tr > 0.0f) {
-        float4 s = ws.smp[path];
+        float4* dst = shadow_target(ws, cur, __float_as_int(si.dp.w));
+        float4 s = *dst;
         s.x = s.x + si.c.x * tr;
         s.y = s.y + si.c.y * tr;
         s.z = s.z + si.c.z * tr;
-        ws.smp[path] = s;
+        *dst = s;
     }
 }
 
@@ -355,13 +375,13 @@ __global__ __launch_bounds__(kBlock) void k_shadow_persistent(SceneView sv, Work
             if (st != 0) {
                 if (st == 1) {
                     const ShadowItem si = ws.sq[slot];
-                    const int path = __float_as_int(si.dp.w);
                     const float tr = 1.0f;
-                    float4 v = ws.smp[path];
+                    float4* dst = shadow_target(ws, cur, __float_as_int(si.dp.w));
+                    float4 v = *dst;
                     v.x = v.x + si.c.x * tr;
                     v.y = v.y + si.c.y * tr;
                     v.z = v.z + si.c.z * tr;
-                    ws.smp[path] = v;
+                    *dst = v;
                 }
                 has = false;
             }
@@ -378,7 +398,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(Workspace ws, float* accum,
     if (p >= npix) return;
     float r = accum[3 * p], g = accum[3 * p + 1], b = accum[3 * p + 2];
     for (int k = 0; k < batch; k++) {
-        const float4 s = ws.smp[(size_t)k * npix + p];
+        const float4 s = ws.done[(size_t)k * npix + p];
         const vec3 c = clamp_sample(vec3(s.x, s.y, s.z));
         r = r + c.x;
         g = g + c.y;
@@ -543,8 +563,8 @@ size_t stack_bytes(const mcpt_scene_s* s) { return (size_t)kBlock * stack_entrie
 constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 constexpr size_t workspace_bytes(int64_t capacity) {
     return 2 * align256(sizeof(RayItem) * capacity) + align256(sizeof(mcpt_packed_hit) * capacity) +
-           align256(sizeof(ShadowItem) * capacity) + 2 * align256(sizeof(float4) * capacity) +
-           align256(sizeof(uint64_t) * capacity) + C_NCOUNTERS * sizeof(int);
+           align256(sizeof(ShadowItem) * capacity) + 5 * align256(sizeof(float4) * capacity) +
+           2 * align256(sizeof(uint64_t) * capacity) + C_NCOUNTERS * sizeof(int);
 }
 void layout_workspace(char* b, int64_t capacity, Workspace& ws) {
     const size_t cap_q = align256(sizeof(RayItem) * capacity), cap_h = align256(sizeof(mcpt_packed_hit) * capacity),
@@ -558,12 +578,16 @@ void layout_workspace(char* b, int64_t capacity, Workspace& ws) {
     b += cap_h;
     ws.sq = (ShadowItem*)b;
     b += cap_s;
-    ws.thr = (float4*)b;
+    for (int q = 0; q < 2; q++) {
+        ws.thr[q] = (float4*)b;
+        b += cap_4;
+        ws.smp[q] = (float4*)b;
+        b += cap_4;
+        ws.rng[q] = (uint64_t*)b;
+        b += cap_r;
+    }
+    ws.done = (float4*)b;
     b += cap_4;
-    ws.smp = (float4*)b;
-    b += cap_4;
-    ws.rng = (uint64_t*)b;
-    b += cap_r;
     ws.counters = (int*)b;
 }
 
@@ -598,7 +622,7 @@ char* g_scratch_ptr[kMaxDevices] = {};
 std::mutex g_scratch_mu[kMaxDevices];  // one no-allocation render per device at a time
 // the largest path count whose workspace fits the fixed array
 constexpr int64_t scratch_capacity() {
-    int64_t c = (int64_t)((kScratchBytes - C_NCOUNTERS * sizeof(int)) / 168);
+    int64_t c = (int64_t)((kScratchBytes - C_NCOUNTERS * sizeof(int)) / 224);
     while (c > 0 && workspace_bytes(c) > kScratchBytes) c--;
     return c;
 }
@@ -709,7 +733,7 @@ int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
             extend_launches++;
             MCPT_TRY(mark(K_EXTEND));
             k_shade<<<std::min((npath + kShadeBlock - 1) / kShadeBlock, shade_blocks), kShadeBlock, 0, st>>>(
-                sv, ws, cur, depth, bg, cam->rr_cap, npix);
+                sv, ws, cur, depth, o->max_depth, bg, cam->rr_cap, npix);
             MCPT_TRY(mark(K_SHADE));
             k_counters_shadow_total<<<1, 64, 0, st>>>(ws.counters, cur);
             if (persistent_shadow)
@@ -740,9 +764,9 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
     // Samples per wavefront pass. Every bounce ends with the slowest rays of
     // its launch, so the late, thin bounces of one sample leave most of the
     // GPU idle; carrying many samples per pass keeps them full. Default: up
-    // to 64 Mi paths (about 10.5 GB of queues and path state at 168 B per
+    // to 64 Mi paths (about 14 GB of queues and path state at 224 B per
     // path), at most a quarter of the context's HBM budget.
-    const int64_t per_path = 2 * 32 + 16 + 48 + 2 * 16 + 8;
+    const int64_t per_path = 2 * 32 + 16 + 48 + 2 * (2 * 16 + 8) + 16;
     int64_t cap_paths = std::min<int64_t>(int64_t(64) << 20, (int64_t)(ctx->budget / 4) / per_path);
     int batch = o->batch > 0 ? o->batch : (int)std::max<int64_t>(1, cap_paths / npix);
     batch = std::max(1, std::min(batch, o->spp));
