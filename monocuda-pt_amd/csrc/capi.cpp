@@ -180,10 +180,22 @@ int mcpt_ctx_create(int device, uint64_t hbm_budget_bytes, mcpt_ctx* out) {
         delete c;
         return set_error(MCPT_ERR_HIP, "hipStreamCreate failed");
     }
+    if (hipStreamCreateWithFlags(&c->c.side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->c.ev_shade, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->c.ev_shadow, hipEventDisableTiming) != hipSuccess) {
+        if (c->c.ev_shade) hipEventDestroy(c->c.ev_shade);
+        if (c->c.side) hipStreamDestroy(c->c.side);
+        hipStreamDestroy(c->c.stream);
+        delete c;
+        return set_error(MCPT_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
+    }
     // the fixed workspace of mcpt_render_noalloc, once per device, with the
     // device current
     int st = scratch_init(device);
     if (st < 0) {
+        hipEventDestroy(c->c.ev_shade);
+        hipEventDestroy(c->c.ev_shadow);
+        hipStreamDestroy(c->c.side);
         hipStreamDestroy(c->c.stream);
         delete c;
         return st;
@@ -201,6 +213,10 @@ int mcpt_ctx_destroy(mcpt_ctx ctx) {
     hipSetDevice(ctx->c.device);
     hipStreamSynchronize(ctx->c.stream);
     if (ctx->c.ws) dev_free(&ctx->c, ctx->c.ws, ctx->c.ws_bytes);
+    hipStreamSynchronize(ctx->c.side);
+    hipEventDestroy(ctx->c.ev_shade);
+    hipEventDestroy(ctx->c.ev_shadow);
+    hipStreamDestroy(ctx->c.side);
     hipStreamDestroy(ctx->c.stream);
     delete ctx;
     return MCPT_OK;

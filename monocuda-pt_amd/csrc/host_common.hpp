@@ -33,6 +33,11 @@ void clear_error();
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // a second stream (default priority) and two events that order it with
+    // `stream`: a render's shadow launches run there beside the next
+    // bounce's closest-hit launch
+    hipStream_t side = nullptr;
+    hipEvent_t ev_shade = nullptr, ev_shadow = nullptr;
     std::mutex alloc_mu;  // guards `used` (a server thread allocates too)
     uint64_t budget = 0;
     uint64_t used = 0;

@@ -87,17 +87,21 @@ __global__ void k_counters_init(int* c, int cur, int count) {
     }
 }
 // start of a bounce: record the previous bounce's queue lengths, clear the
-// queues this bounce fills
+// queues this bounce fills (the shadow counters belong to the shadow launch,
+// which may still run from the previous bounce)
 __global__ void k_counters_bounce(int* c, int cur) {
     if (threadIdx.x == 0) {
         *ns_slot(c, 1 - cur) = 0;
         c[C_WORK] = 0;
-        c[C_WORK_SHADOW] = 0;
         *u64slot(c, C_EXT_TOTAL) += (uint64_t)queue_len(c, cur);
     }
 }
+// before a shadow launch: its work counter, and the shadow-ray total
 __global__ void k_counters_shadow_total(int* c, int cur) {
-    if (threadIdx.x == 0) *u64slot(c, C_SHD_TOTAL) += (uint64_t)shadow_len(c, 1 - cur);
+    if (threadIdx.x == 0) {
+        c[C_WORK_SHADOW] = 0;
+        *u64slot(c, C_SHD_TOTAL) += (uint64_t)shadow_len(c, 1 - cur);
+    }
 }
 __global__ void k_counters_zero_totals(int* c) {
     if (threadIdx.x < C_NCOUNTERS) c[threadIdx.x] = 0;
@@ -667,10 +671,19 @@ int check_render_args(const mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_
 // Enqueues a whole render on the context's stream over the workspace `ws`
 // with `batch` samples per wavefront pass; `tm` (may be NULL) marks the
 // kernel spans. Returns the number of extend launches in *extend_launches.
+//
+// Without `tm`, each bounce's shadow rays run on the context's second stream
+// beside the next bounce's closest-hit rays: neither reads what the other
+// writes (the shadow rays add to the states of the next queue, which only
+// the next shade reads), so the one launch's last slow rays overlap the
+// other's start. The next shade waits for them. With `tm` every launch runs
+// in order on one stream, so each kernel's span is its own.
 int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, mcpt_film_s* film,
                    const Workspace& ws, int batch, Timer* tm, uint64_t* extend_launches_out) {
     Ctx* ctx = s->ctx;
     hipStream_t st = ctx->stream;
+    const bool overlap = tm == nullptr;
+    hipStream_t shs = overlap ? ctx->side : st;  // the shadow launches' stream
     const int W = cam->width, H = cam->height, npix = W * H;
     const SceneView sv = s->view();
     const vec3 bg = load3(cam->background);
@@ -732,17 +745,24 @@ int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
             }
             extend_launches++;
             MCPT_TRY(mark(K_EXTEND));
+            if (overlap && depth > 0) MCPT_HIP(hipStreamWaitEvent(st, ctx->ev_shadow, 0));  // the last shadow rays
             k_shade<<<std::min((npath + kShadeBlock - 1) / kShadeBlock, shade_blocks), kShadeBlock, 0, st>>>(
                 sv, ws, cur, depth, o->max_depth, bg, cam->rr_cap, npix);
             MCPT_TRY(mark(K_SHADE));
-            k_counters_shadow_total<<<1, 64, 0, st>>>(ws.counters, cur);
+            if (overlap) {
+                MCPT_HIP(hipEventRecord(ctx->ev_shade, st));
+                MCPT_HIP(hipStreamWaitEvent(shs, ctx->ev_shade, 0));
+            }
+            k_counters_shadow_total<<<1, 64, 0, shs>>>(ws.counters, cur);
             if (persistent_shadow)
-                k_shadow_persistent<<<std::min(grid, shadow_blocks), kBlock, lds, st>>>(sv, ws, cur, refill);
+                k_shadow_persistent<<<std::min(grid, shadow_blocks), kBlock, lds, shs>>>(sv, ws, cur, refill);
             else
-                k_shadow<<<grid, kBlock, lds, st>>>(sv, ws, cur);
+                k_shadow<<<grid, kBlock, lds, shs>>>(sv, ws, cur);
+            if (overlap) MCPT_HIP(hipEventRecord(ctx->ev_shadow, shs));
             MCPT_TRY(mark(K_SHADOW));
             cur = 1 - cur;
         }
+        if (overlap) MCPT_HIP(hipStreamWaitEvent(st, ctx->ev_shadow, 0));
         k_finalize<<<blocks_for(npix), kBlock, 0, st>>>(ws, film->accum.ptr, npix, nb);
         MCPT_TRY(mark(K_FINALIZE));
     }
@@ -780,8 +800,9 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
     MCPT_HIP(hipEventCreate(&e1));
     MCPT_HIP(hipEventRecord(e0, st));
     uint64_t extend_launches = 0;
-    int rc = enqueue_render(s, cam, o, film, ws, batch, &tm, &extend_launches);
+    int rc = enqueue_render(s, cam, o, film, ws, batch, timed ? &tm : nullptr, &extend_launches);
     if (rc < 0) {
+        (void)hipStreamSynchronize(ctx->side);
         (void)hipStreamSynchronize(st);
         hipEventDestroy(e0);
         hipEventDestroy(e1);
@@ -857,6 +878,7 @@ int render_noalloc(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
     std::lock_guard<std::mutex> lock(g_scratch_mu[dev]);
     uint64_t launches = 0;
     int rc = enqueue_render(s, cam, o, film, ws, batch, nullptr, &launches);
+    if (rc < 0) (void)hipStreamSynchronize(ctx->side);
     hipError_t e = hipStreamSynchronize(ctx->stream);  // the array is free for the next call once this returns
     if (rc < 0) return rc;
     if (e != hipSuccess) return set_error(MCPT_ERR_HIP, "render_noalloc: %s", hipGetErrorString(e));
