@@ -44,6 +44,30 @@ int dev_alloc(Ctx* ctx, void** p, size_t bytes) {
     return MCPT_OK;
 }
 
+int ctx_sync_objects_create(Ctx* c) {
+    for (hipStream_t* sp : {&c->side, &c->lane_stream, &c->lane_side})
+        if (hipStreamCreateWithFlags(sp, hipStreamNonBlocking) != hipSuccess) return MCPT_ERR_HIP;
+    for (hipEvent_t* ep : {&c->ev_shade[0], &c->ev_shade[1], &c->ev_shadow[0], &c->ev_shadow[1], &c->ev_fin[0],
+                           &c->ev_fin[1], &c->ev_join})
+        if (hipEventCreateWithFlags(ep, hipEventDisableTiming) != hipSuccess) return MCPT_ERR_HIP;
+    return MCPT_OK;
+}
+
+void ctx_sync_objects_destroy(Ctx* c) {
+    for (hipStream_t* sp : {&c->side, &c->lane_stream, &c->lane_side})
+        if (*sp) {
+            (void)hipStreamSynchronize(*sp);
+            (void)hipStreamDestroy(*sp);
+            *sp = nullptr;
+        }
+    for (hipEvent_t* ep : {&c->ev_shade[0], &c->ev_shade[1], &c->ev_shadow[0], &c->ev_shadow[1], &c->ev_fin[0],
+                           &c->ev_fin[1], &c->ev_join})
+        if (*ep) {
+            (void)hipEventDestroy(*ep);
+            *ep = nullptr;
+        }
+}
+
 void dev_free(Ctx* ctx, void* p, size_t bytes) {
     if (!p) return;
     std::lock_guard<std::mutex> lock(ctx->alloc_mu);
@@ -180,11 +204,8 @@ int mcpt_ctx_create(int device, uint64_t hbm_budget_bytes, mcpt_ctx* out) {
         delete c;
         return set_error(MCPT_ERR_HIP, "hipStreamCreate failed");
     }
-    if (hipStreamCreateWithFlags(&c->c.side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->c.ev_shade, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->c.ev_shadow, hipEventDisableTiming) != hipSuccess) {
-        if (c->c.ev_shade) hipEventDestroy(c->c.ev_shade);
-        if (c->c.side) hipStreamDestroy(c->c.side);
+    if (ctx_sync_objects_create(&c->c) < 0) {
+        ctx_sync_objects_destroy(&c->c);
         hipStreamDestroy(c->c.stream);
         delete c;
         return set_error(MCPT_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
@@ -193,9 +214,7 @@ int mcpt_ctx_create(int device, uint64_t hbm_budget_bytes, mcpt_ctx* out) {
     // device current
     int st = scratch_init(device);
     if (st < 0) {
-        hipEventDestroy(c->c.ev_shade);
-        hipEventDestroy(c->c.ev_shadow);
-        hipStreamDestroy(c->c.side);
+        ctx_sync_objects_destroy(&c->c);
         hipStreamDestroy(c->c.stream);
         delete c;
         return st;
@@ -213,10 +232,7 @@ int mcpt_ctx_destroy(mcpt_ctx ctx) {
     hipSetDevice(ctx->c.device);
     hipStreamSynchronize(ctx->c.stream);
     if (ctx->c.ws) dev_free(&ctx->c, ctx->c.ws, ctx->c.ws_bytes);
-    hipStreamSynchronize(ctx->c.side);
-    hipEventDestroy(ctx->c.ev_shade);
-    hipEventDestroy(ctx->c.ev_shadow);
-    hipStreamDestroy(ctx->c.side);
+    ctx_sync_objects_destroy(&ctx->c);
     hipStreamDestroy(ctx->c.stream);
     delete ctx;
     return MCPT_OK;

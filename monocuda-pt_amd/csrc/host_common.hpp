@@ -33,11 +33,15 @@ void clear_error();
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // a second stream (default priority) and two events that order it with
-    // `stream`: a render's shadow launches run there beside the next
-    // bounce's closest-hit launch
+    // Streams (default priority) and events of a render's two lanes. Lane 0
+    // runs on `stream` with shadow launches on `side`, lane 1 on lane_stream
+    // and lane_side; ev_shade / ev_shadow order a lane's shade and shadow
+    // launches, ev_fin the lanes' film updates, ev_join lane 1's end with
+    // `stream`.
     hipStream_t side = nullptr;
-    hipEvent_t ev_shade = nullptr, ev_shadow = nullptr;
+    hipStream_t lane_stream = nullptr, lane_side = nullptr;
+    hipEvent_t ev_shade[2] = {nullptr, nullptr}, ev_shadow[2] = {nullptr, nullptr};
+    hipEvent_t ev_fin[2] = {nullptr, nullptr}, ev_join = nullptr;
     std::mutex alloc_mu;  // guards `used` (a server thread allocates too)
     uint64_t budget = 0;
     uint64_t used = 0;
@@ -51,8 +55,11 @@ struct Ctx {
     // reusable wavefront workspace (see render.hip)
     void* ws = nullptr;
     size_t ws_bytes = 0;
-    int64_t ws_capacity = 0;
 };
+
+// the render lanes' streams and events (capi.cpp)
+int ctx_sync_objects_create(Ctx* c);
+void ctx_sync_objects_destroy(Ctx* c);
 
 // every device allocation of a context goes through these two calls, which
 // keep `used` within `budget`

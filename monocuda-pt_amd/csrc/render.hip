@@ -596,21 +596,20 @@ void layout_workspace(char* b, int64_t capacity, Workspace& ws) {
 }
 
 // The wavefront workspace lives in the context and grows to the largest
-// path capacity requested.
-int ensure_workspace(Ctx* ctx, int64_t capacity, Workspace& ws) {
-    if (ctx->ws_capacity < capacity) {
+// size requested: `nlanes` workspaces of `capacity` paths each, one after
+// the other.
+int ensure_workspace(Ctx* ctx, int64_t capacity, int nlanes, Workspace* ws) {
+    const size_t one = workspace_bytes(capacity), total = one * nlanes;
+    if (ctx->ws_bytes < total) {
         if (ctx->ws) dev_free(ctx, ctx->ws, ctx->ws_bytes);
         ctx->ws = nullptr;
         ctx->ws_bytes = 0;
-        ctx->ws_capacity = 0;
         void* p = nullptr;
-        const size_t total = workspace_bytes(capacity);
         MCPT_TRY(dev_alloc(ctx, &p, total));
         ctx->ws = p;
         ctx->ws_bytes = total;
-        ctx->ws_capacity = capacity;
     }
-    layout_workspace((char*)ctx->ws, ctx->ws_capacity, ws);
+    for (int l = 0; l < nlanes; l++) layout_workspace((char*)ctx->ws + l * one, capacity, ws[l]);
     return MCPT_OK;
 }
 
@@ -668,22 +667,29 @@ int check_render_args(const mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_
     return MCPT_OK;
 }
 
-// Enqueues a whole render on the context's stream over the workspace `ws`
-// with `batch` samples per wavefront pass; `tm` (may be NULL) marks the
-// kernel spans. Returns the number of extend launches in *extend_launches.
+// Enqueues a whole render over the workspaces ws[0..nlanes-1], `batch`
+// samples per wavefront pass; `tm` (may be NULL) marks the kernel spans.
+// Returns the number of extend launches in *extend_launches.
 //
-// Without `tm`, each bounce's shadow rays run on the context's second stream
-// beside the next bounce's closest-hit rays: neither reads what the other
-// writes (the shadow rays add to the states of the next queue, which only
-// the next shade reads), so the one launch's last slow rays overlap the
-// other's start. The next shade waits for them. With `tm` every launch runs
-// in order on one stream, so each kernel's span is its own.
+// Without `tm`:
+//   * each bounce's shadow rays run on a second stream beside the next
+//     bounce's closest-hit rays: neither reads what the other writes (the
+//     shadow rays add to the states of the next queue, which only the next
+//     shade reads), so one launch's last slow rays overlap the other's start;
+//     the next shade waits for them;
+//   * with two lanes, consecutive passes alternate between two workspaces
+//     and stream pairs, so one pass's launches fill the tails of the other's.
+//     Passes add to the film in pass order (each finalize waits for the
+//     previous one), so the film is the same bit for bit.
+// With `tm` every launch runs in order on one stream, so each kernel's span
+// is its own.
 int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, mcpt_film_s* film,
-                   const Workspace& ws, int batch, Timer* tm, uint64_t* extend_launches_out) {
+                   const Workspace* ws, int nlanes, int batch, Timer* tm, uint64_t* extend_launches_out) {
     Ctx* ctx = s->ctx;
-    hipStream_t st = ctx->stream;
     const bool overlap = tm == nullptr;
-    hipStream_t shs = overlap ? ctx->side : st;  // the shadow launches' stream
+    if (!overlap) nlanes = 1;
+    hipStream_t mains[2] = {ctx->stream, ctx->lane_stream};
+    hipStream_t sides[2] = {overlap ? ctx->side : ctx->stream, ctx->lane_side};
     const int W = cam->width, H = cam->height, npix = W * H;
     const SceneView sv = s->view();
     const vec3 bg = load3(cam->background);
@@ -694,7 +700,11 @@ int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
     enum { K_CAMERA, K_EXTEND, K_SHADE, K_SHADOW, K_FINALIZE, K_OTHER };
     auto mark = [&](int k) { return tm ? tm->mark(k) : MCPT_OK; };
 
-    k_counters_zero_totals<<<1, C_NCOUNTERS, 0, st>>>(ws.counters);
+    for (int l = 0; l < nlanes; l++) k_counters_zero_totals<<<1, C_NCOUNTERS, 0, ctx->stream>>>(ws[l].counters);
+    if (nlanes > 1) {  // lane 1 starts after the clear above
+        MCPT_HIP(hipEventRecord(ctx->ev_join, ctx->stream));
+        MCPT_HIP(hipStreamWaitEvent(ctx->lane_stream, ctx->ev_join, 0));
+    }
     MCPT_TRY(mark(K_OTHER));
 
     int persistent_blocks = 0, shadow_blocks = 0;
@@ -717,57 +727,84 @@ int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
     if (o->mode == MCPT_MODE_MEGAKERNEL) {
         for (int k = 0; k < o->spp; k++) {
             dim3 grid((W + 15) / 16, (H + 15) / 16);
-            k_megakernel<<<grid, kBlock, lds, st>>>(sv, *cam, o->sample_begin + k, o->seed, o->max_depth,
-                                                   film->accum.ptr, ws.counters);
+            k_megakernel<<<grid, kBlock, lds, ctx->stream>>>(sv, *cam, o->sample_begin + k, o->seed, o->max_depth,
+                                                            film->accum.ptr, ws[0].counters);
             MCPT_TRY(mark(K_EXTEND));
         }
     }
-    for (int k0 = 0; o->mode != MCPT_MODE_MEGAKERNEL && k0 < o->spp; k0 += batch) {
+    int pass = 0;
+    for (int k0 = 0; o->mode != MCPT_MODE_MEGAKERNEL && k0 < o->spp; k0 += batch, pass++) {
+        const int lane = pass % nlanes;
+        const Workspace& w = ws[lane];
+        hipStream_t st = mains[lane], shs = sides[lane];
+        hipEvent_t ev_shade = ctx->ev_shade[lane], ev_shadow = ctx->ev_shadow[lane];
         const int nb = std::min(batch, o->spp - k0);
         const int npath = npix * nb;
         const int grid = blocks_for(npath);
         int cur = 0;
-        k_counters_init<<<1, 64, 0, st>>>(ws.counters, cur, npath);
-        k_camera<<<grid, kBlock, 0, st>>>(*cam, o->sample_begin + k0, nb, o->seed, ws, cur);
+        k_counters_init<<<1, 64, 0, st>>>(w.counters, cur, npath);
+        k_camera<<<grid, kBlock, 0, st>>>(*cam, o->sample_begin + k0, nb, o->seed, w, cur);
         MCPT_TRY(mark(K_CAMERA));
         for (int depth = 0; depth < o->max_depth; depth++) {
-            k_counters_bounce<<<1, 64, 0, st>>>(ws.counters, cur);
+            k_counters_bounce<<<1, 64, 0, st>>>(w.counters, cur);
             MCPT_TRY(mark(K_OTHER));
             if (o->mode == MCPT_MODE_PERSISTENT) {
                 const int blocks = std::min(grid, persistent_blocks);
                 auto kern = count ? (ref ? k_extend_persistent<true, true> : k_extend_persistent<true, false>)
                                   : (ref ? k_extend_persistent<false, true> : k_extend_persistent<false, false>);
-                kern<<<blocks, kBlock, lds, st>>>(sv, ws, cur, refill);
+                kern<<<blocks, kBlock, lds, st>>>(sv, w, cur, refill);
             } else {
                 auto kern = count ? (ref ? k_extend<true, true> : k_extend<true, false>)
                                   : (ref ? k_extend<false, true> : k_extend<false, false>);
-                kern<<<grid, kBlock, lds, st>>>(sv, ws, cur);
+                kern<<<grid, kBlock, lds, st>>>(sv, w, cur);
             }
             extend_launches++;
             MCPT_TRY(mark(K_EXTEND));
-            if (overlap && depth > 0) MCPT_HIP(hipStreamWaitEvent(st, ctx->ev_shadow, 0));  // the last shadow rays
+            if (overlap && depth > 0) MCPT_HIP(hipStreamWaitEvent(st, ev_shadow, 0));  // the last shadow rays
             k_shade<<<std::min((npath + kShadeBlock - 1) / kShadeBlock, shade_blocks), kShadeBlock, 0, st>>>(
-                sv, ws, cur, depth, o->max_depth, bg, cam->rr_cap, npix);
+                sv, w, cur, depth, o->max_depth, bg, cam->rr_cap, npix);
             MCPT_TRY(mark(K_SHADE));
             if (overlap) {
-                MCPT_HIP(hipEventRecord(ctx->ev_shade, st));
-                MCPT_HIP(hipStreamWaitEvent(shs, ctx->ev_shade, 0));
+                MCPT_HIP(hipEventRecord(ev_shade, st));
+                MCPT_HIP(hipStreamWaitEvent(shs, ev_shade, 0));
             }
-            k_counters_shadow_total<<<1, 64, 0, shs>>>(ws.counters, cur);
+            k_counters_shadow_total<<<1, 64, 0, shs>>>(w.counters, cur);
             if (persistent_shadow)
-                k_shadow_persistent<<<std::min(grid, shadow_blocks), kBlock, lds, shs>>>(sv, ws, cur, refill);
+                k_shadow_persistent<<<std::min(grid, shadow_blocks), kBlock, lds, shs>>>(sv, w, cur, refill);
             else
-                k_shadow<<<grid, kBlock, lds, shs>>>(sv, ws, cur);
-            if (overlap) MCPT_HIP(hipEventRecord(ctx->ev_shadow, shs));
+                k_shadow<<<grid, kBlock, lds, shs>>>(sv, w, cur);
+            if (overlap) MCPT_HIP(hipEventRecord(ev_shadow, shs));
             MCPT_TRY(mark(K_SHADOW));
             cur = 1 - cur;
         }
-        if (overlap) MCPT_HIP(hipStreamWaitEvent(st, ctx->ev_shadow, 0));
-        k_finalize<<<blocks_for(npix), kBlock, 0, st>>>(ws, film->accum.ptr, npix, nb);
+        if (overlap) MCPT_HIP(hipStreamWaitEvent(st, ev_shadow, 0));
+        if (nlanes > 1 && pass > 0) MCPT_HIP(hipStreamWaitEvent(st, ctx->ev_fin[1 - lane], 0));  // film order
+        k_finalize<<<blocks_for(npix), kBlock, 0, st>>>(w, film->accum.ptr, npix, nb);
+        if (nlanes > 1) MCPT_HIP(hipEventRecord(ctx->ev_fin[lane], st));
         MCPT_TRY(mark(K_FINALIZE));
+    }
+    if (nlanes > 1) {  // the caller's stream ends after lane 1
+        MCPT_HIP(hipEventRecord(ctx->ev_join, ctx->lane_stream));
+        MCPT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
     }
     MCPT_HIP(hipGetLastError());
     *extend_launches_out = o->mode == MCPT_MODE_MEGAKERNEL ? (uint64_t)o->spp : extend_launches;
+    return MCPT_OK;
+}
+
+// sums of the lanes' u64 totals: extend rays, shadow rays, steps, leaves
+int read_totals(const Workspace* ws, int nlanes, uint64_t totals[4]) {
+    const int at[4] = {C_EXT_TOTAL, C_SHD_TOTAL, C_STEPS, C_LEAVES};
+    for (int i = 0; i < 4; i++) totals[i] = 0;
+    for (int l = 0; l < nlanes; l++) {
+        int ctr[C_NCOUNTERS];
+        MCPT_HIP(hipMemcpy(ctr, ws[l].counters, sizeof(ctr), hipMemcpyDeviceToHost));
+        for (int i = 0; i < 4; i++) {
+            uint64_t v;
+            std::memcpy(&v, ctr + at[i], sizeof(uint64_t));
+            totals[i] += v;
+        }
+    }
     return MCPT_OK;
 }
 
@@ -785,25 +822,28 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
     // its launch, so the late, thin bounces of one sample leave most of the
     // GPU idle; carrying many samples per pass keeps them full. Default: up
     // to 64 Mi paths (about 14 GB of queues and path state at 224 B per
-    // path), at most a quarter of the context's HBM budget.
+    // path), at most a quarter of the context's HBM budget. An untimed
+    // wavefront render of two or more samples splits them over two lanes
+    // (enqueue_render), each with half the paths.
     const int64_t per_path = 2 * 32 + 16 + 48 + 2 * (2 * 16 + 8) + 16;
     int64_t cap_paths = std::min<int64_t>(int64_t(64) << 20, (int64_t)(ctx->budget / 4) / per_path);
     int batch = o->batch > 0 ? o->batch : (int)std::max<int64_t>(1, cap_paths / npix);
     batch = std::max(1, std::min(batch, o->spp));
     if ((int64_t)npix * batch > (int64_t(1) << 30)) batch = std::max(1, (1 << 30) / npix);
+    const int nlanes = (!timed && o->mode != MCPT_MODE_MEGAKERNEL && o->spp >= 2) ? 2 : 1;
+    if (nlanes == 2 && o->batch <= 0) batch = std::max(1, (std::min(batch, o->spp) + 1) / 2);
 
-    Workspace ws;
-    MCPT_TRY(ensure_workspace(ctx, o->mode == MCPT_MODE_MEGAKERNEL ? 1 : (int64_t)npix * batch, ws));
+    Workspace ws[2];
+    MCPT_TRY(ensure_workspace(ctx, o->mode == MCPT_MODE_MEGAKERNEL ? 1 : (int64_t)npix * batch, nlanes, ws));
     Timer tm(timed, st);
     hipEvent_t e0, e1;
     MCPT_HIP(hipEventCreate(&e0));
     MCPT_HIP(hipEventCreate(&e1));
     MCPT_HIP(hipEventRecord(e0, st));
     uint64_t extend_launches = 0;
-    int rc = enqueue_render(s, cam, o, film, ws, batch, timed ? &tm : nullptr, &extend_launches);
+    int rc = enqueue_render(s, cam, o, film, ws, nlanes, batch, timed ? &tm : nullptr, &extend_launches);
     if (rc < 0) {
-        (void)hipStreamSynchronize(ctx->side);
-        (void)hipStreamSynchronize(st);
+        for (hipStream_t x : {ctx->side, ctx->lane_stream, ctx->lane_side, st}) (void)hipStreamSynchronize(x);
         hipEventDestroy(e0);
         hipEventDestroy(e1);
         return rc;
@@ -812,11 +852,8 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
     MCPT_HIP(hipEventSynchronize(e1));
     float total_ms = 0.f;
     MCPT_HIP(hipEventElapsedTime(&total_ms, e0, e1));
-    int ctr[C_NCOUNTERS];
-    MCPT_HIP(hipMemcpy(ctr, ws.counters, sizeof(ctr), hipMemcpyDeviceToHost));
     uint64_t totals[4];  // extend rays, shadow rays, steps, leaves
-    const int at[4] = {C_EXT_TOTAL, C_SHD_TOTAL, C_STEPS, C_LEAVES};
-    for (int i = 0; i < 4; i++) std::memcpy(&totals[i], ctr + at[i], sizeof(uint64_t));
+    MCPT_TRY(read_totals(ws, nlanes, totals));
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->total_ms = total_ms;
@@ -877,7 +914,7 @@ int render_noalloc(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
     layout_workspace(g_scratch_ptr[dev], o->mode == MCPT_MODE_MEGAKERNEL ? 1 : (int64_t)npix * batch, ws);
     std::lock_guard<std::mutex> lock(g_scratch_mu[dev]);
     uint64_t launches = 0;
-    int rc = enqueue_render(s, cam, o, film, ws, batch, nullptr, &launches);
+    int rc = enqueue_render(s, cam, o, film, &ws, 1, batch, nullptr, &launches);
     if (rc < 0) (void)hipStreamSynchronize(ctx->side);
     hipError_t e = hipStreamSynchronize(ctx->stream);  // the array is free for the next call once this returns
     if (rc < 0) return rc;
