@@ -76,7 +76,9 @@ MCPT_HD float mf_reduce_pio2(float x, int* quadrant) {
     return r;
 }
 MCPT_HD float mf_sin(float x) {
-    if (!(mf_fabs(x) < 6400.0f)) return x - x; /* NaN for inf/NaN; out-of-domain -> 0 */
+    /* outside the reduction's exact range the result would be silently wrong:
+       return NaN (quiet, propagates into the sample) instead */
+    if (!(mf_fabs(x) < 6400.0f)) return __builtin_nanf("");
     int q;
     float r = mf_reduce_pio2(x, &q);
     switch (q & 3) {
@@ -87,7 +89,7 @@ MCPT_HD float mf_sin(float x) {
     }
 }
 MCPT_HD float mf_cos(float x) {
-    if (!(mf_fabs(x) < 6400.0f)) return x - x;
+    if (!(mf_fabs(x) < 6400.0f)) return __builtin_nanf("");
     int q;
     float r = mf_reduce_pio2(x, &q);
     switch (q & 3) {

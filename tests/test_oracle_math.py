@@ -75,6 +75,11 @@ def test_special_values(oracle):
     assert abs(f("atan2", -1.0, 1.0) + np.pi / 4) < 1e-7
     assert f("atan2", 0.0, 1.0) == 0.0 and abs(f("atan2", 0.0, -1.0) - np.pi) < 1e-6
     assert abs(f("atan2", 1.0, 0.0) - np.pi / 2) < 1e-7
+    # sin/cos outside the exact reduction range (|x| >= 6400) and non-finite
+    # arguments give NaN, never a plausible number
+    for v in (6400.0, -7000.0, 1e30, np.inf, np.nan):
+        assert np.isnan(f("sin", v)) and np.isnan(f("cos", v)), v
+    assert np.isfinite(f("sin", 6399.0)) and abs(f("sin", 6399.0) - np.sin(np.float64(np.float32(6399.0)))) < 1e-5
 
 
 # ------------------------------------------------------------------ PCG32
