@@ -277,7 +277,7 @@ typedef struct mcpt_render_opts {
                              0 picks the default                           */
     uint32_t flags;
     int32_t batch;        /* wavefront modes: samples in flight per pass
-                             (paths = W*H*batch); 0 picks about 4 Mi paths */
+                             (paths = W*H*batch); 0 picks the default     */
     int32_t reserved;
 } mcpt_render_opts;
 
