@@ -441,11 +441,13 @@ int mcpt_render_batch(const mcpt_scene* scenes, const mcpt_camera* cameras, int3
  * every scene in one pass when a quarter of the HBM budget holds it (a pass
  * ends with its slowest rays, so fewer, fuller passes are faster), else as
  * many samples as fit. Launch shape: the refill threshold, then the
- * persistent blocks per CU, each kept when it lowers the device time per
- * path of a one-sample calibration pass (the samples after the requested
- * range, into scratch films). A tune depends on the workload's shape and is
- * meant to be reused across batches of that shape. search_ms (optional)
- * receives the device time the search spent. */
+ * persistent blocks per CU, each kept when it shortens the closest-hit
+ * launch of a calibration bounce: the camera rays of the sample after the
+ * requested range are traced and shaded once, and the next bounce's extend
+ * launch is timed on up to 2 Mi of its rays for each candidate. No film is
+ * written. A tune depends on the workload's shape and is meant to be reused
+ * across batches of that shape. search_ms (optional) receives the device
+ * time the search spent. */
 int mcpt_batch_autotune(const mcpt_scene* scenes, const mcpt_camera* cameras, int32_t n, const mcpt_render_opts* opts,
                         mcpt_batch_tune* best, double* search_ms);
 

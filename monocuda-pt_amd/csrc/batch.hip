@@ -568,45 +568,65 @@ __global__ __launch_bounds__(kBlock) void k_bb_finalize(const BScene* tab, int n
     a[2] = b;
 }
 
+// Rays of the second bounce an autotune launch traces: enough that a launch
+// lasts well over the ~0.5 ms its slowest rays take by themselves.
+constexpr int kCalRays = 2 << 20;
+
 constexpr size_t kPathBytes = 2 * sizeof(RayItem) + sizeof(mcpt_packed_hit) + sizeof(ShadowItem) + 2 * sizeof(float4) +
                               sizeof(uint64_t);
 
-// One batched render. accum[i] overrides scene i's film (scratch films of
-// the autotuner); nullptr -> films.
-int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const mcpt_render_opts* o,
-                 float* const* accum, const mcpt_batch_tune* tune, mcpt_render_stats* stats) {
-    Ctx* ctx = scenes[0]->ctx;
-    hipStream_t st = ctx->stream;
-    bool moving = false;  // ray records then carry their time too
-    for (int i = 0; i < n; i++) moving = moving || scenes[i]->moving_spheres;
-    std::vector<BScene> tab(n);
-    std::vector<BvhView> bvh(n);
-    int64_t P = 0;
+// Device tables and one wavefront workspace of a batched render; the
+// memory goes back to the context when the object does.
+struct BatchRun {
+    Ctx* ctx = nullptr;
+    std::vector<BScene> tab;
+    std::vector<BvhView> bvh;
+    int n = 0;
+    int64_t P = 0;  // pixels of all scenes
     int max_stack = 1;
+    bool moving = false;  // ray records then carry their time too
+    void* mem = nullptr;
+    size_t bytes = 0;
+    BScene* dtab = nullptr;
+    BvhView* dbvh = nullptr;
+    BWs ws{};
+    size_t lds = 0;
+    int per_cu = 1;  // occupancy limit of the persistent extend
+    BatchRun() = default;
+    BatchRun(const BatchRun&) = delete;
+    BatchRun& operator=(const BatchRun&) = delete;
+    ~BatchRun() {
+        if (mem) dev_free(ctx, mem, bytes);
+    }
+};
+
+// the scene table (accum[i] overrides scene i's film: the autotuner's
+// scratch films; nullptr -> films)
+int batch_tables(const mcpt_scene* scenes, const mcpt_camera* cams, int n, float* const* accum, BatchRun& br) {
+    br.ctx = scenes[0]->ctx;
+    br.n = n;
+    br.tab.resize(n);
+    br.bvh.resize(n);
+    for (int i = 0; i < n; i++) br.moving = br.moving || scenes[i]->moving_spheres;
     for (int i = 0; i < n; i++) {
-        tab[i].sv = scenes[i]->view();
-        tab[i].cam = cams[i];
-        tab[i].accum = accum[i];
-        tab[i].foff = (int)P;
-        tab[i].npix = cams[i].width * cams[i].height;
-        bvh[i] = tab[i].sv.bvh;
-        P += tab[i].npix;
-        max_stack = std::max(max_stack, scenes[i]->max_depth);
+        br.tab[i].sv = scenes[i]->view();
+        br.tab[i].cam = cams[i];
+        br.tab[i].accum = accum[i];
+        br.tab[i].foff = (int)br.P;
+        br.tab[i].npix = cams[i].width * cams[i].height;
+        br.bvh[i] = br.tab[i].sv.bvh;
+        br.P += br.tab[i].npix;
+        br.max_stack = std::max(br.max_stack, scenes[i]->max_depth);
     }
-    if (P > (1 << 30)) return set_error(MCPT_ERR_INVALID, "too many pixels in one batch");
-    // samples of every scene per pass: the tuned value, or as many as fit a
-    // quarter of the budget (at most 64 Mi paths)
-    int nb = tune && tune->samples_per_pass > 0 ? tune->samples_per_pass : 0;
-    if (nb == 0) {
-        const int64_t cap = std::min<int64_t>(int64_t(64) << 20, (int64_t)(ctx->budget / 4) / (int64_t)kPathBytes);
-        nb = (int)std::max<int64_t>(1, cap / P);
-    }
-    nb = std::max(1, std::min(nb, o->spp));
-    if (P * nb > (int64_t(1) << 30)) nb = std::max(1, (int)((int64_t(1) << 30) / P));
-    const int refill = tune && tune->refill_lanes > 0 ? std::min(64, tune->refill_lanes) : 32;
-    const int64_t capacity = P * nb;
-    // device tables and workspace in one allocation
+    if (br.P > (1 << 30)) return set_error(MCPT_ERR_INVALID, "too many pixels in one batch");
+    return MCPT_OK;
+}
+
+// device tables and a workspace of `capacity` paths in one allocation; the
+// tables are copied on the context's stream
+int batch_alloc(BatchRun& br, int64_t capacity) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const int n = br.n;
     const size_t parts[] = {al(n * sizeof(BScene)),        al(n * sizeof(BvhView)),
                             al(capacity * sizeof(RayItem)), al(capacity * sizeof(RayItem)),
                             al(capacity * sizeof(mcpt_packed_hit)), al(capacity * sizeof(ShadowItem)),
@@ -614,15 +634,15 @@ int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const
                             al(capacity * sizeof(uint64_t)), al(C_NCOUNTERS * sizeof(int))};
     size_t total = 0;
     for (size_t x : parts) total += x;
-    void* mem = nullptr;
-    MCPT_TRY(dev_alloc(ctx, &mem, total));
-    char* b = (char*)mem;
+    MCPT_TRY(dev_alloc(br.ctx, &br.mem, total));
+    br.bytes = total;
+    char* b = (char*)br.mem;
     int k = 0;
-    BScene* dtab = (BScene*)b;
+    br.dtab = (BScene*)b;
     b += parts[k++];
-    BvhView* dbvh = (BvhView*)b;
+    br.dbvh = (BvhView*)b;
     b += parts[k++];
-    BWs ws;
+    BWs& ws = br.ws;
     ws.q[0] = (RayItem*)b, b += parts[k++];
     ws.q[1] = (RayItem*)b, b += parts[k++];
     ws.hits = (mcpt_packed_hit*)b, b += parts[k++];
@@ -631,18 +651,52 @@ int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const
     ws.smp = (float4*)b, b += parts[k++];
     ws.rng = (uint64_t*)b, b += parts[k++];
     ws.counters = (int*)b;
-    ws.tb = moving ? 24 : 0;
-    const size_t lds = (size_t)kBlock * stack_entries(max_stack) * sizeof(int);
+    ws.tb = br.moving ? 24 : 0;
+    br.lds = (size_t)kBlock * stack_entries(br.max_stack) * sizeof(int);
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bb_extend, kBlock, lds);
-    per_cu = std::max(1, per_cu);
+    MCPT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bb_extend, kBlock, br.lds));
+    br.per_cu = std::max(1, per_cu);
+    hipStream_t st = br.ctx->stream;
+    MCPT_HIP(hipMemcpyAsync(br.dtab, br.tab.data(), n * sizeof(BScene), hipMemcpyHostToDevice, st));
+    MCPT_HIP(hipMemcpyAsync(br.dbvh, br.bvh.data(), n * sizeof(BvhView), hipMemcpyHostToDevice, st));
+    return MCPT_OK;
+}
+
+// persistent blocks of a tune's grid
+int tuned_blocks(const BatchRun& br, const mcpt_batch_tune* tune) {
+    int per_cu = br.per_cu;
     if (tune && tune->blocks_per_cu > 0) per_cu = std::min(per_cu, tune->blocks_per_cu);
-    const int persistent_blocks = per_cu * ctx->cu_count;
+    return per_cu * br.ctx->cu_count;
+}
+
+// samples of every scene per pass: the tuned value, or as many as fit a
+// quarter of the budget (at most 64 Mi paths)
+int default_samples_per_pass(const Ctx* ctx, int64_t P, int spp) {
+    const int64_t cap = std::min<int64_t>(int64_t(64) << 20, (int64_t)(ctx->budget / 4) / (int64_t)kPathBytes);
+    int nb = (int)std::max<int64_t>(1, std::min<int64_t>(spp, cap / std::max<int64_t>(1, P)));
+    if (P * nb > (int64_t(1) << 30)) nb = std::max(1, (int)((int64_t(1) << 30) / P));
+    return nb;
+}
+
+// One batched render.
+int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const mcpt_render_opts* o,
+                 float* const* accum, const mcpt_batch_tune* tune, mcpt_render_stats* stats) {
+    BatchRun br;
+    MCPT_TRY(batch_tables(scenes, cams, n, accum, br));
+    Ctx* ctx = br.ctx;
+    hipStream_t st = ctx->stream;
+    const int64_t P = br.P;
+    int nb = tune && tune->samples_per_pass > 0 ? tune->samples_per_pass : default_samples_per_pass(ctx, P, o->spp);
+    nb = std::max(1, std::min(nb, o->spp));
+    if (P * nb > (int64_t(1) << 30)) nb = std::max(1, (int)((int64_t(1) << 30) / P));
+    const int refill = tune && tune->refill_lanes > 0 ? std::min(64, tune->refill_lanes) : 32;
+    MCPT_TRY(batch_alloc(br, P * nb));
+    const BWs& ws = br.ws;
+    const int persistent_blocks = tuned_blocks(br, tune);
+    const size_t lds = br.lds;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (e == hipSuccess) e = hipEventCreate(&e0);
+    hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
-    if (e == hipSuccess) e = hipMemcpyAsync(dtab, tab.data(), n * sizeof(BScene), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(dbvh, bvh.data(), n * sizeof(BvhView), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipEventRecord(e0, st);
     uint64_t launches = 0;
     if (e == hipSuccess) {
@@ -653,19 +707,20 @@ int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const
             const int grid = blocks_for(paths);
             int cur = 0;
             k_bb_init<<<1, 64, 0, st>>>(ws.counters, (int)paths);
-            k_bb_camera<<<grid, kBlock, 0, st>>>(dtab, n, (int)P, (uint32_t)(o->sample_begin + k0), cnt, o->seed, ws);
+            k_bb_camera<<<grid, kBlock, 0, st>>>(br.dtab, n, (int)P, (uint32_t)(o->sample_begin + k0), cnt, o->seed,
+                                                 ws);
             for (int d = 0; d < o->max_depth; d++) {
                 k_bb_bounce<<<1, 64, 0, st>>>(ws.counters, cur);
-                k_bb_extend<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(dbvh, ws, cur, refill);
-                k_bb_shade<<<(int)((paths + kShadeBlock - 1) / kShadeBlock), kShadeBlock, 0, st>>>(dtab, ws, cur, d,
+                k_bb_extend<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(br.dbvh, ws, cur, refill);
+                k_bb_shade<<<(int)((paths + kShadeBlock - 1) / kShadeBlock), kShadeBlock, 0, st>>>(br.dtab, ws, cur, d,
                                                                                                   (int)P);
                 k_bb_shadow_total<<<1, 64, 0, st>>>(ws.counters, cur);
-                k_bb_shadow_persistent<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(dtab, ws, cur,
+                k_bb_shadow_persistent<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(br.dtab, ws, cur,
                                                                                                 refill);
                 launches++;
                 cur = 1 - cur;
             }
-            k_bb_finalize<<<blocks_for(P), kBlock, 0, st>>>(dtab, n, (int)P, cnt, ws.smp);
+            k_bb_finalize<<<blocks_for(P), kBlock, 0, st>>>(br.dtab, n, (int)P, cnt, ws.smp);
         }
         e = hipGetLastError();
     }
@@ -678,8 +733,10 @@ int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const
     if (e == hipSuccess) e = hipMemcpy(&tot[1], ws.counters + C_SHD, sizeof(uint64_t), hipMemcpyDeviceToHost);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    dev_free(ctx, mem, total);
-    if (e != hipSuccess) return set_error(MCPT_ERR_HIP, "batched render: %s", hipGetErrorString(e));
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(st);
+        return set_error(MCPT_ERR_HIP, "batched render: %s", hipGetErrorString(e));
+    }
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->total_ms = ms;
@@ -929,69 +986,106 @@ int mcpt_batch_autotune(const mcpt_scene* scenes, const mcpt_camera* cams, int32
     for (int i = 0; i < n; i++) MCPT_TRY(scenes[i]->check_textures());
     MCPT_TRY(check_moving_spheres(scenes, n));
     if (!best) return set_error(MCPT_ERR_INVALID, "best is NULL");
-    Ctx* ctx = scenes[0]->ctx;
-    MCPT_HIP(hipSetDevice(ctx->device));
-    // scratch films (the calibration passes use the scenes' samples after
-    // the requested range, so they draw the same kind of paths)
-    int64_t P = 0;
-    for (int i = 0; i < n; i++) P += (int64_t)cams[i].width * cams[i].height;
-    void* scratch = nullptr;
-    MCPT_TRY(dev_alloc(ctx, &scratch, (size_t)P * 3 * sizeof(float)));
-    std::vector<float*> acc(n);
-    {
-        int64_t off = 0;
-        for (int i = 0; i < n; i++) {
-            acc[i] = (float*)scratch + 3 * off;
-            off += (int64_t)cams[i].width * cams[i].height;
-        }
-    }
-    mcpt_render_opts cal = *o;
-    cal.sample_begin = o->sample_begin + o->spp;
-    cal.spp = 1;
-    double spent = 0.0;
-    auto trial = [&](const mcpt_batch_tune& t, double& per_path_ns) {
-        mcpt_render_stats rs;
-        const int s = render_batch(scenes, cams, n, &cal, acc.data(), &t, &rs);
-        if (s < 0) return s;
-        spent += rs.total_ms;
-        per_path_ns = rs.total_ms * 1e6 / (double)rs.paths;
+    MCPT_HIP(hipSetDevice(scenes[0]->ctx->device));
+    // Calibration: the camera rays of one sample after the requested range
+    // (the same kind of paths) and their first bounce, with the default
+    // shape; then the extend launch of the second bounce, timed on the
+    // first kCalRays of its queue for each candidate shape. The extend
+    // launch is most of a batched render's time and reads nothing it
+    // writes, so it can run again on the same queue; one launch per shape
+    // costs ~1.5 ms where a whole calibration pass costs ~9 ms.
+    BatchRun br;
+    std::vector<float*> no_films(n, nullptr);  // the calibration writes no film
+    MCPT_TRY(batch_tables(scenes, cams, n, no_films.data(), br));
+    Ctx* ctx = br.ctx;
+    hipStream_t st = ctx->stream;
+    const int64_t P = br.P;
+    MCPT_TRY(batch_alloc(br, P));
+    const BWs& ws = br.ws;
+    const int grid = blocks_for(P);
+    std::vector<hipEvent_t> ev;
+    auto event = [&](hipEvent_t* out) {
+        hipEvent_t e = nullptr;
+        MCPT_HIP(hipEventCreate(&e));
+        ev.push_back(e);
+        MCPT_HIP(hipEventRecord(e, st));
+        if (out) *out = e;
         return MCPT_OK;
     };
-    // Queue capacity: every sample of every scene in one pass when the
-    // budget holds it (a pass ends with its slowest rays, so fewer, fuller
-    // passes are faster), otherwise as many samples as fit.
-    const int64_t cap = std::min<int64_t>(int64_t(64) << 20, (int64_t)(ctx->budget / 4) / (int64_t)kPathBytes);
-    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(o->spp, cap / std::max<int64_t>(1, P)));
-    // Launch shape, timed on one-sample passes: the refill threshold, then
-    // the persistent grid, each kept when it lowers the device time per path.
-    mcpt_batch_tune cur = {1, 32, 0, 0};
-    double cur_ns = 0.0;
-    int s = trial(cur, cur_ns);  // warm-up (allocation, caches)
-    if (s >= 0) s = trial(cur, cur_ns);
-    const int refill_c[] = {16, 48};
-    const int blocks_c[] = {4, 2};
-    for (int v : refill_c) {
-        if (s < 0) break;
-        mcpt_batch_tune t = cur;
-        t.refill_lanes = v;
-        double ns = 0.0;
-        s = trial(t, ns);
-        if (s >= 0 && ns < cur_ns) cur = t, cur_ns = ns;
+    auto cleanup = [&](int rc) {
+        (void)hipStreamSynchronize(st);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        return rc;
+    };
+    hipEvent_t e_start = nullptr;
+    int rc = event(&e_start);
+    if (rc < 0) return cleanup(rc);
+    k_b_fill<<<1, C_NCOUNTERS, 0, st>>>(ws.counters, C_NCOUNTERS, 0);
+    k_bb_init<<<1, 64, 0, st>>>(ws.counters, (int)P);
+    k_bb_camera<<<grid, kBlock, 0, st>>>(br.dtab, n, (int)P, (uint32_t)(o->sample_begin + o->spp), 1, o->seed, ws);
+    k_bb_bounce<<<1, 64, 0, st>>>(ws.counters, 0);
+    k_bb_extend<<<std::min(grid, tuned_blocks(br, nullptr)), kBlock, br.lds, st>>>(br.dbvh, ws, 0, 32);
+    if (o->max_depth > 1) k_bb_shade<<<(int)((P + kShadeBlock - 1) / kShadeBlock), kShadeBlock, 0, st>>>(
+        br.dtab, ws, 0, 0, (int)P);
+    if (hipGetLastError() != hipSuccess) return cleanup(set_error(MCPT_ERR_HIP, "autotune: launch failed"));
+    int len[2] = {0, 0};
+    if (hipMemcpyAsync(len, ws.counters + (o->max_depth > 1 ? C_NS1 : C_NS0), sizeof(int), hipMemcpyDeviceToHost,
+                       st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return cleanup(set_error(MCPT_ERR_HIP, "autotune: reading the queue length failed"));
+    const int q = o->max_depth > 1 ? 1 : 0;  // the queue the timed launches trace
+    const int m = std::min(len[0], kCalRays);
+    mcpt_batch_tune cur = {default_samples_per_pass(ctx, P, o->spp), 32, 0, 0};
+    hipEvent_t e_end = e_start;
+    if (m > 0) {
+        k_b_fill<<<1, 64, 0, st>>>(ws.counters + (q ? C_NS1 : C_NS0), 1, m);
+        auto launch = [&](const mcpt_batch_tune& t) {
+            k_b_fill<<<1, 64, 0, st>>>(ws.counters + C_WORK, 1, 0);
+            k_bb_extend<<<std::min(blocks_for(m), tuned_blocks(br, &t)), kBlock, br.lds, st>>>(br.dbvh, ws, q,
+                                                                                                t.refill_lanes);
+        };
+        auto timed = [&](const mcpt_batch_tune& t, float& ms) {
+            hipEvent_t a = nullptr, b = nullptr;
+            MCPT_TRY(event(&a));
+            launch(t);
+            MCPT_TRY(event(&b));
+            MCPT_HIP(hipEventSynchronize(b));
+            MCPT_HIP(hipEventElapsedTime(&ms, a, b));
+            e_end = b;
+            return MCPT_OK;
+        };
+        launch(cur);  // warm-up: the queue's rays and nodes into the caches
+        float cur_ms = 0.f;
+        rc = timed(cur, cur_ms);
+        // the refill threshold, then the grid, each kept when it is faster
+        const int refill_c[] = {16, 48};
+        const int blocks_c[] = {4, 2};
+        for (int v : refill_c) {
+            if (rc < 0) break;
+            mcpt_batch_tune t = cur;
+            t.refill_lanes = v;
+            float ms = 0.f;
+            rc = timed(t, ms);
+            if (rc >= 0 && ms < cur_ms) cur = t, cur_ms = ms;
+        }
+        for (int v : blocks_c) {
+            if (rc < 0 || v >= br.per_cu) continue;
+            mcpt_batch_tune t = cur;
+            t.blocks_per_cu = v;
+            float ms = 0.f;
+            rc = timed(t, ms);
+            if (rc >= 0 && ms < cur_ms) cur = t, cur_ms = ms;
+        }
+        if (rc < 0) return cleanup(rc);
     }
-    for (int v : blocks_c) {
-        if (s < 0) break;
-        mcpt_batch_tune t = cur;
-        t.blocks_per_cu = v;
-        double ns = 0.0;
-        s = trial(t, ns);
-        if (s >= 0 && ns < cur_ns) cur = t, cur_ns = ns;
+    float spent = 0.f;
+    if (e_end != e_start) {
+        if (hipEventElapsedTime(&spent, e_start, e_end) != hipSuccess)
+            return cleanup(set_error(MCPT_ERR_HIP, "autotune: event timing failed"));
     }
-    cur.samples_per_pass = nb;
-    dev_free(ctx, scratch, (size_t)P * 3 * sizeof(float));
-    if (s < 0) return s;
     *best = cur;
     if (search_ms) *search_ms = spent;
-    return MCPT_OK;
+    return cleanup(MCPT_OK);
 }
 
 }  // extern "C"

@@ -81,6 +81,9 @@ def test_batched_render_matches_single_renders(mcpt, ctx):
     best, search_ms = mcpt.batch_autotune(scenes, cams, spp=spp, max_depth=depth, seed=seed)
     print(f"autotune {best.as_dict()} in {search_ms:.1f} ms", flush=True)
     assert best.samples_per_pass >= 1 and best.refill_lanes in (16, 32, 48) and best.blocks_per_cu in (0, 2, 4)
+    # depth 1: the timed launches trace the camera rays themselves
+    one, one_ms = mcpt.batch_autotune(scenes, cams, spp=1, max_depth=1, seed=seed)
+    assert one.samples_per_pass == 1 and one.refill_lanes in (16, 32, 48) and one_ms > 0.0
     films = [ctx.film(c.width, c.height) for c in cams]
     mcpt.render_batch(scenes, cams, films, spp=spp, max_depth=depth, seed=seed, tune=best)
     for f, ref in zip(films, refs):
