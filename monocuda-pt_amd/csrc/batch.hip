@@ -694,14 +694,23 @@ int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const
     const BWs& ws = br.ws;
     const int persistent_blocks = tuned_blocks(br, tune);
     const size_t lds = br.lds;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    // Each bounce's shadow rays run on the context's second stream beside
+    // the next bounce's closest-hit rays: they add light to the samples of
+    // their paths (smp), which only the next shade and finalize read, and
+    // the extend reads neither the shadow queue nor the samples. The next
+    // shade waits for them (and so its shadow items never overwrite a
+    // queue still being read).
+    hipStream_t sd = ctx->side;
+    hipEvent_t e0 = nullptr, e1 = nullptr, ev_shade = nullptr, ev_shadow = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_shade, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev_shadow, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventRecord(e0, st);
     uint64_t launches = 0;
     if (e == hipSuccess) {
         k_b_fill<<<1, C_NCOUNTERS, 0, st>>>(ws.counters, C_NCOUNTERS, 0);
-        for (int k0 = 0; k0 < o->spp; k0 += nb) {
+        for (int k0 = 0; k0 < o->spp && e == hipSuccess; k0 += nb) {
             const int cnt = std::min(nb, o->spp - k0);
             const int64_t paths = P * cnt;
             const int grid = blocks_for(paths);
@@ -709,20 +718,25 @@ int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const
             k_bb_init<<<1, 64, 0, st>>>(ws.counters, (int)paths);
             k_bb_camera<<<grid, kBlock, 0, st>>>(br.dtab, n, (int)P, (uint32_t)(o->sample_begin + k0), cnt, o->seed,
                                                  ws);
-            for (int d = 0; d < o->max_depth; d++) {
+            for (int d = 0; d < o->max_depth && e == hipSuccess; d++) {
                 k_bb_bounce<<<1, 64, 0, st>>>(ws.counters, cur);
                 k_bb_extend<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(br.dbvh, ws, cur, refill);
+                if (d > 0) e = hipStreamWaitEvent(st, ev_shadow, 0);  // the previous bounce's shadow rays
                 k_bb_shade<<<(int)((paths + kShadeBlock - 1) / kShadeBlock), kShadeBlock, 0, st>>>(br.dtab, ws, cur, d,
                                                                                                   (int)P);
-                k_bb_shadow_total<<<1, 64, 0, st>>>(ws.counters, cur);
-                k_bb_shadow_persistent<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(br.dtab, ws, cur,
+                if (e == hipSuccess) e = hipEventRecord(ev_shade, st);
+                if (e == hipSuccess) e = hipStreamWaitEvent(sd, ev_shade, 0);
+                k_bb_shadow_total<<<1, 64, 0, sd>>>(ws.counters, cur);
+                k_bb_shadow_persistent<<<std::min(grid, persistent_blocks), kBlock, lds, sd>>>(br.dtab, ws, cur,
                                                                                                 refill);
+                if (e == hipSuccess) e = hipEventRecord(ev_shadow, sd);
                 launches++;
                 cur = 1 - cur;
             }
+            if (e == hipSuccess) e = hipStreamWaitEvent(st, ev_shadow, 0);
             k_bb_finalize<<<blocks_for(P), kBlock, 0, st>>>(br.dtab, n, (int)P, cnt, ws.smp);
         }
-        e = hipGetLastError();
+        if (e == hipSuccess) e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(e1, st);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -731,8 +745,9 @@ int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
     if (e == hipSuccess) e = hipMemcpy(&tot[0], ws.counters + C_EXT, sizeof(uint64_t), hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(&tot[1], ws.counters + C_SHD, sizeof(uint64_t), hipMemcpyDeviceToHost);
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) (void)hipStreamSynchronize(sd);
+    for (hipEvent_t x : {e0, e1, ev_shade, ev_shadow})
+        if (x) (void)hipEventDestroy(x);
     if (e != hipSuccess) {
         (void)hipStreamSynchronize(st);
         return set_error(MCPT_ERR_HIP, "batched render: %s", hipGetErrorString(e));
