@@ -254,6 +254,13 @@ __global__ __launch_bounds__(kBlock) void k_b_levels(const Seg* seg, int nseg, i
     }
 }
 
+// scene (segment) of each node of ids[]
+__global__ __launch_bounds__(kBlock) void k_b_node_scene(const Seg* seg, int nseg, const int* ids, int n,
+                                                        uint32_t* key) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) key[i] = (uint32_t)seg_of_node(seg, nseg, ids[i]);
+}
+
 // one wide node per even-level binary node of every scene (global index q)
 __global__ __launch_bounds__(kBlock) void k_b_collapse(const Seg* seg, int nseg, const uint32_t* level,
                                                       const int* widx, int n) {
@@ -893,6 +900,8 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
     if (e == hipSuccess) e = hipEventRecord(e0, st);
     uint32_t* lvl_keys = nullptr;  // level of each global internal node
     int* widx = nullptr;           // its wide index (after the level sort)
+    uint32_t* free_keys = nullptr;  // Morton sort buffers, free after the Karras tree
+    int* free_vals = nullptr;
     if (e == hipSuccess) {
         const int n_all = (int)total, n_int = (int)total_int;
         k_b_bounds_chunks<<<n_chunks, kBlock, 0, st>>>(dseg, ns, dchunk, partial);
@@ -931,6 +940,8 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
         k_level_offsets<<<1, 128, 0, st>>>(counts, itiles, n_int, depth + ns);
         lvl_keys = kout;
         widx = vout;
+        free_keys = kin;
+        free_vals = vin;
         e = hipGetLastError();
     }
     std::vector<int> depths(2 * ns + 65, 0);
@@ -949,11 +960,26 @@ int mcpt_lbvh_build_batch(const mcpt_scene* scenes, int32_t n, float* build_ms) 
             if (off[L + 1] > off[L])
                 k_b_refit_level<<<blocks_for(off[L + 1] - off[L]), kBlock, 0, st>>>(dseg, ns, ids_sorted + off[L],
                                                                                     off[L + 1] - off[L]);
-        // wide indices of the even-level nodes (in the level sort's value
-        // buffer, free once the sort is done), then the collapse
+        // wide indices: each scene's even-level nodes in (level, index)
+        // order, as the single-scene build numbers them (a stable sort of
+        // the level-sorted nodes by scene, then ranks), into the level
+        // sort's value buffer; then the collapse
         if (e == hipSuccess) {
-            k_wide_flags<<<blocks_for(n_int), kBlock, 0, st>>>(lvl_keys, n_int, widx);
-            scan_exclusive(widx, n_int, ssum, st);
+            const int itiles = blocks_for(n_int, kTile);
+            k_b_node_scene<<<blocks_for(n_int), kBlock, 0, st>>>(dseg, ns, ids_sorted, n_int, free_keys);
+            uint32_t *ka = free_keys, *kb = lvl_sorted;
+            int *va = ids_sorted, *vb = free_vals;
+            for (int shift = 0; (ns - 1) >> shift; shift += 8) {
+                k_radix_hist<<<itiles, kBlock, 0, st>>>(ka, n_int, shift, itiles, counts);
+                scan_exclusive(counts, 256 * itiles, ssum, st);
+                k_radix_scatter<<<itiles, kBlock, 0, st>>>(ka, va, kb, vb, n_int, shift, itiles, counts);
+                std::swap(ka, kb);
+                std::swap(va, vb);
+            }
+            int* rank = reinterpret_cast<int*>(ka);  // the scene keys are done with
+            k_wide_flags<<<blocks_for(n_int), kBlock, 0, st>>>(va, lvl_keys, n_int, rank);
+            scan_exclusive(rank, n_int, ssum, st);
+            k_wide_scatter<<<blocks_for(n_int), kBlock, 0, st>>>(va, lvl_keys, rank, n_int, widx);
             k_b_collapse<<<blocks_for(n_int), kBlock, 0, st>>>(dseg, ns, lvl_keys, widx, n_int);
             e = hipGetLastError();
         }

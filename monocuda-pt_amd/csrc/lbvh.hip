@@ -368,13 +368,17 @@ int lbvh_build(mcpt_scene_s* s, mcpt_lbvh_dump* dump) {
                                                                                   ids_sorted.ptr + off[L],
                                                                                   off[L + 1] - off[L],
                                                                                   s->root_box.ptr);
-        // 6. four-wide tree: wide indices of the even-level nodes (into vout,
-        // free after the level sort), then one wide node per such node
+        // 6. four-wide tree: wide indices of the even-level nodes in the
+        // level sort's (level, index) order (ranks in lvl_sorted, free after
+        // the offsets, indices into vout, free after the sort), then one wide
+        // node per such node
         int n_wide = 0;
         for (int L = 0; L < 64; L += 2) n_wide += off[L + 1] - off[L];
         s->n_wide = n_wide;
-        k_wide_flags<<<blocks_for(ni), kBlock, 0, st>>>(kout, ni, vout);
-        scan_exclusive(vout, ni, ssum.ptr, st);
+        int* rank = reinterpret_cast<int*>(lvl_sorted.ptr);
+        k_wide_flags<<<blocks_for(ni), kBlock, 0, st>>>(ids_sorted.ptr, kout, ni, rank);
+        scan_exclusive(rank, ni, ssum.ptr, st);
+        k_wide_scatter<<<blocks_for(ni), kBlock, 0, st>>>(ids_sorted.ptr, kout, rank, ni, vout);
         k_collapse<<<blocks_for(ni), kBlock, 0, st>>>(s->nodes.ptr, kout, vout, ni, s->wide.ptr);
         HSTEP(hipGetLastError());
     }

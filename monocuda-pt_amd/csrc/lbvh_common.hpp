@@ -192,10 +192,26 @@ MCPT_HDI void wide_encode(const float (*b)[6], const int* ref, int cnt, WideNode
     out->child = make_int4(ref[0], cnt > 1 ? ref[1] : 0, cnt > 2 ? ref[2] : 0, cnt > 3 ? ref[3] : 0);
 }
 
-// wide flags: 1 for even-level nodes (scanned in place into wide indices)
-__global__ __launch_bounds__(kBlock) void k_wide_flags(const uint32_t* level, int n, int* flag) {
+// Wide node numbering. Wide node w is the w-th even-level binary node in
+// (level, binary index) order. In a Karras tree every internal node inside
+// a node's key range is its descendant or ancestor (ranges nest), so the
+// binary nodes two levels below node p whose indices fall in p's range are
+// exactly p's grandchildren: in (level, index) order they are consecutive,
+// in slot order. Each wide node's inner children therefore have consecutive
+// wide indices, and the walk finds a pending child as base + rank instead
+// of re-reading the parent's references.
+//
+// flag[i] = 1 when node ids[i] (of an array in that order) is at an even
+// level; scanned in place, it becomes the wide index
+__global__ __launch_bounds__(kBlock) void k_wide_flags(const int* ids, const uint32_t* level, int n, int* flag) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) flag[i] = (level[i] & 1u) ? 0 : 1;
+    if (i < n) flag[i] = (level[ids[i]] & 1u) ? 0 : 1;
+}
+// widx[ids[i]] = rank[i] for the even-level nodes
+__global__ __launch_bounds__(kBlock) void k_wide_scatter(const int* ids, const uint32_t* level, const int* rank,
+                                                         int n, int* widx) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n && !(level[ids[i]] & 1u)) widx[ids[i]] = rank[i];
 }
 
 // The wide node of binary node p (at an even level): child boxes come from

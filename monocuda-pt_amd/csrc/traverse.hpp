@@ -218,26 +218,36 @@ MCPT_DEV void test_leaf_ranked(const BvhView& bv, Trav& s, int leaf) {
 
 
 // The wide walk's stack holds one group per wide level: two ints, the wide
-// node and the slots of its children still to visit, in visit order (bits
-// 0-5, two bits each) with their count (bits 8-9). Popping re-reads only
-// the node's 16-byte child references. Height: two ints per wide level,
-// i.e. the binary depth rounded up to even.
-MCPT_DEV void push_group(int* stk, int stride, Trav& s, int node, uint32_t seq) {
-    stk[(2 * s.sp) * stride] = node;
+// index of the node's first inner child and the ranks (among its inner
+// children, in slot order) of the children still to visit, in visit order
+// (bits 0-5, two bits each) with their count (bits 8-9). A node's inner
+// children have consecutive wide indices (the build numbers wide nodes in
+// level, then binary index order), so a pop reads no memory. Height: two
+// ints per wide level, i.e. the binary depth rounded up to even.
+MCPT_DEV void push_group(int* stk, int stride, Trav& s, int base, uint32_t seq) {
+    stk[(2 * s.sp) * stride] = base;
     stk[(2 * s.sp + 1) * stride] = (int)seq;
     s.sp++;
 }
 
+// rank of slot k among the inner slots of `inner` (bit j set: slot j inner)
+MCPT_DEV uint32_t inner_rank(uint32_t inner, int k) { return (uint32_t)__builtin_popcount(inner & ((1u << k) - 1u)); }
+
+// every inner slot of a node, hit or not (the ranks count them all): the
+// used slots (count in the top byte of the origin word) that hold no leaf
+MCPT_DEV uint32_t node_inner(const float4& o, uint32_t leaf_mask) {
+    const uint32_t cnt = __float_as_uint(o.w) >> 24;
+    return ~leaf_mask & ((1u << cnt) - 1u);
+}
+
 // Moves to the next child of the top group; false when the stack is empty.
-MCPT_DEV bool pop_group(const BvhView& bv, int* stk, int stride, Trav& s) {
+MCPT_DEV bool pop_group(int* stk, int stride, Trav& s) {
     if (s.sp == 0) return false;
     const int top = s.sp - 1;
-    const int w = stk[(2 * top) * stride];
+    const int base = stk[(2 * top) * stride];
     const uint32_t seq = (uint32_t)stk[(2 * top + 1) * stride];
-    const uint32_t k = seq & 3u, cnt = seq >> 8;
-    const int4 ch = bv.wide[w].child;
-    const int c[4] = {ch.x, ch.y, ch.z, ch.w};
-    s.node = pick4(c, (int)k);
+    const uint32_t cnt = seq >> 8;
+    s.node = base + (int)(seq & 3u);
     if (cnt > 1) stk[(2 * top + 1) * stride] = (int)(((seq & 0x3Fu) >> 2) | ((cnt - 1) << 8));
     else s.sp = top;
     return true;
@@ -280,7 +290,7 @@ MCPT_DEV bool step_wide(const BvhView& bv, Trav& s, int* stk, int stride, uint32
         if (COUNT) nl++;
         test_leaf_ranked(bv, s, ~pick4(c, j));
     }
-    if (!inner) return !pop_group(bv, stk, stride, s);
+    if (!inner) return !pop_group(stk, stride, s);
     // order the inner children by entry distance (non-inner slots sort last)
     cswap(d[0], k[0], d[1], k[1]);
     cswap(d[2], k[2], d[3], k[3]);
@@ -288,7 +298,11 @@ MCPT_DEV bool step_wide(const BvhView& bv, Trav& s, int* stk, int stride, uint32
     cswap(d[1], k[1], d[3], k[3]);
     cswap(d[1], k[1], d[2], k[2]);
     const uint32_t n = __builtin_popcount(inner);
-    if (n > 1) push_group(stk, stride, s, s.node, (uint32_t)k[1] | (uint32_t)k[2] << 2 | (uint32_t)k[3] << 4 | (n - 1) << 8);
+    if (n > 1) {
+        const uint32_t all = node_inner(o, lm);
+        push_group(stk, stride, s, pick4(c, __builtin_ctz(all)),
+                   inner_rank(all, k[1]) | inner_rank(all, k[2]) << 2 | inner_rank(all, k[3]) << 4 | (n - 1) << 8);
+    }
     s.node = pick4(c, k[0]);
     return false;
 }
@@ -318,14 +332,18 @@ MCPT_DEV bool step_wide_defer(const BvhView& bv, Trav& s, int* stk, int stride, 
     for (int j = 0; j < 4; j++) d[j] = ((inner >> j) & 1u) ? tn[j] : __builtin_inff();
     s.pend = leaves;
     s.pend_ch = ch;
-    if (!inner) return !pop_group(bv, stk, stride, s);
+    if (!inner) return !pop_group(stk, stride, s);
     cswap(d[0], k[0], d[1], k[1]);
     cswap(d[2], k[2], d[3], k[3]);
     cswap(d[0], k[0], d[2], k[2]);
     cswap(d[1], k[1], d[3], k[3]);
     cswap(d[1], k[1], d[2], k[2]);
     const uint32_t n = __builtin_popcount(inner);
-    if (n > 1) push_group(stk, stride, s, s.node, (uint32_t)k[1] | (uint32_t)k[2] << 2 | (uint32_t)k[3] << 4 | (n - 1) << 8);
+    if (n > 1) {
+        const uint32_t all = node_inner(o, lm);
+        push_group(stk, stride, s, pick4(c, __builtin_ctz(all)),
+                   inner_rank(all, k[1]) | inner_rank(all, k[2]) << 2 | inner_rank(all, k[3]) << 4 | (n - 1) << 8);
+    }
     s.node = pick4(c, k[0]);
     return false;
 }
@@ -360,18 +378,19 @@ MCPT_DEV int step_wide_any(const BvhView& bv, Trav& s, int* stk, int stride) {
         leaves &= leaves - 1u;
         if (test_leaf_any(bv, s, ~pick4(c, j))) return 2;
     }
-    if (!inner) return pop_group(bv, stk, stride, s) ? 0 : 1;
+    if (!inner) return pop_group(stk, stride, s) ? 0 : 1;
     // the leftmost inner child next, the others as one group, left to right
     const int first = __builtin_ctz(inner);
     uint32_t rest = inner & (inner - 1u);
     const uint32_t n = __builtin_popcount(rest);
     if (n) {
+        const uint32_t all = node_inner(o, lm);
         uint32_t seq = 0;
         for (int i = 0; i < 3 && rest; i++) {
-            seq |= (uint32_t)__builtin_ctz(rest) << (2 * i);
+            seq |= inner_rank(all, __builtin_ctz(rest)) << (2 * i);
             rest &= rest - 1u;
         }
-        push_group(stk, stride, s, s.node, seq | n << 8);
+        push_group(stk, stride, s, pick4(c, __builtin_ctz(all)), seq | n << 8);
     }
     s.node = pick4(c, first);
     return 0;

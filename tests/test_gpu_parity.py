@@ -92,7 +92,9 @@ def _box(b):
 
 def _check_wide(mcpt, scene, ref):
     """The four-wide tree against the oracle's binary tree: wide node i is
-    the i-th even-level binary node, its children are the binary node's leaf
+    the i-th even-level binary node in (level, index) order, so every wide
+    node's inner children have consecutive indices in slot order; its
+    children are the binary node's leaf
     children and its inner children's children in left-to-right order, every
     decoded box contains the exact one and exceeds it by less than one
     quantization step per side, the scale per axis is the smallest power of
@@ -116,6 +118,7 @@ def _check_wide(mcpt, scene, ref):
                 level[c] = level[p] + 1
                 stack.append(c)
     even = np.flatnonzero(level % 2 == 0)
+    even = even[np.lexsort((even, level[even]))]
     assert len(nodes) == len(even)
     widx = np.full(ni, -1)
     widx[even] = np.arange(len(even))
@@ -132,6 +135,10 @@ def _check_wide(mcpt, scene, ref):
     assert np.array_equal(nodes["meta"] >> 24, cnt)
     valid = np.arange(4)[None, :] < cnt[:, None]
     assert np.array_equal(np.where(valid, nodes["child"], 0), np.where(valid, exp_ref, 0))
+    # inner children consecutive, in slot order (the walk's pops rely on it)
+    for i in range(len(even)):
+        inner = [int(r) for r in nodes["child"][i, :cnt[i]] if r >= 0]
+        assert inner == list(range(inner[0], inner[0] + len(inner))) if inner else True, i
     # every leaf once, every wide node but the root once
     refs = nodes["child"][valid]
     assert np.array_equal(np.sort(~refs[refs < 0]), np.arange(n))
