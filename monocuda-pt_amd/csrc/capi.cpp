@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <vector>
 #include <exception>
 #include <string>
 
@@ -44,24 +45,41 @@ int dev_alloc(Ctx* ctx, void** p, size_t bytes) {
     return MCPT_OK;
 }
 
+// the context's streams and events but `stream`, as pointers
+static std::vector<hipStream_t*> ctx_streams(Ctx* c) {
+    std::vector<hipStream_t*> v = {&c->side};
+    for (int l = 1; l < Ctx::kMaxLanes; l++) {
+        v.push_back(&c->lane_stream[l]);
+        v.push_back(&c->lane_side[l]);
+    }
+    return v;
+}
+static std::vector<hipEvent_t*> ctx_events(Ctx* c) {
+    std::vector<hipEvent_t*> v = {&c->ev_join};
+    for (int l = 0; l < Ctx::kMaxLanes; l++) {
+        v.push_back(&c->ev_shade[l]);
+        v.push_back(&c->ev_shadow[l]);
+        v.push_back(&c->ev_fin[l]);
+    }
+    return v;
+}
+
 int ctx_sync_objects_create(Ctx* c) {
-    for (hipStream_t* sp : {&c->side, &c->lane_stream, &c->lane_side})
+    for (hipStream_t* sp : ctx_streams(c))
         if (hipStreamCreateWithFlags(sp, hipStreamNonBlocking) != hipSuccess) return MCPT_ERR_HIP;
-    for (hipEvent_t* ep : {&c->ev_shade[0], &c->ev_shade[1], &c->ev_shadow[0], &c->ev_shadow[1], &c->ev_fin[0],
-                           &c->ev_fin[1], &c->ev_join})
+    for (hipEvent_t* ep : ctx_events(c))
         if (hipEventCreateWithFlags(ep, hipEventDisableTiming) != hipSuccess) return MCPT_ERR_HIP;
     return MCPT_OK;
 }
 
 void ctx_sync_objects_destroy(Ctx* c) {
-    for (hipStream_t* sp : {&c->side, &c->lane_stream, &c->lane_side})
+    for (hipStream_t* sp : ctx_streams(c))
         if (*sp) {
             (void)hipStreamSynchronize(*sp);
             (void)hipStreamDestroy(*sp);
             *sp = nullptr;
         }
-    for (hipEvent_t* ep : {&c->ev_shade[0], &c->ev_shade[1], &c->ev_shadow[0], &c->ev_shadow[1], &c->ev_fin[0],
-                           &c->ev_fin[1], &c->ev_join})
+    for (hipEvent_t* ep : ctx_events(c))
         if (*ep) {
             (void)hipEventDestroy(*ep);
             *ep = nullptr;

@@ -33,15 +33,16 @@ void clear_error();
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // Streams (default priority) and events of a render's two lanes. Lane 0
-    // runs on `stream` with shadow launches on `side`, lane 1 on lane_stream
-    // and lane_side; ev_shade / ev_shadow order a lane's shade and shadow
-    // launches, ev_fin the lanes' film updates, ev_join lane 1's end with
-    // `stream`.
+    // Streams (default priority) and events of a render's lanes. Lane 0 runs
+    // on `stream` with shadow launches on `side`, lane l > 0 on
+    // lane_stream[l] and lane_side[l]; ev_shade / ev_shadow order a lane's
+    // shade and shadow launches, ev_fin the lanes' film updates, ev_join the
+    // lanes' start after `stream` and `stream`'s wait for the last lane.
+    static constexpr int kMaxLanes = 2;
     hipStream_t side = nullptr;
-    hipStream_t lane_stream = nullptr, lane_side = nullptr;
-    hipEvent_t ev_shade[2] = {nullptr, nullptr}, ev_shadow[2] = {nullptr, nullptr};
-    hipEvent_t ev_fin[2] = {nullptr, nullptr}, ev_join = nullptr;
+    hipStream_t lane_stream[kMaxLanes] = {}, lane_side[kMaxLanes] = {};
+    hipEvent_t ev_shade[kMaxLanes] = {}, ev_shadow[kMaxLanes] = {};
+    hipEvent_t ev_fin[kMaxLanes] = {}, ev_join = nullptr;
     std::mutex alloc_mu;  // guards `used` (a server thread allocates too)
     uint64_t budget = 0;
     uint64_t used = 0;

@@ -255,6 +255,15 @@ __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Work
 // persistent launches, where small blocks fill the gaps (1024 -> 256 cost
 // 8% in a serial launch and gained 2-6% in the overlapped render).
 constexpr int kShadeBlock = 256;
+// Lanes of an untimed wavefront render (enqueue_render), and whether each
+// lane's shadow launches run on a stream of their own. Two lanes with their
+// shadow streams use the four hardware queues a process gets; measured on
+// the 1 M-triangle mesh at 8 / 20 / 32 / 64 spp: 1,001 / 1,273 / 1,370 /
+// 1,472 Mrays/s, against 768 / 1,077 / 1,188 / 1,334 without the shadow
+// streams and at most 824 / 1,150 / 1,286 / 1,417 with three or four lanes
+// (with or without them), whose streams share hardware queues.
+constexpr int kRenderLanes = 2;
+constexpr bool kLaneSideStreams = true;
 __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView sv, Workspace ws, int cur, int depth, int max_depth,
                                                        vec3 bg, float rr_cap, int npix) {
     __shared__ int lds[bin_sort_lds_ints<kShadeBlock>()];
@@ -680,8 +689,9 @@ int check_render_args(const mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_
 //     shadow rays add to the states of the next queue, which only the next
 //     shade reads), so one launch's last slow rays overlap the other's start;
 //     the next shade waits for them;
-//   * with two lanes, consecutive passes alternate between two workspaces
-//     and stream pairs, so one pass's launches fill the tails of the other's.
+//   * with kRenderLanes lanes, consecutive passes go round the lanes'
+//     workspaces and stream pairs, so one pass's launches fill the tails of
+//     the others', each lane's persistent launches on its share of the grid.
 //     Passes add to the film in pass order (each finalize waits for the
 //     previous one), so the film is the same bit for bit.
 // With `tm` every launch runs in order on one stream, so each kernel's span
@@ -691,8 +701,12 @@ int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
     Ctx* ctx = s->ctx;
     const bool overlap = tm == nullptr;
     if (!overlap) nlanes = 1;
-    hipStream_t mains[2] = {ctx->stream, ctx->lane_stream};
-    hipStream_t sides[2] = {overlap ? ctx->side : ctx->stream, ctx->lane_side};
+    nlanes = std::min(nlanes, (int)Ctx::kMaxLanes);
+    hipStream_t mains[Ctx::kMaxLanes], sides[Ctx::kMaxLanes];
+    for (int l = 0; l < Ctx::kMaxLanes; l++) {
+        mains[l] = l == 0 ? ctx->stream : ctx->lane_stream[l];
+        sides[l] = !overlap ? mains[l] : !kLaneSideStreams ? mains[l] : l == 0 ? ctx->side : ctx->lane_side[l];
+    }
     const int W = cam->width, H = cam->height, npix = W * H;
     const SceneView sv = s->view();
     const vec3 bg = load3(cam->background);
@@ -704,9 +718,9 @@ int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
     auto mark = [&](int k) { return tm ? tm->mark(k) : MCPT_OK; };
 
     for (int l = 0; l < nlanes; l++) k_counters_zero_totals<<<1, C_NCOUNTERS, 0, ctx->stream>>>(ws[l].counters);
-    if (nlanes > 1) {  // lane 1 starts after the clear above
+    if (nlanes > 1) {  // the other lanes start after the clear above
         MCPT_HIP(hipEventRecord(ctx->ev_join, ctx->stream));
-        MCPT_HIP(hipStreamWaitEvent(ctx->lane_stream, ctx->ev_join, 0));
+        for (int l = 1; l < nlanes; l++) MCPT_HIP(hipStreamWaitEvent(mains[l], ctx->ev_join, 0));
     }
     MCPT_TRY(mark(K_OTHER));
 
@@ -786,14 +800,15 @@ int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
             cur = 1 - cur;
         }
         if (overlap) MCPT_HIP(hipStreamWaitEvent(st, ev_shadow, 0));
-        if (nlanes > 1 && pass > 0) MCPT_HIP(hipStreamWaitEvent(st, ctx->ev_fin[1 - lane], 0));  // film order
+        if (nlanes > 1 && pass > 0)  // film order: after the previous pass's update
+            MCPT_HIP(hipStreamWaitEvent(st, ctx->ev_fin[(lane + nlanes - 1) % nlanes], 0));
         k_finalize<<<blocks_for(npix), kBlock, 0, st>>>(w, film->accum.ptr, npix, nb);
         if (nlanes > 1) MCPT_HIP(hipEventRecord(ctx->ev_fin[lane], st));
         MCPT_TRY(mark(K_FINALIZE));
     }
-    if (nlanes > 1) {  // the caller's stream ends after lane 1
-        MCPT_HIP(hipEventRecord(ctx->ev_join, ctx->lane_stream));
-        MCPT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+    if (nlanes > 1 && pass > 0) {  // the caller's stream ends after the last film update
+        const int last = (pass - 1) % nlanes;
+        if (last != 0) MCPT_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_fin[last], 0));
     }
     MCPT_HIP(hipGetLastError());
     *extend_launches_out = o->mode == MCPT_MODE_MEGAKERNEL ? (uint64_t)o->spp : extend_launches;
@@ -838,10 +853,11 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
     int batch = o->batch > 0 ? o->batch : (int)std::max<int64_t>(1, cap_paths / npix);
     batch = std::max(1, std::min(batch, o->spp));
     if ((int64_t)npix * batch > (int64_t(1) << 30)) batch = std::max(1, (1 << 30) / npix);
-    const int nlanes = (!timed && o->mode != MCPT_MODE_MEGAKERNEL && o->spp >= 2) ? 2 : 1;
-    if (nlanes == 2 && o->batch <= 0) batch = std::max(1, (std::min(batch, o->spp) + 1) / 2);
+    const int nlanes =
+        (!timed && o->mode != MCPT_MODE_MEGAKERNEL) ? std::max(1, std::min(kRenderLanes, o->spp)) : 1;
+    if (nlanes > 1 && o->batch <= 0) batch = std::max(1, (std::min(batch, o->spp) + nlanes - 1) / nlanes);
 
-    Workspace ws[2];
+    Workspace ws[Ctx::kMaxLanes];
     MCPT_TRY(ensure_workspace(ctx, o->mode == MCPT_MODE_MEGAKERNEL ? 1 : (int64_t)npix * batch, nlanes, ws));
     Timer tm(timed, st);
     hipEvent_t e0, e1;
@@ -851,7 +867,11 @@ int render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_opts* o, m
     uint64_t extend_launches = 0;
     int rc = enqueue_render(s, cam, o, film, ws, nlanes, batch, timed ? &tm : nullptr, &extend_launches);
     if (rc < 0) {
-        for (hipStream_t x : {ctx->side, ctx->lane_stream, ctx->lane_side, st}) (void)hipStreamSynchronize(x);
+        for (int l = 1; l < Ctx::kMaxLanes; l++) {
+            (void)hipStreamSynchronize(ctx->lane_stream[l]);
+            (void)hipStreamSynchronize(ctx->lane_side[l]);
+        }
+        for (hipStream_t x : {ctx->side, st}) (void)hipStreamSynchronize(x);
         hipEventDestroy(e0);
         hipEventDestroy(e1);
         return rc;
