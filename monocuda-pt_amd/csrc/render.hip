@@ -186,19 +186,18 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneView sv, Workspace ws, i
 // leaf tests are deferred and run for all lanes that have some pending once
 // kLeafBatch lanes do (or no lane has a node left to visit).
 constexpr int kLeafBatch = 16;
-template <bool COUNT, bool REF>
-__global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Workspace ws, int cur, int refill) {
-    extern __shared__ int lds_stack[];
-    int* stk = lds_stack + threadIdx.x;
-    const int total = queue_len(ws.counters, cur);
-    if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
+// The persistent closest-hit loop over `total` rays: lanes take rays by
+// index from *work (begin(q, s) starts ray q's walk) and hand each finished
+// walk to finish(q, s). Shared by the extend kernel and the trace hook.
+template <bool COUNT, bool REF, class Begin, class Finish>
+MCPT_DEV void persistent_closest(const BvhView& bv, int* work, int total, int refill, int* stk, Begin begin,
+                                 Finish finish, uint32_t& ns, uint32_t& nl) {
     const int lane = lane_id();
     Trav s;
     s.pend = 0;
     bool has = false, walked = false;  // walked: no node left, only pending leaves
     int slot = -1;
     bool exhausted = false;  // uniform across the wave
-    uint32_t ns = 0, nl = 0;
     while (true) {
         if (!exhausted) {
             const uint64_t idle = ballot64(!has);
@@ -206,14 +205,12 @@ __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Work
             if (nidle >= refill) {
                 const int leader = __builtin_ctzll(idle);
                 int first = 0;
-                if (lane == leader) first = atomicAdd(&ws.counters[C_WORK], nidle);
+                if (lane == leader) first = atomicAdd(work, nidle);
                 first = __shfl(first, leader);
                 if (!has) {
                     const int q = first + rank_below(idle);
                     if (q < total) {
-                        const RayItem it = ws.q[cur][q];
-                        trav_begin<REF>(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
-                        s.time = it.ot.w;
+                        begin(q, s);
                         slot = q;
                         has = true;
                     }
@@ -223,26 +220,43 @@ __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Work
         }
         if (exhausted && ballot64(has) == 0) break;
         if (REF) {
-            if (has && step_closest<COUNT>(sv.bvh, s, stk, kBlock, ns, nl)) {
-                ws.hits[slot] = pack_hit(sv.bvh, s);
+            if (has && step_closest<COUNT>(bv, s, stk, kBlock, ns, nl)) {
+                finish(slot, s);
                 has = false;
             }
             continue;
         }
         // node steps for lanes with nothing pending; leaf tests once enough
         // lanes have leaves pending, or when no lane has a node to visit
-        if (has && !s.pend && !walked) walked = step_wide_defer<COUNT>(sv.bvh, s, stk, kBlock, ns);
+        if (has && !s.pend && !walked) walked = step_wide_defer<COUNT>(bv, s, stk, kBlock, ns);
         const bool want = has && s.pend != 0;
         const uint64_t wm = ballot64(want);
         if (wm && (__popcll(wm) >= kLeafBatch || ballot64(has && !s.pend && !walked) == 0)) {
-            if (want) test_pending<COUNT>(sv.bvh, s, nl);
+            if (want) test_pending<COUNT>(bv, s, nl);
         }
         if (has && walked && !s.pend) {
-            ws.hits[slot] = pack_hit(sv.bvh, s);
+            finish(slot, s);
             has = false;
             walked = false;
         }
     }
+}
+
+template <bool COUNT, bool REF>
+__global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Workspace ws, int cur, int refill) {
+    extern __shared__ int lds_stack[];
+    const int total = queue_len(ws.counters, cur);
+    if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
+    uint32_t ns = 0, nl = 0;
+    const RayItem* q = ws.q[cur];
+    persistent_closest<COUNT, REF>(
+        sv.bvh, &ws.counters[C_WORK], total, refill, lds_stack + threadIdx.x,
+        [&](int i, Trav& s) {
+            const RayItem it = q[i];
+            trav_begin<REF>(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+            s.time = it.ot.w;
+        },
+        [&](int i, const Trav& s) { ws.hits[i] = pack_hit(sv.bvh, s); }, ns, nl);
     count_flush<COUNT>(ws.counters, ns, nl);
 }
 
@@ -559,6 +573,25 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(SceneView sv, const mc
         steps[i] = ns;
         leaves[i] = nl;
     }
+}
+// the hook without counts: the persistent near-first walk of the extend
+// kernel over the caller's rays (their tmax and time included)
+__global__ __launch_bounds__(kBlock) void k_trace_closest_persistent(SceneView sv, const mcpt_ray* rays, int n,
+                                                                     mcpt_packed_hit* hits, int* work, int refill) {
+    extern __shared__ int lds_stack[];
+    if ((int)blockIdx.x * kBlock >= n) return;
+    uint32_t ns = 0, nl = 0;
+    persistent_closest<false, false>(
+        sv.bvh, work, n, refill, lds_stack + threadIdx.x,
+        [&](int i, Trav& s) {
+            const mcpt_ray r = rays[i];
+            trav_begin(s, load3(r.origin), load3(r.dir), r.tmax);
+            s.time = r.time;
+        },
+        [&](int i, const Trav& s) { hits[i] = pack_hit(sv.bvh, s); }, ns, nl);
+}
+__global__ void k_zero_int(int* p) {
+    if (threadIdx.x == 0) *p = 0;
 }
 __global__ __launch_bounds__(kBlock) void k_trace_any(SceneView sv, const mcpt_ray* rays, int n, uint8_t* occ) {
     extern __shared__ int lds_stack[];
@@ -1009,9 +1042,11 @@ int trace_closest_host(mcpt_scene_s* s, const mcpt_ray* rays, int n, mcpt_packed
     DevBuf<mcpt_ray> dr;
     DevBuf<mcpt_packed_hit> dh;
     DevBuf<uint32_t> dcount;
+    DevBuf<int> dwork;
     int status = dr.alloc(ctx, n);
     if (status >= 0) status = dh.alloc(ctx, n);
     if (status >= 0 && count) status = dcount.alloc(ctx, (size_t)2 * n);
+    if (status >= 0 && !count) status = dwork.alloc(ctx, 1);
     hipError_t e = hipSuccess;
     if (status >= 0) {
         e = hipMemcpyAsync(dr.ptr, rays, sizeof(mcpt_ray) * n, hipMemcpyHostToDevice, st);
@@ -1021,10 +1056,15 @@ int trace_closest_host(mcpt_scene_s* s, const mcpt_ray* rays, int n, mcpt_packed
             if (count)
                 k_trace_closest<true, true><<<blocks_for(n), kBlock, stack_bytes(s), st>>>(
                     s->view(), dr.ptr, n, dh.ptr, dcount.ptr, dcount.ptr + n);
-            else
-                k_trace_closest<false, false><<<blocks_for(n), kBlock, stack_bytes(s), st>>>(
-                    s->view(), dr.ptr, n, dh.ptr, nullptr, nullptr);
-            e = hipGetLastError();
+            else {
+                int per_cu = 0;
+                e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_closest_persistent, kBlock,
+                                                                 stack_bytes(s));
+                k_zero_int<<<1, 64, 0, st>>>(dwork.ptr);
+                k_trace_closest_persistent<<<std::min(blocks_for(n), std::max(1, per_cu) * ctx->cu_count), kBlock,
+                                             stack_bytes(s), st>>>(s->view(), dr.ptr, n, dh.ptr, dwork.ptr, 32);
+            }
+            if (e == hipSuccess) e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipMemcpyAsync(hits, dh.ptr, sizeof(mcpt_packed_hit) * n, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess && count) e = hipMemcpyAsync(steps, dcount.ptr, 4 * (size_t)n, hipMemcpyDeviceToHost, st);
@@ -1035,6 +1075,7 @@ int trace_closest_host(mcpt_scene_s* s, const mcpt_ray* rays, int n, mcpt_packed
     dr.release(ctx);
     dh.release(ctx);
     dcount.release(ctx);
+    dwork.release(ctx);
     if (status < 0) return status;
     if (e != hipSuccess) return set_error(MCPT_ERR_HIP, "trace_closest: %s", hipGetErrorString(e));
     return MCPT_OK;
