@@ -590,18 +590,50 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest_persistent(SceneView s
         },
         [&](int i, const Trav& s) { hits[i] = pack_hit(sv.bvh, s); }, ns, nl);
 }
+// the any-hit hook on the persistent scheme of k_shadow_persistent
+__global__ __launch_bounds__(kBlock) void k_trace_any_persistent(SceneView sv, const mcpt_ray* rays, int n,
+                                                                 uint8_t* occ, int* work, int refill) {
+    extern __shared__ int lds_stack[];
+    int* stk = lds_stack + threadIdx.x;
+    if ((int)blockIdx.x * kBlock >= n) return;
+    const int lane = lane_id();
+    Trav s;
+    bool has = false, exhausted = false;
+    int slot = -1;
+    while (true) {
+        if (!exhausted) {
+            const uint64_t idle = ballot64(!has);
+            const int nidle = __popcll(idle);
+            if (nidle >= refill) {
+                const int leader = __builtin_ctzll(idle);
+                int first = 0;
+                if (lane == leader) first = atomicAdd(work, nidle);
+                first = __shfl(first, leader);
+                if (!has) {
+                    const int q = first + rank_below(idle);
+                    if (q < n) {
+                        const mcpt_ray r = rays[q];
+                        trav_begin(s, load3(r.origin), load3(r.dir), r.tmax);
+                        s.time = r.time;
+                        slot = q;
+                        has = true;
+                    }
+                }
+                if (first + nidle >= n) exhausted = true;
+            }
+        }
+        if (exhausted && ballot64(has) == 0) break;
+        if (has) {
+            const int st = step_wide_any(sv.bvh, s, stk, kBlock);
+            if (st != 0) {
+                occ[slot] = st == 2 ? 1 : 0;
+                has = false;
+            }
+        }
+    }
+}
 __global__ void k_zero_int(int* p) {
     if (threadIdx.x == 0) *p = 0;
-}
-__global__ __launch_bounds__(kBlock) void k_trace_any(SceneView sv, const mcpt_ray* rays, int n, uint8_t* occ) {
-    extern __shared__ int lds_stack[];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const mcpt_ray r = rays[i];
-    Trav s;
-    trav_begin(s, load3(r.origin), load3(r.dir), r.tmax);
-    s.time = r.time;
-    occ[i] = trace_any(sv.bvh, s, lds_stack + threadIdx.x, kBlock) ? 1 : 0;
 }
 
 int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
@@ -1089,20 +1121,27 @@ int trace_any_host(mcpt_scene_s* s, const mcpt_ray* rays, int n, uint8_t* occlud
     hipStream_t st = ctx->stream;
     DevBuf<mcpt_ray> dr;
     DevBuf<uint8_t> dout;
+    DevBuf<int> dwork;
     int status = dr.alloc(ctx, n);
     if (status >= 0) status = dout.alloc(ctx, n);
+    if (status >= 0) status = dwork.alloc(ctx, 1);
     hipError_t e = hipSuccess;
     if (status >= 0) {
         e = hipMemcpyAsync(dr.ptr, rays, sizeof(mcpt_ray) * n, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) {
-            k_trace_any<<<blocks_for(n), kBlock, stack_bytes(s), st>>>(s->view(), dr.ptr, n, dout.ptr);
-            e = hipGetLastError();
+            int per_cu = 0;
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_any_persistent, kBlock, stack_bytes(s));
+            k_zero_int<<<1, 64, 0, st>>>(dwork.ptr);
+            k_trace_any_persistent<<<std::min(blocks_for(n), std::max(1, per_cu) * ctx->cu_count), kBlock,
+                                     stack_bytes(s), st>>>(s->view(), dr.ptr, n, dout.ptr, dwork.ptr, 32);
+            if (e == hipSuccess) e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipMemcpyAsync(occluded, dout.ptr, n, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     }
     dr.release(ctx);
     dout.release(ctx);
+    dwork.release(ctx);
     if (status < 0) return status;
     if (e != hipSuccess) return set_error(MCPT_ERR_HIP, "trace_any: %s", hipGetErrorString(e));
     return MCPT_OK;
