@@ -30,7 +30,6 @@ namespace mcpt {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kTile = 64;                      // samples per block iteration
 constexpr int kIn = MCPT_MLP_INPUTS;           // 64
 constexpr int kW = MCPT_MLP_WIDTH;             // 64
 constexpr int kRow = 72;                       // bf16 row stride in LDS and in the weight image (144 B)
@@ -84,57 +83,6 @@ MCPT_DEV void sph2(const vec3& d, float* out) {
     const float z = fminf(fmaxf(d.z, -1.0f), 1.0f);
     out[0] = mf_acos(z) / MCPT_PI_F;
     out[1] = mf_atan2(d.y, d.x) / (2.0f * MCPT_PI_F) + 0.5f;
-}
-
-// ------------------------------------------------------------ MLP tiles
-// copy the weight image into LDS (16-byte pieces)
-MCPT_DEV void load_weights(const __bf16* wimg, __bf16* w) {
-    const uint4* src = reinterpret_cast<const uint4*>(wimg);
-    uint4* dst = reinterpret_cast<uint4*>(w);
-    for (int i = threadIdx.x; i < kImg / 8; i += kBlock) dst[i] = src[i];
-}
-
-// the 64 encoded inputs of the tile's samples; thread t writes inputs
-// 16 (t % 4) .. +15 of sample t / 4
-MCPT_DEV void encode_tile(const float* feat, bool valid, __bf16* x) {
-    const int s = threadIdx.x >> 2, q = threadIdx.x & 3;
-    float f[MCPT_MLP_FEATURES];
-    for (int i = 0; i < MCPT_MLP_FEATURES; i++) f[i] = valid ? feat[i] : 0.0f;
-    for (int j = 0; j < 16; j++) {
-        const int k = 16 * q + j;
-        x[s * kRow + k] = (__bf16)(valid ? enc_feature(k, f) : 0.0f);
-    }
-}
-
-// hidden layer: xout = relu(W xin), W rows 16w.. of this wave
-MCPT_DEV void hidden_forward(const __bf16* W, const __bf16* xin, __bf16* xout) {
-    const int lane = lane_id(), wave = threadIdx.x >> 6;
-    const int r = lane & 15, g = lane >> 4;
-    f32x4 acc[4];
-    for (int nt = 0; nt < 4; nt++) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int kc = 0; kc < 2; kc++) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(W + (16 * wave + r) * kRow + 32 * kc + 8 * g);
-        for (int nt = 0; nt < 4; nt++) {
-            const bf16x8 b = *reinterpret_cast<const bf16x8*>(xin + (16 * nt + r) * kRow + 32 * kc + 8 * g);
-            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nt], 0, 0, 0);
-        }
-    }
-    for (int nt = 0; nt < 4; nt++)
-        for (int e = 0; e < 4; e++)
-            xout[(16 * nt + r) * kRow + 16 * wave + 4 * g + e] = (__bf16)fmaxf(acc[nt][e], 0.0f);
-}
-
-// output layer for samples 16w..16w+15: lanes with g == 0 get y[0..2]
-MCPT_DEV f32x4 output_forward(const __bf16* W4, const __bf16* xin) {
-    const int lane = lane_id(), wave = threadIdx.x >> 6;
-    const int r = lane & 15, g = lane >> 4;
-    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int kc = 0; kc < 2; kc++) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(W4 + r * kRow + 32 * kc + 8 * g);
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(xin + (16 * wave + r) * kRow + 32 * kc + 8 * g);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
-    }
-    return acc;
 }
 
 // ------------------------------------------------------------- kernels
@@ -263,137 +211,193 @@ __global__ __launch_bounds__(kInferBlock) void k_mlp_infer(const __bf16* wimg, c
     }
 }
 
-// Forward + backward over samples [first, first + n) of (feat, targ),
-// tiles block-strided; per-block weight gradients and loss sums.
-__global__ __launch_bounds__(kBlock) void k_mlp_grad(const __bf16* wimg, const float* feat, const float* targ, int first,
-                                                    int n, int n_total_for_mean, float* partial, float* partial_loss) {
-    __shared__ __attribute__((aligned(16))) __bf16 w[kImg];
-    __shared__ __attribute__((aligned(16))) __bf16 x[MCPT_MLP_HIDDEN + 1][kTile * kRow];
-    __shared__ __attribute__((aligned(16))) __bf16 dy[2][kTile * kRow];
-    __shared__ float lsum[kBlock / 64];
-    load_weights(wimg, w);
-    const int lane = lane_id(), wave = threadIdx.x >> 6;
-    const int r = lane & 15, g = lane >> 4;
+// Forward + backward over samples [first, first + n) of (feat, targ), tiles
+// of kGradTile samples block-strided; per-block weight gradients and loss
+// sums. A wave owns 16 samples of a tile and runs the forward pass the way
+// k_mlp_infer does, keeping every layer's input in registers (the MFMA
+// operand layout). The input gradients run backwards in registers the same
+// way, with the transposed weights as A operands (wb: W_l^T with the
+// permuted columns, w4b: W4^T). A weight gradient dW_l = sum over samples of
+// dZ_l^T X_l needs the samples along K, so each layer's dZ_l and X_l are
+// written transposed into LDS (one row per unit, one column per sample of
+// the tile) and the block's waves split dW_l's 16x16 tiles among
+// themselves, accumulating in registers across tiles. bfloat16 roundings:
+// every activation, dY, and every masked input gradient, as the oracle.
+constexpr int kGradWaves = 8;
+constexpr int kGradBlock = 64 * kGradWaves;
+constexpr int kGradTile = 16 * kGradWaves;  // samples per block iteration
+constexpr int kTRow = kGradTile + 8;        // bf16 row stride of the transposed tiles
+constexpr int kW4Row = 40;                  // W4^T: 32 columns (outputs, zero-padded) + pad
+static_assert(kGradTile % 32 == 0, "dW takes samples in K chunks of 32");
+
+// the row (unit) that element e of chunk kc of lane group g carries in the
+// B-operand registers of layer l's input
+MCPT_HDI int operand_row(int l, int kc, int g, int e) {
+    const int k = 32 * kc + 8 * g + e;
+    return l == 0 ? infer_in0(k) : infer_inh(k);
+}
+
+__global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const __bf16* wimg, const float* feat, const float* targ,
+                                                        int first, int n, int n_total_for_mean, float* partial,
+                                                        float* partial_loss) {
+    __shared__ __attribute__((aligned(16))) __bf16 wf[kImg];                          // forward, infer layout
+    __shared__ __attribute__((aligned(16))) __bf16 wb[MCPT_MLP_HIDDEN - 1][kW * kRow];  // W_l^T, l = 1..3
+    __shared__ __attribute__((aligned(16))) __bf16 w4b[kW * kW4Row];                   // W4^T
+    __shared__ __attribute__((aligned(16))) __bf16 xt[2][kW * kTRow];                  // X_l^T
+    __shared__ __attribute__((aligned(16))) __bf16 dt[2][kW * kTRow];                  // dZ_l^T
+    __shared__ float lsum[kGradWaves];
+    load_weights_infer(wimg, wf);
+    for (int i = threadIdx.x; i < (MCPT_MLP_HIDDEN - 1) * kW * kW; i += kGradBlock) {
+        const int l = 1 + i / (kW * kW), in = (i / kW) % kW, k = i % kW;
+        wb[l - 1][in * kRow + k] = wimg[img_row(l, infer_inh(k)) * kRow + in];
+    }
+    for (int i = threadIdx.x; i < kW * 32; i += kGradBlock) {
+        const int in = i / 32, o = i % 32;
+        w4b[in * kW4Row + o] = o < kOutRows ? wimg[img_row(MCPT_MLP_HIDDEN, o) * kRow + in] : (__bf16)0.0f;
+    }
+    __syncthreads();
+    const int lane = lane_id(), wave = (int)threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    const int col = 16 * wave + r;  // this lane's sample column in the transposed tiles
     const float inv_count = 1.0f / float(3 * n_total_for_mean);
-    f32x4 gh[MCPT_MLP_HIDDEN][4];  // hidden dW: rows 16w+4g+e, columns 16nt+r
-    for (int l = 0; l < MCPT_MLP_HIDDEN; l++)
-        for (int nt = 0; nt < 4; nt++) gh[l][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-    f32x4 go = f32x4{0.f, 0.f, 0.f, 0.f};  // output dW: rows 4g+e, columns 16w+r
+    // dW tiles of this wave: hidden layers, output rows 16 ob.., inputs 16 nt0.., 16 (nt0 + 1)..
+    const int ob = wave & 3, nt0 = 2 * (wave >> 2);
+    f32x4 gh[MCPT_MLP_HIDDEN][2];
+    for (int l = 0; l < MCPT_MLP_HIDDEN; l++) gh[l][0] = gh[l][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    f32x4 go = f32x4{0.f, 0.f, 0.f, 0.f};  // output layer (waves 0..3): outputs 4g + e, inputs 16 wave + r
     float loss = 0.0f;
-    const __bf16* W4 = w + img_row(MCPT_MLP_HIDDEN, 0) * kRow;
-    const int ntiles = (n + kTile - 1) / kTile;
+    const int ntiles = (n + kGradTile - 1) / kGradTile;
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int s = tile * kTile + (threadIdx.x >> 2);
-        __syncthreads();
-        encode_tile(feat + (size_t)(first + min(s, n - 1)) * MCPT_MLP_FEATURES, s < n, x[0]);
-        __syncthreads();
-        for (int l = 0; l < MCPT_MLP_HIDDEN; l++) {
-            hidden_forward(w + img_row(l, 0) * kRow, x[l], x[l + 1]);
-            __syncthreads();
+        const int s = tile * kGradTile + col;
+        const bool valid = s < n;
+        // ---- forward, inputs of every layer kept (act[l] = input of layer l)
+        bf16x8 act[MCPT_MLP_HIDDEN + 1][2];
+        {
+            float f[MCPT_MLP_FEATURES];
+            const float* fs = feat + (size_t)(first + min(s, n - 1)) * MCPT_MLP_FEATURES;
+            for (int i = 0; i < MCPT_MLP_FEATURES; i++) f[i] = valid ? fs[i] : 0.0f;
+            encode_lane(f, g, act[0][0], act[0][1]);
         }
-        // output, loss and its gradient dy (columns 0..2 of dy[0], 3..31 zero)
-        const f32x4 y = output_forward(W4, x[MCPT_MLP_HIDDEN]);
-        const int so = tile * kTile + 16 * wave + r;
+#pragma unroll
+        for (int l = 0; l < MCPT_MLP_HIDDEN; l++) {
+            const __bf16* Wl = wf + img_row(l, 0) * kRow;
+            f32x4 d[4];
+            for (int i = 0; i < 4; i++) {
+                const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(Wl + (16 * i + r) * kRow + 8 * g);
+                const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(Wl + (16 * i + r) * kRow + 32 + 8 * g);
+                d[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, act[l][0], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                d[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, act[l][1], d[i], 0, 0, 0);
+            }
+            for (int e = 0; e < 4; e++) {
+                act[l + 1][0][e] = (__bf16)fmaxf(d[0][e], 0.0f);
+                act[l + 1][0][4 + e] = (__bf16)fmaxf(d[1][e], 0.0f);
+                act[l + 1][1][e] = (__bf16)fmaxf(d[2][e], 0.0f);
+                act[l + 1][1][4 + e] = (__bf16)fmaxf(d[3][e], 0.0f);
+            }
+        }
+        const __bf16* W4 = wf + img_row(MCPT_MLP_HIDDEN, 0) * kRow;
+        f32x4 y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8*>(W4 + r * kRow + 8 * g),
+                                                         act[MCPT_MLP_HIDDEN][0], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0,
+                                                         0);
+        y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8*>(W4 + r * kRow + 32 + 8 * g),
+                                                   act[MCPT_MLP_HIDDEN][1], y, 0, 0, 0);
+        // ---- loss and dY (lanes of group 0 hold outputs 0..3 of sample r)
+        bf16x8 dz0, dz1;  // the gradient at the current layer's output, B layout
+        for (int e = 0; e < 8; e++) dz0[e] = dz1[e] = (__bf16)0.0f;
         if (g == 0) {
-            for (int e = 0; e < 3; e++) {
+            for (int e = 0; e < MCPT_MLP_OUTPUTS; e++) {
                 float d = 0.0f;
-                if (so < n) {
-                    const float t = targ[(size_t)(first + so) * 3 + e];
+                if (valid) {
+                    const float t = targ[(size_t)(first + s) * 3 + e];
                     const float den = y[e] * y[e] + 0.01f;
                     const float diff = y[e] - t;
                     loss += diff * diff / den;
                     d = 2.0f * diff / den * inv_count;
                 }
-                dy[0][(16 * wave + r) * kRow + e] = (__bf16)d;
+                dz0[e] = (__bf16)d;
             }
         }
-        for (int j = threadIdx.x; j < kTile * 29; j += kBlock) dy[0][(j / 29) * kRow + 3 + (j % 29)] = (__bf16)0.0f;
-        __syncthreads();
-        // output layer: dW4 (wave w: input columns 16w..) and dX4 -> dy[1]
-        {
-            f32x4 acc = go;
-            for (int kc = 0; kc < 2; kc++) {
-                bf16x8 a, b;
-                for (int e = 0; e < 8; e++) {
-                    a[e] = dy[0][(32 * kc + 8 * g + e) * kRow + r];  // A[o=r][s]; o >= 3 reads zeros
-                    b[e] = x[MCPT_MLP_HIDDEN][(32 * kc + 8 * g + e) * kRow + 16 * wave + r];
-                }
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
-            }
-            go = acc;
-            f32x4 dx[4];
-            for (int nt = 0; nt < 4; nt++) dx[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-            bf16x8 a;  // A[i=in][k=o] = W4[o][in], o = 8g+e < 16 lives in the image
-            for (int e = 0; e < 8; e++) a[e] = (g < 2) ? W4[(8 * g + e) * kRow + 16 * wave + r] : (__bf16)0.0f;
-            for (int nt = 0; nt < 4; nt++) {
-                const bf16x8 b = *reinterpret_cast<const bf16x8*>(&dy[0][(16 * nt + r) * kRow + 8 * g]);
-                dx[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, dx[nt], 0, 0, 0);
-            }
-            for (int nt = 0; nt < 4; nt++)
-                for (int e = 0; e < 4; e++) {
-                    const int si = 16 * nt + r, ii = 16 * wave + 4 * g + e;
-                    const bool on = (float)x[MCPT_MLP_HIDDEN][si * kRow + ii] > 0.0f;
-                    dy[1][si * kRow + ii] = (__bf16)(on ? dx[nt][e] : 0.0f);
-                }
-        }
-        __syncthreads();
-        // hidden layers, last to first; dy[c] holds the gradient at layer l's output
-        int c = 1;
-        for (int l = MCPT_MLP_HIDDEN - 1; l >= 0; l--) {
-            const __bf16* Wl = w + img_row(l, 0) * kRow;
-            // dW_l[o][in] += sum_s dy[s][o] x_l[s][in]: A[o][s], B[s][in]
-            for (int kc = 0; kc < 2; kc++) {
-                bf16x8 a;
-                for (int e = 0; e < 8; e++) a[e] = dy[c][(32 * kc + 8 * g + e) * kRow + 16 * wave + r];
-                for (int nt = 0; nt < 4; nt++) {
-                    bf16x8 b;
-                    for (int e = 0; e < 8; e++) b[e] = x[l][(32 * kc + 8 * g + e) * kRow + 16 * nt + r];
-                    gh[l][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, gh[l][nt], 0, 0, 0);
-                }
-            }
-            if (l > 0) {
-                // dx_l[in][s] = sum_o W_l[o][in] dy[s][o], masked by x_l > 0
-                f32x4 dx[4];
-                for (int nt = 0; nt < 4; nt++) dx[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-                for (int kc = 0; kc < 2; kc++) {
-                    bf16x8 a;
-                    for (int e = 0; e < 8; e++) a[e] = Wl[(32 * kc + 8 * g + e) * kRow + 16 * wave + r];
-                    for (int nt = 0; nt < 4; nt++) {
-                        const bf16x8 b =
-                            *reinterpret_cast<const bf16x8*>(&dy[c][(16 * nt + r) * kRow + 32 * kc + 8 * g]);
-                        dx[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, dx[nt], 0, 0, 0);
-                    }
-                }
-                for (int nt = 0; nt < 4; nt++)
-                    for (int e = 0; e < 4; e++) {
-                        const int si = 16 * nt + r, ii = 16 * wave + 4 * g + e;
-                        const bool on = (float)x[l][si * kRow + ii] > 0.0f;
-                        dy[1 - c][si * kRow + ii] = (__bf16)(on ? dx[nt][e] : 0.0f);
-                    }
+        // ---- backward: stage l writes dZ_l^T and X_l^T, then dW_l from LDS
+        // and dZ_{l-1} in registers
+#pragma unroll
+        for (int l = MCPT_MLP_HIDDEN; l >= 0; l--) {
+            __bf16* X = xt[l & 1];
+            __bf16* D = dt[l & 1];
+            for (int kc = 0; kc < 2; kc++)
+                for (int e = 0; e < 8; e++) X[operand_row(l, kc, g, e) * kTRow + col] = act[l][kc][e];
+            if (l == MCPT_MLP_HIDDEN) {  // dY: rows 0..15 (outputs, then zeros)
+                for (int e = 0; e < 8; e++)
+                    if (8 * g + e < kOutRows) D[(8 * g + e) * kTRow + col] = dz0[e];
+            } else {
+                for (int kc = 0; kc < 2; kc++)
+                    for (int e = 0; e < 8; e++) D[operand_row(1, kc, g, e) * kTRow + col] = (kc ? dz1 : dz0)[e];
             }
             __syncthreads();
-            c = 1 - c;
+            if (l == MCPT_MLP_HIDDEN) {
+                if (wave < 4) {  // dW4: outputs 0..15 x inputs 16 wave.., K = the tile's samples
+                    for (int kc = 0; kc < kGradTile / 32; kc++) {
+                        const bf16x8 a = *reinterpret_cast<const bf16x8*>(D + r * kTRow + 32 * kc + 8 * g);
+                        const bf16x8 b =
+                            *reinterpret_cast<const bf16x8*>(X + (16 * wave + r) * kTRow + 32 * kc + 8 * g);
+                        go = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, go, 0, 0, 0);
+                    }
+                }
+            } else {
+                for (int kc = 0; kc < kGradTile / 32; kc++) {
+                    const bf16x8 a = *reinterpret_cast<const bf16x8*>(D + (16 * ob + r) * kTRow + 32 * kc + 8 * g);
+                    for (int j = 0; j < 2; j++) {
+                        const bf16x8 b =
+                            *reinterpret_cast<const bf16x8*>(X + (16 * (nt0 + j) + r) * kTRow + 32 * kc + 8 * g);
+                        gh[l][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, gh[l][j], 0, 0, 0);
+                    }
+                }
+            }
+            if (l == 0) break;
+            // dX_l = W_l^T dZ_l, masked where layer l's input was clamped to 0
+            f32x4 d[4];
+            for (int i = 0; i < 4; i++) {
+                if (l == MCPT_MLP_HIDDEN) {
+                    const bf16x8 a = *reinterpret_cast<const bf16x8*>(w4b + (16 * i + r) * kW4Row + 8 * g);
+                    d[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, dz0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                } else {
+                    const __bf16* Wt = wb[l - 1];
+                    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(Wt + (16 * i + r) * kRow + 8 * g);
+                    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(Wt + (16 * i + r) * kRow + 32 + 8 * g);
+                    d[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, dz0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                    d[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, dz1, d[i], 0, 0, 0);
+                }
+            }
+            for (int e = 0; e < 4; e++) {
+                dz0[e] = (__bf16)((float)act[l][0][e] > 0.0f ? d[0][e] : 0.0f);
+                dz0[4 + e] = (__bf16)((float)act[l][0][4 + e] > 0.0f ? d[1][e] : 0.0f);
+                dz1[e] = (__bf16)((float)act[l][1][e] > 0.0f ? d[2][e] : 0.0f);
+                dz1[4 + e] = (__bf16)((float)act[l][1][4 + e] > 0.0f ? d[3][e] : 0.0f);
+            }
         }
+        __syncthreads();  // the next tile's writes after every read of this one
     }
     // partial sums of this block, in the flat parameter order
     float* out = partial + (size_t)blockIdx.x * kParams;
     for (int l = 0; l < MCPT_MLP_HIDDEN; l++)
-        for (int nt = 0; nt < 4; nt++)
-            for (int e = 0; e < 4; e++) out[param_index(l, 16 * wave + 4 * g + e, 16 * nt + r)] = gh[l][nt][e];
-    if (g == 0)
-        for (int e = 0; e < 3; e++) out[param_index(MCPT_MLP_HIDDEN, e, 16 * wave + r)] = go[e];
-    // loss: lanes r of g == 0 hold per-sample sums
+        for (int j = 0; j < 2; j++)
+            for (int e = 0; e < 4; e++) out[param_index(l, 16 * ob + 4 * g + e, 16 * (nt0 + j) + r)] = gh[l][j][e];
+    if (wave < 4 && g == 0)
+        for (int e = 0; e < MCPT_MLP_OUTPUTS; e++) out[param_index(MCPT_MLP_HIDDEN, e, 16 * wave + r)] = go[e];
+    // loss: lanes r of group 0 hold per-sample sums
     float ls = loss;
     for (int off = 32; off > 0; off >>= 1) ls += __shfl_xor(ls, off);
     if (lane == 0) lsum[wave] = ls;
     __syncthreads();
-    if (threadIdx.x == 0) partial_loss[blockIdx.x] = (lsum[0] + lsum[1]) + (lsum[2] + lsum[3]);
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int w = 0; w < kGradWaves; w++) t += lsum[w];
+        partial_loss[blockIdx.x] = t;
+    }
 }
 
 // grad[p] = sum over the gradient blocks: 8 threads per parameter each add
 // a contiguous eighth of the blocks in block order, then the eight sums are
-// added in order, so the result does not depend on scheduling; loss likewise
-// (block 0, one thread)
+// added in order, so the result does not depend on scheduling; the loss in
+// block 0 by strided sums and a fixed-order tree
 constexpr int kReduceSplit = 8;
 constexpr int kReduceParams = kBlock / kReduceSplit;  // parameters per block
 __global__ __launch_bounds__(kBlock) void k_mlp_reduce(const float* partial, const float* partial_loss, int nblk,
@@ -412,10 +416,17 @@ __global__ __launch_bounds__(kBlock) void k_mlp_reduce(const float* partial, con
         for (int k = 1; k < kReduceSplit; k++) t += part[k][j];
         grad[p] = t;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blockIdx.x == 0) {  // the loss: strided sums, then a fixed-order tree in LDS
+        __shared__ float ls[kBlock];
         float l = 0.0f;
-        for (int b = 0; b < nblk; b++) l += partial_loss[b];
-        loss_out[0] = l * loss_scale;
+        for (int b = threadIdx.x; b < nblk; b += kBlock) l += partial_loss[b];
+        ls[threadIdx.x] = l;
+        __syncthreads();
+        for (int w = kBlock / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) ls[threadIdx.x] = ls[threadIdx.x] + ls[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) loss_out[0] = ls[0] * loss_scale;
     }
 }
 
@@ -825,10 +836,10 @@ void init_params(uint64_t seed, std::vector<float>& p) {
 // gradient of the mean loss over samples [first, first + n) of device
 // arrays; result in mlp->grad and mlp->loss
 int mlp_gradient_dev(mcpt_mlp_s* mlp, const float* feat, const float* targ, int first, int n, hipStream_t st) {
-    const int ntiles = (n + kTile - 1) / kTile;
+    const int ntiles = (n + kGradTile - 1) / kGradTile;
     const int nblk = std::max(1, std::min(ntiles, kMaxGradBlocks));
-    k_mlp_grad<<<nblk, kBlock, 0, st>>>(mlp->wimg.ptr, feat, targ, first, n, n, mlp->partial.ptr,
-                                        mlp->partial_loss.ptr);
+    k_mlp_grad<<<nblk, kGradBlock, 0, st>>>(mlp->wimg.ptr, feat, targ, first, n, n, mlp->partial.ptr,
+                                            mlp->partial_loss.ptr);
     k_mlp_reduce<<<(kParams + kReduceParams - 1) / kReduceParams, kBlock, 0, st>>>(mlp->partial.ptr, mlp->partial_loss.ptr, nblk,
                                                          mlp->grad.ptr, mlp->loss.ptr, 1.0f / float(3 * n));
     MCPT_HIP(hipGetLastError());
