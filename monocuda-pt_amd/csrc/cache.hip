@@ -7,17 +7,15 @@
 // A(16x32, bf16) * B(32x16, bf16), with lane l = r + 16 g (r < 16, g < 4)
 // holding A[r][8g..8g+7], B[8g..8g+7][r] and D[4g..4g+3][r]. Layers are
 // evaluated transposed, X_{l+1}^T = relu(W_l X_l^T): A is a 16-row block of
-// W_l, B a block of 16 samples' inputs, so both operands are rows of
-// row-major [out][in] / [sample][feature] arrays in LDS and no operand needs
-// a transpose. A block of 4 waves runs a tile of 64 samples: wave w owns
-// output rows 16w..16w+15 of every hidden layer.
+// W_l, B a block of 16 samples' inputs. A wave owns 16 samples and keeps
+// their activations in registers from layer to layer (k_mlp_infer).
 //
 // Training (mcpt_mlp_train, and every frame of mcpt_render_cached): one
-// kernel runs forward and backward per tile and keeps the weight gradients
-// of its tiles in registers; each block writes its partial sums and a
-// second kernel adds the blocks in a fixed order, so gradients do not
-// depend on scheduling. Adam then updates FP32 master weights and writes
-// their BF16 image.
+// kernel runs forward and backward per tile of samples (k_mlp_grad) and
+// keeps the weight gradients of its tiles in registers; each block writes
+// its partial sums and a second kernel adds the blocks in a fixed order, so
+// gradients do not depend on scheduling. Adam then updates FP32 master
+// weights and writes their BF16 image.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
