@@ -370,58 +370,37 @@ __global__ __launch_bounds__(kBlock) void k_bb_camera(const BScene* tab, int nsc
 }
 
 // persistent extend over the rays of every scene (extendKernelPersistent,
-// :3416-3468); a lane reloads the BVH view only when its scene changes
+// :3416-3468) with deferred leaf tests; a lane reloads the BVH view only
+// when its scene changes
 __global__ __launch_bounds__(kBlock) void k_bb_extend(const BvhView* bvhs, BWs ws, int cur, int refill) {
     extern __shared__ int lds_stack[];
-    int* stk = lds_stack + threadIdx.x;
     const int total = queue_len(ws.counters, cur);
     if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
-    const int lane = lane_id();
-    Trav s;
     BvhView bv;
     int bv_scene = -1;
-    bool has = false;
-    int slot = -1;
-    bool exhausted = false;
     uint32_t ns = 0, nl = 0;
-    while (true) {
-        if (!exhausted) {
-            const uint64_t idle = ballot64(!has);
-            const int nidle = __popcll(idle);
-            if (nidle >= refill) {
-                const int leader = __builtin_ctzll(idle);
-                int first = 0;
-                if (lane == leader) first = atomicAdd(&ws.counters[C_WORK], nidle);
-                first = __shfl(first, leader);
-                if (!has) {
-                    const int q = first + rank_below(idle);
-                    if (q < total) {
-                        const RayItem it = ws.q[cur][q];
-                        const int sc = unpack_scene(it.ot.w, ws.tb);
-                        if (sc != bv_scene) {
-                            bv = bvhs[sc];
-                            bv_scene = sc;
-                        }
-                        trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
-                        s.time = unpack_time(it.ot.w, ws.tb);
-                        slot = q;
-                        has = true;
-                    }
-                }
-                if (first + nidle >= total) exhausted = true;
+    const RayItem* q = ws.q[cur];
+    persistent_closest<false, false>(
+        bv, &ws.counters[C_WORK], total, refill, lds_stack + threadIdx.x, kBlock,
+        [&](int i, Trav& s, BvhView& v) {
+            const RayItem it = q[i];
+            const int sc = unpack_scene(it.ot.w, ws.tb);
+            if (sc != bv_scene) {
+                v = bvhs[sc];
+                bv_scene = sc;
             }
-        }
-        if (exhausted && ballot64(has) == 0) break;
-        if (has && step_wide<false>(bv, s, stk, kBlock, ns, nl)) {
-            mcpt_packed_hit h;
+            trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+            s.time = unpack_time(it.ot.w, ws.tb);
+        },
+        [&](int i, const Trav& s, const BvhView& v) {
+            mcpt_packed_hit h;  // prim_index keeps the leaf's code: triangle, or ~sphere
             h.t = s.t;
             h.prim_type = s.leaf >= 0 ? 1 : -1;
-            h.prim_index = s.leaf >= 0 ? bv.leaf_prim[s.leaf] : -1;
+            h.prim_index = s.leaf >= 0 ? v.leaf_prim[s.leaf] : -1;
             h.did_hit = s.leaf >= 0 ? 1 : 0;
-            ws.hits[slot] = h;
-            has = false;
-        }
-    }
+            ws.hits[i] = h;
+        },
+        ns, nl);
 }
 
 // shadeKernel (:3470-3696) with the ray's own scene and camera

@@ -579,51 +579,30 @@ __global__ void k_cc_bounce(int* c, int cur) {
     }
 }
 
-// persistent extend (extendKernelPersistent, :3416-3468)
+// persistent extend (extendKernelPersistent, :3416-3468) with deferred
+// leaf tests
 __global__ __launch_bounds__(kBlock) void k_cc_extend(SceneView sv, CacheWs ws, int cur, int refill) {
     extern __shared__ int lds_stack[];
-    int* stk = lds_stack + threadIdx.x;
     const int total = queue_len(ws.counters, cur);
     if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
-    const int lane = lane_id();
-    Trav s;
-    bool has = false;
-    int slot = -1;
-    bool exhausted = false;
     uint32_t ns = 0, nl = 0;
-    while (true) {
-        if (!exhausted) {
-            const uint64_t idle = ballot64(!has);
-            const int nidle = __popcll(idle);
-            if (nidle >= refill) {
-                const int leader = __builtin_ctzll(idle);
-                int first = 0;
-                if (lane == leader) first = atomicAdd(&ws.counters[C_WORK], nidle);
-                first = __shfl(first, leader);
-                if (!has) {
-                    const int q = first + rank_below(idle);
-                    if (q < total) {
-                        const RayItem it = ws.q[cur][q];
-                        trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
-                        s.time = it.ot.w;
-                        slot = q;
-                        has = true;
-                    }
-                }
-                if (first + nidle >= total) exhausted = true;
-            }
-        }
-        if (exhausted && ballot64(has) == 0) break;
-        if (has && step_wide<false>(sv.bvh, s, stk, kBlock, ns, nl)) {
-            mcpt_packed_hit h;
+    const RayItem* q = ws.q[cur];
+    persistent_closest<false, false>(
+        sv.bvh, &ws.counters[C_WORK], total, refill, lds_stack + threadIdx.x, kBlock,
+        [&](int i, Trav& s, const BvhView&) {
+            const RayItem it = q[i];
+            trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+            s.time = it.ot.w;
+        },
+        [&](int i, const Trav& s, const BvhView& v) {
+            mcpt_packed_hit h;  // prim_index keeps the leaf's code: triangle, or ~sphere
             h.t = s.t;
             h.prim_type = s.leaf >= 0 ? 1 : -1;
-            h.prim_index = s.leaf >= 0 ? sv.bvh.leaf_prim[s.leaf] : -1;
+            h.prim_index = s.leaf >= 0 ? v.leaf_prim[s.leaf] : -1;
             h.did_hit = s.leaf >= 0 ? 1 : 0;
-            ws.hits[slot] = h;
-            has = false;
-        }
-    }
+            ws.hits[i] = h;
+        },
+        ns, nl);
 }
 
 // shadeKernel (:3470-3696). A training path records every diffuse vertex

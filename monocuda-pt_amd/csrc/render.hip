@@ -180,68 +180,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneView sv, Workspace ws, i
     count_flush<COUNT>(ws.counters, ns, nl);
 }
 
-// extendKernelPersistent (:3416-3468): lanes keep their traversal across
-// iterations; a wave with at least `refill` idle lanes takes that many queue
-// slots with one atomic and hands them out by ballot rank. In the fast order
-// leaf tests are deferred and run for all lanes that have some pending once
-// kLeafBatch lanes do (or no lane has a node left to visit).
-constexpr int kLeafBatch = 16;
-// The persistent closest-hit loop over `total` rays: lanes take rays by
-// index from *work (begin(q, s) starts ray q's walk) and hand each finished
-// walk to finish(q, s). Shared by the extend kernel and the trace hook.
-template <bool COUNT, bool REF, class Begin, class Finish>
-MCPT_DEV void persistent_closest(const BvhView& bv, int* work, int total, int refill, int* stk, Begin begin,
-                                 Finish finish, uint32_t& ns, uint32_t& nl) {
-    const int lane = lane_id();
-    Trav s;
-    s.pend = 0;
-    bool has = false, walked = false;  // walked: no node left, only pending leaves
-    int slot = -1;
-    bool exhausted = false;  // uniform across the wave
-    while (true) {
-        if (!exhausted) {
-            const uint64_t idle = ballot64(!has);
-            const int nidle = __popcll(idle);
-            if (nidle >= refill) {
-                const int leader = __builtin_ctzll(idle);
-                int first = 0;
-                if (lane == leader) first = atomicAdd(work, nidle);
-                first = __shfl(first, leader);
-                if (!has) {
-                    const int q = first + rank_below(idle);
-                    if (q < total) {
-                        begin(q, s);
-                        slot = q;
-                        has = true;
-                    }
-                }
-                if (first + nidle >= total) exhausted = true;
-            }
-        }
-        if (exhausted && ballot64(has) == 0) break;
-        if (REF) {
-            if (has && step_closest<COUNT>(bv, s, stk, kBlock, ns, nl)) {
-                finish(slot, s);
-                has = false;
-            }
-            continue;
-        }
-        // node steps for lanes with nothing pending; leaf tests once enough
-        // lanes have leaves pending, or when no lane has a node to visit
-        if (has && !s.pend && !walked) walked = step_wide_defer<COUNT>(bv, s, stk, kBlock, ns);
-        const bool want = has && s.pend != 0;
-        const uint64_t wm = ballot64(want);
-        if (wm && (__popcll(wm) >= kLeafBatch || ballot64(has && !s.pend && !walked) == 0)) {
-            if (want) test_pending<COUNT>(bv, s, nl);
-        }
-        if (has && walked && !s.pend) {
-            finish(slot, s);
-            has = false;
-            walked = false;
-        }
-    }
-}
-
+// extendKernelPersistent (:3416-3468) on the shared persistent loop
 template <bool COUNT, bool REF>
 __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Workspace ws, int cur, int refill) {
     extern __shared__ int lds_stack[];
@@ -250,13 +189,13 @@ __global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Work
     uint32_t ns = 0, nl = 0;
     const RayItem* q = ws.q[cur];
     persistent_closest<COUNT, REF>(
-        sv.bvh, &ws.counters[C_WORK], total, refill, lds_stack + threadIdx.x,
-        [&](int i, Trav& s) {
+        sv.bvh, &ws.counters[C_WORK], total, refill, lds_stack + threadIdx.x, kBlock,
+        [&](int i, Trav& s, const BvhView&) {
             const RayItem it = q[i];
             trav_begin<REF>(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
             s.time = it.ot.w;
         },
-        [&](int i, const Trav& s) { ws.hits[i] = pack_hit(sv.bvh, s); }, ns, nl);
+        [&](int i, const Trav& s, const BvhView&) { ws.hits[i] = pack_hit(sv.bvh, s); }, ns, nl);
     count_flush<COUNT>(ws.counters, ns, nl);
 }
 
@@ -582,13 +521,13 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest_persistent(SceneView s
     if ((int)blockIdx.x * kBlock >= n) return;
     uint32_t ns = 0, nl = 0;
     persistent_closest<false, false>(
-        sv.bvh, work, n, refill, lds_stack + threadIdx.x,
-        [&](int i, Trav& s) {
+        sv.bvh, work, n, refill, lds_stack + threadIdx.x, kBlock,
+        [&](int i, Trav& s, const BvhView&) {
             const mcpt_ray r = rays[i];
             trav_begin(s, load3(r.origin), load3(r.dir), r.tmax);
             s.time = r.time;
         },
-        [&](int i, const Trav& s) { hits[i] = pack_hit(sv.bvh, s); }, ns, nl);
+        [&](int i, const Trav& s, const BvhView&) { hits[i] = pack_hit(sv.bvh, s); }, ns, nl);
 }
 // the any-hit hook on the persistent scheme of k_shadow_persistent
 __global__ __launch_bounds__(kBlock) void k_trace_any_persistent(SceneView sv, const mcpt_ray* rays, int n,

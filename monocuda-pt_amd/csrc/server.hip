@@ -127,59 +127,38 @@ __global__ __launch_bounds__(kBlock) void k_srv_generate(Pool p, int cur, int to
     p.meta[cur][slot] = make_int4(q.job, pixel, q.sample, 0);
 }
 
-// persistent extend over rays of mixed scenes; the scene index rides in
-// the ray record (ot.w)
+// persistent extend over rays of mixed scenes with deferred leaf tests; the
+// scene index rides in the ray record (ot.w) and a lane reloads the BVH
+// view only when it changes
 __global__ __launch_bounds__(kBlock) void k_srv_extend(Pool p, int cur, int refill) {
     extern __shared__ int lds_stack[];
-    int* stk = lds_stack + threadIdx.x;
     const int total = p.counters[S_QCOUNT];
     if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
-    const int lane = lane_id();
-    Trav s;
     BvhView bv;
     int bv_scene = -1;  // scene whose view `bv` holds: refills rarely change it
-    bool has = false;
-    int slot = -1;
-    bool exhausted = false;
     uint32_t ns = 0, nl = 0;
-    while (true) {
-        if (!exhausted) {
-            const uint64_t idle = ballot64(!has);
-            const int nidle = __popcll(idle);
-            if (nidle >= refill) {
-                const int leader = __builtin_ctzll(idle);
-                int first = 0;
-                if (lane == leader) first = atomicAdd(&p.counters[S_WORK], nidle);
-                first = __shfl(first, leader);
-                if (!has) {
-                    const int q = first + rank_below(idle);
-                    if (q < total) {
-                        const RayItem it = p.q[cur][q];
-                        const int sc = unpack_scene(it.ot.w, p.tb);
-                        if (sc != bv_scene) {
-                            bv = p.bvhs[sc];
-                            bv_scene = sc;
-                        }
-                        trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
-                        s.time = unpack_time(it.ot.w, p.tb);
-                        slot = q;
-                        has = true;
-                    }
-                }
-                if (first + nidle >= total) exhausted = true;
+    const RayItem* q = p.q[cur];
+    persistent_closest<false, false>(
+        bv, &p.counters[S_WORK], total, refill, lds_stack + threadIdx.x, kBlock,
+        [&](int i, Trav& s, BvhView& v) {
+            const RayItem it = q[i];
+            const int sc = unpack_scene(it.ot.w, p.tb);
+            if (sc != bv_scene) {
+                v = p.bvhs[sc];
+                bv_scene = sc;
             }
-        }
-        if (exhausted && ballot64(has) == 0) break;
-        if (has && step_wide<false>(bv, s, stk, kBlock, ns, nl)) {
-            mcpt_packed_hit h;
+            trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+            s.time = unpack_time(it.ot.w, p.tb);
+        },
+        [&](int i, const Trav& s, const BvhView& v) {
+            mcpt_packed_hit h;  // prim_index keeps the leaf's code: triangle, or ~sphere
             h.t = s.t;
             h.prim_type = s.leaf >= 0 ? 1 : -1;
-            h.prim_index = s.leaf >= 0 ? bv.leaf_prim[s.leaf] : -1;
+            h.prim_index = s.leaf >= 0 ? v.leaf_prim[s.leaf] : -1;
             h.did_hit = s.leaf >= 0 ? 1 : 0;
-            p.hits[slot] = h;
-            has = false;
-        }
-    }
+            p.hits[i] = h;
+        },
+        ns, nl);
 }
 
 // shadeKernel (src/path_tracer.cu:3470-3696) per queue slot; survivors

@@ -411,6 +411,68 @@ MCPT_DEV void trace_closest(const BvhView& bv, Trav& s, int* stk, int stride, ui
     }
 }
 
+// extendKernelPersistent (src/path_tracer.cu:3416-3468): lanes keep their
+// traversal across iterations; a wave with at least `refill` idle lanes takes
+// that many ray indices from *work with one atomic and hands them out by
+// ballot rank. begin(q, s, bv) starts ray q's walk (and may point the lane's
+// view `bv` at the ray's own scene); finish(q, s, bv) takes the finished
+// walk. In the fast order leaf tests are deferred and run for all lanes that
+// have some pending once kLeafBatch lanes do (or no lane has a node left to
+// visit); REF steps the binary tree in the reference's order.
+constexpr int kLeafBatch = 16;
+template <bool COUNT, bool REF, class View, class Begin, class Finish>
+MCPT_DEV void persistent_closest(View& bv, int* work, int total, int refill, int* stk, int stride, Begin begin,
+                                 Finish finish, uint32_t& ns, uint32_t& nl) {
+    const int lane = lane_id();
+    Trav s;
+    s.pend = 0;
+    bool has = false, walked = false;  // walked: no node left, only pending leaves
+    int slot = -1;
+    bool exhausted = false;  // uniform across the wave
+    while (true) {
+        if (!exhausted) {
+            const uint64_t idle = ballot64(!has);
+            const int nidle = __popcll(idle);
+            if (nidle >= refill) {
+                const int leader = __builtin_ctzll(idle);
+                int first = 0;
+                if (lane == leader) first = atomicAdd(work, nidle);
+                first = __shfl(first, leader);
+                if (!has) {
+                    const int q = first + rank_below(idle);
+                    if (q < total) {
+                        begin(q, s, bv);
+                        slot = q;
+                        has = true;
+                    }
+                }
+                if (first + nidle >= total) exhausted = true;
+            }
+        }
+        if (exhausted && ballot64(has) == 0) break;
+        if (REF) {
+            if (has && step_closest<COUNT>(bv, s, stk, stride, ns, nl)) {
+                finish(slot, s, bv);
+                has = false;
+            }
+            continue;
+        }
+        // node steps for lanes with nothing pending; leaf tests once enough
+        // lanes have leaves pending, or when no lane has a node to visit
+        if (has && !s.pend && !walked) walked = step_wide_defer<COUNT>(bv, s, stk, stride, ns);
+        const bool want = has && s.pend != 0;
+        const uint64_t wm = ballot64(want);
+        if (wm && (__popcll(wm) >= kLeafBatch || ballot64(has && !s.pend && !walked) == 0)) {
+            if (want) test_pending<COUNT>(bv, s, nl);
+        }
+        if (has && walked && !s.pend) {
+            finish(slot, s, bv);
+            has = false;
+            walked = false;
+        }
+    }
+}
+
 // BVH::traverseAnyHit (src/path_tracer.cu:1152-1207) on the four-wide tree:
 // boxes are culled with the fixed limit and the walk stops at the first
 // blocker, children in left-to-right order (measured faster than near-first
