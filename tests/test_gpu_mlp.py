@@ -32,7 +32,7 @@ def test_init_matches_oracle(mcpt, ctx):
     net.close()
 
 
-@pytest.mark.parametrize("n", [1, 63, 64, 1000, 4099])
+@pytest.mark.parametrize("n", [1, 63, 64, 1000, 4099, 100003])
 def test_forward_matches_oracle(mcpt, ctx, n):
     print(f"mlp forward n={n}", flush=True)
     net = ctx.mlp(seed=5)
