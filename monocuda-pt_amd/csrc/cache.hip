@@ -338,6 +338,13 @@ constexpr int kGradTile = 16 * kGradWaves;  // samples per block iteration
 constexpr int kTRow = kGradTile + 8;        // bf16 row stride of the transposed tiles
 constexpr int kW4Row = 40;                  // W4^T: 32 columns (outputs, zero-padded) + pad
 static_assert(kGradTile % 32 == 0, "dW takes samples in K chunks of 32");
+// The transposed weight image the training kernel stages as is: W_l^T for
+// l = 1..3 (row = input unit, columns in operand order over the outputs,
+// kRow apart), then W4^T (row = input unit, the outputs zero-padded to 32,
+// kW4Row apart).
+constexpr int kTImgB = (MCPT_MLP_HIDDEN - 1) * kW * kRow;
+constexpr int kTImg = kTImgB + kW * kW4Row;
+static_assert(kTImgB % 8 == 0 && kTImg % 8 == 0, "16-byte staging copies");
 
 // the row (unit) that element e of chunk kc of lane group g carries in the
 // B-operand registers of layer l's input
@@ -346,27 +353,20 @@ MCPT_HDI int operand_row(int l, int kc, int g, int e) {
     return l == 0 ? infer_in0(k) : infer_inh(k);
 }
 
-__global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const __bf16* wimg, const float* feat, const float* targ,
+__global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const __bf16* wimg, const __bf16* wimg_t, const float* feat,
+                                                        const float* targ,
                                                         int first, int n, int n_total_for_mean, float* partial,
                                                         float* partial_loss) {
     __shared__ __attribute__((aligned(16))) __bf16 wf[kImg];                          // forward, infer layout
-    __shared__ __attribute__((aligned(16))) __bf16 wb[MCPT_MLP_HIDDEN - 1][kW * kRow];  // W_l^T, l = 1..3
-    __shared__ __attribute__((aligned(16))) __bf16 w4b[kW * kW4Row];                   // W4^T
+    __shared__ __attribute__((aligned(16))) __bf16 wt[kTImg];                          // W_l^T, l = 1..3; W4^T
     __shared__ __attribute__((aligned(16))) __bf16 xt[2][kW * kTRow];                  // X_l^T
     __shared__ __attribute__((aligned(16))) __bf16 dt[2][kW * kTRow];                  // dZ_l^T
     __shared__ float lsum[kGradWaves];
     load_weights_infer(wimg, wf);
+    for (int i = threadIdx.x; i < kTImg / 8; i += kGradBlock)
+        reinterpret_cast<uint4*>(wt)[i] = reinterpret_cast<const uint4*>(wimg_t)[i];
     __syncthreads();
-    // the transposed weights from the staged image
-    for (int i = threadIdx.x; i < (MCPT_MLP_HIDDEN - 1) * kW * kW; i += kGradBlock) {
-        const int l = 1 + i / (kW * kW), in = (i / kW) % kW, k = i % kW;
-        wb[l - 1][in * kRow + k] = wf[img_row(l, infer_inh(k)) * kRow + image_colh(in)];
-    }
-    for (int i = threadIdx.x; i < kW * 32; i += kGradBlock) {
-        const int in = i / 32, o = i % 32;
-        w4b[in * kW4Row + o] = o < kOutRows ? wf[img_row(MCPT_MLP_HIDDEN, o) * kRow + image_colh(in)] : (__bf16)0.0f;
-    }
-    __syncthreads();
+    const __bf16* w4b = wt + kTImgB;
     const int lane = lane_id(), wave = (int)threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
     const int col = 16 * wave + r;  // this lane's sample column in the transposed tiles
     const float inv_count = 1.0f / float(3 * n_total_for_mean);
@@ -377,18 +377,28 @@ __global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const __bf16* wimg, con
     f32x4 go = f32x4{0.f, 0.f, 0.f, 0.f};  // output layer (waves 0..3): outputs 4g + e, inputs 16 wave + r
     float loss = 0.0f;
     const int ntiles = (n + kGradTile - 1) / kGradTile;
+    // features and targets of this lane's sample of a tile, loaded one tile
+    // ahead (samples past n read the last one's and contribute nothing)
+    float4 fa, fb, fc;
+    float tg[MCPT_MLP_OUTPUTS];
+    auto fetch = [&](int tile) {
+        const size_t i = (size_t)first + min(tile * kGradTile + col, n - 1);
+        const float* fs = feat + i * MCPT_MLP_FEATURES;
+        fa = make_float4(fs[0], fs[1], fs[2], fs[3]);
+        fb = make_float4(fs[4], fs[5], fs[6], fs[7]);
+        fc = make_float4(fs[8], fs[9], fs[10], 0.0f);
+        for (int e = 0; e < MCPT_MLP_OUTPUTS; e++) tg[e] = targ[i * 3 + e];
+    };
+    if ((int)blockIdx.x < ntiles) fetch(blockIdx.x);
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int s = tile * kGradTile + col;
         const bool valid = s < n;
         // ---- forward, inputs of every layer kept (act[l] = input of layer l)
         bf16x8 act[MCPT_MLP_HIDDEN + 1][2];
-        {
-            const float* fs = feat + (size_t)(first + min(s, n - 1)) * MCPT_MLP_FEATURES;
-            const float4 fa = make_float4(fs[0], fs[1], fs[2], fs[3]);
-            const float4 fb = make_float4(fs[4], fs[5], fs[6], fs[7]);
-            const float4 fc = make_float4(fs[8], fs[9], fs[10], 0.0f);
-            encode_lane(fa, fb, fc, g, act[0][0], act[0][1]);
-        }
+        encode_lane(fa, fb, fc, g, act[0][0], act[0][1]);
+        float t3[MCPT_MLP_OUTPUTS];
+        for (int e = 0; e < MCPT_MLP_OUTPUTS; e++) t3[e] = tg[e];
+        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
 #pragma unroll
         for (int l = 0; l < MCPT_MLP_HIDDEN; l++) {
             const __bf16* Wl = wf + img_row(l, 0) * kRow;
@@ -414,7 +424,7 @@ __global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const __bf16* wimg, con
             for (int e = 0; e < MCPT_MLP_OUTPUTS; e++) {
                 float d = 0.0f;
                 if (valid) {
-                    const float t = targ[(size_t)(first + s) * 3 + e];
+                    const float t = t3[e];
                     const float den = y[e] * y[e] + 0.01f;
                     const float diff = y[e] - t;
                     loss += diff * diff / den;
@@ -466,7 +476,7 @@ __global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const __bf16* wimg, con
                     const bf16x8 a = *reinterpret_cast<const bf16x8*>(w4b + (16 * i + r) * kW4Row + 8 * g);
                     d[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, dz0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
                 } else {
-                    const __bf16* Wt = wb[l - 1];
+                    const __bf16* Wt = wt + (l - 1) * kW * kRow;
                     const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(Wt + (16 * i + r) * kRow + 8 * g);
                     const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(Wt + (16 * i + r) * kRow + 32 + 8 * g);
                     d[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, dz0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
@@ -496,14 +506,18 @@ __global__ __launch_bounds__(kGradBlock) void k_mlp_grad(const __bf16* wimg, con
     }
 }
 
-// grad[p] = sum over the gradient blocks: 8 threads per parameter each add
-// a contiguous eighth of the blocks in block order, then the eight sums are
-// added in order, so the result does not depend on scheduling; the loss in
-// block 0 by strided sums and a fixed-order tree
-constexpr int kReduceSplit = 8;
-constexpr int kReduceParams = kBlock / kReduceSplit;  // parameters per block
-__global__ __launch_bounds__(kBlock) void k_mlp_reduce(const float* partial, const float* partial_loss, int nblk,
-                                                      float* grad, float* loss_out, float loss_scale) {
+// grad[p] = sum over the gradient blocks: a block takes 64 consecutive
+// parameters (one wave reads 256 contiguous bytes of a block's partials)
+// and 16 threads per parameter each add a contiguous sixteenth of the
+// blocks in block order, then the sixteen sums are added in order, so the
+// result does not depend on scheduling; the loss in block 0 by strided sums
+// and a fixed-order tree
+constexpr int kReduceSplit = 16;
+constexpr int kReduceParams = 64;  // parameters per block
+constexpr int kReduceBlock = kReduceSplit * kReduceParams;
+__global__ __launch_bounds__(kReduceBlock) void k_mlp_reduce(const float* partial, const float* partial_loss,
+                                                             int nblk, float* grad, float* loss_out,
+                                                             float loss_scale) {
     __shared__ float part[kReduceSplit][kReduceParams];
     const int j = threadIdx.x % kReduceParams, q = threadIdx.x / kReduceParams;
     const int p = blockIdx.x * kReduceParams + j;
@@ -519,12 +533,12 @@ __global__ __launch_bounds__(kBlock) void k_mlp_reduce(const float* partial, con
         grad[p] = t;
     }
     if (blockIdx.x == 0) {  // the loss: strided sums, then a fixed-order tree in LDS
-        __shared__ float ls[kBlock];
+        __shared__ float ls[kReduceBlock];
         float l = 0.0f;
-        for (int b = threadIdx.x; b < nblk; b += kBlock) l += partial_loss[b];
+        for (int b = threadIdx.x; b < nblk; b += kReduceBlock) l += partial_loss[b];
         ls[threadIdx.x] = l;
         __syncthreads();
-        for (int w = kBlock / 2; w > 0; w >>= 1) {
+        for (int w = kReduceBlock / 2; w > 0; w >>= 1) {
             if ((int)threadIdx.x < w) ls[threadIdx.x] = ls[threadIdx.x] + ls[threadIdx.x + w];
             __syncthreads();
         }
@@ -532,11 +546,18 @@ __global__ __launch_bounds__(kBlock) void k_mlp_reduce(const float* partial, con
     }
 }
 
-// Adam on the FP32 master weights, then their BF16 image; with decay > 0
-// also the exponential average of the weights and its image
+// the position of parameter (layer, o, in) in the transposed image
+// (layers 1..4)
+MCPT_HDI int timg_index(int layer, int o, int in) {
+    return layer < MCPT_MLP_HIDDEN ? (layer - 1) * kW * kRow + in * kRow + image_colh(o) : kTImgB + in * kW4Row + o;
+}
+
+// Adam on the FP32 master weights, then their BF16 images (operand order
+// and transposed); with decay > 0 also the exponential average of the
+// weights and its image
 __global__ __launch_bounds__(kBlock) void k_adam(float* master, float* m, float* v, const float* grad, __bf16* wimg,
-                                                float lr, float bc1, float bc2, float* ema, __bf16* wimg_ema,
-                                                float decay) {
+                                                __bf16* wimg_t, float lr, float bc1, float bc2, float* ema,
+                                                __bf16* wimg_ema, float decay) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= kParams) return;
     const float g = grad[p];
@@ -549,6 +570,7 @@ __global__ __launch_bounds__(kBlock) void k_adam(float* master, float* m, float*
     const int layer = p / (kW * kIn), o = (p % (kW * kIn)) / kIn, in = p % kIn;
     const int at = img_row(layer, o) * kRow + image_col(layer, in);
     wimg[at] = (__bf16)wv;
+    if (layer > 0) wimg_t[timg_index(layer, o, in)] = (__bf16)wv;
     if (decay > 0.0f) {
         const float a = decay * ema[p] + (1.0f - decay) * wv;
         ema[p] = a;
@@ -567,6 +589,20 @@ __global__ __launch_bounds__(kBlock) void k_pack(const float* master, __bf16* wi
     if (col < kIn && (layer < MCPT_MLP_HIDDEN || o < MCPT_MLP_OUTPUTS))
         val = master[param_index(layer, o, image_logical(layer, col))];
     wimg[i] = (__bf16)val;
+}
+// the transposed image (padding stays zero)
+__global__ __launch_bounds__(kBlock) void k_pack_t(const float* master, __bf16* wimg_t) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= kTImg) return;
+    float val = 0.0f;
+    if (i < kTImgB) {
+        const int layer = 1 + i / (kW * kRow), in = (i / kRow) % kW, k = i % kRow;
+        if (k < kW) val = master[param_index(layer, infer_inh(k), in)];
+    } else {
+        const int in = (i - kTImgB) / kW4Row, o = (i - kTImgB) % kW4Row;
+        if (o < MCPT_MLP_OUTPUTS) val = master[param_index(MCPT_MLP_HIDDEN, o, in)];
+    }
+    wimg_t[i] = (__bf16)val;
 }
 
 int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
@@ -891,7 +927,7 @@ using namespace mcpt;
 struct mcpt_mlp_s {
     Ctx* ctx = nullptr;
     DevBuf<float> master, m, v, grad, partial, partial_loss, loss, ema;
-    DevBuf<__bf16> wimg, wimg_ema;  // BF16 images of the weights and of their average
+    DevBuf<__bf16> wimg, wimg_t, wimg_ema;  // BF16 images of the weights (also transposed) and of their average
     DevBuf<float> io;  // staging for the host-array entry points
     int step = 0;
     // cached-render workspace, grown on demand
@@ -926,9 +962,9 @@ void init_params(uint64_t seed, std::vector<float>& p) {
 int mlp_gradient_dev(mcpt_mlp_s* mlp, const float* feat, const float* targ, int first, int n, hipStream_t st) {
     const int ntiles = (n + kGradTile - 1) / kGradTile;
     const int nblk = std::max(1, std::min(ntiles, kMaxGradBlocks));
-    k_mlp_grad<<<nblk, kGradBlock, 0, st>>>(mlp->wimg.ptr, feat, targ, first, n, n, mlp->partial.ptr,
+    k_mlp_grad<<<nblk, kGradBlock, 0, st>>>(mlp->wimg.ptr, mlp->wimg_t.ptr, feat, targ, first, n, n, mlp->partial.ptr,
                                             mlp->partial_loss.ptr);
-    k_mlp_reduce<<<(kParams + kReduceParams - 1) / kReduceParams, kBlock, 0, st>>>(mlp->partial.ptr, mlp->partial_loss.ptr, nblk,
+    k_mlp_reduce<<<(kParams + kReduceParams - 1) / kReduceParams, kReduceBlock, 0, st>>>(mlp->partial.ptr, mlp->partial_loss.ptr, nblk,
                                                          mlp->grad.ptr, mlp->loss.ptr, 1.0f / float(3 * n));
     MCPT_HIP(hipGetLastError());
     return MCPT_OK;
@@ -938,7 +974,8 @@ int mlp_adam(mcpt_mlp_s* mlp, float lr, hipStream_t st, float ema_decay = 0.0f) 
     mlp->step++;
     const float bc1 = (float)(1.0 - std::pow(0.9, mlp->step)), bc2 = (float)(1.0 - std::pow(0.99, mlp->step));
     k_adam<<<blocks_for(kParams), kBlock, 0, st>>>(mlp->master.ptr, mlp->m.ptr, mlp->v.ptr, mlp->grad.ptr,
-                                                   mlp->wimg.ptr, lr, bc1, bc2, mlp->ema.ptr, mlp->wimg_ema.ptr,
+                                                   mlp->wimg.ptr, mlp->wimg_t.ptr, lr, bc1, bc2, mlp->ema.ptr,
+                                                   mlp->wimg_ema.ptr,
                                                    ema_decay);
     MCPT_HIP(hipGetLastError());
     return MCPT_OK;
@@ -1026,6 +1063,7 @@ int mcpt_mlp_create(mcpt_ctx ctx, uint64_t seed, mcpt_mlp* out) {
     if (s >= 0) s = mlp->partial_loss.alloc(c, kMaxGradBlocks);
     if (s >= 0) s = mlp->loss.alloc(c, 4);
     if (s >= 0) s = mlp->wimg.alloc(c, kImg);
+    if (s >= 0) s = mlp->wimg_t.alloc(c, kTImg);
     if (s >= 0) s = mlp->ema.alloc(c, kParams);
     if (s >= 0) s = mlp->wimg_ema.alloc(c, kImg);
     if (s < 0) {
@@ -1045,6 +1083,7 @@ int mcpt_mlp_destroy(mcpt_mlp mlp) {
                              &mlp->loss, &mlp->io, &mlp->ema})
         b->release(c);
     mlp->wimg.release(c);
+    mlp->wimg_t.release(c);
     mlp->wimg_ema.release(c);
     if (mlp->ws_mem) dev_free(c, mlp->ws_mem, mlp->ws_bytes);
     delete mlp;
@@ -1066,6 +1105,7 @@ int mcpt_mlp_set_params(mcpt_mlp mlp, const float* params) {
     MCPT_HIP(hipMemsetAsync(mlp->m.ptr, 0, kParams * sizeof(float), st));
     MCPT_HIP(hipMemsetAsync(mlp->v.ptr, 0, kParams * sizeof(float), st));
     k_pack<<<blocks_for(kImg), kBlock, 0, st>>>(mlp->master.ptr, mlp->wimg.ptr);
+    k_pack_t<<<blocks_for(kTImg), kBlock, 0, st>>>(mlp->master.ptr, mlp->wimg_t.ptr);
     MCPT_HIP(hipMemcpyAsync(mlp->ema.ptr, mlp->master.ptr, kParams * sizeof(float), hipMemcpyDeviceToDevice, st));
     MCPT_HIP(hipMemcpyAsync(mlp->wimg_ema.ptr, mlp->wimg.ptr, kImg * sizeof(__bf16), hipMemcpyDeviceToDevice, st));
     MCPT_HIP(hipGetLastError());
