@@ -662,6 +662,8 @@ def main():
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--shadow-per-ray", action="store_true", help="shadow rays one lane per ray (default: persistent)")
     ap.add_argument("--batch", type=int, default=0, help="samples in flight per wavefront pass (0 = auto)")
+    ap.add_argument("--group-below", type=int, default=0,
+                    help="queues shorter than this walked by 16 lanes per ray (0 = library default, -1 = never)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the traversal-counting pass")
@@ -779,18 +781,20 @@ def main():
         scratch = ctx.film(W, H)
         if args.warmup > 0:
             scene.render(cam, scratch, mode=mode, spp=args.warmup, sample_begin=K, max_depth=args.depth,
-                         seed=args.seed, refill_lanes=args.refill, batch=args.batch, flags=xf)
+                         seed=args.seed, refill_lanes=args.refill, batch=args.batch, flags=xf,
+                         group_below=args.group_below)
         log(f"bench: warmup {args.warmup} done")
         t1 = time.perf_counter()
         st = scene.render(cam, film, mode=mode, spp=K, sample_begin=0, max_depth=args.depth, seed=args.seed,
-                          refill_lanes=args.refill, batch=args.batch, flags=xf)
+                          refill_lanes=args.refill, batch=args.batch, flags=xf, group_below=args.group_below)
         wall_s = time.perf_counter() - t1
         log(f"bench: timed {K} steps in {wall_s:.3f} s (device {st['total_ms']:.1f} ms)")
         # the same samples again with every launch in order on one stream and
         # timed on its own: the per-kernel split and the extend roofline
         scratch.clear()
         kt = scene.render(cam, scratch, mode=mode, spp=K, sample_begin=0, max_depth=args.depth, seed=args.seed,
-                          refill_lanes=args.refill, batch=args.batch, flags=mcpt.RENDER_TIME_KERNELS | xf)
+                          refill_lanes=args.refill, batch=args.batch, flags=mcpt.RENDER_TIME_KERNELS | xf,
+                          group_below=args.group_below)
         assert kt["extend_rays"] == st["extend_rays"]
         log(f"bench: kernel-timed pass {kt['total_ms']:.1f} ms")
         counted = ref_counted = None

@@ -278,7 +278,10 @@ typedef struct mcpt_render_opts {
     uint32_t flags;
     int32_t batch;        /* wavefront modes: samples in flight per pass
                              (paths = W*H*batch); 0 picks the default     */
-    int32_t reserved;
+    int32_t group_below;  /* persistent mode: a closest-hit or shadow queue
+                             shorter than this is walked by 16 lanes per
+                             ray (same hits); 0 picks the default (32768),
+                             < 0 never                                     */
 } mcpt_render_opts;
 
 typedef struct mcpt_render_stats {

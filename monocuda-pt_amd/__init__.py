@@ -96,7 +96,7 @@ class LbvhDump(C.Structure):
 class RenderOpts(C.Structure):
     _fields_ = [("mode", C.c_int32), ("spp", C.c_int32), ("sample_begin", C.c_int32), ("max_depth", C.c_int32),
                 ("seed", C.c_uint64), ("refill_lanes", C.c_int32), ("flags", C.c_uint32), ("batch", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("group_below", C.c_int32)]
 
 
 class RenderStats(C.Structure):
@@ -715,8 +715,8 @@ class Scene:
         return occ
 
     def render(self, camera, film, mode=MODE_PERSISTENT, spp=1, sample_begin=0, max_depth=8, seed=0,
-               refill_lanes=0, flags=0, batch=0):
-        o = RenderOpts(mode, spp, sample_begin, max_depth, seed, refill_lanes, flags, batch, 0)
+               refill_lanes=0, flags=0, batch=0, group_below=0):
+        o = RenderOpts(mode, spp, sample_begin, max_depth, seed, refill_lanes, flags, batch, group_below)
         st = RenderStats()
         _check(_lib.mcpt_render(self.h, C.byref(camera), C.byref(o), film.h, C.byref(st)), "mcpt_render")
         return st.as_dict()

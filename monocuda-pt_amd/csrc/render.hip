@@ -180,11 +180,44 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneView sv, Workspace ws, i
     count_flush<COUNT>(ws.counters, ns, nl);
 }
 
+// A queue shorter than group_below: kGroup lanes per ray (group_closest),
+// each block's groups with their stacks side by side in LDS.
+__device__ __forceinline__ void extend_group(const SceneView& sv, const Workspace& ws, int cur, int total, int* lds,
+                                             int cap, int cap_phys) {
+    if ((int)blockIdx.x * (kBlock / kGroup) >= total) return;  // blocks the queue does not need
+    int* base = lds + (threadIdx.x / kGroup) * group_lds_ints(cap_phys);
+    const RayItem* q = ws.q[cur];
+    BvhView bv = sv.bvh;
+    group_closest(
+        bv, &ws.counters[C_WORK], total, reinterpret_cast<int2*>(base), base + 2 * cap_phys, cap, cap_phys,
+        [&](int i, Trav& s, const BvhView&) {
+            const RayItem it = q[i];
+            trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+            s.time = it.ot.w;
+        },
+        [&](int i, const Trav& s, const BvhView&) { ws.hits[i] = pack_hit(sv.bvh, s); });
+}
+
+// A short queue's launch of the group walk (the persistent kernel below
+// returns at once for it, and this one for a long queue). Kept a launch of
+// its own: in the same kernel the group walk's registers (75 VGPRs against
+// 59) cost the other lane's kernels room beside every persistent launch
+// (64 spp on the 1M-triangle mesh: 1,437 against 1,465 Mrays/s).
+__global__ __launch_bounds__(kBlock) void k_extend_group(SceneView sv, Workspace ws, int cur, int group_below,
+                                                          int cap, int cap_phys) {
+    extern __shared__ int lds[];
+    const int total = queue_len(ws.counters, cur);
+    if (total >= group_below) return;
+    extend_group(sv, ws, cur, total, lds, cap, cap_phys);
+}
+
 // extendKernelPersistent (:3416-3468) on the shared persistent loop
 template <bool COUNT, bool REF>
-__global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Workspace ws, int cur, int refill) {
+__global__ __launch_bounds__(kBlock) void k_extend_persistent(SceneView sv, Workspace ws, int cur, int refill,
+                                                               int group_below) {
     extern __shared__ int lds_stack[];
     const int total = queue_len(ws.counters, cur);
+    if (total < group_below) return;                 // k_extend_group walks a short queue
     if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
     uint32_t ns = 0, nl = 0;
     const RayItem* q = ws.q[cur];
@@ -283,6 +316,16 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneView sv, Workspace w
 }
 
 // ---------------------------------------------------------------- shadow
+// the light of an unblocked shadow ray into its path's sample
+__device__ __forceinline__ void shadow_deliver(const Workspace& ws, int cur, const ShadowItem& si) {
+    float4* dst = shadow_target(ws, cur, __float_as_int(si.dp.w));
+    float4 v = *dst;
+    v.x = v.x + si.c.x;
+    v.y = v.y + si.c.y;
+    v.z = v.z + si.c.z;
+    *dst = v;
+}
+
 // shadowKernel (:3698-3728)
 __global__ __launch_bounds__(kBlock) void k_shadow(SceneView sv, Workspace ws, int cur) {
     extern __shared__ int lds_stack[];
@@ -301,15 +344,43 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneView sv, Workspace ws, i
     }
 }
 
+// A shadow queue shorter than group_below: kGroup lanes per ray (group_any).
+__device__ __forceinline__ void shadow_group(const SceneView& sv, const Workspace& ws, int cur, int total, int* lds,
+                                             int cap, int cap_phys) {
+    if ((int)blockIdx.x * (kBlock / kGroup) >= total) return;  // blocks the queue does not need
+    int* base = lds + (threadIdx.x / kGroup) * group_lds_ints(cap_phys);
+    BvhView bv = sv.bvh;
+    group_any(
+        bv, &ws.counters[C_WORK_SHADOW], total, reinterpret_cast<int2*>(base), base + 2 * cap_phys, cap, cap_phys,
+        [&](int i, Trav& s, const BvhView&) {
+            const ShadowItem si = ws.sq[i];
+            trav_begin(s, vec3(si.om.x, si.om.y, si.om.z), vec3(si.dp.x, si.dp.y, si.dp.z), si.om.w);
+            s.time = si.c.w;
+        },
+        [&](int i, bool blocked) {
+            if (!blocked) shadow_deliver(ws, cur, ws.sq[i]);
+        });
+}
+
 // Shadow rays on the persistent scheme of the extend kernel (scenes without
 // a medium: a plain visibility test), the persistent mode's default. The
 // reference tried this on 32-lane warps and reverted it (README.md:170); on
 // 64-lane waves over the four-wide tree it takes 28 ms where one lane per
 // ray (k_shadow) takes 44 ms on the 1M-triangle mesh at 1024x1024, 64 spp.
-__global__ __launch_bounds__(kBlock) void k_shadow_persistent(SceneView sv, Workspace ws, int cur, int refill) {
+__global__ __launch_bounds__(kBlock) void k_shadow_group(SceneView sv, Workspace ws, int cur, int group_below,
+                                                          int cap, int cap_phys) {
+    extern __shared__ int lds[];
+    const int total = shadow_len(ws.counters, 1 - cur);
+    if (total >= group_below) return;
+    shadow_group(sv, ws, cur, total, lds, cap, cap_phys);
+}
+
+__global__ __launch_bounds__(kBlock) void k_shadow_persistent(SceneView sv, Workspace ws, int cur, int refill,
+                                                               int group_below) {
     extern __shared__ int lds_stack[];
     int* stk = lds_stack + threadIdx.x;
     const int total = shadow_len(ws.counters, 1 - cur);
+    if (total < group_below) return;                 // k_shadow_group tests a short queue
     if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
     const int lane = lane_id();
     Trav s;
@@ -342,16 +413,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow_persistent(SceneView sv, Work
         if (has) {
             const int st = step_wide_any(sv.bvh, s, stk, kBlock);
             if (st != 0) {
-                if (st == 1) {
-                    const ShadowItem si = ws.sq[slot];
-                    const float tr = 1.0f;
-                    float4* dst = shadow_target(ws, cur, __float_as_int(si.dp.w));
-                    float4 v = *dst;
-                    v.x = v.x + si.c.x * tr;
-                    v.y = v.y + si.c.y * tr;
-                    v.z = v.z + si.c.z * tr;
-                    *dst = v;
-                }
+                if (st == 1) shadow_deliver(ws, cur, ws.sq[slot]);
                 has = false;
             }
         }
@@ -578,6 +640,17 @@ __global__ void k_zero_int(int* p) {
 int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 size_t stack_bytes(const mcpt_scene_s* s) { return (size_t)kBlock * stack_entries(s->max_depth) * sizeof(int); }
 
+// Group walks (k_extend_group, k_shadow_group): a queue shorter than
+// kGroupBelow rays is walked by kGroup lanes per ray. The soft stack cap
+// lets every lane of a group pop while the stack holds up to 48 pairs.
+constexpr int kGroupBelow = 32768;
+constexpr int kGroupCap = 96;
+// the group stack's bound: cap + three pending siblings per wide level + 1
+int group_stack_pairs(const mcpt_scene_s* s) { return kGroupCap + 3 * (stack_entries(s->max_depth) / 2 + 1) + 4; }
+size_t group_lds_bytes(const mcpt_scene_s* s) {
+    return (size_t)(kBlock / kGroup) * group_lds_ints(group_stack_pairs(s)) * sizeof(int);
+}
+
 // Layout of a wavefront workspace of `capacity` paths from `base`: two ray
 // queues, hit records, the shadow queue, path state, then the counters.
 constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -728,9 +801,16 @@ int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
     }
     MCPT_TRY(mark(K_OTHER));
 
-    int persistent_blocks = 0, shadow_blocks = 0;
+    int persistent_blocks = 0, shadow_blocks = 0, group_blocks = 0;
     const bool persistent_shadow =
         o->mode == MCPT_MODE_PERSISTENT && !s->has_medium && !(o->flags & MCPT_RENDER_SHADOW_PER_RAY);
+    // Short queues go to the group walks; the counting and reference-order
+    // passes keep one lane per ray (their step counts are the single walk's).
+    const int group_below = o->mode != MCPT_MODE_PERSISTENT || count || ref || o->group_below < 0 ? 0
+                            : o->group_below > 0                                                ? o->group_below
+                                                                                                 : kGroupBelow;
+    const int gcap_phys = group_stack_pairs(s);
+    const size_t glds = group_lds_bytes(s);
     if (o->mode == MCPT_MODE_PERSISTENT) {
         int per_cu = 0;
         // With two lanes each lane's persistent launches take half of a CU's
@@ -742,6 +822,11 @@ int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
         persistent_blocks = std::max(1, (per_cu + nlanes - 1) / nlanes) * ctx->cu_count;
         MCPT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_shadow_persistent, kBlock, lds));
         shadow_blocks = std::max(1, (per_cu + nlanes - 1) / nlanes) * ctx->cu_count;
+        if (group_below > 0) {  // a group per ray at most
+            MCPT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_extend_group, kBlock, glds));
+            const int need = (group_below + kBlock / kGroup - 1) / (kBlock / kGroup);
+            group_blocks = std::min(need, std::max(1, (per_cu + nlanes - 1) / nlanes) * ctx->cu_count);
+        }
     }
     int shade_blocks = 0;
     {
@@ -778,7 +863,10 @@ int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
                 const int blocks = std::min(grid, persistent_blocks);
                 auto kern = count ? (ref ? k_extend_persistent<true, true> : k_extend_persistent<true, false>)
                                   : (ref ? k_extend_persistent<false, true> : k_extend_persistent<false, false>);
-                kern<<<blocks, kBlock, lds, st>>>(sv, w, cur, refill);
+                kern<<<blocks, kBlock, lds, st>>>(sv, w, cur, refill, group_below);
+                if (group_below > 0)
+                    k_extend_group<<<std::min(grid, group_blocks), kBlock, glds, st>>>(sv, w, cur, group_below,
+                                                                                     kGroupCap, gcap_phys);
             } else {
                 auto kern = count ? (ref ? k_extend<true, true> : k_extend<true, false>)
                                   : (ref ? k_extend<false, true> : k_extend<false, false>);
@@ -795,9 +883,13 @@ int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
                 MCPT_HIP(hipStreamWaitEvent(shs, ev_shade, 0));
             }
             k_counters_shadow_total<<<1, 64, 0, shs>>>(w.counters, cur);
-            if (persistent_shadow)
-                k_shadow_persistent<<<std::min(grid, shadow_blocks), kBlock, lds, shs>>>(sv, w, cur, refill);
-            else
+            if (persistent_shadow) {
+                k_shadow_persistent<<<std::min(grid, shadow_blocks), kBlock, lds, shs>>>(sv, w, cur, refill,
+                                                                                        group_below);
+                if (group_below > 0)
+                    k_shadow_group<<<std::min(grid, group_blocks), kBlock, glds, shs>>>(sv, w, cur, group_below,
+                                                                                      kGroupCap, gcap_phys);
+            } else
                 k_shadow<<<grid, kBlock, lds, shs>>>(sv, w, cur);
             if (overlap) MCPT_HIP(hipEventRecord(ev_shadow, shs));
             MCPT_TRY(mark(K_SHADOW));

@@ -485,4 +485,268 @@ MCPT_DEV bool trace_any(const BvhView& bv, Trav& s, int* stk, int stride) {
     return st == 2;
 }
 
+// ------------------------------------------- group walks for short queues
+// A launch lasts as long as its slowest ray, and one lane walking one ray
+// visits that ray's nodes one dependent fetch after another: the late,
+// thin bounces of a render take ~0.6 ms each whatever their ray count. For
+// a short queue kGroup lanes walk each ray together. The group shares one
+// stack in LDS of (wide node, entry distance) pairs and pops up to kGroup of
+// them per iteration, one per lane; it tests the leaves that the popped
+// nodes hit with the tests spread over its lanes, takes the minimum
+// (t, tie rank) over its lanes, and pushes the inner children still entered
+// before that distance, every node's nearest child last, so that the next
+// pops take the nearest child of each node just expanded.
+//
+// Same hits: every box that holds a hit at t is entered before t (the 1e-3
+// padding), so an entry is culled only when its box holds nothing at or
+// before the group's best distance; every leaf that reaches the final t is
+// tested, and the tie rank picks among equal t as in step_wide. The order of
+// the tests changes nothing.
+//
+// Stack bound: an iteration pops at most max(1, (cap - sp) / 3) entries and
+// each pushes at most four, so an iteration that pops two or more leaves at
+// most cap entries; a run of single pops is a depth-first walk on top of
+// those, which keeps at most three pending siblings per wide level. The
+// stack never holds more than cap + 3 * levels + 1 pairs
+// (group_stack_pairs on the host).
+constexpr int kGroup = 16;
+
+// LDS ints per group: its stack of cap_phys (node, entry distance) pairs and
+// the leaf list of one iteration (at most four per popped node)
+__host__ __device__ constexpr int group_lds_ints(int cap_phys) { return 2 * cap_phys + 4 * kGroup; }
+
+MCPT_DEV int group_pops(int sp, int cap) {
+    int n = (cap - sp) / 3;
+    n = n < 1 ? 1 : (n > kGroup ? kGroup : n);
+    return n < sp ? n : sp;
+}
+
+// LDS written by one lane of a wave and read by another: the wave's LDS
+// operations complete in order, and the fences keep the compiler from
+// moving them across this point
+MCPT_DEV void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct GroupLanes {
+    int gl;         // lane within the group
+    int g0;         // the group's first lane in the wave
+    uint64_t gm;    // the group's lanes
+    uint64_t below; // the wave's lanes below this one
+};
+
+MCPT_DEV GroupLanes group_lanes() {
+    GroupLanes g;
+    const int lane = lane_id();
+    g.gl = lane & (kGroup - 1);
+    g.g0 = lane & ~(kGroup - 1);
+    g.gm = ((1ull << kGroup) - 1ull) << g.g0;
+    g.below = (1ull << lane) - 1ull;
+    return g;
+}
+
+// The group's (t, leaf) minimum in every lane: smaller t, and among equal t
+// the leaf of smaller tie rank (a total order on the hits, so the butterfly
+// leaves the same pair in every lane).
+MCPT_DEV void group_min_hit(const BvhView& bv, Trav& s) {
+#pragma unroll
+    for (int m = 1; m < kGroup; m <<= 1) {
+        const float ot = __shfl_xor(s.t, m);
+        const int ol = __shfl_xor(s.leaf, m);
+        bool take = ot < s.t;
+        if (!take && ot == s.t && ol >= 0 && s.leaf >= 0 && ol != s.leaf) take = bv.tie[ol] < bv.tie[s.leaf];
+        if (take) {
+            s.t = ot;
+            s.leaf = ol;
+        }
+    }
+}
+
+// The leaves in `leaves` (child slots of c) of every lane of the group into
+// lf[], in lane order; returns how many the group holds.
+MCPT_DEV int group_leaf_list(const GroupLanes& g, uint32_t leaves, const int c[4], int* lf) {
+    const int nl = __builtin_popcount(leaves);
+    int at = 0, total = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const uint64_t m = ballot64(nl > b) & g.gm;
+        at += __popcll(m & g.below);
+        total += __popcll(m);
+    }
+    while (leaves) {
+        const int j = __builtin_ctz(leaves);
+        leaves &= leaves - 1u;
+        lf[at++] = ~pick4(c, j);
+    }
+    return total;
+}
+
+// Pops up to group_pops entries, one per lane, and expands the popped node
+// of this lane (its child boxes against s.t): the hit leaf slots in *leaves,
+// the hit inner children in k[]/d[] by entry distance (n of them; the other
+// d[] are infinite).
+MCPT_DEV void group_expand(const BvhView& bv, const GroupLanes& g, const Trav& s, const int2* stk, int& sp, int cap,
+                           bool cull, int c[4], uint32_t& leaves, float d[4], int k[4]) {
+    const int npop = group_pops(sp, cap);
+    const int top = sp;
+    sp -= npop;
+    bool valid = false;
+    int node = 0;
+    if (g.gl < npop) {
+        const int2 e = stk[top - 1 - g.gl];
+        node = e.x;
+        valid = !cull || __int_as_float(e.y) < s.t;
+    }
+    leaves = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        c[j] = 0;
+        d[j] = __builtin_inff();
+        k[j] = j;
+    }
+    if (valid) {
+        const WideNode* wp = bv.wide + node;
+        const float4 o = wp->o;
+        const uint4 q0 = wp->q0, q1 = wp->q1;
+        const int4 ch = wp->child;
+        float tn[4];
+        const uint32_t hit = wide_boxes(o, q0, q1, s, tn);
+        c[0] = ch.x;
+        c[1] = ch.y;
+        c[2] = ch.z;
+        c[3] = ch.w;
+        const uint32_t lm = leaf_slots(ch);
+        leaves = hit & lm;
+        const uint32_t inner = hit & ~lm;
+#pragma unroll
+        for (int j = 0; j < 4; j++) d[j] = ((inner >> j) & 1u) ? tn[j] : __builtin_inff();
+        cswap(d[0], k[0], d[1], k[1]);
+        cswap(d[2], k[2], d[3], k[3]);
+        cswap(d[0], k[0], d[2], k[2]);
+        cswap(d[1], k[1], d[3], k[3]);
+        cswap(d[1], k[1], d[2], k[2]);
+    }
+}
+
+// Pushes each lane's first m children of k[]/d[], the group's nearest ones
+// last (rank-major: every lane's fourth, then third, second, nearest).
+MCPT_DEV void group_push(const GroupLanes& g, int2* stk, int& sp, int cap_phys, int m, const int c[4], const float d[4],
+                         const int k[4]) {
+#pragma unroll
+    for (int r = 3; r >= 0; r--) {
+        const uint64_t b = ballot64(m > r) & g.gm;
+        if (m > r) {
+            const int pos = sp + __popcll(b & g.below);
+            if (pos < cap_phys) stk[pos] = make_int2(pick4(c, k[r]), __float_as_int(d[r]));
+        }
+        sp += __popcll(b);
+    }
+    sp = sp < cap_phys ? sp : cap_phys;  // never reached (see the bound above)
+}
+
+// Closest hits of rays [0, total) by groups of kGroup lanes, each group taking
+// its next ray from *work. begin(q, s, bv) starts ray q in every lane of the
+// group; finish(q, s, bv) runs in the group's first lane with the hit.
+//
+// The leaves an iteration's nodes hit are tested in the next iteration, their
+// records loaded beside that iteration's nodes, so an iteration waits for one
+// round of fetches rather than a node fetch and then a leaf fetch; the boxes
+// of that iteration are culled with the best distance of the one before,
+// which only ever lets more through. lf holds two leaf lists, used in turn.
+template <class View, class Begin, class Finish>
+MCPT_DEV void group_closest(View& bv, int* work, int total, int2* stk, int* lf, int cap, int cap_phys, Begin begin,
+                            Finish finish) {
+    const GroupLanes g = group_lanes();
+    Trav s;
+    int ray = -1, sp = 0;
+    while (true) {
+        if (ray == -1) {
+            int q = 0;
+            if (g.gl == 0) q = atomicAdd(work, 1);
+            q = __shfl(q, g.g0);
+            if (q < total) {
+                begin(q, s, bv);
+                ray = q;
+                sp = 1;
+                if (g.gl == 0) stk[0] = make_int2(0, __float_as_int(0.0f));
+                wave_lds_sync();
+            } else {
+                ray = -2;
+            }
+        }
+        if (ballot64(ray >= 0) == 0) break;
+        if (ray < 0) continue;
+        int c[4], k[4];
+        float d[4];
+        uint32_t leaves;
+        group_expand(bv, g, s, stk, sp, cap, true, c, leaves, d, k);
+        wave_lds_sync();  // pops read before pushes overwrite them
+        const int nleaf = group_leaf_list(g, leaves, c, lf);
+        wave_lds_sync();
+        for (int i = g.gl; i < nleaf; i += kGroup) test_leaf_ranked(bv, s, lf[i]);
+        group_min_hit(bv, s);
+        int m = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) m += d[j] < s.t ? 1 : 0;
+        wave_lds_sync();  // leaf list read before the next iteration writes it
+        group_push(g, stk, sp, cap_phys, m, c, d, k);
+        wave_lds_sync();
+        if (sp == 0) {
+            if (g.gl == 0) finish(ray, s, bv);
+            ray = -1;
+        }
+    }
+}
+
+// Any-hit test of rays [0, total) by groups of kGroup lanes (s.t is the
+// ray's fixed limit); finish(q, blocked) runs in the group's first lane.
+// Leaves are tested one iteration late as in group_closest.
+template <class View, class Begin, class Finish>
+MCPT_DEV void group_any(View& bv, int* work, int total, int2* stk, int* lf, int cap, int cap_phys, Begin begin,
+                        Finish finish) {
+    const GroupLanes g = group_lanes();
+    Trav s;
+    int ray = -1, sp = 0;
+    while (true) {
+        if (ray == -1) {
+            int q = 0;
+            if (g.gl == 0) q = atomicAdd(work, 1);
+            q = __shfl(q, g.g0);
+            if (q < total) {
+                begin(q, s, bv);
+                ray = q;
+                sp = 1;
+                if (g.gl == 0) stk[0] = make_int2(0, __float_as_int(0.0f));
+                wave_lds_sync();
+            } else {
+                ray = -2;
+            }
+        }
+        if (ballot64(ray >= 0) == 0) break;
+        if (ray < 0) continue;
+        int c[4], k[4];
+        float d[4];
+        uint32_t leaves;
+        group_expand(bv, g, s, stk, sp, cap, false, c, leaves, d, k);
+        wave_lds_sync();
+        const int nleaf = group_leaf_list(g, leaves, c, lf);
+        wave_lds_sync();
+        bool blk = false;
+        for (int i = g.gl; i < nleaf && !blk; i += kGroup) blk = test_leaf_any(bv, s, lf[i]);
+        const bool blocked = (ballot64(blk) & g.gm) != 0;
+        int m = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) m += d[j] < __builtin_inff() ? 1 : 0;
+        wave_lds_sync();
+        if (!blocked) group_push(g, stk, sp, cap_phys, m, c, d, k);
+        wave_lds_sync();
+        if (blocked || sp == 0) {
+            if (g.gl == 0) finish(ray, blocked);
+            ray = -1;
+        }
+    }
+}
+
 }  // namespace mcpt

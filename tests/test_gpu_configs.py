@@ -117,6 +117,32 @@ def test_mesh_render(mcpt, ctx, oracle, mesh):
     scene.close()
 
 
+def test_mesh_group_walks_same_film(mcpt, ctx, mesh):
+    """Short queues walked by 16 lanes per ray (the default for queues under
+    32,768 rays, and forced for every bounce) give the film of one lane per
+    ray bit for bit; the 128x128 render above has every bounce under the
+    default threshold, so it pins the group walks against the oracle."""
+    hot, cold, mats, sky, _ = mesh
+    W = H = 512
+    spp, depth, seed = 4, 16, 9
+    print("config mesh 512x512 4 spp depth 16: group walks off / default / every bounce", flush=True)
+    scene = ctx.scene(hot, cold, mats, env=mcpt.make_env(sky))
+    scene.build()
+    cam = mcpt.camera_init(mcpt.gen_camera(mcpt.GEN_SOUP, W, H))
+    imgs, rays = [], []
+    for below in (-1, 0, 1 << 30):
+        film = ctx.film(W, H)
+        st = scene.render(cam, film, mode=mcpt.MODE_PERSISTENT, spp=spp, max_depth=depth, seed=seed,
+                          group_below=below)
+        imgs.append(film.read())
+        rays.append((st["extend_rays"], st["shadow_rays"]))
+        film.close()
+    scene.close()
+    assert rays[0] == rays[1] == rays[2]
+    for img in imgs[1:]:
+        assert np.array_equal(img.view(np.uint32), imgs[0].view(np.uint32))
+
+
 def test_batched_scenes_vs_oracle(mcpt, ctx, oracle):
     sizes = np.geomspace(1000, 500000, 8).astype(int)
     S, spp, depth, seed = 64, 8, 8, 3
