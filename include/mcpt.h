@@ -368,7 +368,9 @@ typedef struct mcpt_server_opts {
                                    of the context's free budget)            */
     int32_t policy;         /* which admitted job's paths enter the pool
                                first: MCPT_SCHED_* (0 = shortest first)     */
-    int32_t reserved;
+    int32_t group_below;    /* a closest-hit or shadow queue shorter than
+                               this is walked by 16 lanes per ray (0 =
+                               32768, < 0 never)                           */
 } mcpt_server_opts;
 
 #define MCPT_SCHED_SHORTEST_FIRST 0 /* least remaining paths to start first */

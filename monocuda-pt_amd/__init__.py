@@ -209,7 +209,7 @@ class CacheStats(C.Structure):
 class ServerOpts(C.Structure):
     _fields_ = [("path_capacity", C.c_int32), ("max_jobs", C.c_int32), ("frames_per_job", C.c_int32),
                 ("refill_lanes", C.c_int32), ("film_budget_bytes", C.c_uint64), ("policy", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("group_below", C.c_int32)]
 
 
 class Job(C.Structure):
@@ -787,13 +787,14 @@ class Server:
     film stays on the device until release()."""
 
     def __init__(self, ctx, scenes, path_capacity=0, max_jobs=0, frames_per_job=0, refill_lanes=0,
-                 film_budget_bytes=0, policy=0):
+                 film_budget_bytes=0, policy=0, group_below=0):
         self.ctx = ctx
         self.scenes = list(scenes)
         self.h = VP()
         ctx._children.add(self)
         arr = (VP * len(self.scenes))(*[sc.h for sc in self.scenes])
-        o = ServerOpts(path_capacity, max_jobs, frames_per_job, refill_lanes, film_budget_bytes, policy, 0)
+        o = ServerOpts(path_capacity, max_jobs, frames_per_job, refill_lanes, film_budget_bytes, policy,
+                       group_below)
         _check(_lib.mcpt_server_create(ctx.h, arr, len(self.scenes), C.byref(o), C.byref(self.h)),
                "mcpt_server_create")
 

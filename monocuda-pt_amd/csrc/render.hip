@@ -641,14 +641,11 @@ int blocks_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 size_t stack_bytes(const mcpt_scene_s* s) { return (size_t)kBlock * stack_entries(s->max_depth) * sizeof(int); }
 
 // Group walks (k_extend_group, k_shadow_group): a queue shorter than
-// kGroupBelow rays is walked by kGroup lanes per ray. The soft stack cap
-// lets every lane of a group pop while the stack holds up to 48 pairs.
+// kGroupBelow rays is walked by kGroup lanes per ray.
 constexpr int kGroupBelow = 32768;
-constexpr int kGroupCap = 96;
-// the group stack's bound: cap + three pending siblings per wide level + 1
-int group_stack_pairs(const mcpt_scene_s* s) { return kGroupCap + 3 * (stack_entries(s->max_depth) / 2 + 1) + 4; }
+int scene_group_pairs(const mcpt_scene_s* s) { return group_stack_pairs(stack_entries(s->max_depth)); }
 size_t group_lds_bytes(const mcpt_scene_s* s) {
-    return (size_t)(kBlock / kGroup) * group_lds_ints(group_stack_pairs(s)) * sizeof(int);
+    return (size_t)(kBlock / kGroup) * group_lds_ints(scene_group_pairs(s)) * sizeof(int);
 }
 
 // Layout of a wavefront workspace of `capacity` paths from `base`: two ray
@@ -809,7 +806,7 @@ int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
     const int group_below = o->mode != MCPT_MODE_PERSISTENT || count || ref || o->group_below < 0 ? 0
                             : o->group_below > 0                                                ? o->group_below
                                                                                                  : kGroupBelow;
-    const int gcap_phys = group_stack_pairs(s);
+    const int gcap_phys = scene_group_pairs(s);
     const size_t glds = group_lds_bytes(s);
     if (o->mode == MCPT_MODE_PERSISTENT) {
         int per_cu = 0;

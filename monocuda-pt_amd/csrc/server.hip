@@ -8,10 +8,19 @@
 // shade, shadow, then finish the paths that terminated. Paths of jobs on
 // different scenes, resolutions and sample counts sit in the same queues.
 //
+// The device schedules: the job table lives in HBM, and each step's plan
+// kernel orders the running jobs by the server's policy and hands the free
+// pool slots out as per-job quotas, with no host round trip. The host thread
+// keeps one step queued ahead of the one it reads back: it enqueues step
+// n + 1, then waits for step n's status copy (each job's progress) to retire
+// finished jobs and admit waiting ones. A step's shadow rays and finished
+// paths run on a second stream beside the next step's camera paths and
+// closest-hit launch; that step's commits and shade wait for them.
+//
 // Determinism: a job's paths are keyed by (pixel, sample, seed) like
 // mcpt_render's. A terminated path stores its clamped sample into the
 // job's frame buffer for that sample; when all W*H paths of the oldest open
-// sample are finished, a commit kernel adds the frame into the film. Films
+// sample are finished, the commit kernels add the frame into the film. Films
 // therefore receive the samples of each pixel in sample order, with the
 // same FP32 additions as mcpt_render, whatever else shares the pool.
 //
@@ -34,28 +43,54 @@ namespace mcpt {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxQuotas = 4096;
+constexpr int kPlanThreads = 1024;  // one plan thread per job slot: max_jobs <= 1024
+constexpr int kStatusRing = 4;      // status copies in flight (two are enough)
+// queues shorter than this are walked by 16 lanes per ray (group_closest,
+// group_any), as in mcpt_render
+constexpr int kSrvGroupBelow = 32768;
 
+// A job slot's record. The host writes it whole at admission; from then on
+// the device owns the progress fields.
 struct JobDev {
     mcpt_camera cam;
     float* film;    // W*H*3 linear sums
-    float* frames;  // frames_per_job * W*H*3 clamped samples awaiting commit
+    float* frames;  // nframes * W*H*3 clamped samples awaiting commit
     int scene;
     int max_depth;
     int npix;
     int nframes;
     uint64_t seed;
+    uint64_t id;   // the host's job id
+    int spp;
+    uint32_t seq;  // admission order
+    // progress
+    int active;     // from admission until every sample is in the film
+    int committed;  // samples added to the film
+    int opened;     // samples whose frame slot is in use
+    int ncommit;    // samples the last commit step added (committed - ncommit is the first)
+    int64_t gen;    // next camera path: sample * npix + pixel
 };
 
-struct Quota {  // one run of new camera paths: pixels [pixel0, pixel0+count) of `sample`
-    int job, sample, pixel0, count, prefix;
+struct Quota {  // this step's new camera paths of one job slot
+    int64_t start;  // linear index (sample * npix + pixel) of the first
+    int count;
+    int prefix;     // paths of the lower job slots this step
 };
 
 // Step counters, each in its own 128-byte line: every wave of a step takes
 // slots with atomics on them, and atomics on one line serialize in one L2
 // channel. S_ND holds the next-queue count (low 32 bits) and the done count
-// (high 32 bits) for one combined push.
-enum { S_QCOUNT = 0, S_ND = 32, S_SHADOW = 64, S_WORK = 96, S_WORK_SHADOW = 128, S_NCOUNTERS = 160 };
+// (high 32 bits) for one combined push; S_GEN the first slot and the count
+// of the step's camera paths.
+enum {
+    S_QCOUNT = 0,
+    S_ND = 32,
+    S_SHADOW = 64,
+    S_WORK = 96,
+    S_WORK_SHADOW = 128,
+    S_GEN = 160,
+    S_NCOUNTERS = 192
+};
 
 __device__ __forceinline__ unsigned long long* nd_counter(int* c) {
     return reinterpret_cast<unsigned long long*>(c + S_ND);
@@ -79,60 +114,173 @@ struct Pool {
     int* counters;
     int* frame_done;  // max_jobs * max_frames
     JobDev* jobs;
-    Quota* quotas;
+    Quota* quotas;    // max_jobs
     const SceneView* scenes;
     const BvhView* bvhs;
     int max_frames;
+    int max_jobs;
+    int capacity;
     int tb;  // 24: ray records carry scene and time (see pack_scene_time)
 };
 
 // ------------------------------------------------------------------ kernels
-__global__ void k_srv_begin(int* c) {
-    if (threadIdx.x == 0) {
-        *nd_counter(c) = 0;
-        c[S_SHADOW] = 0;
-        c[S_WORK] = 0;
-        c[S_WORK_SHADOW] = 0;
+// One block, a thread per job slot. The running jobs are ranked by policy
+// (FIFO: admission order; shortest first: fewest paths left to start, ties
+// by admission; aged: shortest first, but the oldest job first takes up to
+// half of the free slots), and the pool's free slots (capacity - paths in
+// flight after the last shade) go to them in that order, each job taking
+// the paths of its open samples not yet started (a sample is open while
+// fewer than nframes samples of the job await their commit).
+__global__ __launch_bounds__(kPlanThreads) void k_srv_plan(Pool p, int policy) {
+    __shared__ long long s_key[kPlanThreads];
+    __shared__ unsigned s_seq[kPlanThreads];
+    __shared__ int s_act[kPlanThreads];
+    __shared__ long long s_scan[kPlanThreads];
+    __shared__ int s_oldest;
+    __shared__ long long s_res;
+    const int j = threadIdx.x;
+    const int cur_count = (int)(uint32_t)*nd_counter(p.counters);  // paths the last shade kept
+    const long long free0 = p.capacity > cur_count ? p.capacity - cur_count : 0;
+    bool act = false;
+    long long want = 0, unstarted = 0;
+    unsigned seq = 0;
+    int opened = 0;
+    if (j < p.max_jobs) {
+        const JobDev& J = p.jobs[j];
+        act = J.active != 0;
+        if (act) {
+            opened = max(J.opened, min(J.spp, J.committed + J.nframes));
+            want = (long long)opened * J.npix - J.gen;
+            unstarted = (long long)J.spp * J.npix - J.gen;
+            seq = J.seq;
+        }
+    }
+    s_key[j] = policy == MCPT_SCHED_FIFO ? (long long)seq : unstarted;
+    s_seq[j] = seq;
+    s_act[j] = act ? 1 : 0;
+    if (j == 0) s_oldest = -1;
+    __syncthreads();
+    // the oldest running job, for the aged policy
+    if (policy == MCPT_SCHED_SHORTEST_FIRST_AGED && j == 0) {
+        unsigned best = 0xFFFFFFFFu;
+        for (int i = 0; i < p.max_jobs; i++)
+            if (s_act[i] && s_seq[i] < best) {
+                best = s_seq[i];
+                s_oldest = i;
+            }
+    }
+    // rank among the running jobs: (key, admission order) is a total order
+    int rank = 0;
+    if (act)
+        for (int i = 0; i < p.max_jobs; i++)
+            rank += s_act[i] && (s_key[i] < s_key[j] || (s_key[i] == s_key[j] && s_seq[i] < seq)) ? 1 : 0;
+    if (j == 0) s_res = 0;
+    __syncthreads();
+    // aged: the oldest job first takes up to half of the free slots; the
+    // rest goes out in rank order (the oldest job's remainder included)
+    const long long reserve = s_oldest == j ? min(want, free0 / 2) : 0;
+    if (s_oldest == j) s_res = reserve;
+    __syncthreads();
+    const long long free1 = free0 - s_res;
+    const long long want1 = want - reserve;
+    // inclusive scan of want1 in rank order (Hillis-Steele over the block)
+    s_scan[j] = 0;
+    __syncthreads();
+    if (act) s_scan[rank] = want1;
+    __syncthreads();
+    for (int off = 1; off < kPlanThreads; off <<= 1) {
+        const long long v = j >= off ? s_scan[j - off] : 0;
+        __syncthreads();
+        s_scan[j] += v;
+        __syncthreads();
+    }
+    long long take = 0;
+    if (act) {
+        const long long before = s_scan[rank] - want1;
+        take = reserve + max(0LL, min(want1, free1 - before));
+    }
+    __syncthreads();
+    // prefix of the takes in slot order, for the generate kernel's search
+    s_scan[j] = take;
+    __syncthreads();
+    for (int off = 1; off < kPlanThreads; off <<= 1) {
+        const long long v = j >= off ? s_scan[j - off] : 0;
+        __syncthreads();
+        s_scan[j] += v;
+        __syncthreads();
+    }
+    if (j < p.max_jobs) {
+        JobDev& J = p.jobs[j];
+        Quota q;
+        q.start = act ? J.gen : 0;
+        q.count = (int)take;
+        q.prefix = (int)(s_scan[j] - take);
+        p.quotas[j] = q;
+        if (act) {
+            J.gen += take;
+            J.opened = opened;
+        }
+    }
+    if (j == kPlanThreads - 1) {
+        const int total = (int)s_scan[kPlanThreads - 1];
+        p.counters[S_GEN] = cur_count;
+        p.counters[S_GEN + 1] = total;
+        p.counters[S_QCOUNT] = cur_count + total;
+        p.counters[S_WORK] = 0;
     }
 }
 
-// camera paths for the quota runs, appended to the current queue at
-// base + g (mcpt_render's camera kernel per path)
-__global__ __launch_bounds__(kBlock) void k_srv_generate(Pool p, int cur, int total, int nq, int base) {
-    const int g = blockIdx.x * kBlock + threadIdx.x;
-    if (g == 0) p.counters[S_QCOUNT] = base + total;
-    if (g >= total) return;
-    int lo = 0, hi = nq - 1;  // last quota with prefix <= g
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (p.quotas[mid].prefix <= g) lo = mid;
-        else hi = mid - 1;
+// camera paths of the step's quotas, appended to the current queue after
+// the paths in flight (mcpt_render's camera kernel per path)
+__global__ __launch_bounds__(kBlock) void k_srv_generate(Pool p, int cur) {
+    const int base = p.counters[S_GEN], total = p.counters[S_GEN + 1];
+    for (int g = blockIdx.x * kBlock + threadIdx.x; g < total; g += gridDim.x * kBlock) {
+        int lo = 0, hi = p.max_jobs - 1;  // last job slot with prefix <= g
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (p.quotas[mid].prefix <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        const Quota q = p.quotas[lo];
+        const JobDev& J = p.jobs[lo];
+        const int64_t L = q.start + (g - q.prefix);
+        const int sample = (int)(L / J.npix);
+        const int pixel = (int)(L - (int64_t)sample * J.npix);
+        Rng rng;
+        mcpt_pcg32_for_path(&rng.g, (uint32_t)pixel, (uint32_t)sample, J.seed);
+        vec3 o, d;
+        float time;
+        camera_ray(J.cam, pixel % J.cam.width, pixel / J.cam.width, rng, o, d, time);
+        const int slot = base + g;
+        RayItem it;
+        it.ot = make_float4(o.x, o.y, o.z, pack_scene_time(J.scene, time, p.tb));
+        it.dp = make_float4(d.x, d.y, d.z, 0.0f);
+        p.q[cur][slot] = it;
+        p.thr[cur][slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        p.smp[cur][slot] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        p.rng[cur][slot] = rng.g.state;
+        p.meta[cur][slot] = make_int4(lo, pixel, sample, 0);
     }
-    const Quota q = p.quotas[lo];
-    const JobDev& J = p.jobs[q.job];
-    const int pixel = q.pixel0 + (g - q.prefix);
-    Rng rng;
-    mcpt_pcg32_for_path(&rng.g, (uint32_t)pixel, (uint32_t)q.sample, J.seed);
-    vec3 o, d;
-    float time;
-    camera_ray(J.cam, pixel % J.cam.width, pixel / J.cam.width, rng, o, d, time);
-    const int slot = base + g;
-    RayItem it;
-    it.ot = make_float4(o.x, o.y, o.z, pack_scene_time(J.scene, time, p.tb));
-    it.dp = make_float4(d.x, d.y, d.z, 0.0f);
-    p.q[cur][slot] = it;
-    p.thr[cur][slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    p.smp[cur][slot] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-    p.rng[cur][slot] = rng.g.state;
-    p.meta[cur][slot] = make_int4(q.job, pixel, q.sample, 0);
 }
 
-// persistent extend over rays of mixed scenes with deferred leaf tests; the
-// scene index rides in the ray record (ot.w) and a lane reloads the BVH
-// view only when it changes
-__global__ __launch_bounds__(kBlock) void k_srv_extend(Pool p, int cur, int refill) {
+// the 16-byte hit record; prim_index keeps the leaf's code (triangle, or
+// ~sphere)
+__device__ __forceinline__ mcpt_packed_hit srv_hit(const Trav& s, const BvhView& v) {
+    mcpt_packed_hit h;
+    h.t = s.t;
+    h.prim_type = s.leaf >= 0 ? 1 : -1;
+    h.prim_index = s.leaf >= 0 ? v.leaf_prim[s.leaf] : -1;
+    h.did_hit = s.leaf >= 0 ? 1 : 0;
+    return h;
+}
+
+// persistent extend over rays of mixed scenes; the scene index rides in the
+// ray record (ot.w) and a lane reloads the BVH view only when it changes.
+// A queue shorter than group_below is k_srv_extend_group's.
+__global__ __launch_bounds__(kBlock) void k_srv_extend(Pool p, int cur, int refill, int group_below) {
     extern __shared__ int lds_stack[];
     const int total = p.counters[S_QCOUNT];
+    if (total < group_below) return;
     if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
     BvhView bv;
     int bv_scene = -1;  // scene whose view `bv` holds: refills rarely change it
@@ -150,90 +298,106 @@ __global__ __launch_bounds__(kBlock) void k_srv_extend(Pool p, int cur, int refi
             trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
             s.time = unpack_time(it.ot.w, p.tb);
         },
-        [&](int i, const Trav& s, const BvhView& v) {
-            mcpt_packed_hit h;  // prim_index keeps the leaf's code: triangle, or ~sphere
-            h.t = s.t;
-            h.prim_type = s.leaf >= 0 ? 1 : -1;
-            h.prim_index = s.leaf >= 0 ? v.leaf_prim[s.leaf] : -1;
-            h.did_hit = s.leaf >= 0 ? 1 : 0;
-            p.hits[i] = h;
+        [&](int i, const Trav& s, const BvhView& v) { p.hits[i] = srv_hit(s, v); }, ns, nl);
+}
+
+// a short queue: 16 lanes per ray (group_closest), each group's view that
+// of its ray's scene
+__global__ __launch_bounds__(kBlock) void k_srv_extend_group(Pool p, int cur, int group_below, int cap,
+                                                              int cap_phys) {
+    extern __shared__ int lds[];
+    const int total = p.counters[S_QCOUNT];
+    if (total >= group_below || (int)blockIdx.x * (kBlock / kGroup) >= total) return;
+    int* base = lds + (threadIdx.x / kGroup) * group_lds_ints(cap_phys);
+    const RayItem* q = p.q[cur];
+    BvhView bv;
+    group_closest(
+        bv, &p.counters[S_WORK], total, reinterpret_cast<int2*>(base), base + 2 * cap_phys, cap, cap_phys,
+        [&](int i, Trav& s, BvhView& v) {
+            const RayItem it = q[i];
+            v = p.bvhs[unpack_scene(it.ot.w, p.tb)];
+            trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+            s.time = unpack_time(it.ot.w, p.tb);
         },
-        ns, nl);
+        [&](int i, const Trav& s, const BvhView& v) { p.hits[i] = srv_hit(s, v); });
 }
 
 // shadeKernel (src/path_tracer.cu:3470-3696) per queue slot; survivors
 // move to the other queue with their state, terminated paths to the done
-// list
-// Each block's slots are shaded in shading-bin order (block_bin_sort).
+// list. Grid-stride over chunks of kShadeBlock slots (the grid is sized to
+// occupancy: the host does not know the queue's length), each chunk shaded
+// in shading-bin order (block_bin_sort).
 constexpr int kShadeBlock = 1024;
 __global__ __launch_bounds__(kShadeBlock) void k_srv_shade(Pool p, int cur) {
     __shared__ int lds[bin_sort_lds_ints<kShadeBlock>()];
     const int count = p.counters[S_QCOUNT];
-    int slot = blockIdx.x * kShadeBlock + threadIdx.x, mat = -1, bin = kShadeBins;
-    if (slot < count) {
-        const mcpt_packed_hit hit = p.hits[slot];
-        const SceneView& sv = p.scenes[unpack_scene(p.q[cur][slot].ot.w, p.tb)];
-        bin = shade_bin(sv, hit.did_hit != 0, hit.prim_index, mat);
-    }
-    block_bin_sort<kShadeBlock>(bin, slot, mat, lds);
-    const bool active = slot >= 0;
-    Bounce out;
-    out.next = false;
-    out.shadow = false;
-    bool finished = false;
-    int scene = 0;
-    float time = 0.0f;
-    PathState ps;
-    Rng rng;
-    int4 m = make_int4(0, 0, 0, 0);
-    if (active) {
-        const RayItem it = p.q[cur][slot];
-        const mcpt_packed_hit hit = p.hits[slot];
-        scene = unpack_scene(it.ot.w, p.tb);
-        time = unpack_time(it.ot.w, p.tb);
-        m = p.meta[cur][slot];
-        const JobDev& J = p.jobs[m.x];
-        const SceneView& sv = p.scenes[scene];
-        const float4 thr = p.thr[cur][slot], smp = p.smp[cur][slot];
-        ps.thr = vec3(thr.x, thr.y, thr.z);
-        ps.prev_pdf = thr.w;
-        ps.smp = vec3(smp.x, smp.y, smp.z);
-        ps.prev_spec = smp.w != 0.0f;
-        rng.g.state = p.rng[cur][slot];
-        rng.g.inc = ((uint64_t)(uint32_t)m.y << 1u) | 1u;
-        shade_bounce(sv, load3(J.cam.background), J.cam.rr_cap, m.w, vec3(it.ot.x, it.ot.y, it.ot.z),
-                     vec3(it.dp.x, it.dp.y, it.dp.z), hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out, time,
-                     mat);
-        // the reference's depth loop ends after shading bounce max_depth-1
-        if (out.next && m.w + 1 >= J.max_depth) out.next = false;
-        finished = !out.next;
-    }
-    // compacted pushes: every lane of the wave takes part
-    int nslot, dslot;
-    wave_push2(nd_counter(p.counters), out.next, finished, nslot, dslot);
-    const int sslot = wave_push(&p.counters[S_SHADOW], out.shadow);
-    const float4 smp_out = make_float4(ps.smp.x, ps.smp.y, ps.smp.z, ps.prev_spec ? 1.0f : 0.0f);
-    if (out.next) {
-        const int n = 1 - cur;
-        RayItem it;
-        it.ot = make_float4(out.no.x, out.no.y, out.no.z, pack_scene_time(scene, time, p.tb));
-        it.dp = make_float4(out.nd.x, out.nd.y, out.nd.z, 0.0f);
-        p.q[n][nslot] = it;
-        p.thr[n][nslot] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
-        p.smp[n][nslot] = smp_out;
-        p.rng[n][nslot] = rng.g.state;
-        p.meta[n][nslot] = make_int4(m.x, m.y, m.z, m.w + 1);
-    }
-    if (finished) {
-        p.done_smp[dslot] = smp_out;
-        p.done_meta[dslot] = m;
-    }
-    if (out.shadow) {
-        ShadowItem si;
-        si.om = make_float4(out.so.x, out.so.y, out.so.z, out.smax);
-        si.dp = make_float4(out.sd.x, out.sd.y, out.sd.z, __int_as_float(out.next ? nslot : ~dslot));
-        si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, pack_scene_time(scene, time, p.tb));
-        p.sq[sslot] = si;
+    for (int chunk = blockIdx.x * kShadeBlock; chunk < count; chunk += gridDim.x * kShadeBlock) {
+        int slot = chunk + (int)threadIdx.x, mat = -1, bin = kShadeBins;
+        if (slot < count) {
+            const mcpt_packed_hit hit = p.hits[slot];
+            const SceneView& sv = p.scenes[unpack_scene(p.q[cur][slot].ot.w, p.tb)];
+            bin = shade_bin(sv, hit.did_hit != 0, hit.prim_index, mat);
+        }
+        block_bin_sort<kShadeBlock>(bin, slot, mat, lds);
+        const bool active = slot >= 0;
+        Bounce out;
+        out.next = false;
+        out.shadow = false;
+        bool finished = false;
+        int scene = 0;
+        float time = 0.0f;
+        PathState ps;
+        Rng rng;
+        int4 m = make_int4(0, 0, 0, 0);
+        if (active) {
+            const RayItem it = p.q[cur][slot];
+            const mcpt_packed_hit hit = p.hits[slot];
+            scene = unpack_scene(it.ot.w, p.tb);
+            time = unpack_time(it.ot.w, p.tb);
+            m = p.meta[cur][slot];
+            const JobDev& J = p.jobs[m.x];
+            const SceneView& sv = p.scenes[scene];
+            const float4 thr = p.thr[cur][slot], smp = p.smp[cur][slot];
+            ps.thr = vec3(thr.x, thr.y, thr.z);
+            ps.prev_pdf = thr.w;
+            ps.smp = vec3(smp.x, smp.y, smp.z);
+            ps.prev_spec = smp.w != 0.0f;
+            rng.g.state = p.rng[cur][slot];
+            rng.g.inc = ((uint64_t)(uint32_t)m.y << 1u) | 1u;
+            shade_bounce(sv, load3(J.cam.background), J.cam.rr_cap, m.w, vec3(it.ot.x, it.ot.y, it.ot.z),
+                         vec3(it.dp.x, it.dp.y, it.dp.z), hit.did_hit != 0, hit.t, hit.prim_index, ps, rng, out,
+                         time, mat);
+            // the reference's depth loop ends after shading bounce max_depth-1
+            if (out.next && m.w + 1 >= J.max_depth) out.next = false;
+            finished = !out.next;
+        }
+        // compacted pushes: every lane of the wave takes part
+        int nslot, dslot;
+        wave_push2(nd_counter(p.counters), out.next, finished, nslot, dslot);
+        const int sslot = wave_push(&p.counters[S_SHADOW], out.shadow);
+        const float4 smp_out = make_float4(ps.smp.x, ps.smp.y, ps.smp.z, ps.prev_spec ? 1.0f : 0.0f);
+        if (out.next) {
+            const int n = 1 - cur;
+            RayItem it;
+            it.ot = make_float4(out.no.x, out.no.y, out.no.z, pack_scene_time(scene, time, p.tb));
+            it.dp = make_float4(out.nd.x, out.nd.y, out.nd.z, 0.0f);
+            p.q[n][nslot] = it;
+            p.thr[n][nslot] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prev_pdf);
+            p.smp[n][nslot] = smp_out;
+            p.rng[n][nslot] = rng.g.state;
+            p.meta[n][nslot] = make_int4(m.x, m.y, m.z, m.w + 1);
+        }
+        if (finished) {
+            p.done_smp[dslot] = smp_out;
+            p.done_meta[dslot] = m;
+        }
+        if (out.shadow) {
+            ShadowItem si;
+            si.om = make_float4(out.so.x, out.so.y, out.so.z, out.smax);
+            si.dp = make_float4(out.sd.x, out.sd.y, out.sd.z, __int_as_float(out.next ? nslot : ~dslot));
+            si.c = make_float4(out.contrib.x, out.contrib.y, out.contrib.z, pack_scene_time(scene, time, p.tb));
+            p.sq[sslot] = si;
+        }
     }
 }
 
@@ -253,10 +417,12 @@ __device__ __forceinline__ void srv_add_shadow(const Pool& p, int cur, const Sha
 // k_srv_extend: a lane walks the any-hit steps of its ray and refills when
 // done; a ray of a scene with a medium is handled whole in one iteration.
 // The light sample is added to the path's state wherever shade moved it.
-__global__ __launch_bounds__(kBlock) void k_srv_shadow_persistent(Pool p, int cur, int refill) {
+// A queue shorter than group_below is k_srv_shadow_group's.
+__global__ __launch_bounds__(kBlock) void k_srv_shadow_persistent(Pool p, int cur, int refill, int group_below) {
     extern __shared__ int lds_stack[];
     int* stk = lds_stack + threadIdx.x;
     const int total = p.counters[S_SHADOW];
+    if (total < group_below) return;
     if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
     const int lane = lane_id();
     Trav s;
@@ -310,63 +476,131 @@ __global__ __launch_bounds__(kBlock) void k_srv_shadow_persistent(Pool p, int cu
     }
 }
 
+// a short shadow queue: 16 lanes per ray (group_any); only for servers
+// whose scenes have no medium (group_below is 0 otherwise)
+__global__ __launch_bounds__(kBlock) void k_srv_shadow_group(Pool p, int cur, int group_below, int cap,
+                                                              int cap_phys) {
+    extern __shared__ int lds[];
+    const int total = p.counters[S_SHADOW];
+    if (total >= group_below || (int)blockIdx.x * (kBlock / kGroup) >= total) return;
+    int* base = lds + (threadIdx.x / kGroup) * group_lds_ints(cap_phys);
+    BvhView bv;
+    group_any(
+        bv, &p.counters[S_WORK_SHADOW], total, reinterpret_cast<int2*>(base), base + 2 * cap_phys, cap, cap_phys,
+        [&](int i, Trav& s, BvhView& v) {
+            const ShadowItem si = p.sq[i];
+            v = p.bvhs[unpack_scene(si.c.w, p.tb)];
+            trav_begin(s, vec3(si.om.x, si.om.y, si.om.z), vec3(si.dp.x, si.dp.y, si.dp.z), si.om.w);
+            s.time = unpack_time(si.c.w, p.tb);
+        },
+        [&](int i, bool blocked) {
+            if (!blocked) srv_add_shadow(p, cur, p.sq[i], 1.0f);
+        });
+}
+
 // terminated paths: clamped sample into the frame buffer of the path's
 // sample (finalizeSampleKernel's clamp, :3740-3755), and the count of
-// finished paths per frame. The terminated paths of a block nearly always
+// finished paths per frame. The terminated paths of a chunk nearly always
 // belong to one or two frames: wave leaders add their group counts into a
 // small LDS table and the block adds each table entry with one atomic.
+// Grid-stride over chunks of kFinishBlock paths.
 constexpr int kFinishBlock = 1024;
 constexpr int kFinishKeys = 32;
 __global__ __launch_bounds__(kFinishBlock) void k_srv_finish(Pool p) {
     __shared__ int lkey[kFinishKeys], lcnt[kFinishKeys];
-    if (threadIdx.x < kFinishKeys) {
-        lkey[threadIdx.x] = -1;
-        lcnt[threadIdx.x] = 0;
-    }
-    __syncthreads();
-    const int d = blockIdx.x * kFinishBlock + threadIdx.x;
-    const bool valid = d < (int)(*nd_counter(p.counters) >> 32);
-    int key = -1;
-    if (valid) {
-        const int4 m = p.done_meta[d];
-        const JobDev& J = p.jobs[m.x];
-        const int fslot = m.z % J.nframes;
-        const float4 s = p.done_smp[d];
-        const vec3 c = clamp_sample(vec3(s.x, s.y, s.z));
-        float* f = J.frames + ((size_t)fslot * J.npix + m.y) * 3;
-        f[0] = c.x;
-        f[1] = c.y;
-        f[2] = c.z;
-        key = m.x * p.max_frames + fslot;
-    }
-    uint64_t todo = ballot64(valid);
-    while (todo) {  // one group per distinct key in the wave
-        const int leader = __builtin_ctzll(todo);
-        const int k = __shfl(key, leader);
-        const uint64_t same = ballot64(valid && key == k);
-        if (lane_id() == leader) {
-            const int n = __popcll(same);
-            bool placed = false;
-            for (int i = 0; i < kFinishKeys && !placed; i++) {
-                const int e = (k + i) & (kFinishKeys - 1);
-                const int prev = atomicCAS(&lkey[e], -1, k);
-                if (prev == -1 || prev == k) {
-                    atomicAdd(&lcnt[e], n);
-                    placed = true;
-                }
-            }
-            if (!placed) atomicAdd(&p.frame_done[k], n);  // table full
+    const int ndone = (int)(*nd_counter(p.counters) >> 32);
+    for (int chunk = blockIdx.x * kFinishBlock; chunk < ndone; chunk += gridDim.x * kFinishBlock) {
+        if (threadIdx.x < kFinishKeys) {
+            lkey[threadIdx.x] = -1;
+            lcnt[threadIdx.x] = 0;
         }
-        todo &= ~same;
+        __syncthreads();
+        const int d = chunk + (int)threadIdx.x;
+        const bool valid = d < ndone;
+        int key = -1;
+        if (valid) {
+            const int4 m = p.done_meta[d];
+            const JobDev& J = p.jobs[m.x];
+            const int fslot = m.z % J.nframes;
+            const float4 s = p.done_smp[d];
+            const vec3 c = clamp_sample(vec3(s.x, s.y, s.z));
+            float* f = J.frames + ((size_t)fslot * J.npix + m.y) * 3;
+            f[0] = c.x;
+            f[1] = c.y;
+            f[2] = c.z;
+            key = m.x * p.max_frames + fslot;
+        }
+        uint64_t todo = ballot64(valid);
+        while (todo) {  // one group per distinct key in the wave
+            const int leader = __builtin_ctzll(todo);
+            const int k = __shfl(key, leader);
+            const uint64_t same = ballot64(valid && key == k);
+            if (lane_id() == leader) {
+                const int n = __popcll(same);
+                bool placed = false;
+                for (int i = 0; i < kFinishKeys && !placed; i++) {
+                    const int e = (k + i) & (kFinishKeys - 1);
+                    const int prev = atomicCAS(&lkey[e], -1, k);
+                    if (prev == -1 || prev == k) {
+                        atomicAdd(&lcnt[e], n);
+                        placed = true;
+                    }
+                }
+                if (!placed) atomicAdd(&p.frame_done[k], n);  // table full
+            }
+            todo &= ~same;
+        }
+        __syncthreads();
+        if (threadIdx.x < kFinishKeys && lcnt[threadIdx.x] > 0)
+            atomicAdd(&p.frame_done[lkey[threadIdx.x]], lcnt[threadIdx.x]);
+        __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x < kFinishKeys && lcnt[threadIdx.x] > 0) atomicAdd(&p.frame_done[lkey[threadIdx.x]], lcnt[threadIdx.x]);
 }
 
-// film += frame (the sample order is the host's commit order)
-__global__ __launch_bounds__(kBlock) void k_srv_commit(float* film, const float* frame, int n3) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n3) film[i] = film[i] + frame[i];
+// After the previous step's finished paths are counted: each running job's
+// oldest open samples whose every path has finished are committed (their
+// frame counters cleared, `committed` advanced, `ncommit` set for
+// k_srv_commit_apply); a job with every sample committed stops running. The
+// step's shade counters are cleared here, after the previous step's shadow
+// and finish launches have read them.
+__global__ __launch_bounds__(kPlanThreads) void k_srv_commit_plan(Pool p) {
+    const int j = threadIdx.x;
+    if (j < p.max_jobs) {
+        JobDev& J = p.jobs[j];
+        int n = 0;
+        if (J.active) {
+            int* fd = p.frame_done + j * p.max_frames;
+            while (J.committed + n < J.opened && fd[(J.committed + n) % J.nframes] == J.npix) {
+                fd[(J.committed + n) % J.nframes] = 0;
+                n++;
+            }
+            J.committed += n;
+            if (J.committed == J.spp) J.active = 0;
+        }
+        J.ncommit = n;
+    }
+    if (j == 0) {
+        *nd_counter(p.counters) = 0;
+        p.counters[S_SHADOW] = 0;
+        p.counters[S_WORK_SHADOW] = 0;
+    }
+}
+
+// film += the committed frames, in sample order (the same FP32 additions as
+// mcpt_render's finalize); grid-stride over each committing job's film
+__global__ __launch_bounds__(kBlock) void k_srv_commit_apply(Pool p) {
+    for (int j = 0; j < p.max_jobs; j++) {
+        const JobDev& J = p.jobs[j];
+        const int n = J.ncommit;
+        if (n == 0) continue;
+        const int first = J.committed - n;
+        const size_t n3 = (size_t)J.npix * 3;
+        for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n3; i += (size_t)gridDim.x * kBlock) {
+            float v = J.film[i];
+            for (int c = 0; c < n; c++) v = v + J.frames[(size_t)((first + c) % J.nframes) * n3 + i];
+            J.film[i] = v;
+        }
+    }
 }
 
 __global__ void k_srv_fill_f(float* p, size_t n, float v) {
@@ -399,26 +633,31 @@ struct JobHost {
     float* film = nullptr;
     float* frames = nullptr;
     size_t film_bytes = 0, frame_bytes = 0;
-    // progress
-    int committed = 0;  // samples added to the film
-    int opened = 0;     // samples whose frame slot is in use
-    int gen_sample = 0, gen_pixel = 0;
-    uint64_t paths = 0;
-    int64_t unstarted() const { return (int64_t)(job.spp - gen_sample) * npix - gen_pixel; }
+    uint64_t paths = 0;  // camera paths started (from the status copies)
+};
+
+// A step's status as the host reads it back: the counters after its shade,
+// then the job table.
+struct StatusSlot {
+    int* counters = nullptr;  // pinned, S_NCOUNTERS ints
+    JobDev* jobs = nullptr;   // pinned, max_jobs records
+    hipEvent_t begin = nullptr, end = nullptr;
 };
 
 struct mcpt_server_s {
     Ctx* ctx = nullptr;
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // plan, camera paths, closest hits, commits, shade
+    hipStream_t side = nullptr;    // shadow rays and finished paths
+    hipEvent_t ev_shade = nullptr, ev_fin = nullptr;
     std::chrono::steady_clock::time_point t0;
     // configuration
     int capacity = 0, max_jobs = 0, frames_opt = 0, refill = 32, max_frames = 64, policy = 0;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    int extend_blocks = 1;  // resident blocks of the persistent extend
-    std::vector<JobHost*> order;
+    int group_below = 0, shadow_group_below = 0, gcap_phys = 0;
+    int extend_blocks = 1, shadow_blocks = 1, shade_blocks = 1, finish_blocks = 1, gen_blocks = 1;
+    int group_blocks = 1, commit_blocks = 1;
     uint64_t film_budget = 0, film_in_use = 0;
-    size_t lds = 0;
+    size_t lds = 0, glds = 0;
     // device memory
     void* pool_mem = nullptr;
     size_t pool_bytes = 0;
@@ -429,9 +668,8 @@ struct mcpt_server_s {
     int max_depth_scene = 1;
     std::vector<mcpt_scene> scenes;
     // pinned staging
-    Quota* h_quotas = nullptr;
-    int* h_status = nullptr;  // counters + frame_done
-    JobDev* h_job = nullptr;
+    JobDev* h_job = nullptr;  // admission records, one per slot
+    StatusSlot ring[kStatusRing];
     // host state
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -440,18 +678,17 @@ struct mcpt_server_s {
     std::vector<uint64_t> running;  // admission order
     std::vector<int> free_slots;
     uint64_t next_id = 1;
+    uint32_t next_seq = 1;
     bool stop = false;
     int error = 0;
     std::string error_msg;
     std::thread worker;
-    // stream state
-    int cur = 0, cur_count = 0;  // current queue and its length (paths in flight)
+    int cur = 0;       // the queue the next step's rays are in
+    int64_t steps = 0; // steps enqueued
     mcpt_server_stats st{};
 };
 
 namespace {
-
-int srv_admit_locked(mcpt_server_s* s);
 
 // Samples of one job in flight. A sample's frame is committed only when its
 // slowest path has ended, while most paths end within a bounce or two, so
@@ -480,11 +717,13 @@ void srv_fail(mcpt_server_s* s, const char* what, hipError_t e) {
         hipError_t e_ = (x);                    \
         if (e_ != hipSuccess) {                 \
             srv_fail(s, #x, e_);                \
-            return;                             \
+            return false;                       \
         }                                       \
     } while (0)
 
-// admit waiting jobs in FIFO order while slots and memory allow (mu held)
+// Admits waiting jobs in FIFO order while slots and memory allow (mu held):
+// the job's record, a cleared film and cleared frame counters, in stream
+// order before any later plan kernel.
 int srv_admit_locked(mcpt_server_s* s) {
     int admitted = 0;
     while (!s->waiting.empty() && !s->free_slots.empty()) {
@@ -508,7 +747,6 @@ int srv_admit_locked(mcpt_server_s* s) {
         s->waiting.pop_front();
         s->running.push_back(j.id);
         admitted++;
-        // device record and cleared film / frame counters (stream order)
         JobDev jd;
         std::memset(&jd, 0, sizeof(jd));
         jd.cam = j.job.camera;
@@ -519,6 +757,12 @@ int srv_admit_locked(mcpt_server_s* s) {
         jd.npix = j.npix;
         jd.nframes = j.nframes;
         jd.seed = j.job.seed;
+        jd.id = j.id;
+        jd.spp = j.job.spp;
+        jd.seq = s->next_seq++;
+        jd.active = 1;
+        // the pinned record is not rewritten before this copy has run: the
+        // slot is reused only after a later status copy showed its job done
         s->h_job[j.slot] = jd;
         (void)hipMemcpyAsync(s->pool.jobs + j.slot, s->h_job + j.slot, sizeof(JobDev), hipMemcpyHostToDevice,
                              s->stream);
@@ -528,133 +772,102 @@ int srv_admit_locked(mcpt_server_s* s) {
     return admitted;
 }
 
-// one bounce step of the pool
-void srv_step(mcpt_server_s* s) {
+// Enqueues one bounce step of the pool (no host synchronisation).
+bool srv_enqueue_step(mcpt_server_s* s) {
     Pool& p = s->pool;
-    hipStream_t st = s->stream;
-    // ---- plan generation into the free part of the current queue
-    int free = s->capacity - s->cur_count;
-    int nq = 0, total = 0;
-    {
-        std::lock_guard<std::mutex> lock(s->mu);
-        std::vector<JobHost*>& order = s->order;
-        order.clear();
-        for (uint64_t id : s->running) order.push_back(&s->jobs[id]);
-        // the oldest running job (running holds admission order) and its
-        // reserve: half of this step's free slots under the aged policy
-        JobHost* oldest = order.empty() ? nullptr : order.front();
-        int reserve = s->policy == MCPT_SCHED_SHORTEST_FIRST_AGED ? free / 2 : 0;
-        if (s->policy != MCPT_SCHED_FIFO)  // least paths left to start first, ties by admission
-            std::stable_sort(order.begin(), order.end(), [](const JobHost* a, const JobHost* b) {
-                return a->unstarted() < b->unstarted();
-            });
-        if (reserve > 0 && oldest) {
-            order.erase(std::find(order.begin(), order.end(), oldest));
-            order.insert(order.begin(), oldest);
-        }
-        // new paths of job j into at most `cap` free slots, in its sample order
-        auto take = [&](JobHost& j, int cap) {
-            while (j.opened < j.job.spp && j.opened - j.committed < j.nframes) j.opened++;  // open frames
-            while (free > 0 && cap > 0 && nq < kMaxQuotas && j.gen_sample < j.opened) {
-                const int n = std::min(j.npix - j.gen_pixel, std::min(free, cap));
-                s->h_quotas[nq] = Quota{j.slot, j.gen_sample, j.gen_pixel, n, total};
-                nq++;
-                total += n;
-                free -= n;
-                cap -= n;
-                j.paths += n;
-                j.gen_pixel += n;
-                if (j.gen_pixel == j.npix) {
-                    j.gen_sample++;
-                    j.gen_pixel = 0;
-                }
-            }
-        };
-        // the oldest job first takes at most its reserve; the other slots go
-        // to the jobs in policy order, the oldest included again
-        for (JobHost* jp : order) take(*jp, (reserve > 0 && jp == oldest) ? reserve : free);
-        if (reserve > 0 && free > 0) take(*oldest, free);
-    }
-    if (s->cur_count == 0 && total == 0) return;
+    hipStream_t st = s->stream, sd = s->side;
     const int cur = s->cur;
-    hipEvent_t e0 = s->ev[0], e1 = s->ev[1];
-    SRV_HIP(hipEventRecord(e0, st));
-    k_srv_begin<<<1, 64, 0, st>>>(p.counters);
-    if (nq) SRV_HIP(hipMemcpyAsync(p.quotas, s->h_quotas, nq * sizeof(Quota), hipMemcpyHostToDevice, st));
-    k_srv_generate<<<std::max(1, blocks_for(total)), kBlock, 0, st>>>(p, cur, total, nq, s->cur_count);
-    const int inflight = s->cur_count + total;
-    const int grid = blocks_for(inflight);
-    k_srv_extend<<<std::min(grid, s->extend_blocks), kBlock, s->lds, st>>>(p, cur, s->refill);
-    k_srv_shade<<<(inflight + kShadeBlock - 1) / kShadeBlock, kShadeBlock, 0, st>>>(p, cur);
-    k_srv_shadow_persistent<<<std::min(grid, s->extend_blocks), kBlock, s->lds, st>>>(p, cur, s->refill);
-    k_srv_finish<<<(inflight + kFinishBlock - 1) / kFinishBlock, kFinishBlock, 0, st>>>(p);
+    StatusSlot& rs = s->ring[s->steps % kStatusRing];
+    SRV_HIP(hipEventRecord(rs.begin, st));
+    k_srv_plan<<<1, kPlanThreads, 0, st>>>(p, s->policy);
+    k_srv_generate<<<s->gen_blocks, kBlock, 0, st>>>(p, cur);
+    k_srv_extend<<<s->extend_blocks, kBlock, s->lds, st>>>(p, cur, s->refill, s->group_below);
+    if (s->group_below > 0)
+        k_srv_extend_group<<<s->group_blocks, kBlock, s->glds, st>>>(p, cur, s->group_below, kGroupCap,
+                                                                     s->gcap_phys);
+    // the previous step's shadow rays add to this queue's states and its
+    // finished paths fill the frames: both before the commits and the shade
+    if (s->steps > 0) SRV_HIP(hipStreamWaitEvent(st, s->ev_fin, 0));
+    k_srv_commit_plan<<<1, kPlanThreads, 0, st>>>(p);
+    k_srv_commit_apply<<<s->commit_blocks, kBlock, 0, st>>>(p);
+    k_srv_shade<<<s->shade_blocks, kShadeBlock, 0, st>>>(p, cur);
+    SRV_HIP(hipEventRecord(s->ev_shade, st));
+    SRV_HIP(hipMemcpyAsync(rs.counters, p.counters, S_NCOUNTERS * sizeof(int), hipMemcpyDeviceToHost, st));
+    SRV_HIP(hipMemcpyAsync(rs.jobs, p.jobs, s->max_jobs * sizeof(JobDev), hipMemcpyDeviceToHost, st));
+    SRV_HIP(hipEventRecord(rs.end, st));
+    SRV_HIP(hipStreamWaitEvent(sd, s->ev_shade, 0));
+    k_srv_shadow_persistent<<<s->shadow_blocks, kBlock, s->lds, sd>>>(p, cur, s->refill, s->shadow_group_below);
+    if (s->shadow_group_below > 0)
+        k_srv_shadow_group<<<s->group_blocks, kBlock, s->glds, sd>>>(p, cur, s->shadow_group_below, kGroupCap,
+                                                                     s->gcap_phys);
+    k_srv_finish<<<s->finish_blocks, kFinishBlock, 0, sd>>>(p);
+    SRV_HIP(hipEventRecord(s->ev_fin, sd));
     SRV_HIP(hipGetLastError());
-    const size_t status_bytes = (S_NCOUNTERS + (size_t)s->max_jobs * s->max_frames) * sizeof(int);
-    SRV_HIP(hipMemcpyAsync(s->h_status, p.counters, S_NCOUNTERS * sizeof(int), hipMemcpyDeviceToHost, st));
-    SRV_HIP(hipMemcpyAsync(s->h_status + S_NCOUNTERS, p.frame_done, status_bytes - S_NCOUNTERS * sizeof(int),
-                           hipMemcpyDeviceToHost, st));
-    SRV_HIP(hipEventRecord(e1, st));
-    SRV_HIP(hipEventSynchronize(e1));
+    s->cur = 1 - cur;
+    s->steps++;
+    return true;
+}
+
+// Reads step `n`'s status back (waiting for it): statistics, the jobs that
+// finished (released to their callers, their slots to waiting jobs).
+bool srv_consume(mcpt_server_s* s, int64_t n) {
+    StatusSlot& rs = s->ring[n % kStatusRing];
+    SRV_HIP(hipEventSynchronize(rs.end));
     float busy = 0.f;
-    SRV_HIP(hipEventElapsedTime(&busy, e0, e1));
-    const int next_count = s->h_status[S_ND];  // low half of the combined next/done counter
-    // ---- bookkeeping
-    std::vector<uint64_t> finished;
-    {
-        std::lock_guard<std::mutex> lock(s->mu);
-        s->st.iterations++;
-        s->st.busy_ms += busy;
-        s->st.paths_started += total;
-        s->st.extend_rays += inflight;
-        s->st.shadow_rays += s->h_status[S_SHADOW];
-        s->cur_count = next_count;
-        s->cur = 1 - cur;
-        const int* fd = s->h_status + S_NCOUNTERS;
-        for (uint64_t id : s->running) {
-            JobHost& j = s->jobs[id];
-            // commit the oldest open samples whose every path has finished
-            while (j.committed < j.opened) {
-                const int fslot = j.committed % j.nframes;
-                if (fd[j.slot * s->max_frames + fslot] < j.npix) break;
-                k_srv_commit<<<blocks_for((int64_t)j.npix * 3), kBlock, 0, st>>>(
-                    j.film, j.frames + (size_t)fslot * j.npix * 3, j.npix * 3);
-                k_srv_fill_i<<<1, 64, 0, st>>>(p.frame_done + j.slot * s->max_frames + fslot, 1, 0);
-                j.committed++;
-            }
-            if (j.committed == j.job.spp) finished.push_back(id);
-        }
-    }
-    if (!finished.empty()) {
-        SRV_HIP(hipStreamSynchronize(st));
-        std::lock_guard<std::mutex> lock(s->mu);
-        for (uint64_t id : finished) {
-            JobHost& j = s->jobs[id];
+    SRV_HIP(hipEventElapsedTime(&busy, rs.begin, rs.end));
+    std::lock_guard<std::mutex> lock(s->mu);
+    s->st.iterations++;
+    s->st.busy_ms += busy;
+    s->st.paths_started += (uint64_t)rs.counters[S_GEN + 1];
+    s->st.extend_rays += (uint64_t)rs.counters[S_QCOUNT];
+    s->st.shadow_rays += (uint64_t)rs.counters[S_SHADOW];
+    bool any = false;
+    for (size_t r = 0; r < s->running.size();) {
+        JobHost& j = s->jobs[s->running[r]];
+        const JobDev& d = rs.jobs[j.slot];
+        if (d.id == j.id) j.paths = (uint64_t)d.gen;  // the slot's record is this job's
+        if (d.id == j.id && !d.active) {
             j.status = MCPT_OK;
             j.finish_ms = now_ms(s->t0);
             s->free_slots.push_back(j.slot);
             j.slot = -1;
-            s->running.erase(std::find(s->running.begin(), s->running.end(), id));
+            s->running.erase(s->running.begin() + r);
             s->st.jobs_done++;
+            any = true;
+        } else {
+            r++;
         }
+    }
+    if (any) {
         srv_admit_locked(s);
         s->cv_done.notify_all();
     }
+    return true;
 }
 
+// The worker: while jobs run, one step is enqueued ahead of the one read
+// back, so the device never waits for the host's bookkeeping.
 void srv_loop(mcpt_server_s* s) {
     if (hipSetDevice(s->device) != hipSuccess) {
         srv_fail(s, "hipSetDevice", hipErrorInvalidDevice);
         return;
     }
+    int64_t outstanding = -1;  // a step enqueued and not yet read back
     while (true) {
+        bool run = false;
         {
             std::unique_lock<std::mutex> lock(s->mu);
             srv_admit_locked(s);
-            s->cv_work.wait(lock, [&] { return s->stop || !s->running.empty(); });
+            s->cv_work.wait(lock, [&] { return s->stop || !s->running.empty() || outstanding >= 0; });
             if (s->stop) break;
+            run = !s->running.empty();
         }
-        srv_step(s);
+        const int64_t n = s->steps;
+        if (run && !srv_enqueue_step(s)) return;
+        if (outstanding >= 0 && !srv_consume(s, outstanding)) return;
+        outstanding = run ? n : -1;
     }
+    if (outstanding >= 0) (void)hipEventSynchronize(s->ring[outstanding % kStatusRing].end);
 }
 
 }  // namespace
@@ -670,10 +883,18 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
         if (!scenes[i]->built) return set_error(MCPT_ERR_INVALID, "scene %d has no BVH", i);
         MCPT_TRY(scenes[i]->check_textures());
     }
-    bool moving = false;
-    for (int i = 0; i < n_scenes; i++) moving = moving || scenes[i]->moving_spheres;
+    bool moving = false, medium = false;
+    for (int i = 0; i < n_scenes; i++) {
+        moving = moving || scenes[i]->moving_spheres;
+        medium = medium || scenes[i]->has_medium;
+    }
     if (moving && n_scenes > 256)
         return set_error(MCPT_ERR_INVALID, "a server with moving spheres takes at most 256 scenes");
+    const int max_jobs = (opts && opts->max_jobs > 0) ? opts->max_jobs : 64;
+    if (max_jobs > kPlanThreads) return set_error(MCPT_ERR_INVALID, "max_jobs %d above %d", max_jobs, kPlanThreads);
+    const int policy = opts ? opts->policy : MCPT_SCHED_SHORTEST_FIRST;
+    if (policy != MCPT_SCHED_SHORTEST_FIRST && policy != MCPT_SCHED_FIFO && policy != MCPT_SCHED_SHORTEST_FIRST_AGED)
+        return set_error(MCPT_ERR_INVALID, "unknown scheduling policy %d", policy);
     Ctx* c = &ctx->c;
     MCPT_HIP(hipSetDevice(c->device));
     auto* s = new mcpt_server_s();
@@ -681,19 +902,18 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
     s->device = c->device;
     s->t0 = std::chrono::steady_clock::now();
     s->capacity = (opts && opts->path_capacity > 0) ? opts->path_capacity : (8 << 20);
-    s->max_jobs = (opts && opts->max_jobs > 0) ? opts->max_jobs : 64;
+    s->max_jobs = max_jobs;
     s->frames_opt = opts ? opts->frames_per_job : 0;
     s->refill = (opts && opts->refill_lanes > 0) ? std::min(64, opts->refill_lanes) : 32;
-    s->policy = opts ? opts->policy : MCPT_SCHED_SHORTEST_FIRST;
-    if (s->policy != MCPT_SCHED_SHORTEST_FIRST && s->policy != MCPT_SCHED_FIFO &&
-        s->policy != MCPT_SCHED_SHORTEST_FIRST_AGED) {
-        delete s;
-        return set_error(MCPT_ERR_INVALID, "unknown scheduling policy %d", opts->policy);
-    }
+    s->policy = policy;
+    s->group_below = !opts || opts->group_below == 0 ? kSrvGroupBelow : std::max(0, opts->group_below);
+    s->shadow_group_below = medium ? 0 : s->group_below;  // ratio tracking walks a medium ray whole
     s->n_scenes = n_scenes;
     s->scenes.assign(scenes, scenes + n_scenes);
     for (int i = 0; i < n_scenes; i++) s->max_depth_scene = std::max(s->max_depth_scene, scenes[i]->max_depth);
     s->lds = (size_t)kBlock * stack_entries(s->max_depth_scene) * sizeof(int);
+    s->gcap_phys = group_stack_pairs(stack_entries(s->max_depth_scene));
+    s->glds = (size_t)(kBlock / kGroup) * group_lds_ints(s->gcap_phys) * sizeof(int);
     // device memory: pool + tables
     const size_t C = s->capacity;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -703,7 +923,7 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
                             al(C * sizeof(int4)),       al(C * sizeof(mcpt_packed_hit)), al(C * sizeof(ShadowItem)),
                             al(C * sizeof(float4)),     al(C * sizeof(int4)),     al(S_NCOUNTERS * sizeof(int)),
                             al((size_t)s->max_jobs * s->max_frames * sizeof(int)), al(s->max_jobs * sizeof(JobDev)),
-                            al(kMaxQuotas * sizeof(Quota)), al(n_scenes * sizeof(SceneView)),
+                            al(s->max_jobs * sizeof(Quota)), al(n_scenes * sizeof(SceneView)),
                             al(n_scenes * sizeof(BvhView))};
     size_t total = 0;
     for (size_t x : parts) total += x;
@@ -740,13 +960,36 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
     p.scenes = s->scene_tab;
     p.bvhs = s->bvh_tab;
     p.max_frames = s->max_frames;
+    p.max_jobs = s->max_jobs;
+    p.capacity = s->capacity;
     p.tb = moving ? 24 : 0;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreate(&s->ev[0]);
-    if (e == hipSuccess) e = hipEventCreate(&s->ev[1]);
-    int per_cu = 0;
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_srv_extend, kBlock, s->lds);
-    s->extend_blocks = std::max(1, per_cu) * c->cu_count;
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_shade, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_fin, hipEventDisableTiming);
+    for (int r = 0; r < kStatusRing && e == hipSuccess; r++) {
+        e = hipEventCreate(&s->ring[r].begin);
+        if (e == hipSuccess) e = hipEventCreate(&s->ring[r].end);
+        if (e == hipSuccess)
+            e = hipHostMalloc((void**)&s->ring[r].counters, S_NCOUNTERS * sizeof(int), hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipHostMalloc((void**)&s->ring[r].jobs, s->max_jobs * sizeof(JobDev), hipHostMallocDefault);
+    }
+    // grids: the persistent and grid-stride kernels run on the occupancy
+    // limit (the host does not know a step's queue lengths)
+    auto per_cu = [&](auto f, int threads, size_t lds) {
+        int n = 0;
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, threads, lds);
+        return std::max(1, n) * c->cu_count;
+    };
+    s->extend_blocks = per_cu(k_srv_extend, kBlock, s->lds);
+    s->shadow_blocks = per_cu(k_srv_shadow_persistent, kBlock, s->lds);
+    s->shade_blocks = per_cu(k_srv_shade, kShadeBlock, 0);
+    s->finish_blocks = per_cu(k_srv_finish, kFinishBlock, 0);
+    s->gen_blocks = per_cu(k_srv_generate, kBlock, 0);
+    s->group_blocks = std::min(per_cu(k_srv_extend_group, kBlock, s->glds),
+                               (s->group_below + kBlock / kGroup - 1) / (kBlock / kGroup) + 1);
+    s->commit_blocks = 2 * c->cu_count;
     std::vector<SceneView> sv(n_scenes);
     std::vector<BvhView> bv(n_scenes);
     for (int i = 0; i < n_scenes; i++) {
@@ -755,23 +998,27 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
     }
     if (e == hipSuccess) e = hipMemcpy(s->scene_tab, sv.data(), n_scenes * sizeof(SceneView), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->bvh_tab, bv.data(), n_scenes * sizeof(BvhView), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_quotas, kMaxQuotas * sizeof(Quota), hipHostMallocDefault);
-    if (e == hipSuccess)
-        e = hipHostMalloc((void**)&s->h_status, (S_NCOUNTERS + (size_t)s->max_jobs * s->max_frames) * sizeof(int),
-                          hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->h_job, s->max_jobs * sizeof(JobDev), hipHostMallocDefault);
     if (e == hipSuccess) {
-        k_srv_fill_i<<<1, 64, 0, s->stream>>>(p.counters, S_NCOUNTERS, 0);
+        // counters, frame counters and the job table (every slot inactive)
+        k_srv_fill_i<<<1, kBlock, 0, s->stream>>>(p.counters, S_NCOUNTERS, 0);
         k_srv_fill_i<<<blocks_for(s->max_jobs * s->max_frames), kBlock, 0, s->stream>>>(p.frame_done,
                                                                                          s->max_jobs * s->max_frames, 0);
+        const int job_ints = (int)(s->max_jobs * sizeof(JobDev) / sizeof(int));
+        k_srv_fill_i<<<blocks_for(job_ints), kBlock, 0, s->stream>>>((int*)p.jobs, job_ints, 0);
         e = hipStreamSynchronize(s->stream);
     }
     if (e != hipSuccess) {
-        if (s->h_quotas) (void)hipHostFree(s->h_quotas);
-        if (s->h_status) (void)hipHostFree(s->h_status);
+        for (StatusSlot& r : s->ring) {
+            if (r.counters) (void)hipHostFree(r.counters);
+            if (r.jobs) (void)hipHostFree(r.jobs);
+            if (r.begin) (void)hipEventDestroy(r.begin);
+            if (r.end) (void)hipEventDestroy(r.end);
+        }
         if (s->h_job) (void)hipHostFree(s->h_job);
-        for (hipEvent_t ev : s->ev)
-            if (ev) (void)hipEventDestroy(ev);
+        if (s->ev_shade) (void)hipEventDestroy(s->ev_shade);
+        if (s->ev_fin) (void)hipEventDestroy(s->ev_fin);
+        if (s->side) (void)hipStreamDestroy(s->side);
         if (s->stream) (void)hipStreamDestroy(s->stream);
         dev_free(c, s->pool_mem, s->pool_bytes);
         delete s;
@@ -903,17 +1150,23 @@ int mcpt_server_destroy(mcpt_server s) {
     if (s->worker.joinable()) s->worker.join();
     (void)hipSetDevice(s->device);
     (void)hipStreamSynchronize(s->stream);
+    (void)hipStreamSynchronize(s->side);
     for (auto& kv : s->jobs) {
         if (kv.second.film) {
             dev_free(s->ctx, kv.second.film, kv.second.film_bytes);
             dev_free(s->ctx, kv.second.frames, kv.second.frame_bytes);
         }
     }
-    (void)hipHostFree(s->h_quotas);
-    (void)hipHostFree(s->h_status);
+    for (StatusSlot& r : s->ring) {
+        (void)hipHostFree(r.counters);
+        (void)hipHostFree(r.jobs);
+        (void)hipEventDestroy(r.begin);
+        (void)hipEventDestroy(r.end);
+    }
     (void)hipHostFree(s->h_job);
-    (void)hipEventDestroy(s->ev[0]);
-    (void)hipEventDestroy(s->ev[1]);
+    (void)hipEventDestroy(s->ev_shade);
+    (void)hipEventDestroy(s->ev_fin);
+    (void)hipStreamDestroy(s->side);
     (void)hipStreamDestroy(s->stream);
     dev_free(s->ctx, s->pool_mem, s->pool_bytes);
     for (mcpt_scene sc : s->scenes) sc->servers--;

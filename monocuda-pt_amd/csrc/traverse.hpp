@@ -515,6 +515,13 @@ constexpr int kGroup = 16;
 // the leaf list of one iteration (at most four per popped node)
 __host__ __device__ constexpr int group_lds_ints(int cap_phys) { return 2 * cap_phys + 4 * kGroup; }
 
+// The soft cap of a group's stack (every lane of a group pops while the
+// stack holds up to 48 pairs), and the stack's bound for a tree whose
+// single-lane walk takes stack_ints ints (two per wide level): the cap,
+// three pending siblings per wide level, one more
+constexpr int kGroupCap = 96;
+__host__ __device__ constexpr int group_stack_pairs(int stack_ints) { return kGroupCap + 3 * (stack_ints / 2 + 1) + 4; }
+
 MCPT_DEV int group_pops(int sp, int cap) {
     int n = (cap - sp) / 3;
     n = n < 1 ? 1 : (n > kGroup ? kGroup : n);
