@@ -43,7 +43,7 @@ namespace mcpt {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kPlanThreads = 1024;  // one plan thread per job slot: max_jobs <= 1024
+constexpr int kMaxJobs = 1024;      // job slots of one server at most
 constexpr int kStatusRing = 4;      // status copies in flight (two are enough)
 // queues shorter than this are walked by 16 lanes per ray (group_closest,
 // group_any), as in mcpt_render
@@ -124,111 +124,131 @@ struct Pool {
 };
 
 // ------------------------------------------------------------------ kernels
-// One block, a thread per job slot. The running jobs are ranked by policy
-// (FIFO: admission order; shortest first: fewest paths left to start, ties
-// by admission; aged: shortest first, but the oldest job first takes up to
-// half of the free slots), and the pool's free slots (capacity - paths in
-// flight after the last shade) go to them in that order, each job taking
-// the paths of its open samples not yet started (a sample is open while
-// fewer than nframes samples of the job await their commit).
-__global__ __launch_bounds__(kPlanThreads) void k_srv_plan(Pool p, int policy) {
-    __shared__ long long s_key[kPlanThreads];
-    __shared__ unsigned s_seq[kPlanThreads];
-    __shared__ int s_act[kPlanThreads];
-    __shared__ long long s_scan[kPlanThreads];
-    __shared__ int s_oldest;
-    __shared__ long long s_res;
-    const int j = threadIdx.x;
+// wave-wide inclusive prefix sum
+__device__ __forceinline__ long long wave_incl_scan(long long v) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const long long u = __shfl_up(v, off);
+        if (lane_id() >= off) v += u;
+    }
+    return v;
+}
+
+// in-place inclusive prefix sum of a[0, n) by one wave
+__device__ void wave_scan_array(long long* a, int n) {
+    long long carry = 0;
+    for (int base = 0; base < n; base += 64) {
+        const int i = base + lane_id();
+        const long long incl = wave_incl_scan(i < n ? a[i] : 0);
+        if (i < n) a[i] = carry + incl;
+        carry += __shfl(incl, 63);
+    }
+}
+
+// One wave (small, so that it finds room beside the other stream's
+// persistent launch), job slots j = lane + 64k. The running jobs are ranked
+// by policy (FIFO: admission order; shortest first: fewest paths left to
+// start, ties by admission; aged: shortest first, but the oldest job first
+// takes up to half of the free slots), and the pool's free slots (capacity -
+// paths in flight after the last shade) go to them in that order, each job
+// taking the paths of its open samples not yet started (a sample is open
+// while fewer than nframes samples of the job await their commit). LDS:
+// plan_lds_bytes(max_jobs).
+__global__ __launch_bounds__(64) void k_srv_plan(Pool p, int policy) {
+    extern __shared__ long long lds_plan[];
+    const int nj = p.max_jobs;
+    long long* s_key = lds_plan;           // [nj] policy key
+    long long* s_want = lds_plan + nj;     // [nj] paths wanted, by rank; then their prefix sums
+    long long* s_take = lds_plan + 2 * nj; // [nj] paths taken, by slot; then their prefix sums
+    unsigned* s_seq = reinterpret_cast<unsigned*>(lds_plan + 3 * nj);  // [nj] admission order, 0 = not running
+    const int lane = lane_id();
     const int cur_count = (int)(uint32_t)*nd_counter(p.counters);  // paths the last shade kept
     const long long free0 = p.capacity > cur_count ? p.capacity - cur_count : 0;
-    bool act = false;
-    long long want = 0, unstarted = 0;
-    unsigned seq = 0;
-    int opened = 0;
-    if (j < p.max_jobs) {
+    for (int j = lane; j < nj; j += 64) {
         const JobDev& J = p.jobs[j];
-        act = J.active != 0;
-        if (act) {
-            opened = max(J.opened, min(J.spp, J.committed + J.nframes));
-            want = (long long)opened * J.npix - J.gen;
-            unstarted = (long long)J.spp * J.npix - J.gen;
+        long long key = 0;
+        unsigned seq = 0;
+        if (J.active) {
             seq = J.seq;
+            key = policy == MCPT_SCHED_FIFO ? (long long)seq : (long long)J.spp * J.npix - J.gen;
         }
+        s_key[j] = key;
+        s_seq[j] = seq;
+        s_want[j] = 0;
     }
-    s_key[j] = policy == MCPT_SCHED_FIFO ? (long long)seq : unstarted;
-    s_seq[j] = seq;
-    s_act[j] = act ? 1 : 0;
-    if (j == 0) s_oldest = -1;
     __syncthreads();
     // the oldest running job, for the aged policy
-    if (policy == MCPT_SCHED_SHORTEST_FIRST_AGED && j == 0) {
+    int oldest = -1;
+    if (policy == MCPT_SCHED_SHORTEST_FIRST_AGED) {
         unsigned best = 0xFFFFFFFFu;
-        for (int i = 0; i < p.max_jobs; i++)
-            if (s_act[i] && s_seq[i] < best) {
+        for (int i = 0; i < nj; i++)
+            if (s_seq[i] && s_seq[i] < best) {
                 best = s_seq[i];
-                s_oldest = i;
+                oldest = i;
             }
     }
-    // rank among the running jobs: (key, admission order) is a total order
-    int rank = 0;
-    if (act)
-        for (int i = 0; i < p.max_jobs; i++)
-            rank += s_act[i] && (s_key[i] < s_key[j] || (s_key[i] == s_key[j] && s_seq[i] < seq)) ? 1 : 0;
-    if (j == 0) s_res = 0;
-    __syncthreads();
-    // aged: the oldest job first takes up to half of the free slots; the
-    // rest goes out in rank order (the oldest job's remainder included)
-    const long long reserve = s_oldest == j ? min(want, free0 / 2) : 0;
-    if (s_oldest == j) s_res = reserve;
-    __syncthreads();
-    const long long free1 = free0 - s_res;
-    const long long want1 = want - reserve;
-    // inclusive scan of want1 in rank order (Hillis-Steele over the block)
-    s_scan[j] = 0;
-    __syncthreads();
-    if (act) s_scan[rank] = want1;
-    __syncthreads();
-    for (int off = 1; off < kPlanThreads; off <<= 1) {
-        const long long v = j >= off ? s_scan[j - off] : 0;
-        __syncthreads();
-        s_scan[j] += v;
-        __syncthreads();
+    // wants, in rank order; (key, admission order) is a total order
+    long long reserve = 0;
+    for (int j = lane; j < nj; j += 64) {
+        if (!s_seq[j]) continue;
+        const JobDev& J = p.jobs[j];
+        const int opened = max(J.opened, min(J.spp, J.committed + J.nframes));
+        const long long want = (long long)opened * J.npix - J.gen;
+        int rank = 0;
+        for (int i = 0; i < nj; i++)
+            rank += s_seq[i] && (s_key[i] < s_key[j] || (s_key[i] == s_key[j] && s_seq[i] < s_seq[j])) ? 1 : 0;
+        const long long r = j == oldest ? min(want, free0 / 2) : 0;
+        if (j == oldest) reserve = r;
+        s_want[rank] = want - r;
     }
-    long long take = 0;
-    if (act) {
-        const long long before = s_scan[rank] - want1;
-        take = reserve + max(0LL, min(want1, free1 - before));
+    // the oldest job's first share (aged), then the rest in rank order
+    for (int off = 32; off > 0; off >>= 1) reserve += __shfl_xor(reserve, off);
+    const long long free1 = free0 - reserve;
+    __syncthreads();
+    wave_scan_array(s_want, nj);
+    __syncthreads();
+    for (int j = lane; j < nj; j += 64) {
+        long long take = 0;
+        if (s_seq[j]) {
+            const JobDev& J = p.jobs[j];
+            const int opened = max(J.opened, min(J.spp, J.committed + J.nframes));
+            const long long want = (long long)opened * J.npix - J.gen;
+            int rank = 0;
+            for (int i = 0; i < nj; i++)
+                rank += s_seq[i] && (s_key[i] < s_key[j] || (s_key[i] == s_key[j] && s_seq[i] < s_seq[j])) ? 1 : 0;
+            const long long r = j == oldest ? min(want, free0 / 2) : 0;
+            const long long want1 = want - r;
+            const long long before = s_want[rank] - want1;
+            take = r + max(0LL, min(want1, free1 - before));
+        }
+        s_take[j] = take;
     }
     __syncthreads();
-    // prefix of the takes in slot order, for the generate kernel's search
-    s_scan[j] = take;
+    wave_scan_array(s_take, nj);
     __syncthreads();
-    for (int off = 1; off < kPlanThreads; off <<= 1) {
-        const long long v = j >= off ? s_scan[j - off] : 0;
-        __syncthreads();
-        s_scan[j] += v;
-        __syncthreads();
-    }
-    if (j < p.max_jobs) {
+    for (int j = lane; j < nj; j += 64) {
         JobDev& J = p.jobs[j];
+        const long long incl = s_take[j], take = incl - (j ? s_take[j - 1] : 0);
         Quota q;
-        q.start = act ? J.gen : 0;
+        q.start = s_seq[j] ? J.gen : 0;
         q.count = (int)take;
-        q.prefix = (int)(s_scan[j] - take);
+        q.prefix = (int)(incl - take);
         p.quotas[j] = q;
-        if (act) {
+        if (s_seq[j]) {
             J.gen += take;
-            J.opened = opened;
+            J.opened = max(J.opened, min(J.spp, J.committed + J.nframes));
         }
     }
-    if (j == kPlanThreads - 1) {
-        const int total = (int)s_scan[kPlanThreads - 1];
+    if (lane == 0) {
+        const int total = (int)s_take[nj - 1];
         p.counters[S_GEN] = cur_count;
         p.counters[S_GEN + 1] = total;
         p.counters[S_QCOUNT] = cur_count + total;
         p.counters[S_WORK] = 0;
     }
 }
+
+int plan_lds_bytes(int max_jobs) { return max_jobs * (3 * (int)sizeof(long long) + (int)sizeof(unsigned)); }
 
 // camera paths of the step's quotas, appended to the current queue after
 // the paths in flight (mcpt_render's camera kernel per path)
@@ -563,7 +583,7 @@ __global__ __launch_bounds__(kFinishBlock) void k_srv_finish(Pool p) {
 // k_srv_commit_apply); a job with every sample committed stops running. The
 // step's shade counters are cleared here, after the previous step's shadow
 // and finish launches have read them.
-__global__ __launch_bounds__(kPlanThreads) void k_srv_commit_plan(Pool p) {
+__global__ __launch_bounds__(kMaxJobs) void k_srv_commit_plan(Pool p) {
     const int j = threadIdx.x;
     if (j < p.max_jobs) {
         JobDev& J = p.jobs[j];
@@ -685,6 +705,11 @@ struct mcpt_server_s {
     std::thread worker;
     int cur = 0;       // the queue the next step's rays are in
     int64_t steps = 0; // steps enqueued
+    // the closest-hit and shadow queue lengths of the last step read back
+    // (the worker's own): the group kernels are launched only when these
+    // were short, since a launch that finds a long queue still waits for
+    // room beside the other stream's persistent kernel
+    int last_rays = 0, last_shadow = 0;
     mcpt_server_stats st{};
 };
 
@@ -779,16 +804,19 @@ bool srv_enqueue_step(mcpt_server_s* s) {
     const int cur = s->cur;
     StatusSlot& rs = s->ring[s->steps % kStatusRing];
     SRV_HIP(hipEventRecord(rs.begin, st));
-    k_srv_plan<<<1, kPlanThreads, 0, st>>>(p, s->policy);
+    k_srv_plan<<<1, 64, plan_lds_bytes(s->max_jobs), st>>>(p, s->policy);
     k_srv_generate<<<s->gen_blocks, kBlock, 0, st>>>(p, cur);
-    k_srv_extend<<<s->extend_blocks, kBlock, s->lds, st>>>(p, cur, s->refill, s->group_below);
-    if (s->group_below > 0)
-        k_srv_extend_group<<<s->group_blocks, kBlock, s->glds, st>>>(p, cur, s->group_below, kGroupCap,
-                                                                     s->gcap_phys);
+    // a queue shorter than gb goes to the group walk; the persistent
+    // kernel takes every queue when no group launch follows
+    const int gb = s->last_rays < 2 * s->group_below ? s->group_below : 0;
+    const int gbs = s->last_shadow < 2 * s->shadow_group_below ? s->shadow_group_below : 0;
+    k_srv_extend<<<s->extend_blocks, kBlock, s->lds, st>>>(p, cur, s->refill, gb);
+    if (gb > 0)
+        k_srv_extend_group<<<s->group_blocks, kBlock, s->glds, st>>>(p, cur, gb, kGroupCap, s->gcap_phys);
     // the previous step's shadow rays add to this queue's states and its
     // finished paths fill the frames: both before the commits and the shade
     if (s->steps > 0) SRV_HIP(hipStreamWaitEvent(st, s->ev_fin, 0));
-    k_srv_commit_plan<<<1, kPlanThreads, 0, st>>>(p);
+    k_srv_commit_plan<<<1, (s->max_jobs + 63) / 64 * 64, 0, st>>>(p);
     k_srv_commit_apply<<<s->commit_blocks, kBlock, 0, st>>>(p);
     k_srv_shade<<<s->shade_blocks, kShadeBlock, 0, st>>>(p, cur);
     SRV_HIP(hipEventRecord(s->ev_shade, st));
@@ -796,10 +824,9 @@ bool srv_enqueue_step(mcpt_server_s* s) {
     SRV_HIP(hipMemcpyAsync(rs.jobs, p.jobs, s->max_jobs * sizeof(JobDev), hipMemcpyDeviceToHost, st));
     SRV_HIP(hipEventRecord(rs.end, st));
     SRV_HIP(hipStreamWaitEvent(sd, s->ev_shade, 0));
-    k_srv_shadow_persistent<<<s->shadow_blocks, kBlock, s->lds, sd>>>(p, cur, s->refill, s->shadow_group_below);
-    if (s->shadow_group_below > 0)
-        k_srv_shadow_group<<<s->group_blocks, kBlock, s->glds, sd>>>(p, cur, s->shadow_group_below, kGroupCap,
-                                                                     s->gcap_phys);
+    k_srv_shadow_persistent<<<s->shadow_blocks, kBlock, s->lds, sd>>>(p, cur, s->refill, gbs);
+    if (gbs > 0)
+        k_srv_shadow_group<<<s->group_blocks, kBlock, s->glds, sd>>>(p, cur, gbs, kGroupCap, s->gcap_phys);
     k_srv_finish<<<s->finish_blocks, kFinishBlock, 0, sd>>>(p);
     SRV_HIP(hipEventRecord(s->ev_fin, sd));
     SRV_HIP(hipGetLastError());
@@ -821,6 +848,8 @@ bool srv_consume(mcpt_server_s* s, int64_t n) {
     s->st.paths_started += (uint64_t)rs.counters[S_GEN + 1];
     s->st.extend_rays += (uint64_t)rs.counters[S_QCOUNT];
     s->st.shadow_rays += (uint64_t)rs.counters[S_SHADOW];
+    s->last_rays = rs.counters[S_QCOUNT];
+    s->last_shadow = rs.counters[S_SHADOW];
     bool any = false;
     for (size_t r = 0; r < s->running.size();) {
         JobHost& j = s->jobs[s->running[r]];
@@ -891,7 +920,7 @@ int mcpt_server_create(mcpt_ctx ctx, const mcpt_scene* scenes, int32_t n_scenes,
     if (moving && n_scenes > 256)
         return set_error(MCPT_ERR_INVALID, "a server with moving spheres takes at most 256 scenes");
     const int max_jobs = (opts && opts->max_jobs > 0) ? opts->max_jobs : 64;
-    if (max_jobs > kPlanThreads) return set_error(MCPT_ERR_INVALID, "max_jobs %d above %d", max_jobs, kPlanThreads);
+    if (max_jobs > kMaxJobs) return set_error(MCPT_ERR_INVALID, "max_jobs %d above %d", max_jobs, kMaxJobs);
     const int policy = opts ? opts->policy : MCPT_SCHED_SHORTEST_FIRST;
     if (policy != MCPT_SCHED_SHORTEST_FIRST && policy != MCPT_SCHED_FIFO && policy != MCPT_SCHED_SHORTEST_FIRST_AGED)
         return set_error(MCPT_ERR_INVALID, "unknown scheduling policy %d", policy);
