@@ -372,9 +372,23 @@ __global__ __launch_bounds__(kBlock) void k_bb_camera(const BScene* tab, int nsc
 // persistent extend over the rays of every scene (extendKernelPersistent,
 // :3416-3468) with deferred leaf tests; a lane reloads the BVH view only
 // when its scene changes
-__global__ __launch_bounds__(kBlock) void k_bb_extend(const BvhView* bvhs, BWs ws, int cur, int refill) {
+// the 16-byte hit record; prim_index keeps the leaf's code (triangle, or
+// ~sphere)
+__device__ __forceinline__ mcpt_packed_hit bb_hit(const Trav& s, const BvhView& v) {
+    mcpt_packed_hit h;
+    h.t = s.t;
+    h.prim_type = s.leaf >= 0 ? 1 : -1;
+    h.prim_index = s.leaf >= 0 ? v.leaf_prim[s.leaf] : -1;
+    h.did_hit = s.leaf >= 0 ? 1 : 0;
+    return h;
+}
+
+// A queue shorter than group_below is k_bb_extend_group's.
+__global__ __launch_bounds__(kBlock) void k_bb_extend(const BvhView* bvhs, BWs ws, int cur, int refill,
+                                                     int group_below) {
     extern __shared__ int lds_stack[];
     const int total = queue_len(ws.counters, cur);
+    if (total < group_below) return;
     if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
     BvhView bv;
     int bv_scene = -1;
@@ -392,15 +406,28 @@ __global__ __launch_bounds__(kBlock) void k_bb_extend(const BvhView* bvhs, BWs w
             trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
             s.time = unpack_time(it.ot.w, ws.tb);
         },
-        [&](int i, const Trav& s, const BvhView& v) {
-            mcpt_packed_hit h;  // prim_index keeps the leaf's code: triangle, or ~sphere
-            h.t = s.t;
-            h.prim_type = s.leaf >= 0 ? 1 : -1;
-            h.prim_index = s.leaf >= 0 ? v.leaf_prim[s.leaf] : -1;
-            h.did_hit = s.leaf >= 0 ? 1 : 0;
-            ws.hits[i] = h;
+        [&](int i, const Trav& s, const BvhView& v) { ws.hits[i] = bb_hit(s, v); }, ns, nl);
+}
+
+// a short queue: 16 lanes per ray (group_closest), each group's view that
+// of its ray's scene
+__global__ __launch_bounds__(kBlock) void k_bb_extend_group(const BvhView* bvhs, BWs ws, int cur, int group_below,
+                                                           int cap, int cap_phys) {
+    extern __shared__ int lds[];
+    const int total = queue_len(ws.counters, cur);
+    if (total >= group_below || (int)blockIdx.x * (kBlock / kGroup) >= total) return;
+    int* base = lds + (threadIdx.x / kGroup) * group_lds_ints(cap_phys);
+    const RayItem* q = ws.q[cur];
+    BvhView bv;
+    group_closest(
+        bv, &ws.counters[C_WORK], total, reinterpret_cast<int2*>(base), base + 2 * cap_phys, cap, cap_phys,
+        [&](int i, Trav& s, BvhView& v) {
+            const RayItem it = q[i];
+            v = bvhs[unpack_scene(it.ot.w, ws.tb)];
+            trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+            s.time = unpack_time(it.ot.w, ws.tb);
         },
-        ns, nl);
+        [&](int i, const Trav& s, const BvhView& v) { ws.hits[i] = bb_hit(s, v); });
 }
 
 // shadeKernel (:3470-3696) with the ray's own scene and camera
@@ -478,10 +505,13 @@ __device__ __forceinline__ void add_shadow(float4* smp, const ShadowItem& si, fl
 // persistent scheme of k_bb_extend: a lane walks the any-hit steps of its
 // ray and refills when done. A ray of a scene with a medium (closest hits
 // through the medium) is handled whole in one iteration.
-__global__ __launch_bounds__(kBlock) void k_bb_shadow_persistent(const BScene* tab, BWs ws, int cur, int refill) {
+// A queue shorter than group_below is k_bb_shadow_group's.
+__global__ __launch_bounds__(kBlock) void k_bb_shadow_persistent(const BScene* tab, BWs ws, int cur, int refill,
+                                                                int group_below) {
     extern __shared__ int lds_stack[];
     int* stk = lds_stack + threadIdx.x;
     const int total = shadow_len(ws.counters, 1 - cur);
+    if (total < group_below) return;
     if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
     const int lane = lane_id();
     Trav s;
@@ -535,6 +565,28 @@ __global__ __launch_bounds__(kBlock) void k_bb_shadow_persistent(const BScene* t
     }
 }
 
+// a short shadow queue: 16 lanes per ray (group_any); only for batches
+// without a medium (group_below is 0 otherwise)
+__global__ __launch_bounds__(kBlock) void k_bb_shadow_group(const BScene* tab, BWs ws, int cur, int group_below,
+                                                           int cap, int cap_phys) {
+    extern __shared__ int lds[];
+    const int total = shadow_len(ws.counters, 1 - cur);
+    if (total >= group_below || (int)blockIdx.x * (kBlock / kGroup) >= total) return;
+    int* base = lds + (threadIdx.x / kGroup) * group_lds_ints(cap_phys);
+    BvhView bv;
+    group_any(
+        bv, &ws.counters[C_WORK_SHADOW], total, reinterpret_cast<int2*>(base), base + 2 * cap_phys, cap, cap_phys,
+        [&](int i, Trav& s, BvhView& v) {
+            const ShadowItem si = ws.sq[i];
+            v = tab[unpack_scene(si.c.w, ws.tb)].sv.bvh;
+            trav_begin(s, vec3(si.om.x, si.om.y, si.om.z), vec3(si.dp.x, si.dp.y, si.dp.z), si.om.w);
+            s.time = unpack_time(si.c.w, ws.tb);
+        },
+        [&](int i, bool blocked) {
+            if (!blocked) add_shadow(ws.smp, ws.sq[i], 1.0f);
+        });
+}
+
 // finalizeSampleKernel (:3740-3755): a pixel's samples in sample order
 __global__ __launch_bounds__(kBlock) void k_bb_finalize(const BScene* tab, int nsc, int P, int nb, const float4* smp) {
     const int q = blockIdx.x * kBlock + threadIdx.x;
@@ -571,6 +623,7 @@ struct BatchRun {
     int64_t P = 0;  // pixels of all scenes
     int max_stack = 1;
     bool moving = false;  // ray records then carry their time too
+    bool medium = false;  // a scene with a medium: shadow rays one lane each
     void* mem = nullptr;
     size_t bytes = 0;
     BScene* dtab = nullptr;
@@ -593,7 +646,10 @@ int batch_tables(const mcpt_scene* scenes, const mcpt_camera* cams, int n, float
     br.n = n;
     br.tab.resize(n);
     br.bvh.resize(n);
-    for (int i = 0; i < n; i++) br.moving = br.moving || scenes[i]->moving_spheres;
+    for (int i = 0; i < n; i++) {
+        br.moving = br.moving || scenes[i]->moving_spheres;
+        br.medium = br.medium || scenes[i]->has_medium;
+    }
     for (int i = 0; i < n; i++) {
         br.tab[i].sv = scenes[i]->view();
         br.tab[i].cam = cams[i];
@@ -680,6 +736,17 @@ int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const
     const BWs& ws = br.ws;
     const int persistent_blocks = tuned_blocks(br, tune);
     const size_t lds = br.lds;
+    // short queues: 16 lanes per ray (mcpt_render's default threshold)
+    const int gb = o->group_below < 0 ? 0 : o->group_below > 0 ? o->group_below : 32768;
+    const int gbs = br.medium ? 0 : gb;
+    const int gcap_phys = group_stack_pairs(stack_entries(br.max_stack));
+    const size_t glds = (size_t)(kBlock / kGroup) * group_lds_ints(gcap_phys) * sizeof(int);
+    int group_blocks = 0;
+    if (gb > 0) {
+        int pc = 0;
+        MCPT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_bb_extend_group, kBlock, glds));
+        group_blocks = std::min(std::max(1, pc) * ctx->cu_count, (gb + kBlock / kGroup - 1) / (kBlock / kGroup));
+    }
     // Each bounce's shadow rays run on the context's second stream beside
     // the next bounce's closest-hit rays: they add light to the samples of
     // their paths (smp), which only the next shade and finalize read, and
@@ -706,7 +773,10 @@ int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const
                                                  ws);
             for (int d = 0; d < o->max_depth && e == hipSuccess; d++) {
                 k_bb_bounce<<<1, 64, 0, st>>>(ws.counters, cur);
-                k_bb_extend<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(br.dbvh, ws, cur, refill);
+                k_bb_extend<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(br.dbvh, ws, cur, refill, gb);
+                if (gb > 0)
+                    k_bb_extend_group<<<std::min(grid, group_blocks), kBlock, glds, st>>>(br.dbvh, ws, cur, gb,
+                                                                                        kGroupCap, gcap_phys);
                 if (d > 0) e = hipStreamWaitEvent(st, ev_shadow, 0);  // the previous bounce's shadow rays
                 k_bb_shade<<<(int)((paths + kShadeBlock - 1) / kShadeBlock), kShadeBlock, 0, st>>>(br.dtab, ws, cur, d,
                                                                                                   (int)P);
@@ -714,7 +784,10 @@ int render_batch(const mcpt_scene* scenes, const mcpt_camera* cams, int n, const
                 if (e == hipSuccess) e = hipStreamWaitEvent(sd, ev_shade, 0);
                 k_bb_shadow_total<<<1, 64, 0, sd>>>(ws.counters, cur);
                 k_bb_shadow_persistent<<<std::min(grid, persistent_blocks), kBlock, lds, sd>>>(br.dtab, ws, cur,
-                                                                                                refill);
+                                                                                                refill, gbs);
+                if (gbs > 0)
+                    k_bb_shadow_group<<<std::min(grid, group_blocks), kBlock, glds, sd>>>(br.dtab, ws, cur, gbs,
+                                                                                        kGroupCap, gcap_phys);
                 if (e == hipSuccess) e = hipEventRecord(ev_shadow, sd);
                 launches++;
                 cur = 1 - cur;
@@ -1044,7 +1117,7 @@ int mcpt_batch_autotune(const mcpt_scene* scenes, const mcpt_camera* cams, int32
     k_bb_init<<<1, 64, 0, st>>>(ws.counters, (int)P);
     k_bb_camera<<<grid, kBlock, 0, st>>>(br.dtab, n, (int)P, (uint32_t)(o->sample_begin + o->spp), 1, o->seed, ws);
     k_bb_bounce<<<1, 64, 0, st>>>(ws.counters, 0);
-    k_bb_extend<<<std::min(grid, tuned_blocks(br, nullptr)), kBlock, br.lds, st>>>(br.dbvh, ws, 0, 32);
+    k_bb_extend<<<std::min(grid, tuned_blocks(br, nullptr)), kBlock, br.lds, st>>>(br.dbvh, ws, 0, 32, 0);
     if (o->max_depth > 1) k_bb_shade<<<(int)((P + kShadeBlock - 1) / kShadeBlock), kShadeBlock, 0, st>>>(
         br.dtab, ws, 0, 0, (int)P);
     if (hipGetLastError() != hipSuccess) return cleanup(set_error(MCPT_ERR_HIP, "autotune: launch failed"));
@@ -1062,7 +1135,7 @@ int mcpt_batch_autotune(const mcpt_scene* scenes, const mcpt_camera* cams, int32
         auto launch = [&](const mcpt_batch_tune& t) {
             k_b_fill<<<1, 64, 0, st>>>(ws.counters + C_WORK, 1, 0);
             k_bb_extend<<<std::min(blocks_for(m), tuned_blocks(br, &t)), kBlock, br.lds, st>>>(br.dbvh, ws, q,
-                                                                                                t.refill_lanes);
+                                                                                                t.refill_lanes, 0);
         };
         auto timed = [&](const mcpt_batch_tune& t, float& ms) {
             hipEvent_t a = nullptr, b = nullptr;

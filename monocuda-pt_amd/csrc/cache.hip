@@ -726,11 +726,24 @@ __global__ void k_cc_bounce(int* c, int cur) {
     }
 }
 
+// the 16-byte hit record; prim_index keeps the leaf's code (triangle, or
+// ~sphere)
+__device__ __forceinline__ mcpt_packed_hit cc_hit(const Trav& s, const BvhView& v) {
+    mcpt_packed_hit h;
+    h.t = s.t;
+    h.prim_type = s.leaf >= 0 ? 1 : -1;
+    h.prim_index = s.leaf >= 0 ? v.leaf_prim[s.leaf] : -1;
+    h.did_hit = s.leaf >= 0 ? 1 : 0;
+    return h;
+}
+
 // persistent extend (extendKernelPersistent, :3416-3468) with deferred
-// leaf tests
-__global__ __launch_bounds__(kBlock) void k_cc_extend(SceneView sv, CacheWs ws, int cur, int refill) {
+// leaf tests; a queue shorter than group_below is k_cc_extend_group's
+__global__ __launch_bounds__(kBlock) void k_cc_extend(SceneView sv, CacheWs ws, int cur, int refill,
+                                                     int group_below) {
     extern __shared__ int lds_stack[];
     const int total = queue_len(ws.counters, cur);
+    if (total < group_below) return;
     if ((int)blockIdx.x * kBlock >= total) return;  // blocks the queue does not need
     uint32_t ns = 0, nl = 0;
     const RayItem* q = ws.q[cur];
@@ -741,15 +754,26 @@ __global__ __launch_bounds__(kBlock) void k_cc_extend(SceneView sv, CacheWs ws, 
             trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
             s.time = it.ot.w;
         },
-        [&](int i, const Trav& s, const BvhView& v) {
-            mcpt_packed_hit h;  // prim_index keeps the leaf's code: triangle, or ~sphere
-            h.t = s.t;
-            h.prim_type = s.leaf >= 0 ? 1 : -1;
-            h.prim_index = s.leaf >= 0 ? v.leaf_prim[s.leaf] : -1;
-            h.did_hit = s.leaf >= 0 ? 1 : 0;
-            ws.hits[i] = h;
+        [&](int i, const Trav& s, const BvhView& v) { ws.hits[i] = cc_hit(s, v); }, ns, nl);
+}
+
+// a short queue: 16 lanes per ray (group_closest)
+__global__ __launch_bounds__(kBlock) void k_cc_extend_group(SceneView sv, CacheWs ws, int cur, int group_below,
+                                                           int cap, int cap_phys) {
+    extern __shared__ int lds[];
+    const int total = queue_len(ws.counters, cur);
+    if (total >= group_below || (int)blockIdx.x * (kBlock / kGroup) >= total) return;
+    int* base = lds + (threadIdx.x / kGroup) * group_lds_ints(cap_phys);
+    const RayItem* q = ws.q[cur];
+    BvhView bv = sv.bvh;
+    group_closest(
+        bv, &ws.counters[C_WORK], total, reinterpret_cast<int2*>(base), base + 2 * cap_phys, cap, cap_phys,
+        [&](int i, Trav& s, const BvhView&) {
+            const RayItem it = q[i];
+            trav_begin(s, vec3(it.ot.x, it.ot.y, it.ot.z), vec3(it.dp.x, it.dp.y, it.dp.z), 1e30f);
+            s.time = it.ot.w;
         },
-        ns, nl);
+        [&](int i, const Trav& s, const BvhView& v) { ws.hits[i] = cc_hit(s, v); });
 }
 
 // shadeKernel (:3470-3696). A training path records every diffuse vertex
@@ -1209,6 +1233,15 @@ int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* cam, const mcpt_rend
     int per_cu = 0;
     MCPT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cc_extend, kBlock, lds));
     const int persistent_blocks = std::max(1, per_cu) * ctx->cu_count;
+    // short queues: 16 lanes per ray (mcpt_render's default threshold)
+    const int gb = o->group_below < 0 ? 0 : o->group_below > 0 ? o->group_below : 32768;
+    const int gcap_phys = group_stack_pairs(stack_entries(scene->max_depth));
+    const size_t glds = (size_t)(kBlock / kGroup) * group_lds_ints(gcap_phys) * sizeof(int);
+    int group_blocks = 0;
+    if (gb > 0) {
+        MCPT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cc_extend_group, kBlock, glds));
+        group_blocks = std::min(std::max(1, per_cu) * ctx->cu_count, (gb + kBlock / kGroup - 1) / (kBlock / kGroup));
+    }
     Frame fr;
     {
         float b[6];
@@ -1244,7 +1277,10 @@ int mcpt_render_cached(mcpt_scene scene, const mcpt_camera* cam, const mcpt_rend
         for (int d = 0; d < depth; d++) {
             fr.depth = d;
             k_cc_bounce<<<1, 64, 0, st>>>(ws.counters, cur);
-            k_cc_extend<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(sv, ws, cur, refill);
+            k_cc_extend<<<std::min(grid, persistent_blocks), kBlock, lds, st>>>(sv, ws, cur, refill, gb);
+            if (gb > 0)
+                k_cc_extend_group<<<std::min(grid, group_blocks), kBlock, glds, st>>>(sv, ws, cur, gb, kGroupCap,
+                                                                                    gcap_phys);
             k_cc_shade<<<grid, kBlock, 0, st>>>(sv, ws, cur, fr, bg, cam->rr_cap);
             k_cc_shadow<<<grid, kBlock, lds, st>>>(sv, ws, cur);
             cur = 1 - cur;
