@@ -642,6 +642,22 @@ int mcpt_load_pfm(const char* path, int32_t* width, int32_t* height, float* rgba
 int mcpt_load_hdr(const char* path, int32_t* width, int32_t* height, float* rgba);
 int mcpt_load_exr(const char* path, int32_t* width, int32_t* height, float* rgba);
 
+/* A NanoVDB float grid (.nvdb) as the dense grid of
+ * mcpt_scene_set_density_grid: the file's first float grid over its root's
+ * index bounding box, with the grid's axis-aligned scale and translation as
+ * voxel size and origin (the world position of the box's min voxel);
+ * *max_density (may be NULL) receives the root's maximum statistic, the
+ * majorant the reference takes. The reference reads the file with
+ * nanovdb::io::readGrid (loadNanoVdbGrid, src/volume_io.cpp:45-53; the
+ * statistics, takeFloatGrid, :11-43) and samples the sparse grid
+ * trilinearly in index space (sampleVolumeDensity,
+ * src/path_tracer.cu:784-789), which the dense grid reproduces. Raw and
+ * ZIP-compressed grids; refuses BLOSC, non-float grids, a non-zero
+ * background and maps with rotation. Two-call pattern: density NULL fills
+ * grid->dims, origin and voxel_size; then density (dims[0]*dims[1]*dims[2]
+ * floats, x fastest) is filled and grid->density points at it. */
+int mcpt_load_nvdb(const char* path, mcpt_density_grid* grid, float* density, float* max_density);
+
 /* ------------------------------------------------------------ OBJ import
  * Wavefront OBJ + MTL into triangle records (loadObject / loadMtlFile,
  * src/path_tracer.cu:2527-2859): fan triangulation, 1-based and negative

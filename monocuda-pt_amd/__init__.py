@@ -299,6 +299,7 @@ _SIGS = {
     "mcpt_load_pfm": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_hdr": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_exr": ([C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
+    "mcpt_load_nvdb": ([C.c_char_p, C.POINTER(DensityGrid), VP, C.POINTER(C.c_float)], C.c_int),
     "mcpt_obj_texture": ([C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
     "mcpt_load_pbrt": ([C.c_char_p, C.c_int64, VP, VP, VP, VP, C.POINTER(PbrtInfo)], C.c_int),
     "mcpt_pbrt_texture": ([C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), VP], C.c_int),
@@ -500,6 +501,20 @@ def load_hdr(path):
 def load_exr(path):
     """mcpt_load_exr: a scanline OpenEXR file as an (H, W, 4) RGBA FP32 array."""
     return _load_float_image(_lib.mcpt_load_exr, "mcpt_load_exr", path)
+
+
+def load_nvdb(path):
+    """mcpt_load_nvdb: a NanoVDB float grid as (density (nz, ny, nx) float32,
+    origin, voxel_size, max_density), ready for Scene.set_density_grid."""
+    import numpy as np
+    g = DensityGrid()
+    mx = C.c_float(0.0)
+    p = os.fspath(path).encode()
+    _check(_lib.mcpt_load_nvdb(p, C.byref(g), None, C.byref(mx)), "mcpt_load_nvdb")
+    nx, ny, nz = g.dims
+    out = np.zeros((nz, ny, nx), np.float32)
+    _check(_lib.mcpt_load_nvdb(p, C.byref(g), _ptr(out), C.byref(mx)), "mcpt_load_nvdb")
+    return out, tuple(g.origin), tuple(g.voxel_size), float(mx.value)
 
 
 def load_env(path):

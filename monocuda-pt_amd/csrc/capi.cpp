@@ -561,6 +561,11 @@ int mcpt_load_exr(const char* path, int32_t* width, int32_t* height, float* rgba
     return guarded("mcpt_load_exr", [&] { return load_env_file(read_exr, path, width, height, rgba); });
 }
 
+int mcpt_load_nvdb(const char* path, mcpt_density_grid* grid, float* density, float* max_density) {
+    if (!path || !grid) return set_error(MCPT_ERR_INVALID, "bad arguments");
+    return guarded("mcpt_load_nvdb", [&] { return read_nvdb(path, grid, density, max_density); });
+}
+
 int mcpt_scene_destroy(mcpt_scene scene) {
     if (!scene) return MCPT_OK;
     const int srv = scene->servers.load();

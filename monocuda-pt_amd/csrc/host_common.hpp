@@ -94,6 +94,9 @@ int read_ppm(const char* path, int& w, int& h, std::vector<uint8_t>* rgba);
 int read_pfm(const char* path, int& w, int& h, std::vector<float>* rgba);
 int read_hdr(const char* path, int& w, int& h, std::vector<float>* rgba);
 int read_exr(const char* path, int& w, int& h, std::vector<float>* rgba);
+// a NanoVDB float grid as a dense density grid (nvdb.cpp); density NULL
+// fills the description only
+int read_nvdb(const char* path, mcpt_density_grid* out, float* density, float* max_density);
 int read_png(const char* path, int& w, int& h, std::vector<uint8_t>* rgba);
 int read_tga(const char* path, int& w, int& h, std::vector<uint8_t>* rgba);
 int read_jpeg(const char* path, int& w, int& h, std::vector<uint8_t>* rgba);
