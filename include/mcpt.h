@@ -273,8 +273,9 @@ typedef struct mcpt_render_opts {
     int32_t sample_begin; /* index of the first sample (resume / sharding)  */
     int32_t max_depth;
     uint64_t seed;
-    int32_t refill_lanes; /* persistent extend: idle lanes before a refill;
-                             0 picks the default                           */
+    int32_t refill_lanes; /* persistent extend and shadow rays: idle lanes
+                             of 64 before a refill; 0 picks the default (24
+                             when the render runs two lanes, else 32)      */
     uint32_t flags;
     int32_t batch;        /* wavefront modes: samples in flight per pass
                              (paths = W*H*batch); 0 picks the default     */

@@ -362,11 +362,6 @@ __device__ __forceinline__ void shadow_group(const SceneView& sv, const Workspac
         });
 }
 
-// Shadow rays on the persistent scheme of the extend kernel (scenes without
-// a medium: a plain visibility test), the persistent mode's default. The
-// reference tried this on 32-lane warps and reverted it (README.md:170); on
-// 64-lane waves over the four-wide tree it takes 28 ms where one lane per
-// ray (k_shadow) takes 44 ms on the 1M-triangle mesh at 1024x1024, 64 spp.
 __global__ __launch_bounds__(kBlock) void k_shadow_group(SceneView sv, Workspace ws, int cur, int group_below,
                                                           int cap, int cap_phys) {
     extern __shared__ int lds[];
@@ -375,6 +370,11 @@ __global__ __launch_bounds__(kBlock) void k_shadow_group(SceneView sv, Workspace
     shadow_group(sv, ws, cur, total, lds, cap, cap_phys);
 }
 
+// Shadow rays on the persistent scheme of the extend kernel (scenes without
+// a medium: a plain visibility test), the persistent mode's default. The
+// reference tried this on 32-lane warps and reverted it (README.md:170); on
+// 64-lane waves over the four-wide tree it took 28 ms where one lane per ray
+// (k_shadow) took 44 ms on the 1M-triangle mesh at 1024x1024, 64 spp.
 __global__ __launch_bounds__(kBlock) void k_shadow_persistent(SceneView sv, Workspace ws, int cur, int refill,
                                                                int group_below) {
     extern __shared__ int lds_stack[];
@@ -643,6 +643,17 @@ size_t stack_bytes(const mcpt_scene_s* s) { return (size_t)kBlock * stack_entrie
 // Group walks (k_extend_group, k_shadow_group): a queue shorter than
 // kGroupBelow rays is walked by kGroup lanes per ray.
 constexpr int kGroupBelow = 32768;
+// Default refill threshold of the persistent extend and shadow waves (idle
+// lanes of 64 before a refill). Each refill is one atomic on the launch's
+// work counter whose round trip the wave waits for, so lower thresholds
+// trade idle lanes for stalls, and the stalls grow with the waves that share
+// the counter. Measured on the 1M-triangle mesh at 1024^2, two to seven runs
+// each: with two lanes (each launch on half of the grid) 16 / 24 / 32 / 48
+// idle lanes gave 1,329 / 1,355 / 1,340 / 1,193 Mrays/s at 20 spp and
+// 1,416 / 1,516 / 1,499 / 1,322 at 64 spp; with one lane on the full grid
+// (the kernel-timed render) 24 took extend from 43.1 to 45.3 ms at 20 spp.
+constexpr int kRefillTwoLanes = 24;
+constexpr int kRefillOneLane = 32;
 int scene_group_pairs(const mcpt_scene_s* s) { return group_stack_pairs(stack_entries(s->max_depth)); }
 size_t group_lds_bytes(const mcpt_scene_s* s) {
     return (size_t)(kBlock / kGroup) * group_lds_ints(scene_group_pairs(s)) * sizeof(int);
@@ -787,7 +798,7 @@ int enqueue_render(mcpt_scene_s* s, const mcpt_camera* cam, const mcpt_render_op
     const size_t lds = stack_bytes(s);
     const bool count = (o->flags & MCPT_RENDER_COUNT_TRAVERSAL) != 0;
     const bool ref = (o->flags & MCPT_RENDER_REFERENCE_ORDER) != 0;
-    int refill = o->refill_lanes > 0 ? std::min(64, o->refill_lanes) : 32;
+    int refill = o->refill_lanes > 0 ? std::min(64, o->refill_lanes) : nlanes > 1 ? kRefillTwoLanes : kRefillOneLane;
     enum { K_CAMERA, K_EXTEND, K_SHADE, K_SHADOW, K_FINALIZE, K_OTHER };
     auto mark = [&](int k) { return tm ? tm->mark(k) : MCPT_OK; };
 

@@ -154,8 +154,11 @@ def _check_wide(mcpt, scene, ref):
     emin, emax = exp_box[:, :, 0:3], exp_box[:, :, 3:6]
     v3 = valid[:, :, None]
     assert np.all(~v3 | (lo <= emin)) and np.all(~v3 | (hi >= emax))
+    # at most one quantization step outside, plus the half ulp by which the
+    # float decode origin + q * 2^e can fall short of an exact multiple (the
+    # encoder then steps once more)
     s3 = scale[:, None, :]
-    assert np.all(~v3 | (emin - lo < s3)) and np.all(~v3 | (hi - emax < s3))
+    assert np.all(~v3 | (emin - lo <= s3 + np.spacing(emin))) and np.all(~v3 | (hi - emax <= s3 + np.spacing(emax)))
     assert np.array_equal(org, np.where(v3, emin, np.inf).min(axis=1))
     ext = np.where(v3, emax.astype(np.float64), -np.inf).max(axis=1) - org.astype(np.float64)
     assert np.all(255.0 * scale >= ext)
