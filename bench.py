@@ -890,6 +890,12 @@ def main():
                 result["roofline"]["note"] = (
                     "achieved counts every node and leaf a ray reads (the algorithmic unit); BVH reuse is served by "
                     "L2 and the Infinity Cache, so the measured L2-to-fabric traffic is lower")
+                if "valu_busy" in k:
+                    result["roofline"]["valu_busy"] = k["valu_busy"]
+                    result["roofline"]["valu_note"] = (
+                        "the kernel's vector ALUs issue on this share of its cycles (rocprofv3 SQ_ACTIVE_INST_VALU x 4 "
+                        "over 1024 SIMDs x the dispatch cycles, in the profile): the walk is bound by VALU issue, "
+                        "with " + f"{100 * k['valu_active_lane_share']:.0f}% of lanes active")
         ref_bytes = 48 * n + 80 * ref_counted["extend_steps"] + 44 * ref_counted["extend_leaves"]
         result["roofline"]["reference_order"] = {
             "steps_per_ray": ref_counted["extend_steps"] / n, "leaves_per_ray": ref_counted["extend_leaves"] / n,
